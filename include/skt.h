@@ -1,0 +1,207 @@
+/*
+ * skt.h — C ABI of the MI355X-native skewT cloud-simulation step.
+ *
+ * This is the drop-in boundary for the reference's environmentGPU class
+ * (Game/environment.cuh:9-183) and its dataClass companion
+ * (Game/dataClass.cuh:67-89). Every entry point returns an skt_status (0 = ok)
+ * instead of breaking into the debugger; skt_env_last_error() gives the text.
+ * Device arrays are owned by the environment; pointers handed out are
+ * borrowed and stay valid (same address) until skt_env_destroy().
+ *
+ * Grid convention (Game/utils.cuh:8-11): index = x + nx*(y + ny*z), y is the
+ * vertical axis; ground columns are indexed x + nx*z.
+ * Work is stream-ordered on one HIP stream per environment; calls on one
+ * environment are not thread-safe, separate environments are independent.
+ */
+#ifndef SKT_H
+#define SKT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct skt_env skt_env;
+
+typedef enum skt_status {
+    SKT_OK = 0,
+    SKT_ERR_ARG = 1,     /* invalid argument (shape, field id, null pointer) */
+    SKT_ERR_HIP = 2,     /* a HIP runtime call failed */
+    SKT_ERR_STATE = 3,   /* call order (e.g. update before init) */
+    SKT_ERR_NOMEM = 4    /* device allocation failed */
+} skt_status;
+
+/* Boundary conditions, Game/config.h:43-46. */
+enum { SKT_NEUMANN = 0, SKT_DIRICHLET = 1, SKT_CUSTOM = 2 };
+/* boundsEnv, Game/config.h:52-59 (the 'up' member is carried but, as in the
+ * reference, the top face uses 'sides'). */
+typedef struct skt_bounds { int32_t sides, up, ground; } skt_bounds;
+
+/* Pressure-solve modes. REFERENCE reproduces calculatePressureProject
+ * (Game/environment.cu:898-987) including its sigma accumulation and stop
+ * test; STANDARD is textbook Jacobi-PCG with a relative tolerance. */
+enum { SKT_PCG_REFERENCE = 0, SKT_PCG_STANDARD = 1 };
+
+typedef struct skt_config {
+    int32_t nx, ny, nz;        /* ny vertical; replaces GRIDSIZESKY{X,Y,Z} (Game/config.h:3-5) */
+    float dx;                  /* voxel size in m; replaces VOXELSIZE (Game/config.h:9) */
+    int32_t device;            /* HIP device ordinal */
+    int32_t pcg_mode;          /* SKT_PCG_REFERENCE or SKT_PCG_STANDARD */
+    float pcg_tol;             /* 1e-5 absolute (reference) or relative to max|r0| */
+    int32_t pcg_max_iter;      /* 200 (Game/environment.cu:907) */
+    int32_t diffusion_sweeps;  /* 20 (Game/environment.cu:649) */
+    int32_t max_substeps;      /* 100 (Game/environment.cu:727) */
+    int32_t maxabs_mode;       /* 0: getMaxDivergence as written (|.| at z=0 only); 1: max|.| */
+    uint32_t noise_seed;       /* first of the five init-noise seeds (100 in Game/environment.cu:308) */
+    int32_t init_noise;        /* 1: add the reference's init perturbations */
+} skt_config;
+
+/* Fills a configuration with the reference's settings for an nx*ny*nz grid. */
+void skt_config_default(skt_config* cfg, int32_t nx, int32_t ny, int32_t nz, float dx);
+
+/* Field ids: the gridDataSkyGPU / gridDataGroundGPU members
+ * (Game/environment.cuh:13-37), the solver work arrays and the per-level
+ * profiles the step reads. */
+typedef enum skt_field {
+    SKT_QV = 0, SKT_QW, SKT_QC, SKT_QR, SKT_QS, SKT_QI, SKT_POTTEMP, SKT_VELX, SKT_VELY, SKT_VELZ, SKT_PRESSURE,
+    SKT_DENSITY,          /* m_densityAir */
+    SKT_DENSITY_OLD,      /* m_oldDensityAir */
+    SKT_PRESSURE_PROJ,    /* pressure correction of the last projection (m_outputArray) */
+    SKT_QRS, SKT_QGR, SKT_QGS, SKT_QGI, SKT_GROUND_P, SKT_GROUND_TIME, SKT_GROUND_T,
+    SKT_CLOUD_COVER,      /* m_dummyArrayGround after groundCoverageFactor */
+    SKT_ISEN_TEMP, SKT_ISEN_VAPOR, SKT_DEFAULT_VELX, SKT_DEFAULT_VELZ, SKT_DEFAULT_PRESSURE,
+    SKT_FIELD_COUNT
+} skt_field;
+
+/* 'parameter' and 'direction' of getParamArray (Game/config.h:37-40,62-65). */
+enum { SKT_PARAM_POTTEMP = 0, SKT_PARAM_QV, SKT_PARAM_QW, SKT_PARAM_QC, SKT_PARAM_QR, SKT_PARAM_QS, SKT_PARAM_QI,
+       SKT_PARAM_WIND, SKT_PARAM_PGROUND, SKT_PARAM_PRESSURE };
+enum { SKT_DIR_LEFT = 0, SKT_DIR_RIGHT, SKT_DIR_UP, SKT_DIR_DOWN, SKT_DIR_FORWARD, SKT_DIR_BACKWARD };
+
+/* Inputs the reference reads from its editor every step
+ * (Game/environment.cu:1013-1018), plus the ground-changed flag (:992). */
+typedef struct skt_env_params {
+    float time;            /* seconds since midnight, 43200 at init (Game/environment.cuh:115) */
+    int32_t day;           /* day of year, 130 */
+    float longitude;       /* used as latitude by irridianceGPU, 52.37 */
+    float sun_strength;    /* 1 */
+    int32_t pause_diurnal; /* 0 */
+    int32_t ground_changed;
+} skt_env_params;
+
+typedef struct skt_step_stats {
+    int32_t pcg_iterations;      /* iterations of the last pressure solve */
+    float pcg_residual;          /* stop-test value after the last iteration */
+    float pcg_residual0;         /* stop-test value of the initial divergence */
+    int32_t n_advect;            /* advectPPMWGPU calls in the last update */
+    int32_t substeps[16];        /* substep count of each of those calls */
+} skt_step_stats;
+
+/* environmentGPU::environmentGPU (Game/environment.cu:71-185). */
+int skt_env_create(const skt_config* cfg, skt_env** out);
+/* environmentGPU::~environmentGPU (Game/environment.cu:187-247). */
+int skt_env_destroy(skt_env* env);
+const char* skt_env_last_error(const skt_env* env);
+/* Runs all later work on this HIP stream (hipStream_t); NULL = the env's own. */
+int skt_env_set_stream(skt_env* env, void* hip_stream);
+int skt_env_synchronize(skt_env* env);
+
+/* Ground heights per column (host int32[nx*nz]); flat 0 by default. Replaces
+ * the PNoise2D terrain of init (Game/environment.cu:321-340). */
+int skt_env_set_ground_height(skt_env* env, const int32_t* ground_height);
+
+/* environmentGPU::init (Game/environment.cu:249-389). All pointers are host
+ * arrays: potTemps[N], velField[3N] (x,y,z per cell, MAC faces), Qv[N],
+ * groundTemp[G] (written back, as the reference does), groundPres[G],
+ * pressures[N] (hPa), smallPressure[ny]. */
+int skt_env_init(skt_env* env, const float* potTemps, const float* velField, const float* Qv, float* groundTemp,
+                 const float* groundPres, const float* pressures, const float* smallPressure);
+
+/* environmentGPU::updateGPU (Game/environment.cu:391-555): one step. */
+int skt_env_update(skt_env* env, float dt, float speed);
+
+/* Per-phase public methods of environmentGPU (Game/environment.cuh:48-93). */
+int skt_env_micro_physics_ground(skt_env* env, float dt, float speed, float irradiance);
+int skt_env_micro_physics_sky(skt_env* env, float dt, float speed);
+/* type: 0 pot. temp, 1 vapour, 2 vel-x, 3 vel-y, 4 vel-z, 5 other mixing ratios */
+int skt_env_diffuse(skt_env* env, int32_t field, int32_t type, float dt);
+int skt_env_advect_ground_water(skt_env* env, float dt, float speed);
+int skt_env_set_temps_at_ground(skt_env* env, float dt, float speed);
+/* profile: a per-level field id (SKT_ISEN_TEMP ...) or -1 for the zero profile */
+int skt_env_advect_ppm(skt_env* env, int32_t field, int32_t profile, skt_bounds bounds, float dt);
+/* fall_type: 0 rain, 1 snow, 2 ice */
+int skt_env_advect_precip(skt_env* env, int32_t field, int32_t fall_type, float dt);
+int skt_env_pressure_project(skt_env* env, float dt);
+/* Solve only; the correction lands in SKT_PRESSURE_PROJ. */
+int skt_env_calculate_pressure_project(skt_env* env, float dt);
+int skt_env_editor_data(skt_env* env);
+int skt_env_irradiance(skt_env* env, float* out);
+int skt_env_ground_coverage_factor(skt_env* env);
+int skt_env_update_ground_temps(skt_env* env, float dt, float speed, float irradiance);
+int skt_env_calculate_buoyancy(skt_env* env, float dt);
+/* initDensity + old-density copy (Game/environment.cu:490-492) */
+int skt_env_init_density(skt_env* env);
+/* calculateNewPressure (Game/kernelSky.cu:1454-1474) */
+int skt_env_calculate_new_pressure(skt_env* env);
+
+/* getParamArray (Game/environment.cu:1076-1125): borrowed device pointer. */
+int skt_env_get_param_array(skt_env* env, int32_t parameter, int32_t direction, float** dev_ptr);
+/* Any field: borrowed device pointer and its element count. */
+int skt_env_get_field_ptr(skt_env* env, int32_t field, void** dev_ptr, int64_t* count);
+int skt_env_copy_field_to_host(skt_env* env, int32_t field, float* host);
+int skt_env_copy_field_from_host(skt_env* env, int32_t field, const float* host);
+
+int skt_env_set_params(skt_env* env, const skt_env_params* params);
+int skt_env_get_params(skt_env* env, skt_env_params* params);
+int skt_env_get_stats(skt_env* env, skt_step_stats* stats);
+
+/* Boundary masks in the layout of computeNeighbourGPU's Neigh struct
+ * (Game/kernelSky.cu:1284-1316): 12 bytes per cell. host_out[12*N]. */
+int skt_env_compute_masks(skt_env* env, uint8_t* host_out);
+/* Microphysics phase mask of the last sky-microphysics call, one byte per
+ * cell: bit0 sky, bit1 T<0, bit2 PTerm1, bit3 PTerm2, bit4 PTerm3, bit5 WTerm
+ * (Game/microPhysics.cu:1128-1130,1177-1178). host_out[N]. */
+int skt_env_phase_mask(skt_env* env, uint8_t* host_out);
+
+/* dataClass microphysics region diagnostic (Game/dataClass.cu:26-81):
+ * sums of the 33 limited process rates over a box of cells. */
+int skt_mphys_region_set(skt_env* env, const int32_t min_pos[3], const int32_t max_pos[3]);
+int skt_mphys_region_clear(skt_env* env);
+int skt_mphys_region_get(skt_env* env, float out[33]);
+
+/* Synthetic sounding (replaces readTable::initEnvironment,
+ * Game/readTable.cpp:364-474, whose input files are not available):
+ * conditionally unstable profile, hydrostatic base state integrated in FP64,
+ * warm bubbles. Outputs are the init() arguments. */
+typedef struct skt_sounding_spec {
+    float surface_theta;      /* K (surface theta = surface temperature) */
+    float surface_pressure;   /* hPa */
+    float lapse_rate;         /* K/m, positive (0.0065) */
+    float tropopause_height;  /* m (11000) */
+    float surface_qv;         /* kg/kg; used when surface_rh <= 0 */
+    float surface_rh;         /* 0..1; > 0 sets surface qv = rh * ws(T_s, p_s) */
+    float qv_scale_height;    /* m (2500) */
+    float shear_u;            /* u at and above shear_height, m/s (linear from 0) */
+    float shear_height;       /* m (6000) */
+    int32_t n_bubbles;        /* 0..8 */
+    float bubble_amp[8];      /* K */
+    float bubble_radius[8];   /* m */
+    float bubble_x[8], bubble_y[8], bubble_z[8]; /* centre in m (y = height) */
+} skt_sounding_spec;
+
+/* Fills spec for the given BASE seed draw of the randomised generator
+ * (surface theta 296-304 K, RH 60-95 %, shear 0-25 m/s, n bubbles of
+ * 1.5-3 K); n_bubbles <= 0 gives the fixed single-bubble case
+ * (theta 300 K, 14 g/kg, no wind, one 2 K bubble of 500 m at 1 km). */
+int skt_sounding_spec_random(skt_sounding_spec* spec, uint64_t seed, int32_t nx, int32_t ny, int32_t nz, float dx,
+                             int32_t n_bubbles);
+int skt_sounding_generate(const skt_sounding_spec* spec, int32_t nx, int32_t ny, int32_t nz, float dx,
+                          float* potTemps, float* velField, float* Qv, float* groundTemp, float* groundPres,
+                          float* pressures, float* smallPressure);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SKT_H */

@@ -1,0 +1,608 @@
+// Host orchestration and C ABI of the skewT cloud step (include/skt.h).
+//
+// The step sequence follows environmentGPU::updateGPU
+// (Game/environment.cu:391-555). Differences in mechanism, not in results:
+// one stream per environment instead of a device-wide sync after every
+// launch; the PCG scalars (alpha, beta, stop test) live on the device and the
+// host only polls the stop flag every few iterations; the advection substep
+// count is reduced on the device and read back once per advected field.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "skt_internal.h"
+
+using namespace skt;
+
+struct skt_env {
+    EnvState s;
+};
+
+namespace {
+
+int setErr(EnvState& e, int code, const std::string& msg) {
+    e.err = msg;
+    return code;
+}
+
+#define HIPCHK(e, call)                                                                              \
+    do {                                                                                             \
+        hipError_t r_ = (call);                                                                      \
+        if (r_ != hipSuccess) return setErr((e), SKT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(r_)); \
+    } while (0)
+
+long fieldLen(const EnvState& e, int f) {
+    if (f <= SKT_PRESSURE_PROJ) return e.N;
+    if (f <= SKT_CLOUD_COVER) return e.G;
+    return e.Y;
+}
+bool validField(int f) { return f >= 0 && f < SKT_FIELD_COUNT; }
+
+// irridianceGPU, Game/environment.cu:1021-1034 (host float arithmetic).
+float irradianceOf(const skt_env_params& p) {
+    const float Gs = 1361.0f * p.sun_strength;
+    const float rd = 1 + 0.034f * cosf(2 * 3.14159265359f * p.day / 365);
+    const float sd = 0.409f * sinf(2 * 3.14159265359f / 365 * (p.day - 81));
+    const float th = p.time / 3600;
+    const float lr = p.longitude * 0.01745329251994329576923690768489f;
+    const float sr = (th - (6.0f + 14.0f / 2.0f)) * (3.14159265359f / 12.0f);
+    return std::max(0.0f, Gs * rd * (sinf(lr) * sinf(sd) + cosf(lr) * cosf(sd) * cosf(sr)));
+}
+
+int readScalars(EnvState& e) {
+    HIPCHK(e, hipMemcpyAsync(e.sc_host, e.sc, sizeof(Scalars), hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(e, hipStreamSynchronize(e.stream));
+    return SKT_OK;
+}
+
+// diffuseGPU, Game/environment.cu:597-673 (boundary table :602-639).
+int diffuse(EnvState& e, int fld, int type, float dt) {
+    Bounds b{BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET};
+    const float* prof = nullptr;
+    switch (type) {
+        case 0: prof = e.f[SKT_ISEN_TEMP]; b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_CUSTOM}; break;
+        case 1: prof = e.f[SKT_ISEN_VAPOR]; b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET}; break;
+        case 2: prof = e.f[SKT_DEFAULT_VELX]; b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET}; break;
+        case 3: prof = nullptr; b = Bounds{BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET}; break;
+        case 4: prof = e.f[SKT_DEFAULT_VELZ]; b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET}; break;
+        default: prof = e.zeros; break;
+    }
+    const float k = 0.005f * dt / (e.dx * e.dx);
+    float* bufs[2] = {e.f[fld], e.work[0]};
+    const int n = e.cfg.diffusion_sweeps;
+    for (int s = 0; s < n; s++) HIPCHK(e, launch_diffuse_sweep(e, bufs[s & 1], bufs[(s + 1) & 1], prof, b, type, k));
+    if (n & 1) HIPCHK(e, launch_copy_sky(e, e.work[0], e.f[fld]));
+    return SKT_OK;
+}
+
+// advectPPMWGPU, Game/environment.cu:692-824. The five Strang passes rotate
+// through three work buffers so that no pass writes what it reads; a velocity
+// component being advected is read in its current (partially advected)
+// state, as in the reference (quirk A11), and the result lands in place.
+int advect(EnvState& e, int fld, const float* prof, Bounds b, float dt) {
+    HIPCHK(e, launch_maxvel(e, dt));
+    int rc = readScalars(e);
+    if (rc) return rc;
+    const int nsub = e.sc_host->nsub;
+    const float dts = e.sc_host->dtsub;
+    if (e.stats.n_advect < 16) e.stats.substeps[e.stats.n_advect] = nsub;
+    e.stats.n_advect++;
+    const float h = dts * 0.5f;
+    float* A = e.f[fld];
+    float *T1 = e.work[1], *T2 = e.work[2], *T3 = e.work[3];
+    float *U = e.f[SKT_VELX], *V = e.f[SKT_VELY], *W = e.f[SKT_VELZ];
+    for (int s = 0; s < nsub; s++) {
+        HIPCHK(e, launch_ppm_pass(e, A, T1, fld == SKT_VELX ? A : U, prof, b, 0, h));
+        HIPCHK(e, launch_ppm_pass(e, T1, T2, fld == SKT_VELY ? A : V, prof, b, 1, h));
+        HIPCHK(e, launch_ppm_pass(e, T2, T1, fld == SKT_VELZ ? T2 : W, prof, b, 2, dts));
+        HIPCHK(e, launch_ppm_pass(e, T1, T3, fld == SKT_VELY ? T2 : V, prof, b, 1, h));
+        HIPCHK(e, launch_ppm_pass(e, T3, A, fld == SKT_VELX ? T3 : U, prof, b, 0, h));
+    }
+    return SKT_OK;
+}
+
+// advectPrecip, Game/environment.cu:826-849 (guessed fall speeds 5/1/8 m/s).
+int precip(EnvState& e, int fld, int type, float dt) {
+    const float maxVel = type == 0 ? 5.0f : (type == 1 ? 1.0f : 8.0f);
+    const float C = maxVel * dt / e.dx;
+    const int sub = int(fminf(ceilf(fabsf(C)), (float)e.cfg.max_substeps));
+    if (sub <= 0) return SKT_OK;
+    const float dts = dt / sub;
+    for (int s = 0; s < sub; s++) HIPCHK(e, launch_precip(e, e.f[fld], type, dts));
+    return SKT_OK;
+}
+
+// calculatePressureProject, Game/environment.cu:898-987.
+int solvePressure(EnvState& e, float dt) {
+    HIPCHK(e, launch_divergence(e, dt));
+    HIPCHK(e, launch_pcg_start(e));
+    const int maxIt = e.cfg.pcg_max_iter;
+    const int chunk = 25;
+    for (int it = 0; it < maxIt; it++) {
+        HIPCHK(e, launch_pcg_A(e, it));
+        HIPCHK(e, launch_pcg_B(e, it));
+        if ((it + 1) % chunk == 0 && it + 1 < maxIt) {
+            int rc = readScalars(e);
+            if (rc) return rc;
+            if (e.sc_host->done) break;
+        }
+    }
+    int rc = readScalars(e);
+    if (rc) return rc;
+    e.stats.pcg_iterations = e.sc_host->iters;
+    e.stats.pcg_residual = e.sc_host->maxr;
+    e.stats.pcg_residual0 = e.sc_host->maxr0;
+    return SKT_OK;
+}
+
+// pressureProject, Game/environment.cu:851-896; with new_pressure the
+// calculateNewPressure that follows it in updateGPU (:501) runs per cell in
+// the same pass.
+int pressureProject(EnvState& e, float dt, int new_pressure) {
+    HIPCHK(e, launch_reset_vel(e));
+    HIPCHK(e, launch_build_A(e));
+    int rc = solvePressure(e, dt);
+    if (rc) return rc;
+    HIPCHK(e, launch_update_pressure_density(e));
+    HIPCHK(e, launch_apply_projection(e, new_pressure));
+    return SKT_OK;
+}
+
+int editorData(EnvState& e) {
+    if (e.params.ground_changed) {
+        HIPCHK(e, launch_editor_reset(e, 9));
+        HIPCHK(e, launch_isen_temp_ground(e));
+    }
+    return SKT_OK;
+}
+
+int microSky(EnvState& e, float dt, float speed) {
+    HIPCHK(e, launch_microphysics(e, dt, speed));
+    if (e.region_active) {
+        HIPCHK(e, launch_region_finalize(e));
+        const unsigned nb = (unsigned)blocks3d(e);
+        HIPCHK(e, hipMemcpyAsync(e.region_host, (float*)(e.region + 33L * nb), 33 * sizeof(float),
+                                 hipMemcpyDeviceToHost, e.stream));
+        HIPCHK(e, hipStreamSynchronize(e.stream));
+    }
+    return SKT_OK;
+}
+
+#define RC(x)                 \
+    do {                      \
+        int rc_ = (x);        \
+        if (rc_) return rc_;  \
+    } while (0)
+
+int update(EnvState& e, float dt, float speed) {
+    const float ds = dt * speed;
+    e.stats.n_advect = 0;
+    RC(editorData(e));
+    const float irr = irradianceOf(e.params);
+    // ground (Game/environment.cu:420-456) and setTempsAtGround (:469)
+    HIPCHK(e, launch_ground_water_snapshot(e, dt, speed));
+    HIPCHK(e, launch_ground_columns(e, 1 | 2 | 4 | 8 | 16, dt, speed, irr));
+    const Bounds bT{BC_CUSTOM, BC_CUSTOM, BC_CUSTOM}, bU{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET},
+        bV{BC_DIRICHLET, BC_DIRICHLET, BC_DIRICHLET}, bW{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET},
+        bQ{BC_DIRICHLET, BC_DIRICHLET, BC_DIRICHLET}, bQv{BC_CUSTOM, BC_CUSTOM, BC_CUSTOM};
+    RC(diffuse(e, SKT_POTTEMP, 0, ds));
+    RC(advect(e, SKT_POTTEMP, e.f[SKT_ISEN_TEMP], bT, ds));
+    HIPCHK(e, launch_buoyancy(e, ds));
+    RC(diffuse(e, SKT_VELX, 2, ds));
+    RC(diffuse(e, SKT_VELY, 3, ds));
+    RC(diffuse(e, SKT_VELZ, 4, ds));
+    RC(advect(e, SKT_VELX, e.f[SKT_DEFAULT_VELX], bU, ds));
+    RC(advect(e, SKT_VELY, e.zeros, bV, ds));
+    RC(advect(e, SKT_VELZ, e.f[SKT_DEFAULT_VELZ], bW, ds));
+    HIPCHK(e, launch_init_density(e, 1));
+    RC(pressureProject(e, ds, 1));
+    RC(diffuse(e, SKT_QV, 1, ds));
+    RC(advect(e, SKT_QV, e.f[SKT_ISEN_VAPOR], bQv, ds));
+    RC(diffuse(e, SKT_QW, 5, ds));
+    RC(advect(e, SKT_QW, e.zeros, bQ, ds));
+    RC(diffuse(e, SKT_QC, 5, ds));
+    RC(advect(e, SKT_QC, e.zeros, bQ, ds));
+    RC(diffuse(e, SKT_QR, 5, ds));
+    RC(precip(e, SKT_QR, 0, ds));
+    RC(advect(e, SKT_QR, e.zeros, bQ, ds));
+    RC(diffuse(e, SKT_QS, 5, ds));
+    RC(precip(e, SKT_QS, 1, ds));
+    RC(advect(e, SKT_QS, e.zeros, bQ, ds));
+    RC(diffuse(e, SKT_QI, 5, ds));
+    RC(precip(e, SKT_QI, 2, ds));
+    RC(advect(e, SKT_QI, e.zeros, bQ, ds));
+    RC(microSky(e, dt, speed));
+    e.params.time += speed * dt * float(!e.params.pause_diurnal);
+    if (e.params.time > 86400.0f) e.params.time = 0.0f;
+    e.params.ground_changed = 0;
+    HIPCHK(e, hipGetLastError());
+    return SKT_OK;
+}
+
+void freeAll(EnvState& e) {
+    auto fr = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    for (int k = 0; k < SKT_FIELD_COUNT; k++) fr(e.f[k]);
+    for (int k = 0; k < 4; k++) fr(e.work[k]);
+    for (int k = 0; k < 6; k++) fr(e.pcg[k]);
+    fr(e.gh);
+    fr(e.A);
+    fr(e.zeros);
+    fr(e.ones);
+    fr(e.partials);
+    fr(e.sc);
+    fr(e.phase);
+    fr(e.region);
+    if (e.sc_host) (void)hipHostFree(e.sc_host);
+    if (e.own_stream) (void)hipStreamDestroy(e.own_stream);
+}
+
+template <class T>
+int dalloc(EnvState& e, T** p, size_t bytes) {
+    hipError_t r = hipMalloc((void**)p, bytes);
+    if (r != hipSuccess) return setErr(e, SKT_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(r));
+    return SKT_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI ---
+extern "C" {
+
+void skt_config_default(skt_config* c, int32_t nx, int32_t ny, int32_t nz, float dx) {
+    if (!c) return;
+    c->nx = nx;
+    c->ny = ny;
+    c->nz = nz;
+    c->dx = dx;
+    c->device = 0;
+    c->pcg_mode = SKT_PCG_REFERENCE;
+    c->pcg_tol = 1e-5f;
+    c->pcg_max_iter = 200;
+    c->diffusion_sweeps = 20;
+    c->max_substeps = 100;
+    c->maxabs_mode = 0;
+    c->noise_seed = 100u;
+    c->init_noise = 1;
+}
+
+int skt_env_create(const skt_config* cfg, skt_env** out) {
+    if (!cfg || !out) return SKT_ERR_ARG;
+    *out = nullptr;
+    if (cfg->nx < 1 || cfg->ny < 2 || cfg->nz < 1 || !(cfg->dx > 0.0f)) return SKT_ERR_ARG;
+    const long N = (long)cfg->nx * cfg->ny * cfg->nz;
+    if (N > (1L << 31) - 1 || cfg->diffusion_sweeps < 0 || cfg->pcg_max_iter < 0 || cfg->max_substeps < 0)
+        return SKT_ERR_ARG;
+    skt_env* env = new skt_env();
+    EnvState& e = env->s;
+    e.cfg = *cfg;
+    e.X = cfg->nx;
+    e.Y = cfg->ny;
+    e.Z = cfg->nz;
+    e.N = N;
+    e.G = (long)cfg->nx * cfg->nz;
+    e.dx = cfg->dx;
+    e.device = cfg->device;
+    auto bail = [&](int rc) {
+        freeAll(e);
+        delete env;
+        return rc;
+    };
+    if (hipSetDevice(e.device) != hipSuccess) return bail(SKT_ERR_HIP);
+    if (hipStreamCreateWithFlags(&e.own_stream, hipStreamNonBlocking) != hipSuccess) return bail(SKT_ERR_HIP);
+    e.stream = e.own_stream;
+    for (int k = 0; k < SKT_FIELD_COUNT; k++)
+        if (dalloc(e, &e.f[k], sizeof(float) * fieldLen(e, k))) return bail(SKT_ERR_NOMEM);
+    for (int k = 0; k < 4; k++)
+        if (dalloc(e, &e.work[k], sizeof(float) * std::max(e.N, 2 * e.G))) return bail(SKT_ERR_NOMEM);
+    for (int k = 0; k < 6; k++)
+        if (dalloc(e, &e.pcg[k], sizeof(float) * e.N)) return bail(SKT_ERR_NOMEM);
+    if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.A, sizeof(float4) * e.N) ||
+        dalloc(e, &e.zeros, sizeof(float) * e.Y) || dalloc(e, &e.ones, sizeof(float) * e.Y) ||
+        dalloc(e, &e.phase, (size_t)e.N))
+        return bail(SKT_ERR_NOMEM);
+    e.maxBlocks = blocks3d(e);
+    if (dalloc(e, &e.partials, sizeof(double) * 2 * (size_t)e.maxBlocks) ||
+        dalloc(e, &e.region, sizeof(double) * 33 * (size_t)e.maxBlocks + 64 * sizeof(float)) ||
+        dalloc(e, &e.sc, sizeof(Scalars)))
+        return bail(SKT_ERR_NOMEM);
+    if (hipHostMalloc((void**)&e.sc_host, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) return bail(SKT_ERR_NOMEM);
+    // environmentGPU ctor defaults (Game/environment.cu:151-168)
+    for (int k = 0; k < SKT_FIELD_COUNT; k++)
+        if (hipMemsetAsync(e.f[k], 0, sizeof(float) * fieldLen(e, k), e.stream) != hipSuccess) return bail(SKT_ERR_HIP);
+    if (hipMemsetAsync(e.gh, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
+        hipMemsetAsync(e.sc, 0, sizeof(Scalars), e.stream) != hipSuccess ||
+        hipMemsetAsync(e.phase, 0, (size_t)e.N, e.stream) != hipSuccess ||
+        hipMemsetAsync(e.A, 0, sizeof(float4) * e.N, e.stream) != hipSuccess)
+        return bail(SKT_ERR_HIP);
+    for (int k = 0; k < 6; k++)
+        if (hipMemsetAsync(e.pcg[k], 0, sizeof(float) * e.N, e.stream) != hipSuccess) return bail(SKT_ERR_HIP);
+    if (launch_fill(e, e.f[SKT_QRS], 0.001f, e.G) || launch_fill(e, e.f[SKT_GROUND_P], 1000.0f, e.G) ||
+        launch_fill(e, e.f[SKT_GROUND_T], 315.15f, e.G) || launch_fill(e, e.zeros, 0.0f, e.Y) ||
+        launch_fill(e, e.ones, 1.0f, e.Y))
+        return bail(SKT_ERR_HIP);
+    if (hipStreamSynchronize(e.stream) != hipSuccess) return bail(SKT_ERR_HIP);
+    // Gamma constants (Game/kernelSky.cu:30-37, Game/microPhysics.cu:31-61)
+    // float arguments from b = 0.8f, d = 0.25f; FP64 tgamma rounded once
+    const float b = 0.8f, d = 0.25f;
+    auto G = [](float x) { return (float)std::tgamma((double)x); };
+    e.gm.R48 = G(4.0f + b);
+    e.gm.S425 = G(4.0f + d);
+    e.gm.gR = G(3 + b);
+    e.gm.gER = G((b + 5.0f) / 2.0f);
+    e.gm.gS = G(3 + d);
+    e.gm.gRC = G(6 + b);
+    e.gm.gSS = G((d + 5.0f) / 2.0f);
+    e.gm.gI = G(3.5f);
+    e.gm.gRI = G(2.75f);
+    e.params = skt_env_params{43200.0f, 130, 52.37f, 1.0f, 0, 1};
+    std::memset(&e.stats, 0, sizeof(e.stats));
+    *out = env;
+    return SKT_OK;
+}
+
+int skt_env_destroy(skt_env* env) {
+    if (!env) return SKT_ERR_ARG;
+    (void)hipSetDevice(env->s.device);
+    (void)hipStreamSynchronize(env->s.stream);
+    freeAll(env->s);
+    delete env;
+    return SKT_OK;
+}
+
+const char* skt_env_last_error(const skt_env* env) { return env ? env->s.err.c_str() : "null environment"; }
+
+int skt_env_set_stream(skt_env* env, void* stream) {
+    if (!env) return SKT_ERR_ARG;
+    env->s.stream = stream ? (hipStream_t)stream : env->s.own_stream;
+    return SKT_OK;
+}
+
+int skt_env_synchronize(skt_env* env) {
+    if (!env) return SKT_ERR_ARG;
+    HIPCHK(env->s, hipStreamSynchronize(env->s.stream));
+    return SKT_OK;
+}
+
+int skt_env_set_ground_height(skt_env* env, const int32_t* gh) {
+    if (!env || !gh) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    for (long g = 0; g < e.G; g++)
+        if (gh[g] < 0 || gh[g] >= e.Y) return setErr(e, SKT_ERR_ARG, "ground height outside [0, ny)");
+    HIPCHK(e, hipMemcpyAsync(e.gh, gh, sizeof(int) * e.G, hipMemcpyHostToDevice, e.stream));
+    HIPCHK(e, hipStreamSynchronize(e.stream));
+    e.params.ground_changed = 1;
+    return SKT_OK;
+}
+
+int skt_env_init(skt_env* env, const float* th, const float* vel, const float* qv, float* gT, const float* gP,
+                 const float* p, const float* pS) {
+    if (!env || !th || !vel || !qv || !gT || !gP || !p || !pS) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    HIPCHK(e, hipSetDevice(e.device));
+    std::vector<float> comp((size_t)e.N);
+    auto upload = [&](int f, const float* src) -> int {
+        HIPCHK(e, hipMemcpyAsync(e.f[f], src, sizeof(float) * fieldLen(e, f), hipMemcpyHostToDevice, e.stream));
+        return SKT_OK;
+    };
+    RC(upload(SKT_POTTEMP, th));
+    for (int c = 0; c < 3; c++) {
+        for (long i = 0; i < e.N; i++) comp[i] = vel[3 * i + c];
+        RC(upload(SKT_VELX + c, comp.data()));
+        HIPCHK(e, hipStreamSynchronize(e.stream));
+    }
+    RC(upload(SKT_QV, qv));
+    RC(upload(SKT_GROUND_T, gT));
+    RC(upload(SKT_GROUND_P, gP));
+    RC(upload(SKT_DEFAULT_PRESSURE, pS));
+    RC(upload(SKT_PRESSURE, p));
+    HIPCHK(e, launch_init_density(e, 1));
+    HIPCHK(e, launch_profiles_from_column(e));
+    if (e.cfg.init_noise) {
+        const uint32_t s = e.cfg.noise_seed;
+        HIPCHK(e, launch_random(e, e.f[SKT_POTTEMP], -0.1f, 0.1f, s + 0));
+        HIPCHK(e, launch_random(e, e.f[SKT_QV], 0.0f, 0.0001f, s + 1));
+        HIPCHK(e, launch_random(e, e.f[SKT_VELX], 0.5f, 0.5f, s + 2));
+        HIPCHK(e, launch_random(e, e.f[SKT_VELY], -0.2f, 0.2f, s + 3));
+        HIPCHK(e, launch_random(e, e.f[SKT_VELZ], 0.5f, 0.5f, s + 4));
+    }
+    HIPCHK(e, launch_editor_reset(e, 10));
+    HIPCHK(e, launch_isen_temp_ground(e));
+    HIPCHK(e, hipStreamSynchronize(e.stream));
+    e.params = skt_env_params{43200.0f, 130, 52.37f, 1.0f, 0, 1};
+    e.initialized = true;
+    return SKT_OK;
+}
+
+int skt_env_update(skt_env* env, float dt, float speed) {
+    if (!env) return SKT_ERR_ARG;
+    if (!env->s.initialized) return setErr(env->s, SKT_ERR_STATE, "update before init");
+    return update(env->s, dt, speed);
+}
+
+int skt_env_micro_physics_ground(skt_env* env, float dt, float speed, float irr) {
+    if (!env) return SKT_ERR_ARG;
+    HIPCHK(env->s, launch_ground_columns(env->s, 4 | 8, dt, speed, irr));
+    return SKT_OK;
+}
+int skt_env_micro_physics_sky(skt_env* env, float dt, float speed) {
+    if (!env) return SKT_ERR_ARG;
+    return microSky(env->s, dt, speed);
+}
+int skt_env_diffuse(skt_env* env, int32_t field, int32_t type, float dt) {
+    if (!env || !validField(field) || field > SKT_PRESSURE || type < 0 || type > 5) return SKT_ERR_ARG;
+    return diffuse(env->s, field, type, dt);
+}
+int skt_env_advect_ground_water(skt_env* env, float dt, float speed) {
+    if (!env) return SKT_ERR_ARG;
+    HIPCHK(env->s, launch_ground_water_snapshot(env->s, dt, speed));
+    return SKT_OK;
+}
+int skt_env_set_temps_at_ground(skt_env* env, float dt, float speed) {
+    if (!env) return SKT_ERR_ARG;
+    HIPCHK(env->s, launch_ground_columns(env->s, 16, dt, speed, 0.0f));
+    return SKT_OK;
+}
+int skt_env_advect_ppm(skt_env* env, int32_t field, int32_t profile, skt_bounds bounds, float dt) {
+    if (!env || !validField(field) || field > SKT_PRESSURE) return SKT_ERR_ARG;
+    if (profile >= 0 && (profile < SKT_ISEN_TEMP || profile >= SKT_FIELD_COUNT)) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    e.stats.n_advect = 0;
+    const float* prof = profile < 0 ? e.zeros : e.f[profile];
+    return advect(e, field, prof, Bounds{bounds.sides, bounds.up, bounds.ground}, dt);
+}
+int skt_env_advect_precip(skt_env* env, int32_t field, int32_t fall_type, float dt) {
+    if (!env || !validField(field) || field > SKT_PRESSURE || fall_type < 0 || fall_type > 2) return SKT_ERR_ARG;
+    return precip(env->s, field, fall_type, dt);
+}
+int skt_env_pressure_project(skt_env* env, float dt) {
+    if (!env) return SKT_ERR_ARG;
+    return pressureProject(env->s, dt, 0);
+}
+int skt_env_calculate_pressure_project(skt_env* env, float dt) {
+    if (!env) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    HIPCHK(e, launch_reset_vel(e));
+    HIPCHK(e, launch_build_A(e));
+    return solvePressure(e, dt);
+}
+int skt_env_editor_data(skt_env* env) {
+    if (!env) return SKT_ERR_ARG;
+    return editorData(env->s);
+}
+int skt_env_irradiance(skt_env* env, float* out) {
+    if (!env || !out) return SKT_ERR_ARG;
+    *out = irradianceOf(env->s.params);
+    return SKT_OK;
+}
+int skt_env_ground_coverage_factor(skt_env* env) {
+    if (!env) return SKT_ERR_ARG;
+    HIPCHK(env->s, launch_ground_columns(env->s, 1, 0.0f, 0.0f, 0.0f));
+    return SKT_OK;
+}
+int skt_env_update_ground_temps(skt_env* env, float dt, float speed, float irr) {
+    if (!env) return SKT_ERR_ARG;
+    HIPCHK(env->s, launch_ground_columns(env->s, 2, dt, speed, irr));
+    return SKT_OK;
+}
+int skt_env_calculate_buoyancy(skt_env* env, float dt) {
+    if (!env) return SKT_ERR_ARG;
+    HIPCHK(env->s, launch_buoyancy(env->s, dt));
+    return SKT_OK;
+}
+int skt_env_init_density(skt_env* env) {
+    if (!env) return SKT_ERR_ARG;
+    HIPCHK(env->s, launch_init_density(env->s, 1));
+    return SKT_OK;
+}
+int skt_env_calculate_new_pressure(skt_env* env) {
+    if (!env) return SKT_ERR_ARG;
+    HIPCHK(env->s, launch_new_pressure(env->s));
+    return SKT_OK;
+}
+
+int skt_env_get_param_array(skt_env* env, int32_t parameter, int32_t direction, float** dev_ptr) {
+    if (!env || !dev_ptr) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    *dev_ptr = nullptr;
+    switch (parameter) {
+        case SKT_PARAM_POTTEMP: *dev_ptr = e.f[SKT_POTTEMP]; break;
+        case SKT_PARAM_QV: *dev_ptr = e.f[SKT_QV]; break;
+        case SKT_PARAM_QW: *dev_ptr = e.f[SKT_QW]; break;
+        case SKT_PARAM_QC: *dev_ptr = e.f[SKT_QC]; break;
+        case SKT_PARAM_QR: *dev_ptr = e.f[SKT_QR]; break;
+        case SKT_PARAM_QS: *dev_ptr = e.f[SKT_QS]; break;
+        case SKT_PARAM_QI: *dev_ptr = e.f[SKT_QI]; break;
+        case SKT_PARAM_WIND:
+            if (direction == SKT_DIR_LEFT || direction == SKT_DIR_RIGHT) *dev_ptr = e.f[SKT_VELX];
+            else if (direction == SKT_DIR_UP || direction == SKT_DIR_DOWN) *dev_ptr = e.f[SKT_VELY];
+            else if (direction == SKT_DIR_FORWARD || direction == SKT_DIR_BACKWARD) *dev_ptr = e.f[SKT_VELZ];
+            break;
+        case SKT_PARAM_PRESSURE: *dev_ptr = e.f[SKT_PRESSURE]; break;
+        default: break;
+    }
+    // PGROUND and unknown ids: the reference prints an error and returns nullptr
+    return *dev_ptr ? SKT_OK : setErr(e, SKT_ERR_ARG, "getParamArray: parameter not supported");
+}
+
+int skt_env_get_field_ptr(skt_env* env, int32_t field, void** dev_ptr, int64_t* count) {
+    if (!env || !validField(field) || !dev_ptr) return SKT_ERR_ARG;
+    *dev_ptr = env->s.f[field];
+    if (count) *count = fieldLen(env->s, field);
+    return SKT_OK;
+}
+int skt_env_copy_field_to_host(skt_env* env, int32_t field, float* host) {
+    if (!env || !validField(field) || !host) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    HIPCHK(e, hipMemcpyAsync(host, e.f[field], sizeof(float) * fieldLen(e, field), hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(e, hipStreamSynchronize(e.stream));
+    return SKT_OK;
+}
+int skt_env_copy_field_from_host(skt_env* env, int32_t field, const float* host) {
+    if (!env || !validField(field) || !host) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    HIPCHK(e, hipMemcpyAsync(e.f[field], host, sizeof(float) * fieldLen(e, field), hipMemcpyHostToDevice, e.stream));
+    HIPCHK(e, hipStreamSynchronize(e.stream));
+    return SKT_OK;
+}
+int skt_env_set_params(skt_env* env, const skt_env_params* p) {
+    if (!env || !p) return SKT_ERR_ARG;
+    env->s.params = *p;
+    return SKT_OK;
+}
+int skt_env_get_params(skt_env* env, skt_env_params* p) {
+    if (!env || !p) return SKT_ERR_ARG;
+    *p = env->s.params;
+    return SKT_OK;
+}
+int skt_env_get_stats(skt_env* env, skt_step_stats* st) {
+    if (!env || !st) return SKT_ERR_ARG;
+    *st = env->s.stats;
+    return SKT_OK;
+}
+int skt_env_compute_masks(skt_env* env, uint8_t* host) {
+    if (!env || !host) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    uint8_t* d = nullptr;
+    if (dalloc(e, &d, 12 * (size_t)e.N)) return SKT_ERR_NOMEM;
+    hipError_t r = launch_masks(e, d);
+    if (r == hipSuccess) r = hipMemcpyAsync(host, d, 12 * (size_t)e.N, hipMemcpyDeviceToHost, e.stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(e.stream);
+    (void)hipFree(d);
+    if (r != hipSuccess) return setErr(e, SKT_ERR_HIP, std::string("masks: ") + hipGetErrorString(r));
+    return SKT_OK;
+}
+int skt_env_phase_mask(skt_env* env, uint8_t* host) {
+    if (!env || !host) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    HIPCHK(e, hipMemcpyAsync(host, e.phase, (size_t)e.N, hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(e, hipStreamSynchronize(e.stream));
+    return SKT_OK;
+}
+int skt_mphys_region_set(skt_env* env, const int32_t mn[3], const int32_t mx[3]) {
+    if (!env || !mn || !mx) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    for (int k = 0; k < 3; k++) {
+        e.rmin[k] = mn[k];
+        e.rmax[k] = mx[k];
+    }
+    e.region_active = 1;
+    std::memset(e.region_host, 0, sizeof(e.region_host));
+    return SKT_OK;
+}
+int skt_mphys_region_clear(skt_env* env) {
+    if (!env) return SKT_ERR_ARG;
+    env->s.region_active = 0;
+    return SKT_OK;
+}
+int skt_mphys_region_get(skt_env* env, float out[33]) {
+    if (!env || !out) return SKT_ERR_ARG;
+    std::memcpy(out, env->s.region_host, sizeof(env->s.region_host));
+    return SKT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,199 @@
+// Device-side physics and boundary helpers for the skewT cloud step on gfx950.
+//
+// Formulas follow the reference's GPU path (Game/meteoformulas.cuh,
+// Game/microPhysics.cu, Game/kernelSky.cu); each expression keeps the
+// reference's operand order so that float rounding matches term by term.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace skt {
+
+// Physical constants, Game/environment.cu:28-60.
+namespace kc {
+constexpr float g = 9.8076f, Hv = 2501000.0f, R = 8.3145f, Rsd = 287.0528f, Rsw = 461.5f;
+constexpr float E = 287.0528f / 461.5f;
+constexpr float Cpd = 1003.5f, Cvl = 4119.0f, Cpi = 2093.0f, Cpds = 800.0f, Mw = 18.02f;
+constexpr float E0v = 2.374e+6f, Lf = 3.3355e+5f, Ls = 2.834e+6f, Ka = 2.40e-2f;
+constexpr float PI = 3.14159265359f, oo = 5.67e-8f, ge = 0.95f;
+}  // namespace kc
+
+enum { BC_NEUMANN = 0, BC_DIRICHLET = 1, BC_CUSTOM = 2 };
+
+struct Bounds {
+    int sides, up, ground;
+};
+
+// Grid geometry. Index = x + X*(y + Y*z); y is vertical; gh[x + X*z] is the
+// highest ground level of each column (cells with y <= gh are ground).
+struct Geom {
+    int X, Y, Z;
+    long N;
+    const int* __restrict__ gh;
+};
+
+struct Gammas {
+    float R48, S425;                    // fall speeds: Gamma(4+b), Gamma(4+d)
+    float gR, gER, gS, gRC, gSS, gI, gRI;  // microphysics gammas
+};
+
+__device__ __forceinline__ long gidx(const Geom& G, int x, int y, int z) {
+    return (long)x + (long)G.X * ((long)y + (long)G.Y * z);
+}
+__device__ __forceinline__ bool isGround(const Geom& G, int x, int y, int z) {
+    return y <= G.gh[x + G.X * z];
+}
+__device__ __forceinline__ float bcv(int cond, float data, float cust) {
+    return cond == BC_NEUMANN ? data : (cond == BC_DIRICHLET ? 0.0f : cust);
+}
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Stencil read of field q at (px,py,pz) from a sky cell: raw for inside sky,
+// the ground condition for ground cells, and the condition of the last inside
+// cell for positions beyond the domain (halo of Game/kernelSky.cu:1765-2041;
+// the top face takes the sides condition).
+__device__ __forceinline__ float extv(const Geom& G, const float* __restrict__ q, const float* __restrict__ prof,
+                                      Bounds b, int px, int py, int pz) {
+    const int bx = clampi(px, 0, G.X - 1), by = clampi(py, 0, G.Y - 1), bz = clampi(pz, 0, G.Z - 1);
+    const bool out = (bx != px) | (by != py) | (bz != pz);
+    const long bi = gidx(G, bx, by, bz);
+    const float cv = prof ? prof[by] : 0.0f;
+    if (isGround(G, bx, by, bz)) return bcv(b.ground, q[bi], cv);
+    if (out) return bcv(b.sides, q[bi], cv);
+    return q[bi];
+}
+
+// Neighbour read through fillNeighbourData with the cell's own value as the
+// boundary datum (Game/kernelSky.cu:1844-1874).
+__device__ __forceinline__ float nbOwn(const Geom& G, const float* __restrict__ v, Bounds b, float cust, int x, int y,
+                                       int z, int ox, int oy, int oz) {
+    const int px = x + ox, py = y + oy, pz = z + oz;
+    const long c = gidx(G, x, y, z);
+    if (px < 0 || px >= G.X || py < 0 || py >= G.Y || pz < 0 || pz >= G.Z) return bcv(b.sides, v[c], cust);
+    if (isGround(G, px, py, pz)) return bcv(b.ground, v[c], cust);
+    return v[gidx(G, px, py, pz)];
+}
+
+__device__ __forceinline__ bool skyAt(const Geom& G, int x, int y, int z) {
+    return x >= 0 && x < G.X && y >= 0 && y < G.Y && z >= 0 && z < G.Z && !isGround(G, x, y, z);
+}
+
+// ------------------------------------------------------------ thermo ---
+// Game/meteoformulas.cuh:16-316
+__device__ __forceinline__ float d_es(float T) { return 0.61078f * expf((17.27f * T) / (T + 237.3f)); }
+__device__ __forceinline__ float d_ei(float T) { return 0.61078f * expf((21.875f * T) / (T + 265.5f)); }
+__device__ __forceinline__ float d_ws(float T, float P) {
+    float Es = d_es(T) * 10.0f;
+    return (kc::E * Es) / (P - Es);
+}
+__device__ __forceinline__ float d_wi(float T, float P) {
+    float Es = d_ei(T) * 10.0f;
+    return (kc::E * Es) / (P - Es);
+}
+__device__ __forceinline__ float d_qs(float T, float P) {
+    float Es = d_ei(T) * 10.0f;
+    return (kc::E * Es) / (P - (1 - kc::E) * Es);
+}
+__device__ __forceinline__ float d_Ni(float T) {
+    const float EI = d_ei(T) * 1000;
+    const float ES = d_es(T) * 1000;
+    return 10000 * expf((12.96f * (ES - EI)) / (EI - 0.639f));
+}
+__device__ __forceinline__ float d_pwater(float T) {
+    return 999.83311f + (0.0752f * T) - (0.0089f * T * T) + (7.36413e-5f * T * T * T) - (4.74639e-7f * T * T * T * T) +
+           (1.34888e-9f * T * T * T * T * T);
+}
+__device__ __forceinline__ float d_Lwater(float T) {
+    return (2500.8f - 2.36f * T + 0.0016f * T * T - 0.00006f * T * T * T) * 1e3f;
+}
+__device__ __forceinline__ float d_Lice(float T) { return (2834.1f - 0.29f * T - 0.004f * T * T) * 1e3f; }
+__device__ __forceinline__ float d_DQVair(float T, float P) {
+    return 2.26e-5f * powf((T + 273.15f) / 273.15f, 1.94f) * (1013.25f / P);
+}
+__device__ __forceinline__ float d_ViscAir(float T) {
+    return powf((T + 273.15f) / 273.15f, 3.0f / 2.0f) * ((273.15f + 111.0f) / (T + 111.0f)) * 1.716e-5f;
+}
+__device__ __forceinline__ float d_slope(float D, float Qj, int type) {
+    float num;
+    if (type == 0) num = kc::PI * 0.99f * 8e-2f;
+    else if (type == 1) num = kc::PI * 0.11f * 3e-2f;
+    else if (type == 2) num = kc::PI * 0.91f * 4e-4f;
+    else return 0.0f;
+    const float den = fmaxf(D * Qj * 0.001f, 1e-14f);
+    return powf((num / den), 0.25f);
+}
+// calculateFallingVelocityGPU: type 3 ('all') evaluates the rain branch only.
+__device__ __forceinline__ float3 d_fallVel(float Qr, float Qs, float Qi, float dens, int type, float GR, float GS,
+                                            float GI) {
+    const bool all = type == 3;
+    float UR = 0.0f, US = 0.0f, UI = 0.0f;
+    if (type == 0 || all) {
+        float sR = d_slope(dens, Qr, 0);
+        UR = 2115.0f * (GR / (6 * powf(sR, 0.8f))) * sqrtf(1.225f / dens);
+        UR *= 0.01f;
+    } else if (type == 1 || all) {
+        float sS = d_slope(dens, Qs, 1);
+        US = 152.93f * (GS / (6 * powf(sS, 0.25f))) * sqrtf(1.225f / dens);
+        US *= 0.01f;
+    } else if (type == 2 || all) {
+        float sI = d_slope(dens, Qi, 2);
+        UI = (GI / (6 * powf(sI, 0.5f))) * powf(4 * kc::g * 100 * 0.91f / (3 * 0.6f * dens * 0.001f), 0.5f);
+        UI *= 0.01f;
+    }
+    return make_float3(UR, US, UI);
+}
+__device__ __forceinline__ float d_cvd(float T, float P, float dens) {
+    const float EI = d_ei(T) * 1000;
+    const float ES = d_es(T) * 1000;
+    const float A = (kc::Ls / (kc::Ka * (T + 273.15f))) * (kc::Ls / (kc::Rsw * (T + 273.15f)) - 1);
+    const float B = (kc::Rsw * (T + 273.15f) * P) / (2.21f * EI);
+    const float Ni = 10000 * expf((12.96f * (ES - EI)) / (EI - 0.639f));
+    return 65.2f * ((powf(Ni, 0.5f) * (ES - EI)) / (powf(dens, 0.5f) * (A + B) * EI));
+}
+__device__ __forceinline__ float d_potTemp(float T, float Pk, float Pt) {
+    return (T + 273.15f) * powf((Pt / Pk), kc::Rsd / kc::Cpd) - 273.15f;
+}
+__device__ __forceinline__ float d_MLR(float T, float P) {
+    float TK = T + 273.15f;
+    float W = d_ws(T, P);
+    float Tw = ((kc::Rsd * TK + kc::Hv * W) / (kc::Cpd + (kc::Hv * kc::Hv * W * kc::E / (kc::Rsd * TK * TK))));
+    return Tw / P;
+}
+
+// PPM flux with van Leer slope and Walcek limiter, Game/kernelSky.cu:270-366.
+__device__ __forceinline__ float ppmFlux(float vel, float L, float Cc, float Rr, float dt, float dx) {
+    float Cn = vel * dt / dx;
+    Cn = Cn > 1.0f ? 1.0f : (Cn < -1.0f ? -1.0f : Cn);
+    const float qMin = fminf(L, fminf(Cc, Rr));
+    const float qMax = fmaxf(L, fmaxf(Cc, Rr));
+    float s = 0.0f;
+    const float dl = Cc - L, dr = Rr - Cc;
+    if (dl * dr > 0.0f) {
+        const float db = Rr - L;
+        const float md = fminf(fabsf(dr), fabsf(dl));
+        const float sl = fminf(0.5f * fabsf(db), md);
+        s = db >= 0.0f ? sl : -sl;
+    }
+    const float qLS = Cc - 0.5f * s, qRS = Cc + 0.5f * s;
+    float B = 1.0f;
+    if (qRS > qMax) {
+        const float den = qRS - Cc;
+        if (den > 1e-16f) B = fminf(B, (qMax - Cc) / den);
+    }
+    if (qLS < qMin) {
+        const float den = Cc - qLS;
+        if (den > 1e-16f) B = fminf(B, (Cc - qMin) / den);
+    }
+    float qR = Cc + B * (qRS - Cc);
+    float qL = Cc + B * (qLS - Cc);
+    if ((qR - Cc) * (Cc - qL) <= 0.0f) {
+        qR = Cc;
+        qL = Cc;
+    }
+    float qq;
+    if (Cn > 0.0f) qq = qR + Cn * (-qL - 2 * qR + 3 * Cc) + Cn * Cn * (qL + qR - 2 * Cc);
+    else qq = qR + Cn * (-2 * qL - qR + 3 * Cc) + Cn * Cn * (qL + qR - 2 * Cc);
+    return vel * qq;
+}
+
+}  // namespace skt

@@ -1,0 +1,91 @@
+// Internal state of one environment and the launch interface between the
+// host orchestration (skt_api.cpp) and the kernels (skt_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include "../../include/skt.h"
+#include "skt_device.h"
+
+namespace skt {
+
+// Device scalars shared by the reduction kernels (one 256-byte block).
+struct Scalars {
+    double sigma0, sigma1;       // PCG sigma values (stored rounded to float, kept as double slots)
+    float alpha, beta;
+    float maxr, maxr0;           // stop-test values
+    int done;                    // pressure solve finished
+    int iters;                   // PCG iterations performed
+    float maxvel;                // advection max face velocity
+    int nsub;                    // advection substeps
+    float dtsub;                 // advection substep length
+    unsigned int counter[4];     // last-block counters of the reductions
+    float tol;                   // stop tolerance in effect
+    int pad[33];
+};
+
+struct EnvState {
+    skt_config cfg;
+    int X, Y, Z;
+    long N, G;
+    float dx;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    int device = 0;
+    std::string err;
+    bool initialized = false;
+
+    float* f[SKT_FIELD_COUNT] = {nullptr};  // field arrays (sizes N, G or Y)
+    int* gh = nullptr;                      // ground height [G]
+    float* work[4] = {nullptr};             // N-sized work buffers
+    float* pcg[6] = {nullptr};              // r, z, s_a, s_b, As, pre
+    float4* A = nullptr;                    // upper coefficients (x,y,z) and diagonal (w)
+    float* zeros = nullptr;                 // [Y] zero profile
+    float* ones = nullptr;                  // [Y] unit profile
+    double* partials = nullptr;             // per-block reduction partials (2 x maxBlocks)
+    int maxBlocks = 0;
+    Scalars* sc = nullptr;                  // device scalars
+    Scalars* sc_host = nullptr;             // pinned mirror for readbacks
+    uint8_t* phase = nullptr;               // [N] microphysics phase mask
+    double* region = nullptr;               // region diagnostic partials
+    int region_active = 0;
+    int rmin[3] = {0, 0, 0}, rmax[3] = {0, 0, 0};
+    float region_host[33] = {0};
+
+    Gammas gm;
+    skt_env_params params;
+    skt_step_stats stats;
+};
+
+// ------------------------------------------------------------- launches ---
+Geom geom(const EnvState& e);
+hipError_t launch_fill(EnvState& e, float* a, float v, long n);
+hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst);
+hipError_t launch_ground_water_snapshot(EnvState& e, float dt, float speed);
+// stages: 1 cloud cover, 2 ground temp, 4 precip hitting ground, 8 ground microphysics, 16 setTempsAtGround
+hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed, float irr);
+hipError_t launch_diffuse_sweep(EnvState& e, const float* in, float* out, const float* prof, Bounds b, int type, float k);
+hipError_t launch_maxvel(EnvState& e, float dt);
+hipError_t launch_ppm_pass(EnvState& e, const float* in, float* out, const float* vel, const float* prof, Bounds b,
+                           int dir, float dt);
+hipError_t launch_precip(EnvState& e, float* q, int type, float dts);
+hipError_t launch_buoyancy(EnvState& e, float dt);
+hipError_t launch_init_density(EnvState& e, int copy_old);
+hipError_t launch_new_pressure(EnvState& e);
+hipError_t launch_reset_vel(EnvState& e);
+hipError_t launch_build_A(EnvState& e);
+hipError_t launch_divergence(EnvState& e, float dt);  // r = div, max -> maxr0, done if 0
+hipError_t launch_pcg_start(EnvState& e);              // z = pre r, sigma0 = z.r
+hipError_t launch_pcg_A(EnvState& e, int it);          // s update, As, sigma1, alpha
+hipError_t launch_pcg_B(EnvState& e, int it);          // p, r update, max r, z, beta
+hipError_t launch_update_pressure_density(EnvState& e);
+hipError_t launch_apply_projection(EnvState& e, int new_pressure);
+hipError_t launch_microphysics(EnvState& e, float dt, float speed);
+hipError_t launch_region_finalize(EnvState& e);
+hipError_t launch_masks(EnvState& e, uint8_t* dev_out);
+hipError_t launch_editor_reset(EnvState& e, int nfields);
+hipError_t launch_isen_temp_ground(EnvState& e);
+hipError_t launch_profiles_from_column(EnvState& e);
+hipError_t launch_random(EnvState& e, float* a, float mn, float mx, uint32_t seed);
+int blocks3d(const EnvState& e);
+
+}  // namespace skt

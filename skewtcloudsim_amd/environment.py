@@ -1,0 +1,217 @@
+"""Python host mirror of the reference's environmentGPU step API
+(Game/environment.cuh:39-98) over the C ABI in include/skt.h.
+
+Method names, argument meaning and call order follow the reference; errors
+raise SktError instead of breaking into a debugger. Arrays are numpy float32
+in the reference's layout (x fastest, then y vertical, then z).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELD, Bounds, Config, Params, SoundingSpec, Stats
+
+
+class SktError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_config(nx, ny, nz, dx=100.0, **overrides):
+    lib = _lib.load_library()
+    cfg = Config()
+    lib.skt_config_default(ctypes.byref(cfg), nx, ny, nz, dx)
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def make_sounding(nx, ny, nz, dx=100.0, seed=None, n_bubbles=0):
+    """Init arrays of the synthetic sounding (skt_sounding_generate).
+
+    seed=None / n_bubbles<=0: the fixed single-bubble case; otherwise the
+    randomised generator drawn from seed with n_bubbles bubbles.
+    Returns a dict with the seven init() arguments.
+    """
+    lib = _lib.load_library()
+    spec = SoundingSpec()
+    rc = lib.skt_sounding_spec_random(ctypes.byref(spec), 0 if seed is None else int(seed), nx, ny, nz, dx,
+                                      0 if seed is None else int(n_bubbles))
+    if rc:
+        raise SktError(f"skt_sounding_spec_random failed ({rc})")
+    n, g = nx * ny * nz, nx * nz
+    out = dict(potTemps=np.empty(n, np.float32), velField=np.empty(3 * n, np.float32), Qv=np.empty(n, np.float32),
+               groundTemp=np.empty(g, np.float32), groundPres=np.empty(g, np.float32),
+               pressures=np.empty(n, np.float32), smallPressure=np.empty(ny, np.float32))
+    rc = lib.skt_sounding_generate(ctypes.byref(spec), nx, ny, nz, dx, *[_ptr(out[k]) for k in (
+        "potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures", "smallPressure")])
+    if rc:
+        raise SktError(f"skt_sounding_generate failed ({rc})")
+    return out
+
+
+class EnvironmentGPU:
+    """One simulation domain resident on one GPU (environmentGPU)."""
+
+    def __init__(self, nx, ny, nz, dx=100.0, **config):
+        self.lib = _lib.load_library()
+        self.cfg = default_config(nx, ny, nz, dx, **config)
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.N, self.G = nx * ny * nz, nx * nz
+        h = ctypes.c_void_p()
+        rc = self.lib.skt_env_create(ctypes.byref(self.cfg), ctypes.byref(h))
+        if rc:
+            raise SktError(f"skt_env_create failed ({rc}) for {nx}x{ny}x{nz}")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = self.lib.skt_env_last_error(self.h)
+            raise SktError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.lib.skt_env_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- environmentGPU API ----
+    def init(self, potTemps, velField, Qv, groundTemp, groundPres, pressures, smallPressure):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in
+                (potTemps, velField, Qv, groundTemp, groundPres, pressures, smallPressure)]
+        self._chk(self.lib.skt_env_init(self.h, *[_ptr(a) for a in arrs]), "init")
+
+    def set_ground_height(self, gh):
+        gh = np.ascontiguousarray(gh, np.int32)
+        self._chk(self.lib.skt_env_set_ground_height(self.h, _ptr(gh)), "set_ground_height")
+
+    def updateGPU(self, dt, speed=1.0):
+        self._chk(self.lib.skt_env_update(self.h, dt, speed), "updateGPU")
+
+    def microPhysicsGroundGPU(self, dt, speed, irradiance):
+        self._chk(self.lib.skt_env_micro_physics_ground(self.h, dt, speed, irradiance), "microPhysicsGroundGPU")
+
+    def microPhysicsSkyGPU(self, dt, speed):
+        self._chk(self.lib.skt_env_micro_physics_sky(self.h, dt, speed), "microPhysicsSkyGPU")
+
+    def diffuseGPU(self, field, type_, dt):
+        self._chk(self.lib.skt_env_diffuse(self.h, FIELD[field], type_, dt), "diffuseGPU")
+
+    def advectGroundWater(self, dt, speed):
+        self._chk(self.lib.skt_env_advect_ground_water(self.h, dt, speed), "advectGroundWater")
+
+    def setTempsAtGround(self, dt, speed):
+        self._chk(self.lib.skt_env_set_temps_at_ground(self.h, dt, speed), "setTempsAtGround")
+
+    def advectPPMWGPU(self, field, profile, bounds, dt):
+        prof = -1 if profile is None else FIELD[profile]
+        self._chk(self.lib.skt_env_advect_ppm(self.h, FIELD[field], prof, Bounds(*bounds), dt), "advectPPMWGPU")
+
+    def advectPrecip(self, field, fall_type, dt):
+        self._chk(self.lib.skt_env_advect_precip(self.h, FIELD[field], fall_type, dt), "advectPrecip")
+
+    def pressureProject(self, dt):
+        self._chk(self.lib.skt_env_pressure_project(self.h, dt), "pressureProject")
+
+    def calculatePressureProject(self, dt):
+        self._chk(self.lib.skt_env_calculate_pressure_project(self.h, dt), "calculatePressureProject")
+
+    def editorDataGPU(self):
+        self._chk(self.lib.skt_env_editor_data(self.h), "editorDataGPU")
+
+    def irridianceGPU(self):
+        v = ctypes.c_float()
+        self._chk(self.lib.skt_env_irradiance(self.h, ctypes.byref(v)), "irridianceGPU")
+        return v.value
+
+    def groundCoverageFactor(self):
+        self._chk(self.lib.skt_env_ground_coverage_factor(self.h), "groundCoverageFactor")
+
+    def updateGroundTemps(self, dt, speed, irradiance):
+        self._chk(self.lib.skt_env_update_ground_temps(self.h, dt, speed, irradiance), "updateGroundTemps")
+
+    def calculateBuoyancy(self, dt):
+        self._chk(self.lib.skt_env_calculate_buoyancy(self.h, dt), "calculateBuoyancy")
+
+    def initDensity(self):
+        self._chk(self.lib.skt_env_init_density(self.h), "initDensity")
+
+    def calculateNewPressure(self):
+        self._chk(self.lib.skt_env_calculate_new_pressure(self.h), "calculateNewPressure")
+
+    def getParamArray(self, parameter, direction="RIGHT"):
+        p = ctypes.c_void_p()
+        self._chk(self.lib.skt_env_get_param_array(self.h, _lib.PARAM[parameter], _lib.DIRECTION[direction],
+                                                   ctypes.byref(p)), "getParamArray")
+        return p.value
+
+    # ---- state access ----
+    def field_len(self, field):
+        i = FIELD[field]
+        return self.N if i <= FIELD["PRESSURE_PROJ"] else (self.G if i <= FIELD["CLOUD_COVER"] else self.ny)
+
+    def get(self, field):
+        a = np.empty(self.field_len(field), np.float32)
+        self._chk(self.lib.skt_env_copy_field_to_host(self.h, FIELD[field], _ptr(a)), f"get {field}")
+        return a
+
+    def set(self, field, values):
+        a = np.ascontiguousarray(values, np.float32).ravel()
+        if a.size != self.field_len(field):
+            raise SktError(f"set {field}: expected {self.field_len(field)} values, got {a.size}")
+        self._chk(self.lib.skt_env_copy_field_from_host(self.h, FIELD[field], _ptr(a)), f"set {field}")
+
+    def device_ptr(self, field):
+        p, n = ctypes.c_void_p(), ctypes.c_int64()
+        self._chk(self.lib.skt_env_get_field_ptr(self.h, FIELD[field], ctypes.byref(p), ctypes.byref(n)), "ptr")
+        return p.value, n.value
+
+    def params(self):
+        p = Params()
+        self._chk(self.lib.skt_env_get_params(self.h, ctypes.byref(p)), "get_params")
+        return p
+
+    def set_params(self, **kw):
+        p = self.params()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._chk(self.lib.skt_env_set_params(self.h, ctypes.byref(p)), "set_params")
+
+    def stats(self):
+        s = Stats()
+        self._chk(self.lib.skt_env_get_stats(self.h, ctypes.byref(s)), "stats")
+        return s
+
+    def masks(self):
+        m = np.empty(12 * self.N, np.uint8)
+        self._chk(self.lib.skt_env_compute_masks(self.h, _ptr(m)), "masks")
+        return m
+
+    def phase_mask(self):
+        m = np.empty(self.N, np.uint8)
+        self._chk(self.lib.skt_env_phase_mask(self.h, _ptr(m)), "phase_mask")
+        return m
+
+    def region_set(self, mn, mx):
+        a, b = np.asarray(mn, np.int32), np.asarray(mx, np.int32)
+        self._chk(self.lib.skt_mphys_region_set(self.h, _ptr(a), _ptr(b)), "region_set")
+
+    def region_clear(self):
+        self._chk(self.lib.skt_mphys_region_clear(self.h), "region_clear")
+
+    def region_get(self):
+        out = np.empty(33, np.float32)
+        self._chk(self.lib.skt_mphys_region_get(self.h, _ptr(out)), "region_get")
+        return out
+
+    def synchronize(self):
+        self._chk(self.lib.skt_env_synchronize(self.h), "synchronize")

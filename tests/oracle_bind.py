@@ -1,0 +1,192 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) — test infrastructure.
+
+Only tests, __graft_entry__.smoke() and bench.py's CPU baseline load this.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref_meteo.so")
+
+FIELDS = [
+    "QV", "QW", "QC", "QR", "QS", "QI", "POTTEMP", "VELX", "VELY", "VELZ", "PRESSURE",
+    "DENSITY", "DENSITY_OLD", "PRESSURE_PROJ",
+    "QRS", "QGR", "QGS", "QGI", "GROUND_P", "GROUND_TIME", "GROUND_T", "CLOUD_COVER",
+    "ISEN_TEMP", "ISEN_VAPOR", "DEFAULT_VELX", "DEFAULT_VELZ", "DEFAULT_PRESSURE",
+]
+FIELD = {n: i for i, n in enumerate(FIELDS)}
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("pcg_mode", ctypes.c_int), ("pcg_tol", ctypes.c_float), ("pcg_max_iter", ctypes.c_int),
+                ("sweeps", ctypes.c_int), ("max_substeps", ctypes.c_int), ("maxabs_mode", ctypes.c_int),
+                ("noise_seed", ctypes.c_uint), ("init_noise", ctypes.c_int), ("threads", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: run 'make -C oracle'")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.sko_create.restype = ctypes.c_void_p
+        L.sko_create.argtypes = [ctypes.c_int] * 3 + [ctypes.c_float, ctypes.c_int, ctypes.POINTER(Opts)]
+        f, vp, i = ctypes.c_float, ctypes.c_void_p, ctypes.c_int
+        sig = {
+            "sko_destroy": [vp], "sko_set_ground_height": [vp, vp], "sko_init": [vp] * 8,
+            "sko_step": [vp, f, f], "sko_get": [vp, i, vp], "sko_set": [vp, i, vp], "sko_get64": [vp, i, vp],
+            "sko_set64": [vp, i, vp], "sko_set_params": [vp, f, i, f, f, i, i], "sko_get_params": [vp, vp, vp],
+            "sko_diffuse": [vp, i, i, f], "sko_advect_ppm": [vp, i, i, i, i, i, f],
+            "sko_ppm_pass": [vp, i, i, i, i, i, i, i, i, f], "sko_advect_precip": [vp, i, i, f],
+            "sko_buoyancy": [vp, f], "sko_init_density": [vp], "sko_new_pressure": [vp],
+            "sko_pressure_project": [vp, f], "sko_calc_pressure_project": [vp, f], "sko_divergence": [vp, f, vp],
+            "sko_build_A": [vp] * 6, "sko_apply_A": [vp, vp, vp], "sko_reset_vel": [vp],
+            "sko_micro_sky": [vp, f, f], "sko_micro_ground": [vp, f, f, f], "sko_ground_water": [vp, f, f],
+            "sko_cloud_cover": [vp], "sko_ground_temps": [vp, f, f, f], "sko_set_temps_at_ground": [vp, f, f],
+            "sko_editor_data": [vp], "sko_masks": [vp, vp], "sko_phase": [vp, vp],
+            "sko_stats": [vp, vp, vp, vp, vp, vp], "sko_region": [vp, i, vp, vp], "sko_region_get": [vp, vp],
+            "sko_set_threads": [i], "sko_fallvel": [f, f, f, f, i, vp], "sko_gammas": [vp],
+        }
+        for name, args in sig.items():
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = None
+        L.sko_set_threads.restype = ctypes.c_int
+        L.sko_irradiance.restype = f
+        L.sko_irradiance.argtypes = [vp]
+        for name, n in [("sko_es", 1), ("sko_ei", 1), ("sko_ws", 2), ("sko_wi", 2), ("sko_qs", 2), ("sko_Ni", 1),
+                        ("sko_pwater", 1), ("sko_Lwater", 1), ("sko_Lice", 1), ("sko_DQVair", 2),
+                        ("sko_ViscAir", 1), ("sko_MLR", 2), ("sko_cvd", 3), ("sko_potTemp", 3)]:
+            getattr(L, name).restype = f
+            getattr(L, name).argtypes = [f] * n
+        L.sko_slopePrecip.restype = f
+        L.sko_slopePrecip.argtypes = [f, f, i]
+        L.sko_potTemp64.restype = ctypes.c_double
+        L.sko_potTemp64.argtypes = [ctypes.c_double] * 3
+        L.sko_hash.restype = ctypes.c_uint32
+        L.sko_hash.argtypes = [ctypes.c_uint32]
+        L.sko_ppm_flux.restype = f
+        L.sko_ppm_flux.argtypes = [f] * 6
+        L.sko_ppm_flux64.restype = ctypes.c_double
+        L.sko_ppm_flux64.argtypes = [ctypes.c_double] * 6
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own CPU thermodynamics, if oracle/_ref was built here."""
+    if not os.path.exists(REF_SO):
+        return None
+    L = ctypes.CDLL(REF_SO)
+    f = ctypes.c_float
+    for name, n in [("ref_es", 1), ("ref_ei", 1), ("ref_ws", 2), ("ref_wi", 2), ("ref_qs", 2), ("ref_Ni", 1),
+                    ("ref_pwater", 1), ("ref_Lwater", 1), ("ref_Lice", 1), ("ref_DQVair", 2), ("ref_ViscAir", 1),
+                    ("ref_MLR", 2), ("ref_cvd", 3), ("ref_potentialTemp", 3), ("ref_gamma", 1)]:
+        getattr(L, name).restype = f
+        getattr(L, name).argtypes = [f] * n
+    L.ref_slopePrecip.restype = f
+    L.ref_slopePrecip.argtypes = [f, f, ctypes.c_int]
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    """One CPU domain of the restated step (FP32 or FP64)."""
+
+    def __init__(self, nx, ny, nz, dx=100.0, fp64=False, pcg_mode=0, pcg_tol=1e-5, pcg_max_iter=200, sweeps=20,
+                 max_substeps=100, maxabs_mode=0, noise_seed=100, init_noise=1, threads=0):
+        self.L = lib()
+        self.nx, self.ny, self.nz, self.dx = nx, ny, nz, dx
+        self.N, self.G = nx * ny * nz, nx * nz
+        o = Opts(pcg_mode, pcg_tol, pcg_max_iter, sweeps, max_substeps, maxabs_mode, noise_seed, init_noise, threads)
+        self.h = ctypes.c_void_p(self.L.sko_create(nx, ny, nz, dx, 1 if fp64 else 0, ctypes.byref(o)))
+        if threads:
+            self.L.sko_set_threads(threads)
+
+    def close(self):
+        if self.h:
+            self.L.sko_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field_len(self, field):
+        i = FIELD[field]
+        return self.N if i <= FIELD["PRESSURE_PROJ"] else (self.G if i <= FIELD["CLOUD_COVER"] else self.ny)
+
+    def init(self, potTemps, velField, Qv, groundTemp, groundPres, pressures, smallPressure):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in
+                (potTemps, velField, Qv, groundTemp, groundPres, pressures, smallPressure)]
+        self.L.sko_init(self.h, *[_p(a) for a in arrs])
+
+    def set_ground_height(self, gh):
+        gh = np.ascontiguousarray(gh, np.int32)
+        self.L.sko_set_ground_height(self.h, _p(gh))
+
+    def step(self, dt, speed=1.0):
+        self.L.sko_step(self.h, dt, speed)
+
+    def get(self, field):
+        a = np.empty(self.field_len(field), np.float32)
+        self.L.sko_get(self.h, FIELD[field], _p(a))
+        return a
+
+    def get64(self, field):
+        a = np.empty(self.field_len(field), np.float64)
+        self.L.sko_get64(self.h, FIELD[field], _p(a))
+        return a
+
+    def set(self, field, v):
+        a = np.ascontiguousarray(v, np.float32).ravel()
+        assert a.size == self.field_len(field)
+        self.L.sko_set(self.h, FIELD[field], _p(a))
+
+    def set64(self, field, v):
+        a = np.ascontiguousarray(v, np.float64).ravel()
+        assert a.size == self.field_len(field)
+        self.L.sko_set64(self.h, FIELD[field], _p(a))
+
+    def set_params(self, time=43200.0, day=130, lon=52.37, sun=1.0, pause=0, ground_changed=0):
+        self.L.sko_set_params(self.h, time, day, lon, sun, pause, ground_changed)
+
+    def params(self):
+        t, g = ctypes.c_float(), ctypes.c_int()
+        self.L.sko_get_params(self.h, ctypes.byref(t), ctypes.byref(g))
+        return t.value, g.value
+
+    def stats(self):
+        it, res, r0, nc = ctypes.c_int(), ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
+        ns = (ctypes.c_int * 16)()
+        self.L.sko_stats(self.h, ctypes.byref(it), ctypes.byref(res), ctypes.byref(r0), ns, ctypes.byref(nc))
+        return dict(pcg_iterations=it.value, pcg_residual=res.value, pcg_residual0=r0.value,
+                    substeps=list(ns)[:min(nc.value, 16)])
+
+    def masks(self):
+        m = np.empty(12 * self.N, np.uint8)
+        self.L.sko_masks(self.h, _p(m))
+        return m
+
+    def phase_mask(self):
+        m = np.empty(self.N, np.uint8)
+        self.L.sko_phase(self.h, _p(m))
+        return m
+
+    def irradiance(self):
+        return self.L.sko_irradiance(self.h)
+
+    def __getattr__(self, name):
+        # per-phase entry points: oracle.diffuse(...), oracle.micro_sky(...), ...
+        fn = getattr(self.__dict__.get("L") or lib(), "sko_" + name)
+        return lambda *a: fn(self.h, *a)

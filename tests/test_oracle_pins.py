@@ -1,0 +1,505 @@
+"""Pins of the CPU oracle (oracle/skt_oracle.cpp) — CPU only.
+
+Each check is chosen so that a plausible transcription slip (dropped term,
+wrong sign, swapped neighbour, wrong boundary) fails it:
+  * thermo functions against the reference's own CPU meteoformulas.cpp,
+    compiled from its source into oracle/_ref (skipped where absent);
+  * closed forms (Magnus at 0 C, latent heats, gamma values, hydrostatic
+    layers, PPM exact shift at Courant 1);
+  * independent formulations: numpy brute force of the Jacobi sweep with
+    padded boundaries, dense-matrix SpMV and LU solve of the pressure system;
+  * invariants: zero velocity is the identity, EOS round trip, warm-branch
+    total-water conservation, boundary masks by brute force.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+RNG = np.random.default_rng(20261203)
+
+
+def ulp_close(a, b, ulps):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    sp = np.spacing(np.maximum(np.abs(a), np.abs(b)).astype(np.float32))
+    return np.all(np.abs(a.astype(np.float64) - b.astype(np.float64)) <= ulps * sp)
+
+
+# --------------------------------------------------------------- thermo ---
+TS = np.linspace(-60.0, 40.0, 41, dtype=np.float32)
+PS = np.linspace(150.0, 1050.0, 19, dtype=np.float32)
+
+
+@pytest.mark.parametrize("name,args", [
+    ("es", 1), ("ei", 1), ("ws", 2), ("wi", 2), ("qs", 2), ("Ni", 1), ("pwater", 1), ("Lwater", 1),
+    ("Lice", 1), ("DQVair", 2), ("ViscAir", 1), ("MLR", 2)])
+def test_thermo_matches_reference_cpu_source(name, args):
+    ref = ob.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (the reference tree is only in the build container)")
+    L = ob.lib()
+    mine, theirs = getattr(L, "sko_" + name), getattr(ref, "ref_" + name)
+    worst = 0.0
+    for T in TS:
+        for P in (PS if args == 2 else [None]):
+            a = mine(T) if args == 1 else mine(T, P)
+            b = theirs(T) if args == 1 else theirs(T, P)
+            if not (np.isfinite(a) and np.isfinite(b)):
+                assert np.isnan(a) == np.isnan(b)
+                continue
+            worst = max(worst, abs(a - b) / max(abs(b), 1e-30))
+    # the reference's CPU build may evaluate pow in double: a few float ulps
+    assert worst < 2e-6, (name, worst)
+
+
+def test_slope_cvd_pottemp_match_reference_cpu_source():
+    ref = ob.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built")
+    L = ob.lib()
+    for D in (0.4, 0.9, 1.2):
+        for Q in (0.0, 1e-7, 1e-4, 3e-3):
+            for t in (0, 1, 2):
+                a, b = L.sko_slopePrecip(D, Q, t), ref.ref_slopePrecip(D, Q, t)
+                assert abs(a - b) <= 2e-6 * abs(b)
+    for T in (-35.0, -20.0, -5.0):
+        for P in (300.0, 600.0, 900.0):
+            for d in (0.0005, 0.001):
+                a, b = L.sko_cvd(T, P, d), ref.ref_cvd(T, P, d)
+                assert abs(a - b) <= 5e-6 * abs(b)
+    for T in (-40.0, 0.0, 25.0):
+        for Pk in (700.0, 950.0):
+            for Pt in (1000.0, 850.0):
+                a, b = L.sko_potTemp(T, Pk, Pt), ref.ref_potentialTemp(T, Pk, Pt)
+                assert abs(a - b) <= 2e-6 * max(abs(b), 1.0)
+
+
+def test_thermo_closed_forms():
+    L = ob.lib()
+    assert L.sko_es(0.0) == np.float32(0.61078)  # Magnus at 0 C
+    assert L.sko_ei(0.0) == np.float32(0.61078)
+    assert L.sko_Lwater(0.0) == np.float32(2500.8e3)
+    assert L.sko_Lice(0.0) == np.float32(2834.1e3)
+    E = np.float32(287.0528) / np.float32(461.5)
+    for T, P in ((10.0, 900.0), (-15.0, 500.0)):
+        es10 = np.float32(L.sko_es(T)) * np.float32(10)
+        assert L.sko_ws(T, P) == np.float32(E * es10 / (np.float32(P) - es10))
+    # potential temperature is the identity at the reference pressure
+    for T in (-30.0, 15.0):
+        assert abs(L.sko_potTemp64(T, 850.0, 850.0) - T) < 1e-12
+    # inverting potTemp recovers T (FP64)
+    th = L.sko_potTemp64(5.0, 700.0, 1000.0)
+    assert abs(L.sko_potTemp64(th, 1000.0, 700.0) - 5.0) < 1e-9
+    # moist lapse rate below the dry adiabat dT/dp = Rd T/(cp p)
+    T, P = 20.0, 900.0
+    dry = 287.0528 * (T + 273.15) / (1003.5 * P)
+    assert 0.3 * dry < L.sko_MLR(T, P) < dry
+
+
+def test_gammas_closed_form():
+    out = np.zeros(10, np.float32)
+    ob.lib().sko_gammas(out.ctypes.data_as(ob.ctypes.c_void_p))
+    R48, S425, gR, gER, gS, gRC, gSS, gI, gRI = out[:9]
+    assert gI == np.float32(15.0 * math.sqrt(math.pi) / 8.0)  # Gamma(3.5)
+    b, d = 0.8, 0.25
+    for got, x in ((R48, 4 + b), (S425, 4 + d), (gR, 3 + b), (gER, (b + 5) / 2), (gS, 3 + d), (gRC, 6 + b),
+                   (gSS, (d + 5) / 2), (gRI, 2.75)):
+        assert ulp_close(got, math.gamma(np.float32(x)), 1)
+
+
+def test_fall_velocity_all_branch_is_rain_only():
+    # quirk A4: type 3 ('all') evaluates the rain branch only
+    L = ob.lib()
+    o3 = np.zeros(3, np.float32)
+    L.sko_fallvel(1e-3, 1e-3, 1e-3, 1.1, 3, o3.ctypes.data_as(ob.ctypes.c_void_p))
+    o0 = np.zeros(3, np.float32)
+    L.sko_fallvel(1e-3, 0.0, 0.0, 1.1, 0, o0.ctypes.data_as(ob.ctypes.c_void_p))
+    assert o3[0] == o0[0] and o3[0] > 1.0 and o3[1] == 0.0 and o3[2] == 0.0
+
+
+# ----------------------------------------------------------------- PPM ---
+def test_ppm_flux_courant_one_is_upwind_value():
+    L = ob.lib()
+    for _ in range(50):
+        l, c, r = RNG.normal(size=3)
+        # C = vel*dt/dx = 1 selects the donor cell value: flux = vel * valC
+        f = L.sko_ppm_flux64(2.0, l, c, r, 50.0, 100.0)
+        assert abs(f - 2.0 * c) < 1e-12
+
+
+def test_ppm_flux_monotone_and_constant():
+    L = ob.lib()
+    for _ in range(200):
+        l, c, r = RNG.normal(size=3)
+        Cn = RNG.uniform(0.01, 0.99)
+        f = L.sko_ppm_flux64(1.0, l, c, r, Cn, 1.0)  # vel 1, dt Cn, dx 1
+        face = f  # vel = 1
+        assert min(l, c, r) - 1e-12 <= face <= max(l, c, r) + 1e-12
+    for v in (-3.0, 0.5, 2.0):
+        assert abs(L.sko_ppm_flux64(v, 4.0, 4.0, 4.0, 0.3, 1.0) - 4.0 * v) < 1e-12
+
+
+def _small_state(nx=10, ny=9, nz=8, dx=100.0, fp64=False, seed=3, **kw):
+    import skewtcloudsim_amd as skt
+    snd = skt.make_sounding(nx, ny, nz, dx, seed=seed, n_bubbles=2)
+    o = ob.Oracle(nx, ny, nz, dx, fp64=fp64, **kw)
+    o.init(*[snd[k] for k in ("potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures",
+                              "smallPressure")])
+    return o, snd
+
+
+def test_ppm_zero_velocity_is_identity():
+    o, _ = _small_state(fp64=True)
+    for f in ("VELX", "VELY", "VELZ"):
+        o.set64(f, np.zeros(o.N))
+    q = RNG.uniform(0, 1, o.N)
+    o.set64("QW", q)
+    before = o.get64("QW")
+    o.advect_ppm(ob.FIELD["QW"], -1, 1, 1, 1, 0.5)
+    # zero velocity: zero substeps (quirk A15), nothing moves
+    assert np.array_equal(o.get64("QW"), before)
+    assert o.stats()["substeps"] == [0]
+
+
+def test_ppm_x_pass_courant_one_shifts_one_cell():
+    nx, ny, nz = 12, 6, 5
+    o, _ = _small_state(nx, ny, nz, fp64=True)
+    o.set64("VELX", np.full(o.N, 4.0))
+    q = RNG.uniform(1, 2, (nz, ny, nx))
+    o.set64("QW", q.ravel())
+    o.set64("QS", np.zeros(o.N))
+    # one X pass, dt = dx/u: the field moves one cell in +x (interior cells)
+    o.ppm_pass(ob.FIELD["QW"], ob.FIELD["QS"], ob.FIELD["VELX"], -1, 2, 2, 2, 0, 25.0)
+    out = o.get64("QS").reshape(nz, ny, nx)
+    sky = slice(1, ny)
+    np.testing.assert_allclose(out[:, sky, 3:nx - 2], q[:, sky, 2:nx - 3], rtol=1e-12, atol=0)
+
+
+def test_ppm_constant_field_preserved_with_custom_bounds():
+    o, _ = _small_state(fp64=True)
+    o.set64("VELX", RNG.uniform(-3, 3, o.N))
+    o.set64("VELY", RNG.uniform(-3, 3, o.N))
+    o.set64("VELZ", RNG.uniform(-3, 3, o.N))
+    o.set64("ISEN_VAPOR", np.full(o.ny, 0.01))
+    o.set64("QV", np.full(o.N, 0.01))
+    o.advect_ppm(ob.FIELD["QV"], ob.FIELD["ISEN_VAPOR"], 2, 2, 2, 0.5)
+    qv = o.get64("QV").reshape(o.nz, o.ny, o.nx)[:, 1:, :]
+    np.testing.assert_allclose(qv, 0.01, rtol=1e-12)
+
+
+# ----------------------------------------------------------- diffusion ---
+def _np_jacobi(q, prof, k, sides, ground, sweeps):
+    """Jacobi sweeps with boundary values by np.pad on a flat-ground grid
+    (row y=0 is ground); the top face takes the sides condition."""
+    nz, ny, nx = q.shape
+    for _ in range(sweeps):
+        p = np.empty((nz + 2, ny + 2, nx + 2))
+        p[1:-1, 1:-1, 1:-1] = q
+        def side(sl_dst, src):
+            if sides == 0:
+                p[sl_dst] = src
+            elif sides == 1:
+                p[sl_dst] = 0.0
+            else:
+                p[sl_dst] = np.broadcast_to(prof[None, :, None], p[sl_dst].shape) if src.ndim == 3 else src
+        # lateral x faces
+        for dst, srcx in ((0, 0), (nx + 1, nx - 1)):
+            v = {0: q[:, :, srcx], 1: 0 * q[:, :, srcx], 2: np.broadcast_to(prof[None, :], (nz, ny))}[sides]
+            p[1:-1, 1:-1, dst] = v
+        for dst, srcz in ((0, 0), (nz + 1, nz - 1)):
+            v = {0: q[srcz], 1: 0 * q[srcz], 2: np.broadcast_to(prof[:, None], (ny, nx))}[sides]
+            p[dst, 1:-1, 1:-1] = v
+        top = {0: q[:, ny - 1, :], 1: 0 * q[:, ny - 1, :], 2: np.full((nz, nx), prof[ny - 1])}[sides]
+        p[1:-1, ny + 1, 1:-1] = top
+        # ground row y=0 as seen from y=1
+        g = {0: q[:, 0, :], 1: 0 * q[:, 0, :], 2: np.full((nz, nx), prof[0])}[ground]
+        p[1:-1, 1, 1:-1] = g
+        s = (p[1:-1, 1:-1, :-2] + p[1:-1, 1:-1, 2:] + p[1:-1, 2:, 1:-1] + p[1:-1, :-2, 1:-1] + p[2:, 1:-1, 1:-1] +
+             p[:-2, 1:-1, 1:-1])
+        new = (q + k * s) / (1 + 6 * k)
+        new[:, 0, :] = q[:, 0, :]
+        q = new
+    return q
+
+
+@pytest.mark.parametrize("field,type_,sides,ground,prof", [
+    ("QW", 5, 0, 1, None),            # other mixing ratios: {N, D, D}
+    ("QV", 1, 2, 1, "ISEN_VAPOR"),    # vapour: {C, C, D}
+    ("VELX", 2, 2, 1, "DEFAULT_VELX"),
+    ("VELY", 3, 0, 1, None),
+])
+def test_diffusion_matches_numpy_brute_force(field, type_, sides, ground, prof):
+    nx, ny, nz = 7, 6, 5
+    o, _ = _small_state(nx, ny, nz, fp64=True, sweeps=20)
+    q = RNG.uniform(0.5, 1.5, (nz, ny, nx))
+    o.set64(field, q.ravel())
+    pr = np.zeros(ny)
+    if prof:
+        pr = RNG.uniform(0.2, 2.0, ny)
+        o.set64(prof, pr)
+    dt = 2e6  # k = 0.005 dt / dx^2 = 1: every neighbour matters
+    o.diffuse(ob.FIELD[field], type_, dt)
+    got = o.get64(field).reshape(nz, ny, nx)
+    want = _np_jacobi(q, pr, 0.005 * dt / 100.0 ** 2, sides, ground, 20)
+    np.testing.assert_allclose(got[:, 1:, :], want[:, 1:, :], rtol=1e-12, atol=1e-12)
+
+
+def test_diffusion_theta_ground_neighbour_uses_ground_temperature():
+    nx, ny, nz = 5, 6, 4
+    o, _ = _small_state(nx, ny, nz, fp64=True, sweeps=1)
+    th = RNG.uniform(299, 301, (nz, ny, nx))
+    o.set64("POTTEMP", th.ravel())
+    isen = RNG.uniform(299, 301, ny)
+    o.set64("ISEN_TEMP", isen)
+    gT = RNG.uniform(290, 310, nx * nz)
+    o.set64("GROUND_T", gT)
+    p = o.get64("PRESSURE").reshape(nz, ny, nx)
+    gP = o.get64("GROUND_P").reshape(nz, nx)
+    dt = 2e6
+    k = 0.005 * dt / 1e4
+    o.diffuse(ob.FIELD["POTTEMP"], 0, dt)
+    got = o.get64("POTTEMP").reshape(nz, ny, nx)
+    # one interior cell at y = 1: the down neighbour is the ground's potential temperature
+    z, y, x = 2, 1, 2
+    kap = 287.0528 / 1003.5
+    tg = (gT.reshape(nz, nx)[z, x]) * (gP[z, x] / p[z, y, x]) ** kap
+    s = th[z, y, x - 1] + th[z, y, x + 1] + th[z, y + 1, x] + tg + th[z + 1, y, x] + th[z - 1, y, x]
+    assert abs(got[z, y, x] - (th[z, y, x] + k * s) / (1 + 6 * k)) < 1e-9
+
+
+# ------------------------------------------------------------ pressure ---
+def _dense_A(o):
+    import ctypes
+    N = o.N
+    arrs = [np.zeros(N) for _ in range(5)]
+    o.L.sko_build_A(o.h, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs])
+    Ax, Ay, Az, Aw, pre = arrs
+    nx, ny, nz = o.nx, o.ny, o.nz
+    M = np.zeros((N, N))
+    idx = lambda x, y, z: x + nx * (y + ny * z)
+    for z in range(nz):
+        for y in range(1, ny):
+            for x in range(nx):
+                i = idx(x, y, z)
+                M[i, i] = Aw[i]
+                for (dx_, dy_, dz_, arr, lower) in ((1, 0, 0, Ax, False), (0, 1, 0, Ay, False), (0, 0, 1, Az, False),
+                                                   (-1, 0, 0, Ax, True), (0, -1, 0, Ay, True), (0, 0, -1, Az, True)):
+                    xx, yy, zz = x + dx_, y + dy_, z + dz_
+                    if not (0 <= xx < nx and 1 <= yy < ny and 0 <= zz < nz):
+                        continue
+                    j = idx(xx, yy, zz)
+                    M[i, j] = math.trunc(arr[j]) if lower else arr[i]
+    sky = np.array([((i // nx) % ny) >= 1 for i in range(N)])
+    return M, sky, Aw, pre
+
+
+def test_spmv_matches_dense_matrix_with_char_truncation():
+    o, _ = _small_state(6, 5, 4)
+    rho = RNG.uniform(0.3, 1.3, o.N).astype(np.float32)
+    o.set("DENSITY", rho)
+    M, sky, Aw, pre = _dense_A(o)
+    s = RNG.normal(size=o.N).astype(np.float32)
+    s[~sky] = 0
+    out = np.zeros(o.N, np.float32)
+    import ctypes
+    o.L.sko_apply_A(o.h, s.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
+    np.testing.assert_allclose(out[sky], (M @ s.astype(np.float64))[sky], rtol=1e-5, atol=1e-5)
+    # lower coefficients are truncated: the operator is not symmetric for light air
+    Ms = M[np.ix_(sky, sky)]
+    assert np.abs(Ms - Ms.T).max() > 0.05
+    # diagonal = -(sum of the six face coefficients), outside faces included
+    assert np.all(np.diag(Ms) >= -Ms.sum(axis=1) + np.diag(Ms) - 1e-5)
+
+
+def test_pcg_standard_mode_matches_dense_solve_for_symmetric_system():
+    # density >= 1.225 makes every coefficient exactly -1: no truncation, SPD system
+    o, _ = _small_state(6, 5, 4, fp64=True, pcg_mode=1, pcg_tol=1e-12, pcg_max_iter=500)
+    o.set64("DENSITY", np.full(o.N, 1.3))
+    o.set64("DENSITY_OLD", np.full(o.N, 1.3))
+    for f in ("VELX", "VELY", "VELZ"):
+        o.set64(f, RNG.uniform(-1, 1, o.N))
+    div = np.zeros(o.N, np.float32)
+    import ctypes
+    o.reset_vel()
+    o.L.sko_divergence(o.h, 0.5, div.ctypes.data_as(ctypes.c_void_p))
+    o.calc_pressure_project(0.5)
+    pp = o.get64("PRESSURE_PROJ")
+    M, sky, _, _ = _dense_A(o)
+    Ms = M[np.ix_(sky, sky)]
+    assert np.array_equal(Ms, Ms.T)
+    want = np.linalg.solve(Ms, div[sky].astype(np.float64))
+    np.testing.assert_allclose(pp[sky], want, rtol=1e-5, atol=1e-8 * np.abs(want).max())  # div handed over in float32
+
+
+def test_pcg_reference_mode_recurrence():
+    """Reference loop of calculatePressureProject on a dense matrix: sigma1 keeps
+    the previous z.r from the second iteration on (it is cleared only before
+    the z.r product), stop test on the signed getMaxDivergence value."""
+    o, _ = _small_state(6, 5, 4, fp64=True, pcg_mode=0, pcg_tol=0.0, pcg_max_iter=7)
+    for f in ("VELX", "VELY", "VELZ"):
+        o.set64(f, RNG.uniform(-1, 1, o.N))
+    o.reset_vel()
+    import ctypes
+    div = np.zeros(o.N, np.float32)
+    o.L.sko_divergence(o.h, 0.5, div.ctypes.data_as(ctypes.c_void_p))
+    M, sky, Aw, pre = _dense_A(o)
+    r = div.astype(np.float64).copy()
+    p = np.zeros(o.N)
+    z = np.where(sky, r * pre, 0.0)
+    s = z.copy()
+    s0 = float(z[sky] @ r[sky])
+    s1 = 0.0
+    for it in range(7):
+        As = M @ s
+        s1 = s1 + float(As[sky] @ s[sky])
+        a = s0 / s1
+        p += a * np.where(sky, s, 0)
+        r -= a * np.where(sky, As, 0)
+        z = np.where(sky, r * pre, 0.0)
+        s1 = float(z[sky] @ r[sky])
+        beta = s1 / s0
+        s = np.where(sky, z + beta * s, 0)
+        s0 = s1
+    o.calc_pressure_project(0.5)
+    got = o.get64("PRESSURE_PROJ")
+    np.testing.assert_allclose(got[sky], p[sky], rtol=1e-5, atol=1e-9)  # div handed over in float32
+    assert o.stats()["pcg_iterations"] == 7
+
+
+# ------------------------------------------------------------ EOS etc. ---
+def test_eos_round_trip():
+    o, _ = _small_state(8, 8, 8)
+    p0 = o.get("PRESSURE")
+    o.init_density()
+    o.new_pressure()
+    p1 = o.get("PRESSURE")
+    sky = (np.arange(o.N) // o.nx) % o.ny >= 1
+    assert np.max(np.abs(p1[sky] - p0[sky]) / p0[sky]) < 4e-7
+
+
+def test_masks_brute_force_with_terrain():
+    nx, ny, nz = 7, 6, 5
+    o, _ = _small_state(nx, ny, nz)
+    gh = RNG.integers(0, 3, nx * nz).astype(np.int32)
+    o.set_ground_height(gh)
+    m = o.masks().reshape(nz, ny, nx, 12)
+    G = gh.reshape(nz, nx)
+    ground = np.zeros((nz, ny, nx), bool)
+    for z in range(nz):
+        for x in range(nx):
+            ground[z, :G[z, x] + 1, x] = True
+    pad = np.pad(ground, 1, constant_values=False)
+    for (k, ax, d) in ((0, 2, -1), (2, 2, 1), (6, 1, -1), (4, 1, 1), (10, 0, -1), (8, 0, 1)):
+        n = [nz, ny, nx][ax]
+        coord = np.indices((nz, ny, nx))[ax]
+        outside = coord == (0 if d < 0 else n - 1)
+        sl = [slice(1, -1)] * 3
+        sl[ax] = slice(1 + d, n + 1 + d)
+        nb_ground = pad[tuple(sl)]
+        if k == 4:  # up: ground only for an inside neighbour
+            typ = ~outside & nb_ground
+        else:
+            typ = (outside & ground) | (~outside & nb_ground)
+        assert np.array_equal(m[..., k].astype(bool), outside), k
+        assert np.array_equal(m[..., k + 1].astype(bool), typ), k
+
+
+def test_warm_microphysics_conserves_total_water():
+    nx, ny, nz = 8, 8, 8
+    o, _ = _small_state(nx, ny, nz, fp64=True)
+    for f, hi in (("QW", 3e-3), ("QC", 1e-3), ("QR", 2e-3), ("QS", 1e-3), ("QI", 1e-3)):
+        o.set64(f, RNG.uniform(0, hi, o.N))
+    o.set64("QV", o.get64("QV") * RNG.uniform(0.8, 1.2, o.N))
+    names = ("QV", "QW", "QC", "QR", "QS", "QI")
+    before = sum(o.get64(f) for f in names)
+    o.micro_sky(0.5, 1.0)
+    after = sum(o.get64(f) for f in names)
+    ph = o.phase_mask()
+    warm = (ph & 1).astype(bool) & ~((ph & 2).astype(bool))
+    assert warm.sum() > 100
+    np.testing.assert_allclose(after[warm], before[warm], rtol=0, atol=1e-15)
+    assert not np.allclose(o.get64("QR")[warm], 0)
+
+
+def test_condensation_step_matches_closed_form():
+    """One saturation-adjustment step in a cloud-free warm cell: the vapour
+    surplus condenses as (Qv - Qws) / (1 + L/cp dQws/dT) (FPVCON with
+    speed = 1 and dt cancelling)."""
+    o, _ = _small_state(4, 4, 4, fp64=True)
+    for f in ("QW", "QC", "QR", "QS", "QI"):
+        o.set64(f, np.zeros(o.N))
+    th = o.get64("POTTEMP")
+    p = o.get64("PRESSURE")
+    gP = o.get64("GROUND_P")
+    i = 1 * o.nx + 2 * o.nx * o.ny + 1
+    kap = 287.0528 / 1003.5
+    T = th[i] * (p[i] / gP[0]) ** kap - 273.15
+    assert T > 0
+    es = 0.61078 * math.exp(17.27 * T / (T + 237.3)) * 10
+    E = 287.0528 / 461.5
+    qws = E * es / (p[i] - es)
+    qv = o.get64("QV")
+    qv[i] = 1.2 * qws
+    o.set64("QV", qv)
+    o.micro_sky(0.5, 1.0)
+    qw = o.get64("QW")
+    der = (E * p[i] / (p[i] - es) ** 2) * (es * (17.27 * 237.3 / (T + 237.3) ** 2))
+    want = (qv[i] - qws) / (1 + 2.374e6 / 1003.5 * der)
+    assert abs(qw[i] - want) < 1e-12 * max(1.0, abs(want)) + 1e-15
+
+
+def test_hydrostatic_sounding_closed_forms():
+    import ctypes
+    import skewtcloudsim_amd as skt
+    from skewtcloudsim_amd import _lib
+    lib = _lib.load_library()
+    nx, ny, nz, dx = 2, 60, 2, 200.0
+    for gamma in (0.0, 0.0065):
+        spec = _lib.SoundingSpec()
+        spec.surface_theta, spec.surface_pressure, spec.lapse_rate = 290.0, 1000.0, gamma
+        spec.tropopause_height, spec.surface_qv, spec.qv_scale_height = 1e9, 0.0, 2500.0
+        spec.shear_height = 6000.0
+        outs = [np.empty(n, np.float32) for n in (nx * ny * nz, 3 * nx * ny * nz, nx * ny * nz, nx * nz, nx * nz,
+                                                    nx * ny * nz, ny)]
+        assert lib.skt_sounding_generate(ctypes.byref(spec), nx, ny, nz, dx,
+                                         *[a.ctypes.data_as(ctypes.c_void_p) for a in outs]) == 0
+        p = outs[6].astype(np.float64)
+        zz = np.arange(ny) * dx
+        g, Rd = 9.8076, 287.0528
+        if gamma == 0.0:
+            want = 1000.0 * np.exp(-g * zz / (Rd * 290.0))
+        else:
+            want = 1000.0 * ((290.0 - gamma * zz) / 290.0) ** (g / (Rd * gamma))
+        np.testing.assert_allclose(p, want, rtol=2e-7)
+        th = outs[0].reshape(nz, ny, nx)[0, :, 0].astype(np.float64)
+        T = 290.0 - gamma * zz
+        np.testing.assert_allclose(th, T * (1000.0 / want) ** (Rd / 1003.5), rtol=2e-7)
+
+
+def test_init_noise_and_profiles():
+    """init: profiles are the (x=0,z=0) column before noise; noise follows the
+    randomCuda hash with seeds 100..104; ground cells are zeroed."""
+    import skewtcloudsim_amd as skt
+    nx, ny, nz = 6, 5, 4
+    snd = skt.make_sounding(nx, ny, nz, 100.0, seed=11, n_bubbles=1)
+    o = ob.Oracle(nx, ny, nz, 100.0)
+    o.init(*[snd[k] for k in ("potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures",
+                              "smallPressure")])
+    th0 = snd["potTemps"].reshape(nz, ny, nx)
+    assert np.array_equal(o.get("ISEN_TEMP"), th0[0, :, 0])
+    L = ob.lib()
+
+    def noise(seed, i, mn, mx):
+        h = L.sko_hash((seed + i) & 0xFFFFFFFF)
+        r = np.float32(h % 1000) / np.float32(1000)
+        return np.float32(r * np.float32(mx - mn) + np.float32(mn))
+    th = o.get("POTTEMP")
+    for i in (nx * ny + 7, 2 * nx * ny + nx + 3):
+        assert th[i] == np.float32(snd["potTemps"][i] + noise(100, i, -0.1, 0.1))
+    u = o.get("VELX").reshape(nz, ny, nx)
+    u0 = snd["velField"].reshape(-1, 3)[:, 0].reshape(nz, ny, nx)
+    assert np.array_equal(u[:, 1:, :], u0[:, 1:, :] + np.float32(0.5))
+    assert np.all(u[:, 0, :] == 0) and np.all(o.get("QV").reshape(nz, ny, nx)[:, 0, :] == 0)
