@@ -1,0 +1,140 @@
+"""Benchmark of the plain FP32 simulation step on one MI355X.
+
+Workload: 256 x 256 x 128 voxels (x, z horizontal; 128 levels vertical) at
+dx = 100 m, dt = 0.5 s, the randomised sounding (seed 20261203, three warm
+bubbles), the reference's solver settings (20 diffusion sweeps, reference-mode
+PCG: absolute tolerance 1e-5, at most 200 iterations). One step is one
+environmentGPU::updateGPU over the whole domain. Inputs are resident in HBM
+before the timed region; the timed region is K steps bracketed by device
+synchronisation.
+
+Prints one JSON line: cell-steps/s (value), the per-kernel-class timing, the
+roofline of the dominant kernel class (algorithmic bytes per launch / average
+launch duration from HIP events on the env's stream) and the CPU baseline (the
+oracle, one step of the same workload on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NX, NY, NZ, DX, DT = 256, 128, 256, 100.0, 0.5
+SEED = 20261203
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(cls, n_sky):
+    """Compulsory HBM bytes of one launch of a kernel class (FP32, per sky cell):
+    diffusion sweep 8 (read x, write x); PPM pass 12 (read q and the face
+    velocity, write q; the unit-density field is computed, not stored);
+    PCG spmv 20 (read z, s_old and the density-derived operator as 4 B of
+    density, write s and As); PCG update 32 (read p, s, r, As, 1/diag; write
+    p, r, z); microphysics 64 (9 fields in, 7 out)."""
+    per = {"diffusion_sweep": 8, "ppm_pass": 12, "pcg_spmv": 20, "pcg_update": 32, "microphysics": 64}
+    return per.get(cls, 0) * n_sky
+
+
+def cpu_baseline(snd_args, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle
+    o = Oracle(NX, NY, NZ, DX, threads=threads)
+    o.init(*snd_args)
+    t0 = time.perf_counter()
+    o.step(DT, 1.0)
+    t1 = time.perf_counter()
+    it = o.stats()["pcg_iterations"]
+    o.close()
+    return (NX * NY * NZ) / (t1 - t0), t1 - t0, it
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the step is single-domain, one GPU per "
+              "process. See DESIGN.md for how the domain would shard.", file=sys.stderr)
+        sys.exit(2)
+
+    import skewtcloudsim_amd as skt
+    snd = skt.make_sounding(NX, NY, NZ, DX, seed=SEED, n_bubbles=3)
+    keys = ("potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures", "smallPressure")
+    env = skt.EnvironmentGPU(NX, NY, NZ, DX)
+    env.init(*[snd[k] for k in keys])
+    for _ in range(args.warmup):
+        env.updateGPU(DT, 1.0)
+    env.device_synchronize()
+
+    env.set_timing(True)
+    env.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        env.updateGPU(DT, 1.0)
+    env.device_synchronize()
+    t1 = time.perf_counter()
+    timing = env.timing()
+    env.set_timing(False)
+    st = env.stats()
+    copy_ms, copy_bytes = env.copy_probe(20)
+    wall = t1 - t0
+    n = NX * NY * NZ
+    n_sky = NX * (NY - 1) * NZ
+    value = n * args.steps / wall
+
+    # dominant kernel class by total event time
+    classes = {c: v for c, v in timing.items() if v[0] > 0}
+    dom = max(classes, key=lambda c: classes[c][1])
+    launches, total_ms = classes[dom]
+    avg_s = total_ms / launches / 1e3
+    alg = algorithmic_bytes(dom, n_sky)
+    achieved = alg / avg_s / 1e9 if alg else None
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cps, secs, it = cpu_baseline([snd[k] for k in keys], threads)
+        cpu = {"value": cps, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+               "sample": f"one step of the same {NX}x{NY}x{NZ} workload after init ({secs:.1f} s, "
+                         f"{it} PCG iterations), oracle/skt_oracle.cpp FP32, OpenMP {threads} threads"}
+
+    out = {
+        "metric": "cell-steps/s at 256x256x128 (FP32)",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded sounding, 3 warm bubbles)",
+        "config": {"workload": "plain FP32 step, 256x256x128 voxels (x,z 256; 128 vertical), dx 100 m, dt 0.5 s",
+                   "nx": NX, "ny_vertical": NY, "nz": NZ, "pcg": "reference mode, abs tol 1e-5, max 200",
+                   "pcg_iterations_last_step": st.pcg_iterations, "diffusion_sweeps": 20,
+                   "advection_substeps_last_step": list(st.substeps)[:st.n_advect]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": alg},
+        "kernel_classes": {c: {"launches": v[0], "total_ms": v[1], "avg_us": (v[1] / v[0] * 1e3) if v[0] else 0.0,
+                               "GBps_algorithmic": (algorithmic_bytes(c, n_sky) / (v[1] / v[0] / 1e3) / 1e9)
+                               if v[0] and algorithmic_bytes(c, n_sky) else None}
+                           for c, v in timing.items()},
+        "copy_probe": {"GBps": copy_bytes / (copy_ms / 1e3) / 1e9, "bytes": copy_bytes, "ms": copy_ms},
+        "cpu_baseline": cpu,
+    }
+    env.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

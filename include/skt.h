@@ -170,6 +170,23 @@ int skt_mphys_region_set(skt_env* env, const int32_t min_pos[3], const int32_t m
 int skt_mphys_region_clear(skt_env* env);
 int skt_mphys_region_get(skt_env* env, float out[33]);
 
+/* Measurement (no counterpart in the reference, whose cudaEvent timing is
+ * commented out at Game/environment.cu:698-702,816-823). With timing enabled
+ * every launch of the step is bracketed by HIP events on the env's stream and
+ * accumulated per kernel class: 0 diffusion sweep, 1 PPM pass, 2 PCG
+ * search-direction + SpMV, 3 PCG update + stop test, 4 microphysics,
+ * 5 max-velocity reduction, 6 precipitation fall, 7 everything else. */
+typedef struct skt_timing {
+    int64_t launches[8];
+    double total_ms[8];
+} skt_timing;
+int skt_env_set_timing(skt_env* env, int32_t enable); /* also clears the accumulators */
+int skt_env_get_timing(skt_env* env, skt_timing* out);
+/* Times a plain device copy of one sky field (nx*ny*nz floats) 'repeats'
+ * times: the measured copy ceiling for roofline reporting. */
+int skt_env_copy_probe(skt_env* env, int32_t repeats, float* ms_per_copy, double* bytes_per_copy);
+int skt_device_synchronize(int32_t device);
+
 /* Synthetic sounding (replaces readTable::initEnvironment,
  * Game/readTable.cpp:364-474, whose input files are not available):
  * conditionally unstable profile, hydrostatic base state integrated in FP64,

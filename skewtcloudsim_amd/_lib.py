@@ -53,6 +53,14 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Timing(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_int64 * 8), ("total_ms", ctypes.c_double * 8)]
+
+
+TIMING_CLASSES = ["diffusion_sweep", "ppm_pass", "pcg_spmv", "pcg_update", "microphysics", "maxvel", "precip_fall",
+                  "other"]
+
+
 class SoundingSpec(ctypes.Structure):
     _fields_ = [
         ("surface_theta", ctypes.c_float), ("surface_pressure", ctypes.c_float), ("lapse_rate", ctypes.c_float),
@@ -106,6 +114,10 @@ SIGNATURES = {
     "skt_mphys_region_set": (_I, [_P, _P, _P]),
     "skt_mphys_region_clear": (_I, [_P]),
     "skt_mphys_region_get": (_I, [_P, _P]),
+    "skt_env_set_timing": (_I, [_P, _I]),
+    "skt_env_get_timing": (_I, [_P, ctypes.POINTER(Timing)]),
+    "skt_env_copy_probe": (_I, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(ctypes.c_double)]),
+    "skt_device_synchronize": (_I, [_I]),
     "skt_sounding_spec_random": (_I, [ctypes.POINTER(SoundingSpec), ctypes.c_uint64, _I, _I, _I, _F, _I]),
     "skt_sounding_generate": (_I, [ctypes.POINTER(SoundingSpec), _I, _I, _I, _F, _P, _P, _P, _P, _P, _P, _P]),
 }

@@ -34,6 +34,49 @@ int setErr(EnvState& e, int code, const std::string& msg) {
         if (r_ != hipSuccess) return setErr((e), SKT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(r_)); \
     } while (0)
 
+hipEvent_t evGet(EnvState& e) {
+    if (!e.evpool.empty()) {
+        hipEvent_t ev = e.evpool.back();
+        e.evpool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+}
+
+// Launch bracketed by HIP events on the env's stream when timing is enabled.
+#define TL(e, cls, call)                                                  \
+    do {                                                                  \
+        hipEvent_t a_ = nullptr, b_ = nullptr;                            \
+        if ((e).timing) {                                                 \
+            a_ = evGet(e);                                                \
+            b_ = evGet(e);                                                \
+            if (a_) (void)hipEventRecord(a_, (e).stream);                 \
+        }                                                                 \
+        HIPCHK(e, call);                                                  \
+        if ((e).timing && a_ && b_) {                                     \
+            (void)hipEventRecord(b_, (e).stream);                         \
+            (e).pending.push_back(EnvState::TimedLaunch{cls, a_, b_});    \
+        }                                                                 \
+    } while (0)
+
+int resolveTiming(EnvState& e) {
+    if (e.pending.empty()) return SKT_OK;
+    HIPCHK(e, hipStreamSynchronize(e.stream));
+    for (auto& t : e.pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            e.t_ms[t.cls] += ms;
+            e.t_count[t.cls]++;
+        }
+        e.evpool.push_back(t.a);
+        e.evpool.push_back(t.b);
+    }
+    e.pending.clear();
+    return SKT_OK;
+}
+
 long fieldLen(const EnvState& e, int f) {
     if (f <= SKT_PRESSURE_PROJ) return e.N;
     if (f <= SKT_CLOUD_COVER) return e.G;
@@ -73,8 +116,8 @@ int diffuse(EnvState& e, int fld, int type, float dt) {
     const float k = 0.005f * dt / (e.dx * e.dx);
     float* bufs[2] = {e.f[fld], e.work[0]};
     const int n = e.cfg.diffusion_sweeps;
-    for (int s = 0; s < n; s++) HIPCHK(e, launch_diffuse_sweep(e, bufs[s & 1], bufs[(s + 1) & 1], prof, b, type, k));
-    if (n & 1) HIPCHK(e, launch_copy_sky(e, e.work[0], e.f[fld]));
+    for (int s = 0; s < n; s++) TL(e, TC_DIFFUSE, launch_diffuse_sweep(e, bufs[s & 1], bufs[(s + 1) & 1], prof, b, type, k));
+    if (n & 1) TL(e, TC_DIFFUSE, launch_copy_sky(e, e.work[0], e.f[fld]));
     return SKT_OK;
 }
 
@@ -83,7 +126,7 @@ int diffuse(EnvState& e, int fld, int type, float dt) {
 // component being advected is read in its current (partially advected)
 // state, as in the reference (quirk A11), and the result lands in place.
 int advect(EnvState& e, int fld, const float* prof, Bounds b, float dt) {
-    HIPCHK(e, launch_maxvel(e, dt));
+    TL(e, TC_MAXVEL, launch_maxvel(e, dt));
     int rc = readScalars(e);
     if (rc) return rc;
     const int nsub = e.sc_host->nsub;
@@ -95,11 +138,11 @@ int advect(EnvState& e, int fld, const float* prof, Bounds b, float dt) {
     float *T1 = e.work[1], *T2 = e.work[2], *T3 = e.work[3];
     float *U = e.f[SKT_VELX], *V = e.f[SKT_VELY], *W = e.f[SKT_VELZ];
     for (int s = 0; s < nsub; s++) {
-        HIPCHK(e, launch_ppm_pass(e, A, T1, fld == SKT_VELX ? A : U, prof, b, 0, h));
-        HIPCHK(e, launch_ppm_pass(e, T1, T2, fld == SKT_VELY ? A : V, prof, b, 1, h));
-        HIPCHK(e, launch_ppm_pass(e, T2, T1, fld == SKT_VELZ ? T2 : W, prof, b, 2, dts));
-        HIPCHK(e, launch_ppm_pass(e, T1, T3, fld == SKT_VELY ? T2 : V, prof, b, 1, h));
-        HIPCHK(e, launch_ppm_pass(e, T3, A, fld == SKT_VELX ? T3 : U, prof, b, 0, h));
+        TL(e, TC_PPM, launch_ppm_pass(e, A, T1, fld == SKT_VELX ? A : U, prof, b, 0, h));
+        TL(e, TC_PPM, launch_ppm_pass(e, T1, T2, fld == SKT_VELY ? A : V, prof, b, 1, h));
+        TL(e, TC_PPM, launch_ppm_pass(e, T2, T1, fld == SKT_VELZ ? T2 : W, prof, b, 2, dts));
+        TL(e, TC_PPM, launch_ppm_pass(e, T1, T3, fld == SKT_VELY ? T2 : V, prof, b, 1, h));
+        TL(e, TC_PPM, launch_ppm_pass(e, T3, A, fld == SKT_VELX ? T3 : U, prof, b, 0, h));
     }
     return SKT_OK;
 }
@@ -111,19 +154,19 @@ int precip(EnvState& e, int fld, int type, float dt) {
     const int sub = int(fminf(ceilf(fabsf(C)), (float)e.cfg.max_substeps));
     if (sub <= 0) return SKT_OK;
     const float dts = dt / sub;
-    for (int s = 0; s < sub; s++) HIPCHK(e, launch_precip(e, e.f[fld], type, dts));
+    for (int s = 0; s < sub; s++) TL(e, TC_PRECIP, launch_precip(e, e.f[fld], type, dts));
     return SKT_OK;
 }
 
 // calculatePressureProject, Game/environment.cu:898-987.
 int solvePressure(EnvState& e, float dt) {
-    HIPCHK(e, launch_divergence(e, dt));
-    HIPCHK(e, launch_pcg_start(e));
+    TL(e, TC_OTHER, launch_divergence(e, dt));
+    TL(e, TC_OTHER, launch_pcg_start(e));
     const int maxIt = e.cfg.pcg_max_iter;
     const int chunk = 25;
     for (int it = 0; it < maxIt; it++) {
-        HIPCHK(e, launch_pcg_A(e, it));
-        HIPCHK(e, launch_pcg_B(e, it));
+        TL(e, TC_PCG_A, launch_pcg_A(e, it));
+        TL(e, TC_PCG_B, launch_pcg_B(e, it));
         if ((it + 1) % chunk == 0 && it + 1 < maxIt) {
             int rc = readScalars(e);
             if (rc) return rc;
@@ -142,27 +185,27 @@ int solvePressure(EnvState& e, float dt) {
 // calculateNewPressure that follows it in updateGPU (:501) runs per cell in
 // the same pass.
 int pressureProject(EnvState& e, float dt, int new_pressure) {
-    HIPCHK(e, launch_reset_vel(e));
-    HIPCHK(e, launch_build_A(e));
+    TL(e, TC_OTHER, launch_reset_vel(e));
+    TL(e, TC_OTHER, launch_build_A(e));
     int rc = solvePressure(e, dt);
     if (rc) return rc;
-    HIPCHK(e, launch_update_pressure_density(e));
-    HIPCHK(e, launch_apply_projection(e, new_pressure));
+    TL(e, TC_OTHER, launch_update_pressure_density(e));
+    TL(e, TC_OTHER, launch_apply_projection(e, new_pressure));
     return SKT_OK;
 }
 
 int editorData(EnvState& e) {
     if (e.params.ground_changed) {
-        HIPCHK(e, launch_editor_reset(e, 9));
-        HIPCHK(e, launch_isen_temp_ground(e));
+        TL(e, TC_OTHER, launch_editor_reset(e, 9));
+        TL(e, TC_OTHER, launch_isen_temp_ground(e));
     }
     return SKT_OK;
 }
 
 int microSky(EnvState& e, float dt, float speed) {
-    HIPCHK(e, launch_microphysics(e, dt, speed));
+    TL(e, TC_MICRO, launch_microphysics(e, dt, speed));
     if (e.region_active) {
-        HIPCHK(e, launch_region_finalize(e));
+        TL(e, TC_OTHER, launch_region_finalize(e));
         const unsigned nb = (unsigned)blocks3d(e);
         HIPCHK(e, hipMemcpyAsync(e.region_host, (float*)(e.region + 33L * nb), 33 * sizeof(float),
                                  hipMemcpyDeviceToHost, e.stream));
@@ -183,21 +226,21 @@ int update(EnvState& e, float dt, float speed) {
     RC(editorData(e));
     const float irr = irradianceOf(e.params);
     // ground (Game/environment.cu:420-456) and setTempsAtGround (:469)
-    HIPCHK(e, launch_ground_water_snapshot(e, dt, speed));
-    HIPCHK(e, launch_ground_columns(e, 1 | 2 | 4 | 8 | 16, dt, speed, irr));
+    TL(e, TC_OTHER, launch_ground_water_snapshot(e, dt, speed));
+    TL(e, TC_OTHER, launch_ground_columns(e, 1 | 2 | 4 | 8 | 16, dt, speed, irr));
     const Bounds bT{BC_CUSTOM, BC_CUSTOM, BC_CUSTOM}, bU{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET},
         bV{BC_DIRICHLET, BC_DIRICHLET, BC_DIRICHLET}, bW{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET},
         bQ{BC_DIRICHLET, BC_DIRICHLET, BC_DIRICHLET}, bQv{BC_CUSTOM, BC_CUSTOM, BC_CUSTOM};
     RC(diffuse(e, SKT_POTTEMP, 0, ds));
     RC(advect(e, SKT_POTTEMP, e.f[SKT_ISEN_TEMP], bT, ds));
-    HIPCHK(e, launch_buoyancy(e, ds));
+    TL(e, TC_OTHER, launch_buoyancy(e, ds));
     RC(diffuse(e, SKT_VELX, 2, ds));
     RC(diffuse(e, SKT_VELY, 3, ds));
     RC(diffuse(e, SKT_VELZ, 4, ds));
     RC(advect(e, SKT_VELX, e.f[SKT_DEFAULT_VELX], bU, ds));
     RC(advect(e, SKT_VELY, e.zeros, bV, ds));
     RC(advect(e, SKT_VELZ, e.f[SKT_DEFAULT_VELZ], bW, ds));
-    HIPCHK(e, launch_init_density(e, 1));
+    TL(e, TC_OTHER, launch_init_density(e, 1));
     RC(pressureProject(e, ds, 1));
     RC(diffuse(e, SKT_QV, 1, ds));
     RC(advect(e, SKT_QV, e.f[SKT_ISEN_VAPOR], bQv, ds));
@@ -219,6 +262,7 @@ int update(EnvState& e, float dt, float speed) {
     if (e.params.time > 86400.0f) e.params.time = 0.0f;
     e.params.ground_changed = 0;
     HIPCHK(e, hipGetLastError());
+    if (e.timing) RC(resolveTiming(e));
     return SKT_OK;
 }
 
@@ -237,6 +281,11 @@ void freeAll(EnvState& e) {
     fr(e.sc);
     fr(e.phase);
     fr(e.region);
+    for (auto& t : e.pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (auto ev : e.evpool) (void)hipEventDestroy(ev);
     if (e.sc_host) (void)hipHostFree(e.sc_host);
     if (e.own_stream) (void)hipStreamDestroy(e.own_stream);
 }
@@ -583,6 +632,50 @@ int skt_env_phase_mask(skt_env* env, uint8_t* host) {
     HIPCHK(e, hipStreamSynchronize(e.stream));
     return SKT_OK;
 }
+int skt_env_set_timing(skt_env* env, int32_t enable) {
+    if (!env) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    RC(resolveTiming(e));
+    e.timing = enable ? 1 : 0;
+    for (int k = 0; k < 8; k++) {
+        e.t_ms[k] = 0.0;
+        e.t_count[k] = 0;
+    }
+    return SKT_OK;
+}
+int skt_env_get_timing(skt_env* env, skt_timing* out) {
+    if (!env || !out) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    RC(resolveTiming(e));
+    for (int k = 0; k < 8; k++) {
+        out->launches[k] = e.t_count[k];
+        out->total_ms[k] = e.t_ms[k];
+    }
+    return SKT_OK;
+}
+int skt_env_copy_probe(skt_env* env, int32_t repeats, float* ms_per_copy, double* bytes_per_copy) {
+    if (!env || repeats < 1 || !ms_per_copy) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    hipEvent_t a = evGet(e), b = evGet(e);
+    if (!a || !b) return setErr(e, SKT_ERR_HIP, "hipEventCreate failed");
+    HIPCHK(e, launch_stream_copy(e, e.work[0], e.work[1], e.N));  // warm
+    HIPCHK(e, hipEventRecord(a, e.stream));
+    for (int r = 0; r < repeats; r++) HIPCHK(e, launch_stream_copy(e, e.work[(r & 1)], e.work[1 - (r & 1)], e.N));
+    HIPCHK(e, hipEventRecord(b, e.stream));
+    HIPCHK(e, hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPCHK(e, hipEventElapsedTime(&ms, a, b));
+    e.evpool.push_back(a);
+    e.evpool.push_back(b);
+    *ms_per_copy = ms / repeats;
+    if (bytes_per_copy) *bytes_per_copy = 8.0 * (double)(e.N / 4 * 4);
+    return SKT_OK;
+}
+int skt_device_synchronize(int32_t device) {
+    if (hipSetDevice(device) != hipSuccess) return SKT_ERR_HIP;
+    return hipDeviceSynchronize() == hipSuccess ? SKT_OK : SKT_ERR_HIP;
+}
+
 int skt_mphys_region_set(skt_env* env, const int32_t mn[3], const int32_t mx[3]) {
     if (!env || !mn || !mx) return SKT_ERR_ARG;
     EnvState& e = env->s;

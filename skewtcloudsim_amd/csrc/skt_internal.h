@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string>
+#include <vector>
 #include "../../include/skt.h"
 #include "skt_device.h"
 
@@ -54,7 +55,20 @@ struct EnvState {
     Gammas gm;
     skt_env_params params;
     skt_step_stats stats;
+
+    // per-class kernel timing (skt_env_set_timing)
+    struct TimedLaunch {
+        int cls;
+        hipEvent_t a, b;
+    };
+    int timing = 0;
+    std::vector<TimedLaunch> pending;
+    std::vector<hipEvent_t> evpool;
+    double t_ms[8] = {0};
+    long long t_count[8] = {0};
 };
+
+enum { TC_DIFFUSE = 0, TC_PPM, TC_PCG_A, TC_PCG_B, TC_MICRO, TC_MAXVEL, TC_PRECIP, TC_OTHER };
 
 // ------------------------------------------------------------- launches ---
 Geom geom(const EnvState& e);
@@ -86,6 +100,7 @@ hipError_t launch_editor_reset(EnvState& e, int nfields);
 hipError_t launch_isen_temp_ground(EnvState& e);
 hipError_t launch_profiles_from_column(EnvState& e);
 hipError_t launch_random(EnvState& e, float* a, float mn, float mx, uint32_t seed);
+hipError_t launch_stream_copy(EnvState& e, const float* src, float* dst, long n);
 int blocks3d(const EnvState& e);
 
 }  // namespace skt

@@ -1198,6 +1198,16 @@ KL void k_random(float* a, long n, float mn, float mx, uint32_t seed) {
         a[i] += rnd * (mx - mn) + mn;
     }
 }
+// Plain streaming copy (float4 loads/stores): the measured bandwidth ceiling.
+KL void k_stream_copy(const float4* __restrict__ s, float4* __restrict__ d, long n4) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n4; i += (long)gridDim.x * 256) d[i] = s[i];
+}
+hipError_t launch_stream_copy(EnvState& e, const float* src, float* dst, long n) {
+    const long n4 = n / 4;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)std::min<long>((n4 + 255) / 256, 8192)), dim3(256), 0, e.stream,
+                       (const float4*)src, (float4*)dst, n4);
+    return hipGetLastError();
+}
 hipError_t launch_random(EnvState& e, float* a, float mn, float mx, uint32_t seed) {
     hipLaunchKernelGGL(k_random, dim3((unsigned)std::min<long>((e.N + 255) / 256, 65536)), dim3(256), 0, e.stream, a,
                        e.N, mn, mx, seed);

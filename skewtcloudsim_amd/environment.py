@@ -213,5 +213,21 @@ class EnvironmentGPU:
         self._chk(self.lib.skt_mphys_region_get(self.h, _ptr(out)), "region_get")
         return out
 
+    def set_timing(self, enable=True):
+        self._chk(self.lib.skt_env_set_timing(self.h, 1 if enable else 0), "set_timing")
+
+    def timing(self):
+        t = _lib.Timing()
+        self._chk(self.lib.skt_env_get_timing(self.h, ctypes.byref(t)), "timing")
+        return {c: (int(t.launches[k]), float(t.total_ms[k])) for k, c in enumerate(_lib.TIMING_CLASSES)}
+
+    def copy_probe(self, repeats=20):
+        ms, nb = ctypes.c_float(), ctypes.c_double()
+        self._chk(self.lib.skt_env_copy_probe(self.h, repeats, ctypes.byref(ms), ctypes.byref(nb)), "copy_probe")
+        return ms.value, nb.value
+
+    def device_synchronize(self):
+        self._chk(self.lib.skt_device_synchronize(self.cfg.device), "device_synchronize")
+
     def synchronize(self):
         self._chk(self.lib.skt_env_synchronize(self.h), "synchronize")

@@ -1,0 +1,250 @@
+"""Parity of the HIP step (libskt.so, through the C ABI) with the CPU oracle.
+
+Same seeded inputs on both sides; each phase is called alone on identical
+states, then whole steps are compared. Tolerances:
+  * masks and microphysics phase masks: bit-exact;
+  * one phase call: relative L2 <= 2e-6 and max-abs <= 1e-5 x field scale
+    (libm/ocml transcendental ulps and FP64 reduction order only);
+  * whole steps: relative L2 per field <= 1e-5 after 1 step, <= 1e-4 after 5.
+Only sky cells are compared: the step leaves ground cells untouched, while
+the reference leaves NaN there (0/0 of the density renormalisation).
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures", "smallPressure")
+SKY = ("QV", "QW", "QC", "QR", "QS", "QI", "POTTEMP", "VELX", "VELY", "VELZ", "PRESSURE", "DENSITY")
+GROUND = ("QRS", "QGR", "QGS", "QGI", "GROUND_T", "GROUND_TIME", "CLOUD_COVER")
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def sky_mask(nx, ny, nz, gh=None):
+    y = (np.arange(nx * ny * nz) // nx) % ny
+    if gh is None:
+        return y >= 1
+    x = np.arange(nx * ny * nz) % nx
+    z = np.arange(nx * ny * nz) // (nx * ny)
+    return y > gh[x + nx * z]
+
+
+def make_pair(nx, ny, nz, dx=100.0, seed=5, n_bubbles=2, **cfg):
+    import skewtcloudsim_amd as skt
+    snd = skt.make_sounding(nx, ny, nz, dx, seed=seed, n_bubbles=n_bubbles)
+    env = skt.EnvironmentGPU(nx, ny, nz, dx, **cfg)
+    ocfg = {}
+    if "pcg_mode" in cfg:
+        ocfg["pcg_mode"] = cfg["pcg_mode"]
+    if "pcg_tol" in cfg:
+        ocfg["pcg_tol"] = cfg["pcg_tol"]
+    if "pcg_max_iter" in cfg:
+        ocfg["pcg_max_iter"] = cfg["pcg_max_iter"]
+    if "maxabs_mode" in cfg:
+        ocfg["maxabs_mode"] = cfg["maxabs_mode"]
+    orc = ob.Oracle(nx, ny, nz, dx, **ocfg)
+    env.init(*[snd[k] for k in KEYS])
+    orc.init(*[snd[k] for k in KEYS])
+    return env, orc
+
+
+def copy_state(env, orc, fields):
+    for f in fields:
+        env.set(f, orc.get(f))
+
+
+def rich_state(env, orc, seed=1):
+    """Clouds, rain, snow, ice, winds and a cold upper layer on both sides."""
+    rng = np.random.default_rng(seed)
+    n = orc.N
+    for f, hi in (("QW", 2.5e-3), ("QC", 1.5e-3), ("QR", 2e-3), ("QS", 1e-3), ("QI", 1e-3)):
+        v = rng.uniform(0, hi, n).astype(np.float32)
+        v[rng.uniform(size=n) < 0.3] = 0.0
+        orc.set(f, v)
+    orc.set("QV", orc.get("QV") * rng.uniform(0.85, 1.15, n).astype(np.float32))
+    for f, s in (("VELX", 3.0), ("VELY", 1.5), ("VELZ", 3.0)):
+        orc.set(f, (orc.get(f) + rng.normal(0, s, n)).astype(np.float32))
+    th = orc.get("POTTEMP").reshape(orc.nz, orc.ny, orc.nx)
+    th[:, orc.ny // 2:, :] -= 30.0  # sub-freezing upper half
+    orc.set("POTTEMP", th.ravel())
+    orc.init_density()
+    g = orc.G
+    orc.set("QGR", rng.uniform(0, 1e-3, g).astype(np.float32))
+    orc.set("QGS", rng.uniform(0, 5e-4, g).astype(np.float32))
+    copy_state(env, orc, ob.FIELDS)
+
+
+def assert_fields(env, orc, fields, mask, l2=2e-6, maxabs=1e-5, tag=""):
+    for f in fields:
+        a, b = env.get(f), orc.get(f)
+        if a.size == mask.size:
+            a, b = a[mask], b[mask]
+        assert np.isfinite(a).all(), (tag, f, "non-finite")
+        scale = max(np.abs(b).max(), 1e-30)
+        r = rel_l2(a, b)
+        m = np.abs(a.astype(np.float64) - b).max() / scale
+        assert r <= l2 and m <= maxabs, (tag, f, r, m)
+
+
+NX, NY, NZ = 20, 24, 16
+
+
+@pytest.fixture(scope="module")
+def pair():
+    env, orc = make_pair(NX, NY, NZ, dx=400.0)
+    yield env, orc
+    env.close()
+    orc.close()
+
+
+def test_init_matches(pair):
+    env, orc = pair
+    m = sky_mask(NX, NY, NZ)
+    assert_fields(env, orc, SKY + GROUND, m, l2=1e-7, maxabs=1e-6, tag="init")
+    for f in ("ISEN_TEMP", "ISEN_VAPOR", "DEFAULT_VELX", "DEFAULT_VELZ", "DEFAULT_PRESSURE"):
+        assert np.array_equal(env.get(f), orc.get(f)), f
+
+
+def test_masks_bit_exact():
+    import skewtcloudsim_amd as skt
+    nx, ny, nz = 13, 9, 7
+    env, orc = make_pair(nx, ny, nz)
+    gh = np.random.default_rng(3).integers(0, 4, nx * nz).astype(np.int32)
+    env.set_ground_height(gh)
+    orc.set_ground_height(gh)
+    assert np.array_equal(env.masks(), orc.masks())
+    env.close()
+
+
+@pytest.mark.parametrize("field,type_", [("POTTEMP", 0), ("QV", 1), ("VELX", 2), ("VELY", 3), ("VELZ", 4),
+                                         ("QR", 5)])
+def test_diffusion_phase(pair, field, type_):
+    env, orc = pair
+    rich_state(env, orc, seed=type_ + 10)
+    env.diffuseGPU(field, type_, 0.5)
+    orc.diffuse(ob.FIELD[field], type_, 0.5)
+    assert_fields(env, orc, (field,), sky_mask(NX, NY, NZ), tag="diffuse")
+    # a large time step makes every neighbour count
+    env.diffuseGPU(field, type_, 1e6)
+    orc.diffuse(ob.FIELD[field], type_, 1e6)
+    assert_fields(env, orc, (field,), sky_mask(NX, NY, NZ), l2=1e-5, maxabs=1e-4, tag="diffuse-k")
+
+
+@pytest.mark.parametrize("field,prof,bounds", [
+    ("POTTEMP", "ISEN_TEMP", (2, 2, 2)), ("VELX", "DEFAULT_VELX", (2, 2, 1)), ("VELY", None, (1, 1, 1)),
+    ("VELZ", "DEFAULT_VELZ", (2, 2, 1)), ("QV", "ISEN_VAPOR", (2, 2, 2)), ("QR", None, (1, 1, 1))])
+def test_advection_phase(pair, field, prof, bounds):
+    env, orc = pair
+    rich_state(env, orc, seed=3)
+    env.advectPPMWGPU(field, prof, bounds, 40.0)  # Courant ~0.3 at 400 m
+    orc.advect_ppm(ob.FIELD[field], -1 if prof is None else ob.FIELD[prof], *bounds, 40.0)
+    assert env.stats().substeps[0] == orc.stats()["substeps"][0]
+    assert_fields(env, orc, (field, "VELX", "VELY", "VELZ"), sky_mask(NX, NY, NZ), l2=2e-6, maxabs=2e-5,
+                  tag="advect")
+
+
+@pytest.mark.parametrize("ftype", [0, 1, 2])
+def test_precip_phase(pair, ftype):
+    env, orc = pair
+    rich_state(env, orc, seed=4)
+    f = ("QR", "QS", "QI")[ftype]
+    env.advectPrecip(f, ftype, 200.0)
+    orc.advect_precip(ob.FIELD[f], ftype, 200.0)
+    assert_fields(env, orc, (f,), sky_mask(NX, NY, NZ), tag="precip")
+
+
+def test_buoyancy_phase(pair):
+    env, orc = pair
+    rich_state(env, orc, seed=5)
+    env.calculateBuoyancy(0.5)
+    orc.buoyancy(0.5)
+    assert_fields(env, orc, ("VELY",), sky_mask(NX, NY, NZ), tag="buoyancy")
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_pressure_projection(mode):
+    env, orc = make_pair(NX, NY, NZ, dx=400.0, pcg_mode=mode, pcg_tol=1e-5 if mode == 0 else 1e-6,
+                         pcg_max_iter=60)
+    rich_state(env, orc, seed=6)
+    env.initDensity()
+    orc.init_density()
+    env.pressureProject(0.5)
+    orc.pressure_project(0.5)
+    so, sg = orc.stats(), env.stats()
+    assert sg.pcg_iterations == so["pcg_iterations"]
+    m = sky_mask(NX, NY, NZ)
+    assert_fields(env, orc, ("PRESSURE_PROJ",), m, l2=1e-4, maxabs=1e-3, tag="pcg")
+    assert_fields(env, orc, ("VELX", "VELY", "VELZ", "PRESSURE", "DENSITY"), m, l2=1e-5, maxabs=1e-4, tag="proj")
+    env.close()
+
+
+def test_microphysics_phase_and_phase_mask(pair):
+    env, orc = pair
+    rich_state(env, orc, seed=7)
+    env.microPhysicsSkyGPU(0.5, 1.0)
+    orc.micro_sky(0.5, 1.0)
+    m = sky_mask(NX, NY, NZ)
+    assert_fields(env, orc, ("QV", "QW", "QC", "QR", "QS", "QI", "POTTEMP"), m, l2=1e-5, maxabs=1e-4, tag="micro")
+    pg, po = env.phase_mask(), orc.phase_mask()
+    assert np.array_equal(pg, po)
+    assert (po & 2).any() and (~po & 2)[m].any()  # both branches exercised
+
+
+def test_ground_phases(pair):
+    env, orc = pair
+    rich_state(env, orc, seed=8)
+    irr = env.irridianceGPU()
+    assert irr == orc.irradiance()
+    env.advectGroundWater(0.5, 1.0)
+    orc.ground_water(0.5, 1.0)
+    env.groundCoverageFactor()
+    orc.cloud_cover()
+    env.updateGroundTemps(0.5, 1.0, irr)
+    orc.ground_temps(0.5, 1.0, irr)
+    env.microPhysicsGroundGPU(0.5, 1.0, irr)
+    orc.micro_ground(0.5, 1.0, irr)
+    env.setTempsAtGround(0.5, 1.0)
+    orc.set_temps_at_ground(0.5, 1.0)
+    assert_fields(env, orc, GROUND + ("QV", "QR", "QS", "QI", "POTTEMP"), sky_mask(NX, NY, NZ), l2=1e-6,
+                  maxabs=1e-5, tag="ground")
+
+
+@pytest.mark.parametrize("shape,dx", [((24, 20, 16), 100.0), ((17, 13, 11), 400.0)])
+def test_full_steps(shape, dx):
+    nx, ny, nz = shape
+    env, orc = make_pair(nx, ny, nz, dx=dx, seed=9, n_bubbles=2)
+    m = sky_mask(nx, ny, nz)
+    for s in range(5):
+        env.updateGPU(0.5, 1.0)
+        orc.step(0.5, 1.0)
+        tol = 1e-5 if s == 0 else 1e-4
+        assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+        assert_fields(env, orc, SKY + GROUND, m, l2=tol, maxabs=50 * tol, tag=f"step{s}")
+    assert abs(env.params().time - orc.params()[0]) < 1e-3
+    env.close()
+
+
+def test_region_diagnostic(pair):
+    env, orc = pair
+    rich_state(env, orc, seed=12)
+    mn, mx = [2, 3, 1], [15, 20, 12]
+    env.region_set(mn, mx)
+    import ctypes
+    a, b = np.asarray(mn, np.int32), np.asarray(mx, np.int32)
+    orc.L.sko_region(orc.h, 1, a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p))
+    env.microPhysicsSkyGPU(0.5, 1.0)
+    orc.micro_sky(0.5, 1.0)
+    g = env.region_get()
+    o = np.zeros(33)
+    orc.L.sko_region_get(orc.h, o.ctypes.data_as(ctypes.c_void_p))
+    env.region_clear()
+    orc.L.sko_region(orc.h, 0, a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p))
+    np.testing.assert_allclose(g, o, rtol=1e-4, atol=1e-12)
+    assert np.count_nonzero(o) > 10
