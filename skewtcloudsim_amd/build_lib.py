@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["skt_kernels.hip", "skt_api.cpp", "skt_sounding.cpp"]
+SOURCES = ["skt_kernels.hip", "skt_misc.hip", "skt_api.cpp", "skt_sounding.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # two roundings for a*b+c, as the CPU restatement is compiled

@@ -158,26 +158,38 @@ int precip(EnvState& e, int fld, int type, float dt) {
     return SKT_OK;
 }
 
-// calculatePressureProject, Game/environment.cu:898-987.
+// calculatePressureProject, Game/environment.cu:898-987. The scalar state
+// lives on the device (two slots); the host looks at it every 25 iterations
+// to stop launching once the solve has converged.
 int solvePressure(EnvState& e, float dt) {
     TL(e, TC_OTHER, launch_divergence(e, dt));
     TL(e, TC_OTHER, launch_pcg_start(e));
     const int maxIt = e.cfg.pcg_max_iter;
     const int chunk = 25;
-    for (int it = 0; it < maxIt; it++) {
+    int it = 0;
+    for (; it < maxIt; it++) {
         TL(e, TC_PCG_A, launch_pcg_A(e, it));
         TL(e, TC_PCG_B, launch_pcg_B(e, it));
         if ((it + 1) % chunk == 0 && it + 1 < maxIt) {
             int rc = readScalars(e);
             if (rc) return rc;
-            if (e.sc_host->done) break;
+            if (e.sc_host->st[1].done) {
+                it++;
+                break;
+            }
         }
+    }
+    int slot = 1;
+    if (it > 0) {
+        TL(e, TC_OTHER, launch_pcg_final(e, it));
+        slot = 0;
     }
     int rc = readScalars(e);
     if (rc) return rc;
-    e.stats.pcg_iterations = e.sc_host->iters;
-    e.stats.pcg_residual = e.sc_host->maxr;
-    e.stats.pcg_residual0 = e.sc_host->maxr0;
+    const PcgState& st = e.sc_host->st[slot];
+    e.stats.pcg_iterations = st.iters;
+    e.stats.pcg_residual = st.maxr;
+    e.stats.pcg_residual0 = st.maxr0;
     return SKT_OK;
 }
 
@@ -274,6 +286,8 @@ void freeAll(EnvState& e) {
     for (int k = 0; k < 4; k++) fr(e.work[k]);
     for (int k = 0; k < 6; k++) fr(e.pcg[k]);
     fr(e.gh);
+    fr(e.ghm1);
+    fr(e.ghm2);
     fr(e.A);
     fr(e.zeros);
     fr(e.ones);
@@ -350,12 +364,13 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
         if (dalloc(e, &e.work[k], sizeof(float) * std::max(e.N, 2 * e.G))) return bail(SKT_ERR_NOMEM);
     for (int k = 0; k < 6; k++)
         if (dalloc(e, &e.pcg[k], sizeof(float) * e.N)) return bail(SKT_ERR_NOMEM);
-    if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.A, sizeof(float4) * e.N) ||
+    if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.ghm1, sizeof(int) * e.G) ||
+        dalloc(e, &e.ghm2, sizeof(int) * e.G) || dalloc(e, &e.A, sizeof(float4) * e.N) ||
         dalloc(e, &e.zeros, sizeof(float) * e.Y) || dalloc(e, &e.ones, sizeof(float) * e.Y) ||
         dalloc(e, &e.phase, (size_t)e.N))
         return bail(SKT_ERR_NOMEM);
     e.maxBlocks = blocks3d(e);
-    if (dalloc(e, &e.partials, sizeof(double) * 2 * (size_t)e.maxBlocks) ||
+    if (dalloc(e, &e.partials, sizeof(double) * PART_COUNT * (size_t)e.maxBlocks) ||
         dalloc(e, &e.region, sizeof(double) * 33 * (size_t)e.maxBlocks + 64 * sizeof(float)) ||
         dalloc(e, &e.sc, sizeof(Scalars)))
         return bail(SKT_ERR_NOMEM);
@@ -364,6 +379,8 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     for (int k = 0; k < SKT_FIELD_COUNT; k++)
         if (hipMemsetAsync(e.f[k], 0, sizeof(float) * fieldLen(e, k), e.stream) != hipSuccess) return bail(SKT_ERR_HIP);
     if (hipMemsetAsync(e.gh, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
+        hipMemsetAsync(e.ghm1, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
+        hipMemsetAsync(e.ghm2, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
         hipMemsetAsync(e.sc, 0, sizeof(Scalars), e.stream) != hipSuccess ||
         hipMemsetAsync(e.phase, 0, (size_t)e.N, e.stream) != hipSuccess ||
         hipMemsetAsync(e.A, 0, sizeof(float4) * e.N, e.stream) != hipSuccess)
@@ -422,7 +439,26 @@ int skt_env_set_ground_height(skt_env* env, const int32_t* gh) {
     EnvState& e = env->s;
     for (long g = 0; g < e.G; g++)
         if (gh[g] < 0 || gh[g] >= e.Y) return setErr(e, SKT_ERR_ARG, "ground height outside [0, ny)");
+    // column maxima over the 3x3 and 5x5 neighbourhoods (clipped at the
+    // domain edge) select the stencil kernels' boundary-free fast paths
+    std::vector<int32_t> m1((size_t)e.G), m2((size_t)e.G);
+    for (int z = 0; z < e.Z; z++)
+        for (int x = 0; x < e.X; x++) {
+            int a = 0, b = 0;
+            for (int dz = -2; dz <= 2; dz++)
+                for (int dx = -2; dx <= 2; dx++) {
+                    const int xx = x + dx, zz = z + dz;
+                    if (xx < 0 || xx >= e.X || zz < 0 || zz >= e.Z) continue;
+                    const int h = gh[xx + (long)e.X * zz];
+                    b = std::max(b, h);
+                    if (dx >= -1 && dx <= 1 && dz >= -1 && dz <= 1) a = std::max(a, h);
+                }
+            m1[x + (size_t)e.X * z] = a;
+            m2[x + (size_t)e.X * z] = b;
+        }
     HIPCHK(e, hipMemcpyAsync(e.gh, gh, sizeof(int) * e.G, hipMemcpyHostToDevice, e.stream));
+    HIPCHK(e, hipMemcpyAsync(e.ghm1, m1.data(), sizeof(int) * e.G, hipMemcpyHostToDevice, e.stream));
+    HIPCHK(e, hipMemcpyAsync(e.ghm2, m2.data(), sizeof(int) * e.G, hipMemcpyHostToDevice, e.stream));
     HIPCHK(e, hipStreamSynchronize(e.stream));
     e.params.ground_changed = 1;
     return SKT_OK;

@@ -26,10 +26,15 @@ struct Bounds {
 
 // Grid geometry. Index = x + X*(y + Y*z); y is vertical; gh[x + X*z] is the
 // highest ground level of each column (cells with y <= gh are ground).
+// ghm1 / ghm2 hold the largest gh over the 3x3 / 5x5 neighbourhood of each
+// column: a cell at y >= ghm + 2 (resp. + 3) has only sky within 1 (2) cells,
+// which selects the boundary-free fast path of the stencil kernels.
 struct Geom {
     int X, Y, Z;
     long N;
     const int* __restrict__ gh;
+    const int* __restrict__ ghm1;
+    const int* __restrict__ ghm2;
 };
 
 struct Gammas {

@@ -9,20 +9,30 @@
 
 namespace skt {
 
-// Device scalars shared by the reduction kernels (one 256-byte block).
-struct Scalars {
-    double sigma0, sigma1;       // PCG sigma values (stored rounded to float, kept as double slots)
+// State of the pressure solve's scalar recurrence (alpha, beta, sigma and
+// the stop test). Two slots: a kernel reads one and its block 0 writes the
+// updated copy into the other, so every block of one launch sees the same input.
+struct PcgState {
+    double sigma0, sigma1;  // sigma values (stored rounded to float, kept in double slots)
     float alpha, beta;
-    float maxr, maxr0;           // stop-test values
-    int done;                    // pressure solve finished
-    int iters;                   // PCG iterations performed
-    float maxvel;                // advection max face velocity
-    int nsub;                    // advection substeps
-    float dtsub;                 // advection substep length
-    unsigned int counter[4];     // last-block counters of the reductions
-    float tol;                   // stop tolerance in effect
-    int pad[33];
+    float maxr, maxr0;      // stop-test values
+    float tol;              // stop tolerance in effect
+    int done;               // solve finished
+    int iters;              // iterations performed
+    int pad[5];
 };
+
+// Device scalars shared by the reduction kernels.
+struct Scalars {
+    PcgState st[2];
+    float maxvel;  // advection max face velocity
+    int nsub;      // advection substeps
+    float dtsub;   // advection substep length
+    int pad[29];
+};
+
+// Regions of the per-block reduction partials (each maxBlocks doubles).
+enum { PART_DIV = 0, PART_START, PART_A, PART_BMAX, PART_BSUM, PART_COUNT };
 
 struct EnvState {
     skt_config cfg;
@@ -37,12 +47,14 @@ struct EnvState {
 
     float* f[SKT_FIELD_COUNT] = {nullptr};  // field arrays (sizes N, G or Y)
     int* gh = nullptr;                      // ground height [G]
+    int* ghm1 = nullptr;                    // max ground height over the 3x3 column neighbourhood [G]
+    int* ghm2 = nullptr;                    // max ground height over the 5x5 column neighbourhood [G]
     float* work[4] = {nullptr};             // N-sized work buffers
     float* pcg[6] = {nullptr};              // r, z, s_a, s_b, As, pre
     float4* A = nullptr;                    // upper coefficients (x,y,z) and diagonal (w)
     float* zeros = nullptr;                 // [Y] zero profile
     float* ones = nullptr;                  // [Y] unit profile
-    double* partials = nullptr;             // per-block reduction partials (2 x maxBlocks)
+    double* partials = nullptr;             // per-block reduction partials (PART_COUNT x maxBlocks)
     int maxBlocks = 0;
     Scalars* sc = nullptr;                  // device scalars
     Scalars* sc_host = nullptr;             // pinned mirror for readbacks
@@ -72,6 +84,15 @@ enum { TC_DIFFUSE = 0, TC_PPM, TC_PCG_A, TC_PCG_B, TC_MICRO, TC_MAXVEL, TC_PRECI
 
 // ------------------------------------------------------------- launches ---
 Geom geom(const EnvState& e);
+// z-marching launch shape: 64x4-cell tiles, each block walks zc planes; about
+// 2048 blocks for the large domains (one reduction partial per block). The
+// grid is one-dimensional; kernels map the block index to a tile.
+struct ZGrid {
+    dim3 grid;
+    int zc;
+    unsigned nblocks;
+};
+ZGrid zgrid(const EnvState& e);
 hipError_t launch_fill(EnvState& e, float* a, float v, long n);
 hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst);
 hipError_t launch_ground_water_snapshot(EnvState& e, float dt, float speed);
@@ -87,10 +108,11 @@ hipError_t launch_init_density(EnvState& e, int copy_old);
 hipError_t launch_new_pressure(EnvState& e);
 hipError_t launch_reset_vel(EnvState& e);
 hipError_t launch_build_A(EnvState& e);
-hipError_t launch_divergence(EnvState& e, float dt);  // r = div, max -> maxr0, done if 0
-hipError_t launch_pcg_start(EnvState& e);              // z = pre r, sigma0 = z.r
-hipError_t launch_pcg_A(EnvState& e, int it);          // s update, As, sigma1, alpha
-hipError_t launch_pcg_B(EnvState& e, int it);          // p, r update, max r, z, beta
+hipError_t launch_divergence(EnvState& e, float dt);  // r = div, partial max |r|
+hipError_t launch_pcg_start(EnvState& e);              // stop tolerance; z = pre r, partial z.r
+hipError_t launch_pcg_A(EnvState& e, int it);          // stop test / beta; s update, As, partial s.As
+hipError_t launch_pcg_B(EnvState& e, int it);          // alpha; p', r, z updates, partial max |r| and z.r
+hipError_t launch_pcg_final(EnvState& e, int it);      // stop test after the last iteration
 hipError_t launch_update_pressure_density(EnvState& e);
 hipError_t launch_apply_projection(EnvState& e, int new_pressure);
 hipError_t launch_microphysics(EnvState& e, float dt, float speed);

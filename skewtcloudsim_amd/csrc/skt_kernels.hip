@@ -1,98 +1,58 @@
-// HIP kernels of the skewT cloud step for gfx950 (MI355X).
+// HIP kernels of the skewT cloud step for gfx950 (MI355X): diffusion, PPM
+// advection, the pressure solve and the reductions that steer the step.
 //
 // Layout: every voxel field is one float array in HBM, index x + X*(y + Y*z)
 // (x fastest, y vertical), exactly the reference's getIdx (Game/utils.cuh:8-11)
-// so device pointers are interchangeable with the reference's. 3D kernels use
-// 256-thread blocks of 64 (x) x 4 (y) cells, one z plane per block row: a
-// wavefront covers 64 consecutive floats of a row (256-byte coalesced lines).
-// Reductions are two-level and deterministic: a wave-shuffle + LDS tree per
-// block into a double partial, then the last block to finish sums the
-// partials in a fixed order and performs the scalar update (alpha, beta,
-// stop test, substep count) on the device.
+// so device pointers are interchangeable with the reference's.
+//
+// Launch shape: 256-thread blocks own a 64 (x) x 4 (y) tile; a wavefront
+// covers 64 consecutive floats of a row (256-byte coalesced lines) and the
+// block walks a run of zc planes (about 2048 blocks on the large domains,
+// which fills the 256 CUs and leaves one reduction partial per block).
+// The kernels are bound by memory latency unless each wave keeps many loads
+// in flight, so every z-marching kernel works on groups of ZB planes: it
+// first issues all loads of the group (stencil neighbours at clamped, always
+// valid addresses, plus the ground heights that decide the boundary rule),
+// then applies the boundary conditions with selects and computes. Column
+// values along z stay in registers between planes.
+//
+// Reductions are deterministic: per-block partials (FP64), summed in a fixed
+// order. The scalar steps of the pressure solve (alpha, beta, the stop test)
+// run in the prologue of the kernel that consumes them: every block reduces
+// the previous kernel's partials to the same value and block 0 stores the
+// updated state in the other of two slots.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "skt_internal.h"
-#include "skt_micro.h"
+#include "skt_reduce.h"
 
 namespace skt {
 
-#define KL __global__ __launch_bounds__(256)
-
-// --------------------------------------------------------- reductions ---
-__device__ __forceinline__ int tid256() { return threadIdx.x + 64 * threadIdx.y; }
-
-template <class T>
-__device__ __forceinline__ T waveSum(T v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-template <class T>
-__device__ __forceinline__ T waveMax(T v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-// Block sum of 256 threads; result valid in thread 0.
-__device__ __forceinline__ double blockSum(double v, double* sh) {
-    v = waveSum(v);
-    const int t = tid256();
-    if ((t & 63) == 0) sh[t >> 6] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (t == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-    __syncthreads();
-    return r;
-}
-__device__ __forceinline__ double blockMax(double v, double* sh) {
-    v = waveMax(v);
-    const int t = tid256();
-    if ((t & 63) == 0) sh[t >> 6] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (t == 0) r = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
-    __syncthreads();
-    return r;
-}
-// Publishes this block's partial and reports whether it is the last block.
-__device__ __forceinline__ bool publishLast(double* part, int bid, double v, unsigned* counter, unsigned nblocks) {
-    __shared__ unsigned s_last;
-    if (tid256() == 0) {
-        part[bid] = v;
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        s_last = (prev == nblocks - 1) ? 1u : 0u;
-    }
-    __syncthreads();
-    return s_last != 0u;
-}
-__device__ __forceinline__ double ldPart(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Fixed-order sum / max over all partials, by the whole (last) block.
-__device__ __forceinline__ double finalSum(const double* part, unsigned n, double* sh) {
-    double v = 0.0;
-    for (unsigned i = tid256(); i < n; i += 256) v += ldPart(part + i);
-    return blockSum(v, sh);
-}
-__device__ __forceinline__ double finalMax(const double* part, unsigned n, double* sh) {
-    double v = 0.0;
-    for (unsigned i = tid256(); i < n; i += 256) v = fmax(v, ldPart(part + i));
-    return blockMax(v, sh);
-}
-
-#define CELL_SETUP                                   \
-    const int x = blockIdx.x * 64 + threadIdx.x;     \
-    const int y = blockIdx.y * 4 + threadIdx.y;      \
-    const int z = blockIdx.z;                        \
-    const bool inb = (x < G.X) && (y < G.Y);         \
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)
-
-static dim3 grid3(const EnvState& e) { return dim3((e.X + 63) / 64, (e.Y + 3) / 4, e.Z); }
-static const dim3 B3(64, 4, 1);
 int blocks3d(const EnvState& e) {
     dim3 g = grid3(e);
     return (int)(g.x * g.y * g.z);
 }
-static dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
-Geom geom(const EnvState& e) { return Geom{e.X, e.Y, e.Z, e.N, e.gh}; }
+Geom geom(const EnvState& e) { return Geom{e.X, e.Y, e.Z, e.N, e.gh, e.ghm1, e.ghm2}; }
+ZGrid zgrid(const EnvState& e) {
+    const int gx = (e.X + 63) / 64, gy = (e.Y + 3) / 4;
+    const long cols = (long)gx * gy;
+    long nzc = std::max(1L, std::min((long)e.Z, (2048 + cols - 1) / cols));
+    const int zc = (int)((e.Z + nzc - 1) / nzc);
+    nzc = (e.Z + zc - 1) / zc;
+    const unsigned nb = (unsigned)(cols * nzc);
+    return ZGrid{dim3(nb), zc, nb};
+}
+
+// Boundary value of a stencil read (the extv rule): ground condition if the
+// clamped position is ground, the sides condition if the position lies
+// outside the domain (the top face too), else the raw value.
+__device__ __forceinline__ float nbv(float raw, bool out, bool gnd, float pv, Bounds b) {
+    return gnd ? bcv(b.ground, raw, pv) : (out ? bcv(b.sides, raw, pv) : raw);
+}
+// The same rule for the unit field (profile value 1).
+__device__ __forceinline__ float unitv(bool out, bool gnd, Bounds b) {
+    return gnd ? (b.ground == BC_DIRICHLET ? 0.0f : 1.0f) : (out ? (b.sides == BC_DIRICHLET ? 0.0f : 1.0f) : 1.0f);
+}
 
 // -------------------------------------------------------------- utility ---
 KL void k_fill(float* a, float v, long n) {
@@ -113,263 +73,163 @@ hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst) {
     hipLaunchKernelGGL(k_copy_sky, grid3(e), B3, 0, e.stream, geom(e), src, dst);
     return hipGetLastError();
 }
-
-// ---------------------------------------------------------- ground phase ---
-// advectGroundWaterGPU, Game/kernelSky.cu:120-253, reading a pre-step snapshot
-// of both stores (the reference's in-place tile update leaves its interior
-// halo unfilled; every column reading its neighbours' old values is intended).
-KL void k_ground_water(Geom G, const float* __restrict__ rs0, const float* __restrict__ gr0, float* __restrict__ rs,
-                       float* __restrict__ gr, float dt, float speed, float dx) {
-    const long g = blockIdx.x * 256L + threadIdx.x;
-    if (g >= (long)G.X * G.Z) return;
-    const int x = (int)(g % G.X), zc = (int)(g / G.X);
-    const long iL = x > 0 ? g - 1 : g, iR = x < G.X - 1 ? g + 1 : g;
-    const long iB = zc > 0 ? g - G.X : g, iF = zc < G.Z - 1 ? g + G.X : g;
-    const float n = 1 / 0.03f;
-    const float kk = 5e-5f;
-    const float diffR = 1 / (dx * dx) * 1.75e-3f;
-    const float diffW = 1 / (dx * dx) * 1.8e-5f;
-    const float grC = gr0[g], rsC = rs0[g];
-    float fGr = grC, fRs = rsC;
-    const float hC = float(G.gh[g]);
-    const long nb[4] = {iL, iR, iB, iF};
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const long j = nb[k];
-        const float hFrom = float(G.gh[j]), grFrom = gr0[j], rsFrom = rs0[j];
-        const float slope = hFrom - hC / 1.0f;
-        const bool fromPrev = slope > 0.0f;
-        const float sRain = fromPrev ? grFrom : grC;
-        const float sWater = fromPrev ? rsFrom : rsC;
-        float changeRain = 0.0f;
-        if (sRain != 0.0f) {
-            const float sp = n * powf(sRain, 2.0f / 3.0f);
-            const float vel = sp * powf(slope, 0.5f);
-            changeRain = vel * sRain;
-        }
-        const float changeWater = kc::g * kk * sWater * slope;
-        const float lapR = (grFrom - grC) * diffR;
-        const float lapSR = (rsFrom - rsC) * diffW;
-        const bool fpR = lapR > 0.0f, fpSR = lapSR > 0.0f;
-        const float fgr = dt * fminf(fabsf(lapR) * speed, fpR ? grFrom : grC) * (lapR > 0.0f ? 1.0f : -1.0f);
-        const float fgw = dt * fminf(fabsf(lapSR) * speed, fpSR ? rsFrom : rsC) * (lapSR > 0.0f ? 1.0f : -1.0f);
-        const float sfr = dt * fminf(fabsf(changeRain) * speed, fromPrev ? grFrom : grC) * (changeRain > 0.0f ? 1.0f : -1.0f);
-        const float sfw = dt * fminf(fabsf(changeWater) * speed, fromPrev ? rsFrom : rsC) * (changeWater > 0.0f ? 1.0f : -1.0f);
-        fGr += fgr + sfr;
-        fRs += fgw + sfw;
-    }
-    gr[g] = fGr;
-    rs[g] = fRs;
+// Plain streaming copy (float4 loads/stores): the measured bandwidth ceiling.
+KL void k_stream_copy(const float4* __restrict__ s, float4* __restrict__ d, long n4) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n4; i += (long)gridDim.x * 256) d[i] = s[i];
 }
-hipError_t launch_ground_water_snapshot(EnvState& e, float dt, float speed) {
-    float* rs0 = e.work[0];
-    float* gr0 = e.work[0] + e.G;
-    hipError_t err = hipMemcpyAsync(rs0, e.f[SKT_QRS], e.G * sizeof(float), hipMemcpyDeviceToDevice, e.stream);
-    if (err != hipSuccess) return err;
-    err = hipMemcpyAsync(gr0, e.f[SKT_QGR], e.G * sizeof(float), hipMemcpyDeviceToDevice, e.stream);
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_ground_water, grid1(e.G), dim3(256), 0, e.stream, geom(e), rs0, gr0, e.f[SKT_QRS],
-                       e.f[SKT_QGR], dt, speed, e.dx);
-    return hipGetLastError();
-}
-
-struct GroundArgs {
-    float *Qv, *Qr, *Qs, *Qi, *Qc, *Qw, *th, *u, *p, *rho;
-    float *Qrs, *Qgr, *Qgs, *Qgi, *gP, *gt, *gT, *cover;
-};
-
-// One thread per ground column: the column-local part of the ground phase of
-// updateGPU (Game/environment.cu:427-455,469) in the reference's order.
-KL void k_ground_columns(Geom G, GroundArgs a, int stages, float dt, float speed, float irr, float dx, Gammas gm) {
-    const long g = blockIdx.x * 256L + threadIdx.x;
-    if (g >= (long)G.X * G.Z) return;
-    const int x = (int)(g % G.X), zc = (int)(g / G.X);
-    const float kap = kc::Rsd / kc::Cpd;
-    const int gh = G.gh[g];
-    if (stages & 1) {  // calculateCloudCoverGPU, Game/kernelSky.cu:1034-1044
-        float tot = 0.0f;
-        for (int y = gh + 1; y < G.Y; y++) {
-            const long i = gidx(G, x, y, zc);
-            tot += (a.Qc[i] + a.Qw[i]) * dx;
-        }
-        a.cover[g] = fminf(tot / 1.2f, 1.0f);
-    }
-    if (stages & 2) {  // calculateGroundTempGPU, Game/kernelSky.cu:1046-1059 (T^4 widened to double)
-        const float gtv = a.gT[g];
-        const double T4 = gtv * gtv * gtv * gtv;
-        a.gT[g] += (dt * speed) * ((1 - a.cover[g]) * (((1 - 0.25f) * irr - kc::ge * kc::oo * T4) / (1.0f * 1500.0f * kc::Cpds)));
-    }
-    if ((stages & (4 | 8 | 16)) && gh + 1 < G.Y) {
-        const long i = gidx(G, x, gh + 1, zc);
-        if (stages & 4) {  // calculatePrecipHittingGroundMicroPhysicsGPU, Game/microPhysics.cu:1544-1636
-            const float Qgr = a.Qgr[g] > 1e-18f ? a.Qgr[g] : 0.0f;
-            const float Qgs = a.Qgs[g] > 1e-18f ? a.Qgs[g] : 0.0f;
-            const float Qgi = a.Qgi[g] > 1e-18f ? a.Qgi[g] : 0.0f;
-            const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
-            const float Qr = a.Qr[i] > 1e-18f ? a.Qr[i] : 0.0f;
-            const float Qs = a.Qs[i] > 1e-18f ? a.Qs[i] : 0.0f;
-            const float Qi = a.Qi[i] > 1e-18f ? a.Qi[i] : 0.0f;
-            float freeze = 0.0f;
-            const float tGC = a.gT[g] - 273.15f;
-            const float ps = a.p[i];
-            const float wind = a.u[i];
-            float3 fv = make_float3(0.0f, 0.0f, 0.0f);
-            const float Dair = a.rho[i];
-            const float tAK = a.th[i] * powf(ps / a.gP[g], kap);
-            if (Qgr > 0.0f || Qgs > 0.0f || Qgi > 0.0f) fv = d_fallVel(Qgr, Qgs, Qgi, Dair, 3, gm.R48, gm.S425, gm.S425);
-            const float GR = dt * fminf(Qr * speed, Qr);
-            const float GS = dt * fminf(Qs * speed, Qs);
-            const float GI = dt * fminf(Qi * speed, Qi);
-            float PGRFR = mp::PGRFR(tAK, tGC, ps, Qr, Qv, Qi, Qgr, Qgs, Dair, wind, fv);
-            if (tAK < 273.15f) {
-                PGRFR = dt * fminf(Qr, speed * PGRFR);
-                freeze += PGRFR;
-            }
-            a.Qr[i] += -GR;
-            a.Qs[i] += -GS;
-            a.Qi[i] += -GI;
-            a.Qgr[g] += GR - PGRFR;
-            a.Qgs[g] += GS;
-            a.Qgi[g] += GI + PGRFR;
-            const float Mair = 0.02896f, Mwater = 0.01802f;
-            const float XV = (Qv / Mwater) / ((Qv / Mwater) + (1 - Qv) / Mair);
-            const float Mth = XV * Mwater + (1 - XV) * Mair;
-            const float YV = XV * (Mwater / Mth);
-            const float yth = YV * 1.33f + (1 - YV) * 1.4f;
-            const float cpth = kc::R / (Mth * (yth - 1));
-            float sum = 0.0f;
-            sum += kc::Lf / cpth * freeze;
-            a.gT[g] += sum;
-        }
-        if (stages & 8) {  // calculateGroundMicroPhysicsGPU, Game/microPhysics.cu:1419-1541
-            const float Qrs = a.Qrs[g] > 1e-18f ? a.Qrs[g] : 0.0f;
-            const float Qgr = a.Qgr[g] > 1e-18f ? a.Qgr[g] : 0.0f;
-            const float Qgs = a.Qgs[g] > 1e-18f ? a.Qgs[g] : 0.0f;
-            const float Qgi = a.Qgi[g] > 1e-18f ? a.Qgi[g] : 0.0f;
-            const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
-            float condens = 0.0f, freeze = 0.0f, depos = 0.0f;
-            const float tGC = a.gT[g] - 273.15f;
-            const float ps = a.p[i];
-            const float wind = a.u[i];
-            const float cc = a.cover[g];
-            const float Dair = a.rho[i];
-            float tme = a.gt[g];
-            const float tAK = a.th[i] * powf(ps / a.gP[g], kap);
-            const float sI = d_slope(Dair, Qgi, 2);
-            float PGREVP = mp::PGREVP(tAK, tGC, ps, Qv, Qgr, irr, Dair, wind);
-            float PGSMLT = mp::PGSMLT(tAK, tGC, ps, Qgs, Qv, Qgr, irr, cc, Dair, wind, dt);
-            float PGGMLT = mp::PGGMLT(tGC, Qgi, Dair, sI);
-            float PGGFR = mp::PGGFR(tAK, tGC, ps, Qgr, Qv, Qgi, Dair, irr, cc, wind, dt);
-            float PGSFR = mp::PGSFR(tAK, tGC, ps, Qv, Qgr, Qgs, Dair, wind, dt);
-            float PGFLW = mp::PGFLW(Qgr);
-            float PGDEVP = mp::PGDEVP(&tme, Qgr, Qgs, Qgi, Qrs);
-            a.gt[g] = tme;
-            PGREVP = dt * fminf(Qgr, speed * PGREVP); condens -= PGREVP;
-            PGSMLT = dt * fminf(Qgs, speed * PGSMLT); freeze -= PGSMLT;
-            PGGMLT = dt * fminf(Qgi, speed * PGGMLT); freeze -= PGGMLT;
-            PGGFR = dt * fminf(Qgr, speed * PGGFR); freeze += PGGFR;
-            PGSFR = dt * fminf(Qgr, speed * PGSFR); freeze += PGSFR;
-            PGFLW = dt * fminf(Qgr, speed * PGFLW);
-            PGDEVP = dt * fminf(Qrs, speed * PGDEVP); condens -= PGDEVP;
-            a.Qrs[g] += PGFLW - PGDEVP;
-            a.Qv[i] += PGREVP + PGDEVP;
-            a.Qgr[g] += PGSMLT + PGGMLT - PGREVP - PGGFR - PGSFR - PGFLW;
-            a.Qgs[g] += PGSFR - PGSMLT;
-            a.Qgi[g] += PGGFR - PGGMLT;
-            const float cps = 1500.0f, rhos = 1500.0f, dl = 0.1f;
-            const float ms = rhos * dl;
-            float sum = 0.0f;
-            sum += d_Lwater(tGC) / (ms * cps) * condens;
-            sum += d_Lice(tGC) / (ms * cps) * depos;
-            sum += kc::Lf / (ms * cps) * freeze;
-            a.gT[g] += sum;
-        }
-        if ((stages & 16) && gh + 1 < G.Y - 1) {  // setTempsAtGroundGPU, Game/kernelSky.cu:255-268
-            const float T = d_potTemp(a.gT[g] - 273.15f, a.p[i], a.gP[g]) + 273.15f;
-            const float d2 = a.th[i] - T;
-            a.th[i] -= d2 * fminf(1.0f, dt * speed);
-        }
-    }
-}
-hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed, float irr) {
-    GroundArgs a{e.f[SKT_QV], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_QC], e.f[SKT_QW], e.f[SKT_POTTEMP],
-                 e.f[SKT_VELX], e.f[SKT_PRESSURE], e.f[SKT_DENSITY], e.f[SKT_QRS], e.f[SKT_QGR], e.f[SKT_QGS],
-                 e.f[SKT_QGI], e.f[SKT_GROUND_P], e.f[SKT_GROUND_TIME], e.f[SKT_GROUND_T], e.f[SKT_CLOUD_COVER]};
-    hipLaunchKernelGGL(k_ground_columns, grid1(e.G), dim3(256), 0, e.stream, geom(e), a, stages, dt, speed, irr, e.dx,
-                       e.gm);
+hipError_t launch_stream_copy(EnvState& e, const float* src, float* dst, long n) {
+    const long n4 = n / 4;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)std::min<long>((n4 + 255) / 256, 8192)), dim3(256), 0, e.stream,
+                       (const float4*)src, (float4*)dst, n4);
     return hipGetLastError();
 }
 
 // ------------------------------------------------------------- diffusion ---
 // One Jacobi sweep = the red+black pair of diffuseRedBlack
-// (Game/kernelSky.cu:43-115; both colours read 'input', quirk A3).
-KL void k_diffuse(Geom G, const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ prof, Bounds b,
-                  int type, float k, const float* __restrict__ gT, const float* __restrict__ gP,
+// (Game/kernelSky.cu:43-115; both colours read 'input', quirk A3):
+// out = (in + k (l + r + u + d + f + b)) / (1 + 6k) on sky cells.
+constexpr int ZB_DIFF = 4;
+template <int ZB>
+KL void k_diffuse(Geom G, int zc, const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ prof,
+                  Bounds b, int type, float k, const float* __restrict__ gT, const float* __restrict__ gP,
                   const float* __restrict__ p) {
-    CELL_SETUP;
+    ZM_SETUP;
     (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
-    const long i = gidx(G, x, y, z);
-    float l = extv(G, in, prof, b, x - 1, y, z), r = extv(G, in, prof, b, x + 1, y, z);
-    float d = extv(G, in, prof, b, x, y - 1, z), u = extv(G, in, prof, b, x, y + 1, z);
-    float f = extv(G, in, prof, b, x, y, z + 1), bb = extv(G, in, prof, b, x, y, z - 1);
-    if (type == 0) {
-        // potential temperature: a ground neighbour reads the potential
-        // temperature of the ground temperature at this cell's pressure (:95-100)
-        auto tg = [&](int gx, int gz) {
-            const long gi = gx + (long)G.X * gz;
-            return d_potTemp(gT[gi] - 273.15f, p[i], gP[gi]) + 273.15f;
-        };
-        if (y - 1 >= 0 && isGround(G, x, y - 1, z)) d = tg(x, z);
-        if (x - 1 >= 0 && isGround(G, x - 1, y, z)) l = tg(x - 1, z);
-        if (x + 1 < G.X && isGround(G, x + 1, y, z)) r = tg(x + 1, z);
-        if (z + 1 < G.Z && isGround(G, x, y, z + 1)) f = tg(x, z + 1);
-        if (z - 1 >= 0 && isGround(G, x, y, z - 1)) bb = tg(x, z - 1);
+    if (!inb || z0 >= z1) return;
+    const int X = G.X, Y = G.Y, Z = G.Z;
+    const long XY = (long)X * Y;
+    const int xl = max(x - 1, 0), xr = min(x + 1, X - 1), yd = max(y - 1, 0), yu = min(y + 1, Y - 1);
+    const int rc = x + X * y, rl = xl + X * y, rr = xr + X * y, rd = x + X * yd, ru = x + X * yu;
+    const float pv = prof ? prof[y] : 0.0f, pvd = prof ? prof[yd] : 0.0f, pvu = prof ? prof[yu] : 0.0f;
+    const float den = 1 + 6 * k;
+    const int zbk = max(z0 - 1, 0);
+    float vb = in[XY * zbk + rc], vc = in[XY * z0 + rc];
+    int ghb = G.gh[x + X * zbk], ghc = G.gh[x + X * z0];
+    for (int zg = z0; zg < z1; zg += ZB) {
+        float L[ZB], R[ZB], D[ZB], U[ZB], F[ZB];
+        int GL[ZB], GR[ZB], GF[ZB];
+#pragma unroll
+        for (int j = 0; j < ZB; j++) {
+            const int z = zg + j;
+            if (z < z1) {
+                const long pz = XY * z;
+                const int zf = min(z + 1, Z - 1);
+                L[j] = in[pz + rl];
+                R[j] = in[pz + rr];
+                D[j] = in[pz + rd];
+                U[j] = in[pz + ru];
+                F[j] = in[XY * zf + rc];
+                GL[j] = G.gh[xl + X * z];
+                GR[j] = G.gh[xr + X * z];
+                GF[j] = G.gh[x + X * zf];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ZB; j++) {
+            const int z = zg + j;
+            if (z < z1) {
+                if (y > ghc) {
+                    const bool oL = x == 0, oR = x == X - 1, oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
+                    // a position outside the domain clamps onto this sky cell, so
+                    // the ground test at the clamped column is the in-domain test
+                    const bool gL = y <= GL[j], gR = y <= GR[j], gD = y - 1 <= ghc;
+                    const bool gF = !oF && y <= GF[j], gB = !oB && y <= ghb;
+                    float l = nbv(L[j], oL, gL, pv, b), r = nbv(R[j], oR, gR, pv, b);
+                    float d = nbv(D[j], false, gD, pvd, b), u = nbv(U[j], oU, false, pvu, b);
+                    float f = nbv(F[j], oF, gF, pv, b), bb = nbv(vb, oB, gB, pv, b);
+                    if (type == 0 && (gL | gR | gD | gF | gB)) {
+                        // potential temperature: a ground neighbour reads the potential
+                        // temperature of the ground temperature at this cell's pressure (:95-100)
+                        const float pc = p[XY * z + rc];
+                        auto tg = [&](int col) { return d_potTemp(gT[col] - 273.15f, pc, gP[col]) + 273.15f; };
+                        if (gD) d = tg(x + X * z);
+                        if (gL) l = tg(xl + X * z);
+                        if (gR) r = tg(xr + X * z);
+                        if (gF) f = tg(x + X * (z + 1));
+                        if (gB) bb = tg(x + X * (z - 1));
+                    }
+                    out[XY * z + rc] = (vc + k * (l + r + u + d + f + bb)) / den;
+                }
+                vb = vc;
+                vc = F[j];
+                ghb = ghc;
+                ghc = GF[j];
+            }
+        }
     }
-    out[i] = (in[i] + k * (l + r + u + d + f + bb)) / (1 + 6 * k);
 }
 hipError_t launch_diffuse_sweep(EnvState& e, const float* in, float* out, const float* prof, Bounds b, int type, float k) {
-    hipLaunchKernelGGL(k_diffuse, grid3(e), B3, 0, e.stream, geom(e), in, out, prof, b, type, k, e.f[SKT_GROUND_T],
-                       e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
+    const ZGrid zg = zgrid(e);
+    hipLaunchKernelGGL(k_diffuse<ZB_DIFF>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, in, out, prof, b, type, k,
+                       e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
     return hipGetLastError();
 }
 
 // ------------------------------------------------------------- advection ---
 // getMaxDivergence over u, v, w (Game/kernelSky.cu:910-953; the three calls of
 // Game/environment.cu:720-722 share one maximum): |a| at z=0 and the signed
-// value elsewhere (mode 0, quirk A6) or |a| everywhere (mode 1). The last
-// block turns it into the substep count and length (:726-730).
-KL void k_maxvel(Geom G, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w, int mode,
-                 double* part, Scalars* sc, unsigned nblocks, float dt, float dx, int maxsub) {
-    CELL_SETUP;
+// value elsewhere (mode 0, quirk A6) or |a| everywhere (mode 1).
+constexpr int ZB_MAXV = 4;
+template <int ZB>
+KL void k_maxvel(Geom G, int zc, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                 int mode, double* part) {
+    ZM_SETUP;
     __shared__ double sh[4];
     double m = 0.0;
-    if (inb && !isGround(G, x, y, z)) {
-        const long i = gidx(G, x, y, z);
-        const bool ab = mode != 0 || z == 0;
-        const float a0 = u[i], a1 = v[i], a2 = w[i];
-        m = fmax(m, (double)(ab ? fabsf(a0) : a0));
-        m = fmax(m, (double)(ab ? fabsf(a1) : a1));
-        m = fmax(m, (double)(ab ? fabsf(a2) : a2));
-    }
-    m = blockMax(m, sh);
-    if (publishLast(part, bid, m, &sc->counter[0], nblocks)) {
-        __threadfence();
-        const double mv = finalMax(part, nblocks, sh);
-        if (tid256() == 0) {
-            const float maxVel = (float)mv;
-            const float C = maxVel * dt / dx;
-            const int ns = int(fminf(ceilf(fabsf(C)), (float)maxsub));
-            sc->maxvel = maxVel;
-            sc->nsub = ns;
-            sc->dtsub = ns > 0 ? dt / ns : 0.0f;
-            sc->counter[0] = 0u;
+    if (inb) {
+        const long XY = (long)G.X * G.Y;
+        const int rc = x + G.X * y;
+        for (int zg = z0; zg < z1; zg += ZB) {
+            float a0[ZB], a1[ZB], a2[ZB];
+            int gh[ZB];
+#pragma unroll
+            for (int j = 0; j < ZB; j++) {
+                const int z = zg + j;
+                if (z < z1) {
+                    const long i = XY * z + rc;
+                    a0[j] = u[i];
+                    a1[j] = v[i];
+                    a2[j] = w[i];
+                    gh[j] = G.gh[x + G.X * z];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < ZB; j++) {
+                const int z = zg + j;
+                if (z < z1 && y > gh[j]) {
+                    const bool ab = mode != 0 || z == 0;
+                    m = fmax(m, (double)(ab ? fabsf(a0[j]) : a0[j]));
+                    m = fmax(m, (double)(ab ? fabsf(a1[j]) : a1[j]));
+                    m = fmax(m, (double)(ab ? fabsf(a2[j]) : a2[j]));
+                }
+            }
         }
     }
+    m = blockMax(m, sh);
+    if (tid256() == 0) part[bid] = m;
+}
+// Substep count and length of advectPPMWGPU (Game/environment.cu:726-730).
+KL void k_maxvel_finalize(const double* __restrict__ part, unsigned n, Scalars* sc, float dt, float dx, int maxsub) {
+    __shared__ double sh[4];
+    double vmax = 0.0;
+    for (unsigned j = tid256(); j < n; j += 256) vmax = fmax(vmax, part[j]);
+    vmax = blockMax(vmax, sh);
+    if (tid256() != 0) return;
+    const float maxVel = (float)vmax;
+    const float C = maxVel * dt / dx;
+    const int ns = int(fminf(ceilf(fabsf(C)), (float)maxsub));
+    sc->maxvel = maxVel;
+    sc->nsub = ns;
+    sc->dtsub = ns > 0 ? dt / ns : 0.0f;
 }
 hipError_t launch_maxvel(EnvState& e, float dt) {
-    hipLaunchKernelGGL(k_maxvel, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ],
-                       e.cfg.maxabs_mode, e.partials, e.sc, (unsigned)blocks3d(e), dt, e.dx, e.cfg.max_substeps);
+    const ZGrid zg = zgrid(e);
+    hipLaunchKernelGGL(k_maxvel<ZB_MAXV>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.f[SKT_VELX], e.f[SKT_VELY],
+                       e.f[SKT_VELZ], e.cfg.maxabs_mode, e.partials);
+    hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(k_maxvel_finalize, dim3(1), B3, 0, e.stream, e.partials, zg.nblocks, e.sc, dt, e.dx,
+                       e.cfg.max_substeps);
     return hipGetLastError();
 }
 
@@ -377,228 +237,204 @@ hipError_t launch_maxvel(EnvState& e, float dt) {
 // with the unit-density pass and divideValues (Game/kernelMaths.cu:38-49):
 // out = (q - c dF(q)) / (1 - c dF(1)) on sky cells, c = dt/dx. The second
 // face velocity reads through the advected field's bounds (:415,486,559).
-KL void k_ppm(Geom G, const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ vel,
-              const float* __restrict__ prof, const float* __restrict__ ones, Bounds b, int dir, float dt, float dx,
-              const float* __restrict__ defU, const float* __restrict__ defW) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
-    const long i = gidx(G, x, y, z);
-    const int ox = dir == 0, oy = dir == 1, oz = dir == 2;
-    float q[5], o[5];
-#pragma unroll
-    for (int k = -2; k <= 2; k++) {
-        const int px = x + k * ox, py = y + k * oy, pz = z + k * oz;
-        q[k + 2] = extv(G, in, prof, b, px, py, pz);
-        // unit field: 1 inside, its boundary condition elsewhere
-        const int bx = clampi(px, 0, G.X - 1), by = clampi(py, 0, G.Y - 1), bz = clampi(pz, 0, G.Z - 1);
-        const bool outside = (bx != px) | (by != py) | (bz != pz);
-        if (isGround(G, bx, by, bz)) o[k + 2] = bcv(b.ground, 1.0f, ones[by]);
-        else if (outside) o[k + 2] = bcv(b.sides, 1.0f, ones[by]);
-        else o[k + 2] = 1.0f;
-    }
-    const float vc = vel[i];
-    const float cust = dir == 0 ? defU[y] : (dir == 1 ? 0.0f : defW[y]);
-    const float vm = nbOwn(G, vel, b, cust, x, y, z, -ox, -oy, -oz);
+// q[0..4] / o[0..4]: the field and the unit field at offsets -2..2 along the
+// pass direction; vc, vm: the face velocities at +1/2 and -1/2.
+__device__ __forceinline__ float ppmCell(const float* q, const float* o, float vc, float vm, float dt, float dx) {
     const float Fr = vc >= 0.0f ? ppmFlux(vc, q[1], q[2], q[3], dt, dx) : ppmFlux(vc, q[4], q[3], q[2], dt, dx);
     const float Fl = vm < 0.0f ? ppmFlux(vm, q[3], q[2], q[1], dt, dx) : ppmFlux(vm, q[0], q[1], q[2], dt, dx);
-    const float Gr = vc >= 0.0f ? ppmFlux(vc, o[1], o[2], o[3], dt, dx) : ppmFlux(vc, o[4], o[3], o[2], dt, dx);
-    const float Gl = vm < 0.0f ? ppmFlux(vm, o[3], o[2], o[1], dt, dx) : ppmFlux(vm, o[0], o[1], o[2], dt, dx);
-    const float num = in[i] - (dt / dx) * (Fr - Fl);
-    const float den = 1.0f - (dt / dx) * (Gr - Gl);
-    out[i] = num / den;
+    float Gr = vc, Gl = vm;
+    // A unit stencil (1,1,1) reconstructs the face value 1 exactly, so away
+    // from boundaries the unit field's flux is the face velocity itself.
+    if (!(o[0] == 1.0f && o[1] == 1.0f && o[3] == 1.0f && o[4] == 1.0f)) {
+        Gr = vc >= 0.0f ? ppmFlux(vc, o[1], o[2], o[3], dt, dx) : ppmFlux(vc, o[4], o[3], o[2], dt, dx);
+        Gl = vm < 0.0f ? ppmFlux(vm, o[3], o[2], o[1], dt, dx) : ppmFlux(vm, o[0], o[1], o[2], dt, dx);
+    }
+    const float c = dt / dx;
+    const float num = q[2] - c * (Fr - Fl);
+    const float den = 1.0f - c * (Gr - Gl);
+    return num / den;
 }
+
+// x pass: the stencil runs along the row.
+constexpr int ZB_PPM = 4;
+template <int ZB>
+KL void k_ppm_x(Geom G, int zc, const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ vel,
+                const float* __restrict__ prof, Bounds b, float dt, float dx, const float* __restrict__ cust) {
+    ZM_SETUP;
+    (void)bid;
+    if (!inb) return;
+    const int X = G.X;
+    const long XY = (long)X * G.Y;
+    int cx[5];
+#pragma unroll
+    for (int s = 0; s < 5; s++) cx[s] = clampi(x + s - 2, 0, X - 1);
+    const int rowy = X * y;
+    const float pv = prof ? prof[y] : 0.0f, cv = cust[y];
+    for (int zg = z0; zg < z1; zg += ZB) {
+        float Q[ZB][5], VC[ZB], VM[ZB];
+        int GH[ZB][5];
+#pragma unroll
+        for (int j = 0; j < ZB; j++) {
+            const int z = zg + j;
+            if (z < z1) {
+                const long pz = XY * z + rowy;
+#pragma unroll
+                for (int s = 0; s < 5; s++) {
+                    Q[j][s] = in[pz + cx[s]];
+                    GH[j][s] = G.gh[cx[s] + X * z];
+                }
+                VC[j] = vel[pz + x];
+                VM[j] = vel[pz + cx[1]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ZB; j++) {
+            const int z = zg + j;
+            if (z < z1 && y > GH[j][2]) {
+                float q[5], o[5];
+#pragma unroll
+                for (int s = 0; s < 5; s++) {
+                    const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
+                    const bool gnd = y <= GH[j][s];
+                    q[s] = nbv(Q[j][s], outp, gnd, pv, b);
+                    o[s] = unitv(outp, gnd, b);
+                }
+                const float vc = VC[j];
+                const float vm = x == 0 ? bcv(b.sides, vc, cv) : (y <= GH[j][1] ? bcv(b.ground, vc, cv) : VM[j]);
+                out[XY * z + rowy + x] = ppmCell(q, o, vc, vm, dt, dx);
+            }
+        }
+    }
+}
+
+// y pass: the stencil runs along the column of this thread.
+template <int ZB>
+KL void k_ppm_y(Geom G, int zc, const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ vel,
+                const float* __restrict__ prof, Bounds b, float dt, float dx) {
+    ZM_SETUP;
+    (void)bid;
+    if (!inb) return;
+    const int X = G.X, Y = G.Y;
+    const long XY = (long)X * Y;
+    int ry[5];
+    float pv[5];
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+        const int by = clampi(y + s - 2, 0, Y - 1);
+        ry[s] = x + X * by;
+        pv[s] = prof ? prof[by] : 0.0f;
+    }
+    for (int zg = z0; zg < z1; zg += ZB) {
+        float Q[ZB][5], VC[ZB], VM[ZB];
+        int GH[ZB];
+#pragma unroll
+        for (int j = 0; j < ZB; j++) {
+            const int z = zg + j;
+            if (z < z1) {
+                const long pz = XY * z;
+#pragma unroll
+                for (int s = 0; s < 5; s++) Q[j][s] = in[pz + ry[s]];
+                VC[j] = vel[pz + ry[2]];
+                VM[j] = vel[pz + ry[1]];
+                GH[j] = G.gh[x + X * z];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ZB; j++) {
+            const int z = zg + j;
+            const int gh = GH[j];
+            if (z < z1 && y > gh) {
+                float q[5], o[5];
+#pragma unroll
+                for (int s = 0; s < 5; s++) {
+                    const int py = y + s - 2;
+                    const bool outp = (py < 0) || (py > Y - 1);
+                    const bool gnd = clampi(py, 0, Y - 1) <= gh;
+                    q[s] = nbv(Q[j][s], outp, gnd, pv[s], b);
+                    o[s] = unitv(outp, gnd, b);
+                }
+                const float vc = VC[j];
+                // y - 1 >= 0 on every sky cell
+                const float vm = (y - 1 <= gh) ? bcv(b.ground, vc, 0.0f) : VM[j];
+                out[XY * z + ry[2]] = ppmCell(q, o, vc, vm, dt, dx);
+            }
+        }
+    }
+}
+
+// z pass: the stencil runs along this thread's march; a window of planes
+// zg-2 .. zg+ZB+1 stays in registers.
+template <int ZB>
+KL void k_ppm_z(Geom G, int zc, const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ vel,
+                const float* __restrict__ prof, Bounds b, float dt, float dx, const float* __restrict__ cust) {
+    ZM_SETUP;
+    (void)bid;
+    if (!inb || z0 >= z1) return;
+    const int X = G.X, Z = G.Z;
+    const long XY = (long)X * G.Y;
+    const int rc = x + X * y;
+    const float pv = prof ? prof[y] : 0.0f, cv = cust[y];
+    float qw[ZB + 4], vw[ZB + 1];
+    int gw[ZB + 4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int bz = clampi(z0 - 2 + s, 0, Z - 1);
+        qw[s] = in[XY * bz + rc];
+        gw[s] = G.gh[x + X * bz];
+    }
+    vw[0] = vel[XY * max(z0 - 1, 0) + rc];
+    for (int zg = z0; zg < z1; zg += ZB) {
+#pragma unroll
+        for (int j = 0; j < ZB; j++) {
+            if (zg + j < z1) {
+                const int bz = min(zg + j + 2, Z - 1);
+                qw[4 + j] = in[XY * bz + rc];
+                gw[4 + j] = G.gh[x + X * bz];
+                vw[1 + j] = vel[XY * (zg + j) + rc];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ZB; j++) {
+            const int z = zg + j;
+            if (z < z1 && y > gw[j + 2]) {
+                float q[5], o[5];
+#pragma unroll
+                for (int s = 0; s < 5; s++) {
+                    const int pz = z + s - 2;
+                    const bool outp = (pz < 0) || (pz > Z - 1);
+                    const bool gnd = y <= gw[j + s];
+                    q[s] = nbv(qw[j + s], outp, gnd, pv, b);
+                    o[s] = unitv(outp, gnd, b);
+                }
+                const float vc = vw[1 + j];
+                const float vm = z == 0 ? bcv(b.sides, vc, cv) : (y <= gw[j + 1] ? bcv(b.ground, vc, cv) : vw[j]);
+                out[XY * z + rc] = ppmCell(q, o, vc, vm, dt, dx);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            qw[s] = qw[s + ZB];
+            gw[s] = gw[s + ZB];
+        }
+        vw[0] = vw[ZB];
+    }
+}
+
 hipError_t launch_ppm_pass(EnvState& e, const float* in, float* out, const float* vel, const float* prof, Bounds b,
                            int dir, float dt) {
-    hipLaunchKernelGGL(k_ppm, grid3(e), B3, 0, e.stream, geom(e), in, out, vel, prof, e.ones, b, dir, dt, e.dx,
-                       e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ]);
-    return hipGetLastError();
-}
-
-// advectPrecipGPU, Game/kernelSky.cu:576-657: one thread per column walks y
-// upwards; each level reads its own and the next level's pre-step value
-// (the column snapshot of the reference), so the update can be in place.
-KL void k_precip(Geom G, float* q, const float* __restrict__ th, const float* __restrict__ p,
-                 const float* __restrict__ qv, const float* __restrict__ gP, int type, float dts, float dx, Gammas gm) {
-    const long g = blockIdx.x * 256L + threadIdx.x;
-    if (g >= (long)G.X * G.Z) return;
-    const int x = (int)(g % G.X), zc = (int)(g / G.X);
-    const float kap = kc::Rsd / kc::Cpd;
-    const float gp = gP[g];
-    auto fvAt = [&](long j, float Qj) {
-        const float T = th[j] * powf(p[j] / gp, kap);
-        const float Tv = T * (0.608f * qv[j] + 1);
-        const float den = p[j] * 100 / (kc::Rsd * Tv);
-        const float3 v3 = d_fallVel(Qj, Qj, Qj, den, type, gm.R48, gm.S425, gm.S425);
-        return type == 0 ? v3.x : (type == 1 ? v3.y : v3.z);
-    };
-    int y0 = G.gh[g] + 1;
-    if (y0 < 0) y0 = 0;
-    if (y0 >= G.Y) return;
-    long i = gidx(G, x, y0, zc);
-    float Qc = q[i];
-    float fv = fvAt(i, Qc);
-    for (int y = y0; y < G.Y; y++) {
-        const bool up = y < G.Y - 1;
-        float Qu = 0.0f, fvu = 0.0f;
-        const long iu = i + G.X;
-        if (up) {
-            Qu = q[iu];
-            fvu = fvAt(iu, Qu);
-        }
-        float nq = Qc;
-        nq -= dts * fminf((fv / dx) * Qc, Qc);
-        if (up) nq += dts * fminf((fvu / dx) * Qu, Qu);
-        q[i] = nq;
-        Qc = Qu;
-        fv = fvu;
-        i = iu;
-    }
-}
-hipError_t launch_precip(EnvState& e, float* q, int type, float dts) {
-    hipLaunchKernelGGL(k_precip, grid1(e.G), dim3(256), 0, e.stream, geom(e), q, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE],
-                       e.f[SKT_QV], e.f[SKT_GROUND_P], type, dts, e.dx, e.gm);
-    return hipGetLastError();
-}
-
-// -------------------------------------------------------------- buoyancy ---
-// buoyancyGPU, Game/kernelSky.cu:1061-1247 (bounds {D,D,D}); the ice test
-// 'T < 0' is on Kelvin and never holds (quirk A17).
-KL void k_buoyancy(Geom G, float* __restrict__ v, const float* __restrict__ th, const float* __restrict__ qv,
-                   const float* __restrict__ Qr, const float* __restrict__ Qs, const float* __restrict__ Qi,
-                   const float* __restrict__ p, const float* __restrict__ gP, const float* __restrict__ dT,
-                   const float* __restrict__ dQ, float dt) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
-    const Bounds b{BC_DIRICHLET, BC_DIRICHLET, BC_DIRICHLET};
-    const float kap = kc::Rsd / kc::Cpd;
-    const long i = gidx(G, x, y, z);
-    const float cur = v[i];
-    const bool up = y < G.Y - 1;
-    const bool down = y > 0 && !isGround(G, x, y - 1, z);
-    const float gps = gP[x + (long)G.X * z];
-    const float psC = p[i];
-    const float psU = up ? p[i + G.X] : psC;
-    const float psD = down ? p[i - G.X] : psC;
-    const bool l = skyAt(G, x - 1, y, z), r = skyAt(G, x + 1, y, z);
-    const bool bk = skyAt(G, x, y, z - 1), fw = skyAt(G, x, y, z + 1);
-    const int validEnv = int(l) + int(r) + int(bk) + int(fw);
-    if (validEnv == 0) return;
-    const float QP = qv[i];
-    const float Qenv = (extv(G, qv, dQ, b, x + 1, y, z) + extv(G, qv, dQ, b, x - 1, y, z) + extv(G, qv, dQ, b, x, y, z + 1) +
-                        extv(G, qv, dQ, b, x, y, z - 1)) / validEnv;
-    const float QenvUp = extv(G, qv, dQ, b, x, y + 1, z);
-    const float QenvDown = extv(G, qv, dQ, b, x, y - 1, z);
-    const float thD = extv(G, th, dT, b, x, y - 1, z), thU = extv(G, th, dT, b, x, y + 1, z);
-    const float TDown = thD * powf(psD / gps, kap);
-    const float T = th[i] * powf(psC / gps, kap);
-    const float TUp = thU * powf(psU / gps, kap);
-    float Tad = th[i], TadU = th[i], TadD = th[i];
-    const float Tenv = (extv(G, th, dT, b, x + 1, y, z) + extv(G, th, dT, b, x - 1, y, z) + extv(G, th, dT, b, x, y, z + 1) +
-                        extv(G, th, dT, b, x, y, z - 1)) / validEnv;
-    const float TenvUp = thU, TenvDown = thD;
-    if (down) {
-        if ((T < 0.0f && QP >= d_wi((TDown - 273.15f), psD)) || QP >= d_ws((TDown - 273.15f), psD)) {
-            TadD = T - d_MLR(T - 273.15f, psC) * (psC - psD);
-            TadD = TadD * powf(gps / psD, kap);
-        }
-    }
-    if (up) {
-        if ((T < 0.0f && QP >= d_wi((TUp - 273.15f), psU)) || QP >= d_ws((TUp - 273.15f), psU)) {
-            TadU = d_MLR(T - 273.15f, psC) * (psU - psC) + T;
-            TadU = TadU * powf(gps / psU, kap);
-        }
-    }
-    float B = 0.0f, BU = 0.0f, BD = 0.0f;
-    const float totalQ = Qr[i] + Qs[i] + Qi[i];
-    if (down) {
-        const float VTP = TadD * (1.0f + 0.608f * QP), VTE = TenvDown * (1.0f + 0.608f * QenvDown);
-        BD = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
-    }
-    {
-        const float VTP = Tad * (1.0f + 0.608f * QP), VTE = Tenv * (1.0f + 0.608f * Qenv);
-        B = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
-    }
-    if (up) {
-        const float VTP = TadU * (1.0f + 0.608f * QP), VTE = TenvUp * (1.0f + 0.608f * QenvUp);
-        BU = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
-    }
-    float bf = B;
-    if (cur > 0.0f) bf = BU;
-    else if (cur < 0.0f) bf = BD;
-    float change = bf * dt;
-    if ((up && cur > 0.0f && -change > cur) || (down && cur < 0.0f && -change < cur)) change = -cur;
-    v[i] = cur + change;
-}
-hipError_t launch_buoyancy(EnvState& e, float dt) {
-    hipLaunchKernelGGL(k_buoyancy, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_VELY], e.f[SKT_POTTEMP], e.f[SKT_QV],
-                       e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], e.f[SKT_ISEN_TEMP],
-                       e.f[SKT_ISEN_VAPOR], dt);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------------------- EOS ---
-// initDensity, Game/kernelSky.cu:1432-1452 (+ the old-density copy)
-KL void k_init_density(Geom G, float* __restrict__ rho, float* __restrict__ rho_old, const float* __restrict__ th,
-                       const float* __restrict__ p, const float* __restrict__ qv, const float* __restrict__ gP) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
-    const long i = gidx(G, x, y, z);
-    const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
-    const float Tv = T * (0.608f * qv[i] + 1);
-    const float d = p[i] * 100 / (kc::Rsd * Tv);
-    rho[i] = d;
-    if (rho_old) rho_old[i] = d;
-}
-hipError_t launch_init_density(EnvState& e, int copy_old) {
-    hipLaunchKernelGGL(k_init_density, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_DENSITY],
-                       copy_old ? e.f[SKT_DENSITY_OLD] : nullptr, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE], e.f[SKT_QV],
-                       e.f[SKT_GROUND_P]);
-    return hipGetLastError();
-}
-// calculateNewPressure, Game/kernelSky.cu:1454-1474
-KL void k_new_pressure(Geom G, float* __restrict__ p, const float* __restrict__ rho, const float* __restrict__ th,
-                       const float* __restrict__ qv, const float* __restrict__ gP) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
-    const long i = gidx(G, x, y, z);
-    const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
-    const float Tv = T * (0.608f * qv[i] + 1);
-    p[i] = rho[i] * kc::Rsd * Tv / 100.0f;
-}
-hipError_t launch_new_pressure(EnvState& e) {
-    hipLaunchKernelGGL(k_new_pressure, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_PRESSURE], e.f[SKT_DENSITY],
-                       e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P]);
+    const ZGrid zg = zgrid(e);
+    if (dir == 0)
+        hipLaunchKernelGGL(k_ppm_x<ZB_PPM>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, in, out, vel, prof, b, dt, e.dx,
+                           e.f[SKT_DEFAULT_VELX]);
+    else if (dir == 1)
+        hipLaunchKernelGGL(k_ppm_y<ZB_PPM>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, in, out, vel, prof, b, dt, e.dx);
+    else
+        hipLaunchKernelGGL(k_ppm_z<ZB_PPM>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, in, out, vel, prof, b, dt, e.dx,
+                           e.f[SKT_DEFAULT_VELZ]);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------- pressure projection ---
-// resetVelPressProj, Game/kernelSky.cu:1648-1682: one thread per (x,y) column;
-// the column stops at its first ground cell (quirk A16).
-KL void k_reset_vel(Geom G, float* __restrict__ u, float* __restrict__ v, float* __restrict__ w,
-                    const float* __restrict__ defU, const float* __restrict__ defW) {
-    const long c = blockIdx.x * 256L + threadIdx.x;
-    if (c >= (long)G.X * G.Y) return;
-    const int x = (int)(c % G.X), y = (int)(c / G.X);
-    for (int z = 0; z < G.Z; z++) {
-        if (isGround(G, x, y, z)) break;
-        const long i = gidx(G, x, y, z);
-        if (y == G.Y - 1 || isGround(G, x, y + 1, z)) v[i] = 0.0f;
-        if (x == G.X - 1) u[i] = defU[y];
-        if (x < G.X - 1 && isGround(G, x + 1, y, z)) u[i] = 0.0f;
-        if (z == G.Z - 1) w[i] = defW[y];
-        if (z < G.Z - 1 && isGround(G, x, y, z + 1)) w[i] = 0.0f;
-    }
-}
-hipError_t launch_reset_vel(EnvState& e) {
-    hipLaunchKernelGGL(k_reset_vel, grid1((long)e.X * e.Y), dim3(256), 0, e.stream, geom(e), e.f[SKT_VELX],
-                       e.f[SKT_VELY], e.f[SKT_VELZ], e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ]);
-    return hipGetLastError();
-}
-
 __device__ __forceinline__ float faceCoef(float cur, float nb) {
     return nb == 0.0f ? 0.0f : -fmaxf(1.0f, 1.225f / (2.0f / (1.0f / cur + 1.0f / nb)));
 }
@@ -640,224 +476,360 @@ hipError_t launch_build_A(EnvState& e) {
     return hipGetLastError();
 }
 
+// Scalar steps of the solve, run by every block of the consuming kernel
+// (fixed-order reductions, identical in every block); block 0 stores the
+// result. Steps follow calculatePressureProject (Game/environment.cu:930-985):
+//   PRO_START: initial stop-test value; done if it is 0 (:936); tolerance
+//   PRO_A0:    sigma0 = z.r (:943)
+//   PRO_A:     stop test of the previous iteration (:967); then sigma1 = z.r,
+//              beta = sigma1 / sigma0, sigma0 = sigma1 (:975-984)
+//   PRO_B:     sigma1 (+= in reference mode: the value is not cleared before
+//              the s.As product from the second iteration on), alpha
+//   PRO_FINAL: stop test after the last iteration
+enum { PRO_START = 0, PRO_A0, PRO_A, PRO_B, PRO_FINAL };
+__device__ void pcgPrologue(int kind, const double* __restrict__ pm, const double* __restrict__ ps, unsigned n,
+                            const PcgState* __restrict__ sin, PcgState* __restrict__ sout, int ref, float tol, int it,
+                            PcgState& S, double* sh) {
+    const int t = tid256();
+    if (t == 0) {
+        if (kind == PRO_START) {
+            S = PcgState{};
+        } else {
+            S = *sin;
+        }
+    }
+    __syncthreads();
+    if (!S.done) {
+        double vmax = 0.0, vsum = 0.0;
+        if (kind == PRO_START || kind == PRO_A || kind == PRO_FINAL) {
+            for (unsigned j = t; j < n; j += 256) vmax = fmax(vmax, pm[j]);
+            vmax = blockMax(vmax, sh);
+        }
+        if (kind == PRO_A0 || kind == PRO_A || kind == PRO_B) {
+            for (unsigned j = t; j < n; j += 256) vsum += ps[j];
+            vsum = blockSum(vsum, sh);
+        }
+        if (t == 0) {
+            switch (kind) {
+                case PRO_START: {
+                    const float m0 = (float)vmax;
+                    S.maxr0 = m0;
+                    S.maxr = m0;
+                    S.iters = 0;
+                    S.done = (m0 == 0.0f) ? 1 : 0;
+                    S.tol = ref ? tol : tol * m0;
+                } break;
+                case PRO_A0:
+                    S.sigma0 = (double)(float)vsum;
+                    S.sigma1 = 0.0;
+                    break;
+                case PRO_A:
+                case PRO_FINAL: {
+                    const float mr = (float)vmax;
+                    S.maxr = mr;
+                    S.iters = it;
+                    if (mr <= S.tol) {
+                        S.done = 1;
+                    } else if (kind == PRO_A) {
+                        const float s1 = (float)vsum;
+                        S.sigma1 = (double)s1;
+                        S.beta = s1 / (float)S.sigma0;
+                        S.sigma0 = (double)s1;
+                    }
+                } break;
+                case PRO_B: {
+                    const float s1 = ref ? (float)((double)(float)S.sigma1 + vsum) : (float)vsum;
+                    S.sigma1 = (double)s1;
+                    S.alpha = (float)S.sigma0 / s1;
+                } break;
+                default: break;
+            }
+        }
+    }
+    if (t == 0 && blockIdx.x == 0) *sout = S;
+    __syncthreads();
+}
+
 // calculateDivergenceGPU, Game/kernelSky.cu:780-860 (density {N,N,D}, velocity
-// {C,C,D}) with the initial stop-test value of calculatePressureProject
-// (Game/environment.cu:933-936) reduced on the device. p' is cleared.
-KL void k_divergence(Geom G, float* __restrict__ r, float* __restrict__ pp, const float* __restrict__ rho,
+// {C,C,D}) with the per-block maxima of the initial stop-test value
+// (Game/environment.cu:933-936). p' is cleared.
+KL void k_divergence(Geom G, int zc, float* __restrict__ r, float* __restrict__ pp, const float* __restrict__ rho,
                      const float* __restrict__ rold, const float* __restrict__ u, const float* __restrict__ v,
                      const float* __restrict__ w, const float* __restrict__ defU, const float* __restrict__ defW,
-                     const float* __restrict__ ones, float dt, int absmode, double* part, Scalars* sc, unsigned nblocks,
-                     int ref, float tol) {
-    CELL_SETUP;
+                     const float* __restrict__ ones, float dt, int absmode, double* part) {
+    ZM_SETUP;
     __shared__ double sh[4];
     double m = 0.0;
     if (inb) {
-        const long i = gidx(G, x, y, z);
-        pp[i] = 0.0f;
-        float dv = 0.0f;
-        if (!isGround(G, x, y, z)) {
-            const Bounds bd{BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET}, bv{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET};
-            const float ur = u[i], ul = nbOwn(G, u, bv, defU[y], x, y, z, -1, 0, 0);
-            const float uu = v[i], ud = nbOwn(G, v, bv, 0.0f, x, y, z, 0, -1, 0);
-            const float uf = w[i], ub = nbOwn(G, w, bv, defW[y], x, y, z, 0, 0, -1);
-            const float cur = rho[i];
-            float dr = extv(G, rho, ones, bd, x + 1, y, z), dl = extv(G, rho, ones, bd, x - 1, y, z);
-            float du = extv(G, rho, ones, bd, x, y + 1, z), dd = extv(G, rho, ones, bd, x, y - 1, z);
-            float df = extv(G, rho, ones, bd, x, y, z + 1), db = extv(G, rho, ones, bd, x, y, z - 1);
-            dr = dr == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dr);
-            dl = dl == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dl);
-            du = du == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / du);
-            dd = dd == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dd);
-            df = df == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / df);
-            db = db == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / db);
-            const float mfd = ((dr * ur - dl * ul) + (du * uu - dd * ud) + (df * uf - db * ub));
-            const float dc = (cur - rold[i]) / dt;
-            dv = (mfd - dc);
-            const bool ab = absmode || z == 0;
-            m = (double)(ab ? fabsf(dv) : dv);
-            m = fmax(0.0, m);
+        for (int z = z0; z < z1; z++) {
+            const long i = gidx(G, x, y, z);
+            pp[i] = 0.0f;
+            float dv = 0.0f;
+            if (!isGround(G, x, y, z)) {
+                const Bounds bd{BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET}, bv{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET};
+                const float ur = u[i], ul = nbOwn(G, u, bv, defU[y], x, y, z, -1, 0, 0);
+                const float uu = v[i], ud = nbOwn(G, v, bv, 0.0f, x, y, z, 0, -1, 0);
+                const float uf = w[i], ub = nbOwn(G, w, bv, defW[y], x, y, z, 0, 0, -1);
+                const float cur = rho[i];
+                float dr = extv(G, rho, ones, bd, x + 1, y, z), dl = extv(G, rho, ones, bd, x - 1, y, z);
+                float du = extv(G, rho, ones, bd, x, y + 1, z), dd = extv(G, rho, ones, bd, x, y - 1, z);
+                float df = extv(G, rho, ones, bd, x, y, z + 1), db = extv(G, rho, ones, bd, x, y, z - 1);
+                dr = dr == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dr);
+                dl = dl == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dl);
+                du = du == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / du);
+                dd = dd == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dd);
+                df = df == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / df);
+                db = db == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / db);
+                const float mfd = ((dr * ur - dl * ul) + (du * uu - dd * ud) + (df * uf - db * ub));
+                const float dc = (cur - rold[i]) / dt;
+                dv = (mfd - dc);
+                const bool ab = absmode || z == 0;
+                m = fmax(m, (double)(ab ? fabsf(dv) : dv));
+            }
+            r[i] = dv;
         }
-        r[i] = dv;
     }
     m = blockMax(m, sh);
-    if (publishLast(part, bid, m, &sc->counter[1], nblocks)) {
-        __threadfence();
-        const double mx = finalMax(part, nblocks, sh);
-        if (tid256() == 0) {
-            const float m0 = (float)mx;
-            sc->maxr0 = m0;
-            sc->maxr = m0;
-            sc->iters = 0;
-            sc->done = (m0 == 0.0f) ? 1 : 0;
-            sc->tol = ref ? tol : tol * m0;
-            sc->counter[1] = 0u;
-        }
-    }
+    if (tid256() == 0) part[bid] = m;
 }
 hipError_t launch_divergence(EnvState& e, float dt) {
     const int ref = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     const int absmode = ref ? e.cfg.maxabs_mode : 1;
-    hipLaunchKernelGGL(k_divergence, grid3(e), B3, 0, e.stream, geom(e), e.pcg[0], e.f[SKT_PRESSURE_PROJ],
+    const ZGrid zg = zgrid(e);
+    hipLaunchKernelGGL(k_divergence, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[0], e.f[SKT_PRESSURE_PROJ],
                        e.f[SKT_DENSITY], e.f[SKT_DENSITY_OLD], e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ],
-                       e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ], e.ones, dt, absmode, e.partials, e.sc,
-                       (unsigned)blocks3d(e), ref, e.cfg.pcg_tol);
+                       e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ], e.ones, dt, absmode,
+                       e.partials + (size_t)PART_DIV * e.maxBlocks);
     return hipGetLastError();
 }
 
-// z = M^-1 r (applyPreconditionerGPU, Game/kernelSky.cu:659-673) and
-// sigma0 = z.r (dotProductGPU, :675-717) before the loop.
-KL void k_pcg_start(Geom G, const float* __restrict__ r, float* __restrict__ zz, const float* __restrict__ pre,
-                    double* part, Scalars* sc, unsigned nblocks) {
-    CELL_SETUP;
+// Prologue PRO_START, then z = M^-1 r (applyPreconditionerGPU,
+// Game/kernelSky.cu:659-673) and the z.r partials (dotProductGPU, :675-717).
+KL void k_pcg_start(Geom G, int zc, const float* __restrict__ r, float* __restrict__ zz, const float* __restrict__ pre,
+                    const double* __restrict__ pin, unsigned n, double* pout, const PcgState* sin, PcgState* sout,
+                    int ref, float tol) {
+    ZM_SETUP;
     __shared__ double sh[4];
-    if (sc->done) return;
+    __shared__ PcgState S;
+    pcgPrologue(PRO_START, pin, nullptr, n, sin, sout, ref, tol, 0, S, sh);
+    if (S.done) return;
     double s = 0.0;
     if (inb) {
-        const long i = gidx(G, x, y, z);
-        float zv = 0.0f;
-        if (!isGround(G, x, y, z)) {
-            zv = r[i] * pre[i];
-            s = (double)(zv * r[i]);
+        for (int z = z0; z < z1; z++) {
+            const long i = gidx(G, x, y, z);
+            float zv = 0.0f;
+            if (!isGround(G, x, y, z)) {
+                zv = r[i] * pre[i];
+                s += (double)(zv * r[i]);
+            }
+            zz[i] = zv;
         }
-        zz[i] = zv;
     }
     s = blockSum(s, sh);
-    if (publishLast(part, bid, s, &sc->counter[2], nblocks)) {
-        __threadfence();
-        const double t = finalSum(part, nblocks, sh);
-        if (tid256() == 0) {
-            sc->sigma0 = (double)(float)t;
-            sc->sigma1 = 0.0;
-            sc->counter[2] = 0u;
-        }
-    }
+    if (tid256() == 0) pout[bid] = s;
 }
 hipError_t launch_pcg_start(EnvState& e) {
-    hipLaunchKernelGGL(k_pcg_start, grid3(e), B3, 0, e.stream, geom(e), e.pcg[0], e.pcg[1], e.pcg[5], e.partials, e.sc,
-                       (unsigned)blocks3d(e));
+    const int ref = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
+    const ZGrid zg = zgrid(e);
+    const size_t M = e.maxBlocks;
+    hipLaunchKernelGGL(k_pcg_start, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[0], e.pcg[1], e.pcg[5],
+                       e.partials + PART_DIV * M, zg.nblocks, e.partials + PART_START * M, &e.sc->st[0],
+                       &e.sc->st[1], ref, e.cfg.pcg_tol);
     return hipGetLastError();
 }
 
-// Search-direction update (endIteration, Game/kernelSky.cu:987-1014) fused
-// into applyAGPU (:719-778) and the s.As product (:675-717):
-// s_new = z + beta s_old (s_new = z on the first iteration), As = A s_new.
-// The neighbours' s_new is recomputed from their z and s_old.
-KL void k_pcg_A(Geom G, int first, const float* __restrict__ zz, const float* __restrict__ s_old,
-                float* __restrict__ s_new, const float4* __restrict__ A, float* __restrict__ As, double* part,
-                Scalars* sc, unsigned nblocks, int ref) {
-    CELL_SETUP;
+// Prologue PRO_A0 / PRO_A, then the search-direction update (endIteration,
+// Game/kernelSky.cu:987-1014) fused into applyAGPU (:719-778) and the s.As
+// partials (:675-717): s_new = z + beta s_old (s_new = z on the first
+// iteration), As = A s_new. s_new of the x/y neighbours is recomputed from
+// their z and s_old; along z it stays in registers.
+constexpr int ZB_PCGA = 2;
+template <int ZB>
+KL void k_pcg_A(Geom G, int zc, int it, const float* __restrict__ zz, const float* __restrict__ s_old,
+                float* __restrict__ s_new, const float4* __restrict__ A, float* __restrict__ As,
+                const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, double* pout,
+                const PcgState* sin, PcgState* sout) {
+    ZM_SETUP;
     __shared__ double sh[4];
-    if (sc->done) return;
-    const float beta = first ? 0.0f : sc->beta;
+    __shared__ PcgState S;
+    pcgPrologue(it == 0 ? PRO_A0 : PRO_A, pm, ps, n, sin, sout, 0, 0.0f, it, S, sh);
+    if (S.done) return;
+    const bool first = it == 0;
+    const float beta = S.beta;
     double acc = 0.0;
-    if (inb) {
-        const long i = gidx(G, x, y, z);
-        if (isGround(G, x, y, z)) {
-            s_new[i] = 0.0f;
-            As[i] = 0.0f;
-        } else {
-            auto sn = [&](long j) { return first ? zz[j] : zz[j] + beta * s_old[j]; };
-            const long XY = (long)G.X * G.Y;
-            const bool sl = skyAt(G, x - 1, y, z), sr = skyAt(G, x + 1, y, z), sd = skyAt(G, x, y - 1, z);
-            const bool su = skyAt(G, x, y + 1, z), sb = skyAt(G, x, y, z - 1), sf = skyAt(G, x, y, z + 1);
-            const float c = sn(i);
-            const float l = sl ? sn(i - 1) : 0.0f, r = sr ? sn(i + 1) : 0.0f;
-            const float d = sd ? sn(i - G.X) : 0.0f, u = su ? sn(i + G.X) : 0.0f;
-            const float bb = sb ? sn(i - XY) : 0.0f, f = sf ? sn(i + XY) : 0.0f;
-            const float4 Ac = A[i];
-            const float AL = sl ? trunc8(A[i - 1].x) : 0.0f;
-            const float AD = sd ? trunc8(A[i - G.X].y) : 0.0f;
-            const float AB = sb ? trunc8(A[i - XY].z) : 0.0f;
-            const float o = Ac.w * c + ((AL * l + AD * d + AB * bb + Ac.x * r + Ac.y * u + Ac.z * f));
-            s_new[i] = c;
-            As[i] = o;
-            acc = (double)(o * c);
+    if (inb && z0 < z1) {
+        const int X = G.X, Y = G.Y, Z = G.Z;
+        const long XY = (long)X * Y;
+        const int xl = max(x - 1, 0), xr = min(x + 1, X - 1), yd = max(y - 1, 0), yu = min(y + 1, Y - 1);
+        const int rc = x + X * y, rl = xl + X * y, rr = xr + X * y, rd = x + X * yd, ru = x + X * yu;
+        auto sn = [&](long j) { return first ? zz[j] : zz[j] + beta * s_old[j]; };
+        // column window: s_new and A.z of the plane below, s_new of this plane
+        const bool skB = z0 > 0 && y > G.gh[x + X * (z0 - 1)];
+        float snb = 0.0f, azb = 0.0f;
+        if (skB) {
+            snb = sn(XY * (z0 - 1) + rc);
+            azb = A[XY * (z0 - 1) + rc].z;
+        }
+        int ghc = G.gh[x + X * z0];
+        bool skC = y > ghc;
+        float snc = skC ? sn(XY * z0 + rc) : 0.0f;
+        for (int zg = z0; zg < z1; zg += ZB) {
+            float SL[ZB], SR[ZB], SD[ZB], SU[ZB], SF[ZB], AL[ZB], AD[ZB];
+            float4 AC[ZB];
+            int GL[ZB], GR[ZB], GF[ZB];
+#pragma unroll
+            for (int j = 0; j < ZB; j++) {
+                const int z = zg + j;
+                if (z < z1) {
+                    const long pz = XY * z;
+                    const int zf = min(z + 1, Z - 1);
+                    SL[j] = sn(pz + rl);
+                    SR[j] = sn(pz + rr);
+                    SD[j] = sn(pz + rd);
+                    SU[j] = sn(pz + ru);
+                    SF[j] = sn(XY * zf + rc);
+                    AC[j] = A[pz + rc];
+                    AL[j] = A[pz + rl].x;
+                    AD[j] = A[pz + rd].y;
+                    GL[j] = G.gh[xl + X * z];
+                    GR[j] = G.gh[xr + X * z];
+                    GF[j] = G.gh[x + X * zf];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < ZB; j++) {
+                const int z = zg + j;
+                if (z < z1) {
+                    const long i = XY * z + rc;
+                    const bool skF = z + 1 < Z && y > GF[j];
+                    const float snf = skF ? SF[j] : 0.0f;
+                    float azc = 0.0f;
+                    if (skC) {
+                        const float4 Ac = AC[j];
+                        const bool sl = x > 0 && y > GL[j], sr = x < X - 1 && y > GR[j];
+                        const bool sd = y - 1 > ghc, su = y + 1 < Y;
+                        const float l = sl ? SL[j] : 0.0f, r = sr ? SR[j] : 0.0f;
+                        const float d = sd ? SD[j] : 0.0f, u = su ? SU[j] : 0.0f;
+                        const float aL = sl ? trunc8(AL[j]) : 0.0f, aD = sd ? trunc8(AD[j]) : 0.0f;
+                        const float aB = trunc8(azb);
+                        const float o = Ac.w * snc + ((aL * l + aD * d + aB * snb + Ac.x * r + Ac.y * u + Ac.z * snf));
+                        azc = Ac.z;
+                        s_new[i] = snc;
+                        As[i] = o;
+                        acc += (double)(o * snc);
+                    } else {
+                        s_new[i] = 0.0f;
+                        As[i] = 0.0f;
+                    }
+                    snb = snc;
+                    azb = azc;
+                    snc = snf;
+                    skC = skF;
+                    ghc = GF[j];
+                }
+            }
         }
     }
     acc = blockSum(acc, sh);
-    if (publishLast(part, bid, acc, &sc->counter[2], nblocks)) {
-        __threadfence();
-        const double t = finalSum(part, nblocks, sh);
-        if (tid256() == 0) {
-            const float s1 = ref ? (float)((double)(float)sc->sigma1 + t) : (float)t;
-            sc->sigma1 = (double)s1;
-            sc->alpha = (float)sc->sigma0 / s1;
-            sc->counter[2] = 0u;
-        }
-    }
+    if (tid256() == 0) pout[bid] = acc;
 }
 hipError_t launch_pcg_A(EnvState& e, int it) {
     float* s_old = e.pcg[2 + ((it + 1) & 1)];
     float* s_new = e.pcg[2 + (it & 1)];
-    hipLaunchKernelGGL(k_pcg_A, grid3(e), B3, 0, e.stream, geom(e), it == 0 ? 1 : 0, e.pcg[1], s_old, s_new, e.A,
-                       e.pcg[4], e.partials, e.sc, (unsigned)blocks3d(e), e.cfg.pcg_mode == SKT_PCG_REFERENCE);
+    const ZGrid zg = zgrid(e);
+    const size_t M = e.maxBlocks;
+    const double* pm = e.partials + PART_BMAX * M;
+    const double* ps = it == 0 ? e.partials + PART_START * M : e.partials + PART_BSUM * M;
+    hipLaunchKernelGGL(k_pcg_A<ZB_PCGA>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, it, e.pcg[1], s_old, s_new, e.A,
+                       e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0]);
     return hipGetLastError();
 }
 
-// updatePandDiv (Game/kernelSky.cu:955-985) + the stop test
-// (Game/environment.cu:963-971) + applyPreconditionerGPU + z.r + endIteration's
-// scalar part: p' += alpha s, r -= alpha As, z = M^-1 r, beta = (z.r)/sigma0.
-KL void k_pcg_B(Geom G, int it, float* __restrict__ pp, float* __restrict__ r, const float* __restrict__ s,
-                const float* __restrict__ As, const float* __restrict__ pre, float* __restrict__ zz, double* part,
-                Scalars* sc, unsigned nblocks, int absmode) {
-    CELL_SETUP;
+// Prologue PRO_B, then updatePandDiv (Game/kernelSky.cu:955-985) + the
+// stop-test partials (Game/environment.cu:963-971) + applyPreconditionerGPU
+// + z.r partials: p' += alpha s, r -= alpha As, z = M^-1 r.
+constexpr int ZB_PCGB = 4;
+template <int ZB>
+KL void k_pcg_B(Geom G, int zc, float* __restrict__ pp, float* __restrict__ r, const float* __restrict__ s,
+                const float* __restrict__ As, const float* __restrict__ pre, float* __restrict__ zz,
+                const double* __restrict__ pin, unsigned n, double* pmax, double* psum, const PcgState* sin,
+                PcgState* sout, int ref, int absmode) {
+    ZM_SETUP;
     __shared__ double sh[4];
-    if (sc->done) return;
-    const float alpha = sc->alpha;
+    __shared__ PcgState S;
+    pcgPrologue(PRO_B, nullptr, pin, n, sin, sout, ref, 0.0f, 0, S, sh);
+    if (S.done) return;
+    const float alpha = S.alpha;
     double m = 0.0, dz = 0.0;
     if (inb) {
-        const long i = gidx(G, x, y, z);
-        float zv = 0.0f;
-        if (!isGround(G, x, y, z)) {
-            pp[i] += alpha * s[i];
-            const float rv = r[i] - alpha * As[i];
-            r[i] = rv;
-            const bool ab = absmode || z == 0;
-            m = fmax(0.0, (double)(ab ? fabsf(rv) : rv));
-            zv = rv * pre[i];
-            dz = (double)(zv * rv);
+        const long XY = (long)G.X * G.Y;
+        const int rc = x + G.X * y;
+        for (int zg = z0; zg < z1; zg += ZB) {
+            float P[ZB], Rv[ZB], Sv[ZB], Av[ZB], Pr[ZB];
+            int gh[ZB];
+#pragma unroll
+            for (int j = 0; j < ZB; j++) {
+                const int z = zg + j;
+                if (z < z1) {
+                    const long i = XY * z + rc;
+                    P[j] = pp[i];
+                    Rv[j] = r[i];
+                    Sv[j] = s[i];
+                    Av[j] = As[i];
+                    Pr[j] = pre[i];
+                    gh[j] = G.gh[x + G.X * z];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < ZB; j++) {
+                const int z = zg + j;
+                if (z < z1) {
+                    const long i = XY * z + rc;
+                    float zv = 0.0f;
+                    if (y > gh[j]) {
+                        pp[i] = P[j] + alpha * Sv[j];
+                        const float rv = Rv[j] - alpha * Av[j];
+                        r[i] = rv;
+                        const bool ab = absmode || z == 0;
+                        m = fmax(m, (double)(ab ? fabsf(rv) : rv));
+                        zv = rv * Pr[j];
+                        dz += (double)(zv * rv);
+                    }
+                    zz[i] = zv;
+                }
+            }
         }
-        zz[i] = zv;
     }
     m = blockMax(m, sh);
     dz = blockSum(dz, sh);
-    double* partM = part;
-    double* partD = part + nblocks;
-    __shared__ unsigned s_last;
     if (tid256() == 0) {
-        partM[bid] = m;
-        partD[bid] = dz;
-        __threadfence();
-        const unsigned prev = atomicAdd(&sc->counter[3], 1u);
-        s_last = (prev == nblocks - 1) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        const double mx = finalMax(partM, nblocks, sh);
-        const double t = finalSum(partD, nblocks, sh);
-        if (tid256() == 0) {
-            const float mr = (float)mx;
-            sc->maxr = mr;
-            sc->iters = it + 1;
-            if (mr <= sc->tol) {
-                sc->done = 1;
-            } else {
-                const float s1 = (float)t;
-                sc->sigma1 = (double)s1;
-                sc->beta = s1 / (float)sc->sigma0;
-                sc->sigma0 = (double)s1;
-            }
-            sc->counter[3] = 0u;
-        }
+        pmax[bid] = m;
+        psum[bid] = dz;
     }
 }
 hipError_t launch_pcg_B(EnvState& e, int it) {
     float* s_new = e.pcg[2 + (it & 1)];
     const int ref = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     const int absmode = ref ? e.cfg.maxabs_mode : 1;
-    hipLaunchKernelGGL(k_pcg_B, grid3(e), B3, 0, e.stream, geom(e), it, e.f[SKT_PRESSURE_PROJ], e.pcg[0], s_new,
-                       e.pcg[4], e.pcg[5], e.pcg[1], e.partials, e.sc, (unsigned)blocks3d(e), absmode);
+    const ZGrid zg = zgrid(e);
+    const size_t M = e.maxBlocks;
+    hipLaunchKernelGGL(k_pcg_B<ZB_PCGB>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.f[SKT_PRESSURE_PROJ], e.pcg[0],
+                       s_new, e.pcg[4], e.pcg[5], e.pcg[1], e.partials + PART_A * M, zg.nblocks,
+                       e.partials + PART_BMAX * M, e.partials + PART_BSUM * M, &e.sc->st[0], &e.sc->st[1], ref,
+                       absmode);
+    return hipGetLastError();
+}
+
+// Stop test after the last iteration (one block): slot 1 -> slot 0.
+KL void k_pcg_final(const double* __restrict__ pm, unsigned n, const PcgState* sin, PcgState* sout, int it) {
+    __shared__ double sh[4];
+    __shared__ PcgState S;
+    pcgPrologue(PRO_FINAL, pm, nullptr, n, sin, sout, 0, 0.0f, it, S, sh);
+}
+hipError_t launch_pcg_final(EnvState& e, int it) {
+    const ZGrid zg = zgrid(e);
+    hipLaunchKernelGGL(k_pcg_final, dim3(1), B3, 0, e.stream, e.partials + (size_t)PART_BMAX * e.maxBlocks,
+                       zg.nblocks, &e.sc->st[1], &e.sc->st[0], it);
     return hipGetLastError();
 }
 
@@ -916,301 +888,6 @@ hipError_t launch_apply_projection(EnvState& e, int new_pressure) {
     hipLaunchKernelGGL(k_apply_projection, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_PRESSURE_PROJ],
                        e.f[SKT_DENSITY], e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ], e.f[SKT_PRESSURE],
                        e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P], new_pressure);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------- microphysics ---
-struct MicroArgs {
-    float *Qv, *Qw, *Qc, *Qr, *Qs, *Qi, *th;
-    const float *rho, *p, *gP;
-    uint8_t* phase;
-    double* region;  // [33 * nblocks] partials or null
-    int rmin[3], rmax[3];
-};
-
-// calculateEnvMicroPhysicsGPU (Game/microPhysics.cu:1035-1417) fused with
-// addHeatGPU (Game/kernelSky.cu:1249-1282): the condens/depos/freeze arrays
-// the reference passes between them are registers here.
-KL void k_microphysics(Geom G, MicroArgs a, float dt, float speed, Gammas gm) {
-    CELL_SETUP;
-    __shared__ double shr[4];
-    float rv[33];
-    bool inRegion = false;
-    if (inb && !isGround(G, x, y, z)) {
-        const long i = gidx(G, x, y, z);
-        const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
-        const float Qw = a.Qw[i] > 1e-18f ? a.Qw[i] : 0.0f;
-        const float Qc = a.Qc[i] > 1e-18f ? a.Qc[i] : 0.0f;
-        const float Qr = a.Qr[i] > 1e-18f ? a.Qr[i] : 0.0f;
-        const float Qs = a.Qs[i] > 1e-18f ? a.Qs[i] : 0.0f;
-        const float Qi = a.Qi[i] > 1e-18f ? a.Qi[i] : 0.0f;
-        const float QwMin = 0.001f, QcMin = 0.001f, QiMin = 0.0006f;
-        const float ps = a.p[i];
-        float3 fv = make_float3(0.0f, 0.0f, 0.0f);
-        const float Dair = a.rho[i];
-        const float tK = a.th[i] * powf(ps / a.gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
-        const float T = tK - 273.15f;
-        if (Qr > 0.0f || Qs > 0.0f || Qi > 0.0f) fv = d_fallVel(Qr, Qs, Qi, Dair, 3, gm.R48, gm.S425, gm.S425);
-        const float QWS = d_ws(T, ps);
-        const float QWI = d_wi(T, ps);
-        const float PT1 = T < 0.0f && Qw + Qc > 0 + 1e-6f ? 1.0f : 0.0f;
-        const float PT2 = T < 0.0f && Qr < 1e-4f && Qs < 1e-4f ? 1.0f : 0.0f;
-        const float PT3 = T < 0.0f && Qr < 1e-4f ? 1.0f : 0.0f;
-        const float sR = d_slope(Dair, Qr, 0), sS = d_slope(Dair, Qs, 1), sI = d_slope(Dair, Qi, 2);
-        const float dts = dt * speed;
-        float PVCON = mp::PVCON(T, ps, Qv, QWS, dt, speed);
-        float PVDEP = mp::PVDEP(T, ps, Qv, QWI, dt, speed);
-        PVDEP = fminf(PVDEP, 0.0f);
-        float PIMLT = mp::PIMLT(T, Qc);
-        float PIDW = mp::PIDW(dt, T, Qc, Qw, Dair, ps);
-        float PIHOM = mp::PIHOM(T, Qw);
-        float PIACR = mp::PIACR(gm, Qr, Qc, Dair, sR);
-        float PRACI = mp::PRACI(gm, Qr, Qc, Dair, sR);
-        float PRAUT = mp::PRAUT(Qw, QwMin, T);
-        float PRACW = mp::PRACW(gm, Qr, Qw, Dair, sR);
-        float PREVP = mp::PREVP(gm, T, Qr, Qv, QWS, Dair, ps, sR, PT1);
-        float PRACS = mp::PRACS(Qr, Qs, PT2, fv, Dair, sS, sR);
-        float PSACW = mp::PSACW(gm, Qs, Qw, Dair, sS);
-        float PSACR = mp::PSACR(Qr, Qs, fv, Dair, sR, sS);
-        float PSACI = mp::PSACI(gm, T, Qs, Qc, Dair, sS);
-        float PSAUT = mp::PSAUT(T, Qc, QcMin);
-        float PSFW = mp::PSFW(dt, T, ps, Qs, Qw, Qc, Qv, QWI, Dair);
-        float PSFI = mp::PSFI(T, ps, Qs, Qc, Qv, QWI);
-        float PSDEP = mp::PSDEP(gm, T, ps, Qv, QWI, Dair, sS);
-        float PSSUB = fmaxf(0.0f, -1 * PSDEP);
-        PSDEP = fmaxf(0.0f, PSDEP);
-        float PSMLT = mp::PSMLT(gm, T, ps, Qs, Qv, Dair, sS, PSACW, PSACR);
-        float PGAUT = mp::PGAUT(T, Qs, QiMin);
-        float PGFR = mp::PGFR(T, Qr, Dair, sR);
-        float PGACW = mp::PGACW(gm, Qi, Qw, sI, Dair);
-        float PGACI = mp::PGACI(gm, Qi, Qc, sI, Dair);
-        float PGACR = mp::PGACR(Qi, Qr, Dair, fv, sR, sI);
-        float PGACS = mp::PGACS(T, Qs, Qi, Dair, fv, sS, sI, false);
-        float PGSUB = mp::PGSUB(gm, T, ps, Qv, QWI, Qi, Dair, sI);
-        float PGMLT = mp::PGMLT(gm, T, ps, Qw, Qr, Qi, Qv, Dair, sI, PGACW, PGACR, dts);
-        float PGDRY = mp::PGDRY(Qw, Qc, Qr, Qs, PGACW, PGACI, PGACR, PGACS, dts);
-        float PGWET = mp::PGWET(gm, T, ps, Qc, Qs, Qi, Qv, fv, Dair, sS, sI, PGACI, PGACS, dts);
-        float PGACR1 = mp::PGACR1(T, Qw, Qc, Qs, Qi, Dair, fv, sS, sI, PGWET, PGACW, PGACI, PGACS, dts);
-        float W = 0.0f;
-        if (PGDRY > PGWET) W = 1.0f;
-        float cond = 0.0f, dep = 0.0f, frz = 0.0f;
-        if (PVCON >= 0) { PVCON = dt * fminf(Qv, speed * PVCON); cond += PVCON; }
-        else if (PVCON < 0) { PVCON = dt * fmaxf(-Qw, speed * PVCON); cond += PVCON; }
-        if (PVDEP >= 0) { PVDEP = dt * fminf(Qv, speed * PVDEP); dep += PVDEP; }
-        else if (PVDEP < 0) { PVDEP = dt * fmaxf(-Qc, speed * PVDEP); dep += PVDEP; }
-        float nQv = a.Qv[i], nQw = a.Qw[i], nQc = a.Qc[i], nQr = a.Qr[i], nQs = a.Qs[i], nQi = a.Qi[i];
-        if (T < 0) {
-            PIDW = dt * fminf(Qw, speed * PIDW); dep += PIDW;
-            PIHOM = dt * fminf(Qw, speed * PIHOM); frz += PIHOM;
-            PIACR = dt * fminf(Qr, speed * PIACR); frz += PIACR;
-            PRACI = dt * fminf(Qc, speed * PRACI);
-            PRAUT = dt * fminf(Qw, speed * PRAUT);
-            PRACW = dt * fminf(Qw, speed * PRACW);
-            PREVP = dt * fminf(Qr, speed * PREVP * (1 - PT1)); cond -= PREVP;
-            PRACS = dt * fminf(Qs, speed * PRACS * (1 - PT2));
-            PSACR = dt * fminf(Qr, speed * PSACR); frz += PSACR;
-            PSACW = dt * fminf(Qw, speed * PSACW); frz += PSACW;
-            PSACI = dt * fminf(Qc, speed * PSACI);
-            PSAUT = dt * fminf(Qc, speed * PSAUT);
-            PSFW = dt * fminf(Qw, speed * PSFW); frz += PSFW;
-            PSFI = dt * fminf(Qc, speed * PSFI);
-            PSDEP = dt * fminf(Qv, speed * PSDEP * (PT1)); dep += PSDEP;
-            PSSUB = dt * fminf(Qs, speed * PSSUB * (1 - PT1)); dep -= PSSUB;
-            PGAUT = dt * fminf(Qs, speed * PGAUT);
-            PGFR = dt * fminf(Qr, speed * PGFR); frz += PGFR;
-            PGACI = dt * fminf(Qc, speed * PGACI);
-            PGACR = dt * fminf(Qr, speed * PGACR * (1 - W)); frz += PGACR;
-            PGACS = dt * fminf(Qs, speed * PGACS);
-            PGSUB = dt * fminf(Qi, speed * PGSUB * (1 - PT1)); dep -= PGSUB;
-            if (W == 1.0f) {
-                if (PGACR1 >= 0) { PGACR1 = dt * fminf(Qr, speed * PGACR1); frz += PGACR1; }
-                else if (PGACR1 < 0) { PGACR1 = dt * fmaxf(-Qi, speed * PGACR1); frz += PGACR1; }
-            }
-            nQv += PSSUB * (1 - PT1) + PGSUB * (1 - PT1) + PREVP * (1 - PT1) - PSDEP * (PT1)-PVCON - PVDEP;
-            nQw += PVCON - PSACW - PSFW - PRAUT - PRACW - PIDW - PIHOM;
-            nQc += PVDEP + PIDW + PIHOM - PSAUT - PSACI - PRACI - PSFI - PGACI;
-            nQr += PRAUT + PRACW - PIACR - PSACR - PGACR * (1 - W) - PGACR1 * (W)-PGFR - PREVP * (1 - PT1);
-            nQs += PSAUT + PSACI + PSACW + PSFW + PSFI + PRACI * (PT3) + PIACR * (PT3) - PGACS - PGAUT -
-                   PRACS * (1 - PT2) + PSACR * (PT2) - PSSUB * (1 - PT1) + PSDEP * (PT1);
-            nQi += PGAUT + PGFR + PGDRY * (1 - W) + PGWET * (W) + PSACR * (1 - PT2) + PRACS * (1 - PT2) +
-                   PRACI * (1 - PT3) + PIACR * (1 - PT3) - PGSUB * (1 - PT1);
-        } else {
-            PIMLT = dt * fminf(Qc, speed * PIMLT); frz -= PIMLT;
-            PIACR = 0.0f;
-            PRAUT = dt * fminf(Qw, speed * PRAUT);
-            PRACW = dt * fminf(Qw, speed * PRACW);
-            PREVP = dt * fminf(Qr, speed * PREVP * (1 - PT1)); cond -= PREVP;
-            PSACW = dt * fminf(Qw, speed * PSACW);
-            PSMLT = dt * fminf(Qs, speed * PSMLT); frz -= PSMLT;
-            PGACW = dt * fminf(Qw, speed * PGACW); frz += PGACW;
-            PGACS = dt * fminf(Qs, speed * PGACS);
-            PGMLT = dt * fminf(Qi, speed * PGMLT); frz -= PGMLT;
-            nQv += PREVP * (1 - PT1) - PVCON - PVDEP;
-            nQw += PVCON + PIMLT - PRAUT - PRACW - PGACW - PSACW;
-            nQc += PVDEP - PIMLT;
-            nQr += PRAUT + PRACW + PSACW + PGACW + PGMLT + PSMLT - PREVP * (1 - PT1);
-            nQs += -PSMLT - PGACS;
-            nQi += -PGMLT + PGACS;
-        }
-        a.Qv[i] = nQv; a.Qw[i] = nQw; a.Qc[i] = nQc; a.Qr[i] = nQr; a.Qs[i] = nQs; a.Qi[i] = nQi;
-        // addHeatGPU with the updated vapour
-        const float Mair = 0.02896f, Mwater = 0.01802f;
-        const float XV = (nQv / Mwater) / ((nQv / Mwater) + (1 - nQv) / Mair);
-        const float Mth = XV * Mwater + (1 - XV) * Mair;
-        const float YV = XV * (Mwater / Mth);
-        const float yth = YV * 1.33f + (1 - YV) * 1.4f;
-        const float cpth = yth * kc::R / (Mth * (yth - 1));
-        const float th = a.th[i];
-        float sum = 0.0f;
-        sum += d_Lwater(th - 273.15f) / cpth * cond;
-        sum += d_Lice(th - 273.15f) / cpth * dep;
-        sum += kc::Lf / cpth * frz;
-        a.th[i] = th + sum;
-        if (a.phase)
-            a.phase[i] = (uint8_t)(1 | (T < 0.0f ? 2 : 0) | (PT1 != 0.0f ? 4 : 0) | (PT2 != 0.0f ? 8 : 0) |
-                                   (PT3 != 0.0f ? 16 : 0) | (W != 0.0f ? 32 : 0));
-        if (a.region && x >= a.rmin[0] && x <= a.rmax[0] && y >= a.rmin[1] && y <= a.rmax[1] && z >= a.rmin[2] &&
-            z <= a.rmax[2]) {
-            inRegion = true;
-            float PVVAP = 0.0f, PVSUB = 0.0f, pc = PVCON, pd = PVDEP;
-            if (pc < 0.0f) { PVVAP = -pc; pc = 0.0f; }
-            if (pd < 0.0f) { PVSUB = -pd; pd = 0.0f; }
-            const float vals[33] = {pc, pd, PIMLT, PIDW, PIHOM, PIACR, PRACI, PRAUT, PRACW, PREVP, PRACS,
-                                    PSACW, PSACR, PSACI, PSAUT, PSFW, PSFI, PSDEP, PSSUB, PSMLT, PGAUT, PGFR,
-                                    PGACW, PGACI, PGACR, PGDRY * (1 - W), PGACS, PGSUB, PGMLT, PGWET * W, PGACR1, PVVAP, PVSUB};
-            for (int k = 0; k < 33; k++) rv[k] = vals[k];
-        }
-    } else if (inb) {
-        if (a.phase) a.phase[gidx(G, x, y, z)] = 0;
-    }
-    if (a.region) {
-        const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-        for (int k = 0; k < 33; k++) {
-            const double s = blockSum(inRegion ? (double)rv[k] : 0.0, shr);
-            if (tid256() == 0) a.region[(long)k * nb + bid] = s;
-        }
-    }
-}
-hipError_t launch_microphysics(EnvState& e, float dt, float speed) {
-    MicroArgs a{e.f[SKT_QV], e.f[SKT_QW], e.f[SKT_QC], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_POTTEMP],
-                e.f[SKT_DENSITY], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], e.phase,
-                e.region_active ? e.region : nullptr, {e.rmin[0], e.rmin[1], e.rmin[2]},
-                {e.rmax[0], e.rmax[1], e.rmax[2]}};
-    hipLaunchKernelGGL(k_microphysics, grid3(e), B3, 0, e.stream, geom(e), a, dt, speed, e.gm);
-    return hipGetLastError();
-}
-// Fixed-order sum of the region partials into 33 floats (written to sc->pad area
-// is not large enough, so into the tail of the region buffer).
-KL void k_region_finalize(double* region, unsigned nb, float* out) {
-    __shared__ double sh[4];
-    for (int k = 0; k < 33; k++) {
-        double v = 0.0;
-        for (unsigned j = tid256(); j < nb; j += 256) v += region[(long)k * nb + j];
-        v = blockSum(v, sh);
-        if (tid256() == 0) out[k] = (float)v;
-    }
-}
-hipError_t launch_region_finalize(EnvState& e) {
-    const unsigned nb = (unsigned)blocks3d(e);
-    float* out = (float*)(e.region + 33L * nb);
-    hipLaunchKernelGGL(k_region_finalize, dim3(1), dim3(64, 4, 1), 0, e.stream, e.region, nb, out);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------ masks / editor ---
-// computeNeighbourGPU, Game/kernelSky.cu:1284-1316 (12 bytes per cell).
-KL void k_masks(Geom G, uint8_t* out) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb) return;
-    uint8_t* m = out + 12 * gidx(G, x, y, z);
-    const bool cg = isGround(G, x, y, z);
-    bool o;
-    o = x == 0; m[0] = o; m[1] = (o && cg) || (!o && isGround(G, x - 1, y, z));
-    o = x == G.X - 1; m[2] = o; m[3] = (o && cg) || (!o && isGround(G, x + 1, y, z));
-    o = y == G.Y - 1; m[4] = o; m[5] = !o && isGround(G, x, y + 1, z);
-    o = y == 0; m[6] = o; m[7] = (o && cg) || (!o && isGround(G, x, y - 1, z));
-    o = z == G.Z - 1; m[8] = o; m[9] = (o && cg) || (!o && isGround(G, x, y, z + 1));
-    o = z == 0; m[10] = o; m[11] = (o && cg) || (!o && isGround(G, x, y, z - 1));
-}
-hipError_t launch_masks(EnvState& e, uint8_t* dev_out) {
-    hipLaunchKernelGGL(k_masks, grid3(e), B3, 0, e.stream, geom(e), dev_out);
-    return hipGetLastError();
-}
-struct Ten {
-    float* a[10];
-};
-// resetValueInGround over n fields (Game/kernelGround.cu:21-36)
-KL void k_reset_ground(Geom G, Ten t, int n) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || !isGround(G, x, y, z)) return;
-    const long i = gidx(G, x, y, z);
-    for (int k = 0; k < n; k++) t.a[k][i] = 0.0f;
-}
-hipError_t launch_editor_reset(EnvState& e, int nfields) {
-    Ten t{{e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_QW], e.f[SKT_QC], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI],
-           e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ]}};
-    hipLaunchKernelGGL(k_reset_ground, grid3(e), B3, 0, e.stream, geom(e), t, nfields);
-    return hipGetLastError();
-}
-// computeIsenTempGroundGPU, Game/kernelGround.cu:38-47
-KL void k_isen_temp_ground(Geom G, float* gT, const float* isen, const float* gP, const float* p) {
-    const long g = blockIdx.x * 256L + threadIdx.x;
-    if (g >= (long)G.X * G.Z) return;
-    const int x = (int)(g % G.X), zc = (int)(g / G.X);
-    const int GH = G.gh[g] + 1 >= G.Y ? G.Y - 1 : G.gh[g] + 1;
-    const float T = d_potTemp(isen[GH - 1] - 273.15f, gP[g], p[gidx(G, x, GH, zc)]);
-    gT[g] = T + 273.15f;
-}
-hipError_t launch_isen_temp_ground(EnvState& e) {
-    hipLaunchKernelGGL(k_isen_temp_ground, grid1(e.G), dim3(256), 0, e.stream, geom(e), e.f[SKT_GROUND_T],
-                       e.f[SKT_ISEN_TEMP], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
-    return hipGetLastError();
-}
-// Default profiles = the (x=0, z=0) column before the init noise
-// (cudaMemcpy2D of Game/environment.cu:299-303).
-KL void k_profiles(Geom G, const float* th, const float* qv, const float* u, const float* w, float* iT, float* iQ,
-                   float* dU, float* dW) {
-    const int y = blockIdx.x * 256 + threadIdx.x;
-    if (y >= G.Y) return;
-    const long i = (long)G.X * y;
-    iT[y] = th[i];
-    iQ[y] = qv[i];
-    dU[y] = u[i];
-    dW[y] = w[i];
-}
-hipError_t launch_profiles_from_column(EnvState& e) {
-    hipLaunchKernelGGL(k_profiles, grid1(e.Y), dim3(256), 0, e.stream, geom(e), e.f[SKT_POTTEMP], e.f[SKT_QV],
-                       e.f[SKT_VELX], e.f[SKT_VELZ], e.f[SKT_ISEN_TEMP], e.f[SKT_ISEN_VAPOR], e.f[SKT_DEFAULT_VELX],
-                       e.f[SKT_DEFAULT_VELZ]);
-    return hipGetLastError();
-}
-// randomArray (Game/kernelMaths.cu:93-116) with the seed every thread intends
-// to read (quirk A10); the hash is the published PCG-RXS-M-XS function.
-KL void k_random(float* a, long n, float mn, float mx, uint32_t seed) {
-    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-        const uint32_t sd = seed + (uint32_t)i;
-        const uint32_t s = sd * 747796405u + 2891336453u;
-        const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
-        const uint32_t h = (w >> 22u) ^ w;
-        const float rnd = float(h % 1000u) / float(1000);
-        a[i] += rnd * (mx - mn) + mn;
-    }
-}
-// Plain streaming copy (float4 loads/stores): the measured bandwidth ceiling.
-KL void k_stream_copy(const float4* __restrict__ s, float4* __restrict__ d, long n4) {
-    for (long i = blockIdx.x * 256L + threadIdx.x; i < n4; i += (long)gridDim.x * 256) d[i] = s[i];
-}
-hipError_t launch_stream_copy(EnvState& e, const float* src, float* dst, long n) {
-    const long n4 = n / 4;
-    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)std::min<long>((n4 + 255) / 256, 8192)), dim3(256), 0, e.stream,
-                       (const float4*)src, (float4*)dst, n4);
-    return hipGetLastError();
-}
-hipError_t launch_random(EnvState& e, float* a, float mn, float mx, uint32_t seed) {
-    hipLaunchKernelGGL(k_random, dim3((unsigned)std::min<long>((e.N + 255) / 256, 65536)), dim3(256), 0, e.stream, a,
-                       e.N, mn, mx, seed);
     return hipGetLastError();
 }
 

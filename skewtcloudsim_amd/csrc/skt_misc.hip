@@ -1,0 +1,667 @@
+// HIP kernels of the column-local and pointwise phases of the skewT cloud
+// step for gfx950: ground phase, precipitation fall, buoyancy, equation of
+// state, boundary velocity reset, bulk microphysics, masks and init helpers.
+// Same layout and launch conventions as skt_kernels.hip.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "skt_internal.h"
+#include "skt_micro.h"
+#include "skt_reduce.h"
+
+namespace skt {
+
+// ---------------------------------------------------------- ground phase ---
+// advectGroundWaterGPU, Game/kernelSky.cu:120-253, reading a pre-step snapshot
+// of both stores (the reference's in-place tile update leaves its interior
+// halo unfilled; every column reading its neighbours' old values is intended).
+KL void k_ground_water(Geom G, const float* __restrict__ rs0, const float* __restrict__ gr0, float* __restrict__ rs,
+                       float* __restrict__ gr, float dt, float speed, float dx) {
+    const long g = blockIdx.x * 256L + threadIdx.x;
+    if (g >= (long)G.X * G.Z) return;
+    const int x = (int)(g % G.X), zc = (int)(g / G.X);
+    const long iL = x > 0 ? g - 1 : g, iR = x < G.X - 1 ? g + 1 : g;
+    const long iB = zc > 0 ? g - G.X : g, iF = zc < G.Z - 1 ? g + G.X : g;
+    const float n = 1 / 0.03f;
+    const float kk = 5e-5f;
+    const float diffR = 1 / (dx * dx) * 1.75e-3f;
+    const float diffW = 1 / (dx * dx) * 1.8e-5f;
+    const float grC = gr0[g], rsC = rs0[g];
+    float fGr = grC, fRs = rsC;
+    const float hC = float(G.gh[g]);
+    const long nb[4] = {iL, iR, iB, iF};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const long j = nb[k];
+        const float hFrom = float(G.gh[j]), grFrom = gr0[j], rsFrom = rs0[j];
+        const float slope = hFrom - hC / 1.0f;
+        const bool fromPrev = slope > 0.0f;
+        const float sRain = fromPrev ? grFrom : grC;
+        const float sWater = fromPrev ? rsFrom : rsC;
+        float changeRain = 0.0f;
+        if (sRain != 0.0f) {
+            const float sp = n * powf(sRain, 2.0f / 3.0f);
+            const float vel = sp * powf(slope, 0.5f);
+            changeRain = vel * sRain;
+        }
+        const float changeWater = kc::g * kk * sWater * slope;
+        const float lapR = (grFrom - grC) * diffR;
+        const float lapSR = (rsFrom - rsC) * diffW;
+        const bool fpR = lapR > 0.0f, fpSR = lapSR > 0.0f;
+        const float fgr = dt * fminf(fabsf(lapR) * speed, fpR ? grFrom : grC) * (lapR > 0.0f ? 1.0f : -1.0f);
+        const float fgw = dt * fminf(fabsf(lapSR) * speed, fpSR ? rsFrom : rsC) * (lapSR > 0.0f ? 1.0f : -1.0f);
+        const float sfr = dt * fminf(fabsf(changeRain) * speed, fromPrev ? grFrom : grC) * (changeRain > 0.0f ? 1.0f : -1.0f);
+        const float sfw = dt * fminf(fabsf(changeWater) * speed, fromPrev ? rsFrom : rsC) * (changeWater > 0.0f ? 1.0f : -1.0f);
+        fGr += fgr + sfr;
+        fRs += fgw + sfw;
+    }
+    gr[g] = fGr;
+    rs[g] = fRs;
+}
+hipError_t launch_ground_water_snapshot(EnvState& e, float dt, float speed) {
+    float* rs0 = e.work[0];
+    float* gr0 = e.work[0] + e.G;
+    hipError_t err = hipMemcpyAsync(rs0, e.f[SKT_QRS], e.G * sizeof(float), hipMemcpyDeviceToDevice, e.stream);
+    if (err != hipSuccess) return err;
+    err = hipMemcpyAsync(gr0, e.f[SKT_QGR], e.G * sizeof(float), hipMemcpyDeviceToDevice, e.stream);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_ground_water, grid1(e.G), dim3(256), 0, e.stream, geom(e), rs0, gr0, e.f[SKT_QRS],
+                       e.f[SKT_QGR], dt, speed, e.dx);
+    return hipGetLastError();
+}
+
+struct GroundArgs {
+    float *Qv, *Qr, *Qs, *Qi, *Qc, *Qw, *th, *u, *p, *rho;
+    float *Qrs, *Qgr, *Qgs, *Qgi, *gP, *gt, *gT, *cover;
+};
+
+// One thread per ground column: the column-local part of the ground phase of
+// updateGPU (Game/environment.cu:427-455,469) in the reference's order.
+KL void k_ground_columns(Geom G, GroundArgs a, int stages, float dt, float speed, float irr, float dx, Gammas gm) {
+    const long g = blockIdx.x * 256L + threadIdx.x;
+    if (g >= (long)G.X * G.Z) return;
+    const int x = (int)(g % G.X), zc = (int)(g / G.X);
+    const float kap = kc::Rsd / kc::Cpd;
+    const int gh = G.gh[g];
+    if (stages & 1) {  // calculateCloudCoverGPU, Game/kernelSky.cu:1034-1044
+        float tot = 0.0f;
+        for (int y = gh + 1; y < G.Y; y++) {
+            const long i = gidx(G, x, y, zc);
+            tot += (a.Qc[i] + a.Qw[i]) * dx;
+        }
+        a.cover[g] = fminf(tot / 1.2f, 1.0f);
+    }
+    if (stages & 2) {  // calculateGroundTempGPU, Game/kernelSky.cu:1046-1059 (T^4 widened to double)
+        const float gtv = a.gT[g];
+        const double T4 = gtv * gtv * gtv * gtv;
+        a.gT[g] += (dt * speed) * ((1 - a.cover[g]) * (((1 - 0.25f) * irr - kc::ge * kc::oo * T4) / (1.0f * 1500.0f * kc::Cpds)));
+    }
+    if ((stages & (4 | 8 | 16)) && gh + 1 < G.Y) {
+        const long i = gidx(G, x, gh + 1, zc);
+        if (stages & 4) {  // calculatePrecipHittingGroundMicroPhysicsGPU, Game/microPhysics.cu:1544-1636
+            const float Qgr = a.Qgr[g] > 1e-18f ? a.Qgr[g] : 0.0f;
+            const float Qgs = a.Qgs[g] > 1e-18f ? a.Qgs[g] : 0.0f;
+            const float Qgi = a.Qgi[g] > 1e-18f ? a.Qgi[g] : 0.0f;
+            const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
+            const float Qr = a.Qr[i] > 1e-18f ? a.Qr[i] : 0.0f;
+            const float Qs = a.Qs[i] > 1e-18f ? a.Qs[i] : 0.0f;
+            const float Qi = a.Qi[i] > 1e-18f ? a.Qi[i] : 0.0f;
+            float freeze = 0.0f;
+            const float tGC = a.gT[g] - 273.15f;
+            const float ps = a.p[i];
+            const float wind = a.u[i];
+            float3 fv = make_float3(0.0f, 0.0f, 0.0f);
+            const float Dair = a.rho[i];
+            const float tAK = a.th[i] * powf(ps / a.gP[g], kap);
+            if (Qgr > 0.0f || Qgs > 0.0f || Qgi > 0.0f) fv = d_fallVel(Qgr, Qgs, Qgi, Dair, 3, gm.R48, gm.S425, gm.S425);
+            const float GR = dt * fminf(Qr * speed, Qr);
+            const float GS = dt * fminf(Qs * speed, Qs);
+            const float GI = dt * fminf(Qi * speed, Qi);
+            float PGRFR = mp::PGRFR(tAK, tGC, ps, Qr, Qv, Qi, Qgr, Qgs, Dair, wind, fv);
+            if (tAK < 273.15f) {
+                PGRFR = dt * fminf(Qr, speed * PGRFR);
+                freeze += PGRFR;
+            }
+            a.Qr[i] += -GR;
+            a.Qs[i] += -GS;
+            a.Qi[i] += -GI;
+            a.Qgr[g] += GR - PGRFR;
+            a.Qgs[g] += GS;
+            a.Qgi[g] += GI + PGRFR;
+            const float Mair = 0.02896f, Mwater = 0.01802f;
+            const float XV = (Qv / Mwater) / ((Qv / Mwater) + (1 - Qv) / Mair);
+            const float Mth = XV * Mwater + (1 - XV) * Mair;
+            const float YV = XV * (Mwater / Mth);
+            const float yth = YV * 1.33f + (1 - YV) * 1.4f;
+            const float cpth = kc::R / (Mth * (yth - 1));
+            float sum = 0.0f;
+            sum += kc::Lf / cpth * freeze;
+            a.gT[g] += sum;
+        }
+        if (stages & 8) {  // calculateGroundMicroPhysicsGPU, Game/microPhysics.cu:1419-1541
+            const float Qrs = a.Qrs[g] > 1e-18f ? a.Qrs[g] : 0.0f;
+            const float Qgr = a.Qgr[g] > 1e-18f ? a.Qgr[g] : 0.0f;
+            const float Qgs = a.Qgs[g] > 1e-18f ? a.Qgs[g] : 0.0f;
+            const float Qgi = a.Qgi[g] > 1e-18f ? a.Qgi[g] : 0.0f;
+            const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
+            float condens = 0.0f, freeze = 0.0f, depos = 0.0f;
+            const float tGC = a.gT[g] - 273.15f;
+            const float ps = a.p[i];
+            const float wind = a.u[i];
+            const float cc = a.cover[g];
+            const float Dair = a.rho[i];
+            float tme = a.gt[g];
+            const float tAK = a.th[i] * powf(ps / a.gP[g], kap);
+            const float sI = d_slope(Dair, Qgi, 2);
+            float PGREVP = mp::PGREVP(tAK, tGC, ps, Qv, Qgr, irr, Dair, wind);
+            float PGSMLT = mp::PGSMLT(tAK, tGC, ps, Qgs, Qv, Qgr, irr, cc, Dair, wind, dt);
+            float PGGMLT = mp::PGGMLT(tGC, Qgi, Dair, sI);
+            float PGGFR = mp::PGGFR(tAK, tGC, ps, Qgr, Qv, Qgi, Dair, irr, cc, wind, dt);
+            float PGSFR = mp::PGSFR(tAK, tGC, ps, Qv, Qgr, Qgs, Dair, wind, dt);
+            float PGFLW = mp::PGFLW(Qgr);
+            float PGDEVP = mp::PGDEVP(&tme, Qgr, Qgs, Qgi, Qrs);
+            a.gt[g] = tme;
+            PGREVP = dt * fminf(Qgr, speed * PGREVP); condens -= PGREVP;
+            PGSMLT = dt * fminf(Qgs, speed * PGSMLT); freeze -= PGSMLT;
+            PGGMLT = dt * fminf(Qgi, speed * PGGMLT); freeze -= PGGMLT;
+            PGGFR = dt * fminf(Qgr, speed * PGGFR); freeze += PGGFR;
+            PGSFR = dt * fminf(Qgr, speed * PGSFR); freeze += PGSFR;
+            PGFLW = dt * fminf(Qgr, speed * PGFLW);
+            PGDEVP = dt * fminf(Qrs, speed * PGDEVP); condens -= PGDEVP;
+            a.Qrs[g] += PGFLW - PGDEVP;
+            a.Qv[i] += PGREVP + PGDEVP;
+            a.Qgr[g] += PGSMLT + PGGMLT - PGREVP - PGGFR - PGSFR - PGFLW;
+            a.Qgs[g] += PGSFR - PGSMLT;
+            a.Qgi[g] += PGGFR - PGGMLT;
+            const float cps = 1500.0f, rhos = 1500.0f, dl = 0.1f;
+            const float ms = rhos * dl;
+            float sum = 0.0f;
+            sum += d_Lwater(tGC) / (ms * cps) * condens;
+            sum += d_Lice(tGC) / (ms * cps) * depos;
+            sum += kc::Lf / (ms * cps) * freeze;
+            a.gT[g] += sum;
+        }
+        if ((stages & 16) && gh + 1 < G.Y - 1) {  // setTempsAtGroundGPU, Game/kernelSky.cu:255-268
+            const float T = d_potTemp(a.gT[g] - 273.15f, a.p[i], a.gP[g]) + 273.15f;
+            const float d2 = a.th[i] - T;
+            a.th[i] -= d2 * fminf(1.0f, dt * speed);
+        }
+    }
+}
+hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed, float irr) {
+    GroundArgs a{e.f[SKT_QV], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_QC], e.f[SKT_QW], e.f[SKT_POTTEMP],
+                 e.f[SKT_VELX], e.f[SKT_PRESSURE], e.f[SKT_DENSITY], e.f[SKT_QRS], e.f[SKT_QGR], e.f[SKT_QGS],
+                 e.f[SKT_QGI], e.f[SKT_GROUND_P], e.f[SKT_GROUND_TIME], e.f[SKT_GROUND_T], e.f[SKT_CLOUD_COVER]};
+    hipLaunchKernelGGL(k_ground_columns, grid1(e.G), dim3(256), 0, e.stream, geom(e), a, stages, dt, speed, irr, e.dx,
+                       e.gm);
+    return hipGetLastError();
+}
+
+// advectPrecipGPU, Game/kernelSky.cu:576-657: one thread per column walks y
+// upwards; each level reads its own and the next level's pre-step value
+// (the column snapshot of the reference), so the update can be in place.
+KL void k_precip(Geom G, float* q, const float* __restrict__ th, const float* __restrict__ p,
+                 const float* __restrict__ qv, const float* __restrict__ gP, int type, float dts, float dx, Gammas gm) {
+    const long g = blockIdx.x * 256L + threadIdx.x;
+    if (g >= (long)G.X * G.Z) return;
+    const int x = (int)(g % G.X), zc = (int)(g / G.X);
+    const float kap = kc::Rsd / kc::Cpd;
+    const float gp = gP[g];
+    auto fvAt = [&](long j, float Qj) {
+        const float T = th[j] * powf(p[j] / gp, kap);
+        const float Tv = T * (0.608f * qv[j] + 1);
+        const float den = p[j] * 100 / (kc::Rsd * Tv);
+        const float3 v3 = d_fallVel(Qj, Qj, Qj, den, type, gm.R48, gm.S425, gm.S425);
+        return type == 0 ? v3.x : (type == 1 ? v3.y : v3.z);
+    };
+    int y0 = G.gh[g] + 1;
+    if (y0 < 0) y0 = 0;
+    if (y0 >= G.Y) return;
+    long i = gidx(G, x, y0, zc);
+    float Qc = q[i];
+    float fv = fvAt(i, Qc);
+    for (int y = y0; y < G.Y; y++) {
+        const bool up = y < G.Y - 1;
+        float Qu = 0.0f, fvu = 0.0f;
+        const long iu = i + G.X;
+        if (up) {
+            Qu = q[iu];
+            fvu = fvAt(iu, Qu);
+        }
+        float nq = Qc;
+        nq -= dts * fminf((fv / dx) * Qc, Qc);
+        if (up) nq += dts * fminf((fvu / dx) * Qu, Qu);
+        q[i] = nq;
+        Qc = Qu;
+        fv = fvu;
+        i = iu;
+    }
+}
+hipError_t launch_precip(EnvState& e, float* q, int type, float dts) {
+    hipLaunchKernelGGL(k_precip, grid1(e.G), dim3(256), 0, e.stream, geom(e), q, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE],
+                       e.f[SKT_QV], e.f[SKT_GROUND_P], type, dts, e.dx, e.gm);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------------- buoyancy ---
+// buoyancyGPU, Game/kernelSky.cu:1061-1247 (bounds {D,D,D}); the ice test
+// 'T < 0' is on Kelvin and never holds (quirk A17).
+KL void k_buoyancy(Geom G, float* __restrict__ v, const float* __restrict__ th, const float* __restrict__ qv,
+                   const float* __restrict__ Qr, const float* __restrict__ Qs, const float* __restrict__ Qi,
+                   const float* __restrict__ p, const float* __restrict__ gP, const float* __restrict__ dT,
+                   const float* __restrict__ dQ, float dt) {
+    CELL_SETUP;
+    (void)bid;
+    if (!inb || isGround(G, x, y, z)) return;
+    const Bounds b{BC_DIRICHLET, BC_DIRICHLET, BC_DIRICHLET};
+    const float kap = kc::Rsd / kc::Cpd;
+    const long i = gidx(G, x, y, z);
+    const float cur = v[i];
+    const bool up = y < G.Y - 1;
+    const bool down = y > 0 && !isGround(G, x, y - 1, z);
+    const float gps = gP[x + (long)G.X * z];
+    const float psC = p[i];
+    const float psU = up ? p[i + G.X] : psC;
+    const float psD = down ? p[i - G.X] : psC;
+    const bool l = skyAt(G, x - 1, y, z), r = skyAt(G, x + 1, y, z);
+    const bool bk = skyAt(G, x, y, z - 1), fw = skyAt(G, x, y, z + 1);
+    const int validEnv = int(l) + int(r) + int(bk) + int(fw);
+    if (validEnv == 0) return;
+    const float QP = qv[i];
+    const float Qenv = (extv(G, qv, dQ, b, x + 1, y, z) + extv(G, qv, dQ, b, x - 1, y, z) + extv(G, qv, dQ, b, x, y, z + 1) +
+                        extv(G, qv, dQ, b, x, y, z - 1)) / validEnv;
+    const float QenvUp = extv(G, qv, dQ, b, x, y + 1, z);
+    const float QenvDown = extv(G, qv, dQ, b, x, y - 1, z);
+    const float thD = extv(G, th, dT, b, x, y - 1, z), thU = extv(G, th, dT, b, x, y + 1, z);
+    const float TDown = thD * powf(psD / gps, kap);
+    const float T = th[i] * powf(psC / gps, kap);
+    const float TUp = thU * powf(psU / gps, kap);
+    float Tad = th[i], TadU = th[i], TadD = th[i];
+    const float Tenv = (extv(G, th, dT, b, x + 1, y, z) + extv(G, th, dT, b, x - 1, y, z) + extv(G, th, dT, b, x, y, z + 1) +
+                        extv(G, th, dT, b, x, y, z - 1)) / validEnv;
+    const float TenvUp = thU, TenvDown = thD;
+    if (down) {
+        if ((T < 0.0f && QP >= d_wi((TDown - 273.15f), psD)) || QP >= d_ws((TDown - 273.15f), psD)) {
+            TadD = T - d_MLR(T - 273.15f, psC) * (psC - psD);
+            TadD = TadD * powf(gps / psD, kap);
+        }
+    }
+    if (up) {
+        if ((T < 0.0f && QP >= d_wi((TUp - 273.15f), psU)) || QP >= d_ws((TUp - 273.15f), psU)) {
+            TadU = d_MLR(T - 273.15f, psC) * (psU - psC) + T;
+            TadU = TadU * powf(gps / psU, kap);
+        }
+    }
+    float B = 0.0f, BU = 0.0f, BD = 0.0f;
+    const float totalQ = Qr[i] + Qs[i] + Qi[i];
+    if (down) {
+        const float VTP = TadD * (1.0f + 0.608f * QP), VTE = TenvDown * (1.0f + 0.608f * QenvDown);
+        BD = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
+    }
+    {
+        const float VTP = Tad * (1.0f + 0.608f * QP), VTE = Tenv * (1.0f + 0.608f * Qenv);
+        B = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
+    }
+    if (up) {
+        const float VTP = TadU * (1.0f + 0.608f * QP), VTE = TenvUp * (1.0f + 0.608f * QenvUp);
+        BU = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
+    }
+    float bf = B;
+    if (cur > 0.0f) bf = BU;
+    else if (cur < 0.0f) bf = BD;
+    float change = bf * dt;
+    if ((up && cur > 0.0f && -change > cur) || (down && cur < 0.0f && -change < cur)) change = -cur;
+    v[i] = cur + change;
+}
+hipError_t launch_buoyancy(EnvState& e, float dt) {
+    hipLaunchKernelGGL(k_buoyancy, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_VELY], e.f[SKT_POTTEMP], e.f[SKT_QV],
+                       e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], e.f[SKT_ISEN_TEMP],
+                       e.f[SKT_ISEN_VAPOR], dt);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------- EOS ---
+// initDensity, Game/kernelSky.cu:1432-1452 (+ the old-density copy)
+KL void k_init_density(Geom G, float* __restrict__ rho, float* __restrict__ rho_old, const float* __restrict__ th,
+                       const float* __restrict__ p, const float* __restrict__ qv, const float* __restrict__ gP) {
+    CELL_SETUP;
+    (void)bid;
+    if (!inb || isGround(G, x, y, z)) return;
+    const long i = gidx(G, x, y, z);
+    const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+    const float Tv = T * (0.608f * qv[i] + 1);
+    const float d = p[i] * 100 / (kc::Rsd * Tv);
+    rho[i] = d;
+    if (rho_old) rho_old[i] = d;
+}
+hipError_t launch_init_density(EnvState& e, int copy_old) {
+    hipLaunchKernelGGL(k_init_density, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_DENSITY],
+                       copy_old ? e.f[SKT_DENSITY_OLD] : nullptr, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE], e.f[SKT_QV],
+                       e.f[SKT_GROUND_P]);
+    return hipGetLastError();
+}
+// calculateNewPressure, Game/kernelSky.cu:1454-1474
+KL void k_new_pressure(Geom G, float* __restrict__ p, const float* __restrict__ rho, const float* __restrict__ th,
+                       const float* __restrict__ qv, const float* __restrict__ gP) {
+    CELL_SETUP;
+    (void)bid;
+    if (!inb || isGround(G, x, y, z)) return;
+    const long i = gidx(G, x, y, z);
+    const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+    const float Tv = T * (0.608f * qv[i] + 1);
+    p[i] = rho[i] * kc::Rsd * Tv / 100.0f;
+}
+hipError_t launch_new_pressure(EnvState& e) {
+    hipLaunchKernelGGL(k_new_pressure, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_PRESSURE], e.f[SKT_DENSITY],
+                       e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P]);
+    return hipGetLastError();
+}
+
+// resetVelPressProj, Game/kernelSky.cu:1648-1682: one thread per (x,y) column;
+// the column stops at its first ground cell (quirk A16).
+KL void k_reset_vel(Geom G, float* __restrict__ u, float* __restrict__ v, float* __restrict__ w,
+                    const float* __restrict__ defU, const float* __restrict__ defW) {
+    const long c = blockIdx.x * 256L + threadIdx.x;
+    if (c >= (long)G.X * G.Y) return;
+    const int x = (int)(c % G.X), y = (int)(c / G.X);
+    for (int z = 0; z < G.Z; z++) {
+        if (isGround(G, x, y, z)) break;
+        const long i = gidx(G, x, y, z);
+        if (y == G.Y - 1 || isGround(G, x, y + 1, z)) v[i] = 0.0f;
+        if (x == G.X - 1) u[i] = defU[y];
+        if (x < G.X - 1 && isGround(G, x + 1, y, z)) u[i] = 0.0f;
+        if (z == G.Z - 1) w[i] = defW[y];
+        if (z < G.Z - 1 && isGround(G, x, y, z + 1)) w[i] = 0.0f;
+    }
+}
+hipError_t launch_reset_vel(EnvState& e) {
+    hipLaunchKernelGGL(k_reset_vel, grid1((long)e.X * e.Y), dim3(256), 0, e.stream, geom(e), e.f[SKT_VELX],
+                       e.f[SKT_VELY], e.f[SKT_VELZ], e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ]);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------- microphysics ---
+struct MicroArgs {
+    float *Qv, *Qw, *Qc, *Qr, *Qs, *Qi, *th;
+    const float *rho, *p, *gP;
+    uint8_t* phase;
+    double* region;  // [33 * nblocks] partials or null
+    int rmin[3], rmax[3];
+};
+
+// calculateEnvMicroPhysicsGPU (Game/microPhysics.cu:1035-1417) fused with
+// addHeatGPU (Game/kernelSky.cu:1249-1282): the condens/depos/freeze arrays
+// the reference passes between them are registers here.
+KL void k_microphysics(Geom G, MicroArgs a, float dt, float speed, Gammas gm) {
+    CELL_SETUP;
+    __shared__ double shr[4];
+    float rv[33];
+    bool inRegion = false;
+    if (inb && !isGround(G, x, y, z)) {
+        const long i = gidx(G, x, y, z);
+        const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
+        const float Qw = a.Qw[i] > 1e-18f ? a.Qw[i] : 0.0f;
+        const float Qc = a.Qc[i] > 1e-18f ? a.Qc[i] : 0.0f;
+        const float Qr = a.Qr[i] > 1e-18f ? a.Qr[i] : 0.0f;
+        const float Qs = a.Qs[i] > 1e-18f ? a.Qs[i] : 0.0f;
+        const float Qi = a.Qi[i] > 1e-18f ? a.Qi[i] : 0.0f;
+        const float QwMin = 0.001f, QcMin = 0.001f, QiMin = 0.0006f;
+        const float ps = a.p[i];
+        float3 fv = make_float3(0.0f, 0.0f, 0.0f);
+        const float Dair = a.rho[i];
+        const float tK = a.th[i] * powf(ps / a.gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+        const float T = tK - 273.15f;
+        if (Qr > 0.0f || Qs > 0.0f || Qi > 0.0f) fv = d_fallVel(Qr, Qs, Qi, Dair, 3, gm.R48, gm.S425, gm.S425);
+        const float QWS = d_ws(T, ps);
+        const float QWI = d_wi(T, ps);
+        const float PT1 = T < 0.0f && Qw + Qc > 0 + 1e-6f ? 1.0f : 0.0f;
+        const float PT2 = T < 0.0f && Qr < 1e-4f && Qs < 1e-4f ? 1.0f : 0.0f;
+        const float PT3 = T < 0.0f && Qr < 1e-4f ? 1.0f : 0.0f;
+        const float sR = d_slope(Dair, Qr, 0), sS = d_slope(Dair, Qs, 1), sI = d_slope(Dair, Qi, 2);
+        const float dts = dt * speed;
+        float PVCON = mp::PVCON(T, ps, Qv, QWS, dt, speed);
+        float PVDEP = mp::PVDEP(T, ps, Qv, QWI, dt, speed);
+        PVDEP = fminf(PVDEP, 0.0f);
+        float PIMLT = mp::PIMLT(T, Qc);
+        float PIDW = mp::PIDW(dt, T, Qc, Qw, Dair, ps);
+        float PIHOM = mp::PIHOM(T, Qw);
+        float PIACR = mp::PIACR(gm, Qr, Qc, Dair, sR);
+        float PRACI = mp::PRACI(gm, Qr, Qc, Dair, sR);
+        float PRAUT = mp::PRAUT(Qw, QwMin, T);
+        float PRACW = mp::PRACW(gm, Qr, Qw, Dair, sR);
+        float PREVP = mp::PREVP(gm, T, Qr, Qv, QWS, Dair, ps, sR, PT1);
+        float PRACS = mp::PRACS(Qr, Qs, PT2, fv, Dair, sS, sR);
+        float PSACW = mp::PSACW(gm, Qs, Qw, Dair, sS);
+        float PSACR = mp::PSACR(Qr, Qs, fv, Dair, sR, sS);
+        float PSACI = mp::PSACI(gm, T, Qs, Qc, Dair, sS);
+        float PSAUT = mp::PSAUT(T, Qc, QcMin);
+        float PSFW = mp::PSFW(dt, T, ps, Qs, Qw, Qc, Qv, QWI, Dair);
+        float PSFI = mp::PSFI(T, ps, Qs, Qc, Qv, QWI);
+        float PSDEP = mp::PSDEP(gm, T, ps, Qv, QWI, Dair, sS);
+        float PSSUB = fmaxf(0.0f, -1 * PSDEP);
+        PSDEP = fmaxf(0.0f, PSDEP);
+        float PSMLT = mp::PSMLT(gm, T, ps, Qs, Qv, Dair, sS, PSACW, PSACR);
+        float PGAUT = mp::PGAUT(T, Qs, QiMin);
+        float PGFR = mp::PGFR(T, Qr, Dair, sR);
+        float PGACW = mp::PGACW(gm, Qi, Qw, sI, Dair);
+        float PGACI = mp::PGACI(gm, Qi, Qc, sI, Dair);
+        float PGACR = mp::PGACR(Qi, Qr, Dair, fv, sR, sI);
+        float PGACS = mp::PGACS(T, Qs, Qi, Dair, fv, sS, sI, false);
+        float PGSUB = mp::PGSUB(gm, T, ps, Qv, QWI, Qi, Dair, sI);
+        float PGMLT = mp::PGMLT(gm, T, ps, Qw, Qr, Qi, Qv, Dair, sI, PGACW, PGACR, dts);
+        float PGDRY = mp::PGDRY(Qw, Qc, Qr, Qs, PGACW, PGACI, PGACR, PGACS, dts);
+        float PGWET = mp::PGWET(gm, T, ps, Qc, Qs, Qi, Qv, fv, Dair, sS, sI, PGACI, PGACS, dts);
+        float PGACR1 = mp::PGACR1(T, Qw, Qc, Qs, Qi, Dair, fv, sS, sI, PGWET, PGACW, PGACI, PGACS, dts);
+        float W = 0.0f;
+        if (PGDRY > PGWET) W = 1.0f;
+        float cond = 0.0f, dep = 0.0f, frz = 0.0f;
+        if (PVCON >= 0) { PVCON = dt * fminf(Qv, speed * PVCON); cond += PVCON; }
+        else if (PVCON < 0) { PVCON = dt * fmaxf(-Qw, speed * PVCON); cond += PVCON; }
+        if (PVDEP >= 0) { PVDEP = dt * fminf(Qv, speed * PVDEP); dep += PVDEP; }
+        else if (PVDEP < 0) { PVDEP = dt * fmaxf(-Qc, speed * PVDEP); dep += PVDEP; }
+        float nQv = a.Qv[i], nQw = a.Qw[i], nQc = a.Qc[i], nQr = a.Qr[i], nQs = a.Qs[i], nQi = a.Qi[i];
+        if (T < 0) {
+            PIDW = dt * fminf(Qw, speed * PIDW); dep += PIDW;
+            PIHOM = dt * fminf(Qw, speed * PIHOM); frz += PIHOM;
+            PIACR = dt * fminf(Qr, speed * PIACR); frz += PIACR;
+            PRACI = dt * fminf(Qc, speed * PRACI);
+            PRAUT = dt * fminf(Qw, speed * PRAUT);
+            PRACW = dt * fminf(Qw, speed * PRACW);
+            PREVP = dt * fminf(Qr, speed * PREVP * (1 - PT1)); cond -= PREVP;
+            PRACS = dt * fminf(Qs, speed * PRACS * (1 - PT2));
+            PSACR = dt * fminf(Qr, speed * PSACR); frz += PSACR;
+            PSACW = dt * fminf(Qw, speed * PSACW); frz += PSACW;
+            PSACI = dt * fminf(Qc, speed * PSACI);
+            PSAUT = dt * fminf(Qc, speed * PSAUT);
+            PSFW = dt * fminf(Qw, speed * PSFW); frz += PSFW;
+            PSFI = dt * fminf(Qc, speed * PSFI);
+            PSDEP = dt * fminf(Qv, speed * PSDEP * (PT1)); dep += PSDEP;
+            PSSUB = dt * fminf(Qs, speed * PSSUB * (1 - PT1)); dep -= PSSUB;
+            PGAUT = dt * fminf(Qs, speed * PGAUT);
+            PGFR = dt * fminf(Qr, speed * PGFR); frz += PGFR;
+            PGACI = dt * fminf(Qc, speed * PGACI);
+            PGACR = dt * fminf(Qr, speed * PGACR * (1 - W)); frz += PGACR;
+            PGACS = dt * fminf(Qs, speed * PGACS);
+            PGSUB = dt * fminf(Qi, speed * PGSUB * (1 - PT1)); dep -= PGSUB;
+            if (W == 1.0f) {
+                if (PGACR1 >= 0) { PGACR1 = dt * fminf(Qr, speed * PGACR1); frz += PGACR1; }
+                else if (PGACR1 < 0) { PGACR1 = dt * fmaxf(-Qi, speed * PGACR1); frz += PGACR1; }
+            }
+            nQv += PSSUB * (1 - PT1) + PGSUB * (1 - PT1) + PREVP * (1 - PT1) - PSDEP * (PT1)-PVCON - PVDEP;
+            nQw += PVCON - PSACW - PSFW - PRAUT - PRACW - PIDW - PIHOM;
+            nQc += PVDEP + PIDW + PIHOM - PSAUT - PSACI - PRACI - PSFI - PGACI;
+            nQr += PRAUT + PRACW - PIACR - PSACR - PGACR * (1 - W) - PGACR1 * (W)-PGFR - PREVP * (1 - PT1);
+            nQs += PSAUT + PSACI + PSACW + PSFW + PSFI + PRACI * (PT3) + PIACR * (PT3) - PGACS - PGAUT -
+                   PRACS * (1 - PT2) + PSACR * (PT2) - PSSUB * (1 - PT1) + PSDEP * (PT1);
+            nQi += PGAUT + PGFR + PGDRY * (1 - W) + PGWET * (W) + PSACR * (1 - PT2) + PRACS * (1 - PT2) +
+                   PRACI * (1 - PT3) + PIACR * (1 - PT3) - PGSUB * (1 - PT1);
+        } else {
+            PIMLT = dt * fminf(Qc, speed * PIMLT); frz -= PIMLT;
+            PIACR = 0.0f;
+            PRAUT = dt * fminf(Qw, speed * PRAUT);
+            PRACW = dt * fminf(Qw, speed * PRACW);
+            PREVP = dt * fminf(Qr, speed * PREVP * (1 - PT1)); cond -= PREVP;
+            PSACW = dt * fminf(Qw, speed * PSACW);
+            PSMLT = dt * fminf(Qs, speed * PSMLT); frz -= PSMLT;
+            PGACW = dt * fminf(Qw, speed * PGACW); frz += PGACW;
+            PGACS = dt * fminf(Qs, speed * PGACS);
+            PGMLT = dt * fminf(Qi, speed * PGMLT); frz -= PGMLT;
+            nQv += PREVP * (1 - PT1) - PVCON - PVDEP;
+            nQw += PVCON + PIMLT - PRAUT - PRACW - PGACW - PSACW;
+            nQc += PVDEP - PIMLT;
+            nQr += PRAUT + PRACW + PSACW + PGACW + PGMLT + PSMLT - PREVP * (1 - PT1);
+            nQs += -PSMLT - PGACS;
+            nQi += -PGMLT + PGACS;
+        }
+        a.Qv[i] = nQv; a.Qw[i] = nQw; a.Qc[i] = nQc; a.Qr[i] = nQr; a.Qs[i] = nQs; a.Qi[i] = nQi;
+        // addHeatGPU with the updated vapour
+        const float Mair = 0.02896f, Mwater = 0.01802f;
+        const float XV = (nQv / Mwater) / ((nQv / Mwater) + (1 - nQv) / Mair);
+        const float Mth = XV * Mwater + (1 - XV) * Mair;
+        const float YV = XV * (Mwater / Mth);
+        const float yth = YV * 1.33f + (1 - YV) * 1.4f;
+        const float cpth = yth * kc::R / (Mth * (yth - 1));
+        const float th = a.th[i];
+        float sum = 0.0f;
+        sum += d_Lwater(th - 273.15f) / cpth * cond;
+        sum += d_Lice(th - 273.15f) / cpth * dep;
+        sum += kc::Lf / cpth * frz;
+        a.th[i] = th + sum;
+        if (a.phase)
+            a.phase[i] = (uint8_t)(1 | (T < 0.0f ? 2 : 0) | (PT1 != 0.0f ? 4 : 0) | (PT2 != 0.0f ? 8 : 0) |
+                                   (PT3 != 0.0f ? 16 : 0) | (W != 0.0f ? 32 : 0));
+        if (a.region && x >= a.rmin[0] && x <= a.rmax[0] && y >= a.rmin[1] && y <= a.rmax[1] && z >= a.rmin[2] &&
+            z <= a.rmax[2]) {
+            inRegion = true;
+            float PVVAP = 0.0f, PVSUB = 0.0f, pc = PVCON, pd = PVDEP;
+            if (pc < 0.0f) { PVVAP = -pc; pc = 0.0f; }
+            if (pd < 0.0f) { PVSUB = -pd; pd = 0.0f; }
+            const float vals[33] = {pc, pd, PIMLT, PIDW, PIHOM, PIACR, PRACI, PRAUT, PRACW, PREVP, PRACS,
+                                    PSACW, PSACR, PSACI, PSAUT, PSFW, PSFI, PSDEP, PSSUB, PSMLT, PGAUT, PGFR,
+                                    PGACW, PGACI, PGACR, PGDRY * (1 - W), PGACS, PGSUB, PGMLT, PGWET * W, PGACR1, PVVAP, PVSUB};
+            for (int k = 0; k < 33; k++) rv[k] = vals[k];
+        }
+    } else if (inb) {
+        if (a.phase) a.phase[gidx(G, x, y, z)] = 0;
+    }
+    if (a.region) {
+        const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+        for (int k = 0; k < 33; k++) {
+            const double s = blockSum(inRegion ? (double)rv[k] : 0.0, shr);
+            if (tid256() == 0) a.region[(long)k * nb + bid] = s;
+        }
+    }
+}
+hipError_t launch_microphysics(EnvState& e, float dt, float speed) {
+    MicroArgs a{e.f[SKT_QV], e.f[SKT_QW], e.f[SKT_QC], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_POTTEMP],
+                e.f[SKT_DENSITY], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], e.phase,
+                e.region_active ? e.region : nullptr, {e.rmin[0], e.rmin[1], e.rmin[2]},
+                {e.rmax[0], e.rmax[1], e.rmax[2]}};
+    hipLaunchKernelGGL(k_microphysics, grid3(e), B3, 0, e.stream, geom(e), a, dt, speed, e.gm);
+    return hipGetLastError();
+}
+// Fixed-order sum of the region partials into 33 floats (written to sc->pad area
+// is not large enough, so into the tail of the region buffer).
+KL void k_region_finalize(double* region, unsigned nb, float* out) {
+    __shared__ double sh[4];
+    for (int k = 0; k < 33; k++) {
+        double v = 0.0;
+        for (unsigned j = tid256(); j < nb; j += 256) v += region[(long)k * nb + j];
+        v = blockSum(v, sh);
+        if (tid256() == 0) out[k] = (float)v;
+    }
+}
+hipError_t launch_region_finalize(EnvState& e) {
+    const unsigned nb = (unsigned)blocks3d(e);
+    float* out = (float*)(e.region + 33L * nb);
+    hipLaunchKernelGGL(k_region_finalize, dim3(1), dim3(64, 4, 1), 0, e.stream, e.region, nb, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ masks / editor ---
+// computeNeighbourGPU, Game/kernelSky.cu:1284-1316 (12 bytes per cell).
+KL void k_masks(Geom G, uint8_t* out) {
+    CELL_SETUP;
+    (void)bid;
+    if (!inb) return;
+    uint8_t* m = out + 12 * gidx(G, x, y, z);
+    const bool cg = isGround(G, x, y, z);
+    bool o;
+    o = x == 0; m[0] = o; m[1] = (o && cg) || (!o && isGround(G, x - 1, y, z));
+    o = x == G.X - 1; m[2] = o; m[3] = (o && cg) || (!o && isGround(G, x + 1, y, z));
+    o = y == G.Y - 1; m[4] = o; m[5] = !o && isGround(G, x, y + 1, z);
+    o = y == 0; m[6] = o; m[7] = (o && cg) || (!o && isGround(G, x, y - 1, z));
+    o = z == G.Z - 1; m[8] = o; m[9] = (o && cg) || (!o && isGround(G, x, y, z + 1));
+    o = z == 0; m[10] = o; m[11] = (o && cg) || (!o && isGround(G, x, y, z - 1));
+}
+hipError_t launch_masks(EnvState& e, uint8_t* dev_out) {
+    hipLaunchKernelGGL(k_masks, grid3(e), B3, 0, e.stream, geom(e), dev_out);
+    return hipGetLastError();
+}
+struct Ten {
+    float* a[10];
+};
+// resetValueInGround over n fields (Game/kernelGround.cu:21-36)
+KL void k_reset_ground(Geom G, Ten t, int n) {
+    CELL_SETUP;
+    (void)bid;
+    if (!inb || !isGround(G, x, y, z)) return;
+    const long i = gidx(G, x, y, z);
+    for (int k = 0; k < n; k++) t.a[k][i] = 0.0f;
+}
+hipError_t launch_editor_reset(EnvState& e, int nfields) {
+    Ten t{{e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_QW], e.f[SKT_QC], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI],
+           e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ]}};
+    hipLaunchKernelGGL(k_reset_ground, grid3(e), B3, 0, e.stream, geom(e), t, nfields);
+    return hipGetLastError();
+}
+// computeIsenTempGroundGPU, Game/kernelGround.cu:38-47
+KL void k_isen_temp_ground(Geom G, float* gT, const float* isen, const float* gP, const float* p) {
+    const long g = blockIdx.x * 256L + threadIdx.x;
+    if (g >= (long)G.X * G.Z) return;
+    const int x = (int)(g % G.X), zc = (int)(g / G.X);
+    const int GH = G.gh[g] + 1 >= G.Y ? G.Y - 1 : G.gh[g] + 1;
+    const float T = d_potTemp(isen[GH - 1] - 273.15f, gP[g], p[gidx(G, x, GH, zc)]);
+    gT[g] = T + 273.15f;
+}
+hipError_t launch_isen_temp_ground(EnvState& e) {
+    hipLaunchKernelGGL(k_isen_temp_ground, grid1(e.G), dim3(256), 0, e.stream, geom(e), e.f[SKT_GROUND_T],
+                       e.f[SKT_ISEN_TEMP], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
+    return hipGetLastError();
+}
+// Default profiles = the (x=0, z=0) column before the init noise
+// (cudaMemcpy2D of Game/environment.cu:299-303).
+KL void k_profiles(Geom G, const float* th, const float* qv, const float* u, const float* w, float* iT, float* iQ,
+                   float* dU, float* dW) {
+    const int y = blockIdx.x * 256 + threadIdx.x;
+    if (y >= G.Y) return;
+    const long i = (long)G.X * y;
+    iT[y] = th[i];
+    iQ[y] = qv[i];
+    dU[y] = u[i];
+    dW[y] = w[i];
+}
+hipError_t launch_profiles_from_column(EnvState& e) {
+    hipLaunchKernelGGL(k_profiles, grid1(e.Y), dim3(256), 0, e.stream, geom(e), e.f[SKT_POTTEMP], e.f[SKT_QV],
+                       e.f[SKT_VELX], e.f[SKT_VELZ], e.f[SKT_ISEN_TEMP], e.f[SKT_ISEN_VAPOR], e.f[SKT_DEFAULT_VELX],
+                       e.f[SKT_DEFAULT_VELZ]);
+    return hipGetLastError();
+}
+// randomArray (Game/kernelMaths.cu:93-116) with the seed every thread intends
+// to read (quirk A10); the hash is the published PCG-RXS-M-XS function.
+KL void k_random(float* a, long n, float mn, float mx, uint32_t seed) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const uint32_t sd = seed + (uint32_t)i;
+        const uint32_t s = sd * 747796405u + 2891336453u;
+        const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+        const uint32_t h = (w >> 22u) ^ w;
+        const float rnd = float(h % 1000u) / float(1000);
+        a[i] += rnd * (mx - mn) + mn;
+    }
+}
+hipError_t launch_random(EnvState& e, float* a, float mn, float mx, uint32_t seed) {
+    hipLaunchKernelGGL(k_random, dim3((unsigned)std::min<long>((e.N + 255) / 256, 65536)), dim3(256), 0, e.stream, a,
+                       e.N, mn, mx, seed);
+    return hipGetLastError();
+}
+
+}  // namespace skt

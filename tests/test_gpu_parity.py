@@ -64,6 +64,11 @@ def rich_state(env, orc, seed=1):
     """Clouds, rain, snow, ice, winds and a cold upper layer on both sides."""
     rng = np.random.default_rng(seed)
     n = orc.N
+    # perturb the initialised state, not what earlier tests left behind
+    if not hasattr(orc, "_base"):
+        orc._base = {f: orc.get(f).copy() for f in ob.FIELDS}
+    for f, v in orc._base.items():
+        orc.set(f, v)
     for f, hi in (("QW", 2.5e-3), ("QC", 1.5e-3), ("QR", 2e-3), ("QS", 1e-3), ("QI", 1e-3)):
         v = rng.uniform(0, hi, n).astype(np.float32)
         v[rng.uniform(size=n) < 0.3] = 0.0
