@@ -102,61 +102,120 @@ int readScalars(EnvState& e) {
 }
 
 // diffuseGPU, Game/environment.cu:597-673 (boundary table :602-639).
-int diffuse(EnvState& e, int fld, int type, float dt) {
-    Bounds b{BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET};
-    const float* prof = nullptr;
+void diffusionBounds(EnvState& e, int type, const float** prof, Bounds* b) {
+    *b = Bounds{BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET};
     switch (type) {
-        case 0: prof = e.f[SKT_ISEN_TEMP]; b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_CUSTOM}; break;
-        case 1: prof = e.f[SKT_ISEN_VAPOR]; b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET}; break;
-        case 2: prof = e.f[SKT_DEFAULT_VELX]; b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET}; break;
-        case 3: prof = nullptr; b = Bounds{BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET}; break;
-        case 4: prof = e.f[SKT_DEFAULT_VELZ]; b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET}; break;
-        default: prof = e.zeros; break;
+        case 0: *prof = e.f[SKT_ISEN_TEMP]; *b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_CUSTOM}; break;
+        case 1: *prof = e.f[SKT_ISEN_VAPOR]; *b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET}; break;
+        case 2: *prof = e.f[SKT_DEFAULT_VELX]; *b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET}; break;
+        case 3: *prof = nullptr; break;
+        case 4: *prof = e.f[SKT_DEFAULT_VELZ]; *b = Bounds{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET}; break;
+        default: *prof = e.zeros; break;
     }
-    const float k = 0.005f * dt / (e.dx * e.dx);
-    float* bufs[2] = {e.f[fld], e.work[0]};
-    const int n = e.cfg.diffusion_sweeps;
-    for (int s = 0; s < n; s++) TL(e, TC_DIFFUSE, launch_diffuse_sweep(e, bufs[s & 1], bufs[(s + 1) & 1], prof, b, type, k));
-    if (n & 1) TL(e, TC_DIFFUSE, launch_copy_sky(e, e.work[0], e.f[fld]));
-    return SKT_OK;
 }
 
+// Diffusion of n independent fields (each reads only itself, the ground
+// temperature and pressure), swept together: field j ping-pongs with work
+// buffer j. Identical to diffusing them one after the other.
+int diffuseFields(EnvState& e, const int* flds, const int* types, int n, float dt) {
+    const float k = 0.005f * dt / (e.dx * e.dx);
+    const int sweeps = e.cfg.diffusion_sweeps;
+    DiffBatch db{};
+    db.n = n;
+    for (int j = 0; j < n; j++) {
+        diffusionBounds(e, types[j], &db.f[j].prof, &db.f[j].b);
+        db.f[j].type = types[j];
+    }
+    for (int s = 0; s < sweeps; s++) {
+        for (int j = 0; j < n; j++) {
+            float* bufs[2] = {e.f[flds[j]], e.work[j]};
+            db.f[j].in = bufs[s & 1];
+            db.f[j].out = bufs[(s + 1) & 1];
+        }
+        TL(e, TC_DIFFUSE, launch_diffuse_batch(e, db, k));
+    }
+    if (sweeps & 1)
+        for (int j = 0; j < n; j++) TL(e, TC_DIFFUSE, launch_copy_sky(e, e.work[j], e.f[flds[j]]));
+    return SKT_OK;
+}
+int diffuse(EnvState& e, int fld, int type, float dt) { return diffuseFields(e, &fld, &type, 1, dt); }
+
 // advectPPMWGPU, Game/environment.cu:692-824. The five Strang passes rotate
-// through three work buffers so that no pass writes what it reads; a velocity
-// component being advected is read in its current (partially advected)
-// state, as in the reference (quirk A11), and the result lands in place.
-int advect(EnvState& e, int fld, const float* prof, Bounds b, float dt) {
+// through three work buffers per field so that no pass writes what it reads;
+// a velocity component being advected is read in its current (partially
+// advected) state, as in the reference (quirk A11), and the result lands in
+// place. Fields that are not velocity components are independent given the
+// velocities and move together in one launch per pass; the substep count
+// comes from the shared velocities, so it is the same for all of them.
+struct AdvField {
+    int fld;
+    const float* prof;
+    Bounds b;
+};
+int advectFields(EnvState& e, const AdvField* fs, int n, float dt) {
     TL(e, TC_MAXVEL, launch_maxvel(e, dt));
     int rc = readScalars(e);
     if (rc) return rc;
     const int nsub = e.sc_host->nsub;
     const float dts = e.sc_host->dtsub;
-    if (e.stats.n_advect < 16) e.stats.substeps[e.stats.n_advect] = nsub;
-    e.stats.n_advect++;
-    const float h = dts * 0.5f;
-    float* A = e.f[fld];
-    float *T1 = e.work[1], *T2 = e.work[2], *T3 = e.work[3];
-    float *U = e.f[SKT_VELX], *V = e.f[SKT_VELY], *W = e.f[SKT_VELZ];
-    for (int s = 0; s < nsub; s++) {
-        TL(e, TC_PPM, launch_ppm_pass(e, A, T1, fld == SKT_VELX ? A : U, prof, b, 0, h));
-        TL(e, TC_PPM, launch_ppm_pass(e, T1, T2, fld == SKT_VELY ? A : V, prof, b, 1, h));
-        TL(e, TC_PPM, launch_ppm_pass(e, T2, T1, fld == SKT_VELZ ? T2 : W, prof, b, 2, dts));
-        TL(e, TC_PPM, launch_ppm_pass(e, T1, T3, fld == SKT_VELY ? T2 : V, prof, b, 1, h));
-        TL(e, TC_PPM, launch_ppm_pass(e, T3, A, fld == SKT_VELX ? T3 : U, prof, b, 0, h));
+    for (int j = 0; j < n; j++) {
+        if (e.stats.n_advect < 16) e.stats.substeps[e.stats.n_advect] = nsub;
+        e.stats.n_advect++;
     }
+    const float h = dts * 0.5f;
+    float *U = e.f[SKT_VELX], *V = e.f[SKT_VELY], *W = e.f[SKT_VELZ];
+    PpmBatch pb[5]{};
+    for (int j = 0; j < n; j++) {
+        const int fld = fs[j].fld;
+        float* A = e.f[fld];
+        float *T1 = e.work[3 * j], *T2 = e.work[3 * j + 1], *T3 = e.work[3 * j + 2];
+        const PpmField f0{A, T1, fld == SKT_VELX ? A : U, fs[j].prof, fs[j].b};
+        const PpmField f1{T1, T2, fld == SKT_VELY ? A : V, fs[j].prof, fs[j].b};
+        const PpmField f2{T2, T1, fld == SKT_VELZ ? T2 : W, fs[j].prof, fs[j].b};
+        const PpmField f3{T1, T3, fld == SKT_VELY ? T2 : V, fs[j].prof, fs[j].b};
+        const PpmField f4{T3, A, fld == SKT_VELX ? T3 : U, fs[j].prof, fs[j].b};
+        pb[0].f[j] = f0;
+        pb[1].f[j] = f1;
+        pb[2].f[j] = f2;
+        pb[3].f[j] = f3;
+        pb[4].f[j] = f4;
+    }
+    for (int q = 0; q < 5; q++) pb[q].n = n;
+    const int dir[5] = {0, 1, 2, 1, 0};
+    const float pdt[5] = {h, h, dts, h, h};
+    for (int s = 0; s < nsub; s++)
+        for (int q = 0; q < 5; q++) TL(e, TC_PPM, launch_ppm_batch(e, pb[q], dir[q], pdt[q]));
     return SKT_OK;
+}
+int advect(EnvState& e, int fld, const float* prof, Bounds b, float dt) {
+    const AdvField f{fld, prof, b};
+    return advectFields(e, &f, 1, dt);
 }
 
 // advectPrecip, Game/environment.cu:826-849 (guessed fall speeds 5/1/8 m/s).
-int precip(EnvState& e, int fld, int type, float dt) {
-    const float maxVel = type == 0 ? 5.0f : (type == 1 ? 1.0f : 8.0f);
-    const float C = maxVel * dt / e.dx;
-    const int sub = int(fminf(ceilf(fabsf(C)), (float)e.cfg.max_substeps));
-    if (sub <= 0) return SKT_OK;
-    const float dts = dt / sub;
-    for (int s = 0; s < sub; s++) TL(e, TC_PRECIP, launch_precip(e, e.f[fld], type, dts));
+// Each field falls on its own substeps; the fields only read themselves and
+// the (unchanged) temperature, pressure and vapour, so they share launches.
+int precipFields(EnvState& e, const int* flds, const int* types, int n, float dt) {
+    int sub[3], maxsub = 0;
+    for (int j = 0; j < n; j++) {
+        const float maxVel = types[j] == 0 ? 5.0f : (types[j] == 1 ? 1.0f : 8.0f);
+        const float C = maxVel * dt / e.dx;
+        sub[j] = int(fminf(ceilf(fabsf(C)), (float)e.cfg.max_substeps));
+        maxsub = std::max(maxsub, sub[j]);
+    }
+    for (int s = 0; s < maxsub; s++) {
+        PrecipBatch pbt{};
+        pbt.n = n;
+        for (int j = 0; j < n; j++) {
+            pbt.q[j] = e.f[flds[j]];
+            pbt.type[j] = types[j];
+            pbt.dts[j] = s < sub[j] ? dt / sub[j] : 0.0f;
+        }
+        TL(e, TC_PRECIP, launch_precip_batch(e, pbt));
+    }
     return SKT_OK;
 }
+int precip(EnvState& e, int fld, int type, float dt) { return precipFields(e, &fld, &type, 1, dt); }
 
 // calculatePressureProject, Game/environment.cu:898-987. The scalar state
 // lives on the device (two slots); the host looks at it every 25 iterations
@@ -246,29 +305,32 @@ int update(EnvState& e, float dt, float speed) {
     RC(diffuse(e, SKT_POTTEMP, 0, ds));
     RC(advect(e, SKT_POTTEMP, e.f[SKT_ISEN_TEMP], bT, ds));
     TL(e, TC_OTHER, launch_buoyancy(e, ds));
-    RC(diffuse(e, SKT_VELX, 2, ds));
-    RC(diffuse(e, SKT_VELY, 3, ds));
-    RC(diffuse(e, SKT_VELZ, 4, ds));
+    {
+        const int f[3] = {SKT_VELX, SKT_VELY, SKT_VELZ}, t[3] = {2, 3, 4};
+        RC(diffuseFields(e, f, t, 3, ds));
+    }
     RC(advect(e, SKT_VELX, e.f[SKT_DEFAULT_VELX], bU, ds));
     RC(advect(e, SKT_VELY, e.zeros, bV, ds));
     RC(advect(e, SKT_VELZ, e.f[SKT_DEFAULT_VELZ], bW, ds));
     TL(e, TC_OTHER, launch_init_density(e, 1));
     RC(pressureProject(e, ds, 1));
-    RC(diffuse(e, SKT_QV, 1, ds));
+    // moisture (Game/environment.cu:508-533): each field is diffused, rained
+    // out and advected on its own; only the precipitation fall speed reads
+    // the vapour field, which is final by then
+    {
+        const int f[6] = {SKT_QV, SKT_QW, SKT_QC, SKT_QR, SKT_QS, SKT_QI}, t[6] = {1, 5, 5, 5, 5, 5};
+        RC(diffuseFields(e, f, t, 6, ds));
+    }
     RC(advect(e, SKT_QV, e.f[SKT_ISEN_VAPOR], bQv, ds));
-    RC(diffuse(e, SKT_QW, 5, ds));
-    RC(advect(e, SKT_QW, e.zeros, bQ, ds));
-    RC(diffuse(e, SKT_QC, 5, ds));
-    RC(advect(e, SKT_QC, e.zeros, bQ, ds));
-    RC(diffuse(e, SKT_QR, 5, ds));
-    RC(precip(e, SKT_QR, 0, ds));
-    RC(advect(e, SKT_QR, e.zeros, bQ, ds));
-    RC(diffuse(e, SKT_QS, 5, ds));
-    RC(precip(e, SKT_QS, 1, ds));
-    RC(advect(e, SKT_QS, e.zeros, bQ, ds));
-    RC(diffuse(e, SKT_QI, 5, ds));
-    RC(precip(e, SKT_QI, 2, ds));
-    RC(advect(e, SKT_QI, e.zeros, bQ, ds));
+    {
+        const int f[3] = {SKT_QR, SKT_QS, SKT_QI}, t[3] = {0, 1, 2};
+        RC(precipFields(e, f, t, 3, ds));
+    }
+    {
+        const AdvField a[5] = {{SKT_QW, e.zeros, bQ}, {SKT_QC, e.zeros, bQ}, {SKT_QR, e.zeros, bQ},
+                               {SKT_QS, e.zeros, bQ}, {SKT_QI, e.zeros, bQ}};
+        RC(advectFields(e, a, 5, ds));
+    }
     RC(microSky(e, dt, speed));
     e.params.time += speed * dt * float(!e.params.pause_diurnal);
     if (e.params.time > 86400.0f) e.params.time = 0.0f;
@@ -283,7 +345,7 @@ void freeAll(EnvState& e) {
         if (p) (void)hipFree(p);
     };
     for (int k = 0; k < SKT_FIELD_COUNT; k++) fr(e.f[k]);
-    for (int k = 0; k < 4; k++) fr(e.work[k]);
+    for (int k = 0; k < SKT_NWORK; k++) fr(e.work[k]);
     for (int k = 0; k < 6; k++) fr(e.pcg[k]);
     fr(e.gh);
     fr(e.ghm1);
@@ -360,7 +422,7 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     e.stream = e.own_stream;
     for (int k = 0; k < SKT_FIELD_COUNT; k++)
         if (dalloc(e, &e.f[k], sizeof(float) * fieldLen(e, k))) return bail(SKT_ERR_NOMEM);
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < SKT_NWORK; k++)
         if (dalloc(e, &e.work[k], sizeof(float) * std::max(e.N, 2 * e.G))) return bail(SKT_ERR_NOMEM);
     for (int k = 0; k < 6; k++)
         if (dalloc(e, &e.pcg[k], sizeof(float) * e.N)) return bail(SKT_ERR_NOMEM);

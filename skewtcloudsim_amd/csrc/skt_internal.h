@@ -34,6 +34,9 @@ struct Scalars {
 // Regions of the per-block reduction partials (each maxBlocks doubles).
 enum { PART_DIV = 0, PART_START, PART_A, PART_BMAX, PART_BSUM, PART_COUNT };
 
+// N-sized work buffers: three per field of a batched advection.
+constexpr int SKT_NWORK = 16;
+
 struct EnvState {
     skt_config cfg;
     int X, Y, Z;
@@ -49,7 +52,7 @@ struct EnvState {
     int* gh = nullptr;                      // ground height [G]
     int* ghm1 = nullptr;                    // max ground height over the 3x3 column neighbourhood [G]
     int* ghm2 = nullptr;                    // max ground height over the 5x5 column neighbourhood [G]
-    float* work[4] = {nullptr};             // N-sized work buffers
+    float* work[SKT_NWORK] = {nullptr};     // N-sized work buffers
     float* pcg[6] = {nullptr};              // r, z, s_a, s_b, As, pre
     float4* A = nullptr;                    // upper coefficients (x,y,z) and diagonal (w)
     float* zeros = nullptr;                 // [Y] zero profile
@@ -98,11 +101,42 @@ hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst);
 hipError_t launch_ground_water_snapshot(EnvState& e, float dt, float speed);
 // stages: 1 cloud cover, 2 ground temp, 4 precip hitting ground, 8 ground microphysics, 16 setTempsAtGround
 hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed, float irr);
-hipError_t launch_diffuse_sweep(EnvState& e, const float* in, float* out, const float* prof, Bounds b, int type, float k);
+// Independent fields swept by one diffusion launch.
+constexpr int SKT_BATCH = 6;
+struct DiffField {
+    const float* in;
+    float* out;
+    const float* prof;
+    Bounds b;
+    int type;
+};
+struct DiffBatch {
+    DiffField f[SKT_BATCH];
+    int n;
+};
+hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k);
 hipError_t launch_maxvel(EnvState& e, float dt);
-hipError_t launch_ppm_pass(EnvState& e, const float* in, float* out, const float* vel, const float* prof, Bounds b,
-                           int dir, float dt);
-hipError_t launch_precip(EnvState& e, float* q, int type, float dts);
+// Independent fields moved by one PPM pass launch.
+struct PpmField {
+    const float* in;
+    float* out;
+    const float* vel;
+    const float* prof;
+    Bounds b;
+};
+struct PpmBatch {
+    PpmField f[SKT_BATCH];
+    int n;
+};
+hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt);
+// Precipitation fields falling in one launch (dts 0: field skipped).
+struct PrecipBatch {
+    float* q[3];
+    int type[3];
+    float dts[3];
+    int n;
+};
+hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt);
 hipError_t launch_buoyancy(EnvState& e, float dt);
 hipError_t launch_init_density(EnvState& e, int copy_old);
 hipError_t launch_new_pressure(EnvState& e);

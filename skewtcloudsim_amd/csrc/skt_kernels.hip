@@ -84,84 +84,131 @@ hipError_t launch_stream_copy(EnvState& e, const float* src, float* dst, long n)
     return hipGetLastError();
 }
 
+// LDS tile of one plane for the 64 x 4 block: rows y0-1 .. y0+4 and columns
+// x0-1 .. x0+64 (the block's cells plus a one-cell halo). Each cell's value
+// is read from HBM once per plane: by its own thread, plus halo rows by
+// waves 0 and 1 and halo columns by lanes 0 and 63. Two buffers and one
+// barrier per plane: a wave can only overwrite a buffer after every wave has
+// passed the next plane's barrier, i.e. finished reading it.
+constexpr int TH = 6, TW = 66;
+struct TileRole {
+    bool own, hasD, hasU, hasL, hasR;
+    unsigned rc, rD, rU, rL, rR;
+};
+__device__ __forceinline__ TileRole tileRole(const Geom& G, int x, int y) {
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    TileRole t;
+    t.own = x < G.X && y < G.Y;
+    t.hasD = ty == 0 && y >= 1 && x < G.X && y - 1 < G.Y;
+    t.hasU = ty == 1 && y + 3 < G.Y && x < G.X;
+    t.hasL = tx == 0 && x >= 1 && y < G.Y;
+    t.hasR = tx == 63 && x + 1 < G.X && y < G.Y;
+    t.rc = t.own ? (unsigned)(x + G.X * y) : 0u;
+    t.rD = t.hasD ? (unsigned)(x + G.X * (y - 1)) : 0u;
+    t.rU = t.hasU ? (unsigned)(x + G.X * (y + 3)) : 0u;
+    t.rL = t.hasL ? (unsigned)(x - 1 + G.X * y) : 0u;
+    t.rR = t.hasR ? (unsigned)(x + 1 + G.X * y) : 0u;
+    return t;
+}
+
 // ------------------------------------------------------------- diffusion ---
 // One Jacobi sweep = the red+black pair of diffuseRedBlack
 // (Game/kernelSky.cu:43-115; both colours read 'input', quirk A3):
 // out = (in + k (l + r + u + d + f + b)) / (1 + 6k) on sky cells.
+// One launch sweeps up to SKT_BATCH independent fields (the grid is the
+// z-march grid repeated per field). Per plane a thread loads its 5 in-plane
+// and front neighbours, ghm1 (largest ground height of the 3x3 columns
+// around) and the ground height of the next plane, for ZB planes at a time.
+// Cells with y >= ghm1 + 2 have no ground neighbour, so only the domain faces
+// need a boundary value; other cells load the neighbours' ground heights.
 constexpr int ZB_DIFF = 4;
 template <int ZB>
-KL void k_diffuse(Geom G, int zc, const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ prof,
-                  Bounds b, int type, float k, const float* __restrict__ gT, const float* __restrict__ gP,
-                  const float* __restrict__ p) {
-    ZM_SETUP;
+KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const float* __restrict__ gT,
+                  const float* __restrict__ gP, const float* __restrict__ p) {
+    const unsigned lin = xcdRemap(blockIdx.x, gridDim.x);
+    const DiffField& fd = db.f[lin / nbf];
+    ZM_SETUP_B(lin % nbf);
     (void)bid;
     if (!inb || z0 >= z1) return;
+    const float* __restrict__ in = fd.in;
+    float* __restrict__ out = fd.out;
+    const float* __restrict__ prof = fd.prof;
+    const Bounds b = fd.b;
+    const int type = fd.type;
     const int X = G.X, Y = G.Y, Z = G.Z;
     const long XY = (long)X * Y;
-    const int xl = max(x - 1, 0), xr = min(x + 1, X - 1), yd = max(y - 1, 0), yu = min(y + 1, Y - 1);
-    const int rc = x + X * y, rl = xl + X * y, rr = xr + X * y, rd = x + X * yd, ru = x + X * yu;
+    const unsigned xl = max(x - 1, 0), xr = min(x + 1, X - 1), yd = max(y - 1, 0), yu = min(y + 1, Y - 1);
+    const unsigned rc = x + X * y, rl = xl + X * y, rr = xr + X * y, rd = x + X * yd, ru = x + X * yu;
     const float pv = prof ? prof[y] : 0.0f, pvd = prof ? prof[yd] : 0.0f, pvu = prof ? prof[yu] : 0.0f;
     const float den = 1 + 6 * k;
     const int zbk = max(z0 - 1, 0);
     float vb = in[XY * zbk + rc], vc = in[XY * z0 + rc];
     int ghb = G.gh[x + X * zbk], ghc = G.gh[x + X * z0];
-    for (int zg = z0; zg < z1; zg += ZB) {
+    auto plane = [&](int z, float L, float R, float D, float U, float F, int gm, int gf) {
+        const bool oL = x == 0, oR = x == X - 1, oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
+        if (y >= gm + 2) {
+            const float l = oL ? bcv(b.sides, L, pv) : L, r = oR ? bcv(b.sides, R, pv) : R;
+            const float u = oU ? bcv(b.sides, U, pvu) : U;
+            const float f = oF ? bcv(b.sides, F, pv) : F, bb = oB ? bcv(b.sides, vb, pv) : vb;
+            (out + XY * z)[rc] = (vc + k * (l + r + u + D + f + bb)) / den;
+        } else if (y > ghc) {
+            const int* gz = G.gh + X * z;
+            // a position outside the domain clamps onto this sky cell, so the
+            // ground test at the clamped column is the in-domain test
+            const bool gL = y <= gz[xl], gR = y <= gz[xr], gD = y - 1 <= ghc;
+            const bool gF = !oF && y <= gf, gB = !oB && y <= ghb;
+            float l = nbv(L, oL, gL, pv, b), r = nbv(R, oR, gR, pv, b);
+            float d = nbv(D, false, gD, pvd, b), u = nbv(U, oU, false, pvu, b);
+            float f = nbv(F, oF, gF, pv, b), bb = nbv(vb, oB, gB, pv, b);
+            if (type == 0 && (gL | gR | gD | gF | gB)) {
+                // potential temperature: a ground neighbour reads the potential
+                // temperature of the ground temperature at this cell's pressure (:95-100)
+                const float pc = p[XY * z + rc];
+                auto tg = [&](int col) { return d_potTemp(gT[col] - 273.15f, pc, gP[col]) + 273.15f; };
+                if (gD) d = tg(x + X * z);
+                if (gL) l = tg(xl + X * z);
+                if (gR) r = tg(xr + X * z);
+                if (gF) f = tg(x + X * (z + 1));
+                if (gB) bb = tg(x + X * (z - 1));
+            }
+            (out + XY * z)[rc] = (vc + k * (l + r + u + d + f + bb)) / den;
+        }
+        vb = vc;
+        vc = F;
+        ghb = ghc;
+        ghc = gf;
+    };
+    auto load = [&](int z, float& L, float& R, float& D, float& U, float& F, int& gm, int& gf) {
+        const int zf = min(z + 1, Z - 1);
+        const float* pl = in + XY * z;
+        L = pl[rl];
+        R = pl[rr];
+        D = pl[rd];
+        U = pl[ru];
+        F = (in + XY * zf)[rc];
+        gm = (G.ghm1 + X * z)[(unsigned)x];
+        gf = (G.gh + X * zf)[(unsigned)x];
+    };
+    int zg = z0;
+    for (; zg + ZB <= z1; zg += ZB) {
         float L[ZB], R[ZB], D[ZB], U[ZB], F[ZB];
-        int GL[ZB], GR[ZB], GF[ZB];
+        int GM[ZB], GF[ZB];
 #pragma unroll
-        for (int j = 0; j < ZB; j++) {
-            const int z = zg + j;
-            if (z < z1) {
-                const long pz = XY * z;
-                const int zf = min(z + 1, Z - 1);
-                L[j] = in[pz + rl];
-                R[j] = in[pz + rr];
-                D[j] = in[pz + rd];
-                U[j] = in[pz + ru];
-                F[j] = in[XY * zf + rc];
-                GL[j] = G.gh[xl + X * z];
-                GR[j] = G.gh[xr + X * z];
-                GF[j] = G.gh[x + X * zf];
-            }
-        }
+        for (int j = 0; j < ZB; j++) load(zg + j, L[j], R[j], D[j], U[j], F[j], GM[j], GF[j]);
 #pragma unroll
-        for (int j = 0; j < ZB; j++) {
-            const int z = zg + j;
-            if (z < z1) {
-                if (y > ghc) {
-                    const bool oL = x == 0, oR = x == X - 1, oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
-                    // a position outside the domain clamps onto this sky cell, so
-                    // the ground test at the clamped column is the in-domain test
-                    const bool gL = y <= GL[j], gR = y <= GR[j], gD = y - 1 <= ghc;
-                    const bool gF = !oF && y <= GF[j], gB = !oB && y <= ghb;
-                    float l = nbv(L[j], oL, gL, pv, b), r = nbv(R[j], oR, gR, pv, b);
-                    float d = nbv(D[j], false, gD, pvd, b), u = nbv(U[j], oU, false, pvu, b);
-                    float f = nbv(F[j], oF, gF, pv, b), bb = nbv(vb, oB, gB, pv, b);
-                    if (type == 0 && (gL | gR | gD | gF | gB)) {
-                        // potential temperature: a ground neighbour reads the potential
-                        // temperature of the ground temperature at this cell's pressure (:95-100)
-                        const float pc = p[XY * z + rc];
-                        auto tg = [&](int col) { return d_potTemp(gT[col] - 273.15f, pc, gP[col]) + 273.15f; };
-                        if (gD) d = tg(x + X * z);
-                        if (gL) l = tg(xl + X * z);
-                        if (gR) r = tg(xr + X * z);
-                        if (gF) f = tg(x + X * (z + 1));
-                        if (gB) bb = tg(x + X * (z - 1));
-                    }
-                    out[XY * z + rc] = (vc + k * (l + r + u + d + f + bb)) / den;
-                }
-                vb = vc;
-                vc = F[j];
-                ghb = ghc;
-                ghc = GF[j];
-            }
-        }
+        for (int j = 0; j < ZB; j++) plane(zg + j, L[j], R[j], D[j], U[j], F[j], GM[j], GF[j]);
+    }
+    for (; zg < z1; zg++) {
+        float L, R, D, U, F;
+        int gm, gf;
+        load(zg, L, R, D, U, F, gm, gf);
+        plane(zg, L, R, D, U, F, gm, gf);
     }
 }
-hipError_t launch_diffuse_sweep(EnvState& e, const float* in, float* out, const float* prof, Bounds b, int type, float k) {
+hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k) {
     const ZGrid zg = zgrid(e);
-    hipLaunchKernelGGL(k_diffuse<ZB_DIFF>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, in, out, prof, b, type, k,
-                       e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
+    hipLaunchKernelGGL(k_diffuse<ZB_DIFF>, dim3(zg.nblocks * db.n), B3, 0, e.stream, geom(e), zg.zc, zg.nblocks, db,
+                       k, e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
     return hipGetLastError();
 }
 
@@ -255,68 +302,97 @@ __device__ __forceinline__ float ppmCell(const float* q, const float* o, float v
     return num / den;
 }
 
-// x pass: the stencil runs along the row.
+// Field of this block in a batched pass (the grid repeats the z-march grid
+// once per field).
+#define PPM_FIELD_SETUP                                                   \
+    const unsigned lin_ = xcdRemap(blockIdx.x, gridDim.x);                \
+    const PpmField& fd_ = pb.f[lin_ / nbf];                               \
+    ZM_SETUP_B(lin_ % nbf);                                               \
+    (void)bid;                                                            \
+    const float* __restrict__ in = fd_.in;                                \
+    float* __restrict__ out = fd_.out;                                    \
+    const float* __restrict__ vel = fd_.vel;                              \
+    const float* __restrict__ prof = fd_.prof;                            \
+    const Bounds b = fd_.b
+
+// x pass: the stencil runs along the row. Cells with y > ghm2 (largest
+// ground height over the 5x5 columns around) see no ground in the stencil.
 constexpr int ZB_PPM = 4;
 template <int ZB>
-KL void k_ppm_x(Geom G, int zc, const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ vel,
-                const float* __restrict__ prof, Bounds b, float dt, float dx, const float* __restrict__ cust) {
-    ZM_SETUP;
-    (void)bid;
+KL void k_ppm_x(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, const float* __restrict__ cust) {
+    PPM_FIELD_SETUP;
     if (!inb) return;
     const int X = G.X;
     const long XY = (long)X * G.Y;
-    int cx[5];
+    unsigned cx[5];
 #pragma unroll
     for (int s = 0; s < 5; s++) cx[s] = clampi(x + s - 2, 0, X - 1);
-    const int rowy = X * y;
+    const unsigned rowy = X * y;
     const float pv = prof ? prof[y] : 0.0f, cv = cust[y];
-    for (int zg = z0; zg < z1; zg += ZB) {
+    auto load = [&](int z, float* Q, float& VC, float& VM, int& gm, int& gc) {
+        const float* pl = in + XY * z + rowy;
+        const float* vl = vel + XY * z + rowy;
+#pragma unroll
+        for (int s = 0; s < 5; s++) Q[s] = pl[cx[s]];
+        VC = vl[(unsigned)x];
+        VM = vl[cx[1]];
+        gm = (G.ghm2 + X * z)[(unsigned)x];
+        gc = (G.gh + X * z)[(unsigned)x];
+    };
+    auto plane = [&](int z, const float* Q, float VC, float VM, int gm, int gc) {
+        if (y <= gc) return;
+        float q[5], o[5];
+        float vm;
+        if (y > gm) {
+#pragma unroll
+            for (int s = 0; s < 5; s++) {
+                const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
+                q[s] = outp ? bcv(b.sides, Q[s], pv) : Q[s];
+                o[s] = outp ? (b.sides == BC_DIRICHLET ? 0.0f : 1.0f) : 1.0f;
+            }
+            vm = x == 0 ? bcv(b.sides, VC, cv) : VM;
+        } else {
+            const int* gz = G.gh + X * z;
+            int gh[5];
+#pragma unroll
+            for (int s = 0; s < 5; s++) gh[s] = gz[cx[s]];
+#pragma unroll
+            for (int s = 0; s < 5; s++) {
+                const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
+                const bool gnd = y <= gh[s];
+                q[s] = nbv(Q[s], outp, gnd, pv, b);
+                o[s] = unitv(outp, gnd, b);
+            }
+            vm = x == 0 ? bcv(b.sides, VC, cv) : (y <= gh[1] ? bcv(b.ground, VC, cv) : VM);
+        }
+        (out + XY * z + rowy)[(unsigned)x] = ppmCell(q, o, VC, vm, dt, dx);
+    };
+    int zg = z0;
+    for (; zg + ZB <= z1; zg += ZB) {
         float Q[ZB][5], VC[ZB], VM[ZB];
-        int GH[ZB][5];
+        int GM[ZB], GC[ZB];
 #pragma unroll
-        for (int j = 0; j < ZB; j++) {
-            const int z = zg + j;
-            if (z < z1) {
-                const long pz = XY * z + rowy;
+        for (int j = 0; j < ZB; j++) load(zg + j, Q[j], VC[j], VM[j], GM[j], GC[j]);
 #pragma unroll
-                for (int s = 0; s < 5; s++) {
-                    Q[j][s] = in[pz + cx[s]];
-                    GH[j][s] = G.gh[cx[s] + X * z];
-                }
-                VC[j] = vel[pz + x];
-                VM[j] = vel[pz + cx[1]];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < ZB; j++) {
-            const int z = zg + j;
-            if (z < z1 && y > GH[j][2]) {
-                float q[5], o[5];
-#pragma unroll
-                for (int s = 0; s < 5; s++) {
-                    const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
-                    const bool gnd = y <= GH[j][s];
-                    q[s] = nbv(Q[j][s], outp, gnd, pv, b);
-                    o[s] = unitv(outp, gnd, b);
-                }
-                const float vc = VC[j];
-                const float vm = x == 0 ? bcv(b.sides, vc, cv) : (y <= GH[j][1] ? bcv(b.ground, vc, cv) : VM[j]);
-                out[XY * z + rowy + x] = ppmCell(q, o, vc, vm, dt, dx);
-            }
-        }
+        for (int j = 0; j < ZB; j++) plane(zg + j, Q[j], VC[j], VM[j], GM[j], GC[j]);
+    }
+    for (; zg < z1; zg++) {
+        float Q[5], VC, VM;
+        int gm, gc;
+        load(zg, Q, VC, VM, gm, gc);
+        plane(zg, Q, VC, VM, gm, gc);
     }
 }
 
-// y pass: the stencil runs along the column of this thread.
+// y pass: the stencil runs along the column of this thread. Cells with
+// y >= gh + 3 see no ground in the stencil.
 template <int ZB>
-KL void k_ppm_y(Geom G, int zc, const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ vel,
-                const float* __restrict__ prof, Bounds b, float dt, float dx) {
-    ZM_SETUP;
-    (void)bid;
+KL void k_ppm_y(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx) {
+    PPM_FIELD_SETUP;
     if (!inb) return;
     const int X = G.X, Y = G.Y;
     const long XY = (long)X * Y;
-    int ry[5];
+    unsigned ry[5];
     float pv[5];
 #pragma unroll
     for (int s = 0; s < 5; s++) {
@@ -324,93 +400,94 @@ KL void k_ppm_y(Geom G, int zc, const float* __restrict__ in, float* __restrict_
         ry[s] = x + X * by;
         pv[s] = prof ? prof[by] : 0.0f;
     }
-    for (int zg = z0; zg < z1; zg += ZB) {
+    auto load = [&](int z, float* Q, float& VC, float& VM, int& gc) {
+        const float* pl = in + XY * z;
+        const float* vl = vel + XY * z;
+#pragma unroll
+        for (int s = 0; s < 5; s++) Q[s] = pl[ry[s]];
+        VC = vl[ry[2]];
+        VM = vl[ry[1]];
+        gc = (G.gh + X * z)[(unsigned)x];
+    };
+    auto plane = [&](int z, const float* Q, float VC, float VM, int gh) {
+        if (y <= gh) return;
+        float q[5], o[5];
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+            const int py = y + s - 2;
+            const bool outp = (py < 0) || (py > Y - 1);
+            const bool gnd = clampi(py, 0, Y - 1) <= gh;
+            q[s] = nbv(Q[s], outp, gnd, pv[s], b);
+            o[s] = unitv(outp, gnd, b);
+        }
+        // y - 1 >= 0 on every sky cell
+        const float vm = (y - 1 <= gh) ? bcv(b.ground, VC, 0.0f) : VM;
+        (out + XY * z)[ry[2]] = ppmCell(q, o, VC, vm, dt, dx);
+    };
+    int zg = z0;
+    for (; zg + ZB <= z1; zg += ZB) {
         float Q[ZB][5], VC[ZB], VM[ZB];
-        int GH[ZB];
+        int GC[ZB];
 #pragma unroll
-        for (int j = 0; j < ZB; j++) {
-            const int z = zg + j;
-            if (z < z1) {
-                const long pz = XY * z;
+        for (int j = 0; j < ZB; j++) load(zg + j, Q[j], VC[j], VM[j], GC[j]);
 #pragma unroll
-                for (int s = 0; s < 5; s++) Q[j][s] = in[pz + ry[s]];
-                VC[j] = vel[pz + ry[2]];
-                VM[j] = vel[pz + ry[1]];
-                GH[j] = G.gh[x + X * z];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < ZB; j++) {
-            const int z = zg + j;
-            const int gh = GH[j];
-            if (z < z1 && y > gh) {
-                float q[5], o[5];
-#pragma unroll
-                for (int s = 0; s < 5; s++) {
-                    const int py = y + s - 2;
-                    const bool outp = (py < 0) || (py > Y - 1);
-                    const bool gnd = clampi(py, 0, Y - 1) <= gh;
-                    q[s] = nbv(Q[j][s], outp, gnd, pv[s], b);
-                    o[s] = unitv(outp, gnd, b);
-                }
-                const float vc = VC[j];
-                // y - 1 >= 0 on every sky cell
-                const float vm = (y - 1 <= gh) ? bcv(b.ground, vc, 0.0f) : VM[j];
-                out[XY * z + ry[2]] = ppmCell(q, o, vc, vm, dt, dx);
-            }
-        }
+        for (int j = 0; j < ZB; j++) plane(zg + j, Q[j], VC[j], VM[j], GC[j]);
+    }
+    for (; zg < z1; zg++) {
+        float Q[5], VC, VM;
+        int gc;
+        load(zg, Q, VC, VM, gc);
+        plane(zg, Q, VC, VM, gc);
     }
 }
 
 // z pass: the stencil runs along this thread's march; a window of planes
-// zg-2 .. zg+ZB+1 stays in registers.
+// zg-2 .. zg+ZB+1 (values and ground heights) stays in registers, so the
+// boundary rule needs no extra loads.
 template <int ZB>
-KL void k_ppm_z(Geom G, int zc, const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ vel,
-                const float* __restrict__ prof, Bounds b, float dt, float dx, const float* __restrict__ cust) {
-    ZM_SETUP;
-    (void)bid;
+KL void k_ppm_z(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, const float* __restrict__ cust) {
+    PPM_FIELD_SETUP;
     if (!inb || z0 >= z1) return;
     const int X = G.X, Z = G.Z;
     const long XY = (long)X * G.Y;
-    const int rc = x + X * y;
+    const unsigned rc = x + X * y;
     const float pv = prof ? prof[y] : 0.0f, cv = cust[y];
     float qw[ZB + 4], vw[ZB + 1];
     int gw[ZB + 4];
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         const int bz = clampi(z0 - 2 + s, 0, Z - 1);
-        qw[s] = in[XY * bz + rc];
-        gw[s] = G.gh[x + X * bz];
+        qw[s] = (in + XY * bz)[rc];
+        gw[s] = (G.gh + X * bz)[(unsigned)x];
     }
-    vw[0] = vel[XY * max(z0 - 1, 0) + rc];
-    for (int zg = z0; zg < z1; zg += ZB) {
+    vw[0] = (vel + XY * max(z0 - 1, 0))[rc];
+    // window slot m holds plane zg - 2 + m; velocity slot m plane zg - 1 + m
+    auto load = [&](int z, float& q, int& g, float& v) {
+        const int bz = min(z + 2, Z - 1);
+        q = (in + XY * bz)[rc];
+        g = (G.gh + X * bz)[(unsigned)x];
+        v = (vel + XY * z)[rc];
+    };
+    auto plane = [&](int z, const float* qs, const int* gs, float vc, float vmr) {
+        if (y <= gs[2]) return;
+        float q[5], o[5];
 #pragma unroll
-        for (int j = 0; j < ZB; j++) {
-            if (zg + j < z1) {
-                const int bz = min(zg + j + 2, Z - 1);
-                qw[4 + j] = in[XY * bz + rc];
-                gw[4 + j] = G.gh[x + X * bz];
-                vw[1 + j] = vel[XY * (zg + j) + rc];
-            }
+        for (int s = 0; s < 5; s++) {
+            const int pz = z + s - 2;
+            const bool outp = (pz < 0) || (pz > Z - 1);
+            const bool gnd = y <= gs[s];
+            q[s] = nbv(qs[s], outp, gnd, pv, b);
+            o[s] = unitv(outp, gnd, b);
         }
+        const float vm = z == 0 ? bcv(b.sides, vc, cv) : (y <= gs[1] ? bcv(b.ground, vc, cv) : vmr);
+        (out + XY * z)[rc] = ppmCell(q, o, vc, vm, dt, dx);
+    };
+    int zg = z0;
+    for (; zg + ZB <= z1; zg += ZB) {
 #pragma unroll
-        for (int j = 0; j < ZB; j++) {
-            const int z = zg + j;
-            if (z < z1 && y > gw[j + 2]) {
-                float q[5], o[5];
+        for (int j = 0; j < ZB; j++) load(zg + j, qw[4 + j], gw[4 + j], vw[1 + j]);
 #pragma unroll
-                for (int s = 0; s < 5; s++) {
-                    const int pz = z + s - 2;
-                    const bool outp = (pz < 0) || (pz > Z - 1);
-                    const bool gnd = y <= gw[j + s];
-                    q[s] = nbv(qw[j + s], outp, gnd, pv, b);
-                    o[s] = unitv(outp, gnd, b);
-                }
-                const float vc = vw[1 + j];
-                const float vm = z == 0 ? bcv(b.sides, vc, cv) : (y <= gw[j + 1] ? bcv(b.ground, vc, cv) : vw[j]);
-                out[XY * z + rc] = ppmCell(q, o, vc, vm, dt, dx);
-            }
-        }
+        for (int j = 0; j < ZB; j++) plane(zg + j, qw + j, gw + j, vw[1 + j], vw[j]);
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             qw[s] = qw[s + ZB];
@@ -418,18 +495,28 @@ KL void k_ppm_z(Geom G, int zc, const float* __restrict__ in, float* __restrict_
         }
         vw[0] = vw[ZB];
     }
+    for (; zg < z1; zg++) {
+        load(zg, qw[4], gw[4], vw[1]);
+        plane(zg, qw, gw, vw[1], vw[0]);
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            qw[s] = qw[s + 1];
+            gw[s] = gw[s + 1];
+        }
+        vw[0] = vw[1];
+    }
 }
 
-hipError_t launch_ppm_pass(EnvState& e, const float* in, float* out, const float* vel, const float* prof, Bounds b,
-                           int dir, float dt) {
+hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) {
     const ZGrid zg = zgrid(e);
+    const dim3 grid(zg.nblocks * pb.n);
     if (dir == 0)
-        hipLaunchKernelGGL(k_ppm_x<ZB_PPM>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, in, out, vel, prof, b, dt, e.dx,
+        hipLaunchKernelGGL(k_ppm_x<ZB_PPM>, grid, B3, 0, e.stream, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
                            e.f[SKT_DEFAULT_VELX]);
     else if (dir == 1)
-        hipLaunchKernelGGL(k_ppm_y<ZB_PPM>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, in, out, vel, prof, b, dt, e.dx);
+        hipLaunchKernelGGL(k_ppm_y<ZB_PPM>, grid, B3, 0, e.stream, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx);
     else
-        hipLaunchKernelGGL(k_ppm_z<ZB_PPM>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, in, out, vel, prof, b, dt, e.dx,
+        hipLaunchKernelGGL(k_ppm_z<ZB_PPM>, grid, B3, 0, e.stream, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
                            e.f[SKT_DEFAULT_VELZ]);
     return hipGetLastError();
 }
@@ -641,94 +728,141 @@ hipError_t launch_pcg_start(EnvState& e) {
 // Prologue PRO_A0 / PRO_A, then the search-direction update (endIteration,
 // Game/kernelSky.cu:987-1014) fused into applyAGPU (:719-778) and the s.As
 // partials (:675-717): s_new = z + beta s_old (s_new = z on the first
-// iteration), As = A s_new. s_new of the x/y neighbours is recomputed from
-// their z and s_old; along z it stays in registers.
-constexpr int ZB_PCGA = 2;
-template <int ZB>
-KL void k_pcg_A(Geom G, int zc, int it, const float* __restrict__ zz, const float* __restrict__ s_old,
+// iteration), As = A s_new. s_new of the in-plane neighbours and their A.x /
+// A.y come from the LDS tile; along z they stay in registers. Cells with
+// y >= ghm1 + 2 have only sky (or the domain edge) around them.
+template <bool FIRST>
+KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __restrict__ s_old,
                 float* __restrict__ s_new, const float4* __restrict__ A, float* __restrict__ As,
                 const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, double* pout,
-                const PcgState* sin, PcgState* sout) {
+                const PcgState* sin, PcgState* sout, int it) {
     ZM_SETUP;
     __shared__ double sh[4];
     __shared__ PcgState S;
-    pcgPrologue(it == 0 ? PRO_A0 : PRO_A, pm, ps, n, sin, sout, 0, 0.0f, it, S, sh);
+    __shared__ float tS[2][TH][TW], tAx[2][TH][TW], tAy[2][TH][TW];
+    pcgPrologue(FIRST ? PRO_A0 : PRO_A, pm, ps, n, sin, sout, 0, 0.0f, it, S, sh);
     if (S.done) return;
-    const bool first = it == 0;
     const float beta = S.beta;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int X = G.X, Y = G.Y, Z = G.Z;
+    const long XY = (long)X * Y;
+    const TileRole t = tileRole(G, x, y);
+    auto snv = [&](float zv, float sv) { return FIRST ? zv : zv + beta * sv; };
+    struct Own {
+        float z, s;
+        float4 a;
+        int gm, gf;
+    };
+    struct Halo {
+        float dz, ds, day, uz, us, lz, ls, lax, rz, rs;
+    };
+    auto loadOwn = [&](int pl, Own& o) {
+        if (t.own && pl < Z) {
+            const long pz = XY * pl;
+            o.z = (zz + pz)[t.rc];
+            if (!FIRST) o.s = (s_old + pz)[t.rc];
+            o.a = (A + pz)[t.rc];
+            o.gm = (G.ghm1 + X * pl)[(unsigned)x];
+            o.gf = (G.gh + X * min(pl + 1, Z - 1))[(unsigned)x];
+        }
+    };
+    auto loadHalo = [&](int pl, Halo& h) {
+        const long pz = XY * pl;
+        if (t.hasD) {
+            h.dz = (zz + pz)[t.rD];
+            if (!FIRST) h.ds = (s_old + pz)[t.rD];
+            h.day = (A + pz)[t.rD].y;
+        }
+        if (t.hasU) {
+            h.uz = (zz + pz)[t.rU];
+            if (!FIRST) h.us = (s_old + pz)[t.rU];
+        }
+        if (t.hasL) {
+            h.lz = (zz + pz)[t.rL];
+            if (!FIRST) h.ls = (s_old + pz)[t.rL];
+            h.lax = (A + pz)[t.rL].x;
+        }
+        if (t.hasR) {
+            h.rz = (zz + pz)[t.rR];
+            if (!FIRST) h.rs = (s_old + pz)[t.rR];
+        }
+    };
+    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    Own oc{0.0f, 0.0f, z4, 0, 0}, of{0.0f, 0.0f, z4, 0, 0}, on{0.0f, 0.0f, z4, 0, 0};
+    Halo hc{}, hn{};
+    // plane below the chunk: s_new and A.z
+    float snb = 0.0f, azb = 0.0f;
+    int ghc = 0;
+    if (t.own) {
+        if (z0 > 0 && y > (G.gh + X * (z0 - 1))[(unsigned)x]) {
+            const long pz = XY * (z0 - 1);
+            snb = snv((zz + pz)[t.rc], FIRST ? 0.0f : (s_old + pz)[t.rc]);
+            azb = (A + pz)[t.rc].z;
+        }
+        ghc = (G.gh + X * z0)[(unsigned)x];
+    }
+    loadOwn(z0, oc);
+    loadOwn(z0 + 1, of);
+    loadHalo(z0, hc);
+    bool skC = t.own && y > ghc;
     double acc = 0.0;
-    if (inb && z0 < z1) {
-        const int X = G.X, Y = G.Y, Z = G.Z;
-        const long XY = (long)X * Y;
-        const int xl = max(x - 1, 0), xr = min(x + 1, X - 1), yd = max(y - 1, 0), yu = min(y + 1, Y - 1);
-        const int rc = x + X * y, rl = xl + X * y, rr = xr + X * y, rd = x + X * yd, ru = x + X * yu;
-        auto sn = [&](long j) { return first ? zz[j] : zz[j] + beta * s_old[j]; };
-        // column window: s_new and A.z of the plane below, s_new of this plane
-        const bool skB = z0 > 0 && y > G.gh[x + X * (z0 - 1)];
-        float snb = 0.0f, azb = 0.0f;
-        if (skB) {
-            snb = sn(XY * (z0 - 1) + rc);
-            azb = A[XY * (z0 - 1) + rc].z;
+    for (int z = z0; z < z1; z++) {
+        const int buf = (z - z0) & 1;
+        if (z + 2 <= z1) loadOwn(z + 2, on);
+        if (z + 1 < z1) loadHalo(z + 1, hn);
+        const float snc = skC ? snv(oc.z, oc.s) : 0.0f;
+        tS[buf][ty + 1][tx + 1] = snc;
+        tAx[buf][ty + 1][tx + 1] = oc.a.x;
+        tAy[buf][ty + 1][tx + 1] = oc.a.y;
+        if (t.hasD) {
+            tS[buf][0][tx + 1] = snv(hc.dz, hc.ds);
+            tAy[buf][0][tx + 1] = hc.day;
         }
-        int ghc = G.gh[x + X * z0];
-        bool skC = y > ghc;
-        float snc = skC ? sn(XY * z0 + rc) : 0.0f;
-        for (int zg = z0; zg < z1; zg += ZB) {
-            float SL[ZB], SR[ZB], SD[ZB], SU[ZB], SF[ZB], AL[ZB], AD[ZB];
-            float4 AC[ZB];
-            int GL[ZB], GR[ZB], GF[ZB];
-#pragma unroll
-            for (int j = 0; j < ZB; j++) {
-                const int z = zg + j;
-                if (z < z1) {
-                    const long pz = XY * z;
-                    const int zf = min(z + 1, Z - 1);
-                    SL[j] = sn(pz + rl);
-                    SR[j] = sn(pz + rr);
-                    SD[j] = sn(pz + rd);
-                    SU[j] = sn(pz + ru);
-                    SF[j] = sn(XY * zf + rc);
-                    AC[j] = A[pz + rc];
-                    AL[j] = A[pz + rl].x;
-                    AD[j] = A[pz + rd].y;
-                    GL[j] = G.gh[xl + X * z];
-                    GR[j] = G.gh[xr + X * z];
-                    GF[j] = G.gh[x + X * zf];
+        if (t.hasU) tS[buf][5][tx + 1] = snv(hc.uz, hc.us);
+        if (t.hasL) {
+            tS[buf][ty + 1][0] = snv(hc.lz, hc.ls);
+            tAx[buf][ty + 1][0] = hc.lax;
+        }
+        if (t.hasR) tS[buf][ty + 1][65] = snv(hc.rz, hc.rs);
+        __syncthreads();
+        const bool skF = t.own && z + 1 < Z && y > oc.gf;
+        const float snf = skF ? snv(of.z, of.s) : 0.0f;
+        float azc = 0.0f;
+        if (t.own) {
+            float* sw = s_new + XY * z;
+            float* aw = As + XY * z;
+            if (skC) {
+                bool sl = x > 0, sr = x < X - 1, sd = true;
+                if (y < oc.gm + 2) {
+                    const int* gz = G.gh + X * z;
+                    sl = sl && y > gz[x - 1];
+                    sr = sr && y > gz[x + 1];
+                    sd = y - 1 > ghc;
                 }
-            }
-#pragma unroll
-            for (int j = 0; j < ZB; j++) {
-                const int z = zg + j;
-                if (z < z1) {
-                    const long i = XY * z + rc;
-                    const bool skF = z + 1 < Z && y > GF[j];
-                    const float snf = skF ? SF[j] : 0.0f;
-                    float azc = 0.0f;
-                    if (skC) {
-                        const float4 Ac = AC[j];
-                        const bool sl = x > 0 && y > GL[j], sr = x < X - 1 && y > GR[j];
-                        const bool sd = y - 1 > ghc, su = y + 1 < Y;
-                        const float l = sl ? SL[j] : 0.0f, r = sr ? SR[j] : 0.0f;
-                        const float d = sd ? SD[j] : 0.0f, u = su ? SU[j] : 0.0f;
-                        const float aL = sl ? trunc8(AL[j]) : 0.0f, aD = sd ? trunc8(AD[j]) : 0.0f;
-                        const float aB = trunc8(azb);
-                        const float o = Ac.w * snc + ((aL * l + aD * d + aB * snb + Ac.x * r + Ac.y * u + Ac.z * snf));
-                        azc = Ac.z;
-                        s_new[i] = snc;
-                        As[i] = o;
-                        acc += (double)(o * snc);
-                    } else {
-                        s_new[i] = 0.0f;
-                        As[i] = 0.0f;
-                    }
-                    snb = snc;
-                    azb = azc;
-                    snc = snf;
-                    skC = skF;
-                    ghc = GF[j];
-                }
+                const bool su = y + 1 < Y;
+                const float l = sl ? tS[buf][ty + 1][tx] : 0.0f, r = sr ? tS[buf][ty + 1][tx + 2] : 0.0f;
+                const float d = sd ? tS[buf][ty][tx + 1] : 0.0f, u = su ? tS[buf][ty + 2][tx + 1] : 0.0f;
+                const float aL = sl ? trunc8(tAx[buf][ty + 1][tx]) : 0.0f;
+                const float aD = sd ? trunc8(tAy[buf][ty][tx + 1]) : 0.0f;
+                const float aB = trunc8(azb);
+                const float4 Ac = oc.a;
+                const float o = Ac.w * snc + ((aL * l + aD * d + aB * snb + Ac.x * r + Ac.y * u + Ac.z * snf));
+                azc = Ac.z;
+                sw[t.rc] = snc;
+                aw[t.rc] = o;
+                acc += (double)(o * snc);
+            } else {
+                sw[t.rc] = 0.0f;
+                aw[t.rc] = 0.0f;
             }
         }
+        snb = snc;
+        azb = azc;
+        skC = skF;
+        ghc = oc.gf;
+        oc = of;
+        of = on;
+        hc = hn;
     }
     acc = blockSum(acc, sh);
     if (tid256() == 0) pout[bid] = acc;
@@ -740,8 +874,12 @@ hipError_t launch_pcg_A(EnvState& e, int it) {
     const size_t M = e.maxBlocks;
     const double* pm = e.partials + PART_BMAX * M;
     const double* ps = it == 0 ? e.partials + PART_START * M : e.partials + PART_BSUM * M;
-    hipLaunchKernelGGL(k_pcg_A<ZB_PCGA>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, it, e.pcg[1], s_old, s_new, e.A,
-                       e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0]);
+    if (it == 0)
+        hipLaunchKernelGGL(k_pcg_A<true>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.A,
+                           e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it);
+    else
+        hipLaunchKernelGGL(k_pcg_A<false>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.A,
+                           e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it);
     return hipGetLastError();
 }
 
@@ -763,41 +901,44 @@ KL void k_pcg_B(Geom G, int zc, float* __restrict__ pp, float* __restrict__ r, c
     double m = 0.0, dz = 0.0;
     if (inb) {
         const long XY = (long)G.X * G.Y;
-        const int rc = x + G.X * y;
-        for (int zg = z0; zg < z1; zg += ZB) {
-            float P[ZB], Rv[ZB], Sv[ZB], Av[ZB], Pr[ZB];
+        const unsigned rc = x + G.X * y;
+        auto load = [&](int z, float& P, float& R, float& Sv, float& Av, float& Pr, int& gh) {
+            const long pz = XY * z;
+            P = (pp + pz)[rc];
+            R = (r + pz)[rc];
+            Sv = (s + pz)[rc];
+            Av = (As + pz)[rc];
+            Pr = (pre + pz)[rc];
+            gh = (G.gh + G.X * z)[(unsigned)x];
+        };
+        auto plane = [&](int z, float P, float R, float Sv, float Av, float Pr, int gh) {
+            const long pz = XY * z;
+            float zv = 0.0f;
+            if (y > gh) {
+                (pp + pz)[rc] = P + alpha * Sv;
+                const float rv = R - alpha * Av;
+                (r + pz)[rc] = rv;
+                const bool ab = absmode || z == 0;
+                m = fmax(m, (double)(ab ? fabsf(rv) : rv));
+                zv = rv * Pr;
+                dz += (double)(zv * rv);
+            }
+            (zz + pz)[rc] = zv;
+        };
+        int zg = z0;
+        for (; zg + ZB <= z1; zg += ZB) {
+            float P[ZB], R[ZB], Sv[ZB], Av[ZB], Pr[ZB];
             int gh[ZB];
 #pragma unroll
-            for (int j = 0; j < ZB; j++) {
-                const int z = zg + j;
-                if (z < z1) {
-                    const long i = XY * z + rc;
-                    P[j] = pp[i];
-                    Rv[j] = r[i];
-                    Sv[j] = s[i];
-                    Av[j] = As[i];
-                    Pr[j] = pre[i];
-                    gh[j] = G.gh[x + G.X * z];
-                }
-            }
+            for (int j = 0; j < ZB; j++) load(zg + j, P[j], R[j], Sv[j], Av[j], Pr[j], gh[j]);
 #pragma unroll
-            for (int j = 0; j < ZB; j++) {
-                const int z = zg + j;
-                if (z < z1) {
-                    const long i = XY * z + rc;
-                    float zv = 0.0f;
-                    if (y > gh[j]) {
-                        pp[i] = P[j] + alpha * Sv[j];
-                        const float rv = Rv[j] - alpha * Av[j];
-                        r[i] = rv;
-                        const bool ab = absmode || z == 0;
-                        m = fmax(m, (double)(ab ? fabsf(rv) : rv));
-                        zv = rv * Pr[j];
-                        dz += (double)(zv * rv);
-                    }
-                    zz[i] = zv;
-                }
-            }
+            for (int j = 0; j < ZB; j++) plane(zg + j, P[j], R[j], Sv[j], Av[j], Pr[j], gh[j]);
+        }
+        for (; zg < z1; zg++) {
+            float P, R, Sv, Av, Pr;
+            int gh;
+            load(zg, P, R, Sv, Av, Pr, gh);
+            plane(zg, P, R, Sv, Av, Pr, gh);
         }
     }
     m = blockMax(m, sh);

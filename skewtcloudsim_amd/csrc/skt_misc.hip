@@ -199,46 +199,62 @@ hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed,
 // advectPrecipGPU, Game/kernelSky.cu:576-657: one thread per column walks y
 // upwards; each level reads its own and the next level's pre-step value
 // (the column snapshot of the reference), so the update can be in place.
-KL void k_precip(Geom G, float* q, const float* __restrict__ th, const float* __restrict__ p,
-                 const float* __restrict__ qv, const float* __restrict__ gP, int type, float dts, float dx, Gammas gm) {
+KL void k_precip(Geom G, PrecipBatch pbt, const float* __restrict__ th, const float* __restrict__ p,
+                 const float* __restrict__ qv, const float* __restrict__ gP, float dx, Gammas gm) {
     const long g = blockIdx.x * 256L + threadIdx.x;
-    if (g >= (long)G.X * G.Z) return;
+    const float dts = pbt.dts[blockIdx.y];
+    if (g >= (long)G.X * G.Z || dts == 0.0f) return;
+    float* q = pbt.q[blockIdx.y];
+    const int type = pbt.type[blockIdx.y];
     const int x = (int)(g % G.X), zc = (int)(g / G.X);
     const float kap = kc::Rsd / kc::Cpd;
     const float gp = gP[g];
-    auto fvAt = [&](long j, float Qj) {
-        const float T = th[j] * powf(p[j] / gp, kap);
-        const float Tv = T * (0.608f * qv[j] + 1);
-        const float den = p[j] * 100 / (kc::Rsd * Tv);
-        const float3 v3 = d_fallVel(Qj, Qj, Qj, den, type, gm.R48, gm.S425, gm.S425);
+    // per level: Q, theta, p, qv; levels y+1 and y+2 are loaded ahead
+    struct Lv {
+        float q, t, pr, v;
+    };
+    auto fvAt = [&](const Lv& L) {
+        const float T = L.t * powf(L.pr / gp, kap);
+        const float Tv = T * (0.608f * L.v + 1);
+        const float den = L.pr * 100 / (kc::Rsd * Tv);
+        const float3 v3 = d_fallVel(L.q, L.q, L.q, den, type, gm.R48, gm.S425, gm.S425);
         return type == 0 ? v3.x : (type == 1 ? v3.y : v3.z);
     };
     int y0 = G.gh[g] + 1;
     if (y0 < 0) y0 = 0;
     if (y0 >= G.Y) return;
-    long i = gidx(G, x, y0, zc);
-    float Qc = q[i];
-    float fv = fvAt(i, Qc);
-    for (int y = y0; y < G.Y; y++) {
-        const bool up = y < G.Y - 1;
-        float Qu = 0.0f, fvu = 0.0f;
-        const long iu = i + G.X;
-        if (up) {
-            Qu = q[iu];
-            fvu = fvAt(iu, Qu);
+    const long i0 = gidx(G, x, y0, zc);
+    auto load = [&](int y, Lv& L) {
+        if (y < G.Y) {
+            const long j = i0 + (long)(y - y0) * G.X;
+            L.q = q[j];
+            L.t = th[j];
+            L.pr = p[j];
+            L.v = qv[j];
         }
+    };
+    Lv l0{0.0f, 0.0f, 0.0f, 0.0f}, l1 = l0, l2 = l0;
+    load(y0, l0);
+    load(y0 + 1, l1);
+    float fv = fvAt(l0);
+    long i = i0;
+    for (int y = y0; y < G.Y; y++, i += G.X) {
+        load(y + 2, l2);
+        const bool up = y < G.Y - 1;
+        const float Qc = l0.q, Qu = up ? l1.q : 0.0f;
+        const float fvu = up ? fvAt(l1) : 0.0f;
         float nq = Qc;
         nq -= dts * fminf((fv / dx) * Qc, Qc);
         if (up) nq += dts * fminf((fvu / dx) * Qu, Qu);
         q[i] = nq;
-        Qc = Qu;
         fv = fvu;
-        i = iu;
+        l0 = l1;
+        l1 = l2;
     }
 }
-hipError_t launch_precip(EnvState& e, float* q, int type, float dts) {
-    hipLaunchKernelGGL(k_precip, grid1(e.G), dim3(256), 0, e.stream, geom(e), q, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE],
-                       e.f[SKT_QV], e.f[SKT_GROUND_P], type, dts, e.dx, e.gm);
+hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt) {
+    hipLaunchKernelGGL(k_precip, dim3((unsigned)((e.G + 255) / 256), pbt.n), dim3(256), 0, e.stream, geom(e), pbt,
+                       e.f[SKT_POTTEMP], e.f[SKT_PRESSURE], e.f[SKT_QV], e.f[SKT_GROUND_P], e.dx, e.gm);
     return hipGetLastError();
 }
 

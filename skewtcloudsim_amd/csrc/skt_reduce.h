@@ -67,6 +67,16 @@ __device__ __forceinline__ int xcdRemap(int b, int nb) {
     const int z1 = min(G.Z, z0 + zc);                              \
     const bool inb = (x < G.X) && (y < G.Y)
 
+// The same for a block whose tile index (already remapped) is given.
+#define ZM_SETUP_B(blin)                                           \
+    const int gx_ = (G.X + 63) >> 6, gy_ = (G.Y + 3) >> 2;         \
+    const int bid = (int)(blin);                                   \
+    const int x = (bid % gx_) * 64 + threadIdx.x;                  \
+    const int y = ((bid / gx_) % gy_) * 4 + threadIdx.y;           \
+    const int z0 = (bid / (gx_ * gy_)) * zc;                       \
+    const int z1 = min(G.Z, z0 + zc);                              \
+    const bool inb = (x < G.X) && (y < G.Y)
+
 static inline dim3 grid3(const EnvState& e) { return dim3((e.X + 63) / 64, (e.Y + 3) / 4, e.Z); }
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 static const dim3 B3(64, 4, 1);
