@@ -171,16 +171,17 @@ int skt_mphys_region_clear(skt_env* env);
 int skt_mphys_region_get(skt_env* env, float out[33]);
 
 /* Measurement (no counterpart in the reference, whose cudaEvent timing is
- * commented out at Game/environment.cu:698-702,816-823). With timing enabled
- * every launch of the step is bracketed by HIP events on the env's stream and
- * accumulated per kernel class: 0 diffusion sweep, 1 PPM pass, 2 PCG
- * search-direction + SpMV, 3 PCG update + stop test, 4 microphysics,
- * 5 max-velocity reduction, 6 precipitation fall, 7 everything else. */
+ * commented out at Game/environment.cu:698-702,816-823). Every launch of a
+ * kernel class selected in class_mask (bit k = class k; 0 = off, 0xFF = all)
+ * is bracketed by HIP events on the env's stream and accumulated per class:
+ * 0 diffusion sweep, 1 PPM pass, 2 PCG search-direction + SpMV, 3 PCG update
+ * + stop test, 4 microphysics, 5 max-velocity reduction, 6 precipitation
+ * fall, 7 everything else. */
 typedef struct skt_timing {
     int64_t launches[8];
     double total_ms[8];
 } skt_timing;
-int skt_env_set_timing(skt_env* env, int32_t enable); /* also clears the accumulators */
+int skt_env_set_timing(skt_env* env, int32_t class_mask); /* also clears the accumulators */
 int skt_env_get_timing(skt_env* env, skt_timing* out);
 /* Times a plain device copy of one sky field (nx*ny*nz floats) 'repeats'
  * times: the measured copy ceiling for roofline reporting. */

@@ -1018,10 +1018,12 @@ template <class R> void buildA(Env<R>& e, Mat<R>& M) {
 }
 
 // applyAGPU, Game/kernelSky.cu:719-778 with bounds {D,D,D}: lower-triangle
-// coefficients are the neighbour's upper coefficient truncated to char.
-// Outside / ground neighbours contribute 0 (their Dirichlet value is 0; the
-// reference multiplies 0 by an out-of-row or uninitialised coefficient, A2).
-template <class R> void applyA(const Env<R>& e, const Mat<R>& M, const R* s, R* out) {
+// coefficients are the neighbour's upper coefficient truncated to char
+// (trunc = true, the reference; quirk A1) or taken as is (trunc = false: the
+// symmetric operator the standard solve needs). Outside / ground neighbours
+// contribute 0 (their Dirichlet value is 0; the reference multiplies 0 by an
+// out-of-row or uninitialised coefficient, A2).
+template <class R> void applyA(const Env<R>& e, const Mat<R>& M, const R* s, R* out, bool trunc = true) {
 #pragma omp parallel for schedule(static)
     for (int z = 0; z < e.Z; z++)
         for (int y = 0; y < e.Y; y++)
@@ -1037,9 +1039,10 @@ template <class R> void applyA(const Env<R>& e, const Mat<R>& M, const R* s, R* 
                 const R d = sd ? s[i - e.X] : R(0), u = su ? s[i + e.X] : R(0);
                 const long XY = (long)e.X * e.Y;
                 const R bb = sb ? s[i - XY] : R(0), f = sf ? s[i + XY] : R(0);
-                const R AL = sl ? trunc8(M.Ax[i - 1]) : R(0);
-                const R AD = sd ? trunc8(M.Ay[i - e.X]) : R(0);
-                const R AB_ = sb ? trunc8(M.Az[i - XY]) : R(0);
+                auto lo = [&](R c) { return trunc ? trunc8(c) : c; };
+                const R AL = sl ? lo(M.Ax[i - 1]) : R(0);
+                const R AD = sd ? lo(M.Ay[i - e.X]) : R(0);
+                const R AB_ = sb ? lo(M.Az[i - XY]) : R(0);
                 out[i] = M.Aw[i] * s[i] + ((AL * l + AD * d + AB_ * bb + M.Ax[i] * r + M.Ay[i] * u + M.Az[i] * f));
             }
 }
@@ -1099,7 +1102,8 @@ template <class R> void divergence(Env<R>& e, R* div, R dt) {
 // the second iteration on, so it carries the previous z.r (alpha =
 // sigma0/(sigma0 + s.As)); the stop test is the getMaxDivergence value
 // (quirk A6) against an absolute tolerance. Standard mode is textbook
-// Jacobi-PCG with max|r| <= tol*max|r0|.
+// Jacobi-PCG on the symmetric operator (lower triangle not truncated) with
+// max|r| <= tol*max|r0|.
 template <class R> void pcgSolve(Env<R>& e, const Mat<R>& M, R* pp, R dt) {
     const long N = e.N;
     std::vector<R> r((size_t)N), z((size_t)N, R(0)), s((size_t)N, R(0)), As((size_t)N, R(0));
@@ -1137,7 +1141,7 @@ template <class R> void pcgSolve(Env<R>& e, const Mat<R>& M, R* pp, R dt) {
     R sigma0 = R(dotSky(e, z.data(), r.data()));
     R sigma1 = R(0);
     for (int it = 0; it < e.o.pcg_max_iter; it++) {
-        applyA(e, M, s.data(), As.data());
+        applyA(e, M, s.data(), As.data(), ref);
         const double sAs = dotSky(e, As.data(), s.data());
         sigma1 = ref ? R((double)sigma1 + sAs) : R(sAs);
         const R alpha = sigma0 / sigma1;

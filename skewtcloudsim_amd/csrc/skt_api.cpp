@@ -49,7 +49,7 @@ hipEvent_t evGet(EnvState& e) {
 #define TL(e, cls, call)                                                  \
     do {                                                                  \
         hipEvent_t a_ = nullptr, b_ = nullptr;                            \
-        if ((e).timing) {                                                 \
+        if ((e).timing & (1 << (cls))) {                                  \
             a_ = evGet(e);                                                \
             b_ = evGet(e);                                                \
             if (a_) (void)hipEventRecord(a_, (e).stream);                 \
@@ -730,11 +730,11 @@ int skt_env_phase_mask(skt_env* env, uint8_t* host) {
     HIPCHK(e, hipStreamSynchronize(e.stream));
     return SKT_OK;
 }
-int skt_env_set_timing(skt_env* env, int32_t enable) {
+int skt_env_set_timing(skt_env* env, int32_t class_mask) {
     if (!env) return SKT_ERR_ARG;
     EnvState& e = env->s;
     RC(resolveTiming(e));
-    e.timing = enable ? 1 : 0;
+    e.timing = class_mask & 0xFF;
     for (int k = 0; k < 8; k++) {
         e.t_ms[k] = 0.0;
         e.t_count[k] = 0;

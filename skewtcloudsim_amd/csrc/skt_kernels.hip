@@ -735,7 +735,7 @@ template <bool FIRST>
 KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __restrict__ s_old,
                 float* __restrict__ s_new, const float4* __restrict__ A, float* __restrict__ As,
                 const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, double* pout,
-                const PcgState* sin, PcgState* sout, int it) {
+                const PcgState* sin, PcgState* sout, int it, int trunc) {
     ZM_SETUP;
     __shared__ double sh[4];
     __shared__ PcgState S;
@@ -842,9 +842,12 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
                 const bool su = y + 1 < Y;
                 const float l = sl ? tS[buf][ty + 1][tx] : 0.0f, r = sr ? tS[buf][ty + 1][tx + 2] : 0.0f;
                 const float d = sd ? tS[buf][ty][tx + 1] : 0.0f, u = su ? tS[buf][ty + 2][tx + 1] : 0.0f;
-                const float aL = sl ? trunc8(tAx[buf][ty + 1][tx]) : 0.0f;
-                const float aD = sd ? trunc8(tAy[buf][ty][tx + 1]) : 0.0f;
-                const float aB = trunc8(azb);
+                // reference mode: char-truncated lower triangle (quirk A1);
+                // standard mode: the symmetric operator
+                auto lo = [&](float c) { return trunc ? trunc8(c) : c; };
+                const float aL = sl ? lo(tAx[buf][ty + 1][tx]) : 0.0f;
+                const float aD = sd ? lo(tAy[buf][ty][tx + 1]) : 0.0f;
+                const float aB = lo(azb);
                 const float4 Ac = oc.a;
                 const float o = Ac.w * snc + ((aL * l + aD * d + aB * snb + Ac.x * r + Ac.y * u + Ac.z * snf));
                 azc = Ac.z;
@@ -874,12 +877,15 @@ hipError_t launch_pcg_A(EnvState& e, int it) {
     const size_t M = e.maxBlocks;
     const double* pm = e.partials + PART_BMAX * M;
     const double* ps = it == 0 ? e.partials + PART_START * M : e.partials + PART_BSUM * M;
+    const int trunc = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     if (it == 0)
         hipLaunchKernelGGL(k_pcg_A<true>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.A,
-                           e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it);
+                           e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it,
+                           trunc);
     else
         hipLaunchKernelGGL(k_pcg_A<false>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.A,
-                           e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it);
+                           e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it,
+                           trunc);
     return hipGetLastError();
 }
 
