@@ -257,7 +257,7 @@ int solvePressure(EnvState& e, float dt) {
 // the same pass.
 int pressureProject(EnvState& e, float dt, int new_pressure) {
     TL(e, TC_OTHER, launch_reset_vel(e));
-    TL(e, TC_OTHER, launch_build_A(e));
+    TL(e, TC_OTHER, launch_build_precon(e));
     int rc = solvePressure(e, dt);
     if (rc) return rc;
     TL(e, TC_OTHER, launch_update_pressure_density(e));
@@ -350,7 +350,7 @@ void freeAll(EnvState& e) {
     fr(e.gh);
     fr(e.ghm1);
     fr(e.ghm2);
-    fr(e.A);
+    fr(e.irho);
     fr(e.zeros);
     fr(e.ones);
     fr(e.partials);
@@ -427,7 +427,7 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     for (int k = 0; k < 6; k++)
         if (dalloc(e, &e.pcg[k], sizeof(float) * e.N)) return bail(SKT_ERR_NOMEM);
     if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.ghm1, sizeof(int) * e.G) ||
-        dalloc(e, &e.ghm2, sizeof(int) * e.G) || dalloc(e, &e.A, sizeof(float4) * e.N) ||
+        dalloc(e, &e.ghm2, sizeof(int) * e.G) || dalloc(e, &e.irho, sizeof(float) * e.N) ||
         dalloc(e, &e.zeros, sizeof(float) * e.Y) || dalloc(e, &e.ones, sizeof(float) * e.Y) ||
         dalloc(e, &e.phase, (size_t)e.N))
         return bail(SKT_ERR_NOMEM);
@@ -445,7 +445,7 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
         hipMemsetAsync(e.ghm2, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
         hipMemsetAsync(e.sc, 0, sizeof(Scalars), e.stream) != hipSuccess ||
         hipMemsetAsync(e.phase, 0, (size_t)e.N, e.stream) != hipSuccess ||
-        hipMemsetAsync(e.A, 0, sizeof(float4) * e.N, e.stream) != hipSuccess)
+        hipMemsetAsync(e.irho, 0, sizeof(float) * e.N, e.stream) != hipSuccess)
         return bail(SKT_ERR_HIP);
     for (int k = 0; k < 6; k++)
         if (hipMemsetAsync(e.pcg[k], 0, sizeof(float) * e.N, e.stream) != hipSuccess) return bail(SKT_ERR_HIP);
@@ -614,7 +614,7 @@ int skt_env_calculate_pressure_project(skt_env* env, float dt) {
     if (!env) return SKT_ERR_ARG;
     EnvState& e = env->s;
     HIPCHK(e, launch_reset_vel(e));
-    HIPCHK(e, launch_build_A(e));
+    HIPCHK(e, launch_build_precon(e));
     return solvePressure(e, dt);
 }
 int skt_env_editor_data(skt_env* env) {

@@ -54,7 +54,7 @@ struct EnvState {
     int* ghm2 = nullptr;                    // max ground height over the 5x5 column neighbourhood [G]
     float* work[SKT_NWORK] = {nullptr};     // N-sized work buffers
     float* pcg[6] = {nullptr};              // r, z, s_a, s_b, As, pre
-    float4* A = nullptr;                    // upper coefficients (x,y,z) and diagonal (w)
+    float* irho = nullptr;                  // 1/density of the last pressure solve (0 in ground)
     float* zeros = nullptr;                 // [Y] zero profile
     float* ones = nullptr;                  // [Y] unit profile
     double* partials = nullptr;             // per-block reduction partials (PART_COUNT x maxBlocks)
@@ -141,7 +141,7 @@ hipError_t launch_buoyancy(EnvState& e, float dt);
 hipError_t launch_init_density(EnvState& e, int copy_old);
 hipError_t launch_new_pressure(EnvState& e);
 hipError_t launch_reset_vel(EnvState& e);
-hipError_t launch_build_A(EnvState& e);
+hipError_t launch_build_precon(EnvState& e);  // 1/density and 1/diagonal
 hipError_t launch_divergence(EnvState& e, float dt);  // r = div, partial max |r|
 hipError_t launch_pcg_start(EnvState& e);              // stop tolerance; z = pre r, partial z.r
 hipError_t launch_pcg_A(EnvState& e, int it);          // stop test / beta; s update, As, partial s.As

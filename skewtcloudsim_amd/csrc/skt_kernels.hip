@@ -528,15 +528,25 @@ __device__ __forceinline__ float faceCoef(float cur, float nb) {
 // char conversion of a lower-triangle coefficient (Game/kernelSky.cu:762-764, quirk A1)
 __device__ __forceinline__ float trunc8(float c) { return fminf(fmaxf(truncf(c), -128.0f), 127.0f); }
 
-// initAMatrix + initPrecon, Game/kernelSky.cu:1320-1430, density bounds {N,D,D}:
-// A.xyz = coefficients to the +x/+y/+z neighbours, A.w = diagonal.
-KL void k_build_A(Geom G, const float* __restrict__ rho, float4* __restrict__ A, float* __restrict__ pre) {
+// Face coefficient of initAMatrix (Game/kernelSky.cu:1372-1377) from the
+// reciprocal densities of the two cells: the reference forms 1/cur + 1/nb
+// itself, so the result is bit-identical. A zero neighbour density (its
+// reciprocal infinite) gives 0, as nb == 0 does there.
+__device__ __forceinline__ float faceI(float ic, float in) {
+    return isinf(in) ? 0.0f : -fmaxf(1.0f, 1.225f / (2.0f / (ic + in)));
+}
+
+// initAMatrix + initPrecon, Game/kernelSky.cu:1320-1430, density bounds
+// {N,D,D}. The operator is not stored: the SpMV rebuilds its coefficients
+// from 1/density (irho, 4 B per cell instead of 16 for a float4 operator).
+// This pass writes irho and the Jacobi preconditioner 1/diagonal.
+KL void k_build_precon(Geom G, const float* __restrict__ rho, float* __restrict__ irho, float* __restrict__ pre) {
     CELL_SETUP;
     (void)bid;
     if (!inb) return;
     const long i = gidx(G, x, y, z);
     if (isGround(G, x, y, z)) {
-        A[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        irho[i] = 0.0f;
         pre[i] = 1.0f;
         return;
     }
@@ -555,11 +565,11 @@ KL void k_build_A(Geom G, const float* __restrict__ rho, float4* __restrict__ A,
     w -= d;
     w -= f;
     w -= bb;
-    A[i] = make_float4(r, u, f, w);
+    irho[i] = 1.0f / c;
     pre[i] = w == 0.0f ? 1.0f : 1.0f / w;
 }
-hipError_t launch_build_A(EnvState& e) {
-    hipLaunchKernelGGL(k_build_A, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_DENSITY], e.A, e.pcg[5]);
+hipError_t launch_build_precon(EnvState& e) {
+    hipLaunchKernelGGL(k_build_precon, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_DENSITY], e.irho, e.pcg[5]);
     return hipGetLastError();
 }
 
@@ -728,18 +738,22 @@ hipError_t launch_pcg_start(EnvState& e) {
 // Prologue PRO_A0 / PRO_A, then the search-direction update (endIteration,
 // Game/kernelSky.cu:987-1014) fused into applyAGPU (:719-778) and the s.As
 // partials (:675-717): s_new = z + beta s_old (s_new = z on the first
-// iteration), As = A s_new. s_new of the in-plane neighbours and their A.x /
-// A.y come from the LDS tile; along z they stay in registers. Cells with
-// y >= ghm1 + 2 have only sky (or the domain edge) around them.
+// iteration), As = A s_new. The operator is rebuilt per cell from 1/density
+// (faceI; diagonal = minus the six face coefficients, in initAMatrix's
+// order; outside faces use the cell's own density, Neumann; ground faces 0).
+// s_new and 1/density of the in-plane neighbours come from the LDS tile;
+// along z they stay in registers, and the back face is the front face of the
+// plane before. Cells with y >= ghm1 + 2 have only sky (or the domain edge)
+// around them.
 template <bool FIRST>
 KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __restrict__ s_old,
-                float* __restrict__ s_new, const float4* __restrict__ A, float* __restrict__ As,
+                float* __restrict__ s_new, const float* __restrict__ irho, float* __restrict__ As,
                 const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, double* pout,
                 const PcgState* sin, PcgState* sout, int it, int trunc) {
     ZM_SETUP;
     __shared__ double sh[4];
     __shared__ PcgState S;
-    __shared__ float tS[2][TH][TW], tAx[2][TH][TW], tAy[2][TH][TW];
+    __shared__ float tS[2][TH][TW], tI[2][TH][TW];
     pcgPrologue(FIRST ? PRO_A0 : PRO_A, pm, ps, n, sin, sout, 0, 0.0f, it, S, sh);
     if (S.done) return;
     const float beta = S.beta;
@@ -748,20 +762,22 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
     const long XY = (long)X * Y;
     const TileRole t = tileRole(G, x, y);
     auto snv = [&](float zv, float sv) { return FIRST ? zv : zv + beta * sv; };
+    // reference mode: char-truncated lower triangle (quirk A1); standard
+    // mode: the symmetric operator
+    auto lo = [&](float c) { return trunc ? trunc8(c) : c; };
     struct Own {
-        float z, s;
-        float4 a;
+        float z, s, i;
         int gm, gf;
     };
     struct Halo {
-        float dz, ds, day, uz, us, lz, ls, lax, rz, rs;
+        float dz, ds, di, uz, us, ui, lz, ls, li, rz, rs, ri;
     };
     auto loadOwn = [&](int pl, Own& o) {
         if (t.own && pl < Z) {
             const long pz = XY * pl;
             o.z = (zz + pz)[t.rc];
             if (!FIRST) o.s = (s_old + pz)[t.rc];
-            o.a = (A + pz)[t.rc];
+            o.i = (irho + pz)[t.rc];
             o.gm = (G.ghm1 + X * pl)[(unsigned)x];
             o.gf = (G.gh + X * min(pl + 1, Z - 1))[(unsigned)x];
         }
@@ -771,35 +787,38 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
         if (t.hasD) {
             h.dz = (zz + pz)[t.rD];
             if (!FIRST) h.ds = (s_old + pz)[t.rD];
-            h.day = (A + pz)[t.rD].y;
+            h.di = (irho + pz)[t.rD];
         }
         if (t.hasU) {
             h.uz = (zz + pz)[t.rU];
             if (!FIRST) h.us = (s_old + pz)[t.rU];
+            h.ui = (irho + pz)[t.rU];
         }
         if (t.hasL) {
             h.lz = (zz + pz)[t.rL];
             if (!FIRST) h.ls = (s_old + pz)[t.rL];
-            h.lax = (A + pz)[t.rL].x;
+            h.li = (irho + pz)[t.rL];
         }
         if (t.hasR) {
             h.rz = (zz + pz)[t.rR];
             if (!FIRST) h.rs = (s_old + pz)[t.rR];
+            h.ri = (irho + pz)[t.rR];
         }
     };
-    const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    Own oc{0.0f, 0.0f, z4, 0, 0}, of{0.0f, 0.0f, z4, 0, 0}, on{0.0f, 0.0f, z4, 0, 0};
+    Own oc{0.0f, 0.0f, 0.0f, 0, 0}, of{0.0f, 0.0f, 0.0f, 0, 0}, on{0.0f, 0.0f, 0.0f, 0, 0};
     Halo hc{}, hn{};
-    // plane below the chunk: s_new and A.z
-    float snb = 0.0f, azb = 0.0f;
+    // plane below the chunk: s_new, and its front face (= this plane's back face)
+    float snb = 0.0f, fb = 0.0f;
+    bool skB = false;
     int ghc = 0;
     if (t.own) {
+        ghc = (G.gh + X * z0)[(unsigned)x];
         if (z0 > 0 && y > (G.gh + X * (z0 - 1))[(unsigned)x]) {
             const long pz = XY * (z0 - 1);
+            skB = true;
             snb = snv((zz + pz)[t.rc], FIRST ? 0.0f : (s_old + pz)[t.rc]);
-            azb = (A + pz)[t.rc].z;
+            if (y > ghc) fb = faceI((irho + pz)[t.rc], (irho + XY * z0)[t.rc]);
         }
-        ghc = (G.gh + X * z0)[(unsigned)x];
     }
     loadOwn(z0, oc);
     loadOwn(z0 + 1, of);
@@ -812,45 +831,62 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
         if (z + 1 < z1) loadHalo(z + 1, hn);
         const float snc = skC ? snv(oc.z, oc.s) : 0.0f;
         tS[buf][ty + 1][tx + 1] = snc;
-        tAx[buf][ty + 1][tx + 1] = oc.a.x;
-        tAy[buf][ty + 1][tx + 1] = oc.a.y;
+        tI[buf][ty + 1][tx + 1] = oc.i;
         if (t.hasD) {
             tS[buf][0][tx + 1] = snv(hc.dz, hc.ds);
-            tAy[buf][0][tx + 1] = hc.day;
+            tI[buf][0][tx + 1] = hc.di;
         }
-        if (t.hasU) tS[buf][5][tx + 1] = snv(hc.uz, hc.us);
+        if (t.hasU) {
+            tS[buf][5][tx + 1] = snv(hc.uz, hc.us);
+            tI[buf][5][tx + 1] = hc.ui;
+        }
         if (t.hasL) {
             tS[buf][ty + 1][0] = snv(hc.lz, hc.ls);
-            tAx[buf][ty + 1][0] = hc.lax;
+            tI[buf][ty + 1][0] = hc.li;
         }
-        if (t.hasR) tS[buf][ty + 1][65] = snv(hc.rz, hc.rs);
+        if (t.hasR) {
+            tS[buf][ty + 1][65] = snv(hc.rz, hc.rs);
+            tI[buf][ty + 1][65] = hc.ri;
+        }
         __syncthreads();
         const bool skF = t.own && z + 1 < Z && y > oc.gf;
         const float snf = skF ? snv(of.z, of.s) : 0.0f;
-        float azc = 0.0f;
+        float ff = 0.0f;
         if (t.own) {
             float* sw = s_new + XY * z;
             float* aw = As + XY * z;
             if (skC) {
-                bool sl = x > 0, sr = x < X - 1, sd = true;
+                const float ic = oc.i;
+                const bool oL = x == 0, oR = x == X - 1, oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
+                bool sl = !oL, sr = !oR, sd = true;
                 if (y < oc.gm + 2) {
                     const int* gz = G.gh + X * z;
                     sl = sl && y > gz[x - 1];
                     sr = sr && y > gz[x + 1];
                     sd = y - 1 > ghc;
                 }
-                const bool su = y + 1 < Y;
-                const float l = sl ? tS[buf][ty + 1][tx] : 0.0f, r = sr ? tS[buf][ty + 1][tx + 2] : 0.0f;
-                const float d = sd ? tS[buf][ty][tx + 1] : 0.0f, u = su ? tS[buf][ty + 2][tx + 1] : 0.0f;
-                // reference mode: char-truncated lower triangle (quirk A1);
-                // standard mode: the symmetric operator
-                auto lo = [&](float c) { return trunc ? trunc8(c) : c; };
-                const float aL = sl ? lo(tAx[buf][ty + 1][tx]) : 0.0f;
-                const float aD = sd ? lo(tAy[buf][ty][tx + 1]) : 0.0f;
-                const float aB = lo(azb);
-                const float4 Ac = oc.a;
-                const float o = Ac.w * snc + ((aL * l + aD * d + aB * snb + Ac.x * r + Ac.y * u + Ac.z * snf));
-                azc = Ac.z;
+                const bool su = !oU;
+                // faces to outside neighbours see the cell's own density
+                float own = 0.0f;
+                if (oL | oR | oU | oF | oB) own = faceI(ic, ic);
+                const float r = oR ? own : (sr ? faceI(ic, tI[buf][ty + 1][tx + 2]) : 0.0f);
+                const float l = oL ? own : (sl ? faceI(ic, tI[buf][ty + 1][tx]) : 0.0f);
+                const float u = oU ? own : faceI(ic, tI[buf][ty + 2][tx + 1]);
+                const float d = sd ? faceI(ic, tI[buf][ty][tx + 1]) : 0.0f;
+                const float f = oF ? own : (skF ? faceI(ic, of.i) : 0.0f);
+                const float bb = oB ? own : (skB ? fb : 0.0f);
+                float w = 0.0f;
+                w -= r;
+                w -= l;
+                w -= u;
+                w -= d;
+                w -= f;
+                w -= bb;
+                const float L = sl ? tS[buf][ty + 1][tx] : 0.0f, R = sr ? tS[buf][ty + 1][tx + 2] : 0.0f;
+                const float D = sd ? tS[buf][ty][tx + 1] : 0.0f, U = su ? tS[buf][ty + 2][tx + 1] : 0.0f;
+                const float aL = sl ? lo(l) : 0.0f, aD = sd ? lo(d) : 0.0f, aB = skB ? lo(bb) : 0.0f;
+                const float o = w * snc + ((aL * L + aD * D + aB * snb + r * R + u * U + f * snf));
+                ff = f;
                 sw[t.rc] = snc;
                 aw[t.rc] = o;
                 acc += (double)(o * snc);
@@ -860,7 +896,8 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
             }
         }
         snb = snc;
-        azb = azc;
+        fb = ff;
+        skB = skC;
         skC = skF;
         ghc = oc.gf;
         oc = of;
@@ -879,11 +916,11 @@ hipError_t launch_pcg_A(EnvState& e, int it) {
     const double* ps = it == 0 ? e.partials + PART_START * M : e.partials + PART_BSUM * M;
     const int trunc = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     if (it == 0)
-        hipLaunchKernelGGL(k_pcg_A<true>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.A,
+        hipLaunchKernelGGL(k_pcg_A<true>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho,
                            e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it,
                            trunc);
     else
-        hipLaunchKernelGGL(k_pcg_A<false>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.A,
+        hipLaunchKernelGGL(k_pcg_A<false>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho,
                            e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it,
                            trunc);
     return hipGetLastError();

@@ -253,3 +253,107 @@ def test_region_diagnostic(pair):
     orc.L.sko_region(orc.h, 0, a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p))
     np.testing.assert_allclose(g, o, rtol=1e-4, atol=1e-12)
     assert np.count_nonzero(o) > 10
+
+
+# ------------------------------------------------------------------ terrain ---
+# Non-flat ground: lateral ground neighbours, ground within the PPM stencil
+# and the slow (near-ground) paths of every stencil kernel.
+
+def _terrain(nx, nz, kind, seed=21):
+    rng = np.random.default_rng(seed)
+    if kind == "random":
+        return rng.integers(0, 5, nx * nz).astype(np.int32)
+    x = np.arange(nx)[None, :]
+    z = np.arange(nz)[:, None]
+    hill = 6.0 * np.exp(-(((x - nx * 0.4) / (nx * 0.2)) ** 2 + ((z - nz * 0.6) / (nz * 0.25)) ** 2))
+    return np.rint(hill).astype(np.int32).ravel()
+
+
+def make_terrain_pair(nx, ny, nz, gh, dx=200.0, seed=31, **cfg):
+    import skewtcloudsim_amd as skt
+    snd = skt.make_sounding(nx, ny, nz, dx, seed=seed, n_bubbles=2)
+    env = skt.EnvironmentGPU(nx, ny, nz, dx, **cfg)
+    ocfg = {k: cfg[k] for k in ("pcg_mode", "pcg_tol", "pcg_max_iter") if k in cfg}
+    orc = ob.Oracle(nx, ny, nz, dx, **ocfg)
+    env.set_ground_height(gh)
+    orc.set_ground_height(gh)
+    env.init(*[snd[k] for k in KEYS])
+    orc.init(*[snd[k] for k in KEYS])
+    return env, orc
+
+
+@pytest.mark.parametrize("shape,kind", [((24, 20, 16), "random"), ((33, 18, 21), "hill"), ((17, 13, 11), "random")])
+def test_terrain_phases(shape, kind):
+    nx, ny, nz = shape
+    gh = _terrain(nx, nz, kind)
+    env, orc = make_terrain_pair(nx, ny, nz, gh)
+    m = sky_mask(nx, ny, nz, gh)
+    assert np.array_equal(env.masks(), orc.masks())
+    rich_state(env, orc, seed=41)
+    for field, type_ in (("POTTEMP", 0), ("QV", 1), ("VELX", 2), ("VELY", 3), ("VELZ", 4), ("QR", 5)):
+        env.diffuseGPU(field, type_, 2e5)  # large k: every neighbour, ground ones included, counts
+        orc.diffuse(ob.FIELD[field], type_, 2e5)
+        assert_fields(env, orc, (field,), m, l2=1e-5, maxabs=1e-4, tag="terrain-diffuse-" + field)
+    rich_state(env, orc, seed=42)
+    for field, prof, bounds in (("POTTEMP", "ISEN_TEMP", (2, 2, 2)), ("VELX", "DEFAULT_VELX", (2, 2, 1)),
+                                ("VELY", None, (1, 1, 1)), ("QR", None, (1, 1, 1)), ("QV", "ISEN_VAPOR", (0, 0, 0))):
+        env.advectPPMWGPU(field, prof, bounds, 60.0)
+        orc.advect_ppm(ob.FIELD[field], -1 if prof is None else ob.FIELD[prof], *bounds, 60.0)
+        assert_fields(env, orc, (field, "VELX", "VELY", "VELZ"), m, l2=1e-5, maxabs=1e-4,
+                      tag="terrain-advect-" + field)
+    rich_state(env, orc, seed=43)
+    for ftype, f in enumerate(("QR", "QS", "QI")):
+        env.advectPrecip(f, ftype, 200.0)
+        orc.advect_precip(ob.FIELD[f], ftype, 200.0)
+        # ice falls in 8 substeps of powf chains here; FP32 rounding alone moves
+        # it by 5e-6 relative L2 against the FP64 oracle on this case
+        assert_fields(env, orc, (f,), m, l2=1e-5, maxabs=1e-4, tag="terrain-precip")
+    rich_state(env, orc, seed=44)
+    env.calculateBuoyancy(0.5)
+    orc.buoyancy(0.5)
+    assert_fields(env, orc, ("VELY",), m, tag="terrain-buoyancy")
+    rich_state(env, orc, seed=45)
+    env.microPhysicsSkyGPU(0.5, 1.0)
+    orc.micro_sky(0.5, 1.0)
+    assert_fields(env, orc, ("QV", "QW", "QC", "QR", "QS", "QI", "POTTEMP"), m, l2=1e-5, maxabs=1e-4,
+                  tag="terrain-micro")
+    assert np.array_equal(env.phase_mask(), orc.phase_mask())
+    env.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_terrain_pressure_projection(mode):
+    nx, ny, nz = 24, 20, 16
+    gh = _terrain(nx, nz, "random")
+    env, orc = make_terrain_pair(nx, ny, nz, gh, pcg_mode=mode, pcg_tol=1e-5 if mode == 0 else 1e-6,
+                                 pcg_max_iter=60)
+    rich_state(env, orc, seed=46)
+    env.initDensity()
+    orc.init_density()
+    env.pressureProject(0.5)
+    orc.pressure_project(0.5)
+    assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+    m = sky_mask(nx, ny, nz, gh)
+    assert_fields(env, orc, ("PRESSURE_PROJ",), m, l2=1e-4, maxabs=1e-3, tag="terrain-pcg")
+    assert_fields(env, orc, ("VELX", "VELY", "VELZ", "PRESSURE", "DENSITY"), m, l2=1e-5, maxabs=1e-4,
+                  tag="terrain-proj")
+    env.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("shape,kind", [((24, 20, 16), "random"), ((33, 18, 21), "hill")])
+def test_terrain_full_steps(shape, kind):
+    nx, ny, nz = shape
+    gh = _terrain(nx, nz, kind)
+    env, orc = make_terrain_pair(nx, ny, nz, gh)
+    m = sky_mask(nx, ny, nz, gh)
+    for s in range(4):
+        env.updateGPU(0.5, 1.0)
+        orc.step(0.5, 1.0)
+        tol = 1e-5 if s == 0 else 1e-4
+        assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+        assert_fields(env, orc, SKY, m, l2=tol, maxabs=50 * tol, tag=f"terrain-step{s}")
+        assert_fields(env, orc, GROUND, np.ones(nx * nz, bool), l2=tol, maxabs=50 * tol, tag=f"terrain-ground{s}")
+    env.close()
+    orc.close()
