@@ -169,11 +169,15 @@ int advectFields(EnvState& e, const AdvField* fs, int n, float dt) {
         const int fld = fs[j].fld;
         float* A = e.f[fld];
         float *T1 = e.work[3 * j], *T2 = e.work[3 * j + 1], *T3 = e.work[3 * j + 2];
-        const PpmField f0{A, T1, fld == SKT_VELX ? A : U, fs[j].prof, fs[j].b};
-        const PpmField f1{T1, T2, fld == SKT_VELY ? A : V, fs[j].prof, fs[j].b};
-        const PpmField f2{T2, T1, fld == SKT_VELZ ? T2 : W, fs[j].prof, fs[j].b};
-        const PpmField f3{T1, T3, fld == SKT_VELY ? T2 : V, fs[j].prof, fs[j].b};
-        const PpmField f4{T3, A, fld == SKT_VELX ? T3 : U, fs[j].prof, fs[j].b};
+        // At ground positions each pass sees what the reference's buffers hold
+        // there: the field's own ground values in passes 1, 3 and 5 (they read
+        // advectArray, whose ground cells no pass writes) and 0 in passes 2 and
+        // 4 (they read m_outputArray, written only at sky cells).
+        const PpmField f0{A, T1, fld == SKT_VELX ? A : U, fs[j].prof, fs[j].b, A};
+        const PpmField f1{T1, T2, fld == SKT_VELY ? A : V, fs[j].prof, fs[j].b, nullptr};
+        const PpmField f2{T2, T1, fld == SKT_VELZ ? T2 : W, fs[j].prof, fs[j].b, A};
+        const PpmField f3{T1, T3, fld == SKT_VELY ? T2 : V, fs[j].prof, fs[j].b, nullptr};
+        const PpmField f4{T3, A, fld == SKT_VELX ? T3 : U, fs[j].prof, fs[j].b, A};
         pb[0].f[j] = f0;
         pb[1].f[j] = f1;
         pb[2].f[j] = f2;
@@ -423,7 +427,9 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     for (int k = 0; k < SKT_FIELD_COUNT; k++)
         if (dalloc(e, &e.f[k], sizeof(float) * fieldLen(e, k))) return bail(SKT_ERR_NOMEM);
     for (int k = 0; k < SKT_NWORK; k++)
-        if (dalloc(e, &e.work[k], sizeof(float) * std::max(e.N, 2 * e.G))) return bail(SKT_ERR_NOMEM);
+        if (dalloc(e, &e.work[k], sizeof(float) * std::max(e.N, 2 * e.G)) ||
+            hipMemsetAsync(e.work[k], 0, sizeof(float) * std::max(e.N, 2 * e.G), e.stream) != hipSuccess)
+            return bail(SKT_ERR_NOMEM);
     for (int k = 0; k < 6; k++)
         if (dalloc(e, &e.pcg[k], sizeof(float) * e.N)) return bail(SKT_ERR_NOMEM);
     if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.ghm1, sizeof(int) * e.G) ||

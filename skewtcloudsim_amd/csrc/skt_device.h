@@ -201,4 +201,36 @@ __device__ __forceinline__ float ppmFlux(float vel, float L, float Cc, float Rr,
     return vel * qq;
 }
 
+// ------------------------------------------------------ pressure operator ---
+// Face coefficient of initAMatrix, Game/kernelSky.cu:1372-1377.
+__device__ __forceinline__ float faceCoef(float cur, float nb) {
+    return nb == 0.0f ? 0.0f : -fmaxf(1.0f, 1.225f / (2.0f / (1.0f / cur + 1.0f / nb)));
+}
+
+// q = a / b rounded to nearest for a > 0 and normal b in [2^-30, 2^30]: the
+// hardware reciprocal estimate refined by one Newton step, then the quotient
+// corrected once with its exact (FMA) remainder (Markstein's iteration).
+// Six instructions against ten for the general IEEE division.
+__device__ __forceinline__ float divNear(float a, float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    r = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+    const float q = a * r;
+    return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
+}
+
+// The face coefficient as a function of s = 1/cur + 1/nb (the sum the
+// reference forms): -max(1, 1.225 / (2 / s)) with both divisions rounded to
+// nearest. For s in [2^-20, 2^20] the short division is used; every float s
+// gives the same bits as the IEEE expression (checked exhaustively over all
+// 2^32 inputs by tests/test_face_coefficient.py); other s take the IEEE path.
+__device__ __forceinline__ float faceOfSum(float s) {
+    if (s >= 9.5367431640625e-07f && s <= 1048576.0f) return -fmaxf(1.0f, divNear(1.225f, divNear(2.0f, s)));
+    return -fmaxf(1.0f, 1.225f / (2.0f / s));
+}
+
+// The face coefficient from the reciprocal densities of the two cells. A
+// zero neighbour density (infinite reciprocal) gives 0, as nb == 0 does in
+// the reference.
+__device__ __forceinline__ float faceI(float ic, float in) { return isinf(in) ? 0.0f : faceOfSum(ic + in); }
+
 }  // namespace skt

@@ -123,6 +123,7 @@ struct PpmField {
     const float* vel;
     const float* prof;
     Bounds b;
+    const float* gsrc;  // values at ground positions (nullptr: 0), used under a NEUMANN ground condition
 };
 struct PpmBatch {
     PpmField f[SKT_BATCH];

@@ -313,7 +313,12 @@ __device__ __forceinline__ float ppmCell(const float* q, const float* o, float v
     float* __restrict__ out = fd_.out;                                    \
     const float* __restrict__ vel = fd_.vel;                              \
     const float* __restrict__ prof = fd_.prof;                            \
-    const Bounds b = fd_.b
+    const Bounds b = fd_.b;                                               \
+    /* value that stands at a ground position: the field's own ground */  \
+    /* value or 0 (read only under a NEUMANN ground condition)        */  \
+    const float* __restrict__ gsrc = fd_.gsrc;                            \
+    const bool gNeu_ = b.ground == BC_NEUMANN && gsrc != nullptr;         \
+    auto graw = [&](long gi) { return gNeu_ ? gsrc[gi] : 0.0f; }
 
 // x pass: the stencil runs along the row. Cells with y > ghm2 (largest
 // ground height over the 5x5 columns around) see no ground in the stencil.
@@ -360,7 +365,7 @@ KL void k_ppm_x(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, c
             for (int s = 0; s < 5; s++) {
                 const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
                 const bool gnd = y <= gh[s];
-                q[s] = nbv(Q[s], outp, gnd, pv, b);
+                q[s] = nbv(gnd ? graw(XY * z + rowy + cx[s]) : Q[s], outp, gnd, pv, b);
                 o[s] = unitv(outp, gnd, b);
             }
             vm = x == 0 ? bcv(b.sides, VC, cv) : (y <= gh[1] ? bcv(b.ground, VC, cv) : VM);
@@ -417,7 +422,7 @@ KL void k_ppm_y(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx) {
             const int py = y + s - 2;
             const bool outp = (py < 0) || (py > Y - 1);
             const bool gnd = clampi(py, 0, Y - 1) <= gh;
-            q[s] = nbv(Q[s], outp, gnd, pv[s], b);
+            q[s] = nbv(gnd ? graw(XY * z + ry[s]) : Q[s], outp, gnd, pv[s], b);
             o[s] = unitv(outp, gnd, b);
         }
         // y - 1 >= 0 on every sky cell
@@ -476,7 +481,7 @@ KL void k_ppm_z(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, c
             const int pz = z + s - 2;
             const bool outp = (pz < 0) || (pz > Z - 1);
             const bool gnd = y <= gs[s];
-            q[s] = nbv(qs[s], outp, gnd, pv, b);
+            q[s] = nbv(gnd ? graw(XY * clampi(pz, 0, Z - 1) + rc) : qs[s], outp, gnd, pv, b);
             o[s] = unitv(outp, gnd, b);
         }
         const float vm = z == 0 ? bcv(b.sides, vc, cv) : (y <= gs[1] ? bcv(b.ground, vc, cv) : vmr);
@@ -522,19 +527,8 @@ hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) 
 }
 
 // ---------------------------------------------------- pressure projection ---
-__device__ __forceinline__ float faceCoef(float cur, float nb) {
-    return nb == 0.0f ? 0.0f : -fmaxf(1.0f, 1.225f / (2.0f / (1.0f / cur + 1.0f / nb)));
-}
 // char conversion of a lower-triangle coefficient (Game/kernelSky.cu:762-764, quirk A1)
 __device__ __forceinline__ float trunc8(float c) { return fminf(fmaxf(truncf(c), -128.0f), 127.0f); }
-
-// Face coefficient of initAMatrix (Game/kernelSky.cu:1372-1377) from the
-// reciprocal densities of the two cells: the reference forms 1/cur + 1/nb
-// itself, so the result is bit-identical. A zero neighbour density (its
-// reciprocal infinite) gives 0, as nb == 0 does there.
-__device__ __forceinline__ float faceI(float ic, float in) {
-    return isinf(in) ? 0.0f : -fmaxf(1.0f, 1.225f / (2.0f / (ic + in)));
-}
 
 // initAMatrix + initPrecon, Game/kernelSky.cu:1320-1430, density bounds
 // {N,D,D}. The operator is not stored: the SpMV rebuilds its coefficients
@@ -769,9 +763,14 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
         float z, s, i;
         int gm, gf;
     };
+    // A thread holds at most one halo row cell (below the tile for wave 0,
+    // above it for wave 1) and one halo column cell (lane 0 left, lane 63 right).
     struct Halo {
-        float dz, ds, di, uz, us, ui, lz, ls, li, rz, rs, ri;
+        float rz, rs, ri, cz, cs, ci;
     };
+    const bool hasRow = t.hasD || t.hasU, hasCol = t.hasL || t.hasR;
+    const unsigned rRow = t.hasD ? t.rD : t.rU, rCol = t.hasL ? t.rL : t.rR;
+    const int rowSlot = t.hasD ? 0 : 5, colSlot = t.hasL ? 0 : 65;
     auto loadOwn = [&](int pl, Own& o) {
         if (t.own && pl < Z) {
             const long pz = XY * pl;
@@ -784,25 +783,15 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
     };
     auto loadHalo = [&](int pl, Halo& h) {
         const long pz = XY * pl;
-        if (t.hasD) {
-            h.dz = (zz + pz)[t.rD];
-            if (!FIRST) h.ds = (s_old + pz)[t.rD];
-            h.di = (irho + pz)[t.rD];
+        if (hasRow) {
+            h.rz = (zz + pz)[rRow];
+            if (!FIRST) h.rs = (s_old + pz)[rRow];
+            h.ri = (irho + pz)[rRow];
         }
-        if (t.hasU) {
-            h.uz = (zz + pz)[t.rU];
-            if (!FIRST) h.us = (s_old + pz)[t.rU];
-            h.ui = (irho + pz)[t.rU];
-        }
-        if (t.hasL) {
-            h.lz = (zz + pz)[t.rL];
-            if (!FIRST) h.ls = (s_old + pz)[t.rL];
-            h.li = (irho + pz)[t.rL];
-        }
-        if (t.hasR) {
-            h.rz = (zz + pz)[t.rR];
-            if (!FIRST) h.rs = (s_old + pz)[t.rR];
-            h.ri = (irho + pz)[t.rR];
+        if (hasCol) {
+            h.cz = (zz + pz)[rCol];
+            if (!FIRST) h.cs = (s_old + pz)[rCol];
+            h.ci = (irho + pz)[rCol];
         }
     };
     Own oc{0.0f, 0.0f, 0.0f, 0, 0}, of{0.0f, 0.0f, 0.0f, 0, 0}, on{0.0f, 0.0f, 0.0f, 0, 0};
@@ -832,21 +821,13 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
         const float snc = skC ? snv(oc.z, oc.s) : 0.0f;
         tS[buf][ty + 1][tx + 1] = snc;
         tI[buf][ty + 1][tx + 1] = oc.i;
-        if (t.hasD) {
-            tS[buf][0][tx + 1] = snv(hc.dz, hc.ds);
-            tI[buf][0][tx + 1] = hc.di;
+        if (hasRow) {
+            tS[buf][rowSlot][tx + 1] = snv(hc.rz, hc.rs);
+            tI[buf][rowSlot][tx + 1] = hc.ri;
         }
-        if (t.hasU) {
-            tS[buf][5][tx + 1] = snv(hc.uz, hc.us);
-            tI[buf][5][tx + 1] = hc.ui;
-        }
-        if (t.hasL) {
-            tS[buf][ty + 1][0] = snv(hc.lz, hc.ls);
-            tI[buf][ty + 1][0] = hc.li;
-        }
-        if (t.hasR) {
-            tS[buf][ty + 1][65] = snv(hc.rz, hc.rs);
-            tI[buf][ty + 1][65] = hc.ri;
+        if (hasCol) {
+            tS[buf][ty + 1][colSlot] = snv(hc.cz, hc.cs);
+            tI[buf][ty + 1][colSlot] = hc.ci;
         }
         __syncthreads();
         const bool skF = t.own && z + 1 < Z && y > oc.gf;
@@ -866,13 +847,18 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
                     sd = y - 1 > ghc;
                 }
                 const bool su = !oU;
+                // every tile index is inside the tile: read unconditionally, select after
+                const float iR = tI[buf][ty + 1][tx + 2], iL = tI[buf][ty + 1][tx];
+                const float iU = tI[buf][ty + 2][tx + 1], iD = tI[buf][ty][tx + 1];
+                const float sR = tS[buf][ty + 1][tx + 2], sL = tS[buf][ty + 1][tx];
+                const float sU = tS[buf][ty + 2][tx + 1], sD = tS[buf][ty][tx + 1];
                 // faces to outside neighbours see the cell's own density
                 float own = 0.0f;
                 if (oL | oR | oU | oF | oB) own = faceI(ic, ic);
-                const float r = oR ? own : (sr ? faceI(ic, tI[buf][ty + 1][tx + 2]) : 0.0f);
-                const float l = oL ? own : (sl ? faceI(ic, tI[buf][ty + 1][tx]) : 0.0f);
-                const float u = oU ? own : faceI(ic, tI[buf][ty + 2][tx + 1]);
-                const float d = sd ? faceI(ic, tI[buf][ty][tx + 1]) : 0.0f;
+                const float r = oR ? own : (sr ? faceI(ic, iR) : 0.0f);
+                const float l = oL ? own : (sl ? faceI(ic, iL) : 0.0f);
+                const float u = oU ? own : faceI(ic, iU);
+                const float d = sd ? faceI(ic, iD) : 0.0f;
                 const float f = oF ? own : (skF ? faceI(ic, of.i) : 0.0f);
                 const float bb = oB ? own : (skB ? fb : 0.0f);
                 float w = 0.0f;
@@ -882,8 +868,8 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
                 w -= d;
                 w -= f;
                 w -= bb;
-                const float L = sl ? tS[buf][ty + 1][tx] : 0.0f, R = sr ? tS[buf][ty + 1][tx + 2] : 0.0f;
-                const float D = sd ? tS[buf][ty][tx + 1] : 0.0f, U = su ? tS[buf][ty + 2][tx + 1] : 0.0f;
+                const float L = sl ? sL : 0.0f, R = sr ? sR : 0.0f;
+                const float D = sd ? sD : 0.0f, U = su ? sU : 0.0f;
                 const float aL = sl ? lo(l) : 0.0f, aD = sd ? lo(d) : 0.0f, aB = skB ? lo(bb) : 0.0f;
                 const float o = w * snc + ((aL * L + aD * D + aB * snb + r * R + u * U + f * snf));
                 ff = f;
