@@ -438,6 +438,7 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
         dalloc(e, &e.phase, (size_t)e.N))
         return bail(SKT_ERR_NOMEM);
     e.maxBlocks = blocks3d(e);
+    e.gA = zgridResident(e, pcgA_resident_blocks(e.device));
     if (dalloc(e, &e.partials, sizeof(double) * PART_COUNT * (size_t)e.maxBlocks) ||
         dalloc(e, &e.region, sizeof(double) * 33 * (size_t)e.maxBlocks + 64 * sizeof(float)) ||
         dalloc(e, &e.sc, sizeof(Scalars)))

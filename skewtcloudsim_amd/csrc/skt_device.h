@@ -207,26 +207,31 @@ __device__ __forceinline__ float faceCoef(float cur, float nb) {
     return nb == 0.0f ? 0.0f : -fmaxf(1.0f, 1.225f / (2.0f / (1.0f / cur + 1.0f / nb)));
 }
 
-// q = a / b rounded to nearest for a > 0 and normal b in [2^-30, 2^30]: the
-// hardware reciprocal estimate refined by one Newton step, then the quotient
-// corrected once with its exact (FMA) remainder (Markstein's iteration).
-// Six instructions against ten for the general IEEE division.
-__device__ __forceinline__ float divNear(float a, float b) {
-    float r = __builtin_amdgcn_rcpf(b);
-    r = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
-    const float q = a * r;
-    return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
-}
-
 // The face coefficient as a function of s = 1/cur + 1/nb (the sum the
 // reference forms): -max(1, 1.225 / (2 / s)) with both divisions rounded to
-// nearest. For s in [2^-20, 2^20] the short division is used; every float s
-// gives the same bits as the IEEE expression (checked exhaustively over all
-// 2^32 inputs by tests/test_face_coefficient.py); other s take the IEEE path.
+// nearest. Scaling by 2 is exact, so 2 / s rounds to 2 RN(1/s) and the second
+// quotient equals RN(0.6125 / RN(1/s)) (0.6125f is 1.225f / 2 exactly). The
+// short form takes RN(1/s) from the hardware reciprocal estimate with one FMA
+// refinement, and the quotient from the estimate of 1/RN(1/s) with one
+// remainder correction (Markstein's iteration): seven instructions. For s in
+// [2^-126, 2^126] every float gives the same bits as the IEEE expression
+// (checked exhaustively over all 2^32 inputs by
+// tests/test_face_coefficient.py); other s take the IEEE path.
+__device__ __forceinline__ bool faceFastOk(float s) { return s >= 0x1p-126f && s <= 0x1p126f; }
+__device__ __forceinline__ float faceFast(float s) {
+    const float e = __builtin_amdgcn_rcpf(s);
+    const float t = __builtin_fmaf(__builtin_fmaf(-s, e, 1.0f), e, e);
+    const float r = __builtin_amdgcn_rcpf(t);
+    const float q = 0.6125f * r;
+    return -fmaxf(1.0f, __builtin_fmaf(__builtin_fmaf(-t, q, 0.6125f), r, q));
+}
 __device__ __forceinline__ float faceOfSum(float s) {
-    if (s >= 9.5367431640625e-07f && s <= 1048576.0f) return -fmaxf(1.0f, divNear(1.225f, divNear(2.0f, s)));
+    if (faceFastOk(s)) return faceFast(s);
     return -fmaxf(1.0f, 1.225f / (2.0f / s));
 }
+// A sky cell's 1/density for which every face sum with another such value
+// lies in the short form's range (or is infinite, a zero face).
+__device__ __forceinline__ bool faceValueOk(float i) { return i >= 0x1p-127f && i <= 0x1p125f; }
 
 // The face coefficient from the reciprocal densities of the two cells. A
 // zero neighbour density (infinite reciprocal) gives 0, as nb == 0 does in

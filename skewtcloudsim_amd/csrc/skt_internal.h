@@ -37,6 +37,14 @@ enum { PART_DIV = 0, PART_START, PART_A, PART_BMAX, PART_BSUM, PART_COUNT };
 // N-sized work buffers: three per field of a batched advection.
 constexpr int SKT_NWORK = 16;
 
+// z-marching launch shape: 64x4-cell tiles, each block walks zc planes; about
+// 2048 blocks for the large domains (one reduction partial per block). The
+// grid is one-dimensional; kernels map the block index to a tile.
+struct ZGrid {
+    dim3 grid;
+    int zc;
+    unsigned nblocks;
+};
 struct EnvState {
     skt_config cfg;
     int X, Y, Z;
@@ -59,6 +67,7 @@ struct EnvState {
     float* ones = nullptr;                  // [Y] unit profile
     double* partials = nullptr;             // per-block reduction partials (PART_COUNT x maxBlocks)
     int maxBlocks = 0;
+    ZGrid gA{};                             // grid of the pressure operator kernel
     Scalars* sc = nullptr;                  // device scalars
     Scalars* sc_host = nullptr;             // pinned mirror for readbacks
     uint8_t* phase = nullptr;               // [N] microphysics phase mask
@@ -87,15 +96,10 @@ enum { TC_DIFFUSE = 0, TC_PPM, TC_PCG_A, TC_PCG_B, TC_MICRO, TC_MAXVEL, TC_PRECI
 
 // ------------------------------------------------------------- launches ---
 Geom geom(const EnvState& e);
-// z-marching launch shape: 64x4-cell tiles, each block walks zc planes; about
-// 2048 blocks for the large domains (one reduction partial per block). The
-// grid is one-dimensional; kernels map the block index to a tile.
-struct ZGrid {
-    dim3 grid;
-    int zc;
-    unsigned nblocks;
-};
 ZGrid zgrid(const EnvState& e);
+// The same tiles with at most 'resident' blocks (one round on the device).
+ZGrid zgridResident(const EnvState& e, long resident);
+long pcgA_resident_blocks(int device);
 hipError_t launch_fill(EnvState& e, float* a, float v, long n);
 hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst);
 hipError_t launch_ground_water_snapshot(EnvState& e, float dt, float speed);

@@ -33,14 +33,22 @@ int blocks3d(const EnvState& e) {
     return (int)(g.x * g.y * g.z);
 }
 Geom geom(const EnvState& e) { return Geom{e.X, e.Y, e.Z, e.N, e.gh, e.ghm1, e.ghm2}; }
-ZGrid zgrid(const EnvState& e) {
+ZGrid zgridChunks(const EnvState& e, long nzc) {
     const int gx = (e.X + 63) / 64, gy = (e.Y + 3) / 4;
     const long cols = (long)gx * gy;
-    long nzc = std::max(1L, std::min((long)e.Z, (2048 + cols - 1) / cols));
+    nzc = std::max(1L, std::min((long)e.Z, nzc));
     const int zc = (int)((e.Z + nzc - 1) / nzc);
     nzc = (e.Z + zc - 1) / zc;
     const unsigned nb = (unsigned)(cols * nzc);
     return ZGrid{dim3(nb), zc, nb};
+}
+ZGrid zgrid(const EnvState& e) {
+    const long cols = (long)((e.X + 63) / 64) * ((e.Y + 3) / 4);
+    return zgridChunks(e, (2048 + cols - 1) / cols);
+}
+ZGrid zgridResident(const EnvState& e, long resident) {
+    const long cols = (long)((e.X + 63) / 64) * ((e.Y + 3) / 4);
+    return resident > 0 ? zgridChunks(e, resident / cols) : zgrid(e);
 }
 
 // Boundary value of a stencil read (the extv rule): ground condition if the
@@ -82,6 +90,25 @@ hipError_t launch_stream_copy(EnvState& e, const float* src, float* dst, long n)
     hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)std::min<long>((n4 + 255) / 256, 8192)), dim3(256), 0, e.stream,
                        (const float4*)src, (float4*)dst, n4);
     return hipGetLastError();
+}
+
+// Raw buffer access: the resource (base address) and a wave-uniform byte
+// offset sit in scalar registers, the lane supplies a 32-bit byte offset, so
+// a lane keeps one VGPR per offset instead of a 64-bit address per array, and
+// moving to the next plane is one scalar add. The records limit is the
+// largest (no range check), so every offset must lie inside the array.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc arrayRsrc(const void* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, -1, 0x00020000);
+}
+__device__ __forceinline__ float bldf(Rsrc r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ int bldi(Rsrc r, unsigned voff, unsigned soff) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void bstf(Rsrc r, unsigned voff, unsigned soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
 }
 
 // LDS tile of one plane for the 64 x 4 block: rows y0-1 .. y0+4 and columns
@@ -735,160 +762,185 @@ hipError_t launch_pcg_start(EnvState& e) {
 // iteration), As = A s_new. The operator is rebuilt per cell from 1/density
 // (faceI; diagonal = minus the six face coefficients, in initAMatrix's
 // order; outside faces use the cell's own density, Neumann; ground faces 0).
+// TRUNC selects the reference mode's char-truncated lower triangle (quirk
+// A1); without it the operator is symmetric.
 // s_new and 1/density of the in-plane neighbours come from the LDS tile;
 // along z they stay in registers, and the back face is the front face of the
-// plane before. Cells with y >= ghm1 + 2 have only sky (or the domain edge)
-// around them.
-template <bool FIRST>
+// plane before.
+// The tile encodes the boundary rules, so a cell reads its four in-plane
+// neighbours without tests: a ground cell holds s = 0 and 1/density = inf
+// (faceI gives 0), a position just outside the domain holds s = 0 and the
+// boundary cell's own 1/density (every thread reads its position clamped into
+// the domain). Faces take the short division (faceFast); it is exact when
+// each sky value lies in faceValueOk's range. The writers of a plane's values
+// check them and raise the plane's block flag otherwise; a flagged plane
+// recomputes every face with faceI.
+// The z loop runs four planes per trip over a ring of four plane slots (own
+// data is read three planes ahead) and two halo slots, so no values move
+// between registers and each load is waited for only where it is used.
+// Byte offsets into the arrays count from plane zb = max(z0 - 1, 0); the
+// launch keeps (chunk + 3 planes) below 4 GiB.
+template <bool FIRST, bool TRUNC>
 KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __restrict__ s_old,
                 float* __restrict__ s_new, const float* __restrict__ irho, float* __restrict__ As,
                 const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, double* pout,
-                const PcgState* sin, PcgState* sout, int it, int trunc) {
+                const PcgState* sin, PcgState* sout, int it) {
     ZM_SETUP;
     __shared__ double sh[4];
     __shared__ PcgState S;
     __shared__ float tS[2][TH][TW], tI[2][TH][TW];
+    // flag of plane k (counted from z0) in slot k % 3: set before the plane's
+    // barrier, read after it, cleared by thread 0 before barrier k - 1 + ... :
+    // slot (k + 1) % 3 is cleared at plane k before its barrier, after every
+    // reader of plane k - 2 has passed barrier k - 1.
+    __shared__ int flagBuf[3];
+    if (threadIdx.y == 0 && threadIdx.x < 3) flagBuf[threadIdx.x] = 0;
     pcgPrologue(FIRST ? PRO_A0 : PRO_A, pm, ps, n, sin, sout, 0, 0.0f, it, S, sh);
     if (S.done) return;
     const float beta = S.beta;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int X = G.X, Y = G.Y, Z = G.Z;
     const long XY = (long)X * Y;
-    const TileRole t = tileRole(G, x, y);
+    const float INF = __builtin_inff();
     auto snv = [&](float zv, float sv) { return FIRST ? zv : zv + beta * sv; };
-    // reference mode: char-truncated lower triangle (quirk A1); standard
-    // mode: the symmetric operator
-    auto lo = [&](float c) { return trunc ? trunc8(c) : c; };
+    auto lo = [&](float c) { return TRUNC ? trunc8(c) : c; };
+    // byte offsets of the lane's cell (clamped into the domain) in a plane
+    // and of its column in a ground-height plane
+    const int xc = min(x, X - 1), yc = min(y, Y - 1);
+    const unsigned ob = 4u * (unsigned)(xc + X * yc), gb = 4u * (unsigned)xc;
+    // One halo cell per lane at most: the row below the tile (wave 0), the
+    // row above it (wave 1), the columns left and right of it (lanes 0-3 and
+    // 4-7 of wave 2, one row each).
+    const int x0 = x - tx, y0 = y - ty;
+    int hx = x, hy = y0 - 1, hrow = 0, hcol = tx + 1;
+    if (ty == 1) hy = y0 + 4, hrow = 5;
+    if (ty == 2) hx = tx < 4 ? x0 - 1 : x0 + 64, hy = y0 + (tx & 3), hrow = 1 + (tx & 3), hcol = tx < 4 ? 0 : 65;
+    const bool hasH = ty < 2 || (ty == 2 && tx < 8);
+    const bool hIn = hx >= 0 && hx < X && hy >= 0 && hy < Y;
+    const int hxc = min(max(hx, 0), X - 1), hyc = min(max(hy, 0), Y - 1);
+    const unsigned hb = 4u * (unsigned)(hxc + X * hyc), hgb = 4u * (unsigned)hxc;
+    const int zb = max(z0 - 1, 0);
+    const unsigned PB = 4u * (unsigned)XY, RB = 4u * (unsigned)X;
+    const Rsrc rZ = arrayRsrc(zz + XY * zb), rS = arrayRsrc(s_old + XY * zb), rI = arrayRsrc(irho + XY * zb);
+    const Rsrc rG = arrayRsrc(G.gh + (long)X * zb);
+    const Rsrc wS = arrayRsrc(s_new + XY * zb), wA = arrayRsrc(As + XY * zb);
+    auto pOff = [&](int pl) { return PB * (unsigned)(pl - zb); };
+    auto gOff = [&](int pl) { return RB * (unsigned)(pl - zb); };
     struct Own {
         float z, s, i;
-        int gm, gf;
+        int gf;  // ground height of the next plane
     };
-    // A thread holds at most one halo row cell (below the tile for wave 0,
-    // above it for wave 1) and one halo column cell (lane 0 left, lane 63 right).
     struct Halo {
-        float rz, rs, ri, cz, cs, ci;
+        float z, s, i;
+        int g;  // ground height of the halo cell's column
     };
-    const bool hasRow = t.hasD || t.hasU, hasCol = t.hasL || t.hasR;
-    const unsigned rRow = t.hasD ? t.rD : t.rU, rCol = t.hasL ? t.rL : t.rR;
-    const int rowSlot = t.hasD ? 0 : 5, colSlot = t.hasL ? 0 : 65;
-    auto loadOwn = [&](int pl, Own& o) {
-        if (t.own && pl < Z) {
-            const long pz = XY * pl;
-            o.z = (zz + pz)[t.rc];
-            if (!FIRST) o.s = (s_old + pz)[t.rc];
-            o.i = (irho + pz)[t.rc];
-            o.gm = (G.ghm1 + X * pl)[(unsigned)x];
-            o.gf = (G.gh + X * min(pl + 1, Z - 1))[(unsigned)x];
+    auto loadOwn = [&](int pl, Own& o) __attribute__((always_inline)) {
+        const unsigned so = pOff(pl);
+        o.z = bldf(rZ, ob, so);
+        if (!FIRST) o.s = bldf(rS, ob, so);
+        o.i = bldf(rI, ob, so);
+        o.gf = bldi(rG, gb, gOff(min(pl + 1, Z - 1)));
+    };
+    auto loadHalo = [&](int pl, Halo& h) __attribute__((always_inline)) {
+        if (hasH) {
+            const unsigned so = pOff(pl);
+            h.z = bldf(rZ, hb, so);
+            if (!FIRST) h.s = bldf(rS, hb, so);
+            h.i = bldf(rI, hb, so);
+            h.g = bldi(rG, hgb, gOff(pl));
         }
     };
-    auto loadHalo = [&](int pl, Halo& h) {
-        const long pz = XY * pl;
-        if (hasRow) {
-            h.rz = (zz + pz)[rRow];
-            if (!FIRST) h.rs = (s_old + pz)[rRow];
-            h.ri = (irho + pz)[rRow];
-        }
-        if (hasCol) {
-            h.cz = (zz + pz)[rCol];
-            if (!FIRST) h.cs = (s_old + pz)[rCol];
-            h.ci = (irho + pz)[rCol];
-        }
-    };
-    Own oc{0.0f, 0.0f, 0.0f, 0, 0}, of{0.0f, 0.0f, 0.0f, 0, 0}, on{0.0f, 0.0f, 0.0f, 0, 0};
-    Halo hc{}, hn{};
     // plane below the chunk: s_new, and its front face (= this plane's back face)
     float snb = 0.0f, fb = 0.0f;
     bool skB = false;
-    int ghc = 0;
-    if (t.own) {
-        ghc = (G.gh + X * z0)[(unsigned)x];
-        if (z0 > 0 && y > (G.gh + X * (z0 - 1))[(unsigned)x]) {
-            const long pz = XY * (z0 - 1);
-            skB = true;
-            snb = snv((zz + pz)[t.rc], FIRST ? 0.0f : (s_old + pz)[t.rc]);
-            if (y > ghc) fb = faceI((irho + pz)[t.rc], (irho + XY * z0)[t.rc]);
-        }
+    int ghc = bldi(rG, gb, gOff(z0));
+    if (inb && z0 > 0 && y > bldi(rG, gb, gOff(z0 - 1))) {
+        skB = true;
+        const unsigned so = pOff(z0 - 1);
+        snb = snv(bldf(rZ, ob, so), FIRST ? 0.0f : bldf(rS, ob, so));
+        if (y > ghc) fb = faceI(bldf(rI, ob, so), bldf(rI, ob, pOff(z0)));
     }
-    loadOwn(z0, oc);
-    loadOwn(z0 + 1, of);
-    loadHalo(z0, hc);
-    bool skC = t.own && y > ghc;
+    bool skC = inb && y > ghc;
+    Own o0{0.0f, 0.0f, 0.0f, 0}, o1{0.0f, 0.0f, 0.0f, 0}, o2{0.0f, 0.0f, 0.0f, 0}, o3{0.0f, 0.0f, 0.0f, 0};
+    Halo h0{0.0f, 0.0f, 0.0f, 0}, h1{0.0f, 0.0f, 0.0f, 0};
+    loadOwn(z0, o0);
+    if (z0 + 1 < Z) loadOwn(z0 + 1, o1);
+    if (z0 + 2 <= z1 && z0 + 2 < Z) loadOwn(z0 + 2, o2);
+    loadHalo(z0, h0);
     double acc = 0.0;
-    for (int z = z0; z < z1; z++) {
-        const int buf = (z - z0) & 1;
-        if (z + 2 <= z1) loadOwn(z + 2, on);
+    int slot = 0;  // (z - z0) % 3
+    // plane z from oc, its front plane from of; plane z + 3 is read into o3n
+    auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, Halo& hc, Halo& hn) __attribute__((always_inline)) {
+        if (z + 3 <= z1 && z + 3 < Z) loadOwn(z + 3, o3n);
         if (z + 1 < z1) loadHalo(z + 1, hn);
+        const bool inZ = z + 1 < Z;
+        const bool skF = inb && inZ && y > oc.gf;
         const float snc = skC ? snv(oc.z, oc.s) : 0.0f;
         tS[buf][ty + 1][tx + 1] = snc;
-        tI[buf][ty + 1][tx + 1] = oc.i;
-        if (hasRow) {
-            tS[buf][rowSlot][tx + 1] = snv(hc.rz, hc.rs);
-            tI[buf][rowSlot][tx + 1] = hc.ri;
+        tI[buf][ty + 1][tx + 1] = skC ? oc.i : (inb ? INF : oc.i);
+        // this plane's sky values (own, front, halo) in the short form's range
+        bool bad = (skC && !faceValueOk(oc.i)) || (skF && !faceValueOk(of.i));
+        if (hasH) {
+            const bool sky = hIn && hy > hc.g;
+            tS[buf][hrow][hcol] = sky ? snv(hc.z, hc.s) : 0.0f;
+            tI[buf][hrow][hcol] = sky ? hc.i : (hIn ? INF : hc.i);
+            bad = bad || (sky && !faceValueOk(hc.i));
         }
-        if (hasCol) {
-            tS[buf][ty + 1][colSlot] = snv(hc.cz, hc.cs);
-            tI[buf][ty + 1][colSlot] = hc.ci;
-        }
+        if (__builtin_amdgcn_ballot_w64(bad)) flagBuf[slot] = 1;
+        if (tx == 0 && ty == 0) flagBuf[slot == 2 ? 0 : slot + 1] = 0;
         __syncthreads();
-        const bool skF = t.own && z + 1 < Z && y > oc.gf;
+        const bool exact = __builtin_amdgcn_readfirstlane(flagBuf[slot]) != 0;
+        slot = slot == 2 ? 0 : slot + 1;
         const float snf = skF ? snv(of.z, of.s) : 0.0f;
-        float ff = 0.0f;
-        if (t.own) {
-            float* sw = s_new + XY * z;
-            float* aw = As + XY * z;
-            if (skC) {
-                const float ic = oc.i;
-                const bool oL = x == 0, oR = x == X - 1, oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
-                bool sl = !oL, sr = !oR, sd = true;
-                if (y < oc.gm + 2) {
-                    const int* gz = G.gh + X * z;
-                    sl = sl && y > gz[x - 1];
-                    sr = sr && y > gz[x + 1];
-                    sd = y - 1 > ghc;
-                }
-                const bool su = !oU;
-                // every tile index is inside the tile: read unconditionally, select after
-                const float iR = tI[buf][ty + 1][tx + 2], iL = tI[buf][ty + 1][tx];
-                const float iU = tI[buf][ty + 2][tx + 1], iD = tI[buf][ty][tx + 1];
-                const float sR = tS[buf][ty + 1][tx + 2], sL = tS[buf][ty + 1][tx];
-                const float sU = tS[buf][ty + 2][tx + 1], sD = tS[buf][ty][tx + 1];
-                // faces to outside neighbours see the cell's own density
-                float own = 0.0f;
-                if (oL | oR | oU | oF | oB) own = faceI(ic, ic);
-                const float r = oR ? own : (sr ? faceI(ic, iR) : 0.0f);
-                const float l = oL ? own : (sl ? faceI(ic, iL) : 0.0f);
-                const float u = oU ? own : faceI(ic, iU);
-                const float d = sd ? faceI(ic, iD) : 0.0f;
-                const float f = oF ? own : (skF ? faceI(ic, of.i) : 0.0f);
-                const float bb = oB ? own : (skB ? fb : 0.0f);
-                float w = 0.0f;
-                w -= r;
-                w -= l;
-                w -= u;
-                w -= d;
-                w -= f;
-                w -= bb;
-                const float L = sl ? sL : 0.0f, R = sr ? sR : 0.0f;
-                const float D = sd ? sD : 0.0f, U = su ? sU : 0.0f;
-                const float aL = sl ? lo(l) : 0.0f, aD = sd ? lo(d) : 0.0f, aB = skB ? lo(bb) : 0.0f;
-                const float o = w * snc + ((aL * L + aD * D + aB * snb + r * R + u * U + f * snf));
-                ff = f;
-                sw[t.rc] = snc;
-                aw[t.rc] = o;
-                acc += (double)(o * snc);
+        float fcur = 0.0f;
+        if (skC) {
+            const float ic = oc.i;
+            const float iR = tI[buf][ty + 1][tx + 2], iL = tI[buf][ty + 1][tx];
+            const float iU = tI[buf][ty + 2][tx + 1], iD = tI[buf][ty][tx + 1];
+            const float sR = tS[buf][ty + 1][tx + 2], sL = tS[buf][ty + 1][tx];
+            const float sU = tS[buf][ty + 2][tx + 1], sD = tS[buf][ty][tx + 1];
+            const float iF = inZ ? (skF ? of.i : INF) : ic;
+            auto face = [&](float in) {
+                // computed before the select, so no branch skips it
+                const float v = faceFast(ic + in);
+                return isinf(in) ? 0.0f : v;
+            };
+            float r, l, u, d, f;
+            if (!exact) {
+                r = face(iR), l = face(iL), u = face(iU), d = face(iD), f = face(iF);
             } else {
-                sw[t.rc] = 0.0f;
-                aw[t.rc] = 0.0f;
+                r = faceI(ic, iR), l = faceI(ic, iL), u = faceI(ic, iU), d = faceI(ic, iD), f = faceI(ic, iF);
             }
+            float bb = skB ? fb : 0.0f;
+            if (z == 0) bb = faceI(ic, ic);
+            float w = 0.0f;
+            w -= r;
+            w -= l;
+            w -= u;
+            w -= d;
+            w -= f;
+            w -= bb;
+            const float aL = x == 0 ? 0.0f : lo(l), aD = lo(d), aB = skB ? lo(bb) : 0.0f;
+            const float o = w * snc + ((aL * sL + aD * sD + aB * snb + r * sR + u * sU + f * snf));
+            fcur = f;
+            bstf(wS, ob, pOff(z), snc);
+            bstf(wA, ob, pOff(z), o);
+            acc += (double)(o * snc);
+        } else if (inb) {
+            bstf(wS, ob, pOff(z), 0.0f);
+            bstf(wA, ob, pOff(z), 0.0f);
         }
         snb = snc;
-        fb = ff;
+        fb = fcur;
         skB = skC;
         skC = skF;
         ghc = oc.gf;
-        oc = of;
-        of = on;
-        hc = hn;
+    };
+    for (int zg = z0; zg < z1; zg += 4) {
+        step(zg, 0, o0, o1, o3, h0, h1);
+        if (zg + 1 < z1) step(zg + 1, 1, o1, o2, o0, h1, h0);
+        if (zg + 2 < z1) step(zg + 2, 0, o2, o3, o1, h0, h1);
+        if (zg + 3 < z1) step(zg + 3, 1, o3, o0, o2, h1, h0);
     }
     acc = blockSum(acc, sh);
     if (tid256() == 0) pout[bid] = acc;
@@ -896,20 +948,27 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
 hipError_t launch_pcg_A(EnvState& e, int it) {
     float* s_old = e.pcg[2 + ((it + 1) & 1)];
     float* s_new = e.pcg[2 + (it & 1)];
-    const ZGrid zg = zgrid(e);
+    const ZGrid zg = e.gA;
+    // byte offsets from the chunk's base plane stay below 4 GiB
+    if ((double)(zg.zc + 3) * (double)e.X * (double)e.Y * 4.0 >= 4294967296.0) return hipErrorInvalidValue;
     const size_t M = e.maxBlocks;
     const double* pm = e.partials + PART_BMAX * M;
     const double* ps = it == 0 ? e.partials + PART_START * M : e.partials + PART_BSUM * M;
-    const int trunc = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
-    if (it == 0)
-        hipLaunchKernelGGL(k_pcg_A<true>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho,
-                           e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it,
-                           trunc);
-    else
-        hipLaunchKernelGGL(k_pcg_A<false>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho,
-                           e.pcg[4], pm, ps, zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it,
-                           trunc);
+    const bool trunc = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
+    auto k = it == 0 ? (trunc ? k_pcg_A<true, true> : k_pcg_A<true, false>)
+                     : (trunc ? k_pcg_A<false, true> : k_pcg_A<false, false>);
+    hipLaunchKernelGGL(k, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho, e.pcg[4], pm, ps,
+                       zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it);
     return hipGetLastError();
+}
+
+// Blocks of the pressure operator kernel the device holds at once: its grid
+// is one such round (a second, partly filled round would double its time).
+long pcgA_resident_blocks(int device) {
+    int nb = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pcg_A<false, true>, 256, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    return (long)nb * cus;
 }
 
 // Prologue PRO_B, then updatePandDiv (Game/kernelSky.cu:955-985) + the
@@ -984,7 +1043,7 @@ hipError_t launch_pcg_B(EnvState& e, int it) {
     const ZGrid zg = zgrid(e);
     const size_t M = e.maxBlocks;
     hipLaunchKernelGGL(k_pcg_B<ZB_PCGB>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.f[SKT_PRESSURE_PROJ], e.pcg[0],
-                       s_new, e.pcg[4], e.pcg[5], e.pcg[1], e.partials + PART_A * M, zg.nblocks,
+                       s_new, e.pcg[4], e.pcg[5], e.pcg[1], e.partials + PART_A * M, e.gA.nblocks,
                        e.partials + PART_BMAX * M, e.partials + PART_BSUM * M, &e.sc->st[0], &e.sc->st[1], ref,
                        absmode);
     return hipGetLastError();
