@@ -55,6 +55,8 @@ typedef struct skt_config {
     int32_t maxabs_mode;       /* 0: getMaxDivergence as written (|.| at z=0 only); 1: max|.| */
     uint32_t noise_seed;       /* first of the five init-noise seeds (100 in Game/environment.cu:308) */
     int32_t init_noise;        /* 1: add the reference's init perturbations */
+    int32_t max_blocks;        /* upper bound on the blocks of one solver launch; 0 sizes launches from the
+                                  device. A small bound gives each block more z planes (or rows). */
 } skt_config;
 
 /* Fills a configuration with the reference's settings for an nx*ny*nz grid. */

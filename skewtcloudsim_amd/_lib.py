@@ -32,6 +32,7 @@ class Config(ctypes.Structure):
         ("device", ctypes.c_int32), ("pcg_mode", ctypes.c_int32), ("pcg_tol", ctypes.c_float),
         ("pcg_max_iter", ctypes.c_int32), ("diffusion_sweeps", ctypes.c_int32), ("max_substeps", ctypes.c_int32),
         ("maxabs_mode", ctypes.c_int32), ("noise_seed", ctypes.c_uint32), ("init_noise", ctypes.c_int32),
+        ("max_blocks", ctypes.c_int32),
     ]
 
 

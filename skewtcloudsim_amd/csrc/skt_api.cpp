@@ -397,6 +397,7 @@ void skt_config_default(skt_config* c, int32_t nx, int32_t ny, int32_t nz, float
     c->maxabs_mode = 0;
     c->noise_seed = 100u;
     c->init_noise = 1;
+    c->max_blocks = 0;
 }
 
 int skt_env_create(const skt_config* cfg, skt_env** out) {
@@ -404,7 +405,8 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     *out = nullptr;
     if (cfg->nx < 1 || cfg->ny < 2 || cfg->nz < 1 || !(cfg->dx > 0.0f)) return SKT_ERR_ARG;
     const long N = (long)cfg->nx * cfg->ny * cfg->nz;
-    if (N > (1L << 31) - 1 || cfg->diffusion_sweeps < 0 || cfg->pcg_max_iter < 0 || cfg->max_substeps < 0)
+    if (N > (1L << 31) - 1 || cfg->diffusion_sweeps < 0 || cfg->pcg_max_iter < 0 || cfg->max_substeps < 0 ||
+        cfg->max_blocks < 0)
         return SKT_ERR_ARG;
     skt_env* env = new skt_env();
     EnvState& e = env->s;
@@ -437,8 +439,8 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
         dalloc(e, &e.zeros, sizeof(float) * e.Y) || dalloc(e, &e.ones, sizeof(float) * e.Y) ||
         dalloc(e, &e.phase, (size_t)e.N))
         return bail(SKT_ERR_NOMEM);
-    e.maxBlocks = blocks3d(e);
-    e.gA = zgridResident(e, pcgA_resident_blocks(e.device));
+    setup_grids(e, pcgA_resident_blocks(e.device));
+    e.maxBlocks = std::max(blocks3d(e), (int)e.gB.nblocks);
     if (dalloc(e, &e.partials, sizeof(double) * PART_COUNT * (size_t)e.maxBlocks) ||
         dalloc(e, &e.region, sizeof(double) * 33 * (size_t)e.maxBlocks + 64 * sizeof(float)) ||
         dalloc(e, &e.sc, sizeof(Scalars)))

@@ -45,6 +45,11 @@ struct ZGrid {
     int zc;
     unsigned nblocks;
 };
+// Flat launch shape of the elementwise solver kernels: nblocks blocks of rpb
+// whole rows (x fastest) each.
+struct FGrid {
+    unsigned nblocks, rpb;
+};
 struct EnvState {
     skt_config cfg;
     int X, Y, Z;
@@ -67,7 +72,9 @@ struct EnvState {
     float* ones = nullptr;                  // [Y] unit profile
     double* partials = nullptr;             // per-block reduction partials (PART_COUNT x maxBlocks)
     int maxBlocks = 0;
+    ZGrid gZ{};                             // z-marching grid of the stencil kernels
     ZGrid gA{};                             // grid of the pressure operator kernel
+    FGrid gB{};                             // grid of the elementwise solver kernels
     Scalars* sc = nullptr;                  // device scalars
     Scalars* sc_host = nullptr;             // pinned mirror for readbacks
     uint8_t* phase = nullptr;               // [N] microphysics phase mask
@@ -97,8 +104,8 @@ enum { TC_DIFFUSE = 0, TC_PPM, TC_PCG_A, TC_PCG_B, TC_MICRO, TC_MAXVEL, TC_PRECI
 // ------------------------------------------------------------- launches ---
 Geom geom(const EnvState& e);
 ZGrid zgrid(const EnvState& e);
-// The same tiles with at most 'resident' blocks (one round on the device).
-ZGrid zgridResident(const EnvState& e, long resident);
+// Fills gZ, gA (at most residentA blocks: one round on the device) and gB.
+void setup_grids(EnvState& e, long residentA);
 long pcgA_resident_blocks(int device);
 hipError_t launch_fill(EnvState& e, float* a, float v, long n);
 hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst);

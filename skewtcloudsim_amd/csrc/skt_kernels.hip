@@ -33,22 +33,28 @@ int blocks3d(const EnvState& e) {
     return (int)(g.x * g.y * g.z);
 }
 Geom geom(const EnvState& e) { return Geom{e.X, e.Y, e.Z, e.N, e.gh, e.ghm1, e.ghm2}; }
-ZGrid zgridChunks(const EnvState& e, long nzc) {
-    const int gx = (e.X + 63) / 64, gy = (e.Y + 3) / 4;
-    const long cols = (long)gx * gy;
+static long zcols(const EnvState& e) { return (long)((e.X + 63) / 64) * ((e.Y + 3) / 4); }
+// nzc chunks of equal length along z (the last one shorter)
+static ZGrid zgridChunks(const EnvState& e, long nzc) {
     nzc = std::max(1L, std::min((long)e.Z, nzc));
     const int zc = (int)((e.Z + nzc - 1) / nzc);
     nzc = (e.Z + zc - 1) / zc;
-    const unsigned nb = (unsigned)(cols * nzc);
+    const unsigned nb = (unsigned)(zcols(e) * nzc);
     return ZGrid{dim3(nb), zc, nb};
 }
-ZGrid zgrid(const EnvState& e) {
-    const long cols = (long)((e.X + 63) / 64) * ((e.Y + 3) / 4);
-    return zgridChunks(e, (2048 + cols - 1) / cols);
-}
-ZGrid zgridResident(const EnvState& e, long resident) {
-    const long cols = (long)((e.X + 63) / 64) * ((e.Y + 3) / 4);
-    return resident > 0 ? zgridChunks(e, resident / cols) : zgrid(e);
+ZGrid zgrid(const EnvState& e) { return e.gZ; }
+void setup_grids(EnvState& e, long residentA) {
+    const long cols = zcols(e), cap = e.cfg.max_blocks;
+    // about 2048 blocks (at least one chunk per tile column), or at most cap
+    e.gZ = zgridChunks(e, cap > 0 ? cap / cols : (2048 + cols - 1) / cols);
+    // the pressure operator: one round of resident blocks
+    long a = residentA > 0 ? residentA : 2048;
+    if (cap > 0) a = std::min(a, cap);
+    e.gA = zgridChunks(e, a / cols);
+    // the flat walks: about 2048 blocks of whole rows
+    const long rows = (long)e.Y * e.Z, nb = std::max(1L, std::min(rows, cap > 0 ? std::min(2048L, cap) : 2048L));
+    const long rpb = (rows + nb - 1) / nb;
+    e.gB = FGrid{(unsigned)((rows + rpb - 1) / rpb), (unsigned)rpb};
 }
 
 // Boundary value of a stencil read (the extv rule): ground condition if the
@@ -605,12 +611,33 @@ hipError_t launch_build_precon(EnvState& e) {
 //              the s.As product from the second iteration on), alpha
 //   PRO_FINAL: stop test after the last iteration
 enum { PRO_START = 0, PRO_A0, PRO_A, PRO_B, PRO_FINAL };
-__device__ void pcgPrologue(int kind, const double* __restrict__ pm, const double* __restrict__ ps, unsigned n,
-                            const PcgState* __restrict__ sin, PcgState* __restrict__ sout, int ref, float tol, int it,
-                            PcgState& S, double* sh) {
+// A lane's share of the previous kernel's partials: elements t, t + 256, ...
+// below 2048 (later ones are read in the prologue itself). Loading them
+// before the kernel issues its first data loads lets both be in flight while
+// the block waits; the prologue then reduces them.
+struct PcgPart {
+    double m[8], s[8];
+};
+template <int KIND>
+__device__ __forceinline__ void pcgPartLoad(const double* __restrict__ pm, const double* __restrict__ ps, unsigned n,
+                                            PcgPart& P) {
+    const unsigned t = tid256();
+    constexpr bool wantM = KIND == PRO_START || KIND == PRO_A || KIND == PRO_FINAL;
+    constexpr bool wantS = KIND == PRO_A0 || KIND == PRO_A || KIND == PRO_B;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const unsigned j = t + 256u * k;
+        P.m[k] = wantM && j < n ? pm[j] : 0.0;
+        P.s[k] = wantS && j < n ? ps[j] : 0.0;
+    }
+}
+template <int KIND>
+__device__ void pcgPrologue(const double* __restrict__ pm, const double* __restrict__ ps, unsigned n,
+                            const PcgPart& P, const PcgState* __restrict__ sin, PcgState* __restrict__ sout, int ref,
+                            float tol, int it, PcgState& S, double* sh) {
     const int t = tid256();
     if (t == 0) {
-        if (kind == PRO_START) {
+        if (KIND == PRO_START) {
             S = PcgState{};
         } else {
             S = *sin;
@@ -619,16 +646,20 @@ __device__ void pcgPrologue(int kind, const double* __restrict__ pm, const doubl
     __syncthreads();
     if (!S.done) {
         double vmax = 0.0, vsum = 0.0;
-        if (kind == PRO_START || kind == PRO_A || kind == PRO_FINAL) {
-            for (unsigned j = t; j < n; j += 256) vmax = fmax(vmax, pm[j]);
+        if (KIND == PRO_START || KIND == PRO_A || KIND == PRO_FINAL) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) vmax = fmax(vmax, P.m[k]);
+            for (unsigned j = t + 2048; j < n; j += 256) vmax = fmax(vmax, pm[j]);
             vmax = blockMax(vmax, sh);
         }
-        if (kind == PRO_A0 || kind == PRO_A || kind == PRO_B) {
-            for (unsigned j = t; j < n; j += 256) vsum += ps[j];
+        if (KIND == PRO_A0 || KIND == PRO_A || KIND == PRO_B) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) vsum += P.s[k];
+            for (unsigned j = t + 2048; j < n; j += 256) vsum += ps[j];
             vsum = blockSum(vsum, sh);
         }
         if (t == 0) {
-            switch (kind) {
+            switch (KIND) {
                 case PRO_START: {
                     const float m0 = (float)vmax;
                     S.maxr0 = m0;
@@ -648,7 +679,7 @@ __device__ void pcgPrologue(int kind, const double* __restrict__ pm, const doubl
                     S.iters = it;
                     if (mr <= S.tol) {
                         S.done = 1;
-                    } else if (kind == PRO_A) {
+                    } else if (KIND == PRO_A) {
                         const float s1 = (float)vsum;
                         S.sigma1 = (double)s1;
                         S.beta = s1 / (float)S.sigma0;
@@ -729,7 +760,9 @@ KL void k_pcg_start(Geom G, int zc, const float* __restrict__ r, float* __restri
     ZM_SETUP;
     __shared__ double sh[4];
     __shared__ PcgState S;
-    pcgPrologue(PRO_START, pin, nullptr, n, sin, sout, ref, tol, 0, S, sh);
+    PcgPart P;
+    pcgPartLoad<PRO_START>(pin, nullptr, n, P);
+    pcgPrologue<PRO_START>(pin, nullptr, n, P, sin, sout, ref, tol, 0, S, sh);
     if (S.done) return;
     double s = 0.0;
     if (inb) {
@@ -789,15 +822,16 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
     __shared__ double sh[4];
     __shared__ PcgState S;
     __shared__ float tS[2][TH][TW], tI[2][TH][TW];
-    // flag of plane k (counted from z0) in slot k % 3: set before the plane's
-    // barrier, read after it, cleared by thread 0 before barrier k - 1 + ... :
-    // slot (k + 1) % 3 is cleared at plane k before its barrier, after every
-    // reader of plane k - 2 has passed barrier k - 1.
+    // Flag of plane k (counted from z0) in slot k % 3: writers set it before
+    // the plane's barrier, every thread reads it after. Thread 0 clears slot
+    // (k + 1) % 3 at plane k before its barrier; the slot's last readers (of
+    // plane k - 2) have all passed barrier k - 1 by then.
     __shared__ int flagBuf[3];
     if (threadIdx.y == 0 && threadIdx.x < 3) flagBuf[threadIdx.x] = 0;
-    pcgPrologue(FIRST ? PRO_A0 : PRO_A, pm, ps, n, sin, sout, 0, 0.0f, it, S, sh);
-    if (S.done) return;
-    const float beta = S.beta;
+    constexpr int KIND = FIRST ? PRO_A0 : PRO_A;
+    PcgPart part;
+    pcgPartLoad<KIND>(pm, ps, n, part);
+    float beta = 0.0f;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int X = G.X, Y = G.Y, Z = G.Z;
     const long XY = (long)X * Y;
@@ -850,23 +884,35 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
             h.g = bldi(rG, hgb, gOff(pl));
         }
     };
-    // plane below the chunk: s_new, and its front face (= this plane's back face)
-    float snb = 0.0f, fb = 0.0f;
-    bool skB = false;
-    int ghc = bldi(rG, gb, gOff(z0));
-    if (inb && z0 > 0 && y > bldi(rG, gb, gOff(z0 - 1))) {
-        skB = true;
+    // the chunk's first planes and the plane below it are read before the
+    // prologue; s_new and the back face of the plane below need beta
+    int ghc = bldi(rG, gb, gOff(z0)), ghb = 0;
+    float bz = 0.0f, bs = 0.0f, bi = 0.0f;
+    if (z0 > 0) {
         const unsigned so = pOff(z0 - 1);
-        snb = snv(bldf(rZ, ob, so), FIRST ? 0.0f : bldf(rS, ob, so));
-        if (y > ghc) fb = faceI(bldf(rI, ob, so), bldf(rI, ob, pOff(z0)));
+        ghb = bldi(rG, gb, gOff(z0 - 1));
+        bz = bldf(rZ, ob, so);
+        if (!FIRST) bs = bldf(rS, ob, so);
+        bi = bldf(rI, ob, so);
     }
-    bool skC = inb && y > ghc;
     Own o0{0.0f, 0.0f, 0.0f, 0}, o1{0.0f, 0.0f, 0.0f, 0}, o2{0.0f, 0.0f, 0.0f, 0}, o3{0.0f, 0.0f, 0.0f, 0};
     Halo h0{0.0f, 0.0f, 0.0f, 0}, h1{0.0f, 0.0f, 0.0f, 0};
     loadOwn(z0, o0);
     if (z0 + 1 < Z) loadOwn(z0 + 1, o1);
     if (z0 + 2 <= z1 && z0 + 2 < Z) loadOwn(z0 + 2, o2);
     loadHalo(z0, h0);
+    pcgPrologue<KIND>(pm, ps, n, part, sin, sout, 0, 0.0f, it, S, sh);
+    if (S.done) return;
+    beta = S.beta;
+    // plane below the chunk: s_new, and its front face (= this plane's back face)
+    float snb = 0.0f, fb = 0.0f;
+    bool skB = false;
+    if (inb && z0 > 0 && y > ghb) {
+        skB = true;
+        snb = snv(bz, bs);
+        if (y > ghc) fb = faceI(bi, o0.i);
+    }
+    bool skC = inb && y > ghc;
     double acc = 0.0;
     int slot = 0;  // (z - z0) % 3
     // plane z from oc, its front plane from of; plane z + 3 is read into o3n
@@ -952,13 +998,15 @@ hipError_t launch_pcg_A(EnvState& e, int it) {
     // byte offsets from the chunk's base plane stay below 4 GiB
     if ((double)(zg.zc + 3) * (double)e.X * (double)e.Y * 4.0 >= 4294967296.0) return hipErrorInvalidValue;
     const size_t M = e.maxBlocks;
+    // the first iteration reduces pcg_start's partials, later ones pcg_B's
     const double* pm = e.partials + PART_BMAX * M;
     const double* ps = it == 0 ? e.partials + PART_START * M : e.partials + PART_BSUM * M;
+    const unsigned np = it == 0 ? e.gZ.nblocks : e.gB.nblocks;
     const bool trunc = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     auto k = it == 0 ? (trunc ? k_pcg_A<true, true> : k_pcg_A<true, false>)
                      : (trunc ? k_pcg_A<false, true> : k_pcg_A<false, false>);
     hipLaunchKernelGGL(k, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho, e.pcg[4], pm, ps,
-                       zg.nblocks, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it);
+                       np, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it);
     return hipGetLastError();
 }
 
@@ -974,75 +1022,104 @@ long pcgA_resident_blocks(int device) {
 // Prologue PRO_B, then updatePandDiv (Game/kernelSky.cu:955-985) + the
 // stop-test partials (Game/environment.cu:963-971) + applyPreconditionerGPU
 // + z.r partials: p' += alpha s, r -= alpha As, z = M^-1 r.
-constexpr int ZB_PCGB = 4;
-template <int ZB>
-KL void k_pcg_B(Geom G, int zc, float* __restrict__ pp, float* __restrict__ r, const float* __restrict__ s,
+// Elementwise, so a block walks whole rows (x fastest) of the Y*Z rows:
+// blockIdx.x takes rows [b rpb, (b + 1) rpb), V cells per lane step (V = 4
+// when X is a multiple of 4: 16-byte accesses). Lanes advance by 256 groups
+// and carry their (column, y, z) along. Ground cells keep p and r (the same
+// bits are stored back) and get z = 0. The partials of pcg_A and then the
+// lane's first two groups are read before the prologue reduces the partials;
+// after that each group's data is read two groups ahead (two register slots,
+// the loop handles one group per slot per trip).
+template <int V>
+KL void k_pcg_B(Geom G, unsigned rpb, float* __restrict__ pp, float* __restrict__ r, const float* __restrict__ s,
                 const float* __restrict__ As, const float* __restrict__ pre, float* __restrict__ zz,
                 const double* __restrict__ pin, unsigned n, double* pmax, double* psum, const PcgState* sin,
                 PcgState* sout, int ref, int absmode) {
-    ZM_SETUP;
+    typedef float __attribute__((ext_vector_type(V))) fv;
+    typedef int __attribute__((ext_vector_type(V))) iv;
     __shared__ double sh[4];
     __shared__ PcgState S;
-    pcgPrologue(PRO_B, nullptr, pin, n, sin, sout, ref, 0.0f, 0, S, sh);
+    PcgPart part;
+    pcgPartLoad<PRO_B>(nullptr, pin, n, part);
+    const int X = G.X, Y = G.Y, q = G.X / V;
+    const long rows = (long)Y * G.Z;
+    const long r0 = (long)blockIdx.x * rpb, r1 = std::min(rows, r0 + (long)rpb);
+    const int t = tid256();
+    const long gend = r1 * q;
+    struct Grp {
+        fv P, R, S, A, Q;
+    };
+    auto load = [&](long gk, Grp& d) __attribute__((always_inline)) {
+        if (gk < gend) {
+            d.P = ((const fv*)pp)[gk];
+            d.R = ((const fv*)r)[gk];
+            d.S = ((const fv*)s)[gk];
+            d.A = ((const fv*)As)[gk];
+            d.Q = ((const fv*)pre)[gk];
+        }
+    };
+    long g = r0 * q + t;
+    Grp d0, d1;
+    load(g, d0);
+    load(g + 256, d1);
+    pcgPrologue<PRO_B>(nullptr, pin, n, part, sin, sout, ref, 0.0f, 0, S, sh);
     if (S.done) return;
     const float alpha = S.alpha;
-    double m = 0.0, dz = 0.0;
-    if (inb) {
-        const long XY = (long)G.X * G.Y;
-        const unsigned rc = x + G.X * y;
-        auto load = [&](int z, float& P, float& R, float& Sv, float& Av, float& Pr, int& gh) {
-            const long pz = XY * z;
-            P = (pp + pz)[rc];
-            R = (r + pz)[rc];
-            Sv = (s + pz)[rc];
-            Av = (As + pz)[rc];
-            Pr = (pre + pz)[rc];
-            gh = (G.gh + G.X * z)[(unsigned)x];
-        };
-        auto plane = [&](int z, float P, float R, float Sv, float Av, float Pr, int gh) {
-            const long pz = XY * z;
-            float zv = 0.0f;
-            if (y > gh) {
-                (pp + pz)[rc] = P + alpha * Sv;
-                const float rv = R - alpha * Av;
-                (r + pz)[rc] = rv;
-                const bool ab = absmode || z == 0;
-                m = fmax(m, (double)(ab ? fabsf(rv) : rv));
-                zv = rv * Pr;
-                dz += (double)(zv * rv);
+    float mf = 0.0f;
+    double dz = 0.0;
+    if (r0 < r1) {
+        const long row = g / q;
+        int gx = (int)(g - row * q), y = (int)(row % Y), z = (int)(row / Y);
+        // a step of 256 groups: dr whole rows and dgx groups
+        const int dr = 256 / q, dgx = 256 % q;
+        auto group = [&](long gk, const Grp& d) __attribute__((always_inline)) {
+            const iv gh = *(const iv*)(G.gh + gx * V + (long)X * z);
+            fv Po = d.P, Ro = d.R, Zo;
+            const bool ab = absmode || z == 0;
+#pragma unroll
+            for (int c = 0; c < V; c++) {
+                float zv = 0.0f;
+                if (y > gh[c]) {
+                    Po[c] = d.P[c] + alpha * d.S[c];
+                    const float rv = d.R[c] - alpha * d.A[c];
+                    Ro[c] = rv;
+                    mf = fmaxf(mf, ab ? fabsf(rv) : rv);
+                    zv = rv * d.Q[c];
+                    dz += (double)(zv * rv);
+                }
+                Zo[c] = zv;
             }
-            (zz + pz)[rc] = zv;
+            ((fv*)pp)[gk] = Po;
+            ((fv*)r)[gk] = Ro;
+            ((fv*)zz)[gk] = Zo;
+            gx += dgx;
+            y += dr;
+            if (gx >= q) gx -= q, ++y;
+            while (y >= Y) y -= Y, ++z;
         };
-        int zg = z0;
-        for (; zg + ZB <= z1; zg += ZB) {
-            float P[ZB], R[ZB], Sv[ZB], Av[ZB], Pr[ZB];
-            int gh[ZB];
-#pragma unroll
-            for (int j = 0; j < ZB; j++) load(zg + j, P[j], R[j], Sv[j], Av[j], Pr[j], gh[j]);
-#pragma unroll
-            for (int j = 0; j < ZB; j++) plane(zg + j, P[j], R[j], Sv[j], Av[j], Pr[j], gh[j]);
-        }
-        for (; zg < z1; zg++) {
-            float P, R, Sv, Av, Pr;
-            int gh;
-            load(zg, P, R, Sv, Av, Pr, gh);
-            plane(zg, P, R, Sv, Av, Pr, gh);
+        for (; g < gend; g += 512) {
+            group(g, d0);
+            load(g + 512, d0);
+            if (g + 256 < gend) {
+                group(g + 256, d1);
+                load(g + 768, d1);
+            }
         }
     }
-    m = blockMax(m, sh);
+    const double m = blockMax((double)mf, sh);
     dz = blockSum(dz, sh);
-    if (tid256() == 0) {
-        pmax[bid] = m;
-        psum[bid] = dz;
+    if (t == 0) {
+        pmax[blockIdx.x] = m;
+        psum[blockIdx.x] = dz;
     }
 }
 hipError_t launch_pcg_B(EnvState& e, int it) {
     float* s_new = e.pcg[2 + (it & 1)];
     const int ref = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     const int absmode = ref ? e.cfg.maxabs_mode : 1;
-    const ZGrid zg = zgrid(e);
     const size_t M = e.maxBlocks;
-    hipLaunchKernelGGL(k_pcg_B<ZB_PCGB>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.f[SKT_PRESSURE_PROJ], e.pcg[0],
+    auto k = e.X % 4 == 0 ? k_pcg_B<4> : k_pcg_B<1>;
+    hipLaunchKernelGGL(k, dim3(e.gB.nblocks), B3, 0, e.stream, geom(e), e.gB.rpb, e.f[SKT_PRESSURE_PROJ], e.pcg[0],
                        s_new, e.pcg[4], e.pcg[5], e.pcg[1], e.partials + PART_A * M, e.gA.nblocks,
                        e.partials + PART_BMAX * M, e.partials + PART_BSUM * M, &e.sc->st[0], &e.sc->st[1], ref,
                        absmode);
@@ -1053,12 +1130,13 @@ hipError_t launch_pcg_B(EnvState& e, int it) {
 KL void k_pcg_final(const double* __restrict__ pm, unsigned n, const PcgState* sin, PcgState* sout, int it) {
     __shared__ double sh[4];
     __shared__ PcgState S;
-    pcgPrologue(PRO_FINAL, pm, nullptr, n, sin, sout, 0, 0.0f, it, S, sh);
+    PcgPart P;
+    pcgPartLoad<PRO_FINAL>(pm, nullptr, n, P);
+    pcgPrologue<PRO_FINAL>(pm, nullptr, n, P, sin, sout, 0, 0.0f, it, S, sh);
 }
 hipError_t launch_pcg_final(EnvState& e, int it) {
-    const ZGrid zg = zgrid(e);
     hipLaunchKernelGGL(k_pcg_final, dim3(1), B3, 0, e.stream, e.partials + (size_t)PART_BMAX * e.maxBlocks,
-                       zg.nblocks, &e.sc->st[1], &e.sc->st[0], it);
+                       e.gB.nblocks, &e.sc->st[1], &e.sc->st[0], it);
     return hipGetLastError();
 }
 

@@ -357,3 +357,47 @@ def test_terrain_full_steps(shape, kind):
         assert_fields(env, orc, GROUND, np.ones(nx * nz, bool), l2=tol, maxabs=50 * tol, tag=f"terrain-ground{s}")
     env.close()
     orc.close()
+
+
+# Launch shapes. The default grids of these small domains give every block a
+# single z plane; max_blocks bounds the blocks of a launch, so each block walks
+# a longer z chunk (and the flat solver kernels longer row runs). For the 6
+# tile columns of a 20 x 24 plane: 1 -> one 16-plane chunk, 18 -> chunks of
+# 6, 6, 4, 42 -> chunks of 3 (the last of 1).
+LAUNCH_CAPS = [1, 18, 42]
+
+
+@pytest.mark.parametrize("cap", LAUNCH_CAPS)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_launch_shapes_pressure(mode, cap):
+    env, orc = make_pair(NX, NY, NZ, dx=400.0, pcg_mode=mode, pcg_tol=1e-5 if mode == 0 else 1e-6,
+                         pcg_max_iter=60, max_blocks=cap)
+    rich_state(env, orc, seed=6)
+    env.initDensity()
+    orc.init_density()
+    env.pressureProject(0.5)
+    orc.pressure_project(0.5)
+    assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+    m = sky_mask(NX, NY, NZ)
+    assert_fields(env, orc, ("PRESSURE_PROJ",), m, l2=1e-4, maxabs=1e-3, tag=f"pcg-cap{cap}")
+    assert_fields(env, orc, ("VELX", "VELY", "VELZ", "PRESSURE", "DENSITY"), m, l2=1e-5, maxabs=1e-4,
+                  tag=f"proj-cap{cap}")
+    env.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("cap", LAUNCH_CAPS)
+def test_launch_shapes_terrain_steps(cap):
+    nx, ny, nz = NX, NY, NZ
+    gh = _terrain(nx, nz, "random")
+    env, orc = make_terrain_pair(nx, ny, nz, gh, max_blocks=cap)
+    m = sky_mask(nx, ny, nz, gh)
+    for s in range(3):
+        env.updateGPU(0.5, 1.0)
+        orc.step(0.5, 1.0)
+        tol = 1e-5 if s == 0 else 1e-4
+        assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+        assert_fields(env, orc, SKY, m, l2=tol, maxabs=50 * tol, tag=f"cap{cap}-step{s}")
+        assert_fields(env, orc, GROUND, np.ones(nx * nz, bool), l2=tol, maxabs=50 * tol, tag=f"cap{cap}-ground{s}")
+    env.close()
+    orc.close()
