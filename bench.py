@@ -11,7 +11,8 @@ bracketed by device synchronisation.
 
 Also measured: the same step with the textbook Jacobi-PCG at a relative
 residual of 1e-6 (the solver the workload description names), and the CPU
-baseline (the oracle, one step of the same workload on the host cores).
+baseline (the oracle, one step of the same workload on the host cores), whose
+result is compared with one GPU step from the same initial state.
 
 Prints one JSON line: cell-steps/s (value), the roofline of the dominant
 kernel class (algorithmic bytes per launch / average launch duration from HIP
@@ -45,7 +46,15 @@ def algorithmic_bytes(cls, n_sky):
     return per.get(cls, 0) * n_sky
 
 
+SKY_FIELDS = ("QV", "QW", "QC", "QR", "QS", "QI", "POTTEMP", "VELX", "VELY", "VELZ", "PRESSURE", "DENSITY")
+
+
 def cpu_baseline(snd_args, threads):
+    """The oracle's step timed on the host; its result also checks one GPU step
+    of the same workload from the same initial state (relative L2 over the sky
+    cells per field, PCG iteration counts)."""
+    import numpy as np
+    import skewtcloudsim_amd as skt
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_bind import Oracle
     o = Oracle(NX, NY, NZ, DX, threads=threads)
@@ -54,8 +63,19 @@ def cpu_baseline(snd_args, threads):
     o.step(DT, 1.0)
     t1 = time.perf_counter()
     it = o.stats()["pcg_iterations"]
+    env = skt.EnvironmentGPU(NX, NY, NZ, DX)
+    env.init(*snd_args)
+    env.updateGPU(DT, 1.0)
+    sky = (np.arange(NX * NY * NZ) // NX) % NY >= 1
+    rel = {}
+    for f in SKY_FIELDS:
+        a, b = env.get(f)[sky].astype(np.float64), o.get(f)[sky].astype(np.float64)
+        rel[f] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+    check = {"rel_l2": rel, "max_rel_l2": max(rel.values()),
+             "pcg_iterations": {"gpu": int(env.stats().pcg_iterations), "oracle": int(it)}}
+    env.close()
     o.close()
-    return (NX * NY * NZ) / (t1 - t0), t1 - t0, it
+    return (NX * NY * NZ) / (t1 - t0), t1 - t0, it, check
 
 
 def traffic_from_profile(kernel_class):
@@ -156,9 +176,10 @@ def main():
     standard = None if args.no_standard else run_standard(snd_args, min(args.steps, 5), 2)
 
     cpu = None
+    parity = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cps, secs, it = cpu_baseline(snd_args, threads)
+        cps, secs, it, parity = cpu_baseline(snd_args, threads)
         cpu = {"value": cps, "unit": "cell-steps/s", "cores": threads, "kind": "port",
                "sample": f"one step of the same {NX}x{NY}x{NZ} workload after init ({secs:.1f} s, "
                          f"{it} PCG iterations), oracle/skt_oracle.cpp FP32, OpenMP {threads} threads"}
@@ -191,6 +212,7 @@ def main():
         "pcg_standard_1e-6": standard,
         "copy_probe": {"GBps": copy_bytes / (copy_ms / 1e3) / 1e9, "bytes": copy_bytes, "ms": copy_ms},
         "cpu_baseline": cpu,
+        "full_size_parity_one_step": parity,
     }
     env.close()
     print(json.dumps(out))

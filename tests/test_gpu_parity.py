@@ -401,3 +401,27 @@ def test_launch_shapes_terrain_steps(cap):
         assert_fields(env, orc, GROUND, np.ones(nx * nz, bool), l2=tol, maxabs=50 * tol, tag=f"cap{cap}-ground{s}")
     env.close()
     orc.close()
+
+
+# A mid-size domain with the default launch shapes: 32 tile columns, so the
+# stencil kernels walk 2-plane chunks, the pressure operator 3-plane chunks
+# (one resident round of blocks) and the flat solver kernels 4-row runs. The
+# two steps run the reference's full 200 PCG iterations.
+@pytest.mark.parametrize("terrain", [False, True])
+def test_mid_size_steps(terrain):
+    nx, ny, nz = 128, 64, 128
+    if terrain:
+        gh = _terrain(nx, nz, "hill")
+        env, orc = make_terrain_pair(nx, ny, nz, gh, dx=100.0, seed=12)
+    else:
+        gh = None
+        env, orc = make_pair(nx, ny, nz, dx=100.0, seed=12, n_bubbles=3)
+    m = sky_mask(nx, ny, nz, gh)
+    for s in range(2):
+        env.updateGPU(0.5, 1.0)
+        orc.step(0.5, 1.0)
+        tol = 1e-5 if s == 0 else 1e-4
+        assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+        assert_fields(env, orc, SKY, m, l2=tol, maxabs=50 * tol, tag=f"mid{'-terrain' if terrain else ''}-step{s}")
+    env.close()
+    orc.close()
