@@ -35,6 +35,13 @@ SEED = 20261203
 HBM_PEAK_GBS = 8000.0
 
 
+# Fields one launch of a batched class moves, on average over a step: the
+# diffusion launches sweep 1, 3 and 6 fields (20 sweeps each), the PPM passes
+# 1 (theta), 1, 1, 1 (u, v, w), 1 (vapour) and 5 fields (5 passes each), the
+# sedimentation launch 3 fields.
+FIELDS_PER_LAUNCH = {"diffusion_sweep": 10.0 / 3.0, "ppm_pass": 10.0 / 6.0, "precip_fall": 3.0}
+
+
 def algorithmic_bytes(cls, n_sky):
     """Compulsory HBM bytes of one launch of a kernel class (FP32, per sky cell):
     diffusion sweep 8 (read x, write x); PPM pass 12 (read q and the face
@@ -42,8 +49,9 @@ def algorithmic_bytes(cls, n_sky):
     PCG spmv 20 (read z, s_old and the density-derived operator as 4 B of
     density, write s and As); PCG update 32 (read p, s, r, As, 1/diag; write
     p, r, z); microphysics 64 (9 fields in, 7 out)."""
-    per = {"diffusion_sweep": 8, "ppm_pass": 12, "pcg_spmv": 20, "pcg_update": 32, "microphysics": 64}
-    return per.get(cls, 0) * n_sky
+    per = {"diffusion_sweep": 8, "ppm_pass": 12, "pcg_spmv": 20, "pcg_update": 32, "microphysics": 64,
+           "precip_fall": 20}
+    return per.get(cls, 0) * n_sky * FIELDS_PER_LAUNCH.get(cls, 1.0)
 
 
 SKY_FIELDS = ("QV", "QW", "QC", "QR", "QS", "QI", "POTTEMP", "VELX", "VELY", "VELZ", "PRESSURE", "DENSITY")

@@ -439,7 +439,7 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
         dalloc(e, &e.zeros, sizeof(float) * e.Y) || dalloc(e, &e.ones, sizeof(float) * e.Y) ||
         dalloc(e, &e.phase, (size_t)e.N))
         return bail(SKT_ERR_NOMEM);
-    setup_grids(e, pcgA_resident_blocks(e.device));
+    setup_grids(e, resident_blocks(e.device));
     e.maxBlocks = std::max(blocks3d(e), (int)e.gB.nblocks);
     if (dalloc(e, &e.partials, sizeof(double) * PART_COUNT * (size_t)e.maxBlocks) ||
         dalloc(e, &e.region, sizeof(double) * 33 * (size_t)e.maxBlocks + 64 * sizeof(float)) ||

@@ -74,6 +74,7 @@ struct EnvState {
     int maxBlocks = 0;
     ZGrid gZ{};                             // z-marching grid of the stencil kernels
     ZGrid gA{};                             // grid of the pressure operator kernel
+    ZGrid gD{};                             // grid of a diffusion sweep (per field)
     FGrid gB{};                             // grid of the elementwise solver kernels
     Scalars* sc = nullptr;                  // device scalars
     Scalars* sc_host = nullptr;             // pinned mirror for readbacks
@@ -104,9 +105,14 @@ enum { TC_DIFFUSE = 0, TC_PPM, TC_PCG_A, TC_PCG_B, TC_MICRO, TC_MAXVEL, TC_PRECI
 // ------------------------------------------------------------- launches ---
 Geom geom(const EnvState& e);
 ZGrid zgrid(const EnvState& e);
-// Fills gZ, gA (at most residentA blocks: one round on the device) and gB.
-void setup_grids(EnvState& e, long residentA);
-long pcgA_resident_blocks(int device);
+// Blocks the device holds at once of the pressure operator kernel and of a
+// diffusion sweep (0 if unknown).
+struct ResidentBlocks {
+    long pcgA, diffuse;
+};
+ResidentBlocks resident_blocks(int device);
+// Fills gZ, gA and gD (one round of resident blocks each) and gB.
+void setup_grids(EnvState& e, ResidentBlocks rb);
 hipError_t launch_fill(EnvState& e, float* a, float v, long n);
 hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst);
 hipError_t launch_ground_water_snapshot(EnvState& e, float dt, float speed);

@@ -43,14 +43,22 @@ static ZGrid zgridChunks(const EnvState& e, long nzc) {
     return ZGrid{dim3(nb), zc, nb};
 }
 ZGrid zgrid(const EnvState& e) { return e.gZ; }
-void setup_grids(EnvState& e, long residentA) {
+void setup_grids(EnvState& e, ResidentBlocks rb) {
     const long cols = zcols(e), cap = e.cfg.max_blocks;
     // about 2048 blocks (at least one chunk per tile column), or at most cap
     e.gZ = zgridChunks(e, cap > 0 ? cap / cols : (2048 + cols - 1) / cols);
-    // the pressure operator: one round of resident blocks
-    long a = residentA > 0 ? residentA : 2048;
-    if (cap > 0) a = std::min(a, cap);
-    e.gA = zgridChunks(e, a / cols);
+    // the pressure operator and a diffusion sweep (per field): one round of
+    // resident blocks
+    auto round = [&](long r) {
+        long a = r > 0 ? r : 2048;
+        if (cap > 0) a = std::min(a, cap);
+        return zgridChunks(e, a / cols);
+    };
+    e.gA = round(rb.pcgA);
+    // a diffusion chunk is at most 64 planes (one bit per plane marks the
+    // cells next to ground)
+    e.gD = round(rb.diffuse);
+    if (e.gD.zc > 64) e.gD = zgridChunks(e, (e.Z + 63) / 64);
     // the flat walks: about 2048 blocks of whole rows
     const long rows = (long)e.Y * e.Z, nb = std::max(1L, std::min(rows, cap > 0 ? std::min(2048L, cap) : 2048L));
     const long rpb = (rows + nb - 1) / nb;
@@ -119,129 +127,189 @@ __device__ __forceinline__ void bstf(Rsrc r, unsigned voff, unsigned soff, float
 
 // LDS tile of one plane for the 64 x 4 block: rows y0-1 .. y0+4 and columns
 // x0-1 .. x0+64 (the block's cells plus a one-cell halo). Each cell's value
-// is read from HBM once per plane: by its own thread, plus halo rows by
-// waves 0 and 1 and halo columns by lanes 0 and 63. Two buffers and one
-// barrier per plane: a wave can only overwrite a buffer after every wave has
-// passed the next plane's barrier, i.e. finished reading it.
+// is read from HBM once per plane: by its own thread, plus the halo rows by
+// waves 0 and 1 and the halo columns by lanes 0-7 of wave 2. Two buffers and
+// one barrier per plane: a wave can only overwrite a buffer after every wave
+// has passed the next plane's barrier, i.e. finished reading it.
 constexpr int TH = 6, TW = 66;
-struct TileRole {
-    bool own, hasD, hasU, hasL, hasR;
-    unsigned rc, rD, rU, rL, rR;
-};
-__device__ __forceinline__ TileRole tileRole(const Geom& G, int x, int y) {
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    TileRole t;
-    t.own = x < G.X && y < G.Y;
-    t.hasD = ty == 0 && y >= 1 && x < G.X && y - 1 < G.Y;
-    t.hasU = ty == 1 && y + 3 < G.Y && x < G.X;
-    t.hasL = tx == 0 && x >= 1 && y < G.Y;
-    t.hasR = tx == 63 && x + 1 < G.X && y < G.Y;
-    t.rc = t.own ? (unsigned)(x + G.X * y) : 0u;
-    t.rD = t.hasD ? (unsigned)(x + G.X * (y - 1)) : 0u;
-    t.rU = t.hasU ? (unsigned)(x + G.X * (y + 3)) : 0u;
-    t.rL = t.hasL ? (unsigned)(x - 1 + G.X * y) : 0u;
-    t.rR = t.hasR ? (unsigned)(x + 1 + G.X * y) : 0u;
-    return t;
-}
 
 // ------------------------------------------------------------- diffusion ---
 // One Jacobi sweep = the red+black pair of diffuseRedBlack
 // (Game/kernelSky.cu:43-115; both colours read 'input', quirk A3):
 // out = (in + k (l + r + u + d + f + b)) / (1 + 6k) on sky cells.
 // One launch sweeps up to SKT_BATCH independent fields (the grid is the
-// z-march grid repeated per field). Per plane a thread loads its 5 in-plane
-// and front neighbours, ghm1 (largest ground height of the 3x3 columns
-// around) and the ground height of the next plane, for ZB planes at a time.
-// Cells with y >= ghm1 + 2 have no ground neighbour, so only the domain faces
-// need a boundary value; other cells load the neighbours' ground heights.
-constexpr int ZB_DIFF = 4;
-template <int ZB>
+// z-march grid repeated per field). Each input value is read from HBM once:
+// a plane's values go through the LDS tile (in-plane neighbours; positions
+// outside the domain hold the clamped cell's value, which is what the
+// boundary rules start from), along z they stay in a ring of four register
+// slots (read three planes ahead, four planes per loop trip, no moves).
+// Cells with y >= ghm1 + 2 (ghm1: largest ground height of the 3x3 columns
+// around) have no ground neighbour, so only the domain faces need a boundary
+// value; the other sky cells are marked and done after the z march, with
+// every neighbour tested for ground.
+// One field per block: blocks of two or three fields (sharing the ground
+// loads and barriers) need over 100 VGPRs and ran slower at half the
+// occupancy; direct neighbour loads without the tile ran slower too.
+template <int NF>
 KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const float* __restrict__ gT,
                   const float* __restrict__ gP, const float* __restrict__ p) {
     const unsigned lin = xcdRemap(blockIdx.x, gridDim.x);
-    const DiffField& fd = db.f[lin / nbf];
+    const int f0 = (int)(lin / nbf) * NF;
     ZM_SETUP_B(lin % nbf);
     (void)bid;
-    if (!inb || z0 >= z1) return;
-    const float* __restrict__ in = fd.in;
-    float* __restrict__ out = fd.out;
-    const float* __restrict__ prof = fd.prof;
-    const Bounds b = fd.b;
-    const int type = fd.type;
+    if (z0 >= z1) return;  // whole block
+    __shared__ float tV[NF][2][TH][TW];
+    const int tx = threadIdx.x, ty = threadIdx.y;
     const int X = G.X, Y = G.Y, Z = G.Z;
     const long XY = (long)X * Y;
-    const unsigned xl = max(x - 1, 0), xr = min(x + 1, X - 1), yd = max(y - 1, 0), yu = min(y + 1, Y - 1);
-    const unsigned rc = x + X * y, rl = xl + X * y, rr = xr + X * y, rd = x + X * yd, ru = x + X * yu;
-    const float pv = prof ? prof[y] : 0.0f, pvd = prof ? prof[yd] : 0.0f, pvu = prof ? prof[yu] : 0.0f;
+    const int xc = min(x, X - 1), yc = min(y, Y - 1);
+    const unsigned ob = 4u * (unsigned)(xc + X * yc), gb = 4u * (unsigned)xc;
+    // halo cells as in the pressure operator: rows by waves 0 and 1, columns
+    // by lanes 0-7 of wave 2; each reads its position clamped into the domain
+    const int x0 = x - tx, y0 = y - ty;
+    int hx = x, hy = y0 - 1, hrow = 0, hcol = tx + 1;
+    if (ty == 1) hy = y0 + 4, hrow = 5;
+    if (ty == 2) hx = tx < 4 ? x0 - 1 : x0 + 64, hy = y0 + (tx & 3), hrow = 1 + (tx & 3), hcol = tx < 4 ? 0 : 65;
+    const bool hasH = ty < 2 || (ty == 2 && tx < 8);
+    const unsigned hb = 4u * (unsigned)(min(max(hx, 0), X - 1) + X * min(max(hy, 0), Y - 1));
+    // the same loads in every plane, as in the pressure operator
+    const unsigned hbL = hasH ? hb : ob;
+    const int zb = max(z0 - 1, 0);
+    const unsigned PB = 4u * (unsigned)XY, RB = 4u * (unsigned)X;
+    const int yd = max(yc - 1, 0), yu = min(yc + 1, Y - 1);
+    Rsrc rV[NF], wV[NF];
+    float pv[NF], pvd[NF], pvu[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const DiffField& fd = db.f[f0 + f];
+        rV[f] = arrayRsrc(fd.in + XY * zb);
+        wV[f] = arrayRsrc(fd.out + XY * zb);
+        pv[f] = fd.prof ? fd.prof[yc] : 0.0f;
+        pvd[f] = fd.prof ? fd.prof[yd] : 0.0f;
+        pvu[f] = fd.prof ? fd.prof[yu] : 0.0f;
+    }
+    const Rsrc rG = arrayRsrc(G.gh + (long)X * zb), rM = arrayRsrc(G.ghm1 + (long)X * zb);
+    auto pOff = [&](int pl) { return PB * (unsigned)(pl - zb); };
+    auto gOff = [&](int pl) { return RB * (unsigned)(pl - zb); };
     const float den = 1 + 6 * k;
-    const int zbk = max(z0 - 1, 0);
-    float vb = in[XY * zbk + rc], vc = in[XY * z0 + rc];
-    int ghb = G.gh[x + X * zbk], ghc = G.gh[x + X * z0];
-    auto plane = [&](int z, float L, float R, float D, float U, float F, int gm, int gf) {
+    struct Own {
+        float v[NF];
+        int gm, gf;  // ghm1 of the plane, ground height of the next plane
+    };
+    struct Halo {
+        float v[NF];
+    };
+    auto loadOwn = [&](int pl, Own& o) __attribute__((always_inline)) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) o.v[f] = bldf(rV[f], ob, pOff(pl));
+        o.gm = bldi(rM, gb, gOff(pl));
+        o.gf = bldi(rG, gb, gOff(min(pl + 1, Z - 1)));
+    };
+    auto loadHalo = [&](int pl, Halo& h) __attribute__((always_inline)) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) h.v[f] = bldf(rV[f], hbL, pOff(pl));
+    };
+    // plane below the chunk (clamped: plane 0 for the first chunk)
+    float vb[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) vb[f] = bldf(rV[f], ob, pOff(zb));
+    int ghb = bldi(rG, gb, gOff(zb)), ghc = bldi(rG, gb, gOff(z0));
+    Own o0{}, o1{}, o2{}, o3{};
+    Halo h0{}, h1{};
+    loadOwn(z0, o0);
+    loadOwn(min(z0 + 1, Z - 1), o1);
+    loadOwn(min(z0 + 2, Z - 1), o2);
+    loadHalo(z0, h0);
+    unsigned long long slow = 0;  // planes (bit z - z0) whose cell is next to ground
+    auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, Halo& hc, Halo& hn) __attribute__((always_inline)) {
+        // loads complete in issue order: the halo (used next plane) goes first,
+        // so waiting for it does not wait for the plane read three ahead
+        loadHalo(min(z + 1, Z - 1), hn);
+        loadOwn(min(z + 3, Z - 1), o3n);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            tV[f][buf][ty + 1][tx + 1] = oc.v[f];
+            if (hasH) tV[f][buf][hrow][hcol] = hc.v[f];
+        }
+        __syncthreads();
+        const int gf = oc.gf;
+        if (inb && y > ghc) {
+            if (y >= oc.gm + 2) {
+                const bool oL = x == 0, oR = x == X - 1, oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
+#pragma unroll
+                for (int f = 0; f < NF; f++) {
+                    const Bounds b = db.f[f0 + f].b;
+                    const float vc = oc.v[f], F = z + 1 < Z ? of.v[f] : vc;
+                    const float L = tV[f][buf][ty + 1][tx], R = tV[f][buf][ty + 1][tx + 2];
+                    const float D = tV[f][buf][ty][tx + 1], U = tV[f][buf][ty + 2][tx + 1];
+                    const float l = oL ? bcv(b.sides, L, pv[f]) : L, r = oR ? bcv(b.sides, R, pv[f]) : R;
+                    const float u = oU ? bcv(b.sides, U, pvu[f]) : U;
+                    const float fr = oF ? bcv(b.sides, F, pv[f]) : F, bb = oB ? bcv(b.sides, vb[f], pv[f]) : vb[f];
+                    bstf(wV[f], ob, pOff(z), (vc + k * (l + r + u + D + fr + bb)) / den);
+                }
+            } else {
+                slow |= 1ull << (z - z0);
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) vb[f] = oc.v[f];
+        ghb = ghc;
+        ghc = gf;
+    };
+    for (int zg = z0; zg < z1; zg += 4) {
+        step(zg, 0, o0, o1, o3, h0, h1);
+        if (zg + 1 < z1) step(zg + 1, 1, o1, o2, o0, h1, h0);
+        if (zg + 2 < z1) step(zg + 2, 0, o2, o3, o1, h0, h1);
+        if (zg + 3 < z1) step(zg + 3, 1, o3, o0, o2, h1, h0);
+    }
+    // Sky cells next to ground (marked in 'slow', one bit per plane of the
+    // chunk): the sweep reads only 'in', so they can follow the fast cells.
+    // Each neighbour is tested and takes the ground or sides condition (nbv);
+    // for the potential temperature a ground neighbour reads the potential
+    // temperature of the ground temperature at this cell's pressure (:95-100).
+    // Neighbour positions are clamped into the domain, as in the tile.
+    const int xl = max(xc - 1, 0), xr = min(xc + 1, X - 1);
+    while (slow) {
+        const int z = z0 + __builtin_ctzll(slow);
+        slow &= slow - 1;
+        const long pz = XY * z;
+        const int zf = min(z + 1, Z - 1), zbk = max(z - 1, 0);
+        const int* gz = G.gh + X * z;
+        const int gc = gz[xc], gf = G.gh[xc + X * zf], gbk = G.gh[xc + X * zbk];
         const bool oL = x == 0, oR = x == X - 1, oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
-        if (y >= gm + 2) {
-            const float l = oL ? bcv(b.sides, L, pv) : L, r = oR ? bcv(b.sides, R, pv) : R;
-            const float u = oU ? bcv(b.sides, U, pvu) : U;
-            const float f = oF ? bcv(b.sides, F, pv) : F, bb = oB ? bcv(b.sides, vb, pv) : vb;
-            (out + XY * z)[rc] = (vc + k * (l + r + u + D + f + bb)) / den;
-        } else if (y > ghc) {
-            const int* gz = G.gh + X * z;
-            // a position outside the domain clamps onto this sky cell, so the
-            // ground test at the clamped column is the in-domain test
-            const bool gL = y <= gz[xl], gR = y <= gz[xr], gD = y - 1 <= ghc;
-            const bool gF = !oF && y <= gf, gB = !oB && y <= ghb;
-            float l = nbv(L, oL, gL, pv, b), r = nbv(R, oR, gR, pv, b);
-            float d = nbv(D, false, gD, pvd, b), u = nbv(U, oU, false, pvu, b);
-            float f = nbv(F, oF, gF, pv, b), bb = nbv(vb, oB, gB, pv, b);
-            if (type == 0 && (gL | gR | gD | gF | gB)) {
-                // potential temperature: a ground neighbour reads the potential
-                // temperature of the ground temperature at this cell's pressure (:95-100)
-                const float pc = p[XY * z + rc];
+        // a position outside the domain clamps onto this sky cell, so the
+        // ground test at the clamped column is the in-domain test
+        const bool gL = y <= gz[xl], gR = y <= gz[xr], gD = y - 1 <= gc;
+        const bool gF = !oF && y <= gf, gB = !oB && y <= gbk;
+        for (int f = 0; f < NF; f++) {
+            const DiffField& fd = db.f[f0 + f];
+            const float* __restrict__ in = fd.in;
+            const Bounds b = fd.b;
+            const float vc = in[pz + xc + X * y];
+            const float L = in[pz + xl + X * y], R = in[pz + xr + X * y];
+            const float D = in[pz + xc + X * yd], U = in[pz + xc + X * yu];
+            const float F = in[XY * zf + xc + X * y], Bk = in[XY * zbk + xc + X * y];
+            float l = nbv(L, oL, gL, pv[f], b), r = nbv(R, oR, gR, pv[f], b);
+            float d = nbv(D, false, gD, pvd[f], b), u = nbv(U, oU, false, pvu[f], b);
+            float fr = nbv(F, oF, gF, pv[f], b), bb = nbv(Bk, oB, gB, pv[f], b);
+            if (fd.type == 0 && (gL | gR | gD | gF | gB)) {
+                const float pc = p[pz + xc + X * y];
                 auto tg = [&](int col) { return d_potTemp(gT[col] - 273.15f, pc, gP[col]) + 273.15f; };
                 if (gD) d = tg(x + X * z);
                 if (gL) l = tg(xl + X * z);
                 if (gR) r = tg(xr + X * z);
-                if (gF) f = tg(x + X * (z + 1));
+                if (gF) fr = tg(x + X * (z + 1));
                 if (gB) bb = tg(x + X * (z - 1));
             }
-            (out + XY * z)[rc] = (vc + k * (l + r + u + d + f + bb)) / den;
+            fd.out[pz + xc + X * y] = (vc + k * (l + r + u + d + fr + bb)) / den;
         }
-        vb = vc;
-        vc = F;
-        ghb = ghc;
-        ghc = gf;
-    };
-    auto load = [&](int z, float& L, float& R, float& D, float& U, float& F, int& gm, int& gf) {
-        const int zf = min(z + 1, Z - 1);
-        const float* pl = in + XY * z;
-        L = pl[rl];
-        R = pl[rr];
-        D = pl[rd];
-        U = pl[ru];
-        F = (in + XY * zf)[rc];
-        gm = (G.ghm1 + X * z)[(unsigned)x];
-        gf = (G.gh + X * zf)[(unsigned)x];
-    };
-    int zg = z0;
-    for (; zg + ZB <= z1; zg += ZB) {
-        float L[ZB], R[ZB], D[ZB], U[ZB], F[ZB];
-        int GM[ZB], GF[ZB];
-#pragma unroll
-        for (int j = 0; j < ZB; j++) load(zg + j, L[j], R[j], D[j], U[j], F[j], GM[j], GF[j]);
-#pragma unroll
-        for (int j = 0; j < ZB; j++) plane(zg + j, L[j], R[j], D[j], U[j], F[j], GM[j], GF[j]);
-    }
-    for (; zg < z1; zg++) {
-        float L, R, D, U, F;
-        int gm, gf;
-        load(zg, L, R, D, U, F, gm, gf);
-        plane(zg, L, R, D, U, F, gm, gf);
     }
 }
 hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k) {
-    const ZGrid zg = zgrid(e);
-    hipLaunchKernelGGL(k_diffuse<ZB_DIFF>, dim3(zg.nblocks * db.n), B3, 0, e.stream, geom(e), zg.zc, zg.nblocks, db,
-                       k, e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
+    const ZGrid zg = e.gD;
+    if ((double)(zg.zc + 3) * (double)e.X * (double)e.Y * 4.0 >= 4294967296.0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_diffuse<1>, dim3(zg.nblocks * db.n), B3, 0, e.stream, geom(e), zg.zc, zg.nblocks, db, k,
+                       e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
     return hipGetLastError();
 }
 
@@ -853,6 +921,7 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
     const bool hIn = hx >= 0 && hx < X && hy >= 0 && hy < Y;
     const int hxc = min(max(hx, 0), X - 1), hyc = min(max(hy, 0), Y - 1);
     const unsigned hb = 4u * (unsigned)(hxc + X * hyc), hgb = 4u * (unsigned)hxc;
+    const unsigned hbL = hasH ? hb : ob, hgbL = hasH ? hgb : gb;
     const int zb = max(z0 - 1, 0);
     const unsigned PB = 4u * (unsigned)XY, RB = 4u * (unsigned)X;
     const Rsrc rZ = arrayRsrc(zz + XY * zb), rS = arrayRsrc(s_old + XY * zb), rI = arrayRsrc(irho + XY * zb);
@@ -875,14 +944,16 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
         o.i = bldf(rI, ob, so);
         o.gf = bldi(rG, gb, gOff(min(pl + 1, Z - 1)));
     };
+    // Every lane issues the same loads in every plane (a lane without a halo
+    // cell reads its own, a prefetch past the chunk reads a clamped plane):
+    // loads complete in issue order, and with no branches around them each
+    // wait covers exactly the loads its values come from.
     auto loadHalo = [&](int pl, Halo& h) __attribute__((always_inline)) {
-        if (hasH) {
-            const unsigned so = pOff(pl);
-            h.z = bldf(rZ, hb, so);
-            if (!FIRST) h.s = bldf(rS, hb, so);
-            h.i = bldf(rI, hb, so);
-            h.g = bldi(rG, hgb, gOff(pl));
-        }
+        const unsigned so = pOff(pl);
+        h.z = bldf(rZ, hbL, so);
+        if (!FIRST) h.s = bldf(rS, hbL, so);
+        h.i = bldf(rI, hbL, so);
+        h.g = bldi(rG, hgbL, gOff(pl));
     };
     // the chunk's first planes and the plane below it are read before the
     // prologue; s_new and the back face of the plane below need beta
@@ -898,8 +969,8 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
     Own o0{0.0f, 0.0f, 0.0f, 0}, o1{0.0f, 0.0f, 0.0f, 0}, o2{0.0f, 0.0f, 0.0f, 0}, o3{0.0f, 0.0f, 0.0f, 0};
     Halo h0{0.0f, 0.0f, 0.0f, 0}, h1{0.0f, 0.0f, 0.0f, 0};
     loadOwn(z0, o0);
-    if (z0 + 1 < Z) loadOwn(z0 + 1, o1);
-    if (z0 + 2 <= z1 && z0 + 2 < Z) loadOwn(z0 + 2, o2);
+    loadOwn(min(z0 + 1, Z - 1), o1);
+    loadOwn(min(z0 + 2, Z - 1), o2);
     loadHalo(z0, h0);
     pcgPrologue<KIND>(pm, ps, n, part, sin, sout, 0, 0.0f, it, S, sh);
     if (S.done) return;
@@ -917,8 +988,10 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
     int slot = 0;  // (z - z0) % 3
     // plane z from oc, its front plane from of; plane z + 3 is read into o3n
     auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, Halo& hc, Halo& hn) __attribute__((always_inline)) {
-        if (z + 3 <= z1 && z + 3 < Z) loadOwn(z + 3, o3n);
-        if (z + 1 < z1) loadHalo(z + 1, hn);
+        // the halo (used next plane) goes first, so waiting for it does not
+        // wait for the plane read three ahead
+        loadHalo(min(z + 1, Z - 1), hn);
+        loadOwn(min(z + 3, Z - 1), o3n);
         const bool inZ = z + 1 < Z;
         const bool skF = inb && inZ && y > oc.gf;
         const float snc = skC ? snv(oc.z, oc.s) : 0.0f;
@@ -1010,13 +1083,18 @@ hipError_t launch_pcg_A(EnvState& e, int it) {
     return hipGetLastError();
 }
 
-// Blocks of the pressure operator kernel the device holds at once: its grid
-// is one such round (a second, partly filled round would double its time).
-long pcgA_resident_blocks(int device) {
+// Blocks of a kernel the device holds at once. The pressure operator's grid
+// is one such round and a diffusion sweep's grid per field is one (a second,
+// partly filled round would nearly double a launch's time).
+template <class K>
+static long resident_blocks(K kernel, int device) {
     int nb = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pcg_A<false, true>, 256, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
     return (long)nb * cus;
+}
+ResidentBlocks resident_blocks(int device) {
+    return ResidentBlocks{resident_blocks(k_pcg_A<false, true>, device), resident_blocks(k_diffuse<1>, device)};
 }
 
 // Prologue PRO_B, then updatePandDiv (Game/kernelSky.cu:955-985) + the
