@@ -92,6 +92,17 @@ typedef struct skt_env_params {
     int32_t ground_changed;
 } skt_env_params;
 
+/* Serving diagnostics of the current state (skt_env_diagnostics). */
+typedef struct skt_diagnostics {
+    float max_updraft;       /* largest VELY (the reference's vertical velocity) over sky cells, m/s;
+                                -inf if there is no sky cell */
+    int32_t cloud_top_level; /* highest level y with QW + QC > threshold at a sky cell; -1 if none */
+    float cloud_top_height;  /* cloud_top_level * dx in m (the level heights of Game/readTable.cpp:390);
+                                0 if none */
+    int32_t pad;
+    double surface_rain;     /* ground rain water QGR summed over the nx*nz columns */
+} skt_diagnostics;
+
 typedef struct skt_step_stats {
     int32_t pcg_iterations;      /* iterations of the last pressure solve */
     float pcg_residual;          /* stop-test value after the last iteration */
@@ -165,6 +176,14 @@ int skt_env_compute_masks(skt_env* env, uint8_t* host_out);
  * cell: bit0 sky, bit1 T<0, bit2 PTerm1, bit3 PTerm2, bit4 PTerm3, bit5 WTerm
  * (Game/microPhysics.cu:1128-1130,1177-1178). host_out[N]. */
 int skt_env_phase_mask(skt_env* env, uint8_t* host_out);
+
+/* Serving diagnostics reduced on the device (maximum updraft, cloud-top level
+ * and height, total ground rain) with cloud meaning QW + QC > cloud_threshold
+ * (kg/kg; 1e-5 is the usual choice). No reference counterpart: it replaces the
+ * editor's copy of the whole state to the host after every step
+ * (editor::GPUSetEnv, Game/editor.cpp:226-269) with two small kernels and a
+ * 256-byte readback. Synchronises the env's stream. */
+int skt_env_diagnostics(skt_env* env, float cloud_threshold, skt_diagnostics* out);
 
 /* dataClass microphysics region diagnostic (Game/dataClass.cu:26-81):
  * sums of the 33 limited process rates over a box of cells. */

@@ -148,6 +148,12 @@ public:
         check(skt_env_get_params(m_env, &p), "params");
         return p;
     }
+    // Serving diagnostics (skt_env_diagnostics; no reference counterpart).
+    skt_diagnostics diagnostics(float cloudThreshold = 1e-5f) {
+        skt_diagnostics d;
+        check(skt_env_diagnostics(m_env, cloudThreshold, &d), "diagnostics");
+        return d;
+    }
     skt_step_stats stats() {
         skt_step_stats s;
         check(skt_env_get_stats(m_env, &s), "stats");

@@ -35,6 +35,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <limits>
 #include <vector>
 #include <string>
 #ifdef _OPENMP
@@ -1823,6 +1824,33 @@ template <class R> Env<R>* makeEnv(int X, int Y, int Z, float dx, const Opts& o)
 
 }  // namespace sko
 
+namespace sko {
+// Serving diagnostics of the state (no reference counterpart: they stand in
+// for the editor's full-state readback after every step, Game/editor.cpp:226-269):
+// the largest vertical velocity (VELY, the reference's vertical) over sky
+// cells, the highest level y whose QW + QC exceeds a threshold at a sky cell
+// (-1 if none; its height is y dx, the level heights of Game/readTable.cpp:390),
+// and the ground rain water QGR summed over all columns in FP64.
+template <class R>
+void diagnostics(const Env<R>& e, float thr, float* maxw, int* level, double* rain) {
+    float w = -std::numeric_limits<float>::infinity();
+    int lev = -1;
+    for (int z = 0; z < e.Z; z++)
+        for (int y = 0; y < e.Y; y++)
+            for (int x = 0; x < e.X; x++) {
+                if (e.ground(x, y, z)) continue;
+                const long i = e.id(x, y, z);
+                w = std::fmax(w, float(e.f[F_V][i]));
+                if (float(e.f[F_QW][i]) + float(e.f[F_QC][i]) > thr) lev = std::max(lev, y);
+            }
+    double r = 0.0;
+    for (long g = 0; g < e.G; g++) r += double(e.f[F_QGR][g]);
+    *maxw = w;
+    *level = lev;
+    *rain = r;
+}
+}  // namespace sko
+
 // ---------------------------------------------------------------- C API ---
 // Handle-based API used by tests (ctypes) and bench.py's CPU baseline.
 using namespace sko;
@@ -2054,6 +2082,10 @@ void sko_region(void* hv, int active, const int* mn, const int* mx) {
         for (int k = 0; k < 3; k++) { e.rmin[k] = mn[k]; e.rmax[k] = mx[k]; }
         for (int k = 0; k < 33; k++) e.rsum[k] = 0;
     });
+}
+void sko_diagnostics(void* hv, float thr, float* maxw, int* level, double* rain) {
+    SkoHandle* h = (SkoHandle*)hv;
+    DISPATCH(h, diagnostics(e, thr, maxw, level, rain));
 }
 void sko_region_get(void* hv, double* out) {
     SkoHandle* h = (SkoHandle*)hv;

@@ -54,6 +54,13 @@ class Stats(ctypes.Structure):
     ]
 
 
+class Diagnostics(ctypes.Structure):
+    _fields_ = [
+        ("max_updraft", ctypes.c_float), ("cloud_top_level", ctypes.c_int32), ("cloud_top_height", ctypes.c_float),
+        ("pad", ctypes.c_int32), ("surface_rain", ctypes.c_double),
+    ]
+
+
 class Timing(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_int64 * 8), ("total_ms", ctypes.c_double * 8)]
 
@@ -110,6 +117,7 @@ SIGNATURES = {
     "skt_env_set_params": (_I, [_P, ctypes.POINTER(Params)]),
     "skt_env_get_params": (_I, [_P, ctypes.POINTER(Params)]),
     "skt_env_get_stats": (_I, [_P, ctypes.POINTER(Stats)]),
+    "skt_env_diagnostics": (_I, [_P, _F, ctypes.POINTER(Diagnostics)]),
     "skt_env_compute_masks": (_I, [_P, _P]),
     "skt_env_phase_mask": (_I, [_P, _P]),
     "skt_mphys_region_set": (_I, [_P, _P, _P]),

@@ -720,6 +720,20 @@ int skt_env_get_stats(skt_env* env, skt_step_stats* st) {
     *st = env->s.stats;
     return SKT_OK;
 }
+int skt_env_diagnostics(skt_env* env, float thr, skt_diagnostics* out) {
+    if (!env || !out) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    HIPCHK(e, hipSetDevice(e.device));
+    HIPCHK(e, launch_diagnostics(e, thr));
+    const int rc = readScalars(e);
+    if (rc) return rc;
+    out->max_updraft = e.sc_host->dg_w;
+    out->cloud_top_level = e.sc_host->dg_level;
+    out->cloud_top_height = e.sc_host->dg_level >= 0 ? (float)e.sc_host->dg_level * e.dx : 0.0f;
+    out->pad = 0;
+    out->surface_rain = e.sc_host->dg_rain;
+    return SKT_OK;
+}
 int skt_env_compute_masks(skt_env* env, uint8_t* host) {
     if (!env || !host) return SKT_ERR_ARG;
     EnvState& e = env->s;

@@ -28,11 +28,15 @@ struct Scalars {
     float maxvel;  // advection max face velocity
     int nsub;      // advection substeps
     float dtsub;   // advection substep length
-    int pad[29];
+    float dg_w;    // serving diagnostics: max vertical velocity over sky cells
+    int dg_level;  //   highest cloudy level (-1: none)
+    int dg_pad;
+    double dg_rain;  //   ground rain water summed over the columns
+    int pad[24];
 };
 
 // Regions of the per-block reduction partials (each maxBlocks doubles).
-enum { PART_DIV = 0, PART_START, PART_A, PART_BMAX, PART_BSUM, PART_COUNT };
+enum { PART_DIV = 0, PART_START, PART_A, PART_BMAX, PART_BSUM, PART_DG_W, PART_DG_L, PART_DG_R, PART_COUNT };
 
 // N-sized work buffers: three per field of a batched advection.
 constexpr int SKT_NWORK = 16;
@@ -133,6 +137,7 @@ struct DiffBatch {
 };
 hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k);
 hipError_t launch_maxvel(EnvState& e, float dt);
+hipError_t launch_diagnostics(EnvState& e, float thr);
 // Independent fields moved by one PPM pass launch.
 struct PpmField {
     const float* in;

@@ -216,16 +216,19 @@ KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const flo
     int ghb = bldi(rG, gb, gOff(zb)), ghc = bldi(rG, gb, gOff(z0));
     Own o0{}, o1{}, o2{}, o3{};
     Halo h0{}, h1{};
+    // prefetches past the chunk read the next chunk's first planes: measured
+    // faster than clamping them to the chunk's last plane
+    const int zlast = Z - 1;
     loadOwn(z0, o0);
-    loadOwn(min(z0 + 1, Z - 1), o1);
-    loadOwn(min(z0 + 2, Z - 1), o2);
+    loadOwn(min(z0 + 1, zlast), o1);
+    loadOwn(min(z0 + 2, zlast), o2);
     loadHalo(z0, h0);
     unsigned long long slow = 0;  // planes (bit z - z0) whose cell is next to ground
     auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, Halo& hc, Halo& hn) __attribute__((always_inline)) {
         // loads complete in issue order: the halo (used next plane) goes first,
         // so waiting for it does not wait for the plane read three ahead
-        loadHalo(min(z + 1, Z - 1), hn);
-        loadOwn(min(z + 3, Z - 1), o3n);
+        loadHalo(min(z + 1, zlast), hn);
+        loadOwn(min(z + 3, zlast), o3n);
 #pragma unroll
         for (int f = 0; f < NF; f++) {
             tV[f][buf][ty + 1][tx + 1] = oc.v[f];
@@ -968,9 +971,12 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
     }
     Own o0{0.0f, 0.0f, 0.0f, 0}, o1{0.0f, 0.0f, 0.0f, 0}, o2{0.0f, 0.0f, 0.0f, 0}, o3{0.0f, 0.0f, 0.0f, 0};
     Halo h0{0.0f, 0.0f, 0.0f, 0}, h1{0.0f, 0.0f, 0.0f, 0};
+    // prefetches past the chunk read the next chunk's first planes: measured
+    // faster than clamping them to the chunk's last plane
+    const int zlast = Z - 1;
     loadOwn(z0, o0);
-    loadOwn(min(z0 + 1, Z - 1), o1);
-    loadOwn(min(z0 + 2, Z - 1), o2);
+    loadOwn(min(z0 + 1, zlast), o1);
+    loadOwn(min(z0 + 2, zlast), o2);
     loadHalo(z0, h0);
     pcgPrologue<KIND>(pm, ps, n, part, sin, sout, 0, 0.0f, it, S, sh);
     if (S.done) return;
@@ -990,8 +996,8 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
     auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, Halo& hc, Halo& hn) __attribute__((always_inline)) {
         // the halo (used next plane) goes first, so waiting for it does not
         // wait for the plane read three ahead
-        loadHalo(min(z + 1, Z - 1), hn);
-        loadOwn(min(z + 3, Z - 1), o3n);
+        loadHalo(min(z + 1, zlast), hn);
+        loadOwn(min(z + 3, zlast), o3n);
         const bool inZ = z + 1 < Z;
         const bool skF = inb && inZ && y > oc.gf;
         const float snc = skC ? snv(oc.z, oc.s) : 0.0f;

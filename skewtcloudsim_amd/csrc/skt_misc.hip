@@ -680,4 +680,69 @@ hipError_t launch_random(EnvState& e, float* a, float mn, float mx, uint32_t see
     return hipGetLastError();
 }
 
+// ------------------------------------------------------ serving diagnostics ---
+// No reference counterpart: these replace the editor's full-state readback
+// after every step (Game/editor.cpp:226-269) with device reductions. Per
+// block: the largest VELY (the reference's vertical velocity) over its sky
+// cells, the highest level whose QW + QC exceeds thr at a sky cell, and, in
+// the blocks whose tile holds level 0, the ground rain water QGR of its
+// columns (FP64). Each ground column lies in exactly one such block.
+KL void k_diag(Geom G, int zc, const float* __restrict__ v, const float* __restrict__ qw,
+               const float* __restrict__ qc, const float* __restrict__ qgr, float thr, double* pw, double* pl,
+               double* pr) {
+    ZM_SETUP;
+    __shared__ double sh[4];
+    float w = -__builtin_inff();
+    int lev = -1;
+    double rain = 0.0;
+    if (inb) {
+        for (int z = z0; z < z1; z++) {
+            const long i = gidx(G, x, y, z);
+            if (!isGround(G, x, y, z)) {
+                w = fmaxf(w, v[i]);
+                if (qw[i] + qc[i] > thr) lev = y;  // y is fixed per lane
+            }
+            if (y == 0) rain += (double)qgr[x + (long)G.X * z];
+        }
+    }
+    const double m = blockMax((double)w, sh);
+    const double l = blockMax((double)lev, sh);
+    const double r = blockSum(rain, sh);
+    if (tid256() == 0) {
+        pw[bid] = m;
+        pl[bid] = l;
+        pr[bid] = r;
+    }
+}
+// One block: the final reductions in a fixed order.
+KL void k_diag_finalize(const double* __restrict__ pw, const double* __restrict__ pl, const double* __restrict__ pr,
+                        unsigned n, Scalars* sc) {
+    __shared__ double sh[4];
+    double m = -__builtin_inf(), l = -1.0, r = 0.0;
+    for (unsigned j = tid256(); j < n; j += 256) {
+        m = fmax(m, pw[j]);
+        l = fmax(l, pl[j]);
+        r += pr[j];
+    }
+    m = blockMax(m, sh);
+    l = blockMax(l, sh);
+    r = blockSum(r, sh);
+    if (tid256() == 0) {
+        sc->dg_w = (float)m;
+        sc->dg_level = (int)l;
+        sc->dg_rain = r;
+    }
+}
+hipError_t launch_diagnostics(EnvState& e, float thr) {
+    const ZGrid zg = zgrid(e);
+    const size_t M = e.maxBlocks;
+    double *pw = e.partials + PART_DG_W * M, *pl = e.partials + PART_DG_L * M, *pr = e.partials + PART_DG_R * M;
+    hipLaunchKernelGGL(k_diag, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.f[SKT_VELY], e.f[SKT_QW], e.f[SKT_QC],
+                       e.f[SKT_QGR], thr, pw, pl, pr);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_diag_finalize, dim3(1), B3, 0, e.stream, pw, pl, pr, zg.nblocks, e.sc);
+    return hipGetLastError();
+}
+
 }  // namespace skt

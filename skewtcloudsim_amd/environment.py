@@ -191,6 +191,14 @@ class EnvironmentGPU:
         self._chk(self.lib.skt_env_get_stats(self.h, ctypes.byref(s)), "stats")
         return s
 
+    def diagnostics(self, cloud_threshold=1e-5):
+        """Maximum updraft, cloud-top level and height, total ground rain
+        (skt_env_diagnostics), as a dict."""
+        d = _lib.Diagnostics()
+        self._chk(self.lib.skt_env_diagnostics(self.h, cloud_threshold, ctypes.byref(d)), "diagnostics")
+        return {"max_updraft": d.max_updraft, "cloud_top_level": d.cloud_top_level,
+                "cloud_top_height": d.cloud_top_height, "surface_rain": d.surface_rain}
+
     def masks(self):
         m = np.empty(12 * self.N, np.uint8)
         self._chk(self.lib.skt_env_compute_masks(self.h, _ptr(m)), "masks")

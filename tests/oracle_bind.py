@@ -52,6 +52,7 @@ def lib():
             "sko_editor_data": [vp], "sko_masks": [vp, vp], "sko_phase": [vp, vp],
             "sko_stats": [vp, vp, vp, vp, vp, vp], "sko_region": [vp, i, vp, vp], "sko_region_get": [vp, vp],
             "sko_set_threads": [i], "sko_fallvel": [f, f, f, f, i, vp], "sko_gammas": [vp],
+            "sko_diagnostics": [vp, f, vp, vp, vp],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args
@@ -185,6 +186,11 @@ class Oracle:
 
     def irradiance(self):
         return self.L.sko_irradiance(self.h)
+
+    def diagnostics(self, thr=1e-5):
+        w, lev, rain = ctypes.c_float(), ctypes.c_int(), ctypes.c_double()
+        self.L.sko_diagnostics(self.h, ctypes.c_float(thr), ctypes.byref(w), ctypes.byref(lev), ctypes.byref(rain))
+        return {"max_updraft": w.value, "cloud_top_level": lev.value, "surface_rain": rain.value}
 
     def __getattr__(self, name):
         # per-phase entry points: oracle.diffuse(...), oracle.micro_sky(...), ...

@@ -425,3 +425,53 @@ def test_mid_size_steps(terrain):
         assert_fields(env, orc, SKY, m, l2=tol, maxabs=50 * tol, tag=f"mid{'-terrain' if terrain else ''}-step{s}")
     env.close()
     orc.close()
+
+
+def _diag_brute(env, gh, thr):
+    nx, ny, nz = env.nx, env.ny, env.nz
+    n = nx * ny * nz
+    y = (np.arange(n) // nx) % ny
+    x = np.arange(n) % nx
+    z = np.arange(n) // (nx * ny)
+    sky = y > gh[x + nx * z]
+    v = env.get("VELY")
+    cloudy = sky & ((env.get("QW") + env.get("QC")) > np.float32(thr))
+    w = np.float32(v[sky].max())
+    lev = int(y[cloudy].max()) if cloudy.any() else -1
+    return w, lev, float(env.get("QGR").astype(np.float64).sum())
+
+
+@pytest.mark.parametrize("cap", [0, 1])
+def test_diagnostics(cap):
+    """skt_env_diagnostics against a numpy brute force on the GPU's own state
+    (exact), and on the oracle's state copied in (against the oracle's
+    diagnostics, exact), with terrain; cap 1 makes each block walk all z."""
+    nx, ny, nz = 24, 20, 16
+    gh = _terrain(nx, nz, "random")
+    env, orc = make_terrain_pair(nx, ny, nz, gh, max_blocks=cap)
+    for _ in range(2):
+        env.updateGPU(0.5, 1.0)
+        orc.step(0.5, 1.0)
+    for thr in (1e-7, 1e-5):
+        d = env.diagnostics(thr)
+        w, lev, rain = _diag_brute(env, gh, thr)
+        assert d["max_updraft"] == w and d["cloud_top_level"] == lev
+        assert d["cloud_top_height"] == (np.float32(lev) * np.float32(env.dx) if lev >= 0 else 0.0)
+        assert d["surface_rain"] == pytest.approx(rain, rel=1e-12, abs=1e-300)
+    for f in ("VELY", "QW", "QC", "QGR"):
+        env.set(f, orc.get(f))
+    d, do = env.diagnostics(1e-5), orc.diagnostics(1e-5)
+    assert d["max_updraft"] == do["max_updraft"] and d["cloud_top_level"] == do["cloud_top_level"]
+    assert d["surface_rain"] == pytest.approx(do["surface_rain"], rel=1e-12, abs=1e-300)
+    # a cloudy ground cell and a huge ground updraft are ignored
+    g = int(np.argmax(gh))
+    x, z = g % nx, g // nx
+    i = x + nx * (int(gh[g]) + ny * z)
+    v, qc = env.get("VELY"), env.get("QC")
+    v[i], qc[i] = 1e6, 1.0
+    env.set("VELY", v)
+    env.set("QC", qc)
+    d2 = env.diagnostics(1e-5)
+    assert d2["max_updraft"] == d["max_updraft"] and d2["cloud_top_level"] == d["cloud_top_level"]
+    env.close()
+    orc.close()

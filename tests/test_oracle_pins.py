@@ -509,3 +509,68 @@ def test_init_noise_and_profiles():
     u0 = snd["velField"].reshape(-1, 3)[:, 0].reshape(nz, ny, nx)
     assert np.array_equal(u[:, 1:, :], u0[:, 1:, :] + np.float32(0.5))
     assert np.all(u[:, 0, :] == 0) and np.all(o.get("QV").reshape(nz, ny, nx)[:, 0, :] == 0)
+
+
+# ------------------------------------------------------- serving diagnostics ---
+def _diag_brute(o, gh, thr):
+    """numpy restatement: sky = y > ground height of the column."""
+    nx, ny, nz = o.nx, o.ny, o.nz
+    y = (np.arange(o.N) // nx) % ny
+    x = np.arange(o.N) % nx
+    z = np.arange(o.N) // (nx * ny)
+    sky = y > gh[x + nx * z]
+    v = o.get("VELY")
+    cloudy = sky & ((o.get("QW") + o.get("QC")) > np.float32(thr))
+    w = np.float32(v[sky].max()) if sky.any() else np.float32(-np.inf)
+    lev = int(y[cloudy].max()) if cloudy.any() else -1
+    return w, lev, float(o.get("QGR").astype(np.float64).sum())
+
+
+def test_diagnostics_closed_form():
+    """A constructed state with known answers: one updraft cell, one cloudy
+    cell at a known level, a uniform ground-rain field; values planted in
+    ground cells (which must be ignored) exceed them."""
+    o, _ = _small_state()
+    nx, ny, nz = o.nx, o.ny, o.nz
+    gh = np.zeros(nx * nz, np.int32)
+    gh[3 + nx * 2] = 4                        # a tall ground column at x=3, z=2
+    o.set_ground_height(gh)
+    idx = lambda x_, y_, z_: x_ + nx * (y_ + ny * z_)
+    v = np.full(o.N, -1.0, np.float32)
+    v[idx(5, 6, 4)] = 7.5
+    v[idx(3, 3, 2)] = 99.0                    # ground cell: ignored
+    qw = np.zeros(o.N, np.float32)
+    qc = np.zeros(o.N, np.float32)
+    qw[idx(1, 5, 1)] = 6e-6
+    qc[idx(1, 5, 1)] = 6e-6                   # 1.2e-5 > 1e-5 at level 5
+    qw[idx(2, 7, 6)] = 9e-6                   # 9e-6 alone: below the threshold
+    qc[idx(3, 4, 2)] = 1.0                    # ground cell (level 4 <= 4): ignored
+    o.set("VELY", v)
+    o.set("QW", qw)
+    o.set("QC", qc)
+    o.set("QGR", np.full(nx * nz, 0.25, np.float32))
+    d = o.diagnostics(1e-5)
+    assert d["max_updraft"] == np.float32(7.5)
+    assert d["cloud_top_level"] == 5
+    assert d["surface_rain"] == 0.25 * nx * nz
+    qw[:] = 0
+    qc[:] = 0
+    o.set("QW", qw)
+    o.set("QC", qc)
+    assert o.diagnostics(1e-5)["cloud_top_level"] == -1
+    o.close()
+
+
+def test_diagnostics_match_brute_force_after_steps():
+    o, _ = _small_state(nx=12, ny=10, nz=9)
+    gh = RNG.integers(0, 3, 12 * 9).astype(np.int32)
+    o.set_ground_height(gh)
+    for _ in range(3):
+        o.step(0.5, 1.0)
+    for thr in (1e-7, 1e-5):
+        d = o.diagnostics(thr)
+        w, lev, rain = _diag_brute(o, gh, thr)
+        assert d["max_updraft"] == w
+        assert d["cloud_top_level"] == lev
+        assert d["surface_rain"] == pytest.approx(rain, rel=1e-12, abs=1e-300)
+    o.close()
