@@ -123,6 +123,9 @@ int skt_env_synchronize(skt_env* env);
 /* Ground heights per column (host int32[nx*nz]); flat 0 by default. Replaces
  * the PNoise2D terrain of init (Game/environment.cu:321-340). */
 int skt_env_set_ground_height(skt_env* env, const int32_t* ground_height);
+/* The ground heights in use (the reference's GHeight, Game/kernelSky.cu; read
+ * by isGround, Game/environment.cu:1071-1074), gh[nx*nz]. */
+int skt_env_get_ground_height(skt_env* env, int32_t* gh);
 
 /* environmentGPU::init (Game/environment.cu:249-389). All pointers are host
  * arrays: potTemps[N], velField[3N] (x,y,z per cell, MAC faces), Qv[N],
@@ -176,6 +179,33 @@ int skt_env_compute_masks(skt_env* env, uint8_t* host_out);
  * cell: bit0 sky, bit1 T<0, bit2 PTerm1, bit3 PTerm2, bit4 PTerm3, bit5 WTerm
  * (Game/microPhysics.cu:1128-1130,1177-1178). host_out[N]. */
 int skt_env_phase_mask(skt_env* env, uint8_t* host_out);
+
+/* Editor edits (environmentGPU::prepareBrushGPU / prepareSelectionGPU,
+ * Game/environment.cu:1127-1260; kernels Game/kernelSky.cu:1476-1663).
+ * param is a getParamArray parameter (SKT_PARAM_*): WIND edits the three
+ * velocity components along dir, PGROUND raises (ground_erase 0) or erases
+ * the ground at the touched cells; other parameters skip ground cells.
+ * The brush adds (1 - d^2/size^2)^smoothness * dt * intensity * value over
+ * the sphere of radius brush_size around pos (clipped as the reference clips
+ * it); the selection sets value over the box [min_pos, max_pos]. A ground
+ * change resets the cells it uncovers to the base profiles, becomes the new
+ * ground height map and runs the ground-change path of editorDataGPU.
+ * Positions beyond the domain are skipped (the reference lets z = nz through
+ * its test, one plane past its arrays). Synchronises the env's stream. */
+int skt_env_apply_brush(skt_env* env, int32_t param, float brush_size, const int32_t pos[3], float smoothness,
+                        float dt, float intensity, float value, const float dir[3], int32_t ground_erase);
+int skt_env_apply_selection(skt_env* env, int32_t param, const int32_t min_pos[3], const int32_t max_pos[3],
+                            float value, const float dir[3], int32_t ground_erase);
+
+/* Terrain of environmentGPU::init (Game/environment.cu:321-340): PNoise2D
+ * (Engine/source/math/geometry.cpp:196-278; seed 100, 12 octaves there) over
+ * an nx*nz ground grid, and the ground heights of initGroundHeightGPU
+ * (Game/kernelGround.cu:13-19), round(noise * max_height) (0.2 there, which
+ * makes every height 0). Computed on the device; noise_out[nx*nz] and/or
+ * heights_out[nx*nz] receive the results (either may be NULL). Heights are
+ * for skt_env_set_ground_height, which accepts [0, ny). */
+int skt_terrain_generate(int32_t device, int32_t seed, int32_t nx, int32_t nz, int32_t octaves, float max_height,
+                         float* noise_out, int32_t* heights_out);
 
 /* Serving diagnostics reduced on the device (maximum updraft, cloud-top level
  * and height, total ground rain) with cloud meaning QW + QC > cloud_threshold

@@ -110,6 +110,29 @@ public:
     }
 
     void editorDataGPU() { check(skt_env_editor_data(m_env), "editorDataGPU"); }
+    // prepareBrushGPU / prepareSelectionGPU (Game/environment.cu:1127-1260);
+    // I3 / F3 are any 3-int / 3-float vector types (int3, float3, glm types).
+    template <class I3, class F3>
+    void prepareBrushGPU(parameter paramType, const float brushSize, const I3 mousePos, const float brushSmoothnes,
+                         const float dt, const float brushIntensity, const float applyValue, const F3 valueDir,
+                         const bool groundErase) {
+        static_assert(sizeof(I3) == 3 * sizeof(int32_t) && sizeof(F3) == 3 * sizeof(float), "3-vectors");
+        check(skt_env_apply_brush(m_env, paramType, brushSize, reinterpret_cast<const int32_t*>(&mousePos),
+                                  brushSmoothnes, dt, brushIntensity, applyValue,
+                                  reinterpret_cast<const float*>(&valueDir), groundErase),
+              "prepareBrushGPU");
+        refreshGround();
+    }
+    template <class I3, class F3>
+    void prepareSelectionGPU(parameter paramType, const I3 minPos, const I3 maxPos, const float applyValue,
+                             const F3 valueDir, const bool groundErase) {
+        static_assert(sizeof(I3) == 3 * sizeof(int32_t) && sizeof(F3) == 3 * sizeof(float), "3-vectors");
+        check(skt_env_apply_selection(m_env, paramType, reinterpret_cast<const int32_t*>(&minPos),
+                                      reinterpret_cast<const int32_t*>(&maxPos), applyValue,
+                                      reinterpret_cast<const float*>(&valueDir), groundErase),
+              "prepareSelectionGPU");
+        refreshGround();
+    }
     float irridianceGPU() {
         float v = 0.0f;
         check(skt_env_irradiance(m_env, &v), "irridianceGPU");
@@ -187,7 +210,16 @@ private:
     skt_env* m_env = nullptr;
     std::vector<int32_t> m_gh;
 
+    size_t m_columns = 0;  // nx * nz
+
+    // the host copy isGround reads, after an edit may have changed the ground
+    void refreshGround() {
+        m_gh.resize(m_columns);
+        check(skt_env_get_ground_height(m_env, m_gh.data()), "groundHeight");
+    }
+
     void create(const skt_config& cfg) {
+        m_columns = (size_t)cfg.nx * (size_t)cfg.nz;
         const int rc = skt_env_create(&cfg, &m_env);
         if (rc != SKT_OK) fail(rc, "environmentGPU: skt_env_create failed");
     }

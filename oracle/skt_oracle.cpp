@@ -1726,6 +1726,104 @@ template <class R> void editorData(Env<R>& e) {
         isenTempGround(e);
     }
 }
+// Editor edits: environmentGPU::prepareBrushGPU / prepareSelectionGPU
+// (Game/environment.cu:1127-1260) with applyBrushGPU, applySelectionGPU,
+// setGround and compareAndResetValuesOutGround (Game/kernelSky.cu:1476-1663).
+// Parameter ids are getParamArray's (0 potential temperature .. 6 Qi, 7 wind,
+// 8 ground, 9 pressure). Positions beyond the domain are skipped (the
+// reference's z test lets z = nz through).
+static const int editField[10] = {F_THETA, F_QV, F_QW, F_QC, F_QR, F_QS, F_QI, F_U, -1, F_P};
+template <class R>
+static void editCell(Env<R>& e, int param, std::vector<int>& gNew, bool& changed, int mX, int mY, int mZ, float v1,
+                     const float* dir, bool erase, bool add) {
+    float v2 = 0.0f, v3 = 0.0f;
+    if (param == 7) {
+        v3 = v1 * dir[2];
+        v2 = v1 * dir[1];
+        v1 *= dir[0];
+    }
+    const long i = e.id(mX, mY, mZ);
+    if (param == 8) {  // setGround against the current heights
+        const long g = mX + (long)e.X * mZ;
+        if (erase && mY <= e.GH[g]) {
+            gNew[g] = std::min(gNew[g], mY);
+            changed = true;
+        } else if (!erase && mY > e.GH[g]) {
+            gNew[g] = std::max(gNew[g], mY);
+            changed = true;
+        }
+    } else if (param == 7) {
+        const float v[3] = {v1, v2, v3};
+        const int fl[3] = {F_U, F_V, F_W};
+        for (int k = 0; k < 3; k++) e.f[fl[k]][i] = add ? R(float(e.f[fl[k]][i]) + v[k]) : R(v[k]);
+    } else {
+        const int fl = editField[param];
+        e.f[fl][i] = add ? R(float(e.f[fl][i]) + v1) : R(v1);
+    }
+}
+template <class R> static void finishEdit(Env<R>& e, const std::vector<int>& gNew, bool changed, bool erase) {
+    if (changed) {
+        if (erase) {  // compareAndResetValuesOutGround: cells turned from ground into sky
+            for (int z = 0; z < e.Z; z++)
+                for (int y = 0; y < e.Y; y++)
+                    for (int x = 0; x < e.X; x++) {
+                        const long g = x + (long)e.X * z;
+                        if (y > e.GH[g] || y <= gNew[g]) continue;
+                        const long i = e.id(x, y, z);
+                        e.f[F_QV][i] = e.f[F_ISEN_QV][y];
+                        e.f[F_QW][i] = e.f[F_QC][i] = e.f[F_QR][i] = e.f[F_QS][i] = e.f[F_QI][i] = R(0);
+                        e.f[F_THETA][i] = e.f[F_ISEN_T][y];
+                        e.f[F_U][i] = e.f[F_V][i] = e.f[F_W][i] = R(0);
+                        e.f[F_P][i] = e.f[F_DEF_P][y];
+                    }
+        }
+        e.GH = gNew;
+        e.groundChanged = true;
+    }
+    editorData(e);
+}
+template <class R>
+void applyBrush(Env<R>& e, int param, float size, const int* pos, float smooth, float dt, float intensity,
+                float value, const float* dir, bool erase) {
+    const int n = int(std::ceil(size)) - int(std::floor(-size));
+    std::vector<int> gNew(e.GH);
+    bool changed = false;
+    const int off = int(std::ceil(size));
+    for (int y = 0; y < n; y++)
+        for (int x = 0; x < n; x++)
+            for (int z = 0; z < e.Z; z++) {
+                const int lX = x - off, lY = y - off, lZ = z - off;
+                const int mX = x + pos[0] - off, mY = y + pos[1] - off, mZ = z + pos[2] - off;
+                if (mX >= e.X || mY >= e.Y || mZ >= e.Z || mX < 0 || mY < 0 || mZ < 0) continue;
+                if (param != 8 && mY <= e.GH[mX + (long)e.X * mZ]) continue;
+                const float eX = lX, eY = lY, eZ = lZ;
+                const float distance = eX * eX + eY * eY + eZ * eZ;
+                const float radius = size * size;
+                if (distance <= radius) {
+                    float v1 = -(distance / radius - 1);
+                    v1 = std::pow(v1, smooth);
+                    v1 *= dt * intensity * value;
+                    editCell(e, param, gNew, changed, mX, mY, mZ, v1, dir, erase, true);
+                }
+            }
+    finishEdit(e, gNew, changed, erase);
+}
+template <class R>
+void applySelection(Env<R>& e, int param, const int* mn, const int* mx, float value, const float* dir, bool erase) {
+    const int nx = std::max(mx[0] - mn[0] + 1, 1), ny = std::max(mx[1] - mn[1] + 1, 1);
+    std::vector<int> gNew(e.GH);
+    bool changed = false;
+    for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++)
+            for (int z = mn[2]; z <= mx[2]; z++) {
+                const int mX = x + mn[0], mY = y + mn[1], mZ = z;
+                if (mX >= e.X || mY >= e.Y || mZ >= e.Z || mX < 0 || mY < 0 || mZ < 0) continue;
+                if (param != 8 && mY <= e.GH[mX + (long)e.X * mZ]) continue;
+                editCell(e, param, gNew, changed, mX, mY, mZ, value, dir, erase, false);
+            }
+    finishEdit(e, gNew, changed, erase);
+}
+
 // irridianceGPU, Game/environment.cu:1021-1034 (host float arithmetic).
 static float irradiance(float time, int day, float lon, float sun) {
     const float Gs = 1361.0f * sun;
@@ -1851,6 +1949,54 @@ void diagnostics(const Env<R>& e, float thr, float* maxw, int* level, double* ra
 }
 }  // namespace sko
 
+namespace sko {
+// Terrain of environmentGPU::init (Game/environment.cu:321-340): PNoise2D
+// (Engine/source/math/geometry.cpp:196-278) and initGroundHeightGPU
+// (Game/kernelGround.cu:13-19). rand() after srand(s) is the MSVC runtime's
+// generator (the reference is a Windows program): s * 214013 + 2531011,
+// bits 16..30. sin and cos of the float angle are taken in double and rounded
+// to float.
+static int msvcRandFirst(int s) {
+    const uint32_t st = (uint32_t)s * 214013u + 2531011u;
+    return (int)((st >> 16) & 0x7FFFu);
+}
+static float noiseDot(int seed, int ix, int iy, float x, float y, int width) {
+    const int r = msvcRandFirst(seed + ix + iy * width);  // randomVector
+    const float a = float(r % 1000) * 0.001f * 3.14159265f * 2;
+    const float vx = (float)std::sin((double)a), vy = (float)std::cos((double)a);
+    const float inv = 1.0f / std::sqrt(vx * vx + vy * vy);  // glm::normalize
+    const float gx = vx * inv, gy = vy * inv;
+    const float dx = std::fabs(x - float(ix)), dy = std::fabs(y - float(iy));
+    return std::fabs(dx * gx + dy * gy);
+}
+static float noiseInterp(float a0, float a1, float w) { return (a1 - a0) * (3.0f - w * 2.0f) * w * w + a0; }
+static float perlin(int seed, float x, float y, int width) {
+    const int x0 = int(x), y0 = int(y), x1 = x0 + 1, y1 = y0 + 1;
+    const float sx = x - float(x0), sy = y - float(y0);
+    float n0 = noiseDot(seed, x0, y0, x, y, width);
+    float n1 = noiseDot(seed, x1, y0, x, y, width);
+    const float ix0 = noiseInterp(n0, n1, sx);
+    n0 = noiseDot(seed, x0, y1, x, y, width);
+    n1 = noiseDot(seed, x1, y1, x, y, width);
+    const float ix1 = noiseInterp(n0, n1, sx);
+    return noiseInterp(ix0, ix1, sy);
+}
+void terrain(int seed, int width, int depth, int octaves, float maxHeight, float* noise, int* heights) {
+    const int FREQ0 = width;
+    for (int z = 0; z < depth; z++)
+        for (int x = 0; x < width; x++) {
+            float val = 0.0f, freq = 1.0f, amp = 1.0f;
+            for (int i = 0; i < octaves; i++) {
+                val += perlin(seed, x * freq / FREQ0, z * freq / FREQ0, width) * amp;
+                freq *= 2.0f;
+                amp /= 2.0f;
+            }
+            noise[x + z * width] = val;
+            heights[x + z * width] = static_cast<int>(std::round(val * maxHeight));
+        }
+}
+}  // namespace sko
+
 // ---------------------------------------------------------------- C API ---
 // Handle-based API used by tests (ctypes) and bench.py's CPU baseline.
 using namespace sko;
@@ -1893,6 +2039,10 @@ int sko_set_threads(int n) {
     (void)n;
 #endif
     return 0;
+}
+void sko_get_ground_height(void* hv, int* gh) {
+    SkoHandle* h = (SkoHandle*)hv;
+    DISPATCH(h, { for (long g = 0; g < e.G; g++) gh[g] = e.GH[g]; });
 }
 void sko_set_ground_height(void* hv, const int* gh) {
     SkoHandle* h = (SkoHandle*)hv;
@@ -2056,6 +2206,15 @@ void sko_set_temps_at_ground(void* hv, float dt, float speed) {
     SkoHandle* h = (SkoHandle*)hv;
     DISPATCH(h, setTempsAtGround(e, decltype(e.dx)(dt * speed)));
 }
+void sko_apply_brush(void* hv, int param, float size, const int* pos, float smooth, float dt, float intensity,
+                     float value, const float* dir, int erase) {
+    SkoHandle* h = (SkoHandle*)hv;
+    DISPATCH(h, applyBrush(e, param, size, pos, smooth, dt, intensity, value, dir, erase != 0));
+}
+void sko_apply_selection(void* hv, int param, const int* mn, const int* mx, float value, const float* dir, int erase) {
+    SkoHandle* h = (SkoHandle*)hv;
+    DISPATCH(h, applySelection(e, param, mn, mx, value, dir, erase != 0));
+}
 void sko_editor_data(void* hv) {
     SkoHandle* h = (SkoHandle*)hv;
     DISPATCH(h, editorData(e));
@@ -2083,6 +2242,11 @@ void sko_region(void* hv, int active, const int* mn, const int* mx) {
         for (int k = 0; k < 33; k++) e.rsum[k] = 0;
     });
 }
+void sko_terrain(int seed, int width, int depth, int octaves, float maxHeight, float* noise, int* heights) {
+    terrain(seed, width, depth, octaves, maxHeight, noise, heights);
+}
+int sko_msvc_rand_first(int s) { return msvcRandFirst(s); }
+float sko_perlin(int seed, float x, float y, int width) { return perlin(seed, x, y, width); }
 void sko_diagnostics(void* hv, float thr, float* maxw, int* level, double* rain) {
     SkoHandle* h = (SkoHandle*)hv;
     DISPATCH(h, diagnostics(e, thr, maxw, level, rain));

@@ -354,6 +354,7 @@ void freeAll(EnvState& e) {
     fr(e.gh);
     fr(e.ghm1);
     fr(e.ghm2);
+    fr(e.ghEdit);
     fr(e.irho);
     fr(e.zeros);
     fr(e.ones);
@@ -436,6 +437,7 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
         if (dalloc(e, &e.pcg[k], sizeof(float) * e.N)) return bail(SKT_ERR_NOMEM);
     if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.ghm1, sizeof(int) * e.G) ||
         dalloc(e, &e.ghm2, sizeof(int) * e.G) || dalloc(e, &e.irho, sizeof(float) * e.N) ||
+        dalloc(e, &e.ghEdit, sizeof(int) * e.G) ||
         dalloc(e, &e.zeros, sizeof(float) * e.Y) || dalloc(e, &e.ones, sizeof(float) * e.Y) ||
         dalloc(e, &e.phase, (size_t)e.N))
         return bail(SKT_ERR_NOMEM);
@@ -532,6 +534,14 @@ int skt_env_set_ground_height(skt_env* env, const int32_t* gh) {
     HIPCHK(e, hipMemcpyAsync(e.ghm2, m2.data(), sizeof(int) * e.G, hipMemcpyHostToDevice, e.stream));
     HIPCHK(e, hipStreamSynchronize(e.stream));
     e.params.ground_changed = 1;
+    return SKT_OK;
+}
+
+int skt_env_get_ground_height(skt_env* env, int32_t* gh) {
+    if (!env || !gh) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    HIPCHK(e, hipMemcpyAsync(gh, e.gh, sizeof(int) * e.G, hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(e, hipStreamSynchronize(e.stream));
     return SKT_OK;
 }
 
@@ -720,6 +730,81 @@ int skt_env_get_stats(skt_env* env, skt_step_stats* st) {
     *st = env->s.stats;
     return SKT_OK;
 }
+// environmentGPU::prepareBrushGPU / prepareSelectionGPU
+// (Game/environment.cu:1127-1260): the edit runs on a copy of the ground
+// heights; when it touched them, an erase first resets the cells it uncovered
+// (compareAndResetValuesOutGround), then the copy becomes the ground (with its
+// derived maps) and the ground-change path of editorDataGPU runs.
+static int applyEdit(skt_env* env, EditArgs& a, bool brush) {
+    EnvState& e = env->s;
+    a.a[0] = a.a[1] = a.a[2] = nullptr;
+    switch (a.param) {
+        case SKT_PARAM_POTTEMP: a.a[0] = e.f[SKT_POTTEMP]; break;
+        case SKT_PARAM_QV: a.a[0] = e.f[SKT_QV]; break;
+        case SKT_PARAM_QW: a.a[0] = e.f[SKT_QW]; break;
+        case SKT_PARAM_QC: a.a[0] = e.f[SKT_QC]; break;
+        case SKT_PARAM_QR: a.a[0] = e.f[SKT_QR]; break;
+        case SKT_PARAM_QS: a.a[0] = e.f[SKT_QS]; break;
+        case SKT_PARAM_QI: a.a[0] = e.f[SKT_QI]; break;
+        case SKT_PARAM_WIND:
+            a.a[0] = e.f[SKT_VELX];
+            a.a[1] = e.f[SKT_VELY];
+            a.a[2] = e.f[SKT_VELZ];
+            break;
+        case SKT_PARAM_PGROUND: break;
+        case SKT_PARAM_PRESSURE: a.a[0] = e.f[SKT_PRESSURE]; break;
+        default: return setErr(e, SKT_ERR_ARG, "edit: unknown parameter");
+    }
+    HIPCHK(e, hipSetDevice(e.device));
+    HIPCHK(e, hipMemcpyAsync(e.ghEdit, e.gh, sizeof(int) * e.G, hipMemcpyDeviceToDevice, e.stream));
+    HIPCHK(e, hipMemsetAsync(&e.sc->edit_ground, 0, sizeof(int), e.stream));
+    HIPCHK(e, brush ? launch_apply_brush(e, a) : launch_apply_selection(e, a));
+    int rc = readScalars(e);
+    if (rc) return rc;
+    if (e.sc_host->edit_ground) {
+        if (a.erase) HIPCHK(e, launch_reset_uncovered(e));
+        std::vector<int32_t> h((size_t)e.G);
+        HIPCHK(e, hipMemcpyAsync(h.data(), e.ghEdit, sizeof(int) * e.G, hipMemcpyDeviceToHost, e.stream));
+        HIPCHK(e, hipStreamSynchronize(e.stream));
+        rc = skt_env_set_ground_height(env, h.data());
+        if (rc) return rc;
+    }
+    rc = editorData(e);
+    if (rc) return rc;
+    HIPCHK(e, hipStreamSynchronize(e.stream));
+    return SKT_OK;
+}
+int skt_env_apply_brush(skt_env* env, int32_t param, float brush_size, const int32_t pos[3], float smoothness,
+                        float dt, float intensity, float value, const float dir[3], int32_t ground_erase) {
+    if (!env || !pos || !dir) return SKT_ERR_ARG;
+    EditArgs a{};
+    a.param = param;
+    a.size = brush_size;
+    a.smooth = smoothness;
+    a.dt = dt;
+    a.intensity = intensity;
+    a.value = value;
+    a.erase = ground_erase != 0;
+    for (int k = 0; k < 3; k++) {
+        a.pos[k] = pos[k];
+        a.dir[k] = dir[k];
+    }
+    return applyEdit(env, a, true);
+}
+int skt_env_apply_selection(skt_env* env, int32_t param, const int32_t min_pos[3], const int32_t max_pos[3],
+                            float value, const float dir[3], int32_t ground_erase) {
+    if (!env || !min_pos || !max_pos || !dir) return SKT_ERR_ARG;
+    EditArgs a{};
+    a.param = param;
+    a.value = value;
+    a.erase = ground_erase != 0;
+    for (int k = 0; k < 3; k++) {
+        a.mn[k] = min_pos[k];
+        a.mx[k] = max_pos[k];
+        a.dir[k] = dir[k];
+    }
+    return applyEdit(env, a, false);
+}
 int skt_env_diagnostics(skt_env* env, float thr, skt_diagnostics* out) {
     if (!env || !out) return SKT_ERR_ARG;
     EnvState& e = env->s;
@@ -792,6 +877,26 @@ int skt_env_copy_probe(skt_env* env, int32_t repeats, float* ms_per_copy, double
     if (bytes_per_copy) *bytes_per_copy = 8.0 * (double)(e.N / 4 * 4);
     return SKT_OK;
 }
+int skt_terrain_generate(int32_t device, int32_t seed, int32_t nx, int32_t nz, int32_t octaves, float max_height,
+                         float* noise_out, int32_t* heights_out) {
+    if (nx < 1 || nz < 1 || octaves < 0 || (!noise_out && !heights_out)) return SKT_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return SKT_ERR_HIP;
+    const size_t n = (size_t)nx * nz;
+    float* dn = nullptr;
+    int* dh = nullptr;
+    if (hipMalloc((void**)&dn, n * sizeof(float)) != hipSuccess) return SKT_ERR_NOMEM;
+    if (hipMalloc((void**)&dh, n * sizeof(int)) != hipSuccess) {
+        (void)hipFree(dn);
+        return SKT_ERR_NOMEM;
+    }
+    hipError_t r = launch_terrain(seed, nx, nz, octaves, max_height, dn, dh, nullptr);
+    if (r == hipSuccess && noise_out) r = hipMemcpy(noise_out, dn, n * sizeof(float), hipMemcpyDeviceToHost);
+    if (r == hipSuccess && heights_out) r = hipMemcpy(heights_out, dh, n * sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(dn);
+    (void)hipFree(dh);
+    return r == hipSuccess ? SKT_OK : SKT_ERR_HIP;
+}
+
 int skt_device_synchronize(int32_t device) {
     if (hipSetDevice(device) != hipSuccess) return SKT_ERR_HIP;
     return hipDeviceSynchronize() == hipSuccess ? SKT_OK : SKT_ERR_HIP;

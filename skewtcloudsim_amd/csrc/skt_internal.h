@@ -30,7 +30,7 @@ struct Scalars {
     float dtsub;   // advection substep length
     float dg_w;    // serving diagnostics: max vertical velocity over sky cells
     int dg_level;  //   highest cloudy level (-1: none)
-    int dg_pad;
+    int edit_ground;  // an edit touched the ground heights
     double dg_rain;  //   ground rain water summed over the columns
     int pad[24];
 };
@@ -69,6 +69,7 @@ struct EnvState {
     int* gh = nullptr;                      // ground height [G]
     int* ghm1 = nullptr;                    // max ground height over the 3x3 column neighbourhood [G]
     int* ghm2 = nullptr;                    // max ground height over the 5x5 column neighbourhood [G]
+    int* ghEdit = nullptr;                  // ground heights being edited by a brush or selection [G]
     float* work[SKT_NWORK] = {nullptr};     // N-sized work buffers
     float* pcg[6] = {nullptr};              // r, z, s_a, s_b, As, pre
     float* irho = nullptr;                  // 1/density of the last pressure solve (0 in ground)
@@ -138,6 +139,23 @@ struct DiffBatch {
 hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k);
 hipError_t launch_maxvel(EnvState& e, float dt);
 hipError_t launch_diagnostics(EnvState& e, float thr);
+// Brush / selection edits (environmentGPU::prepareBrushGPU / prepareSelectionGPU).
+struct EditArgs {
+    int param;               // parameter id (POTTEMP 0 .. QI 6, WIND 7, PGROUND 8, PRESSURE 9)
+    float* a[3];             // the parameter's arrays (wind: x, y, z components)
+    float value, dir[3];     // applyValue, valueDir
+    int erase;               // groundErase
+    // brush
+    float size, smooth, intensity, dt;
+    int pos[3];
+    // selection
+    int mn[3], mx[3];
+};
+hipError_t launch_apply_brush(EnvState& e, const EditArgs& a);
+hipError_t launch_apply_selection(EnvState& e, const EditArgs& a);
+hipError_t launch_reset_uncovered(EnvState& e);
+hipError_t launch_terrain(int seed, int width, int depth, int octaves, float maxHeight, float* noise, int* heights,
+                          hipStream_t stream);
 // Independent fields moved by one PPM pass launch.
 struct PpmField {
     const float* in;

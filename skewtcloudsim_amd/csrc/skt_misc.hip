@@ -745,4 +745,186 @@ hipError_t launch_diagnostics(EnvState& e, float thr) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- terrain ---
+// PNoise2D (Engine/source/math/geometry.cpp:196-278) and initGroundHeightGPU
+// (Game/kernelGround.cu:13-19): one thread per ground column. The gradient
+// angle of a lattice point comes from the C runtime's rand() after
+// srand(seed + ix + iy * width); the reference is a Windows (MSBuild) program,
+// so that is the MSVC runtime's generator: state = state * 214013 + 2531011,
+// value = (state >> 16) & 0x7FFF. Every float expression keeps the reference's
+// operand order; glm::normalize is v * (1 / sqrt(dot(v, v))).
+__device__ __forceinline__ float2 noiseGradient(int seed, int ix, int iy, int width) {
+    const unsigned st = (unsigned)(seed + ix + iy * width) * 214013u + 2531011u;
+    const int r = (int)((st >> 16) & 0x7FFFu);
+    const float a = float(r % 1000) * 0.001f * 3.14159265f * 2;
+    const float vx = sinf(a), vy = cosf(a);
+    const float inv = 1.0f / sqrtf(vx * vx + vy * vy);
+    return make_float2(vx * inv, vy * inv);
+}
+__device__ __forceinline__ float noiseDot(int seed, int ix, int iy, float x, float y, int width) {
+    const float2 g = noiseGradient(seed, ix, iy, width);
+    const float dx = fabsf(x - float(ix)), dy = fabsf(y - float(iy));
+    return fabsf(dx * g.x + dy * g.y);
+}
+__device__ __forceinline__ float noiseInterp(float a0, float a1, float w) {
+    return (a1 - a0) * (3.0f - w * 2.0f) * w * w + a0;
+}
+__device__ float perlin2(int seed, float x, float y, int width) {
+    const int x0 = int(x), y0 = int(y), x1 = x0 + 1, y1 = y0 + 1;
+    const float sx = x - float(x0), sy = y - float(y0);
+    float n0 = noiseDot(seed, x0, y0, x, y, width);
+    float n1 = noiseDot(seed, x1, y0, x, y, width);
+    const float ix0 = noiseInterp(n0, n1, sx);
+    n0 = noiseDot(seed, x0, y1, x, y, width);
+    n1 = noiseDot(seed, x1, y1, x, y, width);
+    const float ix1 = noiseInterp(n0, n1, sx);
+    return noiseInterp(ix0, ix1, sy);
+}
+KL void k_terrain(int seed, int width, int depth, int octaves, float maxHeight, float* __restrict__ noise,
+                  int* __restrict__ heights) {
+    const long g = blockIdx.x * 256L + tid256();
+    if (g >= (long)width * depth) return;
+    const int x = (int)(g % width), z = (int)(g / width);
+    const int FREQ0 = width;
+    float val = 0.0f, freq = 1.0f, amp = 1.0f;
+    for (int i = 0; i < octaves; i++) {
+        val += perlin2(seed, x * freq / FREQ0, z * freq / FREQ0, width) * amp;
+        freq *= 2.0f;
+        amp /= 2.0f;
+    }
+    noise[g] = val;
+    heights[g] = static_cast<int>(roundf(val * maxHeight));
+}
+hipError_t launch_terrain(int seed, int width, int depth, int octaves, float maxHeight, float* noise, int* heights,
+                          hipStream_t stream) {
+    const long n = (long)width * depth;
+    hipLaunchKernelGGL(k_terrain, dim3((unsigned)((n + 255) / 256)), B3, 0, stream, seed, width, depth, octaves,
+                       maxHeight, noise, heights);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ edits ---
+// setGround (Game/kernelSky.cu:1620-1633): against the current heights
+// (G.gh), erasing at a ground cell lowers the edited column to y, raising at a
+// sky cell lifts it to y; either reports a change.
+__device__ __forceinline__ void setGround(const Geom& G, int* ghEdit, int* changed, int x, int y, int z, int erase) {
+    const int g = x + G.X * z;
+    if (erase && y <= G.gh[g]) {
+        atomicMin(&ghEdit[g], y);
+        *changed = 1;
+    } else if (!erase && y > G.gh[g]) {
+        atomicMax(&ghEdit[g], y);
+        *changed = 1;
+    }
+}
+// Adds (brush) or sets (selection) the parameter value at one cell.
+__device__ __forceinline__ void applyEditCell(const Geom& G, const EditArgs& a, int* ghEdit, int* changed, int mX,
+                                              int mY, int mZ, float v1, bool add) {
+    float v2 = 0.0f, v3 = 0.0f;
+    if (a.param == 7) {
+        v3 = v1 * a.dir[2];
+        v2 = v1 * a.dir[1];
+        v1 *= a.dir[0];
+    }
+    const long i = gidx(G, mX, mY, mZ);
+    if (a.param == 8) {
+        setGround(G, ghEdit, changed, mX, mY, mZ, a.erase);
+    } else if (a.param == 7) {
+        if (add) {
+            a.a[0][i] += v1;
+            a.a[1][i] += v2;
+            a.a[2][i] += v3;
+        } else {
+            a.a[0][i] = v1;
+            a.a[1][i] = v2;
+            a.a[2][i] = v3;
+        }
+    } else {
+        if (add) a.a[0][i] += v1;
+        else a.a[0][i] = v1;
+    }
+}
+// applyBrushGPU (Game/kernelSky.cu:1476-1556): thread (x, y) of the
+// n x n square, n = ceil(size) - floor(-size), and every z of the domain; the
+// cell at (x, y, z) - ceil(size) from the brush position gets
+// (1 - d^2/size^2)^smooth * dt * intensity * value inside the sphere. A
+// position beyond the domain is skipped (the reference's z test lets z = nz
+// through, one plane past its arrays).
+KL void k_apply_brush(Geom G, EditArgs a, int n, int* __restrict__ ghEdit, int* changed) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (x >= n || y >= n) return;
+    const int off = int(ceilf(a.size));
+    const int lX = x - off, lY = y - off, lZ = z - off;
+    const int mX = x + a.pos[0] - off, mY = y + a.pos[1] - off, mZ = z + a.pos[2] - off;
+    if (mX >= G.X || mY >= G.Y || mZ >= G.Z || mX < 0 || mY < 0 || mZ < 0) return;
+    if (a.param != 8 && mY <= G.gh[mX + G.X * mZ]) return;  // can't brush the ground
+    const float eX = lX, eY = lY, eZ = lZ;
+    const float distance = eX * eX + eY * eY + eZ * eZ;
+    const float radius = a.size * a.size;
+    if (distance <= radius) {
+        float v1 = -(distance / radius - 1);
+        v1 = powf(v1, a.smooth);
+        v1 *= a.dt * a.intensity * a.value;
+        applyEditCell(G, a, ghEdit, changed, mX, mY, mZ, v1, true);
+    }
+}
+// applySelectionGPU (Game/kernelSky.cu:1558-1618): the box [mn, mx] (an
+// empty x or y range still runs one thread at mn) gets the value.
+KL void k_apply_selection(Geom G, EditArgs a, int nx, int ny, int* __restrict__ ghEdit, int* changed) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = a.mn[2] + (int)blockIdx.z;
+    if (x >= nx || y >= ny) return;
+    const int mX = x + a.mn[0], mY = y + a.mn[1], mZ = z;
+    if (mX >= G.X || mY >= G.Y || mZ >= G.Z || mX < 0 || mY < 0 || mZ < 0) return;
+    if (a.param != 8 && mY <= G.gh[mX + G.X * mZ]) return;
+    applyEditCell(G, a, ghEdit, changed, mX, mY, mZ, a.value, false);
+}
+// compareAndResetValuesOutGround (Game/kernelSky.cu:1635-1663): cells that an
+// erase turned from ground into sky (new height < y <= old height) get the
+// base profiles: vapour and potential temperature of the isentropic state,
+// the default pressure, no condensate and no wind.
+KL void k_reset_uncovered(Geom G, const int* __restrict__ ghNew, const float* __restrict__ isT,
+                          const float* __restrict__ isV, const float* __restrict__ defP, float* Qv, float* Qw,
+                          float* Qc, float* Qr, float* Qs, float* Qi, float* th, float* u, float* v, float* w,
+                          float* p) {
+    CELL_SETUP;
+    (void)bid;
+    if (!inb) return;
+    const int g = x + G.X * z;
+    if (y > G.gh[g] || y <= ghNew[g]) return;
+    const long i = gidx(G, x, y, z);
+    Qv[i] = isV[y];
+    Qw[i] = 0.0f;
+    Qc[i] = 0.0f;
+    Qr[i] = 0.0f;
+    Qs[i] = 0.0f;
+    Qi[i] = 0.0f;
+    th[i] = isT[y];
+    u[i] = 0.0f;
+    v[i] = 0.0f;
+    w[i] = 0.0f;
+    p[i] = defP[y];
+}
+hipError_t launch_apply_brush(EnvState& e, const EditArgs& a) {
+    const int n = int(std::ceil(a.size)) - int(std::floor(-a.size));
+    if (n <= 0) return hipSuccess;
+    const dim3 grid((n + 63) / 64, (n + 3) / 4, e.Z);
+    hipLaunchKernelGGL(k_apply_brush, grid, B3, 0, e.stream, geom(e), a, n, e.ghEdit, &e.sc->edit_ground);
+    return hipGetLastError();
+}
+hipError_t launch_apply_selection(EnvState& e, const EditArgs& a) {
+    const int nx = std::max(a.mx[0] - a.mn[0] + 1, 1), ny = std::max(a.mx[1] - a.mn[1] + 1, 1);
+    const int nz = a.mx[2] - a.mn[2] + 1;
+    if (nz <= 0) return hipSuccess;
+    const dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
+    hipLaunchKernelGGL(k_apply_selection, grid, B3, 0, e.stream, geom(e), a, nx, ny, e.ghEdit, &e.sc->edit_ground);
+    return hipGetLastError();
+}
+hipError_t launch_reset_uncovered(EnvState& e) {
+    hipLaunchKernelGGL(k_reset_uncovered, grid3(e), B3, 0, e.stream, geom(e), e.ghEdit, e.f[SKT_ISEN_TEMP],
+                       e.f[SKT_ISEN_VAPOR], e.f[SKT_DEFAULT_PRESSURE], e.f[SKT_QV], e.f[SKT_QW], e.f[SKT_QC],
+                       e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_POTTEMP], e.f[SKT_VELX], e.f[SKT_VELY],
+                       e.f[SKT_VELZ], e.f[SKT_PRESSURE]);
+    return hipGetLastError();
+}
+
 }  // namespace skt

@@ -30,6 +30,19 @@ def default_config(nx, ny, nz, dx=100.0, **overrides):
     return cfg
 
 
+def terrain(nx, nz, seed=100, octaves=12, max_height=0.2, device=0):
+    """Terrain noise and ground heights of environmentGPU::init
+    (skt_terrain_generate: PNoise2D, then round(noise * max_height)).
+    Returns (noise float32[nz*nx], heights int32[nz*nx]), column x + nx*z."""
+    lib = _lib.load_library()
+    noise = np.empty(nx * nz, np.float32)
+    heights = np.empty(nx * nz, np.int32)
+    rc = lib.skt_terrain_generate(device, seed, nx, nz, octaves, max_height, _ptr(noise), _ptr(heights))
+    if rc:
+        raise SktError(f"skt_terrain_generate failed ({rc})")
+    return noise, heights
+
+
 def make_sounding(nx, ny, nz, dx=100.0, seed=None, n_bubbles=0):
     """Init arrays of the synthetic sounding (skt_sounding_generate).
 
@@ -190,6 +203,32 @@ class EnvironmentGPU:
         s = Stats()
         self._chk(self.lib.skt_env_get_stats(self.h, ctypes.byref(s)), "stats")
         return s
+
+    _PARAMS = {"POTTEMP": 0, "QV": 1, "QW": 2, "QC": 3, "QR": 4, "QS": 5, "QI": 6, "WIND": 7, "PGROUND": 8,
+               "PRESSURE": 9}
+
+    def prepareBrushGPU(self, param, brushSize, mousePos, brushSmoothness, dt, brushIntensity, applyValue,
+                        valueDir=(0.0, 0.0, 0.0), groundErase=False):
+        """environmentGPU::prepareBrushGPU (skt_env_apply_brush)."""
+        p = self._PARAMS.get(param, param)
+        pos = np.asarray(mousePos, np.int32)
+        d = np.asarray(valueDir, np.float32)
+        self._chk(self.lib.skt_env_apply_brush(self.h, p, brushSize, _ptr(pos), brushSmoothness, dt, brushIntensity,
+                                               applyValue, _ptr(d), int(bool(groundErase))), "prepareBrushGPU")
+
+    def prepareSelectionGPU(self, param, minPos, maxPos, applyValue, valueDir=(0.0, 0.0, 0.0), groundErase=False):
+        """environmentGPU::prepareSelectionGPU (skt_env_apply_selection)."""
+        p = self._PARAMS.get(param, param)
+        mn, mx = np.asarray(minPos, np.int32), np.asarray(maxPos, np.int32)
+        d = np.asarray(valueDir, np.float32)
+        self._chk(self.lib.skt_env_apply_selection(self.h, p, _ptr(mn), _ptr(mx), applyValue, _ptr(d),
+                                                   int(bool(groundErase))), "prepareSelectionGPU")
+
+    def ground_height(self):
+        """The ground heights in use, int32[nz*nx] (column x + nx*z)."""
+        gh = np.empty(self.G, np.int32)
+        self._chk(self.lib.skt_env_get_ground_height(self.h, _ptr(gh)), "ground_height")
+        return gh
 
     def diagnostics(self, cloud_threshold=1e-5):
         """Maximum updraft, cloud-top level and height, total ground rain

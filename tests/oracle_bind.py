@@ -52,7 +52,9 @@ def lib():
             "sko_editor_data": [vp], "sko_masks": [vp, vp], "sko_phase": [vp, vp],
             "sko_stats": [vp, vp, vp, vp, vp, vp], "sko_region": [vp, i, vp, vp], "sko_region_get": [vp, vp],
             "sko_set_threads": [i], "sko_fallvel": [f, f, f, f, i, vp], "sko_gammas": [vp],
-            "sko_diagnostics": [vp, f, vp, vp, vp],
+            "sko_diagnostics": [vp, f, vp, vp, vp], "sko_terrain": [i, i, i, i, f, vp, vp],
+            "sko_apply_brush": [vp, i, f, vp, f, f, f, f, vp, i], "sko_apply_selection": [vp, i, vp, vp, f, vp, i],
+            "sko_get_ground_height": [vp, vp],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args
@@ -65,6 +67,10 @@ def lib():
                         ("sko_ViscAir", 1), ("sko_MLR", 2), ("sko_cvd", 3), ("sko_potTemp", 3)]:
             getattr(L, name).restype = f
             getattr(L, name).argtypes = [f] * n
+        L.sko_msvc_rand_first.restype = i
+        L.sko_msvc_rand_first.argtypes = [i]
+        L.sko_perlin.restype = f
+        L.sko_perlin.argtypes = [i, f, f, i]
         L.sko_slopePrecip.restype = f
         L.sko_slopePrecip.argtypes = [f, f, i]
         L.sko_potTemp64.restype = ctypes.c_double
@@ -186,6 +192,19 @@ class Oracle:
 
     def irradiance(self):
         return self.L.sko_irradiance(self.h)
+
+    def ground_height(self):
+        gh = np.empty(self.nx * self.nz, np.int32)
+        self.L.sko_get_ground_height(self.h, _p(gh))
+        return gh
+
+    def brush(self, param, size, pos, smooth, dt, intensity, value, direction=(0.0, 0.0, 0.0), erase=False):
+        p, d = np.asarray(pos, np.int32), np.asarray(direction, np.float32)
+        self.L.sko_apply_brush(self.h, param, size, _p(p), smooth, dt, intensity, value, _p(d), int(bool(erase)))
+
+    def selection(self, param, mn, mx, value, direction=(0.0, 0.0, 0.0), erase=False):
+        a, b, d = np.asarray(mn, np.int32), np.asarray(mx, np.int32), np.asarray(direction, np.float32)
+        self.L.sko_apply_selection(self.h, param, _p(a), _p(b), value, _p(d), int(bool(erase)))
 
     def diagnostics(self, thr=1e-5):
         w, lev, rain = ctypes.c_float(), ctypes.c_int(), ctypes.c_double()

@@ -475,3 +475,70 @@ def test_diagnostics(cap):
     assert d2["max_updraft"] == d["max_updraft"] and d2["cloud_top_level"] == d["cloud_top_level"]
     env.close()
     orc.close()
+
+
+@pytest.mark.parametrize("nx,nz,octaves,maxh", [(24, 20, 12, 0.2), (64, 48, 12, 6.0), (33, 17, 4, 3.0)])
+def test_terrain_generate(nx, nz, octaves, maxh):
+    """skt_terrain_generate against the oracle: the noise within a few float
+    ulps (sin/cos of the device library against double-rounded ones), the
+    heights equal except where noise * max_height lies within 1e-4 of a tie."""
+    import skewtcloudsim_amd as skt
+    noise, h = skt.terrain(nx, nz, seed=100, octaves=octaves, max_height=maxh)
+    on = np.empty(nx * nz, np.float32)
+    oh = np.empty(nx * nz, np.int32)
+    ob.lib().sko_terrain(100, nx, nz, octaves, maxh, on.ctypes.data, oh.ctypes.data)
+    assert np.allclose(noise, on, rtol=2e-6, atol=2e-6)
+    v = on.astype(np.float64) * maxh
+    near_tie = np.abs(v - np.floor(v) - 0.5) < 1e-4
+    assert np.array_equal(h[~near_tie], oh[~near_tie])
+    if maxh == 0.2:
+        assert np.all(h == 0)
+
+
+EDITS = [
+    ("brush", dict(param=7, size=2.5, pos=(6, 8, 5), smooth=1.0, dt=0.5, intensity=2.0, value=3.0,
+                   direction=(1.0, 0.5, -0.25))),
+    ("brush", dict(param=1, size=3.0, pos=(0, 6, 15), smooth=2.0, dt=0.5, intensity=1.5, value=2e-3)),
+    ("brush", dict(param=0, size=1.5, pos=(23, 19, 8), smooth=0.5, dt=1.0, intensity=1.0, value=4.0)),
+    ("selection", dict(param=3, mn=(3, 2, 4), mx=(9, 7, 6), value=5e-4)),
+    ("selection", dict(param=7, mn=(10, 5, 1), mx=(12, 9, 2), value=2.0, direction=(0.0, 1.0, 0.0))),
+    ("selection", dict(param=2, mn=(8, 3, 3), mx=(5, 6, 3), value=1e-4)),   # empty x range: one column
+    ("brush", dict(param=8, size=2.0, pos=(9, 4, 9), smooth=1.0, dt=1.0, intensity=1.0, value=1.0,
+                   erase=True)),
+    ("selection", dict(param=8, mn=(15, 6, 3), mx=(16, 7, 4), value=0.0, erase=False)),
+    ("selection", dict(param=8, mn=(2, 1, 12), mx=(4, 5, 13), value=0.0, erase=True)),
+]
+
+
+def test_editor_edits():
+    """Brush and selection edits (prepareBrushGPU / prepareSelectionGPU) on
+    every kind of parameter, clipped at the domain edges, raising and
+    erasing ground, against the oracle; then a step from the edited state."""
+    nx, ny, nz = 24, 20, 16
+    gh = _terrain(nx, nz, "hill")
+    env, orc = make_terrain_pair(nx, ny, nz, gh)
+    fields = SKY[:-1] + GROUND
+    for kind, a in EDITS:
+        a = dict(a)
+        if kind == "brush":
+            env.prepareBrushGPU(a["param"], a["size"], a["pos"], a["smooth"], a["dt"], a["intensity"], a["value"],
+                                a.get("direction", (0.0, 0.0, 0.0)), a.get("erase", False))
+            orc.brush(a["param"], a["size"], a["pos"], a["smooth"], a["dt"], a["intensity"], a["value"],
+                      a.get("direction", (0.0, 0.0, 0.0)), a.get("erase", False))
+        else:
+            env.prepareSelectionGPU(a["param"], a["mn"], a["mx"], a["value"], a.get("direction", (0.0, 0.0, 0.0)),
+                                    a.get("erase", False))
+            orc.selection(a["param"], a["mn"], a["mx"], a["value"], a.get("direction", (0.0, 0.0, 0.0)),
+                          a.get("erase", False))
+        assert np.array_equal(env.ground_height(), orc.ground_height()), (kind, a)
+        m = sky_mask(nx, ny, nz, env.ground_height())
+        # powf of the device library against the host's: a float ulp or two
+        assert_fields(env, orc, fields, m, l2=1e-6, maxabs=1e-5, tag=f"edit {kind} {a['param']}")
+    assert not np.array_equal(env.ground_height(), gh)
+    env.updateGPU(0.5, 1.0)
+    orc.step(0.5, 1.0)
+    assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+    m = sky_mask(nx, ny, nz, env.ground_height())
+    assert_fields(env, orc, SKY, m, l2=1e-5, maxabs=5e-4, tag="edit-step")
+    env.close()
+    orc.close()

@@ -574,3 +574,115 @@ def test_diagnostics_match_brute_force_after_steps():
         assert d["cloud_top_level"] == lev
         assert d["surface_rain"] == pytest.approx(rain, rel=1e-12, abs=1e-300)
     o.close()
+
+
+# -------------------------------------------------------------------- terrain ---
+def test_terrain_msvc_rand_known_answers():
+    """rand() after srand(s) in the MSVC runtime: the documented first values
+    41 (s = 1) and 38 (s = 0), and the LCG step for another seed."""
+    L = ob.lib()
+    assert L.sko_msvc_rand_first(1) == 41
+    assert L.sko_msvc_rand_first(0) == 38
+    s = 123456
+    assert L.sko_msvc_rand_first(s) == ((s * 214013 + 2531011) % 2**32 >> 16) & 0x7FFF
+
+
+def test_terrain_perlin_properties():
+    """Perlin noise (Engine/source/math/geometry.cpp:196-246): zero at lattice
+    points, continuous across cell edges (the cubic weight reaches 1), and
+    bounded by the distance to the nearest corner (unit gradients)."""
+    L = ob.lib()
+    w = 16
+    for ix, iy in [(0, 0), (3, 5), (7, 1)]:
+        assert L.sko_perlin(100, float(ix), float(iy), w) == 0.0
+    for x in (1.0, 2.0, 5.0):
+        for y in (0.25, 1.5, 3.75):
+            a = L.sko_perlin(100, np.nextafter(np.float32(x), np.float32(0)), y, w)
+            b = L.sko_perlin(100, x, y, w)
+            assert abs(a - b) < 1e-5
+    for x, y in RNG.uniform(0, 8, (200, 2)):
+        fx, fy = x - math.floor(x), y - math.floor(y)
+        dmax = max(math.hypot(a, b) for a in (fx, 1 - fx) for b in (fy, 1 - fy))
+        assert 0.0 <= L.sko_perlin(7, float(x), float(y), w) <= dmax + 1e-6
+
+
+def test_terrain_default_is_flat():
+    """The reference's init uses max height 0.2 with 12 octaves: the octave sum
+    stays below 2.5, so every ground height rounds to 0."""
+    nx, nz = 24, 20
+    noise = np.empty(nx * nz, np.float32)
+    h = np.empty(nx * nz, np.int32)
+    ob.lib().sko_terrain(100, nx, nz, 12, 0.2, noise.ctypes.data, h.ctypes.data)
+    assert np.all(h == 0) and noise.max() < 2.5 and noise.min() >= 0.0
+    # roundf (halves away from zero) of a non-negative value is floor(v + 0.5);
+    # scaling by 4 is exact in float
+    ob.lib().sko_terrain(100, nx, nz, 12, 4.0, noise.ctypes.data, h.ctypes.data)
+    assert np.array_equal(h, np.floor(noise.astype(np.float64) * 4.0 + 0.5).astype(np.int32))
+    assert h.max() > 0
+
+
+# ------------------------------------------------------------------- edits ---
+def _edit_state():
+    o, _ = _small_state(nx=12, ny=10, nz=9)
+    gh = np.zeros(12 * 9, np.int32)
+    gh[4 + 12 * 3] = 4                         # column x=4, z=3 of height 4
+    o.set_ground_height(gh)
+    o.editor_data()
+    return o, gh
+
+
+def test_edit_brush_closed_form():
+    """applyBrushGPU: the centre gets dt * intensity * value (distance 0);
+    a cell at +ceil(size) along x is outside the thread square even when
+    inside the sphere (the reference's n = ceil(s) - floor(-s) threads per
+    axis start at -ceil(s)); cells beyond the radius are untouched."""
+    o, gh = _edit_state()
+    L = ob.lib()
+    nx, ny = 12, 10
+    idx = lambda x, y, z: x + nx * (y + ny * z)
+    before = o.get("QV").copy()
+    pos = np.asarray([6, 5, 4], np.int32)
+    d = np.zeros(3, np.float32)
+    L.sko_apply_brush(o.h, 1, 3.0, pos.ctypes.data, 2.0, 0.5, 1.5, 2.0, d.ctypes.data, 0)
+    after = o.get("QV")
+    assert after[idx(6, 5, 4)] == np.float32(before[idx(6, 5, 4)] + np.float32(0.5 * 1.5 * 2.0))
+    assert after[idx(9, 5, 4)] == before[idx(9, 5, 4)]            # +3 along x: not covered
+    assert after[idx(3, 5, 4)] == before[idx(3, 5, 4)]            # -3 along x: weight 0 adds 0
+    w = np.float32(1 - 1.0 / 9.0) ** np.float32(2.0)
+    assert after[idx(7, 5, 4)] == np.float32(before[idx(7, 5, 4)] + np.float32(w * np.float32(1.5)))
+    assert np.array_equal(after[idx(0, 0, 0):idx(0, 0, 1)], before[idx(0, 0, 0):idx(0, 0, 1)])
+    o.close()
+
+
+def test_edit_selection_and_ground():
+    """applySelectionGPU sets the box's sky cells only; erasing ground lowers
+    the column to the lowest touched ground level and gives the uncovered
+    cells the base profiles; raising lifts it and zeroes the buried cells."""
+    o, gh = _edit_state()
+    L = ob.lib()
+    nx, ny, nz = 12, 10, 9
+    idx = lambda x, y, z: x + nx * (y + ny * z)
+    d = np.zeros(3, np.float32)
+    mn, mx = np.asarray([3, 2, 3], np.int32), np.asarray([5, 6, 3], np.int32)
+    L.sko_apply_selection(o.h, 2, mn.ctypes.data, mx.ctypes.data, 7e-4, d.ctypes.data, 0)
+    qw = o.get("QW")
+    assert qw[idx(3, 2, 3)] == np.float32(7e-4) and qw[idx(5, 6, 3)] == np.float32(7e-4)
+    assert qw[idx(4, 2, 3)] == 0.0                                 # ground cell of the tall column
+    assert qw[idx(4, 5, 3)] == np.float32(7e-4)
+    # erase ground in the tall column from level 2 up
+    mn, mx = np.asarray([4, 2, 3], np.int32), np.asarray([4, 6, 3], np.int32)
+    L.sko_apply_selection(o.h, 8, mn.ctypes.data, mx.ctypes.data, 0.0, d.ctypes.data, 1)
+    qv, th = o.get("QV"), o.get("POTTEMP")
+    isv, ist = o.get("ISEN_VAPOR"), o.get("ISEN_TEMP")
+    assert o.ground_height()[4 + nx * 3] == 2
+    for y in (3, 4):                                               # uncovered: 2 < y <= 4
+        assert qv[idx(4, y, 3)] == isv[y] and th[idx(4, y, 3)] == ist[y]
+    assert qv[idx(4, 2, 3)] == 0.0                                 # still ground
+    # raise: a sky cell at level 6 in column x=7, z=5 becomes ground up to 6
+    q = o.get("QV")
+    q[idx(7, 4, 5)] = 1e-3
+    o.set("QV", q)
+    p6 = np.asarray([7, 6, 5], np.int32)
+    L.sko_apply_selection(o.h, 8, p6.ctypes.data, p6.ctypes.data, 0.0, d.ctypes.data, 0)
+    assert o.get("QV")[idx(7, 4, 5)] == 0.0 and o.ground_height()[7 + nx * 5] == 6
+    o.close()
