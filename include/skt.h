@@ -29,7 +29,8 @@ typedef enum skt_status {
     SKT_ERR_ARG = 1,     /* invalid argument (shape, field id, null pointer) */
     SKT_ERR_HIP = 2,     /* a HIP runtime call failed */
     SKT_ERR_STATE = 3,   /* call order (e.g. update before init) */
-    SKT_ERR_NOMEM = 4    /* device allocation failed */
+    SKT_ERR_NOMEM = 4,   /* device allocation failed */
+    SKT_ERR_INPUT = 5    /* sounding input missing or malformed (skt_input_last_error) */
 } skt_status;
 
 /* Boundary conditions, Game/config.h:43-46. */
@@ -268,6 +269,58 @@ int skt_sounding_spec_random(skt_sounding_spec* spec, uint64_t seed, int32_t nx,
 int skt_sounding_generate(const skt_sounding_spec* spec, int32_t nx, int32_t ny, int32_t nz, float dx,
                           float* potTemps, float* velField, float* Qv, float* groundTemp, float* groundPres,
                           float* pressures, float* smallPressure);
+
+/* ---- Sounding ingestion: readTable (Game/readTable.h:8-47, Game/readTable.cpp). ----
+ * A radiosonde profile, one row per sample from the ground up: the columns of
+ * radioSondeData (Game/readTable.h:8-15). Readers fill caller arrays of
+ * 'capacity' rows; when the profile has more rows they return SKT_ERR_ARG with
+ * 'rows' set to the count needed. Errors of these functions (no env) are
+ * described by skt_input_last_error() on the calling thread. */
+typedef struct skt_sounding_table {
+    int32_t rows;        /* rows filled (out) or given (in) */
+    int32_t capacity;    /* length of each column array */
+    float* altitude;     /* m */
+    float* pressure;     /* hPa */
+    float* dew_point;    /* deg C */
+    float* temperature;  /* deg C */
+    float* wind_speed;   /* m/s */
+    float* wind_dir;     /* deg, direction the wind blows from */
+} skt_sounding_table;
+
+/* Text of the last error of a reader or of skt_sounding_init_fields. */
+const char* skt_input_last_error(void);
+
+/* readTable::readKNMIFile (Game/readTable.cpp:28-129): a KNMI .mwx archive,
+ * member SynchronizedSoundingData.xml. */
+int skt_sounding_read_knmi(const char* mwx_path, skt_sounding_table* table);
+/* The KNMI row parser on the text of SynchronizedSoundingData.xml
+ * (Game/readTable.cpp:43-129). */
+int skt_sounding_parse_knmi(const char* xml, int64_t len, skt_sounding_table* table);
+
+/* readTable::readDWDFile (Game/readTable.cpp:134-354): a DWD
+ * sekundenwerte_aero zip. The reference always reads launch index 161 of
+ * the metadata's list (Game/readTable.cpp:206); date_index selects it here
+ * (SKT_DWD_REFERENCE_DATE_INDEX for the reference's choice). */
+enum { SKT_DWD_REFERENCE_DATE_INDEX = 161 };
+int skt_sounding_read_dwd(const char* zip_path, int32_t date_index, skt_sounding_table* table);
+/* The DWD metadata and product parsers on the member texts
+ * (Game/readTable.cpp:160-354). */
+int skt_sounding_parse_dwd(const char* metadata, int64_t metadata_len, const char* product, int64_t product_len,
+                           int32_t date_index, skt_sounding_table* table);
+
+/* readTable::initEnvironment (Game/readTable.cpp:364-474): the init()
+ * arrays of an nx*ny*nz grid from a sounding, on the host (same outputs
+ * as skt_sounding_generate). The sounding must reach (ny-1)*dx above its
+ * first row. */
+int skt_sounding_init_fields(const skt_sounding_table* table, int32_t nx, int32_t ny, int32_t nz, float dx,
+                             float* potTemps, float* velField, float* Qv, float* groundTemp, float* groundPres,
+                             float* pressures, float* smallPressure);
+/* readTable::initEnvironment followed by environmentGPU::init
+ * (Game/readTable.cpp:364-474, Game/environment.cu:249-389): the per-level
+ * values are computed on the host and broadcast over the grid on the device,
+ * with no host copy of the 3-D fields. Same state as skt_env_init on the
+ * skt_sounding_init_fields arrays. */
+int skt_env_init_sounding(skt_env* env, const skt_sounding_table* table);
 
 #ifdef __cplusplus
 }

@@ -241,6 +241,67 @@ private:
     }
 };
 
+// radioSondeData (Game/readTable.h:8-47) with owned columns.
+struct radioSondeData {
+    std::vector<float> altitude, pressure, dewPoint, temperature, windSpeed, windDir;
+    size_t dataSize = 0;
+    skt_sounding_table table() {
+        return skt_sounding_table{(int32_t)dataSize, (int32_t)dataSize, altitude.data(), pressure.data(),
+                                  dewPoint.data(), temperature.data(), windSpeed.data(), windDir.data()};
+    }
+};
+struct skewTInfo {
+    radioSondeData data;
+};
+
+// readTable (Game/readTable.h:49-88): radiosonde readers and the initial
+// state built from the profile. Errors throw skt::error (the reference
+// prints and returns without data).
+class readTable {
+public:
+    void readKNMIFile(const char* file) {
+        read([&](skt_sounding_table* t) { return skt_sounding_read_knmi(file, t); });
+    }
+    // The reference reads launch index 161 of the archive's metadata.
+    void readDWDFile(const char* file, int dateIndex = SKT_DWD_REFERENCE_DATE_INDEX) {
+        read([&](skt_sounding_table* t) { return skt_sounding_read_dwd(file, dateIndex, t); });
+    }
+    // initEnvironment (Game/readTable.cpp:364-474) into env.
+    void initEnvironment(environmentGPU& env) {
+        skt_sounding_table t = skewTData.data.table();
+        const int st = skt_env_init_sounding(env.handle(), &t);
+        if (st != SKT_OK) throw error(st, std::string("initEnvironment: ") + skt_env_last_error(env.handle()));
+    }
+    skewTInfo skewTData;
+
+private:
+    template <class F>
+    void read(F call) {
+        radioSondeData& d = skewTData.data;
+        size_t cap = 8192;
+        for (int attempt = 0; attempt < 2; attempt++) {
+            for (auto* c : {&d.altitude, &d.pressure, &d.dewPoint, &d.temperature, &d.windSpeed, &d.windDir})
+                c->assign(cap, 0.0f);
+            skt_sounding_table t{0, (int32_t)cap, d.altitude.data(), d.pressure.data(), d.dewPoint.data(),
+                                 d.temperature.data(), d.windSpeed.data(), d.windDir.data()};
+            const int st = call(&t);
+            if (st == SKT_ERR_ARG && (size_t)t.rows > cap) {
+                cap = (size_t)t.rows;
+                continue;
+            }
+            if (st != SKT_OK) {
+                d = radioSondeData{};
+                throw error(st, std::string("readTable: ") + skt_input_last_error());
+            }
+            for (auto* c : {&d.altitude, &d.pressure, &d.dewPoint, &d.temperature, &d.windSpeed, &d.windDir})
+                c->resize((size_t)t.rows);
+            d.dataSize = (size_t)t.rows;
+            return;
+        }
+        throw error(SKT_ERR_INPUT, "readTable: profile did not fit");
+    }
+};
+
 }  // namespace skt
 
 #endif  // SKT_ENVIRONMENT_HPP

@@ -38,6 +38,7 @@
 #include <limits>
 #include <vector>
 #include <string>
+#include <sstream>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1995,6 +1996,242 @@ void terrain(int seed, int width, int depth, int octaves, float maxHeight, float
             heights[x + z * width] = static_cast<int>(std::round(val * maxHeight));
         }
 }
+
+// -------------------------------------------------------- sounding input ---
+// readTable (Game/readTable.cpp), restated on the standard streams the
+// reference reads with, so getline/seekg/tellg/stof behave as there. A reader
+// returns 0 with the six columns, or -1 where the reference would crash, read
+// out of bounds or print an error and return without data.
+struct Sounding {
+    std::vector<float> alt, p, dew, T, ws, wd;
+};
+static bool stofOk(const std::string& w, float& v) {
+    try {
+        v = std::stof(w);
+        return true;
+    } catch (...) {
+        return false;
+    }
+}
+static int copyRows(Sounding& o, std::vector<float>* cols[6], int row) {
+    if (row <= 0) return -1;
+    for (int k = 0; k < 6; k++)
+        if ((int)cols[k]->size() < row) return -1;  // the reference's memcpy would over-read
+    for (int k = 0; k < 6; k++) cols[k]->resize((size_t)row);
+    o.alt = *cols[0]; o.p = *cols[1]; o.dew = *cols[2]; o.T = *cols[3]; o.ws = *cols[4]; o.wd = *cols[5];
+    return 0;
+}
+// readKNMIFile, Game/readTable.cpp:43-129 (text of SynchronizedSoundingData.xml).
+static int readKnmi(const std::string& text, Sounding& o) {
+    std::stringstream s(text);
+    std::string word, line;
+    int row = 0;
+    std::vector<float> pressure, temperature, dewPoint, windDir, windSpeed, altitude;
+    std::getline(s, line);
+    while (std::getline(s, line)) {
+        std::stringstream ss(line);
+        int count = 0;
+        while (ss >> word) {
+            float v;
+            auto field = [&](size_t skip) {  // &word[skip] then pop_back
+                if (word.size() <= skip) return false;
+                word = word.substr(skip);
+                word.pop_back();
+                return stofOk(word, v);
+            };
+            if (count >= 4 && count <= 8) {
+                const size_t skip[5] = {10, 13, 10, 9, 11};
+                if (!field(skip[count - 4])) return -1;
+                if (count == 4) pressure.push_back(v);
+                if (count == 5) temperature.push_back(v - 273.15f);
+                if (count == 6) {
+                    if ((size_t)row >= temperature.size()) return -1;
+                    dewPoint.push_back(temperature[row] - ((100 - v) * 0.2f) * ((temperature[row] + 273.15f) * 0.01f));
+                }
+                if (count == 7) windDir.push_back(v);
+                if (count == 8) windSpeed.push_back(v);
+            } else if (word.substr(0, 8) == "Altitude") {
+                if (!field(10)) return -1;
+                altitude.push_back(v);
+            }
+            count++;
+        }
+        row++;
+    }
+    row--;
+    std::vector<float>* cols[6] = {&altitude, &pressure, &dewPoint, &temperature, &windSpeed, &windDir};
+    return copyRows(o, cols, row);
+}
+// readDWDFile, Game/readTable.cpp:160-204: launch dates of the metadata member.
+static int dwdDates(const std::string& meta, std::vector<std::string>& all) {
+    std::stringstream s(meta);
+    std::string line;
+    std::getline(s, line);
+    while (std::getline(s, line)) {
+        std::stringstream ss(line);
+        std::string word;
+        int count = 0;
+        while (std::getline(ss, word, ';')) {
+            if (count == 1) {
+                if (word.size() < 2) return -1;
+                word.pop_back();
+                word.pop_back();
+                all.push_back(word);
+            } else if (count == 8) {
+                float g;
+                if (!stofOk(word, g)) return -1;
+                break;
+            }
+            count++;
+        }
+    }
+    return all.empty() ? -1 : 0;
+}
+// readDWDFile, Game/readTable.cpp:224-340: rows of one launch in the product member.
+static int dwdRows(const std::string& product, const std::string& targetDate, Sounding& o) {
+    std::stringstream s(product);
+    bool reachedDate = false;
+    std::string line;
+    while (!reachedDate) {
+        std::getline(s, line);
+        std::getline(s, line);
+        std::string word;
+        std::stringstream ss(line);
+        int count = 0;
+        while (std::getline(ss, word, ';')) {
+            if (count == 1) {
+                if (word == targetDate) {
+                    s.seekg(-100000, std::ios::cur);
+                    if (s.fail()) s.clear();
+                    reachedDate = true;
+                    std::getline(s, line);
+                } else {
+                    s.seekg(100000, std::ios::cur);
+                }
+                break;
+            }
+            count++;
+        }
+        if (s.tellg() == -1) return -1;
+    }
+    reachedDate = false;
+    std::string word;
+    int row = 0;
+    bool quit = false;
+    std::vector<float> pressure, temperature, dewPoint, windDir, windSpeed, altitude;
+    while (std::getline(s, line) && !quit) {
+        std::stringstream ss(line);
+        int count = 0;
+        while (std::getline(ss, word, ';')) {
+            if (!reachedDate) {
+                if (count == 1 && word == targetDate) {
+                    reachedDate = true;
+                    count--;
+                }
+                count++;
+            }
+            if (reachedDate) {
+                std::vector<float>* col = nullptr;
+                if (count == 1 && word != targetDate) {
+                    reachedDate = false;
+                    quit = true;
+                }
+                if (count == 6) col = &altitude;
+                if (count == 7) col = &pressure;
+                if (count == 8) col = &temperature;
+                if (count == 10) col = &dewPoint;
+                if (count == 11) col = &windSpeed;
+                if (count == 12) col = &windDir;
+                float v;
+                if (col) {
+                    if (!stofOk(word, v)) return -1;
+                    col->push_back(v);
+                }
+                count++;
+            }
+        }
+        if (reachedDate) row++;
+        if (s.tellg() == -1) return -1;
+    }
+    std::vector<float>* cols[6] = {&altitude, &pressure, &dewPoint, &temperature, &windSpeed, &windDir};
+    return copyRows(o, cols, row);
+}
+// lerpEnvValue, Game/readTable.cpp:356-362.
+static float lerpEnv(float H1, float H2, float HC, float V1, float V2) {
+    if (HC == H1) return V1;
+    if (HC == H2) return V2;
+    const float t = (HC - H1) / (H2 - H1);
+    return V1 + t * (V2 - V1);
+}
+// readTable::initEnvironment, Game/readTable.cpp:364-474, with the reference's
+// full-grid loops (including the in-place MAC average); potentialTemp and ws
+// are the Engine's (Engine/source/math/meteoformulas.cpp:19-23,184-187).
+static int initFields(const Sounding& d, int X, int Y, int Z, float dx, float* potTempOut, float* vel, float* Qv,
+                      float* groundTemp, float* groundPressure, float* pressures, float* pressuresSmall) {
+    const long N = (long)X * Y * Z;
+    const int rows = (int)d.alt.size();
+    auto getIndexAtHeight = [&](float h) {  // Game/readTable.cpp:506-519
+        for (int i = 0; i < rows; i++)
+            if (h <= d.alt[i]) return i;
+        return -1;
+    };
+    auto gidx = [&](int x, int y, int z) { return (long)x + (long)X * (y + (long)Y * z); };
+    std::vector<int> indices((size_t)Y);
+    std::vector<float> potTempSmall((size_t)Y, 0.0f);
+    std::fill(pressuresSmall, pressuresSmall + Y, 0.0f);
+    int j = 0;
+    for (float y = 0; y < Y * dx; y += dx) {
+        if (j >= Y) return -1;  // the reference would write past its level arrays
+        const float H0 = d.alt[0];
+        const int i = getIndexAtHeight(y + H0);
+        if (i < 0) return -1;  // the reference would index row -1
+        const int Pi = i == 0 ? 0 : i - 1;
+        pressuresSmall[j] = lerpEnv(d.alt[Pi] - H0, d.alt[i] - H0, y, d.p[Pi], d.p[i]);
+        potTempSmall[j] = lerpEnv(d.alt[Pi] - H0, d.alt[i] - H0, y, d.T[Pi], d.T[i]);
+        indices[j] = i;
+        j++;
+    }
+    if (j != Y) return -1;
+    for (int k = 0; k < Y; k++) potTempSmall[k] = potTemp<float>(potTempSmall[k], pressuresSmall[k], d.p[0]);
+    for (int z = 0; z < Z; z++)
+        for (int i = 0; i < Y; i++)
+            for (int x = 0; x < X; x++) potTempOut[gidx(x, i, z)] = static_cast<float>(potTempSmall[i] + 273.15f);
+    const float PI = K<float>::PI;
+    for (int i = 0; i < Y; i++) {
+        const float H0 = d.alt[0];
+        const int idx = indices[i];
+        const int Pidx = idx == 0 ? 0 : idx - 1;
+        const int Pi = i == 0 ? 0 : i - 1;
+        const float vX = std::sin((d.wd[idx] - 180.0f) * (PI / 180.0f)) * d.ws[idx];
+        const float vZ = std::cos((d.wd[idx] - 180.0f) * (PI / 180.0f)) * d.ws[idx];
+        const float psValue = lerpEnv(d.alt[Pidx] - H0, d.alt[idx] - H0, i * dx, pressuresSmall[Pi], pressuresSmall[i]);
+        const float QvValue = ws<float>(lerpEnv(d.alt[Pidx] - H0, d.alt[idx] - H0, i * dx, d.dew[Pidx], d.dew[idx]), psValue);
+        for (int z = 0; z < Z; z++)
+            for (int x = 0; x < X; x++) {
+                const long c = gidx(x, i, z);
+                vel[3 * c] = vX;
+                vel[3 * c + 1] = 0.0f;
+                vel[3 * c + 2] = vZ;
+                Qv[c] = QvValue;
+                pressures[c] = psValue;
+            }
+    }
+    for (int z = 0; z < Z; z++)
+        for (int y = 0; y < Y; y++)
+            for (int x = 0; x < X; x++) {
+                const int Nx = x + 1 >= X ? x : x + 1, Ny = y + 1 >= Y ? y : y + 1, Nz = z + 1 >= Z ? z : z + 1;
+                const long c = gidx(x, y, z);
+                vel[3 * c] = (vel[3 * c] + vel[3 * gidx(Nx, y, z)]) / 2.0f;
+                vel[3 * c + 1] = (vel[3 * c + 1] + vel[3 * gidx(x, Ny, z) + 1]) / 2.0f;
+                vel[3 * c + 2] = (vel[3 * c + 2] + vel[3 * gidx(x, y, Nz) + 2]) / 2.0f;
+            }
+    for (long g = 0; g < (long)X * Z; g++) {
+        groundTemp[g] = d.T[0] + 273.15f;
+        groundPressure[g] = d.p[0];
+    }
+    (void)N;
+    return 0;
+}
 }  // namespace sko
 
 // ---------------------------------------------------------------- C API ---
@@ -2256,6 +2493,36 @@ void sko_region_get(void* hv, double* out) {
     DISPATCH(h, { for (int k = 0; k < 33; k++) out[k] = e.rsum[k]; });
 }
 // Thermo entry points for pinning against the reference's CPU meteoformulas.
+static int soundingOut(const Sounding& d, int cap, int* rows, float* out) {
+    *rows = (int)d.alt.size();
+    if (*rows > cap) return -2;
+    const std::vector<float>* c[6] = {&d.alt, &d.p, &d.dew, &d.T, &d.ws, &d.wd};
+    for (int k = 0; k < 6; k++) std::copy(c[k]->begin(), c[k]->end(), out + (size_t)k * cap);
+    return 0;
+}
+// out: six columns of cap floats (altitude, pressure, dew point, temperature, wind speed, wind dir).
+int sko_sounding_knmi(const char* text, long len, int cap, int* rows, float* out) {
+    Sounding d;
+    if (readKnmi(std::string(text, (size_t)len), d)) return -1;
+    return soundingOut(d, cap, rows, out);
+}
+int sko_sounding_dwd(const char* meta, long mlen, const char* prod, long plen, int date_index, int cap, int* rows,
+                     float* out) {
+    std::vector<std::string> dates;
+    if (dwdDates(std::string(meta, (size_t)mlen), dates)) return -1;
+    if (date_index < 0 || date_index >= (int)dates.size()) return -1;
+    Sounding d;
+    if (dwdRows(std::string(prod, (size_t)plen), dates[(size_t)date_index] + "12", d)) return -1;
+    return soundingOut(d, cap, rows, out);
+}
+int sko_sounding_init(int rows, const float* cols, int X, int Y, int Z, float dx, float* th, float* vel, float* qv,
+                      float* gT, float* gP, float* p, float* pS) {
+    Sounding d;
+    std::vector<float>* c[6] = {&d.alt, &d.p, &d.dew, &d.T, &d.ws, &d.wd};
+    for (int k = 0; k < 6; k++) c[k]->assign(cols + (size_t)k * rows, cols + (size_t)(k + 1) * rows);
+    if (rows < 1) return -1;
+    return initFields(d, X, Y, Z, dx, th, vel, qv, gT, gP, p, pS);
+}
 float sko_es(float T) { return es<float>(T); }
 float sko_ei(float T) { return ei<float>(T); }
 float sko_ws(float T, float P) { return ws<float>(T, P); }

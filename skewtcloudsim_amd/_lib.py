@@ -69,6 +69,12 @@ TIMING_CLASSES = ["diffusion_sweep", "ppm_pass", "pcg_spmv", "pcg_update", "micr
                   "other"]
 
 
+# skt_sounding_table columns, in declaration order (radioSondeData, Game/readTable.h:8-15)
+SOUNDING_COLUMNS = ["altitude", "pressure", "dew_point", "temperature", "wind_speed", "wind_dir"]
+DWD_REFERENCE_DATE_INDEX = 161
+SKT_ERR_ARG = 1
+
+
 class SoundingSpec(ctypes.Structure):
     _fields_ = [
         ("surface_theta", ctypes.c_float), ("surface_pressure", ctypes.c_float), ("lapse_rate", ctypes.c_float),
@@ -77,6 +83,11 @@ class SoundingSpec(ctypes.Structure):
         ("n_bubbles", ctypes.c_int32), ("bubble_amp", ctypes.c_float * 8), ("bubble_radius", ctypes.c_float * 8),
         ("bubble_x", ctypes.c_float * 8), ("bubble_y", ctypes.c_float * 8), ("bubble_z", ctypes.c_float * 8),
     ]
+
+
+class SoundingTable(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int32), ("capacity", ctypes.c_int32)] + [
+        (n, ctypes.POINTER(ctypes.c_float)) for n in SOUNDING_COLUMNS]
 
 
 _P = ctypes.c_void_p
@@ -133,6 +144,14 @@ SIGNATURES = {
     "skt_device_synchronize": (_I, [_I]),
     "skt_sounding_spec_random": (_I, [ctypes.POINTER(SoundingSpec), ctypes.c_uint64, _I, _I, _I, _F, _I]),
     "skt_sounding_generate": (_I, [ctypes.POINTER(SoundingSpec), _I, _I, _I, _F, _P, _P, _P, _P, _P, _P, _P]),
+    "skt_input_last_error": (ctypes.c_char_p, []),
+    "skt_sounding_read_knmi": (_I, [ctypes.c_char_p, ctypes.POINTER(SoundingTable)]),
+    "skt_sounding_parse_knmi": (_I, [ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(SoundingTable)]),
+    "skt_sounding_read_dwd": (_I, [ctypes.c_char_p, _I, ctypes.POINTER(SoundingTable)]),
+    "skt_sounding_parse_dwd": (_I, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64, _I,
+                                    ctypes.POINTER(SoundingTable)]),
+    "skt_sounding_init_fields": (_I, [ctypes.POINTER(SoundingTable), _I, _I, _I, _F, _P, _P, _P, _P, _P, _P, _P]),
+    "skt_env_init_sounding": (_I, [_P, ctypes.POINTER(SoundingTable)]),
 }
 
 _lib = None

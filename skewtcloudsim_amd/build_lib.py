@@ -9,7 +9,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["skt_kernels.hip", "skt_misc.hip", "skt_api.cpp", "skt_sounding.cpp"]
+SOURCES = ["skt_kernels.hip", "skt_misc.hip", "skt_api.cpp", "skt_sounding.cpp", "skt_ingest.cpp"]
+LIBS = ["-lz"]  # zip members of the radiosonde archives are deflated
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # two roundings for a*b+c, as the CPU restatement is compiled
@@ -30,7 +31,7 @@ def hipcc():
 def build(verbose=False):
     out = os.path.join(HERE, "libskt.so")
     tmp = out + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES] + LIBS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include "skt_ingest.h"
 #include "skt_internal.h"
 
 using namespace skt;
@@ -545,6 +546,27 @@ int skt_env_get_ground_height(skt_env* env, int32_t* gh) {
     return SKT_OK;
 }
 
+// Steps of environmentGPU::init (Game/environment.cu:249-389) after the
+// initial fields are on the device.
+static int finishInit(EnvState& e) {
+    HIPCHK(e, launch_init_density(e, 1));
+    HIPCHK(e, launch_profiles_from_column(e));
+    if (e.cfg.init_noise) {
+        const uint32_t s = e.cfg.noise_seed;
+        HIPCHK(e, launch_random(e, e.f[SKT_POTTEMP], -0.1f, 0.1f, s + 0));
+        HIPCHK(e, launch_random(e, e.f[SKT_QV], 0.0f, 0.0001f, s + 1));
+        HIPCHK(e, launch_random(e, e.f[SKT_VELX], 0.5f, 0.5f, s + 2));
+        HIPCHK(e, launch_random(e, e.f[SKT_VELY], -0.2f, 0.2f, s + 3));
+        HIPCHK(e, launch_random(e, e.f[SKT_VELZ], 0.5f, 0.5f, s + 4));
+    }
+    HIPCHK(e, launch_editor_reset(e, 10));
+    HIPCHK(e, launch_isen_temp_ground(e));
+    HIPCHK(e, hipStreamSynchronize(e.stream));
+    e.params = skt_env_params{43200.0f, 130, 52.37f, 1.0f, 0, 1};
+    e.initialized = true;
+    return SKT_OK;
+}
+
 int skt_env_init(skt_env* env, const float* th, const float* vel, const float* qv, float* gT, const float* gP,
                  const float* p, const float* pS) {
     if (!env || !th || !vel || !qv || !gT || !gP || !p || !pS) return SKT_ERR_ARG;
@@ -566,22 +588,33 @@ int skt_env_init(skt_env* env, const float* th, const float* vel, const float* q
     RC(upload(SKT_GROUND_P, gP));
     RC(upload(SKT_DEFAULT_PRESSURE, pS));
     RC(upload(SKT_PRESSURE, p));
-    HIPCHK(e, launch_init_density(e, 1));
-    HIPCHK(e, launch_profiles_from_column(e));
-    if (e.cfg.init_noise) {
-        const uint32_t s = e.cfg.noise_seed;
-        HIPCHK(e, launch_random(e, e.f[SKT_POTTEMP], -0.1f, 0.1f, s + 0));
-        HIPCHK(e, launch_random(e, e.f[SKT_QV], 0.0f, 0.0001f, s + 1));
-        HIPCHK(e, launch_random(e, e.f[SKT_VELX], 0.5f, 0.5f, s + 2));
-        HIPCHK(e, launch_random(e, e.f[SKT_VELY], -0.2f, 0.2f, s + 3));
-        HIPCHK(e, launch_random(e, e.f[SKT_VELZ], 0.5f, 0.5f, s + 4));
+    return finishInit(e);
+}
+
+int skt_env_init_sounding(skt_env* env, const skt_sounding_table* table) {
+    if (!env) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    skt::SoundingLevels L;
+    if (int rc = skt::soundingLevels(table, e.Y, e.dx, L)) return setErr(e, rc, skt_input_last_error());
+    HIPCHK(e, hipSetDevice(e.device));
+    // theta, u, w, qv, p per level, staged in a work array (5*ny <= nx*ny*nz
+    // floats for every grid with nx*nz >= 5).
+    const size_t Y = (size_t)e.Y;
+    if (5 * Y > (size_t)std::max(e.N, 2 * e.G)) return setErr(e, SKT_ERR_ARG, "grid too small for the sounding broadcast");
+    std::vector<float> lv(5 * Y);
+    for (size_t y = 0; y < Y; y++) {
+        lv[y] = L.theta[y];
+        lv[Y + y] = L.velX[y];
+        lv[2 * Y + y] = L.velZ[y];
+        lv[3 * Y + y] = L.qv[y];
+        lv[4 * Y + y] = L.p[y];
     }
-    HIPCHK(e, launch_editor_reset(e, 10));
-    HIPCHK(e, launch_isen_temp_ground(e));
+    HIPCHK(e, hipMemcpyAsync(e.work[0], lv.data(), sizeof(float) * lv.size(), hipMemcpyHostToDevice, e.stream));
+    HIPCHK(e, hipMemcpyAsync(e.f[SKT_DEFAULT_PRESSURE], L.pSmall.data(), sizeof(float) * Y, hipMemcpyHostToDevice,
+                             e.stream));
+    HIPCHK(e, launch_sounding_fill(e, e.work[0], L.groundT, L.groundP));
     HIPCHK(e, hipStreamSynchronize(e.stream));
-    e.params = skt_env_params{43200.0f, 130, 52.37f, 1.0f, 0, 1};
-    e.initialized = true;
-    return SKT_OK;
+    return finishInit(e);
 }
 
 int skt_env_update(skt_env* env, float dt, float speed) {

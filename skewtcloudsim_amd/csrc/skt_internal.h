@@ -196,6 +196,7 @@ hipError_t launch_masks(EnvState& e, uint8_t* dev_out);
 hipError_t launch_editor_reset(EnvState& e, int nfields);
 hipError_t launch_isen_temp_ground(EnvState& e);
 hipError_t launch_profiles_from_column(EnvState& e);
+hipError_t launch_sounding_fill(EnvState& e, const float* levels, float groundT, float groundP);
 hipError_t launch_random(EnvState& e, float* a, float mn, float mx, uint32_t seed);
 hipError_t launch_stream_copy(EnvState& e, const float* src, float* dst, long n);
 int blocks3d(const EnvState& e);

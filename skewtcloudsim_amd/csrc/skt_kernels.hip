@@ -213,7 +213,7 @@ KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const flo
     float vb[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) vb[f] = bldf(rV[f], ob, pOff(zb));
-    int ghb = bldi(rG, gb, gOff(zb)), ghc = bldi(rG, gb, gOff(z0));
+    int ghc = bldi(rG, gb, gOff(z0));
     Own o0{}, o1{}, o2{}, o3{};
     Halo h0{}, h1{};
     // prefetches past the chunk read the next chunk's first planes: measured
@@ -256,7 +256,6 @@ KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const flo
         }
 #pragma unroll
         for (int f = 0; f < NF; f++) vb[f] = oc.v[f];
-        ghb = ghc;
         ghc = gf;
     };
     for (int zg = z0; zg < z1; zg += 4) {

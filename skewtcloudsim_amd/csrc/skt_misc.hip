@@ -680,6 +680,34 @@ hipError_t launch_random(EnvState& e, float* a, float mn, float mx, uint32_t see
     return hipGetLastError();
 }
 
+// Broadcast of a sounding's per-level initial state over every cell
+// (Game/readTable.cpp:404-441,467-471). lv holds ny values each of theta,
+// face u, face w, qv and pressure; the vertical face wind is 0.
+KL void k_sounding_fill(long N, int X, int Y, long G, const float* __restrict__ lv, float* th, float* u, float* v,
+                        float* w, float* qv, float* p, float* gT, float* gP, float groundT, float groundP) {
+    const long stride = (long)gridDim.x * 256;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < N; i += stride) {
+        const int y = (int)((i / X) % Y);
+        th[i] = lv[y];
+        u[i] = lv[Y + y];
+        v[i] = 0.0f;
+        w[i] = lv[2 * Y + y];
+        qv[i] = lv[3 * Y + y];
+        p[i] = lv[4 * Y + y];
+    }
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < G; i += stride) {
+        gT[i] = groundT;
+        gP[i] = groundP;
+    }
+}
+hipError_t launch_sounding_fill(EnvState& e, const float* lv, float groundT, float groundP) {
+    hipLaunchKernelGGL(k_sounding_fill, dim3((unsigned)std::min<long>((e.N + 255) / 256, 16384)), dim3(256), 0,
+                       e.stream, e.N, e.X, e.Y, e.G, lv, e.f[SKT_POTTEMP], e.f[SKT_VELX], e.f[SKT_VELY],
+                       e.f[SKT_VELZ], e.f[SKT_QV], e.f[SKT_PRESSURE], e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], groundT,
+                       groundP);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------ serving diagnostics ---
 // No reference counterpart: these replace the editor's full-state readback
 // after every step (Game/editor.cpp:226-269) with device reductions. Per

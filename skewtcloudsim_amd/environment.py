@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import FIELD, Bounds, Config, Params, SoundingSpec, Stats
+from ._lib import FIELD, SOUNDING_COLUMNS, Bounds, Config, Params, SoundingSpec, SoundingTable, Stats
 
 
 class SktError(RuntimeError):
@@ -65,6 +65,115 @@ def make_sounding(nx, ny, nz, dx=100.0, seed=None, n_bubbles=0):
     if rc:
         raise SktError(f"skt_sounding_generate failed ({rc})")
     return out
+
+
+class _Sounding:
+    """Column arrays of one radiosonde profile (radioSondeData, Game/readTable.h:8-47)."""
+
+    def __init__(self, rows=0):
+        for n in SOUNDING_COLUMNS:
+            setattr(self, n, np.zeros(rows, np.float32))
+
+    @property
+    def dataSize(self):
+        return len(self.altitude)
+
+    def table(self):
+        """An skt_sounding_table over these arrays (kept alive by self)."""
+        cols = [np.ascontiguousarray(getattr(self, n), np.float32) for n in SOUNDING_COLUMNS]
+        rows = len(cols[0])
+        if any(len(c) != rows for c in cols):
+            raise SktError("sounding columns differ in length")
+        for n, c in zip(SOUNDING_COLUMNS, cols):
+            setattr(self, n, c)
+        t = SoundingTable(rows, rows, *[c.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) for c in cols])
+        return t
+
+
+def _read_table(call):
+    """Runs a reader taking an skt_sounding_table, growing the arrays once if needed."""
+    lib = _lib.load_library()
+    cap = 8192
+    for _ in range(2):
+        d = _Sounding(cap)
+        t = SoundingTable(0, cap, *[getattr(d, n).ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+                                     for n in SOUNDING_COLUMNS])
+        rc = call(lib, t)
+        if rc == _lib.SKT_ERR_ARG and t.rows > cap:
+            cap = t.rows
+            continue
+        if rc:
+            raise SktError(f"sounding input: status {rc}: {lib.skt_input_last_error().decode()}")
+        for n in SOUNDING_COLUMNS:
+            setattr(d, n, getattr(d, n)[:t.rows].copy())
+        return d
+    raise SktError("sounding input: table did not fit after growing")
+
+
+class ReadTable:
+    """Host mirror of readTable (Game/readTable.h:49-88): reads a radiosonde
+    profile into .skewTData.data and builds the initial state from it.
+    Readers raise SktError where the reference would print an error and
+    return (or crash)."""
+
+    def __init__(self):
+        self.data = _Sounding(0)
+
+    @property
+    def skewTData(self):
+        return self
+
+    def readKNMIFile(self, path):
+        """readKNMIFile (Game/readTable.cpp:28-129): a KNMI .mwx archive."""
+        self.data = _read_table(lambda lib, t: lib.skt_sounding_read_knmi(str(path).encode(), ctypes.byref(t)))
+
+    def readDWDFile(self, path, date_index=_lib.DWD_REFERENCE_DATE_INDEX):
+        """readDWDFile (Game/readTable.cpp:134-354): a DWD sekundenwerte_aero
+        zip; the reference reads launch index 161 (the default here)."""
+        self.data = _read_table(
+            lambda lib, t: lib.skt_sounding_read_dwd(str(path).encode(), int(date_index), ctypes.byref(t)))
+
+    def parseKNMI(self, text):
+        """The KNMI row parser on the text of SynchronizedSoundingData.xml."""
+        b = text.encode() if isinstance(text, str) else bytes(text)
+        self.data = _read_table(lambda lib, t: lib.skt_sounding_parse_knmi(b, len(b), ctypes.byref(t)))
+
+    def parseDWD(self, metadata, product, date_index=_lib.DWD_REFERENCE_DATE_INDEX):
+        """The DWD metadata and product parsers on the member texts."""
+        m = metadata.encode() if isinstance(metadata, str) else bytes(metadata)
+        p = product.encode() if isinstance(product, str) else bytes(product)
+        self.data = _read_table(
+            lambda lib, t: lib.skt_sounding_parse_dwd(m, len(m), p, len(p), int(date_index), ctypes.byref(t)))
+
+    def set_data(self, **cols):
+        """Sets the profile from column arrays (altitude, pressure, dew_point,
+        temperature, wind_speed, wind_dir)."""
+        d = _Sounding(0)
+        for n in SOUNDING_COLUMNS:
+            setattr(d, n, np.ascontiguousarray(cols[n], np.float32))
+        d.table()  # length check
+        self.data = d
+
+    def init_fields(self, nx, ny, nz, dx=100.0):
+        """initEnvironment's init() arrays on the host (skt_sounding_init_fields)."""
+        lib = _lib.load_library()
+        n, g = nx * ny * nz, nx * nz
+        out = dict(potTemps=np.empty(n, np.float32), velField=np.empty(3 * n, np.float32),
+                   Qv=np.empty(n, np.float32), groundTemp=np.empty(g, np.float32),
+                   groundPres=np.empty(g, np.float32), pressures=np.empty(n, np.float32),
+                   smallPressure=np.empty(ny, np.float32))
+        t = self.data.table()
+        rc = lib.skt_sounding_init_fields(ctypes.byref(t), nx, ny, nz, dx, *[_ptr(out[k]) for k in (
+            "potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures", "smallPressure")])
+        if rc:
+            raise SktError(f"skt_sounding_init_fields: status {rc}: {lib.skt_input_last_error().decode()}")
+        return out
+
+    def initEnvironment(self, env):
+        """initEnvironment (Game/readTable.cpp:364-474) into env: per-level
+        values on the host, broadcast over the grid on the device."""
+        t = self.data.table()
+        env._chk(env.lib.skt_env_init_sounding(env.h, ctypes.byref(t)), "initEnvironment")
 
 
 class EnvironmentGPU:

@@ -13,7 +13,40 @@ struct Vec3 {
     float x, y, z;
 };
 
-int main() {
+// readTable through the facade: the DWD fixture read, initEnvironment on the
+// device, against skt_env_init on the skt_sounding_init_fields arrays.
+static int soundingCheck(const char* dwdPath) {
+    const int nx = 16, ny = 12, nz = 8;
+    const float dx = 100.0f;
+    const size_t N = (size_t)nx * ny * nz, G = (size_t)nx * nz;
+    skt::readTable reader;
+    reader.readDWDFile(dwdPath);
+    if (reader.skewTData.data.dataSize == 0) return 1;
+    skt::environmentGPU env(nx, ny, nz, dx);
+    reader.initEnvironment(env);
+    std::vector<float> th(N), vel(3 * N), qv(N), gT(G), gP(G), p(N), ps(ny);
+    skt_sounding_table t = reader.skewTData.data.table();
+    if (skt_sounding_init_fields(&t, nx, ny, nz, dx, th.data(), vel.data(), qv.data(), gT.data(), gP.data(), p.data(),
+                                 ps.data()) != SKT_OK)
+        return 1;
+    skt::environmentGPU ref(nx, ny, nz, dx);
+    ref.init(th.data(), reinterpret_cast<Vec3*>(vel.data()), qv.data(), gT.data(), gP.data(), p.data(), ps.data());
+    int bad = 0;
+    for (int f = 0; f < SKT_FIELD_COUNT; f++) {
+        std::vector<float> a = env.copyToHost((skt_field)f), b = ref.copyToHost((skt_field)f);
+        if (a.size() != b.size() || std::memcmp(a.data(), b.data(), a.size() * 4) != 0) bad++;
+    }
+    bool threw = false;
+    try {
+        reader.readKNMIFile("/nonexistent/file.mwx");
+    } catch (const skt::error& e) {
+        threw = e.status == SKT_ERR_INPUT;
+    }
+    if (!threw) bad++;
+    return bad;
+}
+
+int main(int argc, char** argv) {
     const int nx = 24, ny = 20, nz = 16;
     const float dx = 100.0f;
     const size_t N = (size_t)nx * ny * nz, G = (size_t)nx * nz;
@@ -64,6 +97,7 @@ int main() {
     if (!threw) bad++;
     env.synchronize();
     skt_env_destroy(raw);
+    if (argc > 1) bad += soundingCheck(argv[1]);
     if (bad) {
         std::printf("facade mismatch: %d\n", bad);
         return 1;

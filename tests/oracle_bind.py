@@ -81,8 +81,55 @@ def lib():
         L.sko_ppm_flux.argtypes = [f] * 6
         L.sko_ppm_flux64.restype = ctypes.c_double
         L.sko_ppm_flux64.argtypes = [ctypes.c_double] * 6
+        cp, lg = ctypes.c_char_p, ctypes.c_long
+        L.sko_sounding_knmi.restype = i
+        L.sko_sounding_knmi.argtypes = [cp, lg, i, vp, vp]
+        L.sko_sounding_dwd.restype = i
+        L.sko_sounding_dwd.argtypes = [cp, lg, cp, lg, i, i, vp, vp]
+        L.sko_sounding_init.restype = i
+        L.sko_sounding_init.argtypes = [i, vp, i, i, i, f] + [vp] * 7
         _lib = L
     return _lib
+
+
+SOUNDING_COLUMNS = ["altitude", "pressure", "dew_point", "temperature", "wind_speed", "wind_dir"]
+
+
+def _sounding_out(call, cap=8192):
+    rows = np.zeros(1, np.int32)
+    out = np.zeros((6, cap), np.float32)
+    rc = call(cap, _p(rows), _p(out))
+    if rc == -2:
+        return _sounding_out(call, int(rows[0]))
+    if rc:
+        return None
+    return {n: out[k, :rows[0]].copy() for k, n in enumerate(SOUNDING_COLUMNS)}
+
+
+def sounding_knmi(text):
+    """readKNMIFile's parser on the XML text: dict of columns, or None where the reference fails."""
+    b = text.encode() if isinstance(text, str) else bytes(text)
+    return _sounding_out(lambda cap, r, o: lib().sko_sounding_knmi(b, len(b), cap, r, o))
+
+
+def sounding_dwd(meta, product, date_index=161):
+    """readDWDFile's parsers on the member texts: dict of columns, or None where the reference fails."""
+    m = meta.encode() if isinstance(meta, str) else bytes(meta)
+    p = product.encode() if isinstance(product, str) else bytes(product)
+    return _sounding_out(lambda cap, r, o: lib().sko_sounding_dwd(m, len(m), p, len(p), date_index, cap, r, o))
+
+
+def sounding_init(cols, nx, ny, nz, dx):
+    """initEnvironment's init() arrays from sounding columns, or None where the reference fails."""
+    rows = len(cols["altitude"])
+    c = np.ascontiguousarray(np.stack([np.asarray(cols[n], np.float32) for n in SOUNDING_COLUMNS]))
+    n, g = nx * ny * nz, nx * nz
+    out = dict(potTemps=np.zeros(n, np.float32), velField=np.zeros(3 * n, np.float32), Qv=np.zeros(n, np.float32),
+               groundTemp=np.zeros(g, np.float32), groundPres=np.zeros(g, np.float32),
+               pressures=np.zeros(n, np.float32), smallPressure=np.zeros(ny, np.float32))
+    rc = lib().sko_sounding_init(rows, _p(c), nx, ny, nz, dx, *[_p(out[k]) for k in (
+        "potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures", "smallPressure")])
+    return None if rc else out
 
 
 def ref_lib():

@@ -4,7 +4,9 @@ libskt.so the way a C++ caller of the reference's environmentGPU would use it.
 CPU: the facade and tests/cpp/facade_step.cpp compile and link (every symbol
 they use resolves in libskt.so). GPU: the program steps one environment
 through the facade and one through the raw C ABI; the sky fields must agree
-bit for bit, and the facade must reject foreign device arrays.
+bit for bit, and the facade must reject foreign device arrays. Its readTable
+reads the DWD fixture and initialises an environment on the device, which
+must equal skt_env_init on the skt_sounding_init_fields arrays.
 """
 import os
 import shutil
@@ -38,5 +40,6 @@ def test_facade_compiles_and_links(tmp_path):
 @pytest.mark.gpu
 def test_facade_steps_like_the_c_abi(tmp_path):
     exe = _build(tmp_path)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    dwd = os.path.join(ROOT, "tests", "golden", "sekundenwerte_aero_01303_fixture.zip")
+    r = subprocess.run([exe, dwd], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "facade ok" in r.stdout, r.stdout + r.stderr

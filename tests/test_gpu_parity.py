@@ -542,3 +542,46 @@ def test_editor_edits():
     assert_fields(env, orc, SKY, m, l2=1e-5, maxabs=5e-4, tag="edit-step")
     env.close()
     orc.close()
+
+
+@pytest.mark.parametrize("source", ["knmi", "dwd"])
+def test_sounding_initial_state_and_steps(source):
+    """initEnvironment from the radiosonde fixtures: the device broadcast
+    (skt_env_init_sounding) equals skt_env_init on the host arrays bit for bit,
+    and the state steps like the oracle initialised from its own restatement."""
+    import os
+    import sys
+    import skewtcloudsim_amd as skt
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_soundings as gen
+    nx, ny, nz, dx = 24, 20, 16, 100.0
+    r = skt.ReadTable()
+    if source == "knmi":
+        r.readKNMIFile(gen.KNMI_PATH)
+    else:
+        r.readDWDFile(gen.DWD_PATH)
+    print(f"sounding {source}: {r.data.dataSize} rows, grid {nx}x{ny}x{nz}", flush=True)
+    env = skt.EnvironmentGPU(nx, ny, nz, dx)
+    r.initEnvironment(env)
+    host = r.init_fields(nx, ny, nz, dx)
+    ref = skt.EnvironmentGPU(nx, ny, nz, dx)
+    ref.init(*[host[k] for k in KEYS])
+    for f in ob.FIELDS:
+        assert np.array_equal(env.get(f), ref.get(f), equal_nan=True), f
+    ref.close()
+    cols = {n: getattr(r.data, n) for n in ob.SOUNDING_COLUMNS}
+    want = ob.sounding_init(cols, nx, ny, nz, dx)
+    for k in KEYS:
+        assert np.array_equal(host[k], want[k]), k
+    orc = ob.Oracle(nx, ny, nz, dx)
+    orc.init(*[want[k] for k in KEYS])
+    m = sky_mask(nx, ny, nz)
+    assert_fields(env, orc, SKY + GROUND, m, l2=1e-7, maxabs=1e-6, tag="sounding init")
+    for s in range(3):
+        env.updateGPU(0.5, 1.0)
+        orc.step(0.5, 1.0)
+        assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+        tol = 1e-5 if s == 0 else 1e-4
+        assert_fields(env, orc, SKY + GROUND, m, l2=tol, maxabs=50 * tol, tag=f"sounding step{s}")
+    env.close()
+    orc.close()
