@@ -14,6 +14,10 @@ residual of 1e-6 (the solver the workload description names), and the CPU
 baseline (the oracle, one step of the same workload on the host cores), whose
 result is compared with one GPU step from the same initial state.
 
+Also timed (key ingestion_diagnostics_terrain, off with --no-rows): reading
+the DWD fixture archive, initialising the grid from it on the device against
+the host-array init, the serving-diagnostics readback and terrain generation.
+
 Prints one JSON line: cell-steps/s (value), the roofline of the dominant
 kernel class (algorithmic bytes per launch / average launch duration from HIP
 events on the env's stream during the timed region; only that class carries
@@ -123,6 +127,57 @@ def run_standard(snd_args, steps, warmup):
             "pcg": "textbook Jacobi-PCG, relative residual 1e-6, max 1000", "pcg_iterations": ks}
 
 
+def measure_next_rows(reps=20):
+    """Wall times at the bench grid of the paths around the step: sounding
+    ingestion (DWD fixture archive -> levels on the host -> device broadcast,
+    against the host-array init over PCIe), the serving diagnostics readback
+    and terrain generation. Kernel times of the device parts come from
+    rocprofv3 (k_sounding_fill, k_diag, k_terrain)."""
+    import numpy as np
+    import skewtcloudsim_amd as skt
+    fixture = os.path.join(ROOT, "tests", "golden", "sekundenwerte_aero_01303_fixture.zip")
+    out = {}
+    r = skt.ReadTable()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r.readDWDFile(fixture)
+    out["dwd_read_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    out["dwd_rows"] = int(r.data.dataSize)
+    env = skt.EnvironmentGPU(NX, NY, NZ, DX)
+    r.initEnvironment(env)  # first call: allocations and code paging outside the timing
+    env.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r.initEnvironment(env)
+    env.device_synchronize()
+    out["init_sounding_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    dev = {f: env.get(f) for f in ("POTTEMP", "VELX", "QV", "PRESSURE")}
+    t0 = time.perf_counter()
+    host = r.init_fields(NX, NY, NZ, DX)
+    out["init_fields_host_ms"] = (time.perf_counter() - t0) * 1e3
+    keys = ("potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures", "smallPressure")
+    t0 = time.perf_counter()
+    env.init(*[host[k] for k in keys])
+    env.device_synchronize()
+    out["init_host_arrays_ms"] = (time.perf_counter() - t0) * 1e3
+    out["device_equals_host_path"] = all(np.array_equal(dev[f], env.get(f)) for f in dev)
+    env.updateGPU(DT, 1.0)
+    env.diagnostics()
+    env.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d = env.diagnostics()
+    out["diagnostics_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    out["diagnostics"] = d
+    env.close()
+    skt.terrain(NX, NZ, seed=100, octaves=12, max_height=6.0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        skt.terrain(NX, NZ, seed=100, octaves=12, max_height=6.0)
+    out["terrain_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +186,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-standard", action="store_true", help="skip the relative-1e-6 PCG measurement")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-rows", action="store_true", help="skip the ingestion/diagnostics/terrain timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the step is single-domain, one GPU per "
@@ -182,6 +238,7 @@ def main():
     traffic = traffic_from_profile(dom)
 
     standard = None if args.no_standard else run_standard(snd_args, min(args.steps, 5), 2)
+    rows = None if args.no_rows else measure_next_rows()
 
     cpu = None
     parity = None
@@ -221,6 +278,7 @@ def main():
         "copy_probe": {"GBps": copy_bytes / (copy_ms / 1e3) / 1e9, "bytes": copy_bytes, "ms": copy_ms},
         "cpu_baseline": cpu,
         "full_size_parity_one_step": parity,
+        "ingestion_diagnostics_terrain": rows,
     }
     env.close()
     print(json.dumps(out))
