@@ -234,6 +234,11 @@ typedef struct skt_timing {
     double total_ms[8];
 } skt_timing;
 int skt_env_set_timing(skt_env* env, int32_t class_mask); /* also clears the accumulators */
+/* Time only every stride-th launch of each enabled class (1, the default:
+ * every launch); launches and total_ms then cover the timed launches. Each
+ * timed launch costs about 3 us of device idle time, so a sample keeps the
+ * timing's cost on a step small. */
+int skt_env_set_timing_stride(skt_env* env, int32_t stride);
 int skt_env_get_timing(skt_env* env, skt_timing* out);
 /* Times a plain device copy of one sky field (nx*ny*nz floats) 'repeats'
  * times: the measured copy ceiling for roofline reporting. */

@@ -139,6 +139,7 @@ SIGNATURES = {
     "skt_mphys_region_clear": (_I, [_P]),
     "skt_mphys_region_get": (_I, [_P, _P]),
     "skt_env_set_timing": (_I, [_P, _I]),
+    "skt_env_set_timing_stride": (_I, [_P, _I]),
     "skt_env_get_timing": (_I, [_P, ctypes.POINTER(Timing)]),
     "skt_env_copy_probe": (_I, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(ctypes.c_double)]),
     "skt_device_synchronize": (_I, [_I]),

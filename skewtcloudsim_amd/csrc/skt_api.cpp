@@ -46,20 +46,37 @@ hipEvent_t evGet(EnvState& e) {
     return ev;
 }
 
-// Launch bracketed by HIP events on the env's stream when timing is enabled.
-#define TL(e, cls, call)                                                  \
-    do {                                                                  \
-        hipEvent_t a_ = nullptr, b_ = nullptr;                            \
-        if ((e).timing & (1 << (cls))) {                                  \
-            a_ = evGet(e);                                                \
-            b_ = evGet(e);                                                \
-            if (a_) (void)hipEventRecord(a_, (e).stream);                 \
-        }                                                                 \
-        HIPCHK(e, call);                                                  \
-        if ((e).timing && a_ && b_) {                                     \
-            (void)hipEventRecord(b_, (e).stream);                         \
-            (e).pending.push_back(EnvState::TimedLaunch{cls, a_, b_});    \
-        }                                                                 \
+// Launch timed with HIP events on the env's stream when its class is
+// enabled (every t_stride-th launch of the class): attached to the kernel's
+// dispatch for the single-kernel classes (kTimedOnDispatch), else recorded
+// around the call.
+#define TL(e, cls, call)                                                          \
+    do {                                                                          \
+        hipEvent_t a_ = nullptr, b_ = nullptr;                                    \
+        const bool on_ = ((e).timing & (1 << (cls))) != 0 &&                      \
+                         (e).t_seen[cls]++ % (e).t_stride == 0;                    \
+        if (on_) {                                                                \
+            a_ = evGet(e);                                                        \
+            b_ = evGet(e);                                                        \
+        }                                                                         \
+        const bool ok_ = a_ && b_;                                                \
+        if (ok_ && kTimedOnDispatch[cls]) {                                       \
+            (e).kevA = a_;                                                        \
+            (e).kevB = b_;                                                        \
+        } else if (ok_) {                                                         \
+            (void)hipEventRecord(a_, (e).stream);                                 \
+        }                                                                         \
+        const hipError_t r_ = (call);                                             \
+        const bool armedLeft_ = (e).kevA != nullptr;                              \
+        (e).kevA = (e).kevB = nullptr;                                            \
+        if (r_ != hipSuccess) return setErr((e), SKT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(r_)); \
+        if (ok_ && !kTimedOnDispatch[cls]) (void)hipEventRecord(b_, (e).stream); \
+        if (ok_ && armedLeft_) return setErr((e), SKT_ERR_STATE, "timed launch issued no kernel: " #call); \
+        if (ok_) (e).pending.push_back(EnvState::TimedLaunch{cls, a_, b_});        \
+        if ((e).pending.size() >= 4096) {                                         \
+            const int rt_ = resolveTiming(e);                                     \
+            if (rt_) return rt_;                                                  \
+        }                                                                         \
     } while (0)
 
 int resolveTiming(EnvState& e) {
@@ -223,24 +240,35 @@ int precipFields(EnvState& e, const int* flds, const int* types, int n, float dt
 int precip(EnvState& e, int fld, int type, float dt) { return precipFields(e, &fld, &type, 1, dt); }
 
 // calculatePressureProject, Game/environment.cu:898-987. The scalar state
-// lives on the device (two slots); the host looks at it every 25 iterations
-// to stop launching once the solve has converged.
+// lives on the device (two slots) and every kernel of a finished solve
+// returns at once. After each chunk of 25 iterations the state is copied to
+// pinned host memory behind an event, and the host waits for the copy of the
+// chunk before: the device always has the current chunk queued while the
+// host decides whether to enqueue the next. A solve that converges in chunk
+// c stops after chunk c + 1 has been queued (its kernels exit at once). The
+// final state is copied the same way and read by skt_env_get_stats.
 int solvePressure(EnvState& e, float dt) {
     TL(e, TC_OTHER, launch_divergence(e, dt));
     TL(e, TC_OTHER, launch_pcg_start(e));
     const int maxIt = e.cfg.pcg_max_iter;
     const int chunk = 25;
-    int it = 0;
+    int it = 0, c = 0;
     for (; it < maxIt; it++) {
         TL(e, TC_PCG_A, launch_pcg_A(e, it));
         TL(e, TC_PCG_B, launch_pcg_B(e, it));
         if ((it + 1) % chunk == 0 && it + 1 < maxIt) {
-            int rc = readScalars(e);
-            if (rc) return rc;
-            if (e.sc_host->st[1].done) {
-                it++;
-                break;
+            const int w = c & 1;
+            HIPCHK(e, hipMemcpyAsync(&e.pcg_snap[w], &e.sc->st[1], sizeof(PcgState), hipMemcpyDeviceToHost,
+                                     e.stream));
+            HIPCHK(e, hipEventRecord(e.snap_ev[w], e.stream));
+            if (c > 0) {
+                HIPCHK(e, hipEventSynchronize(e.snap_ev[w ^ 1]));
+                if (e.pcg_snap[w ^ 1].done) {
+                    it++;
+                    break;
+                }
             }
+            c++;
         }
     }
     int slot = 1;
@@ -248,12 +276,9 @@ int solvePressure(EnvState& e, float dt) {
         TL(e, TC_OTHER, launch_pcg_final(e, it));
         slot = 0;
     }
-    int rc = readScalars(e);
-    if (rc) return rc;
-    const PcgState& st = e.sc_host->st[slot];
-    e.stats.pcg_iterations = st.iters;
-    e.stats.pcg_residual = st.maxr;
-    e.stats.pcg_residual0 = st.maxr0;
+    HIPCHK(e, hipMemcpyAsync(&e.pcg_snap[2], &e.sc->st[slot], sizeof(PcgState), hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(e, hipEventRecord(e.snap_ev[2], e.stream));
+    e.pcg_stats_pending = true;
     return SKT_OK;
 }
 
@@ -341,7 +366,6 @@ int update(EnvState& e, float dt, float speed) {
     if (e.params.time > 86400.0f) e.params.time = 0.0f;
     e.params.ground_changed = 0;
     HIPCHK(e, hipGetLastError());
-    if (e.timing) RC(resolveTiming(e));
     return SKT_OK;
 }
 
@@ -369,6 +393,9 @@ void freeAll(EnvState& e) {
     }
     for (auto ev : e.evpool) (void)hipEventDestroy(ev);
     if (e.sc_host) (void)hipHostFree(e.sc_host);
+    if (e.pcg_snap) (void)hipHostFree(e.pcg_snap);
+    for (int k = 0; k < 3; k++)
+        if (e.snap_ev[k]) (void)hipEventDestroy(e.snap_ev[k]);
     if (e.own_stream) (void)hipStreamDestroy(e.own_stream);
 }
 
@@ -449,6 +476,10 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
         dalloc(e, &e.sc, sizeof(Scalars)))
         return bail(SKT_ERR_NOMEM);
     if (hipHostMalloc((void**)&e.sc_host, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) return bail(SKT_ERR_NOMEM);
+    if (hipHostMalloc((void**)&e.pcg_snap, 3 * sizeof(PcgState), hipHostMallocDefault) != hipSuccess)
+        return bail(SKT_ERR_NOMEM);
+    for (int k = 0; k < 3; k++)
+        if (hipEventCreateWithFlags(&e.snap_ev[k], hipEventDisableTiming) != hipSuccess) return bail(SKT_ERR_HIP);
     // environmentGPU ctor defaults (Game/environment.cu:151-168)
     for (int k = 0; k < SKT_FIELD_COUNT; k++)
         if (hipMemsetAsync(e.f[k], 0, sizeof(float) * fieldLen(e, k), e.stream) != hipSuccess) return bail(SKT_ERR_HIP);
@@ -760,7 +791,15 @@ int skt_env_get_params(skt_env* env, skt_env_params* p) {
 }
 int skt_env_get_stats(skt_env* env, skt_step_stats* st) {
     if (!env || !st) return SKT_ERR_ARG;
-    *st = env->s.stats;
+    EnvState& e = env->s;
+    if (e.pcg_stats_pending) {
+        HIPCHK(e, hipEventSynchronize(e.snap_ev[2]));
+        e.stats.pcg_iterations = e.pcg_snap[2].iters;
+        e.stats.pcg_residual = e.pcg_snap[2].maxr;
+        e.stats.pcg_residual0 = e.pcg_snap[2].maxr0;
+        e.pcg_stats_pending = false;
+    }
+    *st = e.stats;
     return SKT_OK;
 }
 // environmentGPU::prepareBrushGPU / prepareSelectionGPU
@@ -879,7 +918,13 @@ int skt_env_set_timing(skt_env* env, int32_t class_mask) {
     for (int k = 0; k < 8; k++) {
         e.t_ms[k] = 0.0;
         e.t_count[k] = 0;
+        e.t_seen[k] = 0;
     }
+    return SKT_OK;
+}
+int skt_env_set_timing_stride(skt_env* env, int32_t stride) {
+    if (!env || stride < 1) return SKT_ERR_ARG;
+    env->s.t_stride = stride;
     return SKT_OK;
 }
 int skt_env_get_timing(skt_env* env, skt_timing* out) {

@@ -1,6 +1,7 @@
 // Internal state of one environment and the launch interface between the
 // host orchestration (skt_api.cpp) and the kernels (skt_kernels.hip).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <string>
 #include <vector>
@@ -103,9 +104,36 @@ struct EnvState {
     std::vector<hipEvent_t> evpool;
     double t_ms[8] = {0};
     long long t_count[8] = {0};
+    long long t_seen[8] = {0};  // launches of each class since skt_env_set_timing
+    int t_stride = 1;           // time every t_stride-th launch of a class
+    // pressure-solve state snapshots in pinned host memory: two alternating
+    // chunk snapshots and the final state, each behind its event
+    PcgState* pcg_snap = nullptr;
+    hipEvent_t snap_ev[3] = {nullptr, nullptr, nullptr};
+    bool pcg_stats_pending = false;
+    // events armed for the next klaunch (a timed single-kernel launch)
+    hipEvent_t kevA = nullptr, kevB = nullptr;
 };
 
 enum { TC_DIFFUSE = 0, TC_PPM, TC_PCG_A, TC_PCG_B, TC_MICRO, TC_MAXVEL, TC_PRECIP, TC_OTHER };
+// Classes whose launch functions issue one kernel through klaunch, so their
+// timing events ride on the kernel's own dispatch.
+constexpr bool kTimedOnDispatch[8] = {true, true, true, true, true, false, true, false};
+
+// Launch on the env's stream. When a timed launch has armed kevA/kevB, the
+// events are attached to this dispatch (hipExtLaunchKernelGGL records the
+// kernel's own start and end) and disarmed. Marker events recorded between
+// kernels instead cost several microseconds of idle device time per launch.
+template <class F, class... Args>
+void klaunch(EnvState& e, F kernel, dim3 grid, dim3 block, uint32_t shm, Args... args) {
+    if (e.kevA && e.kevB) {
+        hipEvent_t a = e.kevA, b = e.kevB;
+        e.kevA = e.kevB = nullptr;
+        hipExtLaunchKernelGGL(kernel, grid, block, shm, e.stream, a, b, 0u, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shm, e.stream, args...);
+    }
+}
 
 // ------------------------------------------------------------- launches ---
 Geom geom(const EnvState& e);

@@ -92,7 +92,7 @@ KL void k_copy_sky(Geom G, const float* __restrict__ s, float* __restrict__ d) {
     d[i] = s[i];
 }
 hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst) {
-    hipLaunchKernelGGL(k_copy_sky, grid3(e), B3, 0, e.stream, geom(e), src, dst);
+    klaunch(e, k_copy_sky, grid3(e), B3, 0, geom(e), src, dst);
     return hipGetLastError();
 }
 // Plain streaming copy (float4 loads/stores): the measured bandwidth ceiling.
@@ -310,8 +310,8 @@ KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const flo
 hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k) {
     const ZGrid zg = e.gD;
     if ((double)(zg.zc + 3) * (double)e.X * (double)e.Y * 4.0 >= 4294967296.0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_diffuse<1>, dim3(zg.nblocks * db.n), B3, 0, e.stream, geom(e), zg.zc, zg.nblocks, db, k,
-                       e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
+    klaunch(e, k_diffuse<1>, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k,
+            e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
     return hipGetLastError();
 }
 
@@ -619,13 +619,13 @@ hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) 
     const ZGrid zg = zgrid(e);
     const dim3 grid(zg.nblocks * pb.n);
     if (dir == 0)
-        hipLaunchKernelGGL(k_ppm_x<ZB_PPM>, grid, B3, 0, e.stream, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
-                           e.f[SKT_DEFAULT_VELX]);
+        klaunch(e, k_ppm_x<ZB_PPM>, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
+                e.f[SKT_DEFAULT_VELX]);
     else if (dir == 1)
-        hipLaunchKernelGGL(k_ppm_y<ZB_PPM>, grid, B3, 0, e.stream, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx);
+        klaunch(e, k_ppm_y<ZB_PPM>, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx);
     else
-        hipLaunchKernelGGL(k_ppm_z<ZB_PPM>, grid, B3, 0, e.stream, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
-                           e.f[SKT_DEFAULT_VELZ]);
+        klaunch(e, k_ppm_z<ZB_PPM>, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
+                e.f[SKT_DEFAULT_VELZ]);
     return hipGetLastError();
 }
 
@@ -1083,8 +1083,8 @@ hipError_t launch_pcg_A(EnvState& e, int it) {
     const bool trunc = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     auto k = it == 0 ? (trunc ? k_pcg_A<true, true> : k_pcg_A<true, false>)
                      : (trunc ? k_pcg_A<false, true> : k_pcg_A<false, false>);
-    hipLaunchKernelGGL(k, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho, e.pcg[4], pm, ps,
-                       np, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it);
+    klaunch(e, k, zg.grid, B3, 0, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho, e.pcg[4], pm, ps,
+            np, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it);
     return hipGetLastError();
 }
 
@@ -1202,10 +1202,10 @@ hipError_t launch_pcg_B(EnvState& e, int it) {
     const int absmode = ref ? e.cfg.maxabs_mode : 1;
     const size_t M = e.maxBlocks;
     auto k = e.X % 4 == 0 ? k_pcg_B<4> : k_pcg_B<1>;
-    hipLaunchKernelGGL(k, dim3(e.gB.nblocks), B3, 0, e.stream, geom(e), e.gB.rpb, e.f[SKT_PRESSURE_PROJ], e.pcg[0],
-                       s_new, e.pcg[4], e.pcg[5], e.pcg[1], e.partials + PART_A * M, e.gA.nblocks,
-                       e.partials + PART_BMAX * M, e.partials + PART_BSUM * M, &e.sc->st[0], &e.sc->st[1], ref,
-                       absmode);
+    klaunch(e, k, dim3(e.gB.nblocks), B3, 0, geom(e), e.gB.rpb, e.f[SKT_PRESSURE_PROJ], e.pcg[0],
+            s_new, e.pcg[4], e.pcg[5], e.pcg[1], e.partials + PART_A * M, e.gA.nblocks,
+            e.partials + PART_BMAX * M, e.partials + PART_BSUM * M, &e.sc->st[0], &e.sc->st[1], ref,
+            absmode);
     return hipGetLastError();
 }
 

@@ -253,8 +253,8 @@ KL void k_precip(Geom G, PrecipBatch pbt, const float* __restrict__ th, const fl
     }
 }
 hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt) {
-    hipLaunchKernelGGL(k_precip, dim3((unsigned)((e.G + 255) / 256), pbt.n), dim3(256), 0, e.stream, geom(e), pbt,
-                       e.f[SKT_POTTEMP], e.f[SKT_PRESSURE], e.f[SKT_QV], e.f[SKT_GROUND_P], e.dx, e.gm);
+    klaunch(e, k_precip, dim3((unsigned)((e.G + 255) / 256), pbt.n), dim3(256), 0, geom(e), pbt,
+            e.f[SKT_POTTEMP], e.f[SKT_PRESSURE], e.f[SKT_QV], e.f[SKT_GROUND_P], e.dx, e.gm);
     return hipGetLastError();
 }
 
@@ -572,7 +572,7 @@ hipError_t launch_microphysics(EnvState& e, float dt, float speed) {
                 e.f[SKT_DENSITY], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], e.phase,
                 e.region_active ? e.region : nullptr, {e.rmin[0], e.rmin[1], e.rmin[2]},
                 {e.rmax[0], e.rmax[1], e.rmax[2]}};
-    hipLaunchKernelGGL(k_microphysics, grid3(e), B3, 0, e.stream, geom(e), a, dt, speed, e.gm);
+    klaunch(e, k_microphysics, grid3(e), B3, 0, geom(e), a, dt, speed, e.gm);
     return hipGetLastError();
 }
 // Fixed-order sum of the region partials into 33 floats (written to sc->pad area

@@ -369,12 +369,14 @@ class EnvironmentGPU:
         self._chk(self.lib.skt_mphys_region_get(self.h, _ptr(out)), "region_get")
         return out
 
-    def set_timing(self, enable=True, classes=None):
+    def set_timing(self, enable=True, classes=None, stride=1):
         """Per-class kernel timing with HIP events; classes: names from
-        TIMING_CLASSES to time (default all), enable=False switches it off."""
+        TIMING_CLASSES to time (default all), stride: time every stride-th
+        launch of a class; enable=False switches it off."""
         mask = 0
         if enable:
             mask = 0xFF if classes is None else sum(1 << _lib.TIMING_CLASSES.index(c) for c in classes)
+        self._chk(self.lib.skt_env_set_timing_stride(self.h, int(stride)), "set_timing_stride")
         self._chk(self.lib.skt_env_set_timing(self.h, mask), "set_timing")
 
     def timing(self):

@@ -585,3 +585,40 @@ def test_sounding_initial_state_and_steps(source):
         assert_fields(env, orc, SKY + GROUND, m, l2=tol, maxabs=50 * tol, tag=f"sounding step{s}")
     env.close()
     orc.close()
+
+
+def test_kernel_timing_classes():
+    """Per-class timing: every class of a step is timed (events on the
+    kernel's dispatch for single-kernel classes, around the call otherwise),
+    and timing leaves the results bit-identical."""
+    nx, ny, nz = 24, 20, 16
+    env, orc = make_pair(nx, ny, nz, seed=13)
+    orc.close()
+    import skewtcloudsim_amd as skt
+    snd = skt.make_sounding(nx, ny, nz, 100.0, seed=13, n_bubbles=2)
+    ref = skt.EnvironmentGPU(nx, ny, nz, 100.0)
+    ref.init(*[snd[k] for k in KEYS])
+    env.set_timing(True)
+    for _ in range(2):
+        env.updateGPU(0.5, 1.0)
+        ref.updateGPU(0.5, 1.0)
+    t = env.timing()
+    for c, (n, ms) in t.items():
+        assert n > 0 and ms > 0.0, (c, n, ms)
+    assert t["pcg_spmv"][0] == t["pcg_update"][0]
+    env.set_timing(True, classes=["pcg_spmv"])
+    env.updateGPU(0.5, 1.0)
+    ref.updateGPU(0.5, 1.0)
+    t2 = env.timing()
+    assert t2["pcg_spmv"][0] == env.stats().pcg_iterations and t2["pcg_update"][0] == 0
+    # every third launch of the class
+    env.set_timing(True, classes=["pcg_spmv"], stride=3)
+    env.updateGPU(0.5, 1.0)
+    ref.updateGPU(0.5, 1.0)
+    k = env.stats().pcg_iterations
+    assert env.timing()["pcg_spmv"][0] == (k + 2) // 3
+    env.set_timing(False)
+    for f in SKY + GROUND:
+        assert np.array_equal(env.get(f), ref.get(f), equal_nan=True), f
+    env.close()
+    ref.close()
