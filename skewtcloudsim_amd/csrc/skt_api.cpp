@@ -379,6 +379,7 @@ void freeAll(EnvState& e) {
     fr(e.gh);
     fr(e.ghm1);
     fr(e.ghm2);
+    fr(e.ghz);
     fr(e.ghEdit);
     fr(e.irho);
     fr(e.zeros);
@@ -464,7 +465,8 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     for (int k = 0; k < 6; k++)
         if (dalloc(e, &e.pcg[k], sizeof(float) * e.N)) return bail(SKT_ERR_NOMEM);
     if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.ghm1, sizeof(int) * e.G) ||
-        dalloc(e, &e.ghm2, sizeof(int) * e.G) || dalloc(e, &e.irho, sizeof(float) * e.N) ||
+        dalloc(e, &e.ghm2, sizeof(int) * e.G) || dalloc(e, &e.ghz, sizeof(int) * e.G) ||
+        dalloc(e, &e.irho, sizeof(float) * e.N) ||
         dalloc(e, &e.ghEdit, sizeof(int) * e.G) ||
         dalloc(e, &e.zeros, sizeof(float) * e.Y) || dalloc(e, &e.ones, sizeof(float) * e.Y) ||
         dalloc(e, &e.phase, (size_t)e.N))
@@ -486,6 +488,7 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     if (hipMemsetAsync(e.gh, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
         hipMemsetAsync(e.ghm1, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
         hipMemsetAsync(e.ghm2, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
+        hipMemsetAsync(e.ghz, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
         hipMemsetAsync(e.sc, 0, sizeof(Scalars), e.stream) != hipSuccess ||
         hipMemsetAsync(e.phase, 0, (size_t)e.N, e.stream) != hipSuccess ||
         hipMemsetAsync(e.irho, 0, sizeof(float) * e.N, e.stream) != hipSuccess)
@@ -546,7 +549,7 @@ int skt_env_set_ground_height(skt_env* env, const int32_t* gh) {
         if (gh[g] < 0 || gh[g] >= e.Y) return setErr(e, SKT_ERR_ARG, "ground height outside [0, ny)");
     // column maxima over the 3x3 and 5x5 neighbourhoods (clipped at the
     // domain edge) select the stencil kernels' boundary-free fast paths
-    std::vector<int32_t> m1((size_t)e.G), m2((size_t)e.G);
+    std::vector<int32_t> m1((size_t)e.G), m2((size_t)e.G), mz((size_t)e.G);
     for (int z = 0; z < e.Z; z++)
         for (int x = 0; x < e.X; x++) {
             int a = 0, b = 0;
@@ -560,10 +563,15 @@ int skt_env_set_ground_height(skt_env* env, const int32_t* gh) {
                 }
             m1[x + (size_t)e.X * z] = a;
             m2[x + (size_t)e.X * z] = b;
+            // running maximum along z: a column (x, y) of the z walk in the
+            // velocity reset is still open at plane z iff y > mz
+            const int h = gh[x + (long)e.X * z];
+            mz[x + (size_t)e.X * z] = z == 0 ? h : std::max(h, mz[x + (size_t)e.X * (z - 1)]);
         }
     HIPCHK(e, hipMemcpyAsync(e.gh, gh, sizeof(int) * e.G, hipMemcpyHostToDevice, e.stream));
     HIPCHK(e, hipMemcpyAsync(e.ghm1, m1.data(), sizeof(int) * e.G, hipMemcpyHostToDevice, e.stream));
     HIPCHK(e, hipMemcpyAsync(e.ghm2, m2.data(), sizeof(int) * e.G, hipMemcpyHostToDevice, e.stream));
+    HIPCHK(e, hipMemcpyAsync(e.ghz, mz.data(), sizeof(int) * e.G, hipMemcpyHostToDevice, e.stream));
     HIPCHK(e, hipStreamSynchronize(e.stream));
     e.params.ground_changed = 1;
     return SKT_OK;

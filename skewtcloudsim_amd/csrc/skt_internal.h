@@ -70,6 +70,7 @@ struct EnvState {
     int* gh = nullptr;                      // ground height [G]
     int* ghm1 = nullptr;                    // max ground height over the 3x3 column neighbourhood [G]
     int* ghm2 = nullptr;                    // max ground height over the 5x5 column neighbourhood [G]
+    int* ghz = nullptr;                     // max ground height over planes 0..z of each x column [G]
     int* ghEdit = nullptr;                  // ground heights being edited by a brush or selection [G]
     float* work[SKT_NWORK] = {nullptr};     // N-sized work buffers
     float* pcg[6] = {nullptr};              // r, z, s_a, s_b, As, pre

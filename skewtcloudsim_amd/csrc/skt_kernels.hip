@@ -389,20 +389,29 @@ hipError_t launch_maxvel(EnvState& e, float dt) {
 // face velocity reads through the advected field's bounds (:415,486,559).
 // q[0..4] / o[0..4]: the field and the unit field at offsets -2..2 along the
 // pass direction; vc, vm: the face velocities at +1/2 and -1/2.
-__device__ __forceinline__ float ppmCell(const float* q, const float* o, float vc, float vm, float dt, float dx) {
-    const float Fr = vc >= 0.0f ? ppmFlux(vc, q[1], q[2], q[3], dt, dx) : ppmFlux(vc, q[4], q[3], q[2], dt, dx);
-    const float Fl = vm < 0.0f ? ppmFlux(vm, q[3], q[2], q[1], dt, dx) : ppmFlux(vm, q[0], q[1], q[2], dt, dx);
+__device__ __forceinline__ float ppmFaceR(const float* q, float vc, float dt, float dx) {
+    return vc >= 0.0f ? ppmFlux(vc, q[1], q[2], q[3], dt, dx) : ppmFlux(vc, q[4], q[3], q[2], dt, dx);
+}
+__device__ __forceinline__ float ppmFaceL(const float* q, float vm, float dt, float dx) {
+    return vm < 0.0f ? ppmFlux(vm, q[3], q[2], q[1], dt, dx) : ppmFlux(vm, q[0], q[1], q[2], dt, dx);
+}
+// The cell update from its two face fluxes of the field (Fr, Fl).
+__device__ __forceinline__ float ppmFinish(const float* q, const float* o, float vc, float vm, float Fr, float Fl,
+                                          float dt, float dx) {
     float Gr = vc, Gl = vm;
     // A unit stencil (1,1,1) reconstructs the face value 1 exactly, so away
     // from boundaries the unit field's flux is the face velocity itself.
     if (!(o[0] == 1.0f && o[1] == 1.0f && o[3] == 1.0f && o[4] == 1.0f)) {
-        Gr = vc >= 0.0f ? ppmFlux(vc, o[1], o[2], o[3], dt, dx) : ppmFlux(vc, o[4], o[3], o[2], dt, dx);
-        Gl = vm < 0.0f ? ppmFlux(vm, o[3], o[2], o[1], dt, dx) : ppmFlux(vm, o[0], o[1], o[2], dt, dx);
+        Gr = ppmFaceR(o, vc, dt, dx);
+        Gl = ppmFaceL(o, vm, dt, dx);
     }
     const float c = dt / dx;
     const float num = q[2] - c * (Fr - Fl);
     const float den = 1.0f - c * (Gr - Gl);
     return num / den;
+}
+__device__ __forceinline__ float ppmCell(const float* q, const float* o, float vc, float vm, float dt, float dx) {
+    return ppmFinish(q, o, vc, vm, ppmFaceR(q, vc, dt, dx), ppmFaceL(q, vm, dt, dx), dt, dx);
 }
 
 // Field of this block in a batched pass (the grid repeats the z-march grid
@@ -492,60 +501,96 @@ KL void k_ppm_x(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, c
     }
 }
 
-// y pass: the stencil runs along the column of this thread. Cells with
-// y >= gh + 3 see no ground in the stencil.
-template <int ZB>
-KL void k_ppm_y(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx) {
-    PPM_FIELD_SETUP;
-    if (!inb) return;
+// y pass as a march up the columns: each thread owns the column (x, z) of
+// a chunk of yc levels; the five-point stencil is a register window that
+// gains one level per step (one load of the field and one of the face
+// velocity per cell, issued a group of four levels ahead). The flux through
+// a cell's lower face is the upper-face flux of the cell below, computed one
+// step earlier (same face velocity, same three stencil values), so one field
+// flux is evaluated per cell; the chunk's first cell and a cell just above
+// the ground (whose lower face velocity takes the ground condition) evaluate
+// it directly. Blocks are 64 (x) x 4 (z) columns of one chunk of one field.
+constexpr int YC_PPM = 16;
+constexpr int YG_PPM = 4;
+KL void k_ppm_ym(Geom G, int yc, unsigned nyc, unsigned nbf, PpmBatch pb, float dt, float dx) {
+    const unsigned lin = blockIdx.x;
+    const PpmField& fd = pb.f[lin / nbf];
+    const unsigned rem = lin % nbf;
+    const int chunk = (int)(rem % nyc);
+    const unsigned tile = rem / nyc;
+    const int gx = (G.X + 63) >> 6;
+    const int x = (int)(tile % (unsigned)gx) * 64 + threadIdx.x;
+    const int z = (int)(tile / (unsigned)gx) * 4 + threadIdx.y;
+    if (x >= G.X || z >= G.Z) return;
+    const float* __restrict__ in = fd.in;
+    float* __restrict__ out = fd.out;
+    const float* __restrict__ vel = fd.vel;
+    const float* __restrict__ prof = fd.prof;
+    const Bounds b = fd.b;
+    const float* __restrict__ gsrc = fd.gsrc;
+    const bool gNeu = b.ground == BC_NEUMANN && gsrc != nullptr;
     const int X = G.X, Y = G.Y;
-    const long XY = (long)X * Y;
-    unsigned ry[5];
-    float pv[5];
-#pragma unroll
-    for (int s = 0; s < 5; s++) {
-        const int by = clampi(y + s - 2, 0, Y - 1);
-        ry[s] = x + X * by;
-        pv[s] = prof ? prof[by] : 0.0f;
-    }
-    auto load = [&](int z, float* Q, float& VC, float& VM, int& gc) {
-        const float* pl = in + XY * z;
-        const float* vl = vel + XY * z;
-#pragma unroll
-        for (int s = 0; s < 5; s++) Q[s] = pl[ry[s]];
-        VC = vl[ry[2]];
-        VM = vl[ry[1]];
-        gc = (G.gh + X * z)[(unsigned)x];
+    const long col = (long)X * Y * z + x;  // cell (x, 0, z)
+    const int gh = G.gh[x + (long)X * z];
+    const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
+    // field and unit values at level py as a stencil sees them, from the
+    // field's value at the clamped level
+    auto pos = [&](int py, float raw, float& qv, float& ov) {
+        const int by = clampi(py, 0, Y - 1);
+        const bool outp = (py < 0) || (py > Y - 1);
+        const bool gnd = by <= gh;
+        if (gnd) raw = gNeu ? gsrc[col + (long)X * by] : 0.0f;
+        qv = nbv(raw, outp, gnd, prof ? prof[by] : 0.0f, b);
+        ov = unitv(outp, gnd, b);
     };
-    auto plane = [&](int z, const float* Q, float VC, float VM, int gh) {
-        if (y <= gh) return;
-        float q[5], o[5];
+    float q[5], o[5];
 #pragma unroll
-        for (int s = 0; s < 5; s++) {
-            const int py = y + s - 2;
-            const bool outp = (py < 0) || (py > Y - 1);
-            const bool gnd = clampi(py, 0, Y - 1) <= gh;
-            q[s] = nbv(gnd ? graw(XY * z + ry[s]) : Q[s], outp, gnd, pv[s], b);
-            o[s] = unitv(outp, gnd, b);
+    for (int s = 0; s < 4; s++) {
+        const int py = y0 - 2 + s;
+        pos(py, in[col + (long)X * clampi(py, 0, Y - 1)], q[s], o[s]);
+    }
+    // raw field values at levels g + 2 .. g + 5 and face velocities at g .. g + 3
+    auto fetch = [&](int g, float* Q, float* V) {
+#pragma unroll
+        for (int j = 0; j < YG_PPM; j++) {
+            Q[j] = in[col + (long)X * min(g + j + 2, Y - 1)];
+            V[j] = vel[col + (long)X * min(g + j, Y - 1)];
         }
-        // y - 1 >= 0 on every sky cell
-        const float vm = (y - 1 <= gh) ? bcv(b.ground, VC, 0.0f) : VM;
-        (out + XY * z)[ry[2]] = ppmCell(q, o, VC, vm, dt, dx);
     };
-    int zg = z0;
-    for (; zg + ZB <= z1; zg += ZB) {
-        float Q[ZB][5], VC[ZB], VM[ZB];
-        int GC[ZB];
+    float vPrev = vel[col + (long)X * max(y0 - 1, 0)];
+    float fPrev = 0.0f;
+    auto group = [&](int g, const float* Q, const float* V) {
 #pragma unroll
-        for (int j = 0; j < ZB; j++) load(zg + j, Q[j], VC[j], VM[j], GC[j]);
+        for (int j = 0; j < YG_PPM; j++) {
+            const int y = g + j;
+            if (y >= y1) break;
+            pos(y + 2, Q[j], q[4], o[4]);
+            const float vc = V[j];
+            if (y > gh) {
+                // y - 1 >= 0 on every sky cell
+                const bool below = y - 1 <= gh;
+                const float vm = below ? bcv(b.ground, vc, 0.0f) : vPrev;
+                const float Fr = ppmFaceR(q, vc, dt, dx);
+                const float Fl = (y > y0 && !below) ? fPrev : ppmFaceL(q, vm, dt, dx);
+                fPrev = Fr;
+                out[col + (long)X * y] = ppmFinish(q, o, vc, vm, Fr, Fl, dt, dx);
+            }
+            vPrev = vc;
 #pragma unroll
-        for (int j = 0; j < ZB; j++) plane(zg + j, Q[j], VC[j], VM[j], GC[j]);
-    }
-    for (; zg < z1; zg++) {
-        float Q[5], VC, VM;
-        int gc;
-        load(zg, Q, VC, VM, gc);
-        plane(zg, Q, VC, VM, gc);
+            for (int s = 0; s < 4; s++) {
+                q[s] = q[s + 1];
+                o[s] = o[s + 1];
+            }
+        }
+    };
+    float Qa[YG_PPM], Va[YG_PPM], Qb[YG_PPM], Vb[YG_PPM];
+    fetch(y0, Qa, Va);
+    for (int g = y0; g < y1; g += 2 * YG_PPM) {
+        if (g + YG_PPM < y1) fetch(g + YG_PPM, Qb, Vb);
+        group(g, Qa, Va);
+        if (g + YG_PPM >= y1) break;
+        if (g + 2 * YG_PPM < y1) fetch(g + 2 * YG_PPM, Qa, Va);
+        group(g + YG_PPM, Qb, Vb);
     }
 }
 
@@ -576,6 +621,10 @@ KL void k_ppm_z(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, c
         g = (G.gh + X * bz)[(unsigned)x];
         v = (vel + XY * z)[rc];
     };
+    // The flux through the back face of plane z is the front-face flux of
+    // plane z - 1 when this thread computed that plane (same face velocity,
+    // same stencil values); else it is evaluated here.
+    float fPrev = 0.0f;
     auto plane = [&](int z, const float* qs, const int* gs, float vc, float vmr) {
         if (y <= gs[2]) return;
         float q[5], o[5];
@@ -588,7 +637,10 @@ KL void k_ppm_z(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, c
             o[s] = unitv(outp, gnd, b);
         }
         const float vm = z == 0 ? bcv(b.sides, vc, cv) : (y <= gs[1] ? bcv(b.ground, vc, cv) : vmr);
-        (out + XY * z)[rc] = ppmCell(q, o, vc, vm, dt, dx);
+        const float Fr = ppmFaceR(q, vc, dt, dx);
+        const float Fl = (z > z0 && y > gs[1]) ? fPrev : ppmFaceL(q, vm, dt, dx);
+        fPrev = Fr;
+        (out + XY * z)[rc] = ppmFinish(q, o, vc, vm, Fr, Fl, dt, dx);
     };
     int zg = z0;
     for (; zg + ZB <= z1; zg += ZB) {
@@ -621,8 +673,11 @@ hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) 
     if (dir == 0)
         klaunch(e, k_ppm_x<ZB_PPM>, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
                 e.f[SKT_DEFAULT_VELX]);
-    else if (dir == 1)
-        klaunch(e, k_ppm_y<ZB_PPM>, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx);
+    else if (dir == 1) {
+        const unsigned nyc = (unsigned)((e.Y + YC_PPM - 1) / YC_PPM);
+        const unsigned nbf = (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * nyc;
+        klaunch(e, k_ppm_ym, dim3(nbf * pb.n), B3, 0, geom(e), YC_PPM, nyc, nbf, pb, dt, e.dx);
+    }
     else
         klaunch(e, k_ppm_z<ZB_PPM>, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
                 e.f[SKT_DEFAULT_VELZ]);

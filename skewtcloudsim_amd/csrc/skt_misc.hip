@@ -372,26 +372,26 @@ hipError_t launch_new_pressure(EnvState& e) {
     return hipGetLastError();
 }
 
-// resetVelPressProj, Game/kernelSky.cu:1648-1682: one thread per (x,y) column;
-// the column stops at its first ground cell (quirk A16).
-KL void k_reset_vel(Geom G, float* __restrict__ u, float* __restrict__ v, float* __restrict__ w,
-                    const float* __restrict__ defU, const float* __restrict__ defW) {
-    const long c = blockIdx.x * 256L + threadIdx.x;
-    if (c >= (long)G.X * G.Y) return;
-    const int x = (int)(c % G.X), y = (int)(c / G.X);
-    for (int z = 0; z < G.Z; z++) {
-        if (isGround(G, x, y, z)) break;
-        const long i = gidx(G, x, y, z);
-        if (y == G.Y - 1 || isGround(G, x, y + 1, z)) v[i] = 0.0f;
-        if (x == G.X - 1) u[i] = defU[y];
-        if (x < G.X - 1 && isGround(G, x + 1, y, z)) u[i] = 0.0f;
-        if (z == G.Z - 1) w[i] = defW[y];
-        if (z < G.Z - 1 && isGround(G, x, y, z + 1)) w[i] = 0.0f;
-    }
+// resetVelPressProj, Game/kernelSky.cu:1648-1682. The reference walks each
+// (x, y) column along z and stops at its first ground cell (quirk A16), so a
+// cell is reset iff no cell of its column at or before its plane is ground:
+// y > ghz, the largest ground height over planes 0..z of its (x, z) column.
+// One thread per cell; the resets of different cells are independent.
+KL void k_reset_vel(Geom G, const int* __restrict__ ghz, float* __restrict__ u, float* __restrict__ v,
+                    float* __restrict__ w, const float* __restrict__ defU, const float* __restrict__ defW) {
+    CELL_SETUP;
+    (void)bid;
+    if (!inb || y <= ghz[x + (long)G.X * z]) return;
+    const long i = gidx(G, x, y, z);
+    if (y == G.Y - 1 || isGround(G, x, y + 1, z)) v[i] = 0.0f;
+    if (x == G.X - 1) u[i] = defU[y];
+    if (x < G.X - 1 && isGround(G, x + 1, y, z)) u[i] = 0.0f;
+    if (z == G.Z - 1) w[i] = defW[y];
+    if (z < G.Z - 1 && isGround(G, x, y, z + 1)) w[i] = 0.0f;
 }
 hipError_t launch_reset_vel(EnvState& e) {
-    hipLaunchKernelGGL(k_reset_vel, grid1((long)e.X * e.Y), dim3(256), 0, e.stream, geom(e), e.f[SKT_VELX],
-                       e.f[SKT_VELY], e.f[SKT_VELZ], e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ]);
+    hipLaunchKernelGGL(k_reset_vel, grid3(e), B3, 0, e.stream, geom(e), e.ghz, e.f[SKT_VELX], e.f[SKT_VELY],
+                       e.f[SKT_VELZ], e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ]);
     return hipGetLastError();
 }
 
