@@ -33,13 +33,13 @@ int blocks3d(const EnvState& e) {
     return (int)(g.x * g.y * g.z);
 }
 Geom geom(const EnvState& e) { return Geom{e.X, e.Y, e.Z, e.N, e.gh, e.ghm1, e.ghm2}; }
-static long zcols(const EnvState& e) { return (long)((e.X + 63) / 64) * ((e.Y + 3) / 4); }
-// nzc chunks of equal length along z (the last one shorter)
-static ZGrid zgridChunks(const EnvState& e, long nzc) {
+static long zcols(const EnvState& e, int rows = 4) { return (long)((e.X + 63) / 64) * ((e.Y + rows - 1) / rows); }
+// nzc chunks of equal length along z (the last one shorter), tiles of 64 x rows
+static ZGrid zgridChunks(const EnvState& e, long nzc, int rows = 4) {
     nzc = std::max(1L, std::min((long)e.Z, nzc));
     const int zc = (int)((e.Z + nzc - 1) / nzc);
     nzc = (e.Z + zc - 1) / zc;
-    const unsigned nb = (unsigned)(zcols(e) * nzc);
+    const unsigned nb = (unsigned)(zcols(e, rows) * nzc);
     return ZGrid{dim3(nb), zc, nb};
 }
 ZGrid zgrid(const EnvState& e) { return e.gZ; }
@@ -55,10 +55,14 @@ void setup_grids(EnvState& e, ResidentBlocks rb) {
         return zgridChunks(e, a / cols);
     };
     e.gA = round(rb.pcgA);
-    // a diffusion chunk is at most 64 planes (one bit per plane marks the
-    // cells next to ground)
-    e.gD = round(rb.diffuse);
-    if (e.gD.zc > 64) e.gD = zgridChunks(e, (e.Z + 63) / 64);
+    // diffusion: tiles of 64 x 8; a chunk is at most 64 planes (one bit per
+    // plane marks the cells next to ground)
+    {
+        long a = rb.diffuse > 0 ? rb.diffuse : 2048;
+        if (cap > 0) a = std::min(a, cap);
+        e.gD = zgridChunks(e, a / zcols(e, 8), 8);
+        if (e.gD.zc > 64) e.gD = zgridChunks(e, (e.Z + 63) / 64, 8);
+    }
     // the flat walks: about 2048 blocks of whole rows
     const long rows = (long)e.Y * e.Z, nb = std::max(1L, std::min(rows, cap > 0 ? std::min(2048L, cap) : 2048L));
     const long rpb = (rows + nb - 1) / nb;
@@ -147,116 +151,112 @@ constexpr int TH = 6, TW = 66;
 // around) have no ground neighbour, so only the domain faces need a boundary
 // value; the other sky cells are marked and done after the z march, with
 // every neighbour tested for ground.
-// One field per block: blocks of two or three fields (sharing the ground
-// loads and barriers) need over 100 VGPRs and ran slower at half the
-// occupancy; direct neighbour loads without the tile ran slower too.
-template <int NF>
+// One field per block. A block owns a 64 (x) x 8 (y) tile: each thread
+// sweeps two cells of its column, rows yb + ty and yb + ty + 4, so the
+// per-plane scalar work (offsets, loop, barrier) and the ground-height loads
+// of a column serve two cells, and the tile's halo is 144 cells for 512.
+// Blocks of two or three fields (sharing the ground loads and barriers)
+// needed over 100 VGPRs and ran slower at half the occupancy; direct
+// neighbour loads without the tile ran slower too.
+constexpr int DTH = 10;  // tile rows yb - 1 .. yb + 8
 KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const float* __restrict__ gT,
                   const float* __restrict__ gP, const float* __restrict__ p) {
     const unsigned lin = xcdRemap(blockIdx.x, gridDim.x);
-    const int f0 = (int)(lin / nbf) * NF;
-    ZM_SETUP_B(lin % nbf);
-    (void)bid;
-    if (z0 >= z1) return;  // whole block
-    __shared__ float tV[NF][2][TH][TW];
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    const DiffField& fd = db.f[lin / nbf];
+    const unsigned bl = lin % nbf;
     const int X = G.X, Y = G.Y, Z = G.Z;
+    const int gx = (X + 63) >> 6, gy = (Y + 7) >> 3;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x = (int)(bl % (unsigned)gx) * 64 + tx;
+    const int yb = (int)((bl / (unsigned)gx) % (unsigned)gy) * 8;
+    const int z0 = (int)(bl / (unsigned)(gx * gy)) * zc, z1 = min(Z, z0 + zc);
+    if (z0 >= z1) return;  // whole block
+    __shared__ float tV[2][DTH][TW];
     const long XY = (long)X * Y;
-    const int xc = min(x, X - 1), yc = min(y, Y - 1);
-    const unsigned ob = 4u * (unsigned)(xc + X * yc), gb = 4u * (unsigned)xc;
-    // halo cells as in the pressure operator: rows by waves 0 and 1, columns
-    // by lanes 0-7 of wave 2; each reads its position clamped into the domain
-    const int x0 = x - tx, y0 = y - ty;
-    int hx = x, hy = y0 - 1, hrow = 0, hcol = tx + 1;
-    if (ty == 1) hy = y0 + 4, hrow = 5;
-    if (ty == 2) hx = tx < 4 ? x0 - 1 : x0 + 64, hy = y0 + (tx & 3), hrow = 1 + (tx & 3), hcol = tx < 4 ? 0 : 65;
-    const bool hasH = ty < 2 || (ty == 2 && tx < 8);
+    const int ya = yb + ty, yb4 = yb + ty + 4;  // the thread's two rows
+    const bool inA = x < X && ya < Y, inB = x < X && yb4 < Y;
+    const int xc = min(x, X - 1), yca = min(ya, Y - 1), ycb = min(yb4, Y - 1);
+    const unsigned obA = 4u * (unsigned)(xc + X * yca), obB = 4u * (unsigned)(xc + X * ycb), gb = 4u * (unsigned)xc;
+    // halo cells: the row below the tile (wave 0), the row above it (wave 1),
+    // the columns left and right of it (lanes 0-7 and 8-15 of wave 2, one row
+    // each); each reads its position clamped into the domain
+    const int x0 = x - tx;
+    int hx = x, hy = yb - 1, hrow = 0, hcol = tx + 1;
+    if (ty == 1) hy = yb + 8, hrow = 9;
+    if (ty == 2) hx = tx < 8 ? x0 - 1 : x0 + 64, hy = yb + (tx & 7), hrow = 1 + (tx & 7), hcol = tx < 8 ? 0 : 65;
+    const bool hasH = ty < 2 || (ty == 2 && tx < 16);
     const unsigned hb = 4u * (unsigned)(min(max(hx, 0), X - 1) + X * min(max(hy, 0), Y - 1));
     // the same loads in every plane, as in the pressure operator
-    const unsigned hbL = hasH ? hb : ob;
+    const unsigned hbL = hasH ? hb : obA;
     const int zb = max(z0 - 1, 0);
     const unsigned PB = 4u * (unsigned)XY, RB = 4u * (unsigned)X;
-    const int yd = max(yc - 1, 0), yu = min(yc + 1, Y - 1);
-    Rsrc rV[NF], wV[NF];
-    float pv[NF], pvd[NF], pvu[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-        const DiffField& fd = db.f[f0 + f];
-        rV[f] = arrayRsrc(fd.in + XY * zb);
-        wV[f] = arrayRsrc(fd.out + XY * zb);
-        pv[f] = fd.prof ? fd.prof[yc] : 0.0f;
-        pvd[f] = fd.prof ? fd.prof[yd] : 0.0f;
-        pvu[f] = fd.prof ? fd.prof[yu] : 0.0f;
-    }
+    const Rsrc rV = arrayRsrc(fd.in + XY * zb), wV = arrayRsrc(fd.out + XY * zb);
+    const Bounds b = fd.b;
+    auto prof = [&](int yy) { return fd.prof ? fd.prof[yy] : 0.0f; };
+    const float pvA = prof(yca), pvuA = prof(min(yca + 1, Y - 1));
+    const float pvB = prof(ycb), pvuB = prof(min(ycb + 1, Y - 1));
     const Rsrc rG = arrayRsrc(G.gh + (long)X * zb), rM = arrayRsrc(G.ghm1 + (long)X * zb);
     auto pOff = [&](int pl) { return PB * (unsigned)(pl - zb); };
     auto gOff = [&](int pl) { return RB * (unsigned)(pl - zb); };
     const float den = 1 + 6 * k;
     struct Own {
-        float v[NF];
+        float a, b;  // the two rows
         int gm, gf;  // ghm1 of the plane, ground height of the next plane
     };
-    struct Halo {
-        float v[NF];
-    };
     auto loadOwn = [&](int pl, Own& o) __attribute__((always_inline)) {
-#pragma unroll
-        for (int f = 0; f < NF; f++) o.v[f] = bldf(rV[f], ob, pOff(pl));
+        o.a = bldf(rV, obA, pOff(pl));
+        o.b = bldf(rV, obB, pOff(pl));
         o.gm = bldi(rM, gb, gOff(pl));
         o.gf = bldi(rG, gb, gOff(min(pl + 1, Z - 1)));
     };
-    auto loadHalo = [&](int pl, Halo& h) __attribute__((always_inline)) {
-#pragma unroll
-        for (int f = 0; f < NF; f++) h.v[f] = bldf(rV[f], hbL, pOff(pl));
-    };
     // plane below the chunk (clamped: plane 0 for the first chunk)
-    float vb[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) vb[f] = bldf(rV[f], ob, pOff(zb));
+    float vbA = bldf(rV, obA, pOff(zb)), vbB = bldf(rV, obB, pOff(zb));
     int ghc = bldi(rG, gb, gOff(z0));
     Own o0{}, o1{}, o2{}, o3{};
-    Halo h0{}, h1{};
+    float h0 = 0.0f, h1 = 0.0f;
     // prefetches past the chunk read the next chunk's first planes: measured
     // faster than clamping them to the chunk's last plane
     const int zlast = Z - 1;
     loadOwn(z0, o0);
     loadOwn(min(z0 + 1, zlast), o1);
     loadOwn(min(z0 + 2, zlast), o2);
-    loadHalo(z0, h0);
-    unsigned long long slow = 0;  // planes (bit z - z0) whose cell is next to ground
-    auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, Halo& hc, Halo& hn) __attribute__((always_inline)) {
+    h0 = bldf(rV, hbL, pOff(z0));
+    // planes (bit z - z0) whose cell of row a / row b is next to ground
+    unsigned long long slowA = 0, slowB = 0;
+    const bool oL = x == 0, oR = x == X - 1;
+    // fast sweep of one cell: tile row r, this plane's value vc, the front
+    // plane's F (this cell's value on the last plane), the back plane's vb
+    auto sweep = [&](int buf, int r, int z, int y, float vc, float F, float vb, float pv, float pvu, unsigned ob)
+        __attribute__((always_inline)) {
+        const float L = tV[buf][r][tx], R = tV[buf][r][tx + 2];
+        const float D = tV[buf][r - 1][tx + 1], U = tV[buf][r + 1][tx + 1];
+        const bool oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
+        const float l = oL ? bcv(b.sides, L, pv) : L, rr = oR ? bcv(b.sides, R, pv) : R;
+        const float u = oU ? bcv(b.sides, U, pvu) : U;
+        const float fr = oF ? bcv(b.sides, F, pv) : F, bb = oB ? bcv(b.sides, vb, pv) : vb;
+        bstf(wV, ob, pOff(z), (vc + k * (l + rr + u + D + fr + bb)) / den);
+    };
+    auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, float& hc, float& hn) __attribute__((always_inline)) {
         // loads complete in issue order: the halo (used next plane) goes first,
         // so waiting for it does not wait for the plane read three ahead
-        loadHalo(min(z + 1, zlast), hn);
+        hn = bldf(rV, hbL, pOff(min(z + 1, zlast)));
         loadOwn(min(z + 3, zlast), o3n);
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-            tV[f][buf][ty + 1][tx + 1] = oc.v[f];
-            if (hasH) tV[f][buf][hrow][hcol] = hc.v[f];
-        }
+        tV[buf][ty + 1][tx + 1] = oc.a;
+        tV[buf][ty + 5][tx + 1] = oc.b;
+        if (hasH) tV[buf][hrow][hcol] = hc;
         __syncthreads();
-        const int gf = oc.gf;
-        if (inb && y > ghc) {
-            if (y >= oc.gm + 2) {
-                const bool oL = x == 0, oR = x == X - 1, oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
-#pragma unroll
-                for (int f = 0; f < NF; f++) {
-                    const Bounds b = db.f[f0 + f].b;
-                    const float vc = oc.v[f], F = z + 1 < Z ? of.v[f] : vc;
-                    const float L = tV[f][buf][ty + 1][tx], R = tV[f][buf][ty + 1][tx + 2];
-                    const float D = tV[f][buf][ty][tx + 1], U = tV[f][buf][ty + 2][tx + 1];
-                    const float l = oL ? bcv(b.sides, L, pv[f]) : L, r = oR ? bcv(b.sides, R, pv[f]) : R;
-                    const float u = oU ? bcv(b.sides, U, pvu[f]) : U;
-                    const float fr = oF ? bcv(b.sides, F, pv[f]) : F, bb = oB ? bcv(b.sides, vb[f], pv[f]) : vb[f];
-                    bstf(wV[f], ob, pOff(z), (vc + k * (l + r + u + D + fr + bb)) / den);
-                }
-            } else {
-                slow |= 1ull << (z - z0);
-            }
+        const bool inZ = z + 1 < Z;
+        if (inA && ya > ghc) {
+            if (ya >= oc.gm + 2) sweep(buf, ty + 1, z, ya, oc.a, inZ ? of.a : oc.a, vbA, pvA, pvuA, obA);
+            else slowA |= 1ull << (z - z0);
         }
-#pragma unroll
-        for (int f = 0; f < NF; f++) vb[f] = oc.v[f];
-        ghc = gf;
+        if (inB && yb4 > ghc) {
+            if (yb4 >= oc.gm + 2) sweep(buf, ty + 5, z, yb4, oc.b, inZ ? of.b : oc.b, vbB, pvB, pvuB, obB);
+            else slowB |= 1ull << (z - z0);
+        }
+        vbA = oc.a;
+        vbB = oc.b;
+        ghc = oc.gf;
     };
     for (int zg = z0; zg < z1; zg += 4) {
         step(zg, 0, o0, o1, o3, h0, h1);
@@ -264,36 +264,36 @@ KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const flo
         if (zg + 2 < z1) step(zg + 2, 0, o2, o3, o1, h0, h1);
         if (zg + 3 < z1) step(zg + 3, 1, o3, o0, o2, h1, h0);
     }
-    // Sky cells next to ground (marked in 'slow', one bit per plane of the
-    // chunk): the sweep reads only 'in', so they can follow the fast cells.
-    // Each neighbour is tested and takes the ground or sides condition (nbv);
-    // for the potential temperature a ground neighbour reads the potential
-    // temperature of the ground temperature at this cell's pressure (:95-100).
-    // Neighbour positions are clamped into the domain, as in the tile.
+    // Sky cells next to ground (marked in slowA / slowB, one bit per plane of
+    // the chunk): the sweep reads only 'in', so they can follow the fast
+    // cells. Each neighbour is tested and takes the ground or sides condition
+    // (nbv); for the potential temperature a ground neighbour reads the
+    // potential temperature of the ground temperature at this cell's pressure
+    // (:95-100). Neighbour positions are clamped into the domain, as in the tile.
     const int xl = max(xc - 1, 0), xr = min(xc + 1, X - 1);
-    while (slow) {
-        const int z = z0 + __builtin_ctzll(slow);
-        slow &= slow - 1;
-        const long pz = XY * z;
-        const int zf = min(z + 1, Z - 1), zbk = max(z - 1, 0);
-        const int* gz = G.gh + X * z;
-        const int gc = gz[xc], gf = G.gh[xc + X * zf], gbk = G.gh[xc + X * zbk];
-        const bool oL = x == 0, oR = x == X - 1, oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
-        // a position outside the domain clamps onto this sky cell, so the
-        // ground test at the clamped column is the in-domain test
-        const bool gL = y <= gz[xl], gR = y <= gz[xr], gD = y - 1 <= gc;
-        const bool gF = !oF && y <= gf, gB = !oB && y <= gbk;
-        for (int f = 0; f < NF; f++) {
-            const DiffField& fd = db.f[f0 + f];
-            const float* __restrict__ in = fd.in;
-            const Bounds b = fd.b;
+    const float* __restrict__ in = fd.in;
+    auto slowCells = [&](unsigned long long slow, int y) {
+        const int yd = max(y - 1, 0), yu = min(y + 1, Y - 1);
+        const float pv = prof(y), pvd = prof(yd), pvu = prof(yu);
+        while (slow) {
+            const int z = z0 + __builtin_ctzll(slow);
+            slow &= slow - 1;
+            const long pz = XY * z;
+            const int zf = min(z + 1, Z - 1), zbk = max(z - 1, 0);
+            const int* gz = G.gh + X * z;
+            const int gc = gz[xc], gf = G.gh[xc + X * zf], gbk = G.gh[xc + X * zbk];
+            const bool oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
+            // a position outside the domain clamps onto this sky cell, so the
+            // ground test at the clamped column is the in-domain test
+            const bool gL = y <= gz[xl], gR = y <= gz[xr], gD = y - 1 <= gc;
+            const bool gF = !oF && y <= gf, gB = !oB && y <= gbk;
             const float vc = in[pz + xc + X * y];
             const float L = in[pz + xl + X * y], R = in[pz + xr + X * y];
             const float D = in[pz + xc + X * yd], U = in[pz + xc + X * yu];
             const float F = in[XY * zf + xc + X * y], Bk = in[XY * zbk + xc + X * y];
-            float l = nbv(L, oL, gL, pv[f], b), r = nbv(R, oR, gR, pv[f], b);
-            float d = nbv(D, false, gD, pvd[f], b), u = nbv(U, oU, false, pvu[f], b);
-            float fr = nbv(F, oF, gF, pv[f], b), bb = nbv(Bk, oB, gB, pv[f], b);
+            float l = nbv(L, oL, gL, pv, b), r = nbv(R, oR, gR, pv, b);
+            float d = nbv(D, false, gD, pvd, b), u = nbv(U, oU, false, pvu, b);
+            float fr = nbv(F, oF, gF, pv, b), bb = nbv(Bk, oB, gB, pv, b);
             if (fd.type == 0 && (gL | gR | gD | gF | gB)) {
                 const float pc = p[pz + xc + X * y];
                 auto tg = [&](int col) { return d_potTemp(gT[col] - 273.15f, pc, gP[col]) + 273.15f; };
@@ -305,12 +305,14 @@ KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const flo
             }
             fd.out[pz + xc + X * y] = (vc + k * (l + r + u + d + fr + bb)) / den;
         }
-    }
+    };
+    slowCells(slowA, ya);
+    slowCells(slowB, yb4);
 }
 hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k) {
     const ZGrid zg = e.gD;
     if ((double)(zg.zc + 3) * (double)e.X * (double)e.Y * 4.0 >= 4294967296.0) return hipErrorInvalidValue;
-    klaunch(e, k_diffuse<1>, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k,
+    klaunch(e, k_diffuse, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k,
             e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
     return hipGetLastError();
 }
@@ -1154,7 +1156,7 @@ static long resident_blocks(K kernel, int device) {
     return (long)nb * cus;
 }
 ResidentBlocks resident_blocks(int device) {
-    return ResidentBlocks{resident_blocks(k_pcg_A<false, true>, device), resident_blocks(k_diffuse<1>, device)};
+    return ResidentBlocks{resident_blocks(k_pcg_A<false, true>, device), resident_blocks(k_diffuse, device)};
 }
 
 // Prologue PRO_B, then updatePandDiv (Game/kernelSky.cu:955-985) + the
