@@ -33,7 +33,8 @@ struct Scalars {
     int dg_level;  //   highest cloudy level (-1: none)
     int edit_ground;  // an edit touched the ground heights
     double dg_rain;  //   ground rain water summed over the columns
-    int pad[24];
+    int face_exact;  // a sky 1/density of this solve lies outside the short face form's range
+    int pad[23];
 };
 
 // Regions of the per-block reduction partials (each maxBlocks doubles).

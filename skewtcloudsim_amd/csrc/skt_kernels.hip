@@ -694,7 +694,11 @@ __device__ __forceinline__ float trunc8(float c) { return fminf(fmaxf(truncf(c),
 // {N,D,D}. The operator is not stored: the SpMV rebuilds its coefficients
 // from 1/density (irho, 4 B per cell instead of 16 for a float4 operator).
 // This pass writes irho and the Jacobi preconditioner 1/diagonal.
-KL void k_build_precon(Geom G, const float* __restrict__ rho, float* __restrict__ irho, float* __restrict__ pre) {
+KL void k_set_int(int* a, int v) {
+    if (threadIdx.x == 0) *a = v;
+}
+KL void k_build_precon(Geom G, const float* __restrict__ rho, float* __restrict__ irho, float* __restrict__ pre,
+                       int* __restrict__ faceExact) {
     CELL_SETUP;
     (void)bid;
     if (!inb) return;
@@ -719,11 +723,18 @@ KL void k_build_precon(Geom G, const float* __restrict__ rho, float* __restrict_
     w -= d;
     w -= f;
     w -= bb;
-    irho[i] = 1.0f / c;
+    const float ic = 1.0f / c;
+    irho[i] = ic;
     pre[i] = w == 0.0f ? 1.0f : 1.0f / w;
+    // the SpMV's short face form is exact when every sky 1/density is in range
+    if (!faceValueOk(ic)) atomicOr(faceExact, 1);
 }
 hipError_t launch_build_precon(EnvState& e) {
-    hipLaunchKernelGGL(k_build_precon, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_DENSITY], e.irho, e.pcg[5]);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, e.stream, &e.sc->face_exact, 0);
+    hipError_t r = hipGetLastError();
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(k_build_precon, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_DENSITY], e.irho, e.pcg[5],
+                       &e.sc->face_exact);
     return hipGetLastError();
 }
 
@@ -932,9 +943,10 @@ hipError_t launch_pcg_start(EnvState& e) {
 // (faceI gives 0), a position just outside the domain holds s = 0 and the
 // boundary cell's own 1/density (every thread reads its position clamped into
 // the domain). Faces take the short division (faceFast); it is exact when
-// each sky value lies in faceValueOk's range. The writers of a plane's values
-// check them and raise the plane's block flag otherwise; a flagged plane
-// recomputes every face with faceI.
+// each sky value lies in faceValueOk's range, which the preconditioner pass
+// checks for the whole solve (faceExact set: every face takes faceI).
+// s_new of the front plane, computed for this plane's front face, is the
+// next plane's own s_new.
 // The z loop runs four planes per trip over a ring of four plane slots (own
 // data is read three planes ahead) and two halo slots, so no values move
 // between registers and each load is waited for only where it is used.
@@ -944,17 +956,12 @@ template <bool FIRST, bool TRUNC>
 KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __restrict__ s_old,
                 float* __restrict__ s_new, const float* __restrict__ irho, float* __restrict__ As,
                 const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, double* pout,
-                const PcgState* sin, PcgState* sout, int it) {
+                const PcgState* sin, PcgState* sout, int it, const int* __restrict__ faceExact) {
     ZM_SETUP;
     __shared__ double sh[4];
     __shared__ PcgState S;
     __shared__ float tS[2][TH][TW], tI[2][TH][TW];
-    // Flag of plane k (counted from z0) in slot k % 3: writers set it before
-    // the plane's barrier, every thread reads it after. Thread 0 clears slot
-    // (k + 1) % 3 at plane k before its barrier; the slot's last readers (of
-    // plane k - 2) have all passed barrier k - 1 by then.
-    __shared__ int flagBuf[3];
-    if (threadIdx.y == 0 && threadIdx.x < 3) flagBuf[threadIdx.x] = 0;
+    const bool exact = *faceExact != 0;
     constexpr int KIND = FIRST ? PRO_A0 : PRO_A;
     PcgPart part;
     pcgPartLoad<KIND>(pm, ps, n, part);
@@ -1046,8 +1053,8 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
         if (y > ghc) fb = faceI(bi, o0.i);
     }
     bool skC = inb && y > ghc;
+    float snc = skC ? snv(o0.z, o0.s) : 0.0f;  // s_new of the current plane
     double acc = 0.0;
-    int slot = 0;  // (z - z0) % 3
     // plane z from oc, its front plane from of; plane z + 3 is read into o3n
     auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, Halo& hc, Halo& hn) __attribute__((always_inline)) {
         // the halo (used next plane) goes first, so waiting for it does not
@@ -1056,22 +1063,14 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
         loadOwn(min(z + 3, zlast), o3n);
         const bool inZ = z + 1 < Z;
         const bool skF = inb && inZ && y > oc.gf;
-        const float snc = skC ? snv(oc.z, oc.s) : 0.0f;
         tS[buf][ty + 1][tx + 1] = snc;
         tI[buf][ty + 1][tx + 1] = skC ? oc.i : (inb ? INF : oc.i);
-        // this plane's sky values (own, front, halo) in the short form's range
-        bool bad = (skC && !faceValueOk(oc.i)) || (skF && !faceValueOk(of.i));
         if (hasH) {
             const bool sky = hIn && hy > hc.g;
             tS[buf][hrow][hcol] = sky ? snv(hc.z, hc.s) : 0.0f;
             tI[buf][hrow][hcol] = sky ? hc.i : (hIn ? INF : hc.i);
-            bad = bad || (sky && !faceValueOk(hc.i));
         }
-        if (__builtin_amdgcn_ballot_w64(bad)) flagBuf[slot] = 1;
-        if (tx == 0 && ty == 0) flagBuf[slot == 2 ? 0 : slot + 1] = 0;
         __syncthreads();
-        const bool exact = __builtin_amdgcn_readfirstlane(flagBuf[slot]) != 0;
-        slot = slot == 2 ? 0 : slot + 1;
         const float snf = skF ? snv(of.z, of.s) : 0.0f;
         float fcur = 0.0f;
         if (skC) {
@@ -1112,6 +1111,7 @@ KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __res
             bstf(wA, ob, pOff(z), 0.0f);
         }
         snb = snc;
+        snc = snf;
         fb = fcur;
         skB = skC;
         skC = skF;
@@ -1141,7 +1141,7 @@ hipError_t launch_pcg_A(EnvState& e, int it) {
     auto k = it == 0 ? (trunc ? k_pcg_A<true, true> : k_pcg_A<true, false>)
                      : (trunc ? k_pcg_A<false, true> : k_pcg_A<false, false>);
     klaunch(e, k, zg.grid, B3, 0, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho, e.pcg[4], pm, ps,
-            np, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it);
+            np, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it, &e.sc->face_exact);
     return hipGetLastError();
 }
 

@@ -575,8 +575,8 @@ hipError_t launch_microphysics(EnvState& e, float dt, float speed) {
     klaunch(e, k_microphysics, grid3(e), B3, 0, geom(e), a, dt, speed, e.gm);
     return hipGetLastError();
 }
-// Fixed-order sum of the region partials into 33 floats (written to sc->pad area
-// is not large enough, so into the tail of the region buffer).
+// Fixed-order sum of the region partials into 33 floats at the tail of the
+// region buffer.
 KL void k_region_finalize(double* region, unsigned nb, float* out) {
     __shared__ double sh[4];
     for (int k = 0; k < 33; k++) {
