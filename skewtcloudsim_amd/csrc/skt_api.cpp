@@ -170,12 +170,33 @@ struct AdvField {
     const float* prof;
     Bounds b;
 };
-int advectFields(EnvState& e, const AdvField* fs, int n, float dt) {
+// The substep count from the current velocities, launched ahead of the
+// advection that uses it: the count is copied to pinned host memory behind an
+// event, so the host reads it after queueing the work in between (the
+// diffusion of the advected fields) instead of draining the device. Valid
+// until the velocities change (advectFields takes it while mv_ready holds).
+int startMaxvel(EnvState& e, float dt) {
     TL(e, TC_MAXVEL, launch_maxvel(e, dt));
-    int rc = readScalars(e);
-    if (rc) return rc;
-    const int nsub = e.sc_host->nsub;
-    const float dts = e.sc_host->dtsub;
+    HIPCHK(e, hipMemcpyAsync(e.mv_snap, &e.sc->maxvel, sizeof(MaxvelSnap), hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(e, hipEventRecord(e.mv_ev, e.stream));
+    e.mv_ready = true;
+    e.mv_dt = dt;
+    return SKT_OK;
+}
+int advectFields(EnvState& e, const AdvField* fs, int n, float dt) {
+    int nsub;
+    float dts;
+    if (e.mv_ready && e.mv_dt == dt) {
+        HIPCHK(e, hipEventSynchronize(e.mv_ev));
+        nsub = e.mv_snap->nsub;
+        dts = e.mv_snap->dtsub;
+    } else {
+        TL(e, TC_MAXVEL, launch_maxvel(e, dt));
+        int rc = readScalars(e);
+        if (rc) return rc;
+        nsub = e.sc_host->nsub;
+        dts = e.sc_host->dtsub;
+    }
     for (int j = 0; j < n; j++) {
         if (e.stats.n_advect < 16) e.stats.substeps[e.stats.n_advect] = nsub;
         e.stats.n_advect++;
@@ -321,8 +342,16 @@ int microSky(EnvState& e, float dt, float speed) {
         if (rc_) return rc_;  \
     } while (0)
 
+// Clears the launched-ahead substep count when update() leaves, also on errors.
+struct MaxvelScope {
+    EnvState& e;
+    explicit MaxvelScope(EnvState& s) : e(s) { e.mv_ready = false; }
+    ~MaxvelScope() { e.mv_ready = false; }
+};
+
 int update(EnvState& e, float dt, float speed) {
     const float ds = dt * speed;
+    MaxvelScope mvs(e);
     e.stats.n_advect = 0;
     RC(editorData(e));
     const float irr = irradianceOf(e.params);
@@ -332,8 +361,12 @@ int update(EnvState& e, float dt, float speed) {
     const Bounds bT{BC_CUSTOM, BC_CUSTOM, BC_CUSTOM}, bU{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET},
         bV{BC_DIRICHLET, BC_DIRICHLET, BC_DIRICHLET}, bW{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET},
         bQ{BC_DIRICHLET, BC_DIRICHLET, BC_DIRICHLET}, bQv{BC_CUSTOM, BC_CUSTOM, BC_CUSTOM};
+    // the velocities are unchanged until buoyancy: theta's substep count is
+    // launched ahead of its diffusion
+    RC(startMaxvel(e, ds));
     RC(diffuse(e, SKT_POTTEMP, 0, ds));
     RC(advect(e, SKT_POTTEMP, e.f[SKT_ISEN_TEMP], bT, ds));
+    e.mv_ready = false;
     TL(e, TC_OTHER, launch_buoyancy(e, ds));
     {
         const int f[3] = {SKT_VELX, SKT_VELY, SKT_VELZ}, t[3] = {2, 3, 4};
@@ -346,7 +379,10 @@ int update(EnvState& e, float dt, float speed) {
     RC(pressureProject(e, ds, 1));
     // moisture (Game/environment.cu:508-533): each field is diffused, rained
     // out and advected on its own; only the precipitation fall speed reads
-    // the vapour field, which is final by then
+    // the vapour field, which is final by then. The velocities are final for
+    // the step, so both moisture advections take one substep count, launched
+    // ahead of the moisture diffusion.
+    RC(startMaxvel(e, ds));
     {
         const int f[6] = {SKT_QV, SKT_QW, SKT_QC, SKT_QR, SKT_QS, SKT_QI}, t[6] = {1, 5, 5, 5, 5, 5};
         RC(diffuseFields(e, f, t, 6, ds));
@@ -394,6 +430,8 @@ void freeAll(EnvState& e) {
     }
     for (auto ev : e.evpool) (void)hipEventDestroy(ev);
     if (e.sc_host) (void)hipHostFree(e.sc_host);
+    if (e.mv_snap) (void)hipHostFree(e.mv_snap);
+    if (e.mv_ev) (void)hipEventDestroy(e.mv_ev);
     if (e.pcg_snap) (void)hipHostFree(e.pcg_snap);
     for (int k = 0; k < 3; k++)
         if (e.snap_ev[k]) (void)hipEventDestroy(e.snap_ev[k]);
@@ -482,6 +520,9 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
         return bail(SKT_ERR_NOMEM);
     for (int k = 0; k < 3; k++)
         if (hipEventCreateWithFlags(&e.snap_ev[k], hipEventDisableTiming) != hipSuccess) return bail(SKT_ERR_HIP);
+    if (hipHostMalloc((void**)&e.mv_snap, sizeof(MaxvelSnap), hipHostMallocDefault) != hipSuccess)
+        return bail(SKT_ERR_NOMEM);
+    if (hipEventCreateWithFlags(&e.mv_ev, hipEventDisableTiming) != hipSuccess) return bail(SKT_ERR_HIP);
     // environmentGPU ctor defaults (Game/environment.cu:151-168)
     for (int k = 0; k < SKT_FIELD_COUNT; k++)
         if (hipMemsetAsync(e.f[k], 0, sizeof(float) * fieldLen(e, k), e.stream) != hipSuccess) return bail(SKT_ERR_HIP);

@@ -1,6 +1,7 @@
 // Internal state of one environment and the launch interface between the
 // host orchestration (skt_api.cpp) and the kernels (skt_kernels.hip).
 #pragma once
+#include <cstddef>
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <string>
@@ -36,6 +37,16 @@ struct Scalars {
     int face_exact;  // a sky 1/density of this solve lies outside the short face form's range
     int pad[23];
 };
+
+// Host copy of the advection substep fields of Scalars (maxvel, nsub, dtsub).
+struct MaxvelSnap {
+    float maxvel;
+    int nsub;
+    float dtsub;
+};
+static_assert(offsetof(Scalars, nsub) == offsetof(Scalars, maxvel) + offsetof(MaxvelSnap, nsub) &&
+                  offsetof(Scalars, dtsub) == offsetof(Scalars, maxvel) + offsetof(MaxvelSnap, dtsub),
+              "MaxvelSnap mirrors the substep fields of Scalars");
 
 // Regions of the per-block reduction partials (each maxBlocks doubles).
 enum { PART_DIV = 0, PART_START, PART_A, PART_BMAX, PART_BSUM, PART_DG_W, PART_DG_L, PART_DG_R, PART_COUNT };
@@ -113,6 +124,11 @@ struct EnvState {
     PcgState* pcg_snap = nullptr;
     hipEvent_t snap_ev[3] = {nullptr, nullptr, nullptr};
     bool pcg_stats_pending = false;
+    // substep count launched ahead of an advection (startMaxvel)
+    MaxvelSnap* mv_snap = nullptr;
+    hipEvent_t mv_ev = nullptr;
+    bool mv_ready = false;
+    float mv_dt = 0.0f;
     // events armed for the next klaunch (a timed single-kernel launch)
     hipEvent_t kevA = nullptr, kevB = nullptr;
 };
