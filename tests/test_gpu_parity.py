@@ -644,3 +644,23 @@ def test_pressure_solve_out_of_range_density():
     m = sky_mask(NX, NY, NZ)
     assert_fields(env, orc, ("PRESSURE_PROJ",), m, l2=1e-4, maxabs=1e-3, tag="pcg exact faces")
     env.close()
+
+
+@pytest.mark.parametrize("shape", [(8, 2, 3), (65, 9, 5), (70, 33, 4)])
+def test_edge_shapes_steps(shape):
+    """Shapes at the edges of the launch layouts: two levels (one sky level),
+    x just past one 64-lane tile, ny not a multiple of the 8-row tiles or the
+    16-level column chunks, nz below one z group. Three steps against the
+    oracle."""
+    nx, ny, nz = shape
+    print(f"edge shape {shape}", flush=True)
+    env, orc = make_pair(nx, ny, nz, dx=100.0, seed=9, n_bubbles=2)
+    m = sky_mask(nx, ny, nz)
+    for s in range(3):
+        env.updateGPU(0.5, 1.0)
+        orc.step(0.5, 1.0)
+        assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+        tol = 1e-5 if s == 0 else 1e-4
+        assert_fields(env, orc, SKY + GROUND, m, l2=tol, maxabs=50 * tol, tag=f"{shape} step{s}")
+    env.close()
+    orc.close()
