@@ -159,10 +159,11 @@ ZGrid zgrid(const EnvState& e);
 // Blocks the device holds at once of the pressure operator kernel and of a
 // diffusion sweep (0 if unknown).
 struct ResidentBlocks {
-    long pcgA, diffuse;
+    long pcgA, diffuse;  // blocks the device holds at once (0: unknown)
+    int cus;             // compute units (0: unknown)
 };
 ResidentBlocks resident_blocks(int device);
-// Fills gZ, gA and gD (one round of resident blocks each) and gB.
+// Fills gZ, gA and gD (at most one round of resident blocks each) and gB.
 void setup_grids(EnvState& e, ResidentBlocks rb);
 hipError_t launch_fill(EnvState& e, float* a, float v, long n);
 hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst);

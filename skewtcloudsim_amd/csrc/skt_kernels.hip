@@ -54,7 +54,10 @@ void setup_grids(EnvState& e, ResidentBlocks rb) {
         if (cap > 0) a = std::min(a, cap);
         return zgridChunks(e, a / cols);
     };
-    e.gA = round(rb.pcgA);
+    // The pressure operator takes at most five blocks per CU (of the seven it
+    // could hold): on MI355X its launch then measured 4 % faster at 256 x 128
+    // x 256 and 320 x 128 x 256, and no slower at the other shapes timed.
+    e.gA = round(rb.cus > 0 ? std::min(rb.pcgA, 5L * rb.cus) : rb.pcgA);
     // diffusion: tiles of 64 x 8; a chunk is at most 64 planes (one bit per
     // plane marks the cells next to ground)
     {
@@ -1145,18 +1148,20 @@ hipError_t launch_pcg_A(EnvState& e, int it) {
     return hipGetLastError();
 }
 
-// Blocks of a kernel the device holds at once. The pressure operator's grid
-// is one such round and a diffusion sweep's grid per field is one (a second,
-// partly filled round would nearly double a launch's time).
+// Blocks of a kernel the device holds at once. A diffusion sweep's grid per
+// field is one such round and the pressure operator's grid at most one (a
+// second, partly filled round would nearly double a launch's time).
 template <class K>
-static long resident_blocks(K kernel, int device) {
-    int nb = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess) return 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+static long resident_blocks(K kernel, int device, int cus) {
+    int nb = 0;
+    if (cus <= 0 || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess) return 0;
     return (long)nb * cus;
 }
 ResidentBlocks resident_blocks(int device) {
-    return ResidentBlocks{resident_blocks(k_pcg_A<false, true>, device), resident_blocks(k_diffuse, device)};
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 0;
+    return ResidentBlocks{resident_blocks(k_pcg_A<false, true>, device, cus), resident_blocks(k_diffuse, device, cus),
+                          cus};
 }
 
 // Prologue PRO_B, then updatePandDiv (Game/kernelSky.cu:955-985) + the
