@@ -269,7 +269,10 @@ KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const flo
     }
     // Sky cells next to ground (marked in slowA / slowB, one bit per plane of
     // the chunk): the sweep reads only 'in', so they can follow the fast
-    // cells. Each neighbour is tested and takes the ground or sides condition
+    // cells. The block's four waves share them: wave w takes the marked planes
+    // z0 + w, z0 + w + 4, ... of all eight rows of its columns, so a row of
+    // marked cells (all of them in one wave) does not hold the block for the
+    // whole chunk's length. Each neighbour is tested and takes the ground or sides condition
     // (nbv); for the potential temperature a ground neighbour reads the
     // potential temperature of the ground temperature at this cell's pressure
     // (:95-100). Neighbour positions are clamped into the domain, as in the tile.
@@ -309,8 +312,15 @@ KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const flo
             fd.out[pz + xc + X * y] = (vc + k * (l + r + u + d + fr + bb)) / den;
         }
     };
-    slowCells(slowA, ya);
-    slowCells(slowB, yb4);
+    __shared__ unsigned long long sMask[8][64];
+    sMask[ty][tx] = slowA;
+    sMask[ty + 4][tx] = slowB;
+    __syncthreads();
+    const unsigned long long mine = 0x1111111111111111ull << ty;
+    for (int r = 0; r < 8; r++) {
+        const unsigned long long m = sMask[r][tx] & mine;
+        if (m) slowCells(m, yb + r);
+    }
 }
 hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k) {
     const ZGrid zg = e.gD;
