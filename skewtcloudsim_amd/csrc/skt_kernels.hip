@@ -447,72 +447,105 @@ __device__ __forceinline__ float ppmCell(const float* q, const float* o, float v
     const bool gNeu_ = b.ground == BC_NEUMANN && gsrc != nullptr;         \
     auto graw = [&](long gi) { return gNeu_ ? gsrc[gi] : 0.0f; }
 
-// x pass: the stencil runs along the row. Cells with y > ghm2 (largest
-// ground height over the 5x5 columns around) see no ground in the stencil.
-constexpr int ZB_PPM = 4;
-template <int ZB>
-KL void k_ppm_x(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, const float* __restrict__ cust) {
-    PPM_FIELD_SETUP;
-    if (!inb) return;
-    const int X = G.X;
+// x pass as a march along the rows: each wave owns one row (y, z) of a
+// plane at a time and walks its 64-cell segments from x = 0 upward. The flux
+// through a cell's left face is the right-face flux of the cell to its left
+// (same face velocity, same three stencil values, bit for bit): lane l takes
+// lane l - 1's, lane 0 takes lane 63's from the segment before. Only x = 0
+// and a cell whose left neighbour is ground (its left face velocity takes the
+// ground condition) evaluate it directly, so a row of X cells evaluates about
+// X field fluxes instead of 2 X. Cells with y > ghm2 (largest ground height
+// over the 5x5 columns around) see no ground in the stencil. A block is four
+// rows (one per wave) over a chunk of zc planes of one field; the loads of up
+// to XSEG_PPM segments are issued before any is used.
+constexpr int XSEG_PPM = 2;
+KL void k_ppm_xr(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, const float* __restrict__ cust) {
+    const unsigned lin = xcdRemap(blockIdx.x, gridDim.x);
+    const PpmField& fd = pb.f[lin / nbf];
+    const unsigned rem = lin % nbf;
+    const int gy = (G.Y + 3) >> 2;
+    const int y = (int)(rem % (unsigned)gy) * 4 + (int)threadIdx.y;
+    const int z0 = (int)(rem / (unsigned)gy) * zc, z1 = min(G.Z, z0 + zc);
+    if (y >= G.Y || z0 >= z1) return;  // whole waves (one row per wave)
+    const float* __restrict__ in = fd.in;
+    float* __restrict__ out = fd.out;
+    const float* __restrict__ vel = fd.vel;
+    const float* __restrict__ prof = fd.prof;
+    const Bounds b = fd.b;
+    // value that stands at a ground position: the field's own ground value
+    // or 0 (read only under a NEUMANN ground condition)
+    const float* __restrict__ gsrc = fd.gsrc;
+    const bool gNeu = b.ground == BC_NEUMANN && gsrc != nullptr;
+    const int X = G.X, tx = (int)threadIdx.x;
     const long XY = (long)X * G.Y;
-    unsigned cx[5];
-#pragma unroll
-    for (int s = 0; s < 5; s++) cx[s] = clampi(x + s - 2, 0, X - 1);
     const unsigned rowy = X * y;
     const float pv = prof ? prof[y] : 0.0f, cv = cust[y];
-    auto load = [&](int z, float* Q, float& VC, float& VM, int& gm, int& gc) {
+    const int nseg = (X + 63) >> 6;
+    for (int z = z0; z < z1; z++) {
         const float* pl = in + XY * z + rowy;
         const float* vl = vel + XY * z + rowy;
+        const int* gz = G.gh + X * z;
+        const int* gmz = G.ghm2 + X * z;
+        float carry = 0.0f;  // right-face flux of the last cell of the segment before
+        for (int k0 = 0; k0 < nseg; k0 += XSEG_PPM) {
+            float Q[XSEG_PPM][5], VC[XSEG_PPM], VM[XSEG_PPM];
+            int GM[XSEG_PPM], GC[XSEG_PPM];
 #pragma unroll
-        for (int s = 0; s < 5; s++) Q[s] = pl[cx[s]];
-        VC = vl[(unsigned)x];
-        VM = vl[cx[1]];
-        gm = (G.ghm2 + X * z)[(unsigned)x];
-        gc = (G.gh + X * z)[(unsigned)x];
-    };
-    auto plane = [&](int z, const float* Q, float VC, float VM, int gm, int gc) {
-        if (y <= gc) return;
-        float q[5], o[5];
-        float vm;
-        if (y > gm) {
+            for (int j = 0; j < XSEG_PPM; j++) {
+                // lanes past the row (and segments past the last) read its last cell
+                const int xr = min((k0 + j) * 64 + tx, X - 1);
 #pragma unroll
-            for (int s = 0; s < 5; s++) {
-                const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
-                q[s] = outp ? bcv(b.sides, Q[s], pv) : Q[s];
-                o[s] = outp ? (b.sides == BC_DIRICHLET ? 0.0f : 1.0f) : 1.0f;
+                for (int s = 0; s < 5; s++) Q[j][s] = pl[clampi(xr + s - 2, 0, X - 1)];
+                VC[j] = vl[xr];
+                VM[j] = vl[max(xr - 1, 0)];
+                GM[j] = gmz[xr];
+                GC[j] = gz[xr];
             }
-            vm = x == 0 ? bcv(b.sides, VC, cv) : VM;
-        } else {
-            const int* gz = G.gh + X * z;
-            int gh[5];
 #pragma unroll
-            for (int s = 0; s < 5; s++) gh[s] = gz[cx[s]];
+            for (int j = 0; j < XSEG_PPM; j++) {
+                if (k0 + j >= nseg) break;  // uniform
+                const int x = (k0 + j) * 64 + tx;
+                const bool sky = x < X && y > GC[j];
+                float q[5], o[5], vm = 0.0f, Fr = 0.0f;
+                bool leftSky = false;
+                if (sky) {
+                    if (y > GM[j]) {
 #pragma unroll
-            for (int s = 0; s < 5; s++) {
-                const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
-                const bool gnd = y <= gh[s];
-                q[s] = nbv(gnd ? graw(XY * z + rowy + cx[s]) : Q[s], outp, gnd, pv, b);
-                o[s] = unitv(outp, gnd, b);
+                        for (int s = 0; s < 5; s++) {
+                            const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
+                            q[s] = outp ? bcv(b.sides, Q[j][s], pv) : Q[j][s];
+                            o[s] = outp ? (b.sides == BC_DIRICHLET ? 0.0f : 1.0f) : 1.0f;
+                        }
+                        vm = x == 0 ? bcv(b.sides, VC[j], cv) : VM[j];
+                        leftSky = x > 0;
+                    } else {
+                        int gh[5];
+#pragma unroll
+                        for (int s = 0; s < 5; s++) gh[s] = gz[clampi(x + s - 2, 0, X - 1)];
+#pragma unroll
+                        for (int s = 0; s < 5; s++) {
+                            const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
+                            const bool gnd = y <= gh[s];
+                            const float raw =
+                                gnd ? (gNeu ? gsrc[XY * z + rowy + clampi(x + s - 2, 0, X - 1)] : 0.0f) : Q[j][s];
+                            q[s] = nbv(raw, outp, gnd, pv, b);
+                            o[s] = unitv(outp, gnd, b);
+                        }
+                        vm = x == 0 ? bcv(b.sides, VC[j], cv) : (y <= gh[1] ? bcv(b.ground, VC[j], cv) : VM[j]);
+                        leftSky = x > 0 && y > gh[1];
+                    }
+                    Fr = ppmFaceR(q, VC[j], dt, dx);
+                }
+                // every lane of the wave takes part in both exchanges
+                const float up = __shfl_up(Fr, 1);
+                const float last = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Fr), 63));
+                if (sky) {
+                    const float Fl = leftSky ? (tx > 0 ? up : carry) : ppmFaceL(q, vm, dt, dx);
+                    out[XY * z + rowy + x] = ppmFinish(q, o, VC[j], vm, Fr, Fl, dt, dx);
+                }
+                carry = last;
             }
-            vm = x == 0 ? bcv(b.sides, VC, cv) : (y <= gh[1] ? bcv(b.ground, VC, cv) : VM);
         }
-        (out + XY * z + rowy)[(unsigned)x] = ppmCell(q, o, VC, vm, dt, dx);
-    };
-    int zg = z0;
-    for (; zg + ZB <= z1; zg += ZB) {
-        float Q[ZB][5], VC[ZB], VM[ZB];
-        int GM[ZB], GC[ZB];
-#pragma unroll
-        for (int j = 0; j < ZB; j++) load(zg + j, Q[j], VC[j], VM[j], GM[j], GC[j]);
-#pragma unroll
-        for (int j = 0; j < ZB; j++) plane(zg + j, Q[j], VC[j], VM[j], GM[j], GC[j]);
-    }
-    for (; zg < z1; zg++) {
-        float Q[5], VC, VM;
-        int gm, gc;
-        load(zg, Q, VC, VM, gm, gc);
-        plane(zg, Q, VC, VM, gm, gc);
     }
 }
 
@@ -609,6 +642,7 @@ KL void k_ppm_ym(Geom G, int yc, unsigned nyc, unsigned nbf, PpmBatch pb, float 
     }
 }
 
+constexpr int ZB_PPM = 4;  // planes of the z pass whose loads go together
 // z pass: the stencil runs along this thread's march; a window of planes
 // zg-2 .. zg+ZB+1 (values and ground heights) stays in registers, so the
 // boundary rule needs no extra loads.
@@ -685,9 +719,14 @@ KL void k_ppm_z(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, c
 hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) {
     const ZGrid zg = zgrid(e);
     const dim3 grid(zg.nblocks * pb.n);
-    if (dir == 0)
-        klaunch(e, k_ppm_x<ZB_PPM>, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
-                e.f[SKT_DEFAULT_VELX]);
+    if (dir == 0) {
+        // rows of four per block, chunks of zc planes: about 2048 blocks per field
+        const long gy = (e.Y + 3) / 4;
+        const long nzc = std::max(1L, std::min((long)e.Z, 2048 / gy));
+        const int zc = (int)((e.Z + nzc - 1) / nzc);
+        const unsigned nbf = (unsigned)(gy * ((e.Z + zc - 1) / zc));
+        klaunch(e, k_ppm_xr, dim3(nbf * pb.n), B3, 0, geom(e), zc, nbf, pb, dt, e.dx, e.f[SKT_DEFAULT_VELX]);
+    }
     else if (dir == 1) {
         const unsigned nyc = (unsigned)((e.Y + YC_PPM - 1) / YC_PPM);
         const unsigned nbf = (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * nyc;
