@@ -113,6 +113,10 @@ int zipExtract(const std::string& z, const ZipMember& m, std::string& out) {
         return fail(SKT_ERR_INPUT, "corrupt zip local header for " + m.name);
     const size_t data = (size_t)m.local + 30 + rd16(b + m.local + 26) + rd16(b + m.local + 28);
     if ((uint64_t)data + m.csize > n) return fail(SKT_ERR_INPUT, "truncated zip member " + m.name);
+    // deflate expands at most about 1032:1, so a larger stated size is corrupt
+    // (and would only allocate memory that inflate can never fill)
+    if (m.method == 8 && (uint64_t)m.usize > (uint64_t)m.csize * 1032u + 1024u)
+        return fail(SKT_ERR_INPUT, "corrupt zip member size for " + m.name);
     out.assign(m.usize, '\0');
     if (m.method == 0) {
         if (m.csize != m.usize) return fail(SKT_ERR_INPUT, "corrupt stored zip member " + m.name);

@@ -147,6 +147,16 @@ def test_dwd_archive_errors(tmp_path):
     r.readKNMIFile(str(stored))
     assert r.data.dataSize == gen.KNMI_ROWS
     assert lib.skt_input_last_error() is not None
+    # a deflated member whose directory entry claims more than deflate can
+    # expand to is rejected before anything is allocated for it
+    raw = bytearray(open(gen.KNMI_PATH, "rb").read())
+    cd = raw.find(b"PK\x01\x02")
+    assert cd > 0 and raw[cd + 46:cd + 46 + 28] == b"SynchronizedSoundingData.xml"
+    raw[cd + 24:cd + 28] = (0xFFFFFF00).to_bytes(4, "little")
+    huge = tmp_path / "huge.mwx"
+    huge.write_bytes(bytes(raw))
+    with pytest.raises(skt.SktError, match="corrupt zip member size"):
+        skt.ReadTable().readKNMIFile(str(huge))
 
 
 def test_table_capacity_reports_rows():
