@@ -196,19 +196,35 @@ hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed,
     return hipGetLastError();
 }
 
-// advectPrecipGPU, Game/kernelSky.cu:576-657: one thread per column walks y
-// upwards; each level reads its own and the next level's pre-step value
-// (the column snapshot of the reference), so the update can be in place.
+// advectPrecipGPU, Game/kernelSky.cu:576-657. Each level reads its own and
+// the next level's pre-step value (the column snapshot of the reference), so
+// the update can be in place. A block holds 64 columns; its four waves walk
+// y upwards through the four chunks of Y / 4 levels of those columns. Every
+// thread reads the pre-step value of the level above its chunk (which the
+// chunk above overwrites) before the block's barrier, and writes only after
+// it, so the chunks run side by side with the results of one walk per column.
+constexpr int PRECIP_CHUNKS = 4;
 KL void k_precip(Geom G, PrecipBatch pbt, const float* __restrict__ th, const float* __restrict__ p,
                  const float* __restrict__ qv, const float* __restrict__ gP, float dx, Gammas gm) {
-    const long g = blockIdx.x * 256L + threadIdx.x;
     const float dts = pbt.dts[blockIdx.y];
-    if (g >= (long)G.X * G.Z || dts == 0.0f) return;
+    if (dts == 0.0f) return;  // the whole block
+    const long g = blockIdx.x * 64L + threadIdx.x;
+    const bool col = g < (long)G.X * G.Z;
+    const long gc = col ? g : 0;
     float* q = pbt.q[blockIdx.y];
     const int type = pbt.type[blockIdx.y];
-    const int x = (int)(g % G.X), zc = (int)(g / G.X);
+    const int x = (int)(gc % G.X), zc = (int)(gc / G.X);
     const float kap = kc::Rsd / kc::Cpd;
-    const float gp = gP[g];
+    const float gp = gP[gc];
+    int y0 = G.gh[gc] + 1;
+    if (y0 < 0) y0 = 0;
+    const int yc = (G.Y + PRECIP_CHUNKS - 1) / PRECIP_CHUNKS;
+    const int ylo = (int)threadIdx.y * yc, yhi = min(G.Y, ylo + yc);
+    const int ya = max(ylo, y0);
+    const bool work = col && ya < yhi;
+    const float qTop = work && yhi < G.Y ? q[gidx(G, x, yhi, zc)] : 0.0f;
+    __syncthreads();
+    if (!work) return;
     // per level: Q, theta, p, qv; levels y+1 and y+2 are loaded ahead
     struct Lv {
         float q, t, pr, v;
@@ -220,25 +236,22 @@ KL void k_precip(Geom G, PrecipBatch pbt, const float* __restrict__ th, const fl
         const float3 v3 = d_fallVel(L.q, L.q, L.q, den, type, gm.R48, gm.S425, gm.S425);
         return type == 0 ? v3.x : (type == 1 ? v3.y : v3.z);
     };
-    int y0 = G.gh[g] + 1;
-    if (y0 < 0) y0 = 0;
-    if (y0 >= G.Y) return;
-    const long i0 = gidx(G, x, y0, zc);
+    const long i0 = gidx(G, x, ya, zc);
     auto load = [&](int y, Lv& L) {
-        if (y < G.Y) {
-            const long j = i0 + (long)(y - y0) * G.X;
-            L.q = q[j];
+        if (y <= yhi && y < G.Y) {
+            const long j = i0 + (long)(y - ya) * G.X;
+            L.q = y == yhi ? qTop : q[j];
             L.t = th[j];
             L.pr = p[j];
             L.v = qv[j];
         }
     };
     Lv l0{0.0f, 0.0f, 0.0f, 0.0f}, l1 = l0, l2 = l0;
-    load(y0, l0);
-    load(y0 + 1, l1);
+    load(ya, l0);
+    load(ya + 1, l1);
     float fv = fvAt(l0);
     long i = i0;
-    for (int y = y0; y < G.Y; y++, i += G.X) {
+    for (int y = ya; y < yhi; y++, i += G.X) {
         load(y + 2, l2);
         const bool up = y < G.Y - 1;
         const float Qc = l0.q, Qu = up ? l1.q : 0.0f;
@@ -253,7 +266,7 @@ KL void k_precip(Geom G, PrecipBatch pbt, const float* __restrict__ th, const fl
     }
 }
 hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt) {
-    klaunch(e, k_precip, dim3((unsigned)((e.G + 255) / 256), pbt.n), dim3(256), 0, geom(e), pbt,
+    klaunch(e, k_precip, dim3((unsigned)((e.G + 63) / 64), pbt.n), dim3(64, PRECIP_CHUNKS), 0, geom(e), pbt,
             e.f[SKT_POTTEMP], e.f[SKT_PRESSURE], e.f[SKT_QV], e.f[SKT_GROUND_P], e.dx, e.gm);
     return hipGetLastError();
 }
