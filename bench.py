@@ -191,7 +191,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the step is single-domain, one GPU per "
-              "process. See DESIGN.md for how the domain would shard.", file=sys.stderr)
+              "process (on several GPUs the domain would split into z slabs, one per GPU).", file=sys.stderr)
         sys.exit(2)
 
     import skewtcloudsim_amd as skt
