@@ -3,6 +3,7 @@
 // state, boundary velocity reset, bulk microphysics, masks and init helpers.
 // Same layout and launch conventions as skt_kernels.hip.
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <algorithm>
 #include "skt_internal.h"
 #include "skt_micro.h"
@@ -274,77 +275,120 @@ hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt) {
 // -------------------------------------------------------------- buoyancy ---
 // buoyancyGPU, Game/kernelSky.cu:1061-1247 (bounds {D,D,D}); the ice test
 // 'T < 0' is on Kelvin and never holds (quirk A17).
-KL void k_buoyancy(Geom G, float* __restrict__ v, const float* __restrict__ th, const float* __restrict__ qv,
-                   const float* __restrict__ Qr, const float* __restrict__ Qs, const float* __restrict__ Qi,
-                   const float* __restrict__ p, const float* __restrict__ gP, const float* __restrict__ dT,
-                   const float* __restrict__ dQ, float dt) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
-    const Bounds b{BC_DIRICHLET, BC_DIRICHLET, BC_DIRICHLET};
+// A march up the columns: each thread walks a chunk of YC_BUOY levels of its
+// (x, z) column. A level's temperature theta (p / p_ground)^kappa and its
+// saturation mixing ratio are computed once and carried: the cell below uses
+// them as its upper neighbour's, the cell itself as its own, the cell above
+// as its lower neighbour's (the same expressions on the same values as the
+// per-cell form, so the same bits). With the bounds {D,D,D} a neighbour reads
+// its raw value when it is a sky cell and 0 otherwise; the level above a sky
+// cell inside the domain is sky. Blocks are 64 (x) x 4 (z) columns of one
+// chunk; a chunk that starts above the ground reads the level below it.
+constexpr int YC_BUOY = 16;
+KL void k_buoyancy(Geom G, int yc, unsigned nyc, float* __restrict__ v, const float* __restrict__ th,
+                   const float* __restrict__ qv, const float* __restrict__ Qr, const float* __restrict__ Qs,
+                   const float* __restrict__ Qi, const float* __restrict__ p, const float* __restrict__ gP, float dt) {
+    const unsigned lin = blockIdx.x;
+    const int chunk = (int)(lin % nyc);
+    const unsigned tile = lin / nyc;
+    const int gx = (G.X + 63) >> 6;
+    const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
+    const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
+    if (x >= G.X || z >= G.Z) return;
+    const int X = G.X, Y = G.Y;
+    const long col = (long)X * Y * z + x;  // cell (x, 0, z)
+    const long gc = x + (long)X * z;
+    const int gh = G.gh[gc];
+    const float gps = gP[gc];
     const float kap = kc::Rsd / kc::Cpd;
-    const long i = gidx(G, x, y, z);
-    const float cur = v[i];
-    const bool up = y < G.Y - 1;
-    const bool down = y > 0 && !isGround(G, x, y - 1, z);
-    const float gps = gP[x + (long)G.X * z];
-    const float psC = p[i];
-    const float psU = up ? p[i + G.X] : psC;
-    const float psD = down ? p[i - G.X] : psC;
-    const bool l = skyAt(G, x - 1, y, z), r = skyAt(G, x + 1, y, z);
-    const bool bk = skyAt(G, x, y, z - 1), fw = skyAt(G, x, y, z + 1);
-    const int validEnv = int(l) + int(r) + int(bk) + int(fw);
-    if (validEnv == 0) return;
-    const float QP = qv[i];
-    const float Qenv = (extv(G, qv, dQ, b, x + 1, y, z) + extv(G, qv, dQ, b, x - 1, y, z) + extv(G, qv, dQ, b, x, y, z + 1) +
-                        extv(G, qv, dQ, b, x, y, z - 1)) / validEnv;
-    const float QenvUp = extv(G, qv, dQ, b, x, y + 1, z);
-    const float QenvDown = extv(G, qv, dQ, b, x, y - 1, z);
-    const float thD = extv(G, th, dT, b, x, y - 1, z), thU = extv(G, th, dT, b, x, y + 1, z);
-    const float TDown = thD * powf(psD / gps, kap);
-    const float T = th[i] * powf(psC / gps, kap);
-    const float TUp = thU * powf(psU / gps, kap);
-    float Tad = th[i], TadU = th[i], TadD = th[i];
-    const float Tenv = (extv(G, th, dT, b, x + 1, y, z) + extv(G, th, dT, b, x - 1, y, z) + extv(G, th, dT, b, x, y, z + 1) +
-                        extv(G, th, dT, b, x, y, z - 1)) / validEnv;
-    const float TenvUp = thU, TenvDown = thD;
-    if (down) {
-        if ((T < 0.0f && QP >= d_wi((TDown - 273.15f), psD)) || QP >= d_ws((TDown - 273.15f), psD)) {
-            TadD = T - d_MLR(T - 273.15f, psC) * (psC - psD);
-            TadD = TadD * powf(gps / psD, kap);
+    // ground heights of the four horizontal neighbour columns (INT_MAX
+    // outside the domain: no level there is sky)
+    const int ghL = x > 0 ? G.gh[gc - 1] : INT_MAX, ghR = x < X - 1 ? G.gh[gc + 1] : INT_MAX;
+    const int ghB = z > 0 ? G.gh[gc - X] : INT_MAX, ghF = z < G.Z - 1 ? G.gh[gc + X] : INT_MAX;
+    const int y0 = max(chunk * yc, gh + 1), y1 = min(Y, chunk * yc + yc);
+    if (y0 >= y1) return;
+    struct Lev {
+        float th, p, q, T, ws;
+    };
+    auto level = [&](int l, Lev& L) __attribute__((always_inline)) {
+        const long j = col + (long)X * l;
+        L.th = th[j];
+        L.p = p[j];
+        L.q = qv[j];
+        L.T = L.th * powf(L.p / gps, kap);
+        L.ws = d_ws((L.T - 273.15f), L.p);
+    };
+    Lev lm{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, lc = lm, lp = lm;  // levels y - 1, y, y + 1
+    if (y0 - 1 >= 0 && y0 - 1 > gh) level(y0 - 1, lm);
+    level(y0, lc);
+    for (int y = y0; y < y1; y++) {
+        const bool up = y < Y - 1;
+        if (up) level(y + 1, lp);
+        const bool down = y > 0 && y - 1 > gh;
+        const long i = col + (long)X * y;
+        const bool l = y > ghL, r = y > ghR, bk = y > ghB, fw = y > ghF;
+        const int validEnv = int(l) + int(r) + int(bk) + int(fw);
+        if (validEnv > 0) {
+            const float cur = v[i];
+            const float psC = lc.p;
+            const float psU = up ? lp.p : psC;
+            const float psD = down ? lm.p : psC;
+            const float QP = lc.q;
+            const float Qenv = ((r ? qv[i + 1] : 0.0f) + (l ? qv[i - 1] : 0.0f) + (fw ? qv[i + (long)X * Y] : 0.0f) +
+                                (bk ? qv[i - (long)X * Y] : 0.0f)) / validEnv;
+            const float QenvUp = up ? lp.q : 0.0f;
+            const float QenvDown = down ? lm.q : 0.0f;
+            const float thD = down ? lm.th : 0.0f, thU = up ? lp.th : 0.0f;
+            const float T = lc.T;
+            float Tad = lc.th, TadU = lc.th, TadD = lc.th;
+            const float Tenv = ((r ? th[i + 1] : 0.0f) + (l ? th[i - 1] : 0.0f) + (fw ? th[i + (long)X * Y] : 0.0f) +
+                                (bk ? th[i - (long)X * Y] : 0.0f)) / validEnv;
+            const float TenvUp = thU, TenvDown = thD;
+            if (down) {
+                // lower neighbour: TDown = thD (psD / gps)^kappa is its carried temperature
+                const float TDown = lm.T;
+                if ((T < 0.0f && QP >= d_wi((TDown - 273.15f), psD)) || QP >= lm.ws) {
+                    TadD = T - d_MLR(T - 273.15f, psC) * (psC - psD);
+                    TadD = TadD * powf(gps / psD, kap);
+                }
+            }
+            if (up) {
+                const float TUp = lp.T;
+                if ((T < 0.0f && QP >= d_wi((TUp - 273.15f), psU)) || QP >= lp.ws) {
+                    TadU = d_MLR(T - 273.15f, psC) * (psU - psC) + T;
+                    TadU = TadU * powf(gps / psU, kap);
+                }
+            }
+            float B = 0.0f, BU = 0.0f, BD = 0.0f;
+            const float totalQ = Qr[i] + Qs[i] + Qi[i];
+            if (down) {
+                const float VTP = TadD * (1.0f + 0.608f * QP), VTE = TenvDown * (1.0f + 0.608f * QenvDown);
+                BD = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
+            }
+            {
+                const float VTP = Tad * (1.0f + 0.608f * QP), VTE = Tenv * (1.0f + 0.608f * Qenv);
+                B = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
+            }
+            if (up) {
+                const float VTP = TadU * (1.0f + 0.608f * QP), VTE = TenvUp * (1.0f + 0.608f * QenvUp);
+                BU = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
+            }
+            float bf = B;
+            if (cur > 0.0f) bf = BU;
+            else if (cur < 0.0f) bf = BD;
+            float change = bf * dt;
+            if ((up && cur > 0.0f && -change > cur) || (down && cur < 0.0f && -change < cur)) change = -cur;
+            v[i] = cur + change;
         }
+        lm = lc;
+        lc = lp;
     }
-    if (up) {
-        if ((T < 0.0f && QP >= d_wi((TUp - 273.15f), psU)) || QP >= d_ws((TUp - 273.15f), psU)) {
-            TadU = d_MLR(T - 273.15f, psC) * (psU - psC) + T;
-            TadU = TadU * powf(gps / psU, kap);
-        }
-    }
-    float B = 0.0f, BU = 0.0f, BD = 0.0f;
-    const float totalQ = Qr[i] + Qs[i] + Qi[i];
-    if (down) {
-        const float VTP = TadD * (1.0f + 0.608f * QP), VTE = TenvDown * (1.0f + 0.608f * QenvDown);
-        BD = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
-    }
-    {
-        const float VTP = Tad * (1.0f + 0.608f * QP), VTE = Tenv * (1.0f + 0.608f * Qenv);
-        B = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
-    }
-    if (up) {
-        const float VTP = TadU * (1.0f + 0.608f * QP), VTE = TenvUp * (1.0f + 0.608f * QenvUp);
-        BU = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
-    }
-    float bf = B;
-    if (cur > 0.0f) bf = BU;
-    else if (cur < 0.0f) bf = BD;
-    float change = bf * dt;
-    if ((up && cur > 0.0f && -change > cur) || (down && cur < 0.0f && -change < cur)) change = -cur;
-    v[i] = cur + change;
 }
 hipError_t launch_buoyancy(EnvState& e, float dt) {
-    hipLaunchKernelGGL(k_buoyancy, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_VELY], e.f[SKT_POTTEMP], e.f[SKT_QV],
-                       e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], e.f[SKT_ISEN_TEMP],
-                       e.f[SKT_ISEN_VAPOR], dt);
+    const unsigned nyc = (unsigned)((e.Y + YC_BUOY - 1) / YC_BUOY);
+    const unsigned nb = (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * nyc;
+    hipLaunchKernelGGL(k_buoyancy, dim3(nb), B3, 0, e.stream, geom(e), YC_BUOY, nyc, e.f[SKT_VELY], e.f[SKT_POTTEMP],
+                       e.f[SKT_QV], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], dt);
     return hipGetLastError();
 }
 
