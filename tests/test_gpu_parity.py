@@ -750,3 +750,31 @@ def test_full_size_steps_are_reproducible():
     for f in fa:
         assert np.isfinite(fa[f]).any(), f
         assert np.array_equal(fa[f].view(np.uint32), fb[f].view(np.uint32)), f
+
+
+def test_full_size_advection_and_diffusion_bit_exact():
+    """The advection phase (a moisture field and the three velocity
+    components) and the diffusion phase (potential temperature with its
+    ground-neighbour rule, and rain) on the 256x128x256 workload equal the
+    oracle's bit for bit on every sky cell: the launch shapes' chunk and
+    segment boundaries at full size change no result."""
+    nx, ny, nz = 256, 128, 256
+    print("full-size bit-exact phases: set-up", flush=True)
+    env, orc = make_pair(nx, ny, nz, dx=100.0, seed=20261203, n_bubbles=3)
+    m = sky_mask(nx, ny, nz)
+    rich_state(env, orc, seed=77)
+    print("full-size bit-exact phases: advection", flush=True)
+    env.advectPPMWGPU("QV", "ISEN_VAPOR", (2, 2, 2), 2.0)
+    orc.advect_ppm(ob.FIELD["QV"], ob.FIELD["ISEN_VAPOR"], 2, 2, 2, 2.0)
+    assert env.stats().substeps[0] == orc.stats()["substeps"][0]
+    assert_bits(env, orc, ("QV",), m, tag="full-advect")
+    env.advectPPMWGPU("VELX", "DEFAULT_VELX", (2, 2, 1), 2.0)
+    orc.advect_ppm(ob.FIELD["VELX"], ob.FIELD["DEFAULT_VELX"], 2, 2, 1, 2.0)
+    assert_bits(env, orc, ("VELX", "VELY", "VELZ"), m, tag="full-advect-u")
+    print("full-size bit-exact phases: diffusion", flush=True)
+    for field, type_ in (("POTTEMP", 0), ("QR", 5)):
+        env.diffuseGPU(field, type_, 0.5)
+        orc.diffuse(ob.FIELD[field], type_, 0.5)
+        assert_bits(env, orc, (field,), m, tag="full-diffuse-" + field)
+    env.close()
+    orc.close()
