@@ -778,3 +778,31 @@ def test_full_size_advection_and_diffusion_bit_exact():
         assert_bits(env, orc, (field,), m, tag="full-diffuse-" + field)
     env.close()
     orc.close()
+
+
+def test_full_size_step_against_oracle():
+    """One whole step of the 256x128x256 workload (reference solver, 200 PCG
+    iterations) against the oracle from the same initial state: equal PCG
+    iteration counts, and every sky and ground field within relative L2
+    5e-5 (the worst field, cloud water, measures about 1.3e-5: microphysics
+    transcendental ulps)."""
+    nx, ny, nz = 256, 128, 256
+    print("full-size step: set-up", flush=True)
+    env, orc = make_pair(nx, ny, nz, dx=100.0, seed=20261203, n_bubbles=3)
+    m = sky_mask(nx, ny, nz)
+    print("full-size step: device", flush=True)
+    env.updateGPU(0.5, 1.0)
+    print("full-size step: oracle", flush=True)
+    orc.step(0.5, 1.0)
+    assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+    worst = {}
+    for f in SKY + GROUND:
+        a, b = env.get(f), orc.get(f)
+        if a.size == m.size:
+            a, b = a[m], b[m]
+        assert np.isfinite(a).all(), f
+        worst[f] = rel_l2(a, b)
+        assert worst[f] <= 5e-5, (f, worst[f])
+    print(" ".join(f"{f} {v:.1e}" for f, v in sorted(worst.items(), key=lambda kv: -kv[1])[:5]), flush=True)
+    env.close()
+    orc.close()
