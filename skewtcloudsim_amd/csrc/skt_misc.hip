@@ -205,6 +205,7 @@ hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed,
 // chunk above overwrites) before the block's barrier, and writes only after
 // it, so the chunks run side by side with the results of one walk per column.
 constexpr int PRECIP_CHUNKS = 4;
+static_assert(64 * PRECIP_CHUNKS <= 256, "k_precip blocks (64 x PRECIP_CHUNKS) must fit the 256-thread launch bound");
 KL void k_precip(Geom G, PrecipBatch pbt, const float* __restrict__ th, const float* __restrict__ p,
                  const float* __restrict__ qv, const float* __restrict__ gP, float dx, Gammas gm) {
     const float dts = pbt.dts[blockIdx.y];
