@@ -892,27 +892,65 @@ __device__ void pcgPrologue(const double* __restrict__ pm, const double* __restr
 // calculateDivergenceGPU, Game/kernelSky.cu:780-860 (density {N,N,D}, velocity
 // {C,C,D}) with the per-block maxima of the initial stop-test value
 // (Game/environment.cu:933-936). p' is cleared.
-KL void k_divergence(Geom G, int zc, float* __restrict__ r, float* __restrict__ pp, const float* __restrict__ rho,
-                     const float* __restrict__ rold, const float* __restrict__ u, const float* __restrict__ v,
-                     const float* __restrict__ w, const float* __restrict__ defU, const float* __restrict__ defW,
-                     const float* __restrict__ ones, float dt, int absmode, double* part) {
-    ZM_SETUP;
+// A march up the columns: each thread walks a chunk of YC_DIV levels of its
+// (x, z) column. The ground heights of the column and of its four neighbour
+// columns are read once, and the density and v of the level below come from
+// the step before. Each neighbour value follows extv / nbOwn: a position
+// beyond the domain clamps onto this sky cell and takes the sides condition,
+// a ground position the ground condition (the level above a sky cell is
+// sky); the arithmetic per cell is unchanged. Blocks are 64 (x) x 4 (z)
+// columns of one chunk, each writing one partial maximum.
+constexpr int YC_DIV = 16;
+static unsigned divBlocks(const EnvState& e) {
+    return (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
+}
+KL void k_divergence(Geom G, int yc, unsigned nyc, float* __restrict__ r, float* __restrict__ pp,
+                     const float* __restrict__ rho, const float* __restrict__ rold, const float* __restrict__ u,
+                     const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ defU,
+                     const float* __restrict__ defW, float dt, int absmode, double* part) {
     __shared__ double sh[4];
+    const unsigned lin = blockIdx.x;
+    const int chunk = (int)(lin % nyc);
+    const unsigned tile = lin / nyc;
+    const int gx = (G.X + 63) >> 6;
+    const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
+    const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
     double m = 0.0;
-    if (inb) {
-        for (int z = z0; z < z1; z++) {
-            const long i = gidx(G, x, y, z);
+    if (x < G.X && z < G.Z) {
+        const int X = G.X, Y = G.Y, Z = G.Z;
+        const long XY = (long)X * Y;
+        const long col = XY * z + x;  // cell (x, 0, z)
+        const long gc = x + (long)X * z;
+        const int gh = G.gh[gc];
+        const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
+        const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
+        const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
+        const Bounds bd{BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET}, bv{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET};
+        const bool ab = absmode || z == 0;
+        const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
+        float rhoB = 0.0f, vB = 0.0f;  // density and v of the level below
+        if (y0 > 0 && y0 < y1) {
+            rhoB = rho[col + (long)X * (y0 - 1)];
+            vB = v[col + (long)X * (y0 - 1)];
+        }
+        for (int y = y0; y < y1; y++) {
+            const long i = col + (long)X * y;
             pp[i] = 0.0f;
+            const float cur = rho[i], vc = v[i];
             float dv = 0.0f;
-            if (!isGround(G, x, y, z)) {
-                const Bounds bd{BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET}, bv{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET};
-                const float ur = u[i], ul = nbOwn(G, u, bv, defU[y], x, y, z, -1, 0, 0);
-                const float uu = v[i], ud = nbOwn(G, v, bv, 0.0f, x, y, z, 0, -1, 0);
-                const float uf = w[i], ub = nbOwn(G, w, bv, defW[y], x, y, z, 0, 0, -1);
-                const float cur = rho[i];
-                float dr = extv(G, rho, ones, bd, x + 1, y, z), dl = extv(G, rho, ones, bd, x - 1, y, z);
-                float du = extv(G, rho, ones, bd, x, y + 1, z), dd = extv(G, rho, ones, bd, x, y - 1, z);
-                float df = extv(G, rho, ones, bd, x, y, z + 1), db = extv(G, rho, ones, bd, x, y, z - 1);
+            if (y > gh) {
+                const float ur = u[i];
+                const float ul = !inL ? bcv(bv.sides, ur, defU[y]) : (y <= ghL ? bcv(bv.ground, ur, defU[y]) : u[i - 1]);
+                const float uu = vc;
+                const float ud = y - 1 < 0 ? bcv(bv.sides, vc, 0.0f) : (y - 1 <= gh ? bcv(bv.ground, vc, 0.0f) : vB);
+                const float uf = w[i];
+                const float ub = !inB ? bcv(bv.sides, uf, defW[y]) : (y <= ghB ? bcv(bv.ground, uf, defW[y]) : w[i - XY]);
+                float dr = !inR ? bcv(bd.sides, cur, 1.0f) : (y <= ghR ? bcv(bd.ground, rho[i + 1], 1.0f) : rho[i + 1]);
+                float dl = !inL ? bcv(bd.sides, cur, 1.0f) : (y <= ghL ? bcv(bd.ground, rho[i - 1], 1.0f) : rho[i - 1]);
+                float du = y + 1 > Y - 1 ? bcv(bd.sides, cur, 1.0f) : rho[i + X];
+                float dd = y - 1 < 0 ? bcv(bd.sides, cur, 1.0f) : (y - 1 <= gh ? bcv(bd.ground, rhoB, 1.0f) : rhoB);
+                float df = !inF ? bcv(bd.sides, cur, 1.0f) : (y <= ghF ? bcv(bd.ground, rho[i + XY], 1.0f) : rho[i + XY]);
+                float db = !inB ? bcv(bd.sides, cur, 1.0f) : (y <= ghB ? bcv(bd.ground, rho[i - XY], 1.0f) : rho[i - XY]);
                 dr = dr == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dr);
                 dl = dl == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dl);
                 du = du == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / du);
@@ -922,22 +960,23 @@ KL void k_divergence(Geom G, int zc, float* __restrict__ r, float* __restrict__ 
                 const float mfd = ((dr * ur - dl * ul) + (du * uu - dd * ud) + (df * uf - db * ub));
                 const float dc = (cur - rold[i]) / dt;
                 dv = (mfd - dc);
-                const bool ab = absmode || z == 0;
                 m = fmax(m, (double)(ab ? fabsf(dv) : dv));
             }
             r[i] = dv;
+            rhoB = cur;
+            vB = vc;
         }
     }
     m = blockMax(m, sh);
-    if (tid256() == 0) part[bid] = m;
+    if (tid256() == 0) part[blockIdx.x] = m;
 }
 hipError_t launch_divergence(EnvState& e, float dt) {
     const int ref = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     const int absmode = ref ? e.cfg.maxabs_mode : 1;
-    const ZGrid zg = zgrid(e);
-    hipLaunchKernelGGL(k_divergence, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[0], e.f[SKT_PRESSURE_PROJ],
-                       e.f[SKT_DENSITY], e.f[SKT_DENSITY_OLD], e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ],
-                       e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ], e.ones, dt, absmode,
+    const unsigned nyc = (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
+    hipLaunchKernelGGL(k_divergence, dim3(divBlocks(e)), B3, 0, e.stream, geom(e), YC_DIV, nyc, e.pcg[0],
+                       e.f[SKT_PRESSURE_PROJ], e.f[SKT_DENSITY], e.f[SKT_DENSITY_OLD], e.f[SKT_VELX], e.f[SKT_VELY],
+                       e.f[SKT_VELZ], e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ], dt, absmode,
                        e.partials + (size_t)PART_DIV * e.maxBlocks);
     return hipGetLastError();
 }
@@ -974,7 +1013,7 @@ hipError_t launch_pcg_start(EnvState& e) {
     const ZGrid zg = zgrid(e);
     const size_t M = e.maxBlocks;
     hipLaunchKernelGGL(k_pcg_start, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[0], e.pcg[1], e.pcg[5],
-                       e.partials + PART_DIV * M, zg.nblocks, e.partials + PART_START * M, &e.sc->st[0],
+                       e.partials + PART_DIV * M, divBlocks(e), e.partials + PART_START * M, &e.sc->st[0],
                        &e.sc->st[1], ref, e.cfg.pcg_tol);
     return hipGetLastError();
 }
