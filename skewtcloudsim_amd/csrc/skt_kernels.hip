@@ -1396,38 +1396,65 @@ hipError_t launch_update_pressure_density(EnvState& e) {
 
 // applyPresProjGPU, Game/kernelSky.cu:862-908 (pressure {D,N,N}, density
 // {N,N,D}), optionally followed per cell by calculateNewPressure.
-KL void k_apply_projection(Geom G, const float* __restrict__ pp, const float* __restrict__ rho, float* __restrict__ u,
-                           float* __restrict__ v, float* __restrict__ w, float* __restrict__ p,
+// A march up the columns like the divergence: each thread walks a chunk of
+// YC_DIV levels of its (x, z) column, reads the ground heights of the +x and
+// +z columns once and carries p' and the density of the level above into the
+// next step, where they are the cell's own. Each neighbour value follows
+// nbOwn (beyond the domain: the sides condition on the cell's own value;
+// ground: the ground condition), with the same arithmetic per cell.
+KL void k_apply_projection(Geom G, int yc, unsigned nyc, const float* __restrict__ pp, const float* __restrict__ rho,
+                           float* __restrict__ u, float* __restrict__ v, float* __restrict__ w, float* __restrict__ p,
                            const float* __restrict__ th, const float* __restrict__ qv, const float* __restrict__ gP,
                            int newp) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
+    const unsigned lin = blockIdx.x;
+    const int chunk = (int)(lin % nyc);
+    const unsigned tile = lin / nyc;
+    const int gx = (G.X + 63) >> 6;
+    const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
+    const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
+    if (x >= G.X || z >= G.Z) return;
+    const int X = G.X, Y = G.Y, Z = G.Z;
+    const long XY = (long)X * Y;
+    const long col = XY * z + x;  // cell (x, 0, z)
+    const long gc = x + (long)X * z;
+    const int gh = G.gh[gc];
+    const bool inR = x < X - 1, inF = z < Z - 1;
+    const int ghR = inR ? G.gh[gc + 1] : 0, ghF = inF ? G.gh[gc + X] : 0;
     const Bounds bp{BC_DIRICHLET, BC_NEUMANN, BC_NEUMANN}, bd{BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET};
-    const long i = gidx(G, x, y, z);
-    const float cp = pp[i], cd = rho[i];
-    const float pr = nbOwn(G, pp, bp, 0.0f, x, y, z, 1, 0, 0);
-    const float pu = nbOwn(G, pp, bp, 0.0f, x, y, z, 0, 1, 0);
-    const float pf = nbOwn(G, pp, bp, 0.0f, x, y, z, 0, 0, 1);
-    float dr = nbOwn(G, rho, bd, 0.0f, x, y, z, 1, 0, 0);
-    float du = nbOwn(G, rho, bd, 0.0f, x, y, z, 0, 1, 0);
-    float df = nbOwn(G, rho, bd, 0.0f, x, y, z, 0, 0, 1);
-    dr = dr == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / dr);
-    du = du == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / du);
-    df = df == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / df);
-    u[i] += (pr - cp) / dr;
-    v[i] += (pu - cp) / du;
-    w[i] += (pf - cp) / df;
-    if (newp) {
-        const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
-        const float Tv = T * (0.608f * qv[i] + 1);
-        p[i] = cd * kc::Rsd * Tv / 100.0f;
+    const float gps = gP[gc];
+    const int y0 = max(chunk * yc, gh + 1), y1 = min(Y, chunk * yc + yc);
+    if (y0 >= y1) return;
+    float cp = pp[col + (long)X * y0], cd = rho[col + (long)X * y0];
+    for (int y = y0; y < y1; y++) {
+        const long i = col + (long)X * y;
+        const bool up = y + 1 <= Y - 1;  // the level above a sky cell is sky
+        const float ppU = up ? pp[i + X] : 0.0f, rhoU = up ? rho[i + X] : 0.0f;
+        const float pr = !inR ? bcv(bp.sides, cp, 0.0f) : (y <= ghR ? bcv(bp.ground, cp, 0.0f) : pp[i + 1]);
+        const float pu = !up ? bcv(bp.sides, cp, 0.0f) : ppU;
+        const float pf = !inF ? bcv(bp.sides, cp, 0.0f) : (y <= ghF ? bcv(bp.ground, cp, 0.0f) : pp[i + XY]);
+        float dr = !inR ? bcv(bd.sides, cd, 0.0f) : (y <= ghR ? bcv(bd.ground, cd, 0.0f) : rho[i + 1]);
+        float du = !up ? bcv(bd.sides, cd, 0.0f) : rhoU;
+        float df = !inF ? bcv(bd.sides, cd, 0.0f) : (y <= ghF ? bcv(bd.ground, cd, 0.0f) : rho[i + XY]);
+        dr = dr == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / dr);
+        du = du == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / du);
+        df = df == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / df);
+        u[i] += (pr - cp) / dr;
+        v[i] += (pu - cp) / du;
+        w[i] += (pf - cp) / df;
+        if (newp) {
+            const float T = th[i] * powf(p[i] / gps, kc::Rsd / kc::Cpd);
+            const float Tv = T * (0.608f * qv[i] + 1);
+            p[i] = cd * kc::Rsd * Tv / 100.0f;
+        }
+        cp = ppU;
+        cd = rhoU;
     }
 }
 hipError_t launch_apply_projection(EnvState& e, int new_pressure) {
-    hipLaunchKernelGGL(k_apply_projection, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_PRESSURE_PROJ],
-                       e.f[SKT_DENSITY], e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ], e.f[SKT_PRESSURE],
-                       e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P], new_pressure);
+    const unsigned nyc = (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
+    hipLaunchKernelGGL(k_apply_projection, dim3(divBlocks(e)), B3, 0, e.stream, geom(e), YC_DIV, nyc,
+                       e.f[SKT_PRESSURE_PROJ], e.f[SKT_DENSITY], e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ],
+                       e.f[SKT_PRESSURE], e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P], new_pressure);
     return hipGetLastError();
 }
 
