@@ -742,6 +742,13 @@ hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) 
 // char conversion of a lower-triangle coefficient (Game/kernelSky.cu:762-764, quirk A1)
 __device__ __forceinline__ float trunc8(float c) { return fminf(fmaxf(truncf(c), -128.0f), 127.0f); }
 
+// Column marches (preconditioner, divergence, projection): chunks of YC_DIV
+// levels, blocks of 64 (x) x 4 (z) columns of one chunk.
+constexpr int YC_DIV = 16;
+static unsigned divBlocks(const EnvState& e) {
+    return (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
+}
+
 // initAMatrix + initPrecon, Game/kernelSky.cu:1320-1430, density bounds
 // {N,D,D}. The operator is not stored: the SpMV rebuilds its coefficients
 // from 1/density (irho, 4 B per cell instead of 16 for a float4 operator).
@@ -749,44 +756,78 @@ __device__ __forceinline__ float trunc8(float c) { return fminf(fmaxf(truncf(c),
 KL void k_set_int(int* a, int v) {
     if (threadIdx.x == 0) *a = v;
 }
-KL void k_build_precon(Geom G, const float* __restrict__ rho, float* __restrict__ irho, float* __restrict__ pre,
-                       int* __restrict__ faceExact) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb) return;
-    const long i = gidx(G, x, y, z);
-    if (isGround(G, x, y, z)) {
-        irho[i] = 0.0f;
-        pre[i] = 1.0f;
-        return;
-    }
+// A march up the columns in the chunks of the divergence march: the ground
+// heights of the column and its four neighbour columns are read once and the
+// densities of the levels below and above are carried. Each neighbour read
+// follows extv: beyond the domain the sides condition on the cell's own
+// density (the clamped position), at ground the ground condition.
+KL void k_build_precon(Geom G, int yc, unsigned nyc, const float* __restrict__ rho, float* __restrict__ irho,
+                       float* __restrict__ pre, int* __restrict__ faceExact) {
+    const unsigned lin = blockIdx.x;
+    const int chunk = (int)(lin % nyc);
+    const unsigned tile = lin / nyc;
+    const int gx = (G.X + 63) >> 6;
+    const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
+    const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
+    if (x >= G.X || z >= G.Z) return;
+    const int X = G.X, Y = G.Y, Z = G.Z;
+    const long XY = (long)X * Y;
+    const long col = XY * z + x;  // cell (x, 0, z)
+    const long gc = x + (long)X * z;
+    const int gh = G.gh[gc];
+    const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
+    const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
+    const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
     const Bounds b{BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET};
-    const float c = rho[i];
-    const float r = faceCoef(c, extv(G, rho, nullptr, b, x + 1, y, z));
-    const float l = faceCoef(c, extv(G, rho, nullptr, b, x - 1, y, z));
-    const float u = faceCoef(c, extv(G, rho, nullptr, b, x, y + 1, z));
-    const float d = faceCoef(c, extv(G, rho, nullptr, b, x, y - 1, z));
-    const float f = faceCoef(c, extv(G, rho, nullptr, b, x, y, z + 1));
-    const float bb = faceCoef(c, extv(G, rho, nullptr, b, x, y, z - 1));
-    float w = 0.0f;
-    w -= r;
-    w -= l;
-    w -= u;
-    w -= d;
-    w -= f;
-    w -= bb;
-    const float ic = 1.0f / c;
-    irho[i] = ic;
-    pre[i] = w == 0.0f ? 1.0f : 1.0f / w;
-    // the SpMV's short face form is exact when every sky 1/density is in range
-    if (!faceValueOk(ic)) atomicOr(faceExact, 1);
+    const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
+    if (y0 >= y1) return;
+    float rd = y0 > 0 ? rho[col + (long)X * (y0 - 1)] : 0.0f;  // density of the level below
+    float c = rho[col + (long)X * y0];
+    bool bad = false;
+    for (int y = y0; y < y1; y++) {
+        const long i = col + (long)X * y;
+        const float ru = y < Y - 1 ? rho[i + X] : 0.0f;
+        if (y <= gh) {
+            irho[i] = 0.0f;
+            pre[i] = 1.0f;
+        } else {
+            const float nR = !inR ? bcv(b.sides, c, 0.0f) : (y <= ghR ? bcv(b.ground, rho[i + 1], 0.0f) : rho[i + 1]);
+            const float nL = !inL ? bcv(b.sides, c, 0.0f) : (y <= ghL ? bcv(b.ground, rho[i - 1], 0.0f) : rho[i - 1]);
+            const float nU = y + 1 > Y - 1 ? bcv(b.sides, c, 0.0f) : ru;  // the level above a sky cell is sky
+            const float nD = y - 1 < 0 ? bcv(b.sides, c, 0.0f) : (y - 1 <= gh ? bcv(b.ground, rd, 0.0f) : rd);
+            const float nF = !inF ? bcv(b.sides, c, 0.0f) : (y <= ghF ? bcv(b.ground, rho[i + XY], 0.0f) : rho[i + XY]);
+            const float nB = !inB ? bcv(b.sides, c, 0.0f) : (y <= ghB ? bcv(b.ground, rho[i - XY], 0.0f) : rho[i - XY]);
+            const float r = faceCoef(c, nR);
+            const float l = faceCoef(c, nL);
+            const float u = faceCoef(c, nU);
+            const float d = faceCoef(c, nD);
+            const float f = faceCoef(c, nF);
+            const float bb = faceCoef(c, nB);
+            float w = 0.0f;
+            w -= r;
+            w -= l;
+            w -= u;
+            w -= d;
+            w -= f;
+            w -= bb;
+            const float ic = 1.0f / c;
+            irho[i] = ic;
+            pre[i] = w == 0.0f ? 1.0f : 1.0f / w;
+            // the SpMV's short face form is exact when every sky 1/density is in range
+            bad |= !faceValueOk(ic);
+        }
+        rd = c;
+        c = ru;
+    }
+    if (bad) atomicOr(faceExact, 1);
 }
 hipError_t launch_build_precon(EnvState& e) {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, e.stream, &e.sc->face_exact, 0);
     hipError_t r = hipGetLastError();
     if (r != hipSuccess) return r;
-    hipLaunchKernelGGL(k_build_precon, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_DENSITY], e.irho, e.pcg[5],
-                       &e.sc->face_exact);
+    const unsigned nyc = (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
+    hipLaunchKernelGGL(k_build_precon, dim3(divBlocks(e)), B3, 0, e.stream, geom(e), YC_DIV, nyc, e.f[SKT_DENSITY],
+                       e.irho, e.pcg[5], &e.sc->face_exact);
     return hipGetLastError();
 }
 
@@ -900,10 +941,6 @@ __device__ void pcgPrologue(const double* __restrict__ pm, const double* __restr
 // a ground position the ground condition (the level above a sky cell is
 // sky); the arithmetic per cell is unchanged. Blocks are 64 (x) x 4 (z)
 // columns of one chunk, each writing one partial maximum.
-constexpr int YC_DIV = 16;
-static unsigned divBlocks(const EnvState& e) {
-    return (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
-}
 KL void k_divergence(Geom G, int yc, unsigned nyc, float* __restrict__ r, float* __restrict__ pp,
                      const float* __restrict__ rho, const float* __restrict__ rold, const float* __restrict__ u,
                      const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ defU,
