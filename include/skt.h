@@ -198,6 +198,16 @@ int skt_env_apply_brush(skt_env* env, int32_t param, float brush_size, const int
 int skt_env_apply_selection(skt_env* env, int32_t param, const int32_t min_pos[3], const int32_t max_pos[3],
                             float value, const float dir[3], int32_t ground_erase);
 
+/* environmentGPU::resetParameterGPU (Game/environment.cu:1274-1295), called
+ * by the editor for each parameter (Game/editor.cpp:590-598). POTTEMP and QV
+ * go back to their isentropic profiles, WIND to the default u and w with
+ * v = 0, PRESSURE to the default pressure; the profile resets follow
+ * setToDefault (Game/kernelSky.cu:1714-1728), which walks each (x, y) column
+ * along z from plane 0 and stops at its first ground cell. QW .. QI are set
+ * to 0 in every cell (setToValue). PGROUND is accepted and changes nothing,
+ * as in the reference; other ids give SKT_ERR_ARG. Stream-ordered. */
+int skt_env_reset_parameter(skt_env* env, int32_t param);
+
 /* Terrain of environmentGPU::init (Game/environment.cu:321-340): PNoise2D
  * (Engine/source/math/geometry.cpp:196-278; seed 100, 12 octaves there) over
  * an nx*nz ground grid, and the ground heights of initGroundHeightGPU

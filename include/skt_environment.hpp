@@ -133,6 +133,10 @@ public:
               "prepareSelectionGPU");
         refreshGround();
     }
+    // resetParameterGPU (Game/environment.cu:1274-1295)
+    void resetParameterGPU(parameter paramType) {
+        check(skt_env_reset_parameter(m_env, paramType), "resetParameterGPU");
+    }
     float irridianceGPU() {
         float v = 0.0f;
         check(skt_env_irradiance(m_env, &v), "irridianceGPU");

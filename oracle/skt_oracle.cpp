@@ -1825,6 +1825,41 @@ void applySelection(Env<R>& e, int param, const int* mn, const int* mx, float va
     finishEdit(e, gNew, changed, erase);
 }
 
+// environmentGPU::resetParameterGPU (Game/environment.cu:1274-1295).
+// setToDefault (Game/kernelSky.cu:1714-1728) walks each (x, y) column along z
+// from plane 0 and stops at its first ground cell, writing the per-level
+// default on the way; setToValue (Game/kernelMaths.cu:12-23) writes every
+// cell, ground included. POTTEMP and QV take their isentropic profiles,
+// WIND the default u and w with v = 0, PRESSURE the default pressure, each
+// condensate 0. PGROUND and unknown ids do nothing.
+template <class R> void resetParameter(Env<R>& e, int param) {
+    auto toDefault = [&](int fld, int prof) {
+        for (int y = 0; y < e.Y; y++)
+            for (int x = 0; x < e.X; x++)
+                for (int z = 0; z < e.Z; z++) {
+                    if (e.ground(x, y, z)) break;
+                    e.f[fld][e.id(x, y, z)] = e.f[prof][y];
+                }
+    };
+    auto toZero = [&](int fld) { std::fill(e.f[fld].begin(), e.f[fld].end(), R(0)); };
+    switch (param) {
+        case 0: toDefault(F_THETA, F_ISEN_T); break;
+        case 1: toDefault(F_QV, F_ISEN_QV); break;
+        case 2: toZero(F_QW); break;
+        case 3: toZero(F_QC); break;
+        case 4: toZero(F_QR); break;
+        case 5: toZero(F_QS); break;
+        case 6: toZero(F_QI); break;
+        case 7:
+            toDefault(F_U, F_DEF_U);
+            toZero(F_V);
+            toDefault(F_W, F_DEF_W);
+            break;
+        case 9: toDefault(F_P, F_DEF_P); break;
+        default: break;
+    }
+}
+
 // irridianceGPU, Game/environment.cu:1021-1034 (host float arithmetic).
 static float irradiance(float time, int day, float lon, float sun) {
     const float Gs = 1361.0f * sun;
@@ -2451,6 +2486,10 @@ void sko_apply_brush(void* hv, int param, float size, const int* pos, float smoo
 void sko_apply_selection(void* hv, int param, const int* mn, const int* mx, float value, const float* dir, int erase) {
     SkoHandle* h = (SkoHandle*)hv;
     DISPATCH(h, applySelection(e, param, mn, mx, value, dir, erase != 0));
+}
+void sko_reset_parameter(void* hv, int param) {
+    SkoHandle* h = (SkoHandle*)hv;
+    DISPATCH(h, resetParameter(e, param));
 }
 void sko_editor_data(void* hv) {
     SkoHandle* h = (SkoHandle*)hv;

@@ -133,6 +133,7 @@ SIGNATURES = {
     "skt_env_apply_brush": (_I, [_P, _I, _F, _P, _F, _F, _F, _F, _P, _I]),
     "skt_env_get_ground_height": (_I, [_P, _P]),
     "skt_env_apply_selection": (_I, [_P, _I, _P, _P, _F, _P, _I]),
+    "skt_env_reset_parameter": (_I, [_P, _I]),
     "skt_env_compute_masks": (_I, [_P, _P]),
     "skt_env_phase_mask": (_I, [_P, _P]),
     "skt_mphys_region_set": (_I, [_P, _P, _P]),

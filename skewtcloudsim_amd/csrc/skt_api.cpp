@@ -926,6 +926,30 @@ int skt_env_apply_selection(skt_env* env, int32_t param, const int32_t min_pos[3
     }
     return applyEdit(env, a, false);
 }
+// environmentGPU::resetParameterGPU (Game/environment.cu:1274-1295).
+int skt_env_reset_parameter(skt_env* env, int32_t param) {
+    if (!env) return SKT_ERR_ARG;
+    EnvState& e = env->s;
+    HIPCHK(e, hipSetDevice(e.device));
+    switch (param) {
+        case SKT_PARAM_POTTEMP: HIPCHK(e, launch_reset_param(e, e.f[SKT_POTTEMP], e.f[SKT_ISEN_TEMP])); break;
+        case SKT_PARAM_QV: HIPCHK(e, launch_reset_param(e, e.f[SKT_QV], e.f[SKT_ISEN_VAPOR])); break;
+        case SKT_PARAM_QW: HIPCHK(e, launch_reset_param(e, e.f[SKT_QW], nullptr)); break;
+        case SKT_PARAM_QC: HIPCHK(e, launch_reset_param(e, e.f[SKT_QC], nullptr)); break;
+        case SKT_PARAM_QR: HIPCHK(e, launch_reset_param(e, e.f[SKT_QR], nullptr)); break;
+        case SKT_PARAM_QS: HIPCHK(e, launch_reset_param(e, e.f[SKT_QS], nullptr)); break;
+        case SKT_PARAM_QI: HIPCHK(e, launch_reset_param(e, e.f[SKT_QI], nullptr)); break;
+        case SKT_PARAM_WIND:
+            HIPCHK(e, launch_reset_param(e, e.f[SKT_VELX], e.f[SKT_DEFAULT_VELX]));
+            HIPCHK(e, launch_reset_param(e, e.f[SKT_VELY], nullptr));
+            HIPCHK(e, launch_reset_param(e, e.f[SKT_VELZ], e.f[SKT_DEFAULT_VELZ]));
+            break;
+        case SKT_PARAM_PRESSURE: HIPCHK(e, launch_reset_param(e, e.f[SKT_PRESSURE], e.f[SKT_DEFAULT_PRESSURE])); break;
+        case SKT_PARAM_PGROUND: break;  // the reference has no reset for the ground
+        default: return setErr(e, SKT_ERR_ARG, "resetParameterGPU: unknown parameter");
+    }
+    return SKT_OK;
+}
 int skt_env_diagnostics(skt_env* env, float thr, skt_diagnostics* out) {
     if (!env || !out) return SKT_ERR_ARG;
     EnvState& e = env->s;

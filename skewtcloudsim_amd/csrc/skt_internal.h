@@ -229,6 +229,8 @@ hipError_t launch_buoyancy(EnvState& e, float dt);
 hipError_t launch_init_density(EnvState& e, int copy_old);
 hipError_t launch_new_pressure(EnvState& e);
 hipError_t launch_reset_vel(EnvState& e);
+// setToDefault (prof) or setToValue 0 (prof null) of one field
+hipError_t launch_reset_param(EnvState& e, float* a, const float* prof);
 hipError_t launch_build_precon(EnvState& e);  // 1/density and 1/diagonal
 hipError_t launch_divergence(EnvState& e, float dt);  // r = div, partial max |r|
 hipError_t launch_pcg_start(EnvState& e);              // stop tolerance; z = pre r, partial z.r

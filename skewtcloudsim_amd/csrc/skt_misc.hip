@@ -453,6 +453,24 @@ hipError_t launch_reset_vel(EnvState& e) {
     return hipGetLastError();
 }
 
+// setToDefault (Game/kernelSky.cu:1714-1728) with a per-level profile: the
+// reference walks each (x, y) column along z and returns at its first ground
+// cell, so a cell takes prof[y] iff y > ghz (the walk reaches it, as in
+// k_reset_vel). Without a profile, setToValue(0) (Game/kernelMaths.cu:12-23)
+// writes every cell, ground included.
+KL void k_reset_param(Geom G, const int* __restrict__ ghz, float* __restrict__ a, const float* __restrict__ prof) {
+    CELL_SETUP;
+    (void)bid;
+    if (!inb) return;
+    const long i = gidx(G, x, y, z);
+    if (!prof) a[i] = 0.0f;
+    else if (y > ghz[x + (long)G.X * z]) a[i] = prof[y];
+}
+hipError_t launch_reset_param(EnvState& e, float* a, const float* prof) {
+    hipLaunchKernelGGL(k_reset_param, grid3(e), B3, 0, e.stream, geom(e), e.ghz, a, prof);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------- microphysics ---
 struct MicroArgs {
     float *Qv, *Qw, *Qc, *Qr, *Qs, *Qi, *th;

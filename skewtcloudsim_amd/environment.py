@@ -333,6 +333,11 @@ class EnvironmentGPU:
         self._chk(self.lib.skt_env_apply_selection(self.h, p, _ptr(mn), _ptr(mx), applyValue, _ptr(d),
                                                    int(bool(groundErase))), "prepareSelectionGPU")
 
+    def resetParameterGPU(self, param):
+        """environmentGPU::resetParameterGPU (skt_env_reset_parameter)."""
+        p = self._PARAMS.get(param, param)
+        self._chk(self.lib.skt_env_reset_parameter(self.h, p), "resetParameterGPU")
+
     def ground_height(self):
         """The ground heights in use, int32[nz*nx] (column x + nx*z)."""
         gh = np.empty(self.G, np.int32)

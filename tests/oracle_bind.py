@@ -54,7 +54,7 @@ def lib():
             "sko_set_threads": [i], "sko_fallvel": [f, f, f, f, i, vp], "sko_gammas": [vp],
             "sko_diagnostics": [vp, f, vp, vp, vp], "sko_terrain": [i, i, i, i, f, vp, vp],
             "sko_apply_brush": [vp, i, f, vp, f, f, f, f, vp, i], "sko_apply_selection": [vp, i, vp, vp, f, vp, i],
-            "sko_get_ground_height": [vp, vp],
+            "sko_get_ground_height": [vp, vp], "sko_reset_parameter": [vp, i],
         }
         for name, args in sig.items():
             getattr(L, name).argtypes = args

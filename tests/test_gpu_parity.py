@@ -274,10 +274,21 @@ def test_region_diagnostic(pair):
 # Non-flat ground: lateral ground neighbours, ground within the PPM stencil
 # and the slow (near-ground) paths of every stencil kernel.
 
-def _terrain(nx, nz, kind, seed=21):
+def _terrain(nx, nz, kind, seed=21, ny=None):
     rng = np.random.default_rng(seed)
     if kind == "random":
         return rng.integers(0, 5, nx * nz).astype(np.int32)
+    if kind == "tall":
+        # Heights spread over the whole column, so ground crosses the 16-level
+        # chunks of the column marches (a chunk starting at or inside ground,
+        # a chunk wholly ground, neighbour heights >= 16), with the chunk
+        # edges 15, 16, 17, 31, 32 and one column that is ground to the top.
+        gh = rng.integers(0, ny - 8, nx * nz)
+        gh[rng.uniform(size=nx * nz) < 0.4] = rng.integers(0, 4)
+        cols = rng.choice(nx * nz, 12, replace=False)
+        for c, h in zip(cols, (15, 16, 17, 31, 32, ny - 1, 15, 16, 17, 31, 32, 14)):
+            gh[c] = h
+        return gh.astype(np.int32)
     x = np.arange(nx)[None, :]
     z = np.arange(nz)[:, None]
     hill = 6.0 * np.exp(-(((x - nx * 0.4) / (nx * 0.2)) ** 2 + ((z - nz * 0.6) / (nz * 0.25)) ** 2))
@@ -297,10 +308,11 @@ def make_terrain_pair(nx, ny, nz, gh, dx=200.0, seed=31, **cfg):
     return env, orc
 
 
-@pytest.mark.parametrize("shape,kind", [((24, 20, 16), "random"), ((33, 18, 21), "hill"), ((17, 13, 11), "random")])
+@pytest.mark.parametrize("shape,kind", [((24, 20, 16), "random"), ((33, 18, 21), "hill"), ((17, 13, 11), "random"),
+                                        ((40, 48, 24), "tall")])
 def test_terrain_phases(shape, kind):
     nx, ny, nz = shape
-    gh = _terrain(nx, nz, kind)
+    gh = _terrain(nx, nz, kind, ny=ny)
     env, orc = make_terrain_pair(nx, ny, nz, gh)
     m = sky_mask(nx, ny, nz, gh)
     assert np.array_equal(env.masks(), orc.masks())
@@ -340,10 +352,10 @@ def test_terrain_phases(shape, kind):
     orc.close()
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_terrain_pressure_projection(mode):
-    nx, ny, nz = 24, 20, 16
-    gh = _terrain(nx, nz, "random")
+@pytest.mark.parametrize("mode,kind", [(0, "random"), (1, "random"), (0, "tall"), (1, "tall")])
+def test_terrain_pressure_projection(mode, kind):
+    nx, ny, nz = (24, 20, 16) if kind == "random" else (40, 48, 24)
+    gh = _terrain(nx, nz, kind, ny=ny)
     env, orc = make_terrain_pair(nx, ny, nz, gh, pcg_mode=mode, pcg_tol=1e-5 if mode == 0 else 1e-6,
                                  pcg_max_iter=60)
     rich_state(env, orc, seed=46)
@@ -360,10 +372,10 @@ def test_terrain_pressure_projection(mode):
     orc.close()
 
 
-@pytest.mark.parametrize("shape,kind", [((24, 20, 16), "random"), ((33, 18, 21), "hill")])
+@pytest.mark.parametrize("shape,kind", [((24, 20, 16), "random"), ((33, 18, 21), "hill"), ((40, 48, 24), "tall")])
 def test_terrain_full_steps(shape, kind):
     nx, ny, nz = shape
-    gh = _terrain(nx, nz, kind)
+    gh = _terrain(nx, nz, kind, ny=ny)
     env, orc = make_terrain_pair(nx, ny, nz, gh)
     m = sky_mask(nx, ny, nz, gh)
     for s in range(4):
@@ -804,5 +816,27 @@ def test_full_size_step_against_oracle():
         worst[f] = rel_l2(a, b)
         assert worst[f] <= 5e-5, (f, worst[f])
     print(" ".join(f"{f} {v:.1e}" for f, v in sorted(worst.items(), key=lambda kv: -kv[1])[:5]), flush=True)
+    env.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("param", ["POTTEMP", "QV", "QW", "QC", "QR", "QS", "QI", "WIND", "PGROUND", "PRESSURE"])
+def test_reset_parameter(param):
+    """resetParameterGPU on tall terrain: the setToDefault z walk stops at a
+    column's first ground cell, setToValue clears every cell; every cell of
+    every field equals the oracle bit for bit."""
+    nx, ny, nz = 40, 48, 24
+    gh = _terrain(nx, nz, "tall", ny=ny)
+    env, orc = make_terrain_pair(nx, ny, nz, gh)
+    rich_state(env, orc, seed=47)
+    for f in ob.FIELDS[:11]:
+        assert np.array_equal(env.get(f).view(np.uint32), orc.get(f).view(np.uint32)), f
+    env.resetParameterGPU(param)
+    orc.reset_parameter(env._PARAMS[param])
+    env.synchronize()
+    for f in ob.FIELDS[:11]:
+        a, b = env.get(f), orc.get(f)
+        nd = int((a.view(np.uint32) != b.view(np.uint32)).sum())
+        assert nd == 0, (param, f, nd)
     env.close()
     orc.close()

@@ -686,3 +686,46 @@ def test_edit_selection_and_ground():
     L.sko_apply_selection(o.h, 8, p6.ctypes.data, p6.ctypes.data, 0.0, d.ctypes.data, 0)
     assert o.get("QV")[idx(7, 4, 5)] == 0.0 and o.ground_height()[7 + nx * 5] == 6
     o.close()
+
+
+@pytest.mark.parametrize("param", [0, 1, 2, 6, 7, 8, 9])
+def test_reset_parameter_matches_brute_force(param):
+    """resetParameterGPU (Game/environment.cu:1274-1295): setToDefault's z walk
+    (Game/kernelSky.cu:1714-1728) stops at the first ground cell of each (x, y)
+    column; setToValue clears every cell. Brute force in numpy on terrain
+    that puts ground at the middle of some z walks."""
+    nx, ny, nz = 9, 7, 11
+    orc = ob.Oracle(nx, ny, nz, 100.0, init_noise=0)
+    rng = np.random.default_rng(param + 3)
+    gh = rng.integers(0, 5, nx * nz).astype(np.int32)
+    orc.set_ground_height(gh)
+    vals = {}
+    for f in ob.FIELDS:
+        v = rng.normal(size=orc.field_len(f)).astype(np.float32)
+        orc.set(f, v)
+        vals[f] = v.reshape(nz, ny, nx) if v.size == orc.N else v
+    orc.reset_parameter(param)
+    ground = np.arange(ny)[None, :, None] <= gh.reshape(nz, nx)[:, None, :]  # [z, y, x]
+    reached = np.cumsum(ground, axis=0) == 0  # no ground at or before this plane of the walk
+    exp = {f: v.copy() for f, v in vals.items()}
+
+    def to_default(f, prof):
+        a = exp[f]
+        a[reached] = np.broadcast_to(vals[prof][None, :, None], a.shape)[reached]
+
+    if param == 0:
+        to_default("POTTEMP", "ISEN_TEMP")
+    elif param == 1:
+        to_default("QV", "ISEN_VAPOR")
+    elif 2 <= param <= 6:
+        exp[("QW", "QC", "QR", "QS", "QI")[param - 2]][:] = 0.0
+    elif param == 7:
+        to_default("VELX", "DEFAULT_VELX")
+        exp["VELY"][:] = 0.0
+        to_default("VELZ", "DEFAULT_VELZ")
+    elif param == 9:
+        to_default("PRESSURE", "DEFAULT_PRESSURE")
+    assert reached.any() and (~reached & ~ground).any()  # some walks stop with sky beyond
+    for f in ob.FIELDS[:11]:
+        assert np.array_equal(orc.get(f), exp[f].ravel()), (param, f)
+    orc.close()
