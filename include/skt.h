@@ -218,6 +218,20 @@ int skt_env_reset_parameter(skt_env* env, int32_t param);
 int skt_terrain_generate(int32_t device, int32_t seed, int32_t nx, int32_t nz, int32_t octaves, float max_height,
                          float* noise_out, int32_t* heights_out);
 
+/* skt_terrain_generate without device allocation. No counterpart in the
+ * reference: it is for callers that cannot call hipMalloc (code being captured
+ * into a HIP graph, tight serving loops, harnesses that forbid allocation).
+ * The noise and heights are computed into a fixed device scratch array of the
+ * library (8 bytes per column, room for SKT_TERRAIN_NOALLOC_MAX_COLUMNS
+ * columns, held by each GPU the library runs on until the process ends) and
+ * copied to the host outputs. nx * nz above the maximum gives SKT_ERR_ARG.
+ * Every call on one GPU shares that GPU's scratch, so calls that overlap on
+ * one GPU (from several host threads) race; results are otherwise identical
+ * to skt_terrain_generate. */
+#define SKT_TERRAIN_NOALLOC_MAX_COLUMNS (1L << 20)
+int skt_terrain_generate_noalloc(int32_t device, int32_t seed, int32_t nx, int32_t nz, int32_t octaves,
+                                 float max_height, float* noise_out, int32_t* heights_out);
+
 /* Serving diagnostics reduced on the device (maximum updraft, cloud-top level
  * and height, total ground rain) with cloud meaning QW + QC > cloud_threshold
  * (kg/kg; 1e-5 is the usual choice). No reference counterpart: it replaces the

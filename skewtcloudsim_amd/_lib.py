@@ -130,6 +130,7 @@ SIGNATURES = {
     "skt_env_get_stats": (_I, [_P, ctypes.POINTER(Stats)]),
     "skt_env_diagnostics": (_I, [_P, _F, ctypes.POINTER(Diagnostics)]),
     "skt_terrain_generate": (_I, [_I, _I, _I, _I, _I, _F, _P, _P]),
+    "skt_terrain_generate_noalloc": (_I, [_I, _I, _I, _I, _I, _F, _P, _P]),
     "skt_env_apply_brush": (_I, [_P, _I, _F, _P, _F, _F, _F, _F, _P, _I]),
     "skt_env_get_ground_height": (_I, [_P, _P]),
     "skt_env_apply_selection": (_I, [_P, _I, _P, _P, _F, _P, _I]),

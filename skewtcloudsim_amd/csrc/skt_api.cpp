@@ -1048,6 +1048,21 @@ int skt_terrain_generate(int32_t device, int32_t seed, int32_t nx, int32_t nz, i
     return r == hipSuccess ? SKT_OK : SKT_ERR_HIP;
 }
 
+int skt_terrain_generate_noalloc(int32_t device, int32_t seed, int32_t nx, int32_t nz, int32_t octaves,
+                                 float max_height, float* noise_out, int32_t* heights_out) {
+    if (nx < 1 || nz < 1 || octaves < 0 || (!noise_out && !heights_out)) return SKT_ERR_ARG;
+    if ((long)nx * nz > SKT_TERRAIN_NOALLOC_MAX_COLUMNS) return SKT_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return SKT_ERR_HIP;
+    float* dn = nullptr;
+    int* dh = nullptr;
+    if (terrain_scratch(device, &dn, &dh) != hipSuccess) return SKT_ERR_HIP;
+    const size_t n = (size_t)nx * nz;
+    hipError_t r = launch_terrain(seed, nx, nz, octaves, max_height, dn, dh, nullptr);
+    if (r == hipSuccess && noise_out) r = hipMemcpy(noise_out, dn, n * sizeof(float), hipMemcpyDeviceToHost);
+    if (r == hipSuccess && heights_out) r = hipMemcpy(heights_out, dh, n * sizeof(int), hipMemcpyDeviceToHost);
+    return r == hipSuccess ? SKT_OK : SKT_ERR_HIP;
+}
+
 int skt_device_synchronize(int32_t device) {
     if (hipSetDevice(device) != hipSuccess) return SKT_ERR_HIP;
     return hipDeviceSynchronize() == hipSuccess ? SKT_OK : SKT_ERR_HIP;

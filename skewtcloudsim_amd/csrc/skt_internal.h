@@ -203,6 +203,10 @@ hipError_t launch_apply_selection(EnvState& e, const EditArgs& a);
 hipError_t launch_reset_uncovered(EnvState& e);
 hipError_t launch_terrain(int seed, int width, int depth, int octaves, float maxHeight, float* noise, int* heights,
                           hipStream_t stream);
+// The device scratch of the allocation-free terrain entry point on 'device'
+// (which must be current): room for SKT_TERRAIN_NOALLOC_MAX_COLUMNS columns of
+// noise and of heights. The address is looked up once per device.
+hipError_t terrain_scratch(int device, float** noise, int** heights);
 // Independent fields moved by one PPM pass launch.
 struct PpmField {
     const float* in;

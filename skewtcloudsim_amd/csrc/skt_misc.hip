@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <climits>
 #include <algorithm>
+#include <mutex>
 #include "skt_internal.h"
 #include "skt_micro.h"
 #include "skt_reduce.h"
@@ -905,6 +906,35 @@ hipError_t launch_terrain(int seed, int width, int depth, int octaves, float max
     hipLaunchKernelGGL(k_terrain, dim3((unsigned)((n + 255) / 256)), B3, 0, stream, seed, width, depth, octaves,
                        maxHeight, noise, heights);
     return hipGetLastError();
+}
+
+// Device scratch of the allocation-free terrain entry point
+// (skt_terrain_generate_noalloc): the noise of up to kTerrainScratchColumns
+// columns, then their heights. Each GPU the library runs on holds its own
+// copy until the process ends. Host code never passes the array's name to a
+// kernel or copy (on the host it is a stand-in, not a device address): it
+// uses the address hipGetSymbolAddress gives for the device, looked up once
+// per device on its first call with that device current.
+constexpr long kTerrainScratchColumns = SKT_TERRAIN_NOALLOC_MAX_COLUMNS;
+static __device__ __align__(256) unsigned char scratch[8 * kTerrainScratchColumns];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratchMu;
+void* scratchPtr[kMaxDevices] = {};
+}  // namespace
+hipError_t terrain_scratch(int device, float** noise, int** heights) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratchMu);
+    if (!scratchPtr[device]) {
+        void* p = nullptr;
+        const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (r != hipSuccess) return r;
+        if (!p) return hipErrorInvalidSymbol;
+        scratchPtr[device] = p;
+    }
+    *noise = static_cast<float*>(scratchPtr[device]);
+    *heights = reinterpret_cast<int*>(static_cast<unsigned char*>(scratchPtr[device]) + 4 * kTerrainScratchColumns);
+    return hipSuccess;
 }
 
 // ------------------------------------------------------------------ edits ---
