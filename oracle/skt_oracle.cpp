@@ -2426,6 +2426,15 @@ void sko_divergence(void* hv, float dt, float* out) {
         for (long i = 0; i < e.N; i++) out[i] = float(d[i]);
     });
 }
+void sko_divergence64(void* hv, double dt, double* out) {
+    SkoHandle* h = (SkoHandle*)hv;
+    DISPATCH(h, {
+        typedef decltype(e.dx) RR;
+        std::vector<RR> d((size_t)e.N);
+        divergence(e, d.data(), RR(dt));
+        for (long i = 0; i < e.N; i++) out[i] = double(d[i]);
+    });
+}
 void sko_build_A(void* hv, double* Ax, double* Ay, double* Az, double* Aw, double* pre) {
     SkoHandle* h = (SkoHandle*)hv;
     DISPATCH(h, {
@@ -2587,6 +2596,75 @@ void sko_gammas(float* out10) {
     Gammas<float> g;
     const float v[10] = {g.R48, g.S425, g.gR, g.gER, g.gS, g.gRC, g.gSS, g.gI, g.gRI, 0.0f};
     for (int k = 0; k < 10; k++) out10[k] = v[k];
+}
+// Raw (unlimited) process rates of calculateEnvMicroPhysicsGPU for one cell
+// state, in FP64: in = {T (C), ps (hPa), Qv, Qw, Qc, Qr, Qs, Qi, Dair, dt,
+// speed}; out (32) = {PVCON, PVDEP (before the <= 0 clamp), PIMLT, PIDW,
+// PIHOM, PIACR, PRACI, PRAUT, PRACW, PREVP, PRACS, PSACW, PSACR, PSACI, PSAUT,
+// PSFW, PSFI, PSDEP (signed, before the split into PSDEP / PSSUB), PSMLT,
+// PGAUT, PGFR, PGACW, PGACI, PGACR, PGACS, PGSUB, PGMLT, PGDRY, PGWET,
+// PGACR1, PT1, PT2}. Inputs are used as given (no clipping at 1e-18).
+void sko_mp_rates64(const double* in, double* out) {
+    typedef double R;
+    static const Gammas<R> gm;
+    MP<R> mp(gm);
+    const R T = in[0], ps = in[1], Qv = in[2], Qw = in[3], Qc = in[4], Qr = in[5], Qs = in[6], Qi = in[7];
+    const R Dair = in[8], dt = in[9], speed = in[10];
+    const R QwMin = R(0.001), QcMin = R(0.001), QiMin = R(0.0006);
+    Vel3<R> fv{R(0), R(0), R(0)};
+    if (Qr > R(0) || Qs > R(0) || Qi > R(0)) fv = fallVel<R>(Qr, Qs, Qi, Dair, 3, gm.R48, gm.S425, gm.S425);
+    const R QWS = ws(T, ps), QWI = wi(T, ps);
+    const R PT1 = (T < R(0) && Qw + Qc > R(0) + R(1e-6)) ? R(1) : R(0);
+    const R PT2 = (T < R(0) && Qr < R(1e-4) && Qs < R(1e-4)) ? R(1) : R(0);
+    const R sR = slopePrecip(Dair, Qr, 0), sS = slopePrecip(Dair, Qs, 1), sI = slopePrecip(Dair, Qi, 2);
+    const R dts = dt * speed;
+    const R PSACW = mp.FPSACW(Qs, Qw, Dair, sS), PSACR = mp.FPSACR(Qr, Qs, fv, Dair, sR, sS);
+    const R PGACW = mp.FPGACW(Qi, Qw, sI, Dair), PGACI = mp.FPGACI(Qi, Qc, sI, Dair);
+    const R PGACR = mp.FPGACR(Qi, Qr, Dair, fv, sR, sI), PGACS = mp.FPGACS(T, Qs, Qi, Dair, fv, sS, sI, false);
+    const R PGWET = mp.FPGWET(T, ps, Qc, Qs, Qi, Qv, fv, Dair, sS, sI, PGACI, PGACS, dts);
+    const R v[32] = {mp.FPVCON(T, ps, Qv, QWS, dt, speed), mp.FPVDEP(T, ps, Qv, QWI, dt, speed), mp.FPIMLT(T, Qc),
+                     mp.FPIDW(dt, T, Qc, Qw, Dair, ps), mp.FPIHOM(T, Qw), mp.FPIACR(Qr, Qc, Dair, sR),
+                     mp.FPRACI(Qr, Qc, Dair, sR), mp.FPRAUT(Qw, QwMin, T), mp.FPRACW(Qr, Qw, Dair, sR),
+                     mp.FPREVP(T, Qr, Qv, QWS, Dair, ps, sR, PT1), mp.FPRACS(Qr, Qs, PT2, fv, Dair, sS, sR), PSACW,
+                     PSACR, mp.FPSACI(T, Qs, Qc, Dair, sS), mp.FPSAUT(T, Qc, QcMin),
+                     mp.FPSFW(dt, T, ps, Qs, Qw, Qc, Qv, QWI, Dair), mp.FPSFI(dt, T, ps, Qs, Qw, Qc, Qv, QWI, Dair),
+                     mp.FPSDEP(T, ps, Qv, QWI, Dair, sS), mp.FPSMLT(T, ps, Qw, Qr, Qs, Qv, Dair, sS, PSACW, PSACR, dts),
+                     mp.FPGAUT(T, Qs, QiMin), mp.FPGFR(T, Qr, Dair, sR), PGACW, PGACI, PGACR, PGACS,
+                     mp.FPGSUB(T, ps, Qv, QWI, Qi, Dair, sI), mp.FPGMLT(T, ps, Qw, Qr, Qi, Qv, Dair, sI, PGACW, PGACR, dts),
+                     mp.FPGDRY(Qw, Qc, Qr, Qs, PGACW, PGACI, PGACR, PGACS, dts), PGWET,
+                     mp.FPGACR1(T, Qw, Qc, Qs, Qi, Dair, fv, sS, sI, PGWET, PGACW, PGACI, PGACS, dts), PT1, PT2};
+    for (int k = 0; k < 32; k++) out[k] = v[k];
+}
+// Ground-surface rates (Game/microPhysics.cu:687-1032) for one column in
+// FP64: in = {tAK (K), tGC (C), ps, Qv, Qgr, Qgs, Qgi, Qrs, irr, cc, Dair,
+// wind, dt, time, Qr, Qi}; out (9) = {PGREVP, PGSMLT, PGGMLT, PGGFR, PGSFR,
+// PGFLW, PGDEVP, time after FPGDEVP, PGRFR (sky Qr, Qi over the ground stores,
+// no fall speeds)}.
+void sko_ground_rates64(const double* in, double* out) {
+    typedef double R;
+    const R tAK = in[0], tGC = in[1], ps = in[2], Qv = in[3], Qgr = in[4], Qgs = in[5], Qgi = in[6], Qrs = in[7];
+    const R irr = in[8], cc = in[9], Dair = in[10], wind = in[11], dt = in[12];
+    R time = in[13];
+    const R slopeI = slopePrecip(Dair, Qgi, 2);
+    out[0] = FPGREVP<R>(tAK, tGC, ps, Qv, Qgr, irr, Dair, wind);
+    out[1] = FPGSMLT<R>(tAK, tGC, ps, Qgs, Qv, Qgr, irr, cc, Dair, wind, dt);
+    out[2] = FPGGMLT<R>(tGC, Qgi, Dair, slopeI);
+    out[3] = FPGGFR<R>(tAK, tGC, ps, Qgr, Qv, Qgi, Dair, irr, cc, wind, dt);
+    out[4] = FPGSFR<R>(tAK, tGC, ps, Qv, Qgr, Qgs, Dair, wind, dt);
+    out[5] = FPGFLW<R>(Qgr);
+    out[6] = FPGDEVP<R>(&time, Qgr, Qgs, Qgi, Qrs);
+    out[7] = time;
+    out[8] = FPGRFR<R>(tAK, tGC, ps, in[14], Qv, in[15], Qgr, Qgs, Dair, wind, Vel3<R>{R(0), R(0), R(0)});
+}
+// applyPresProjGPU alone with a given pressure correction pp[N].
+void sko_apply_projection(void* hv, const double* pp) {
+    SkoHandle* h = (SkoHandle*)hv;
+    DISPATCH(h, {
+        typedef decltype(e.dx) RR;
+        std::vector<RR> q((size_t)e.N);
+        for (long i = 0; i < e.N; i++) q[i] = RR(pp[i]);
+        applyProjection(e, q.data());
+    });
 }
 uint32_t sko_hash(uint32_t s) { return pcgHash(s); }
 float sko_ppm_flux(float vel, float L, float C, float R_, float dt, float dx) { return ppmFlux<float>(vel, L, C, R_, dt, dx); }

@@ -8,7 +8,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+# SKT_ORACLE_SO points the tests at another build of the oracle (the mutation
+# tests use it to run the pins against oracles with planted slips).
+ORACLE_SO = os.environ.get("SKT_ORACLE_SO") or os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref_meteo.so")
 
 FIELDS = [
@@ -75,6 +77,13 @@ def lib():
         L.sko_slopePrecip.argtypes = [f, f, i]
         L.sko_potTemp64.restype = ctypes.c_double
         L.sko_potTemp64.argtypes = [ctypes.c_double] * 3
+        for name in ("sko_mp_rates64", "sko_ground_rates64"):
+            getattr(L, name).restype = None
+            getattr(L, name).argtypes = [vp, vp]
+        L.sko_divergence64.restype = None
+        L.sko_divergence64.argtypes = [vp, ctypes.c_double, vp]
+        L.sko_apply_projection.restype = None
+        L.sko_apply_projection.argtypes = [vp, vp]
         L.sko_hash.restype = ctypes.c_uint32
         L.sko_hash.argtypes = [ctypes.c_uint32]
         L.sko_ppm_flux.restype = f
