@@ -232,6 +232,50 @@ int skt_terrain_generate(int32_t device, int32_t seed, int32_t nx, int32_t nz, i
 int skt_terrain_generate_noalloc(int32_t device, int32_t seed, int32_t nx, int32_t nz, int32_t octaves,
                                  float max_height, float* noise_out, int32_t* heights_out);
 
+/* ---- Ensembles: several environments advanced by one launch per phase. ----
+ * No counterpart in the reference, which holds one environment per process
+ * (its grid and boundary data are __constant__ symbols, Game/kernelSky.cu:15-21).
+ * A batch borrows initialised environments (members, any grid shapes) and
+ * advances all of them by one updateGPU each per skt_batch_step: every phase
+ * is one launch whose blocks walk a work list over all members' tiles, and
+ * the step is replayed as a HIP graph. Each member's results are those of
+ * skt_env_update on it alone, bit for bit. Members must share the solver
+ * mode, the PCG iteration limit and the diffusion sweep count; while in a
+ * batch a member's work runs on the batch's stream. Not thread-safe. */
+typedef struct skt_batch skt_batch;
+typedef struct skt_batch_options {
+    int32_t use_graph;             /* 1: replay each step as a HIP graph (default); 0: issue the launches */
+    int32_t advection_substep_cap; /* advection substep passes per advection in a step (default 4): a member
+                                      needing more (Courant number above the cap) is reported by
+                                      skt_batch_synchronize */
+    int32_t launch_mode;           /* launch width of bodies not tuned: 0 one block per work item (default),
+                                      1 or 2 a persistent grid of 1 or 2 rounds of resident blocks */
+} skt_batch_options;
+typedef struct skt_batch_tuning {
+    double step_ms[3];       /* one ensemble step with every body at launch width 0, 1, 2 */
+    double tuned_step_ms;    /* the sum of each body's fastest width */
+    int32_t bodies;          /* distinct kernel bodies of the step */
+    int32_t persistent_bodies; /* bodies whose fastest width is persistent */
+} skt_batch_tuning;
+void skt_batch_options_default(skt_batch_options* o);
+int skt_batch_create(int32_t device, skt_batch** out);
+int skt_batch_destroy(skt_batch* b);
+const char* skt_batch_last_error(const skt_batch* b);
+int skt_batch_set_options(skt_batch* b, const skt_batch_options* o);
+/* Adds an initialised environment; member_id (out) names it in skt_batch_remove. */
+int skt_batch_add(skt_batch* b, skt_env* env, int32_t* member_id);
+int skt_batch_remove(skt_batch* b, int32_t member_id);
+int skt_batch_size(const skt_batch* b, int32_t* n);
+/* One updateGPU(dt, speed) of every member (Game/environment.cu:391-555). */
+int skt_batch_step(skt_batch* b, float dt, float speed);
+/* Waits for the batch's work; SKT_ERR_STATE if a member's advection needed
+ * more substeps than the cap since the last call. */
+int skt_batch_synchronize(skt_batch* b);
+/* Runs three ensemble steps (ordinary steps of every member), one per launch
+ * width, timing every launch, and keeps each body's fastest width for this
+ * set of member shapes (used by every later step of such an ensemble). */
+int skt_batch_autotune(skt_batch* b, float dt, float speed, skt_batch_tuning* out);
+
 /* Serving diagnostics reduced on the device (maximum updraft, cloud-top level
  * and height, total ground rain) with cloud meaning QW + QC > cloud_threshold
  * (kg/kg; 1e-5 is the usual choice). No reference counterpart: it replaces the

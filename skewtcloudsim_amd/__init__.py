@@ -5,7 +5,8 @@ gfx950 in libskt.so; this package is the Python mirror of its C ABI
 (include/skt.h). Nothing here computes on the CPU.
 """
 from ._lib import FIELD, FIELDS, NEUMANN, DIRICHLET, CUSTOM, PCG_REFERENCE, PCG_STANDARD, load_library
-from .environment import EnvironmentGPU, ReadTable, SktError, default_config, make_sounding, terrain
+from .environment import (EnsembleGPU, EnvironmentGPU, ReadTable, SktError, default_config, make_sounding,
+                          terrain)
 
 __all__ = ["FIELD", "FIELDS", "NEUMANN", "DIRICHLET", "CUSTOM", "PCG_REFERENCE", "PCG_STANDARD", "load_library",
-           "EnvironmentGPU", "ReadTable", "SktError", "default_config", "make_sounding", "terrain"]
+           "EnsembleGPU", "EnvironmentGPU", "ReadTable", "SktError", "default_config", "make_sounding", "terrain"]

@@ -90,6 +90,16 @@ class SoundingTable(ctypes.Structure):
         (n, ctypes.POINTER(ctypes.c_float)) for n in SOUNDING_COLUMNS]
 
 
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("use_graph", ctypes.c_int32), ("advection_substep_cap", ctypes.c_int32),
+                ("launch_mode", ctypes.c_int32)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("step_ms", ctypes.c_double * 3), ("tuned_step_ms", ctypes.c_double), ("bodies", ctypes.c_int32),
+                ("persistent_bodies", ctypes.c_int32)]
+
+
 _P = ctypes.c_void_p
 _F = ctypes.c_float
 _I = ctypes.c_int32
@@ -155,6 +165,17 @@ SIGNATURES = {
                                     ctypes.POINTER(SoundingTable)]),
     "skt_sounding_init_fields": (_I, [ctypes.POINTER(SoundingTable), _I, _I, _I, _F, _P, _P, _P, _P, _P, _P, _P]),
     "skt_env_init_sounding": (_I, [_P, ctypes.POINTER(SoundingTable)]),
+    "skt_batch_options_default": (None, [ctypes.POINTER(BatchOptions)]),
+    "skt_batch_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "skt_batch_destroy": (_I, [_P]),
+    "skt_batch_last_error": (ctypes.c_char_p, [_P]),
+    "skt_batch_set_options": (_I, [_P, ctypes.POINTER(BatchOptions)]),
+    "skt_batch_add": (_I, [_P, _P, ctypes.POINTER(_I)]),
+    "skt_batch_remove": (_I, [_P, _I]),
+    "skt_batch_size": (_I, [_P, ctypes.POINTER(_I)]),
+    "skt_batch_step": (_I, [_P, _F, _F]),
+    "skt_batch_synchronize": (_I, [_P]),
+    "skt_batch_autotune": (_I, [_P, _F, _F, ctypes.POINTER(BatchTuning)]),
 }
 
 _lib = None

@@ -18,10 +18,6 @@
 
 using namespace skt;
 
-struct skt_env {
-    EnvState s;
-};
-
 namespace {
 
 int setErr(EnvState& e, int code, const std::string& msg) {
@@ -53,7 +49,7 @@ hipEvent_t evGet(EnvState& e) {
 #define TL(e, cls, call)                                                          \
     do {                                                                          \
         hipEvent_t a_ = nullptr, b_ = nullptr;                                    \
-        const bool on_ = ((e).timing & (1 << (cls))) != 0 &&                      \
+        const bool on_ = !(e).rec && ((e).timing & (1 << (cls))) != 0 &&         \
                          (e).t_seen[cls]++ % (e).t_stride == 0;                    \
         if (on_) {                                                                \
             a_ = evGet(e);                                                        \
@@ -102,8 +98,10 @@ long fieldLen(const EnvState& e, int f) {
 }
 bool validField(int f) { return f >= 0 && f < SKT_FIELD_COUNT; }
 
+}  // namespace
+
 // irridianceGPU, Game/environment.cu:1021-1034 (host float arithmetic).
-float irradianceOf(const skt_env_params& p) {
+float skt::irradianceOf(const skt_env_params& p) {
     const float Gs = 1361.0f * p.sun_strength;
     const float rd = 1 + 0.034f * cosf(2 * 3.14159265359f * p.day / 365);
     const float sd = 0.409f * sinf(2 * 3.14159265359f / 365 * (p.day - 81));
@@ -112,6 +110,8 @@ float irradianceOf(const skt_env_params& p) {
     const float sr = (th - (6.0f + 14.0f / 2.0f)) * (3.14159265359f / 12.0f);
     return std::max(0.0f, Gs * rd * (sinf(lr) * sinf(sd) + cosf(lr) * cosf(sd) * cosf(sr)));
 }
+
+namespace {
 
 int readScalars(EnvState& e) {
     HIPCHK(e, hipMemcpyAsync(e.sc_host, e.sc, sizeof(Scalars), hipMemcpyDeviceToHost, e.stream));
@@ -177,6 +177,11 @@ struct AdvField {
 // until the velocities change (advectFields takes it while mv_ready holds).
 int startMaxvel(EnvState& e, float dt) {
     TL(e, TC_MAXVEL, launch_maxvel(e, dt));
+    if (e.rec) {  // the passes read the count on the device
+        e.mv_ready = true;
+        e.mv_dt = dt;
+        return SKT_OK;
+    }
     HIPCHK(e, hipMemcpyAsync(e.mv_snap, &e.sc->maxvel, sizeof(MaxvelSnap), hipMemcpyDeviceToHost, e.stream));
     HIPCHK(e, hipEventRecord(e.mv_ev, e.stream));
     e.mv_ready = true;
@@ -186,7 +191,13 @@ int startMaxvel(EnvState& e, float dt) {
 int advectFields(EnvState& e, const AdvField* fs, int n, float dt) {
     int nsub;
     float dts;
-    if (e.mv_ready && e.mv_dt == dt) {
+    if (e.rec) {
+        // recorded ensemble step: rec->adv_cap substep passes, each reading
+        // the substep length on the device and skipped past the member's count
+        if (!(e.mv_ready && e.mv_dt == dt)) TL(e, TC_MAXVEL, launch_maxvel(e, dt));
+        nsub = e.rec->adv_cap;
+        dts = 0.0f;
+    } else if (e.mv_ready && e.mv_dt == dt) {
         HIPCHK(e, hipEventSynchronize(e.mv_ev));
         nsub = e.mv_snap->nsub;
         dts = e.mv_snap->dtsub;
@@ -227,7 +238,14 @@ int advectFields(EnvState& e, const AdvField* fs, int n, float dt) {
     const int dir[5] = {0, 1, 2, 1, 0};
     const float pdt[5] = {h, h, dts, h, h};
     for (int s = 0; s < nsub; s++)
-        for (int q = 0; q < 5; q++) TL(e, TC_PPM, launch_ppm_batch(e, pb[q], dir[q], pdt[q]));
+        for (int q = 0; q < 5; q++) {
+            if (e.rec) {
+                pb[q].dsc = e.sc;
+                pb[q].sub = s;
+                pb[q].half = dir[q] != 2;
+            }
+            TL(e, TC_PPM, launch_ppm_batch(e, pb[q], dir[q], pdt[q]));
+        }
     return SKT_OK;
 }
 int advect(EnvState& e, int fld, const float* prof, Bounds b, float dt) {
@@ -246,6 +264,9 @@ int precipFields(EnvState& e, const int* flds, const int* types, int n, float dt
         sub[j] = int(fminf(ceilf(fabsf(C)), (float)e.cfg.max_substeps));
         maxsub = std::max(maxsub, sub[j]);
     }
+    // a recorded ensemble step issues the ensemble's largest count; the extra
+    // launches carry step 0 for every field and return at once
+    if (e.rec) maxsub = std::max(maxsub, e.rec->precip_cap);
     for (int s = 0; s < maxsub; s++) {
         PrecipBatch pbt{};
         pbt.n = n;
@@ -277,7 +298,9 @@ int solvePressure(EnvState& e, float dt) {
     for (; it < maxIt; it++) {
         TL(e, TC_PCG_A, launch_pcg_A(e, it));
         TL(e, TC_PCG_B, launch_pcg_B(e, it));
-        if ((it + 1) % chunk == 0 && it + 1 < maxIt) {
+        // a recorded ensemble step holds every iteration (those after the
+        // member's solve has finished return at once)
+        if (!e.rec && (it + 1) % chunk == 0 && it + 1 < maxIt) {
             const int w = c & 1;
             HIPCHK(e, hipMemcpyAsync(&e.pcg_snap[w], &e.sc->st[1], sizeof(PcgState), hipMemcpyDeviceToHost,
                                      e.stream));
@@ -297,6 +320,7 @@ int solvePressure(EnvState& e, float dt) {
         TL(e, TC_OTHER, launch_pcg_final(e, it));
         slot = 0;
     }
+    if (e.rec) return SKT_OK;  // the ensemble reads the final state from the device
     HIPCHK(e, hipMemcpyAsync(&e.pcg_snap[2], &e.sc->st[slot], sizeof(PcgState), hipMemcpyDeviceToHost, e.stream));
     HIPCHK(e, hipEventRecord(e.snap_ev[2], e.stream));
     e.pcg_stats_pending = true;
@@ -316,7 +340,8 @@ int pressureProject(EnvState& e, float dt, int new_pressure) {
     return SKT_OK;
 }
 
-int editorData(EnvState& e) {
+}  // namespace
+int skt::editor_data(EnvState& e) {
     if (e.params.ground_changed) {
         TL(e, TC_OTHER, launch_editor_reset(e, 9));
         TL(e, TC_OTHER, launch_isen_temp_ground(e));
@@ -324,9 +349,11 @@ int editorData(EnvState& e) {
     return SKT_OK;
 }
 
+namespace {
+
 int microSky(EnvState& e, float dt, float speed) {
     TL(e, TC_MICRO, launch_microphysics(e, dt, speed));
-    if (e.region_active) {
+    if (e.region_active && !e.rec) {
         TL(e, TC_OTHER, launch_region_finalize(e));
         const unsigned nb = (unsigned)blocks3d(e);
         HIPCHK(e, hipMemcpyAsync(e.region_host, (float*)(e.region + 33L * nb), 33 * sizeof(float),
@@ -349,11 +376,17 @@ struct MaxvelScope {
     ~MaxvelScope() { e.mv_ready = false; }
 };
 
-int update(EnvState& e, float dt, float speed) {
+}  // namespace
+
+// environmentGPU::updateGPU (Game/environment.cu:391-555). While the env
+// holds a recorder (an ensemble step being recorded) the ground-change path
+// and the host bookkeeping after the step are left to the ensemble, which
+// runs them per member and step.
+int skt::update(EnvState& e, float dt, float speed) {
     const float ds = dt * speed;
     MaxvelScope mvs(e);
     e.stats.n_advect = 0;
-    RC(editorData(e));
+    if (!e.rec) RC(editor_data(e));
     const float irr = irradianceOf(e.params);
     // ground (Game/environment.cu:420-456) and setTempsAtGround (:469)
     TL(e, TC_OTHER, launch_ground_water_snapshot(e, dt, speed));
@@ -398,12 +431,19 @@ int update(EnvState& e, float dt, float speed) {
         RC(advectFields(e, a, 5, ds));
     }
     RC(microSky(e, dt, speed));
-    e.params.time += speed * dt * float(!e.params.pause_diurnal);
-    if (e.params.time > 86400.0f) e.params.time = 0.0f;
-    e.params.ground_changed = 0;
+    if (!e.rec) afterStep(e, dt, speed);
     HIPCHK(e, hipGetLastError());
     return SKT_OK;
 }
+
+// Host bookkeeping at the end of updateGPU (Game/environment.cu:535-552).
+void skt::afterStep(EnvState& e, float dt, float speed) {
+    e.params.time += speed * dt * float(!e.params.pause_diurnal);
+    if (e.params.time > 86400.0f) e.params.time = 0.0f;
+    e.params.ground_changed = 0;
+}
+
+namespace {
 
 void freeAll(EnvState& e) {
     auto fr = [](void* p) {
@@ -700,6 +740,7 @@ int skt_env_init_sounding(skt_env* env, const skt_sounding_table* table) {
 int skt_env_update(skt_env* env, float dt, float speed) {
     if (!env) return SKT_ERR_ARG;
     if (!env->s.initialized) return setErr(env->s, SKT_ERR_STATE, "update before init");
+    env->s.stats_on_device = false;
     return update(env->s, dt, speed);
 }
 
@@ -751,7 +792,7 @@ int skt_env_calculate_pressure_project(skt_env* env, float dt) {
 }
 int skt_env_editor_data(skt_env* env) {
     if (!env) return SKT_ERR_ARG;
-    return editorData(env->s);
+    return editor_data(env->s);
 }
 int skt_env_irradiance(skt_env* env, float* out) {
     if (!env || !out) return SKT_ERR_ARG;
@@ -841,6 +882,21 @@ int skt_env_get_params(skt_env* env, skt_env_params* p) {
 int skt_env_get_stats(skt_env* env, skt_step_stats* st) {
     if (!env || !st) return SKT_ERR_ARG;
     EnvState& e = env->s;
+    if (e.stats_on_device) {
+        // after an ensemble step: the final solver state (slot 0 after the
+        // last iteration's stop test) and the last advection's substep count
+        PcgState ps{};
+        MaxvelSnap mv{};
+        HIPCHK(e, hipMemcpyAsync(&ps, &e.sc->st[0], sizeof(PcgState), hipMemcpyDeviceToHost, e.stream));
+        HIPCHK(e, hipMemcpyAsync(&mv, &e.sc->maxvel, sizeof(MaxvelSnap), hipMemcpyDeviceToHost, e.stream));
+        HIPCHK(e, hipStreamSynchronize(e.stream));
+        e.stats.pcg_iterations = ps.iters;
+        e.stats.pcg_residual = ps.maxr;
+        e.stats.pcg_residual0 = ps.maxr0;
+        for (int k = 0; k < std::min(e.stats.n_advect, 16); k++) e.stats.substeps[k] = mv.nsub;
+        e.pcg_stats_pending = false;
+        e.stats_on_device = false;
+    }
     if (e.pcg_stats_pending) {
         HIPCHK(e, hipEventSynchronize(e.snap_ev[2]));
         e.stats.pcg_iterations = e.pcg_snap[2].iters;
@@ -890,7 +946,7 @@ static int applyEdit(skt_env* env, EditArgs& a, bool brush) {
         rc = skt_env_set_ground_height(env, h.data());
         if (rc) return rc;
     }
-    rc = editorData(e);
+    rc = editor_data(e);
     if (rc) return rc;
     HIPCHK(e, hipStreamSynchronize(e.stream));
     return SKT_OK;

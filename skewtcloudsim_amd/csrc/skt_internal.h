@@ -35,7 +35,8 @@ struct Scalars {
     int edit_ground;  // an edit touched the ground heights
     double dg_rain;  //   ground rain water summed over the columns
     int face_exact;  // a sky 1/density of this solve lies outside the short face form's range
-    int pad[23];
+    int sub_overflow;  // an ensemble advection needed more substeps than its recorded passes
+    int pad[22];
 };
 
 // Host copy of the advection substep fields of Scalars (maxvel, nsub, dtsub).
@@ -67,6 +68,49 @@ struct ZGrid {
 struct FGrid {
     unsigned nblocks, rpb;
 };
+// Tile index of a block in its launch, and the launch's tile count.
+struct VB {
+    unsigned id, n;
+};
+// Work list of an ensemble launch: member m owns the work indices
+// [start[m], start[m] + nb[m]) (start[M] = total).
+struct WorkMap {
+    const unsigned* __restrict__ start;
+    const unsigned* __restrict__ nb;
+    int M;
+    unsigned total;
+};
+// Every kernel is compiled with __launch_bounds__(256).
+constexpr unsigned kMaxBlockThreads = 256;
+inline bool blockFits(unsigned bx, unsigned by, unsigned bz) {
+    return bx > 0 && by > 0 && bz > 0 && (unsigned long long)bx * by * bz <= kMaxBlockThreads;
+}
+// One recorded launch of an ensemble step: the ensemble launcher of its body,
+// its tile count and block shape in this member's own launch, and the bytes
+// of its argument record.
+struct RecLaunch {
+    hipError_t (*fn)(const void* dargs, const WorkMap& wm, unsigned grid, dim3 block, uint32_t shm,
+                     hipStream_t s) = nullptr;
+    int (*occ)(int threads, uint32_t shm) = nullptr;
+    unsigned nb = 0;
+    dim3 block;
+    uint32_t shm = 0;
+    std::vector<unsigned char> args;
+};
+// Recording of one member's step. While an env holds a recorder, launches go
+// to it instead of the device, and the step takes the values it would read
+// back from the device (advection substeps, solver progress) on the device:
+// each advection runs adv_cap substep passes that the kernels skip past the
+// member's own count, each sedimentation precip_cap launches, and the
+// pressure solve all its iterations (a finished solve's kernels return at
+// once).
+struct Recorder {
+    std::vector<RecLaunch> launches;
+    int adv_cap = 1;
+    int precip_cap = 1;
+    const float* irr = nullptr;  // device slot of this member's irradiance
+};
+
 struct EnvState {
     skt_config cfg;
     int X, Y, Z;
@@ -131,27 +175,36 @@ struct EnvState {
     float mv_dt = 0.0f;
     // events armed for the next klaunch (a timed single-kernel launch)
     hipEvent_t kevA = nullptr, kevB = nullptr;
+    // set while an ensemble step of this member is being recorded
+    Recorder* rec = nullptr;
+    // the last step ran in an ensemble: its solver state is read from the device
+    bool stats_on_device = false;
 };
+
+}  // namespace skt
+
+// The ABI's environment handle.
+struct skt_env {
+    skt::EnvState s;
+};
+
+namespace skt {
 
 enum { TC_DIFFUSE = 0, TC_PPM, TC_PCG_A, TC_PCG_B, TC_MICRO, TC_MAXVEL, TC_PRECIP, TC_OTHER };
 // Classes whose launch functions issue one kernel through klaunch, so their
 // timing events ride on the kernel's own dispatch.
 constexpr bool kTimedOnDispatch[8] = {true, true, true, true, true, false, true, false};
 
-// Launch on the env's stream. When a timed launch has armed kevA/kevB, the
-// events are attached to this dispatch (hipExtLaunchKernelGGL records the
-// kernel's own start and end) and disarmed. Marker events recorded between
-// kernels instead cost several microseconds of idle device time per launch.
-template <class F, class... Args>
-void klaunch(EnvState& e, F kernel, dim3 grid, dim3 block, uint32_t shm, Args... args) {
-    if (e.kevA && e.kevB) {
-        hipEvent_t a = e.kevA, b = e.kevB;
-        e.kevA = e.kevB = nullptr;
-        hipExtLaunchKernelGGL(kernel, grid, block, shm, e.stream, a, b, 0u, args...);
-    } else {
-        hipLaunchKernelGGL(kernel, grid, block, shm, e.stream, args...);
-    }
-}
+
+// --------------------------------------------------- step (skt_api.cpp) ---
+// environmentGPU::updateGPU on one env, or its recording when e.rec is set.
+int update(EnvState& e, float dt, float speed);
+// The ground-change path of editorDataGPU (Game/environment.cu:989-1019).
+int editor_data(EnvState& e);
+// Host bookkeeping after a step: the time of day and the ground flag.
+void afterStep(EnvState& e, float dt, float speed);
+// irridianceGPU (Game/environment.cu:1021-1034).
+float irradianceOf(const skt_env_params& p);
 
 // ------------------------------------------------------------- launches ---
 Geom geom(const EnvState& e);
@@ -219,6 +272,11 @@ struct PpmField {
 struct PpmBatch {
     PpmField f[SKT_BATCH];
     int n;
+    // set in a recorded ensemble step: the pass takes its time step from the
+    // member's device scalars (substep sub, half = 1 for the x and y passes)
+    // and is skipped when sub is not below the member's substep count
+    const Scalars* dsc;
+    int sub, half;
 };
 hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt);
 // Precipitation fields falling in one launch (dts 0: field skipped).

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include "skt_internal.h"
+#include "skt_launch.h"
 #include "skt_reduce.h"
 
 namespace skt {
@@ -91,16 +92,17 @@ hipError_t launch_fill(EnvState& e, float* a, float v, long n) {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<long>((n + 255) / 256, 65536)), dim3(256), 0, e.stream, a, v, n);
     return hipGetLastError();
 }
-KL void k_copy_sky(Geom G, const float* __restrict__ s, float* __restrict__ d) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
-    const long i = gidx(G, x, y, z);
-    d[i] = s[i];
-}
+struct KCopySky {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const float* __restrict__ s, float* __restrict__ d) {
+        CELL_SETUP_V;
+        (void)bid;
+        if (!inb || isGround(G, x, y, z)) return;
+        const long i = gidx(G, x, y, z);
+        d[i] = s[i];
+    }
+};
 hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst) {
-    klaunch(e, k_copy_sky, grid3(e), B3, 0, geom(e), src, dst);
-    return hipGetLastError();
+    return klaunch<KCopySky>(e, grid3(e), B3, 0, geom(e), src, dst);
 }
 // Plain streaming copy (float4 loads/stores): the measured bandwidth ceiling.
 KL void k_stream_copy(const float4* __restrict__ s, float4* __restrict__ d, long n4) {
@@ -162,172 +164,173 @@ constexpr int TH = 6, TW = 66;
 // needed over 100 VGPRs and ran slower at half the occupancy; direct
 // neighbour loads without the tile ran slower too.
 constexpr int DTH = 10;  // tile rows yb - 1 .. yb + 8
-KL void k_diffuse(Geom G, int zc, unsigned nbf, DiffBatch db, float k, const float* __restrict__ gT,
-                  const float* __restrict__ gP, const float* __restrict__ p) {
-    const unsigned lin = xcdRemap(blockIdx.x, gridDim.x);
-    const DiffField& fd = db.f[lin / nbf];
-    const unsigned bl = lin % nbf;
-    const int X = G.X, Y = G.Y, Z = G.Z;
-    const int gx = (X + 63) >> 6, gy = (Y + 7) >> 3;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x = (int)(bl % (unsigned)gx) * 64 + tx;
-    const int yb = (int)((bl / (unsigned)gx) % (unsigned)gy) * 8;
-    const int z0 = (int)(bl / (unsigned)(gx * gy)) * zc, z1 = min(Z, z0 + zc);
-    if (z0 >= z1) return;  // whole block
-    __shared__ float tV[2][DTH][TW];
-    const long XY = (long)X * Y;
-    const int ya = yb + ty, yb4 = yb + ty + 4;  // the thread's two rows
-    const bool inA = x < X && ya < Y, inB = x < X && yb4 < Y;
-    const int xc = min(x, X - 1), yca = min(ya, Y - 1), ycb = min(yb4, Y - 1);
-    const unsigned obA = 4u * (unsigned)(xc + X * yca), obB = 4u * (unsigned)(xc + X * ycb), gb = 4u * (unsigned)xc;
-    // halo cells: the row below the tile (wave 0), the row above it (wave 1),
-    // the columns left and right of it (lanes 0-7 and 8-15 of wave 2, one row
-    // each); each reads its position clamped into the domain
-    const int x0 = x - tx;
-    int hx = x, hy = yb - 1, hrow = 0, hcol = tx + 1;
-    if (ty == 1) hy = yb + 8, hrow = 9;
-    if (ty == 2) hx = tx < 8 ? x0 - 1 : x0 + 64, hy = yb + (tx & 7), hrow = 1 + (tx & 7), hcol = tx < 8 ? 0 : 65;
-    const bool hasH = ty < 2 || (ty == 2 && tx < 16);
-    const unsigned hb = 4u * (unsigned)(min(max(hx, 0), X - 1) + X * min(max(hy, 0), Y - 1));
-    // the same loads in every plane, as in the pressure operator
-    const unsigned hbL = hasH ? hb : obA;
-    const int zb = max(z0 - 1, 0);
-    const unsigned PB = 4u * (unsigned)XY, RB = 4u * (unsigned)X;
-    const Rsrc rV = arrayRsrc(fd.in + XY * zb), wV = arrayRsrc(fd.out + XY * zb);
-    const Bounds b = fd.b;
-    auto prof = [&](int yy) { return fd.prof ? fd.prof[yy] : 0.0f; };
-    const float pvA = prof(yca), pvuA = prof(min(yca + 1, Y - 1));
-    const float pvB = prof(ycb), pvuB = prof(min(ycb + 1, Y - 1));
-    const Rsrc rG = arrayRsrc(G.gh + (long)X * zb), rM = arrayRsrc(G.ghm1 + (long)X * zb);
-    auto pOff = [&](int pl) { return PB * (unsigned)(pl - zb); };
-    auto gOff = [&](int pl) { return RB * (unsigned)(pl - zb); };
-    const float den = 1 + 6 * k;
-    struct Own {
-        float a, b;  // the two rows
-        int gm, gf;  // ghm1 of the plane, ground height of the next plane
-    };
-    auto loadOwn = [&](int pl, Own& o) __attribute__((always_inline)) {
-        o.a = bldf(rV, obA, pOff(pl));
-        o.b = bldf(rV, obB, pOff(pl));
-        o.gm = bldi(rM, gb, gOff(pl));
-        o.gf = bldi(rG, gb, gOff(min(pl + 1, Z - 1)));
-    };
-    // plane below the chunk (clamped: plane 0 for the first chunk)
-    float vbA = bldf(rV, obA, pOff(zb)), vbB = bldf(rV, obB, pOff(zb));
-    int ghc = bldi(rG, gb, gOff(z0));
-    Own o0{}, o1{}, o2{}, o3{};
-    float h0 = 0.0f, h1 = 0.0f;
-    // prefetches past the chunk read the next chunk's first planes: measured
-    // faster than clamping them to the chunk's last plane
-    const int zlast = Z - 1;
-    loadOwn(z0, o0);
-    loadOwn(min(z0 + 1, zlast), o1);
-    loadOwn(min(z0 + 2, zlast), o2);
-    h0 = bldf(rV, hbL, pOff(z0));
-    // planes (bit z - z0) whose cell of row a / row b is next to ground
-    unsigned long long slowA = 0, slowB = 0;
-    const bool oL = x == 0, oR = x == X - 1;
-    // fast sweep of one cell: tile row r, this plane's value vc, the front
-    // plane's F (this cell's value on the last plane), the back plane's vb
-    auto sweep = [&](int buf, int r, int z, int y, float vc, float F, float vb, float pv, float pvu, unsigned ob)
-        __attribute__((always_inline)) {
-        const float L = tV[buf][r][tx], R = tV[buf][r][tx + 2];
-        const float D = tV[buf][r - 1][tx + 1], U = tV[buf][r + 1][tx + 1];
-        const bool oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
-        const float l = oL ? bcv(b.sides, L, pv) : L, rr = oR ? bcv(b.sides, R, pv) : R;
-        const float u = oU ? bcv(b.sides, U, pvu) : U;
-        const float fr = oF ? bcv(b.sides, F, pv) : F, bb = oB ? bcv(b.sides, vb, pv) : vb;
-        bstf(wV, ob, pOff(z), (vc + k * (l + rr + u + D + fr + bb)) / den);
-    };
-    auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, float& hc, float& hn) __attribute__((always_inline)) {
-        // loads complete in issue order: the halo (used next plane) goes first,
-        // so waiting for it does not wait for the plane read three ahead
-        hn = bldf(rV, hbL, pOff(min(z + 1, zlast)));
-        loadOwn(min(z + 3, zlast), o3n);
-        tV[buf][ty + 1][tx + 1] = oc.a;
-        tV[buf][ty + 5][tx + 1] = oc.b;
-        if (hasH) tV[buf][hrow][hcol] = hc;
-        __syncthreads();
-        const bool inZ = z + 1 < Z;
-        if (inA && ya > ghc) {
-            if (ya >= oc.gm + 2) sweep(buf, ty + 1, z, ya, oc.a, inZ ? of.a : oc.a, vbA, pvA, pvuA, obA);
-            else slowA |= 1ull << (z - z0);
-        }
-        if (inB && yb4 > ghc) {
-            if (yb4 >= oc.gm + 2) sweep(buf, ty + 5, z, yb4, oc.b, inZ ? of.b : oc.b, vbB, pvB, pvuB, obB);
-            else slowB |= 1ull << (z - z0);
-        }
-        vbA = oc.a;
-        vbB = oc.b;
-        ghc = oc.gf;
-    };
-    for (int zg = z0; zg < z1; zg += 4) {
-        step(zg, 0, o0, o1, o3, h0, h1);
-        if (zg + 1 < z1) step(zg + 1, 1, o1, o2, o0, h1, h0);
-        if (zg + 2 < z1) step(zg + 2, 0, o2, o3, o1, h0, h1);
-        if (zg + 3 < z1) step(zg + 3, 1, o3, o0, o2, h1, h0);
-    }
-    // Sky cells next to ground (marked in slowA / slowB, one bit per plane of
-    // the chunk): the sweep reads only 'in', so they can follow the fast
-    // cells. The block's four waves share them: wave w takes the marked planes
-    // z0 + w, z0 + w + 4, ... of all eight rows of its columns, so a row of
-    // marked cells (all of them in one wave) does not hold the block for the
-    // whole chunk's length. Each neighbour is tested and takes the ground or sides condition
-    // (nbv); for the potential temperature a ground neighbour reads the
-    // potential temperature of the ground temperature at this cell's pressure
-    // (:95-100). Neighbour positions are clamped into the domain, as in the tile.
-    const int xl = max(xc - 1, 0), xr = min(xc + 1, X - 1);
-    const float* __restrict__ in = fd.in;
-    auto slowCells = [&](unsigned long long slow, int y) {
-        const int yd = max(y - 1, 0), yu = min(y + 1, Y - 1);
-        const float pv = prof(y), pvd = prof(yd), pvu = prof(yu);
-        while (slow) {
-            const int z = z0 + __builtin_ctzll(slow);
-            slow &= slow - 1;
-            const long pz = XY * z;
-            const int zf = min(z + 1, Z - 1), zbk = max(z - 1, 0);
-            const int* gz = G.gh + X * z;
-            const int gc = gz[xc], gf = G.gh[xc + X * zf], gbk = G.gh[xc + X * zbk];
+struct KDiffuse {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, unsigned nbf, const DiffBatch& db, float k, const float* __restrict__ gT,
+                      const float* __restrict__ gP, const float* __restrict__ p) {
+        const unsigned lin = xcdRemap(vb.id, vb.n);
+        const DiffField& fd = db.f[lin / nbf];
+        const unsigned bl = lin % nbf;
+        const int X = G.X, Y = G.Y, Z = G.Z;
+        const int gx = (X + 63) >> 6, gy = (Y + 7) >> 3;
+        const int tx = threadIdx.x, ty = threadIdx.y;
+        const int x = (int)(bl % (unsigned)gx) * 64 + tx;
+        const int yb = (int)((bl / (unsigned)gx) % (unsigned)gy) * 8;
+        const int z0 = (int)(bl / (unsigned)(gx * gy)) * zc, z1 = min(Z, z0 + zc);
+        if (z0 >= z1) return;  // whole block
+        __shared__ float tV[2][DTH][TW];
+        const long XY = (long)X * Y;
+        const int ya = yb + ty, yb4 = yb + ty + 4;  // the thread's two rows
+        const bool inA = x < X && ya < Y, inB = x < X && yb4 < Y;
+        const int xc = min(x, X - 1), yca = min(ya, Y - 1), ycb = min(yb4, Y - 1);
+        const unsigned obA = 4u * (unsigned)(xc + X * yca), obB = 4u * (unsigned)(xc + X * ycb), gb = 4u * (unsigned)xc;
+        // halo cells: the row below the tile (wave 0), the row above it (wave 1),
+        // the columns left and right of it (lanes 0-7 and 8-15 of wave 2, one row
+        // each); each reads its position clamped into the domain
+        const int x0 = x - tx;
+        int hx = x, hy = yb - 1, hrow = 0, hcol = tx + 1;
+        if (ty == 1) hy = yb + 8, hrow = 9;
+        if (ty == 2) hx = tx < 8 ? x0 - 1 : x0 + 64, hy = yb + (tx & 7), hrow = 1 + (tx & 7), hcol = tx < 8 ? 0 : 65;
+        const bool hasH = ty < 2 || (ty == 2 && tx < 16);
+        const unsigned hb = 4u * (unsigned)(min(max(hx, 0), X - 1) + X * min(max(hy, 0), Y - 1));
+        // the same loads in every plane, as in the pressure operator
+        const unsigned hbL = hasH ? hb : obA;
+        const int zb = max(z0 - 1, 0);
+        const unsigned PB = 4u * (unsigned)XY, RB = 4u * (unsigned)X;
+        const Rsrc rV = arrayRsrc(fd.in + XY * zb), wV = arrayRsrc(fd.out + XY * zb);
+        const Bounds b = fd.b;
+        auto prof = [&](int yy) { return fd.prof ? fd.prof[yy] : 0.0f; };
+        const float pvA = prof(yca), pvuA = prof(min(yca + 1, Y - 1));
+        const float pvB = prof(ycb), pvuB = prof(min(ycb + 1, Y - 1));
+        const Rsrc rG = arrayRsrc(G.gh + (long)X * zb), rM = arrayRsrc(G.ghm1 + (long)X * zb);
+        auto pOff = [&](int pl) { return PB * (unsigned)(pl - zb); };
+        auto gOff = [&](int pl) { return RB * (unsigned)(pl - zb); };
+        const float den = 1 + 6 * k;
+        struct Own {
+            float a, b;  // the two rows
+            int gm, gf;  // ghm1 of the plane, ground height of the next plane
+        };
+        auto loadOwn = [&](int pl, Own& o) __attribute__((always_inline)) {
+            o.a = bldf(rV, obA, pOff(pl));
+            o.b = bldf(rV, obB, pOff(pl));
+            o.gm = bldi(rM, gb, gOff(pl));
+            o.gf = bldi(rG, gb, gOff(min(pl + 1, Z - 1)));
+        };
+        // plane below the chunk (clamped: plane 0 for the first chunk)
+        float vbA = bldf(rV, obA, pOff(zb)), vbB = bldf(rV, obB, pOff(zb));
+        int ghc = bldi(rG, gb, gOff(z0));
+        Own o0{}, o1{}, o2{}, o3{};
+        float h0 = 0.0f, h1 = 0.0f;
+        // prefetches past the chunk read the next chunk's first planes: measured
+        // faster than clamping them to the chunk's last plane
+        const int zlast = Z - 1;
+        loadOwn(z0, o0);
+        loadOwn(min(z0 + 1, zlast), o1);
+        loadOwn(min(z0 + 2, zlast), o2);
+        h0 = bldf(rV, hbL, pOff(z0));
+        // planes (bit z - z0) whose cell of row a / row b is next to ground
+        unsigned long long slowA = 0, slowB = 0;
+        const bool oL = x == 0, oR = x == X - 1;
+        // fast sweep of one cell: tile row r, this plane's value vc, the front
+        // plane's F (this cell's value on the last plane), the back plane's vb
+        auto sweep = [&](int buf, int r, int z, int y, float vc, float F, float vb, float pv, float pvu, unsigned ob)
+            __attribute__((always_inline)) {
+            const float L = tV[buf][r][tx], R = tV[buf][r][tx + 2];
+            const float D = tV[buf][r - 1][tx + 1], U = tV[buf][r + 1][tx + 1];
             const bool oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
-            // a position outside the domain clamps onto this sky cell, so the
-            // ground test at the clamped column is the in-domain test
-            const bool gL = y <= gz[xl], gR = y <= gz[xr], gD = y - 1 <= gc;
-            const bool gF = !oF && y <= gf, gB = !oB && y <= gbk;
-            const float vc = in[pz + xc + X * y];
-            const float L = in[pz + xl + X * y], R = in[pz + xr + X * y];
-            const float D = in[pz + xc + X * yd], U = in[pz + xc + X * yu];
-            const float F = in[XY * zf + xc + X * y], Bk = in[XY * zbk + xc + X * y];
-            float l = nbv(L, oL, gL, pv, b), r = nbv(R, oR, gR, pv, b);
-            float d = nbv(D, false, gD, pvd, b), u = nbv(U, oU, false, pvu, b);
-            float fr = nbv(F, oF, gF, pv, b), bb = nbv(Bk, oB, gB, pv, b);
-            if (fd.type == 0 && (gL | gR | gD | gF | gB)) {
-                const float pc = p[pz + xc + X * y];
-                auto tg = [&](int col) { return d_potTemp(gT[col] - 273.15f, pc, gP[col]) + 273.15f; };
-                if (gD) d = tg(x + X * z);
-                if (gL) l = tg(xl + X * z);
-                if (gR) r = tg(xr + X * z);
-                if (gF) fr = tg(x + X * (z + 1));
-                if (gB) bb = tg(x + X * (z - 1));
+            const float l = oL ? bcv(b.sides, L, pv) : L, rr = oR ? bcv(b.sides, R, pv) : R;
+            const float u = oU ? bcv(b.sides, U, pvu) : U;
+            const float fr = oF ? bcv(b.sides, F, pv) : F, bb = oB ? bcv(b.sides, vb, pv) : vb;
+            bstf(wV, ob, pOff(z), (vc + k * (l + rr + u + D + fr + bb)) / den);
+        };
+        auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, float& hc, float& hn) __attribute__((always_inline)) {
+            // loads complete in issue order: the halo (used next plane) goes first,
+            // so waiting for it does not wait for the plane read three ahead
+            hn = bldf(rV, hbL, pOff(min(z + 1, zlast)));
+            loadOwn(min(z + 3, zlast), o3n);
+            tV[buf][ty + 1][tx + 1] = oc.a;
+            tV[buf][ty + 5][tx + 1] = oc.b;
+            if (hasH) tV[buf][hrow][hcol] = hc;
+            __syncthreads();
+            const bool inZ = z + 1 < Z;
+            if (inA && ya > ghc) {
+                if (ya >= oc.gm + 2) sweep(buf, ty + 1, z, ya, oc.a, inZ ? of.a : oc.a, vbA, pvA, pvuA, obA);
+                else slowA |= 1ull << (z - z0);
             }
-            fd.out[pz + xc + X * y] = (vc + k * (l + r + u + d + fr + bb)) / den;
+            if (inB && yb4 > ghc) {
+                if (yb4 >= oc.gm + 2) sweep(buf, ty + 5, z, yb4, oc.b, inZ ? of.b : oc.b, vbB, pvB, pvuB, obB);
+                else slowB |= 1ull << (z - z0);
+            }
+            vbA = oc.a;
+            vbB = oc.b;
+            ghc = oc.gf;
+        };
+        for (int zg = z0; zg < z1; zg += 4) {
+            step(zg, 0, o0, o1, o3, h0, h1);
+            if (zg + 1 < z1) step(zg + 1, 1, o1, o2, o0, h1, h0);
+            if (zg + 2 < z1) step(zg + 2, 0, o2, o3, o1, h0, h1);
+            if (zg + 3 < z1) step(zg + 3, 1, o3, o0, o2, h1, h0);
         }
-    };
-    __shared__ unsigned long long sMask[8][64];
-    sMask[ty][tx] = slowA;
-    sMask[ty + 4][tx] = slowB;
-    __syncthreads();
-    const unsigned long long mine = 0x1111111111111111ull << ty;
-    for (int r = 0; r < 8; r++) {
-        const unsigned long long m = sMask[r][tx] & mine;
-        if (m) slowCells(m, yb + r);
+        // Sky cells next to ground (marked in slowA / slowB, one bit per plane of
+        // the chunk): the sweep reads only 'in', so they can follow the fast
+        // cells. The block's four waves share them: wave w takes the marked planes
+        // z0 + w, z0 + w + 4, ... of all eight rows of its columns, so a row of
+        // marked cells (all of them in one wave) does not hold the block for the
+        // whole chunk's length. Each neighbour is tested and takes the ground or sides condition
+        // (nbv); for the potential temperature a ground neighbour reads the
+        // potential temperature of the ground temperature at this cell's pressure
+        // (:95-100). Neighbour positions are clamped into the domain, as in the tile.
+        const int xl = max(xc - 1, 0), xr = min(xc + 1, X - 1);
+        const float* __restrict__ in = fd.in;
+        auto slowCells = [&](unsigned long long slow, int y) {
+            const int yd = max(y - 1, 0), yu = min(y + 1, Y - 1);
+            const float pv = prof(y), pvd = prof(yd), pvu = prof(yu);
+            while (slow) {
+                const int z = z0 + __builtin_ctzll(slow);
+                slow &= slow - 1;
+                const long pz = XY * z;
+                const int zf = min(z + 1, Z - 1), zbk = max(z - 1, 0);
+                const int* gz = G.gh + X * z;
+                const int gc = gz[xc], gf = G.gh[xc + X * zf], gbk = G.gh[xc + X * zbk];
+                const bool oU = y == Y - 1, oF = z == Z - 1, oB = z == 0;
+                // a position outside the domain clamps onto this sky cell, so the
+                // ground test at the clamped column is the in-domain test
+                const bool gL = y <= gz[xl], gR = y <= gz[xr], gD = y - 1 <= gc;
+                const bool gF = !oF && y <= gf, gB = !oB && y <= gbk;
+                const float vc = in[pz + xc + X * y];
+                const float L = in[pz + xl + X * y], R = in[pz + xr + X * y];
+                const float D = in[pz + xc + X * yd], U = in[pz + xc + X * yu];
+                const float F = in[XY * zf + xc + X * y], Bk = in[XY * zbk + xc + X * y];
+                float l = nbv(L, oL, gL, pv, b), r = nbv(R, oR, gR, pv, b);
+                float d = nbv(D, false, gD, pvd, b), u = nbv(U, oU, false, pvu, b);
+                float fr = nbv(F, oF, gF, pv, b), bb = nbv(Bk, oB, gB, pv, b);
+                if (fd.type == 0 && (gL | gR | gD | gF | gB)) {
+                    const float pc = p[pz + xc + X * y];
+                    auto tg = [&](int col) { return d_potTemp(gT[col] - 273.15f, pc, gP[col]) + 273.15f; };
+                    if (gD) d = tg(x + X * z);
+                    if (gL) l = tg(xl + X * z);
+                    if (gR) r = tg(xr + X * z);
+                    if (gF) fr = tg(x + X * (z + 1));
+                    if (gB) bb = tg(x + X * (z - 1));
+                }
+                fd.out[pz + xc + X * y] = (vc + k * (l + r + u + d + fr + bb)) / den;
+            }
+        };
+        __shared__ unsigned long long sMask[8][64];
+        sMask[ty][tx] = slowA;
+        sMask[ty + 4][tx] = slowB;
+        __syncthreads();
+        const unsigned long long mine = 0x1111111111111111ull << ty;
+        for (int r = 0; r < 8; r++) {
+            const unsigned long long m = sMask[r][tx] & mine;
+            if (m) slowCells(m, yb + r);
+        }
     }
-}
+};
 hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k) {
     const ZGrid zg = e.gD;
     if ((double)(zg.zc + 3) * (double)e.X * (double)e.Y * 4.0 >= 4294967296.0) return hipErrorInvalidValue;
-    klaunch(e, k_diffuse, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k,
-            e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
-    return hipGetLastError();
+    return klaunch<KDiffuse>(e, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k, e.f[SKT_GROUND_T],
+                             e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
 }
 
 // ------------------------------------------------------------- advection ---
@@ -336,66 +339,73 @@ hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k) {
 // value elsewhere (mode 0, quirk A6) or |a| everywhere (mode 1).
 constexpr int ZB_MAXV = 4;
 template <int ZB>
-KL void k_maxvel(Geom G, int zc, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
-                 int mode, double* part) {
-    ZM_SETUP;
-    __shared__ double sh[4];
-    double m = 0.0;
-    if (inb) {
-        const long XY = (long)G.X * G.Y;
-        const int rc = x + G.X * y;
-        for (int zg = z0; zg < z1; zg += ZB) {
-            float a0[ZB], a1[ZB], a2[ZB];
-            int gh[ZB];
-#pragma unroll
-            for (int j = 0; j < ZB; j++) {
-                const int z = zg + j;
-                if (z < z1) {
-                    const long i = XY * z + rc;
-                    a0[j] = u[i];
-                    a1[j] = v[i];
-                    a2[j] = w[i];
-                    gh[j] = G.gh[x + G.X * z];
+struct KMaxvel {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+                     int mode, double* part) {
+        ZM_SETUP_V;
+        __shared__ double sh[4];
+        double m = 0.0;
+        if (inb) {
+            const long XY = (long)G.X * G.Y;
+            const int rc = x + G.X * y;
+            for (int zg = z0; zg < z1; zg += ZB) {
+                float a0[ZB], a1[ZB], a2[ZB];
+                int gh[ZB];
+    #pragma unroll
+                for (int j = 0; j < ZB; j++) {
+                    const int z = zg + j;
+                    if (z < z1) {
+                        const long i = XY * z + rc;
+                        a0[j] = u[i];
+                        a1[j] = v[i];
+                        a2[j] = w[i];
+                        gh[j] = G.gh[x + G.X * z];
+                    }
                 }
-            }
-#pragma unroll
-            for (int j = 0; j < ZB; j++) {
-                const int z = zg + j;
-                if (z < z1 && y > gh[j]) {
-                    const bool ab = mode != 0 || z == 0;
-                    m = fmax(m, (double)(ab ? fabsf(a0[j]) : a0[j]));
-                    m = fmax(m, (double)(ab ? fabsf(a1[j]) : a1[j]));
-                    m = fmax(m, (double)(ab ? fabsf(a2[j]) : a2[j]));
+    #pragma unroll
+                for (int j = 0; j < ZB; j++) {
+                    const int z = zg + j;
+                    if (z < z1 && y > gh[j]) {
+                        const bool ab = mode != 0 || z == 0;
+                        m = fmax(m, (double)(ab ? fabsf(a0[j]) : a0[j]));
+                        m = fmax(m, (double)(ab ? fabsf(a1[j]) : a1[j]));
+                        m = fmax(m, (double)(ab ? fabsf(a2[j]) : a2[j]));
+                    }
                 }
             }
         }
+        m = blockMax(m, sh);
+        if (tid256() == 0) part[bid] = m;
     }
-    m = blockMax(m, sh);
-    if (tid256() == 0) part[bid] = m;
-}
+};
 // Substep count and length of advectPPMWGPU (Game/environment.cu:726-730).
-KL void k_maxvel_finalize(const double* __restrict__ part, unsigned n, Scalars* sc, float dt, float dx, int maxsub) {
-    __shared__ double sh[4];
-    double vmax = 0.0;
-    for (unsigned j = tid256(); j < n; j += 256) vmax = fmax(vmax, part[j]);
-    vmax = blockMax(vmax, sh);
-    if (tid256() != 0) return;
-    const float maxVel = (float)vmax;
-    const float C = maxVel * dt / dx;
-    const int ns = int(fminf(ceilf(fabsf(C)), (float)maxsub));
-    sc->maxvel = maxVel;
-    sc->nsub = ns;
-    sc->dtsub = ns > 0 ? dt / ns : 0.0f;
-}
+struct KMaxvelFinalize {
+    static __device__ __forceinline__ void run(VB vb, const double* __restrict__ part, unsigned n, Scalars* sc,
+                                               float dt, float dx, int maxsub, int cap) {
+        __shared__ double sh[4];
+        double vmax = 0.0;
+        for (unsigned j = tid256(); j < n; j += 256) vmax = fmax(vmax, part[j]);
+        vmax = blockMax(vmax, sh);
+        if (tid256() != 0) return;
+        const float maxVel = (float)vmax;
+        const float C = maxVel * dt / dx;
+        const int ns = int(fminf(ceilf(fabsf(C)), (float)maxsub));
+        sc->maxvel = maxVel;
+        sc->nsub = ns;
+        sc->dtsub = ns > 0 ? dt / ns : 0.0f;
+        if (cap > 0 && ns > cap) sc->sub_overflow = 1;
+    }
+};
 hipError_t launch_maxvel(EnvState& e, float dt) {
     const ZGrid zg = zgrid(e);
-    hipLaunchKernelGGL(k_maxvel<ZB_MAXV>, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.f[SKT_VELX], e.f[SKT_VELY],
-                       e.f[SKT_VELZ], e.cfg.maxabs_mode, e.partials);
-    hipError_t r = hipGetLastError();
+    hipError_t r = klaunch<KMaxvel<ZB_MAXV>>(e, zg.grid, B3, 0, geom(e), zg.zc, e.f[SKT_VELX], e.f[SKT_VELY],
+                                             e.f[SKT_VELZ], e.cfg.maxabs_mode, e.partials);
     if (r != hipSuccess) return r;
-    hipLaunchKernelGGL(k_maxvel_finalize, dim3(1), B3, 0, e.stream, e.partials, zg.nblocks, e.sc, dt, e.dx,
-                       e.cfg.max_substeps);
-    return hipGetLastError();
+    // a recorded ensemble step runs adv_cap substep passes: flag a member
+    // that needs more
+    const int cap = e.rec ? e.rec->adv_cap : 0;
+    return klaunch<KMaxvelFinalize>(e, dim3(1), B3, 0, e.partials, zg.nblocks, e.sc, dt, e.dx, e.cfg.max_substeps,
+                                    cap);
 }
 
 // One directional PPM pass of advectPPMX/Y/Z (Game/kernelSky.cu:368-574) fused
@@ -429,10 +439,21 @@ __device__ __forceinline__ float ppmCell(const float* q, const float* o, float v
     return ppmFinish(q, o, vc, vm, ppmFaceR(q, vc, dt, dx), ppmFaceL(q, vm, dt, dx), dt, dx);
 }
 
+// The pass's time step: the given one or, in a recorded ensemble step
+// (pb.dsc set), the member's advection substep length read on the device
+// (half of it for the x and y passes); false for a pass past the member's
+// substep count, which the block skips.
+__device__ __forceinline__ bool ppmPassDt(const PpmBatch& pb, float& dt) {
+    if (!pb.dsc) return true;
+    if (pb.sub >= pb.dsc->nsub) return false;
+    dt = pb.half ? pb.dsc->dtsub * 0.5f : pb.dsc->dtsub;
+    return true;
+}
+
 // Field of this block in a batched pass (the grid repeats the z-march grid
 // once per field).
 #define PPM_FIELD_SETUP                                                   \
-    const unsigned lin_ = xcdRemap(blockIdx.x, gridDim.x);                \
+    const unsigned lin_ = xcdRemap(vb.id, vb.n);                          \
     const PpmField& fd_ = pb.f[lin_ / nbf];                               \
     ZM_SETUP_B(lin_ % nbf);                                               \
     (void)bid;                                                            \
@@ -459,95 +480,100 @@ __device__ __forceinline__ float ppmCell(const float* q, const float* o, float v
 // rows (one per wave) over a chunk of zc planes of one field; the loads of up
 // to XSEG_PPM segments are issued before any is used.
 constexpr int XSEG_PPM = 2;
-KL void k_ppm_xr(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, const float* __restrict__ cust) {
-    const unsigned lin = xcdRemap(blockIdx.x, gridDim.x);
-    const PpmField& fd = pb.f[lin / nbf];
-    const unsigned rem = lin % nbf;
-    const int gy = (G.Y + 3) >> 2;
-    const int y = (int)(rem % (unsigned)gy) * 4 + (int)threadIdx.y;
-    const int z0 = (int)(rem / (unsigned)gy) * zc, z1 = min(G.Z, z0 + zc);
-    if (y >= G.Y || z0 >= z1) return;  // whole waves (one row per wave)
-    const float* __restrict__ in = fd.in;
-    float* __restrict__ out = fd.out;
-    const float* __restrict__ vel = fd.vel;
-    const float* __restrict__ prof = fd.prof;
-    const Bounds b = fd.b;
-    // value that stands at a ground position: the field's own ground value
-    // or 0 (read only under a NEUMANN ground condition)
-    const float* __restrict__ gsrc = fd.gsrc;
-    const bool gNeu = b.ground == BC_NEUMANN && gsrc != nullptr;
-    const int X = G.X, tx = (int)threadIdx.x;
-    const long XY = (long)X * G.Y;
-    const unsigned rowy = X * y;
-    const float pv = prof ? prof[y] : 0.0f, cv = cust[y];
-    const int nseg = (X + 63) >> 6;
-    for (int z = z0; z < z1; z++) {
-        const float* pl = in + XY * z + rowy;
-        const float* vl = vel + XY * z + rowy;
-        const int* gz = G.gh + X * z;
-        const int* gmz = G.ghm2 + X * z;
-        float carry = 0.0f;  // right-face flux of the last cell of the segment before
-        for (int k0 = 0; k0 < nseg; k0 += XSEG_PPM) {
-            float Q[XSEG_PPM][5], VC[XSEG_PPM], VM[XSEG_PPM];
-            int GM[XSEG_PPM], GC[XSEG_PPM];
-#pragma unroll
-            for (int j = 0; j < XSEG_PPM; j++) {
-                // lanes past the row (and segments past the last) read its last cell
-                const int xr = min((k0 + j) * 64 + tx, X - 1);
-#pragma unroll
-                for (int s = 0; s < 5; s++) Q[j][s] = pl[clampi(xr + s - 2, 0, X - 1)];
-                VC[j] = vl[xr];
-                VM[j] = vl[max(xr - 1, 0)];
-                GM[j] = gmz[xr];
-                GC[j] = gz[xr];
-            }
-#pragma unroll
-            for (int j = 0; j < XSEG_PPM; j++) {
-                if (k0 + j >= nseg) break;  // uniform
-                const int x = (k0 + j) * 64 + tx;
-                const bool sky = x < X && y > GC[j];
-                float q[5], o[5], vm = 0.0f, Fr = 0.0f;
-                bool leftSky = false;
-                if (sky) {
-                    if (y > GM[j]) {
-#pragma unroll
-                        for (int s = 0; s < 5; s++) {
-                            const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
-                            q[s] = outp ? bcv(b.sides, Q[j][s], pv) : Q[j][s];
-                            o[s] = outp ? (b.sides == BC_DIRICHLET ? 0.0f : 1.0f) : 1.0f;
+struct KPpmXr {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, unsigned nbf, const PpmBatch& pb,
+                                               float dtArg, float dx, const float* __restrict__ cust) {
+        float dt = dtArg;
+        if (!ppmPassDt(pb, dt)) return;  // the whole block
+        const unsigned lin = xcdRemap(vb.id, vb.n);
+        const PpmField& fd = pb.f[lin / nbf];
+        const unsigned rem = lin % nbf;
+        const int gy = (G.Y + 3) >> 2;
+        const int y = (int)(rem % (unsigned)gy) * 4 + (int)threadIdx.y;
+        const int z0 = (int)(rem / (unsigned)gy) * zc, z1 = min(G.Z, z0 + zc);
+        if (y >= G.Y || z0 >= z1) return;  // whole waves (one row per wave)
+        const float* __restrict__ in = fd.in;
+        float* __restrict__ out = fd.out;
+        const float* __restrict__ vel = fd.vel;
+        const float* __restrict__ prof = fd.prof;
+        const Bounds b = fd.b;
+        // value that stands at a ground position: the field's own ground value
+        // or 0 (read only under a NEUMANN ground condition)
+        const float* __restrict__ gsrc = fd.gsrc;
+        const bool gNeu = b.ground == BC_NEUMANN && gsrc != nullptr;
+        const int X = G.X, tx = (int)threadIdx.x;
+        const long XY = (long)X * G.Y;
+        const unsigned rowy = X * y;
+        const float pv = prof ? prof[y] : 0.0f, cv = cust[y];
+        const int nseg = (X + 63) >> 6;
+        for (int z = z0; z < z1; z++) {
+            const float* pl = in + XY * z + rowy;
+            const float* vl = vel + XY * z + rowy;
+            const int* gz = G.gh + X * z;
+            const int* gmz = G.ghm2 + X * z;
+            float carry = 0.0f;  // right-face flux of the last cell of the segment before
+            for (int k0 = 0; k0 < nseg; k0 += XSEG_PPM) {
+                float Q[XSEG_PPM][5], VC[XSEG_PPM], VM[XSEG_PPM];
+                int GM[XSEG_PPM], GC[XSEG_PPM];
+    #pragma unroll
+                for (int j = 0; j < XSEG_PPM; j++) {
+                    // lanes past the row (and segments past the last) read its last cell
+                    const int xr = min((k0 + j) * 64 + tx, X - 1);
+    #pragma unroll
+                    for (int s = 0; s < 5; s++) Q[j][s] = pl[clampi(xr + s - 2, 0, X - 1)];
+                    VC[j] = vl[xr];
+                    VM[j] = vl[max(xr - 1, 0)];
+                    GM[j] = gmz[xr];
+                    GC[j] = gz[xr];
+                }
+    #pragma unroll
+                for (int j = 0; j < XSEG_PPM; j++) {
+                    if (k0 + j >= nseg) break;  // uniform
+                    const int x = (k0 + j) * 64 + tx;
+                    const bool sky = x < X && y > GC[j];
+                    float q[5], o[5], vm = 0.0f, Fr = 0.0f;
+                    bool leftSky = false;
+                    if (sky) {
+                        if (y > GM[j]) {
+    #pragma unroll
+                            for (int s = 0; s < 5; s++) {
+                                const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
+                                q[s] = outp ? bcv(b.sides, Q[j][s], pv) : Q[j][s];
+                                o[s] = outp ? (b.sides == BC_DIRICHLET ? 0.0f : 1.0f) : 1.0f;
+                            }
+                            vm = x == 0 ? bcv(b.sides, VC[j], cv) : VM[j];
+                            leftSky = x > 0;
+                        } else {
+                            int gh[5];
+    #pragma unroll
+                            for (int s = 0; s < 5; s++) gh[s] = gz[clampi(x + s - 2, 0, X - 1)];
+    #pragma unroll
+                            for (int s = 0; s < 5; s++) {
+                                const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
+                                const bool gnd = y <= gh[s];
+                                const float raw =
+                                    gnd ? (gNeu ? gsrc[XY * z + rowy + clampi(x + s - 2, 0, X - 1)] : 0.0f) : Q[j][s];
+                                q[s] = nbv(raw, outp, gnd, pv, b);
+                                o[s] = unitv(outp, gnd, b);
+                            }
+                            vm = x == 0 ? bcv(b.sides, VC[j], cv) : (y <= gh[1] ? bcv(b.ground, VC[j], cv) : VM[j]);
+                            leftSky = x > 0 && y > gh[1];
                         }
-                        vm = x == 0 ? bcv(b.sides, VC[j], cv) : VM[j];
-                        leftSky = x > 0;
-                    } else {
-                        int gh[5];
-#pragma unroll
-                        for (int s = 0; s < 5; s++) gh[s] = gz[clampi(x + s - 2, 0, X - 1)];
-#pragma unroll
-                        for (int s = 0; s < 5; s++) {
-                            const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
-                            const bool gnd = y <= gh[s];
-                            const float raw =
-                                gnd ? (gNeu ? gsrc[XY * z + rowy + clampi(x + s - 2, 0, X - 1)] : 0.0f) : Q[j][s];
-                            q[s] = nbv(raw, outp, gnd, pv, b);
-                            o[s] = unitv(outp, gnd, b);
-                        }
-                        vm = x == 0 ? bcv(b.sides, VC[j], cv) : (y <= gh[1] ? bcv(b.ground, VC[j], cv) : VM[j]);
-                        leftSky = x > 0 && y > gh[1];
+                        Fr = ppmFaceR(q, VC[j], dt, dx);
                     }
-                    Fr = ppmFaceR(q, VC[j], dt, dx);
+                    // every lane of the wave takes part in both exchanges
+                    const float up = __shfl_up(Fr, 1);
+                    const float last = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Fr), 63));
+                    if (sky) {
+                        const float Fl = leftSky ? (tx > 0 ? up : carry) : ppmFaceL(q, vm, dt, dx);
+                        out[XY * z + rowy + x] = ppmFinish(q, o, VC[j], vm, Fr, Fl, dt, dx);
+                    }
+                    carry = last;
                 }
-                // every lane of the wave takes part in both exchanges
-                const float up = __shfl_up(Fr, 1);
-                const float last = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Fr), 63));
-                if (sky) {
-                    const float Fl = leftSky ? (tx > 0 ? up : carry) : ppmFaceL(q, vm, dt, dx);
-                    out[XY * z + rowy + x] = ppmFinish(q, o, VC[j], vm, Fr, Fl, dt, dx);
-                }
-                carry = last;
             }
         }
     }
-}
+};
 
 // y pass as a march up the columns: each thread owns the column (x, z) of
 // a chunk of yc levels; the five-point stencil is a register window that
@@ -560,161 +586,171 @@ KL void k_ppm_xr(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, 
 // it directly. Blocks are 64 (x) x 4 (z) columns of one chunk of one field.
 constexpr int YC_PPM = 16;
 constexpr int YG_PPM = 4;
-KL void k_ppm_ym(Geom G, int yc, unsigned nyc, unsigned nbf, PpmBatch pb, float dt, float dx) {
-    const unsigned lin = blockIdx.x;
-    const PpmField& fd = pb.f[lin / nbf];
-    const unsigned rem = lin % nbf;
-    const int chunk = (int)(rem % nyc);
-    const unsigned tile = rem / nyc;
-    const int gx = (G.X + 63) >> 6;
-    const int x = (int)(tile % (unsigned)gx) * 64 + threadIdx.x;
-    const int z = (int)(tile / (unsigned)gx) * 4 + threadIdx.y;
-    if (x >= G.X || z >= G.Z) return;
-    const float* __restrict__ in = fd.in;
-    float* __restrict__ out = fd.out;
-    const float* __restrict__ vel = fd.vel;
-    const float* __restrict__ prof = fd.prof;
-    const Bounds b = fd.b;
-    const float* __restrict__ gsrc = fd.gsrc;
-    const bool gNeu = b.ground == BC_NEUMANN && gsrc != nullptr;
-    const int X = G.X, Y = G.Y;
-    const long col = (long)X * Y * z + x;  // cell (x, 0, z)
-    const int gh = G.gh[x + (long)X * z];
-    const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
-    // field and unit values at level py as a stencil sees them, from the
-    // field's value at the clamped level
-    auto pos = [&](int py, float raw, float& qv, float& ov) {
-        const int by = clampi(py, 0, Y - 1);
-        const bool outp = (py < 0) || (py > Y - 1);
-        const bool gnd = by <= gh;
-        if (gnd) raw = gNeu ? gsrc[col + (long)X * by] : 0.0f;
-        qv = nbv(raw, outp, gnd, prof ? prof[by] : 0.0f, b);
-        ov = unitv(outp, gnd, b);
-    };
-    float q[5], o[5];
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const int py = y0 - 2 + s;
-        pos(py, in[col + (long)X * clampi(py, 0, Y - 1)], q[s], o[s]);
-    }
-    // raw field values at levels g + 2 .. g + 5 and face velocities at g .. g + 3
-    auto fetch = [&](int g, float* Q, float* V) {
-#pragma unroll
-        for (int j = 0; j < YG_PPM; j++) {
-            Q[j] = in[col + (long)X * min(g + j + 2, Y - 1)];
-            V[j] = vel[col + (long)X * min(g + j, Y - 1)];
+struct KPpmYm {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, unsigned nbf,
+                                               const PpmBatch& pb, float dtArg, float dx) {
+        float dt = dtArg;
+        if (!ppmPassDt(pb, dt)) return;  // the whole block
+        const unsigned lin = vb.id;
+        const PpmField& fd = pb.f[lin / nbf];
+        const unsigned rem = lin % nbf;
+        const int chunk = (int)(rem % nyc);
+        const unsigned tile = rem / nyc;
+        const int gx = (G.X + 63) >> 6;
+        const int x = (int)(tile % (unsigned)gx) * 64 + threadIdx.x;
+        const int z = (int)(tile / (unsigned)gx) * 4 + threadIdx.y;
+        if (x >= G.X || z >= G.Z) return;
+        const float* __restrict__ in = fd.in;
+        float* __restrict__ out = fd.out;
+        const float* __restrict__ vel = fd.vel;
+        const float* __restrict__ prof = fd.prof;
+        const Bounds b = fd.b;
+        const float* __restrict__ gsrc = fd.gsrc;
+        const bool gNeu = b.ground == BC_NEUMANN && gsrc != nullptr;
+        const int X = G.X, Y = G.Y;
+        const long col = (long)X * Y * z + x;  // cell (x, 0, z)
+        const int gh = G.gh[x + (long)X * z];
+        const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
+        // field and unit values at level py as a stencil sees them, from the
+        // field's value at the clamped level
+        auto pos = [&](int py, float raw, float& qv, float& ov) {
+            const int by = clampi(py, 0, Y - 1);
+            const bool outp = (py < 0) || (py > Y - 1);
+            const bool gnd = by <= gh;
+            if (gnd) raw = gNeu ? gsrc[col + (long)X * by] : 0.0f;
+            qv = nbv(raw, outp, gnd, prof ? prof[by] : 0.0f, b);
+            ov = unitv(outp, gnd, b);
+        };
+        float q[5], o[5];
+    #pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int py = y0 - 2 + s;
+            pos(py, in[col + (long)X * clampi(py, 0, Y - 1)], q[s], o[s]);
         }
-    };
-    float vPrev = vel[col + (long)X * max(y0 - 1, 0)];
-    float fPrev = 0.0f;
-    auto group = [&](int g, const float* Q, const float* V) {
-#pragma unroll
-        for (int j = 0; j < YG_PPM; j++) {
-            const int y = g + j;
-            if (y >= y1) break;
-            pos(y + 2, Q[j], q[4], o[4]);
-            const float vc = V[j];
-            if (y > gh) {
-                // y - 1 >= 0 on every sky cell
-                const bool below = y - 1 <= gh;
-                const float vm = below ? bcv(b.ground, vc, 0.0f) : vPrev;
-                const float Fr = ppmFaceR(q, vc, dt, dx);
-                const float Fl = (y > y0 && !below) ? fPrev : ppmFaceL(q, vm, dt, dx);
-                fPrev = Fr;
-                out[col + (long)X * y] = ppmFinish(q, o, vc, vm, Fr, Fl, dt, dx);
+        // raw field values at levels g + 2 .. g + 5 and face velocities at g .. g + 3
+        auto fetch = [&](int g, float* Q, float* V) {
+    #pragma unroll
+            for (int j = 0; j < YG_PPM; j++) {
+                Q[j] = in[col + (long)X * min(g + j + 2, Y - 1)];
+                V[j] = vel[col + (long)X * min(g + j, Y - 1)];
             }
-            vPrev = vc;
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                q[s] = q[s + 1];
-                o[s] = o[s + 1];
+        };
+        float vPrev = vel[col + (long)X * max(y0 - 1, 0)];
+        float fPrev = 0.0f;
+        auto group = [&](int g, const float* Q, const float* V) {
+    #pragma unroll
+            for (int j = 0; j < YG_PPM; j++) {
+                const int y = g + j;
+                if (y >= y1) break;
+                pos(y + 2, Q[j], q[4], o[4]);
+                const float vc = V[j];
+                if (y > gh) {
+                    // y - 1 >= 0 on every sky cell
+                    const bool below = y - 1 <= gh;
+                    const float vm = below ? bcv(b.ground, vc, 0.0f) : vPrev;
+                    const float Fr = ppmFaceR(q, vc, dt, dx);
+                    const float Fl = (y > y0 && !below) ? fPrev : ppmFaceL(q, vm, dt, dx);
+                    fPrev = Fr;
+                    out[col + (long)X * y] = ppmFinish(q, o, vc, vm, Fr, Fl, dt, dx);
+                }
+                vPrev = vc;
+    #pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    q[s] = q[s + 1];
+                    o[s] = o[s + 1];
+                }
             }
+        };
+        float Qa[YG_PPM], Va[YG_PPM], Qb[YG_PPM], Vb[YG_PPM];
+        fetch(y0, Qa, Va);
+        for (int g = y0; g < y1; g += 2 * YG_PPM) {
+            if (g + YG_PPM < y1) fetch(g + YG_PPM, Qb, Vb);
+            group(g, Qa, Va);
+            if (g + YG_PPM >= y1) break;
+            if (g + 2 * YG_PPM < y1) fetch(g + 2 * YG_PPM, Qa, Va);
+            group(g + YG_PPM, Qb, Vb);
         }
-    };
-    float Qa[YG_PPM], Va[YG_PPM], Qb[YG_PPM], Vb[YG_PPM];
-    fetch(y0, Qa, Va);
-    for (int g = y0; g < y1; g += 2 * YG_PPM) {
-        if (g + YG_PPM < y1) fetch(g + YG_PPM, Qb, Vb);
-        group(g, Qa, Va);
-        if (g + YG_PPM >= y1) break;
-        if (g + 2 * YG_PPM < y1) fetch(g + 2 * YG_PPM, Qa, Va);
-        group(g + YG_PPM, Qb, Vb);
     }
-}
+};
 
 constexpr int ZB_PPM = 4;  // planes of the z pass whose loads go together
 // z pass: the stencil runs along this thread's march; a window of planes
 // zg-2 .. zg+ZB+1 (values and ground heights) stays in registers, so the
 // boundary rule needs no extra loads.
 template <int ZB>
-KL void k_ppm_z(Geom G, int zc, unsigned nbf, PpmBatch pb, float dt, float dx, const float* __restrict__ cust) {
-    PPM_FIELD_SETUP;
-    if (!inb || z0 >= z1) return;
-    const int X = G.X, Z = G.Z;
-    const long XY = (long)X * G.Y;
-    const unsigned rc = x + X * y;
-    const float pv = prof ? prof[y] : 0.0f, cv = cust[y];
-    float qw[ZB + 4], vw[ZB + 1];
-    int gw[ZB + 4];
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const int bz = clampi(z0 - 2 + s, 0, Z - 1);
-        qw[s] = (in + XY * bz)[rc];
-        gw[s] = (G.gh + X * bz)[(unsigned)x];
-    }
-    vw[0] = (vel + XY * max(z0 - 1, 0))[rc];
-    // window slot m holds plane zg - 2 + m; velocity slot m plane zg - 1 + m
-    auto load = [&](int z, float& q, int& g, float& v) {
-        const int bz = min(z + 2, Z - 1);
-        q = (in + XY * bz)[rc];
-        g = (G.gh + X * bz)[(unsigned)x];
-        v = (vel + XY * z)[rc];
-    };
-    // The flux through the back face of plane z is the front-face flux of
-    // plane z - 1 when this thread computed that plane (same face velocity,
-    // same stencil values); else it is evaluated here.
-    float fPrev = 0.0f;
-    auto plane = [&](int z, const float* qs, const int* gs, float vc, float vmr) {
-        if (y <= gs[2]) return;
-        float q[5], o[5];
-#pragma unroll
-        for (int s = 0; s < 5; s++) {
-            const int pz = z + s - 2;
-            const bool outp = (pz < 0) || (pz > Z - 1);
-            const bool gnd = y <= gs[s];
-            q[s] = nbv(gnd ? graw(XY * clampi(pz, 0, Z - 1) + rc) : qs[s], outp, gnd, pv, b);
-            o[s] = unitv(outp, gnd, b);
-        }
-        const float vm = z == 0 ? bcv(b.sides, vc, cv) : (y <= gs[1] ? bcv(b.ground, vc, cv) : vmr);
-        const float Fr = ppmFaceR(q, vc, dt, dx);
-        const float Fl = (z > z0 && y > gs[1]) ? fPrev : ppmFaceL(q, vm, dt, dx);
-        fPrev = Fr;
-        (out + XY * z)[rc] = ppmFinish(q, o, vc, vm, Fr, Fl, dt, dx);
-    };
-    int zg = z0;
-    for (; zg + ZB <= z1; zg += ZB) {
-#pragma unroll
-        for (int j = 0; j < ZB; j++) load(zg + j, qw[4 + j], gw[4 + j], vw[1 + j]);
-#pragma unroll
-        for (int j = 0; j < ZB; j++) plane(zg + j, qw + j, gw + j, vw[1 + j], vw[j]);
-#pragma unroll
+struct KPpmZ {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, unsigned nbf, const PpmBatch& pb,
+                                               float dtArg, float dx, const float* __restrict__ cust) {
+        float dt = dtArg;
+        if (!ppmPassDt(pb, dt)) return;  // the whole block
+        PPM_FIELD_SETUP;
+        if (!inb || z0 >= z1) return;
+        const int X = G.X, Z = G.Z;
+        const long XY = (long)X * G.Y;
+        const unsigned rc = x + X * y;
+        const float pv = prof ? prof[y] : 0.0f, cv = cust[y];
+        float qw[ZB + 4], vw[ZB + 1];
+        int gw[ZB + 4];
+    #pragma unroll
         for (int s = 0; s < 4; s++) {
-            qw[s] = qw[s + ZB];
-            gw[s] = gw[s + ZB];
+            const int bz = clampi(z0 - 2 + s, 0, Z - 1);
+            qw[s] = (in + XY * bz)[rc];
+            gw[s] = (G.gh + X * bz)[(unsigned)x];
         }
-        vw[0] = vw[ZB];
-    }
-    for (; zg < z1; zg++) {
-        load(zg, qw[4], gw[4], vw[1]);
-        plane(zg, qw, gw, vw[1], vw[0]);
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            qw[s] = qw[s + 1];
-            gw[s] = gw[s + 1];
+        vw[0] = (vel + XY * max(z0 - 1, 0))[rc];
+        // window slot m holds plane zg - 2 + m; velocity slot m plane zg - 1 + m
+        auto load = [&](int z, float& q, int& g, float& v) {
+            const int bz = min(z + 2, Z - 1);
+            q = (in + XY * bz)[rc];
+            g = (G.gh + X * bz)[(unsigned)x];
+            v = (vel + XY * z)[rc];
+        };
+        // The flux through the back face of plane z is the front-face flux of
+        // plane z - 1 when this thread computed that plane (same face velocity,
+        // same stencil values); else it is evaluated here.
+        float fPrev = 0.0f;
+        auto plane = [&](int z, const float* qs, const int* gs, float vc, float vmr) {
+            if (y <= gs[2]) return;
+            float q[5], o[5];
+    #pragma unroll
+            for (int s = 0; s < 5; s++) {
+                const int pz = z + s - 2;
+                const bool outp = (pz < 0) || (pz > Z - 1);
+                const bool gnd = y <= gs[s];
+                q[s] = nbv(gnd ? graw(XY * clampi(pz, 0, Z - 1) + rc) : qs[s], outp, gnd, pv, b);
+                o[s] = unitv(outp, gnd, b);
+            }
+            const float vm = z == 0 ? bcv(b.sides, vc, cv) : (y <= gs[1] ? bcv(b.ground, vc, cv) : vmr);
+            const float Fr = ppmFaceR(q, vc, dt, dx);
+            const float Fl = (z > z0 && y > gs[1]) ? fPrev : ppmFaceL(q, vm, dt, dx);
+            fPrev = Fr;
+            (out + XY * z)[rc] = ppmFinish(q, o, vc, vm, Fr, Fl, dt, dx);
+        };
+        int zg = z0;
+        for (; zg + ZB <= z1; zg += ZB) {
+    #pragma unroll
+            for (int j = 0; j < ZB; j++) load(zg + j, qw[4 + j], gw[4 + j], vw[1 + j]);
+    #pragma unroll
+            for (int j = 0; j < ZB; j++) plane(zg + j, qw + j, gw + j, vw[1 + j], vw[j]);
+    #pragma unroll
+            for (int s = 0; s < 4; s++) {
+                qw[s] = qw[s + ZB];
+                gw[s] = gw[s + ZB];
+            }
+            vw[0] = vw[ZB];
         }
-        vw[0] = vw[1];
+        for (; zg < z1; zg++) {
+            load(zg, qw[4], gw[4], vw[1]);
+            plane(zg, qw, gw, vw[1], vw[0]);
+    #pragma unroll
+            for (int s = 0; s < 4; s++) {
+                qw[s] = qw[s + 1];
+                gw[s] = gw[s + 1];
+            }
+            vw[0] = vw[1];
+        }
     }
-}
+};
 
 hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) {
     const ZGrid zg = zgrid(e);
@@ -725,17 +761,14 @@ hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) 
         const long nzc = std::max(1L, std::min((long)e.Z, 2048 / gy));
         const int zc = (int)((e.Z + nzc - 1) / nzc);
         const unsigned nbf = (unsigned)(gy * ((e.Z + zc - 1) / zc));
-        klaunch(e, k_ppm_xr, dim3(nbf * pb.n), B3, 0, geom(e), zc, nbf, pb, dt, e.dx, e.f[SKT_DEFAULT_VELX]);
+        return klaunch<KPpmXr>(e, dim3(nbf * pb.n), B3, 0, geom(e), zc, nbf, pb, dt, e.dx, e.f[SKT_DEFAULT_VELX]);
     }
-    else if (dir == 1) {
+    if (dir == 1) {
         const unsigned nyc = (unsigned)((e.Y + YC_PPM - 1) / YC_PPM);
         const unsigned nbf = (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * nyc;
-        klaunch(e, k_ppm_ym, dim3(nbf * pb.n), B3, 0, geom(e), YC_PPM, nyc, nbf, pb, dt, e.dx);
+        return klaunch<KPpmYm>(e, dim3(nbf * pb.n), B3, 0, geom(e), YC_PPM, nyc, nbf, pb, dt, e.dx);
     }
-    else
-        klaunch(e, k_ppm_z<ZB_PPM>, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
-                e.f[SKT_DEFAULT_VELZ]);
-    return hipGetLastError();
+    return klaunch<KPpmZ<ZB_PPM>>(e, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx, e.f[SKT_DEFAULT_VELZ]);
 }
 
 // ---------------------------------------------------- pressure projection ---
@@ -753,82 +786,84 @@ static unsigned divBlocks(const EnvState& e) {
 // {N,D,D}. The operator is not stored: the SpMV rebuilds its coefficients
 // from 1/density (irho, 4 B per cell instead of 16 for a float4 operator).
 // This pass writes irho and the Jacobi preconditioner 1/diagonal.
-KL void k_set_int(int* a, int v) {
-    if (threadIdx.x == 0) *a = v;
-}
+struct KSetInt {
+    static __device__ __forceinline__ void run(VB vb, int* a, int v) {
+        if (threadIdx.x == 0) *a = v;
+    }
+};
 // A march up the columns in the chunks of the divergence march: the ground
 // heights of the column and its four neighbour columns are read once and the
 // densities of the levels below and above are carried. Each neighbour read
 // follows extv: beyond the domain the sides condition on the cell's own
 // density (the clamped position), at ground the ground condition.
-KL void k_build_precon(Geom G, int yc, unsigned nyc, const float* __restrict__ rho, float* __restrict__ irho,
-                       float* __restrict__ pre, int* __restrict__ faceExact) {
-    const unsigned lin = blockIdx.x;
-    const int chunk = (int)(lin % nyc);
-    const unsigned tile = lin / nyc;
-    const int gx = (G.X + 63) >> 6;
-    const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
-    const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
-    if (x >= G.X || z >= G.Z) return;
-    const int X = G.X, Y = G.Y, Z = G.Z;
-    const long XY = (long)X * Y;
-    const long col = XY * z + x;  // cell (x, 0, z)
-    const long gc = x + (long)X * z;
-    const int gh = G.gh[gc];
-    const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
-    const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
-    const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
-    const Bounds b{BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET};
-    const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
-    if (y0 >= y1) return;
-    float rd = y0 > 0 ? rho[col + (long)X * (y0 - 1)] : 0.0f;  // density of the level below
-    float c = rho[col + (long)X * y0];
-    bool bad = false;
-    for (int y = y0; y < y1; y++) {
-        const long i = col + (long)X * y;
-        const float ru = y < Y - 1 ? rho[i + X] : 0.0f;
-        if (y <= gh) {
-            irho[i] = 0.0f;
-            pre[i] = 1.0f;
-        } else {
-            const float nR = !inR ? bcv(b.sides, c, 0.0f) : (y <= ghR ? bcv(b.ground, rho[i + 1], 0.0f) : rho[i + 1]);
-            const float nL = !inL ? bcv(b.sides, c, 0.0f) : (y <= ghL ? bcv(b.ground, rho[i - 1], 0.0f) : rho[i - 1]);
-            const float nU = y + 1 > Y - 1 ? bcv(b.sides, c, 0.0f) : ru;  // the level above a sky cell is sky
-            const float nD = y - 1 < 0 ? bcv(b.sides, c, 0.0f) : (y - 1 <= gh ? bcv(b.ground, rd, 0.0f) : rd);
-            const float nF = !inF ? bcv(b.sides, c, 0.0f) : (y <= ghF ? bcv(b.ground, rho[i + XY], 0.0f) : rho[i + XY]);
-            const float nB = !inB ? bcv(b.sides, c, 0.0f) : (y <= ghB ? bcv(b.ground, rho[i - XY], 0.0f) : rho[i - XY]);
-            const float r = faceCoef(c, nR);
-            const float l = faceCoef(c, nL);
-            const float u = faceCoef(c, nU);
-            const float d = faceCoef(c, nD);
-            const float f = faceCoef(c, nF);
-            const float bb = faceCoef(c, nB);
-            float w = 0.0f;
-            w -= r;
-            w -= l;
-            w -= u;
-            w -= d;
-            w -= f;
-            w -= bb;
-            const float ic = 1.0f / c;
-            irho[i] = ic;
-            pre[i] = w == 0.0f ? 1.0f : 1.0f / w;
-            // the SpMV's short face form is exact when every sky 1/density is in range
-            bad |= !faceValueOk(ic);
+struct KBuildPrecon {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, const float* __restrict__ rho, float* __restrict__ irho,
+                           float* __restrict__ pre, int* __restrict__ faceExact) {
+        const unsigned lin = vb.id;
+        const int chunk = (int)(lin % nyc);
+        const unsigned tile = lin / nyc;
+        const int gx = (G.X + 63) >> 6;
+        const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
+        const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
+        if (x >= G.X || z >= G.Z) return;
+        const int X = G.X, Y = G.Y, Z = G.Z;
+        const long XY = (long)X * Y;
+        const long col = XY * z + x;  // cell (x, 0, z)
+        const long gc = x + (long)X * z;
+        const int gh = G.gh[gc];
+        const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
+        const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
+        const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
+        const Bounds b{BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET};
+        const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
+        if (y0 >= y1) return;
+        float rd = y0 > 0 ? rho[col + (long)X * (y0 - 1)] : 0.0f;  // density of the level below
+        float c = rho[col + (long)X * y0];
+        bool bad = false;
+        for (int y = y0; y < y1; y++) {
+            const long i = col + (long)X * y;
+            const float ru = y < Y - 1 ? rho[i + X] : 0.0f;
+            if (y <= gh) {
+                irho[i] = 0.0f;
+                pre[i] = 1.0f;
+            } else {
+                const float nR = !inR ? bcv(b.sides, c, 0.0f) : (y <= ghR ? bcv(b.ground, rho[i + 1], 0.0f) : rho[i + 1]);
+                const float nL = !inL ? bcv(b.sides, c, 0.0f) : (y <= ghL ? bcv(b.ground, rho[i - 1], 0.0f) : rho[i - 1]);
+                const float nU = y + 1 > Y - 1 ? bcv(b.sides, c, 0.0f) : ru;  // the level above a sky cell is sky
+                const float nD = y - 1 < 0 ? bcv(b.sides, c, 0.0f) : (y - 1 <= gh ? bcv(b.ground, rd, 0.0f) : rd);
+                const float nF = !inF ? bcv(b.sides, c, 0.0f) : (y <= ghF ? bcv(b.ground, rho[i + XY], 0.0f) : rho[i + XY]);
+                const float nB = !inB ? bcv(b.sides, c, 0.0f) : (y <= ghB ? bcv(b.ground, rho[i - XY], 0.0f) : rho[i - XY]);
+                const float r = faceCoef(c, nR);
+                const float l = faceCoef(c, nL);
+                const float u = faceCoef(c, nU);
+                const float d = faceCoef(c, nD);
+                const float f = faceCoef(c, nF);
+                const float bb = faceCoef(c, nB);
+                float w = 0.0f;
+                w -= r;
+                w -= l;
+                w -= u;
+                w -= d;
+                w -= f;
+                w -= bb;
+                const float ic = 1.0f / c;
+                irho[i] = ic;
+                pre[i] = w == 0.0f ? 1.0f : 1.0f / w;
+                // the SpMV's short face form is exact when every sky 1/density is in range
+                bad |= !faceValueOk(ic);
+            }
+            rd = c;
+            c = ru;
         }
-        rd = c;
-        c = ru;
+        if (bad) atomicOr(faceExact, 1);
     }
-    if (bad) atomicOr(faceExact, 1);
-}
+};
 hipError_t launch_build_precon(EnvState& e) {
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, e.stream, &e.sc->face_exact, 0);
-    hipError_t r = hipGetLastError();
+    hipError_t r = klaunch<KSetInt>(e, dim3(1), dim3(64), 0, &e.sc->face_exact, 0);
     if (r != hipSuccess) return r;
     const unsigned nyc = (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
-    hipLaunchKernelGGL(k_build_precon, dim3(divBlocks(e)), B3, 0, e.stream, geom(e), YC_DIV, nyc, e.f[SKT_DENSITY],
-                       e.irho, e.pcg[5], &e.sc->face_exact);
-    return hipGetLastError();
+    return klaunch<KBuildPrecon>(e, dim3(divBlocks(e)), B3, 0, geom(e), YC_DIV, nyc, e.f[SKT_DENSITY], e.irho,
+                                 e.pcg[5], &e.sc->face_exact);
 }
 
 // Scalar steps of the solve, run by every block of the consuming kernel
@@ -865,7 +900,7 @@ __device__ __forceinline__ void pcgPartLoad(const double* __restrict__ pm, const
 template <int KIND>
 __device__ void pcgPrologue(const double* __restrict__ pm, const double* __restrict__ ps, unsigned n,
                             const PcgPart& P, const PcgState* __restrict__ sin, PcgState* __restrict__ sout, int ref,
-                            float tol, int it, PcgState& S, double* sh) {
+                            float tol, int it, PcgState& S, double* sh, unsigned tile) {
     const int t = tid256();
     if (t == 0) {
         if (KIND == PRO_START) {
@@ -926,7 +961,7 @@ __device__ void pcgPrologue(const double* __restrict__ pm, const double* __restr
             }
         }
     }
-    if (t == 0 && blockIdx.x == 0) *sout = S;
+    if (t == 0 && tile == 0) *sout = S;
     __syncthreads();
 }
 
@@ -941,118 +976,120 @@ __device__ void pcgPrologue(const double* __restrict__ pm, const double* __restr
 // a ground position the ground condition (the level above a sky cell is
 // sky); the arithmetic per cell is unchanged. Blocks are 64 (x) x 4 (z)
 // columns of one chunk, each writing one partial maximum.
-KL void k_divergence(Geom G, int yc, unsigned nyc, float* __restrict__ r, float* __restrict__ pp,
-                     const float* __restrict__ rho, const float* __restrict__ rold, const float* __restrict__ u,
-                     const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ defU,
-                     const float* __restrict__ defW, float dt, int absmode, double* part) {
-    __shared__ double sh[4];
-    const unsigned lin = blockIdx.x;
-    const int chunk = (int)(lin % nyc);
-    const unsigned tile = lin / nyc;
-    const int gx = (G.X + 63) >> 6;
-    const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
-    const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
-    double m = 0.0;
-    if (x < G.X && z < G.Z) {
-        const int X = G.X, Y = G.Y, Z = G.Z;
-        const long XY = (long)X * Y;
-        const long col = XY * z + x;  // cell (x, 0, z)
-        const long gc = x + (long)X * z;
-        const int gh = G.gh[gc];
-        const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
-        const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
-        const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
-        const Bounds bd{BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET}, bv{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET};
-        const bool ab = absmode || z == 0;
-        const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
-        float rhoB = 0.0f, vB = 0.0f;  // density and v of the level below
-        if (y0 > 0 && y0 < y1) {
-            rhoB = rho[col + (long)X * (y0 - 1)];
-            vB = v[col + (long)X * (y0 - 1)];
-        }
-        for (int y = y0; y < y1; y++) {
-            const long i = col + (long)X * y;
-            pp[i] = 0.0f;
-            const float cur = rho[i], vc = v[i];
-            float dv = 0.0f;
-            if (y > gh) {
-                const float ur = u[i];
-                const float ul = !inL ? bcv(bv.sides, ur, defU[y]) : (y <= ghL ? bcv(bv.ground, ur, defU[y]) : u[i - 1]);
-                const float uu = vc;
-                const float ud = y - 1 < 0 ? bcv(bv.sides, vc, 0.0f) : (y - 1 <= gh ? bcv(bv.ground, vc, 0.0f) : vB);
-                const float uf = w[i];
-                const float ub = !inB ? bcv(bv.sides, uf, defW[y]) : (y <= ghB ? bcv(bv.ground, uf, defW[y]) : w[i - XY]);
-                float dr = !inR ? bcv(bd.sides, cur, 1.0f) : (y <= ghR ? bcv(bd.ground, rho[i + 1], 1.0f) : rho[i + 1]);
-                float dl = !inL ? bcv(bd.sides, cur, 1.0f) : (y <= ghL ? bcv(bd.ground, rho[i - 1], 1.0f) : rho[i - 1]);
-                float du = y + 1 > Y - 1 ? bcv(bd.sides, cur, 1.0f) : rho[i + X];
-                float dd = y - 1 < 0 ? bcv(bd.sides, cur, 1.0f) : (y - 1 <= gh ? bcv(bd.ground, rhoB, 1.0f) : rhoB);
-                float df = !inF ? bcv(bd.sides, cur, 1.0f) : (y <= ghF ? bcv(bd.ground, rho[i + XY], 1.0f) : rho[i + XY]);
-                float db = !inB ? bcv(bd.sides, cur, 1.0f) : (y <= ghB ? bcv(bd.ground, rho[i - XY], 1.0f) : rho[i - XY]);
-                dr = dr == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dr);
-                dl = dl == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dl);
-                du = du == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / du);
-                dd = dd == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dd);
-                df = df == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / df);
-                db = db == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / db);
-                const float mfd = ((dr * ur - dl * ul) + (du * uu - dd * ud) + (df * uf - db * ub));
-                const float dc = (cur - rold[i]) / dt;
-                dv = (mfd - dc);
-                m = fmax(m, (double)(ab ? fabsf(dv) : dv));
+struct KDivergence {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, float* __restrict__ r, float* __restrict__ pp,
+                         const float* __restrict__ rho, const float* __restrict__ rold, const float* __restrict__ u,
+                         const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ defU,
+                         const float* __restrict__ defW, float dt, int absmode, double* part) {
+        __shared__ double sh[4];
+        const unsigned lin = vb.id;
+        const int chunk = (int)(lin % nyc);
+        const unsigned tile = lin / nyc;
+        const int gx = (G.X + 63) >> 6;
+        const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
+        const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
+        double m = 0.0;
+        if (x < G.X && z < G.Z) {
+            const int X = G.X, Y = G.Y, Z = G.Z;
+            const long XY = (long)X * Y;
+            const long col = XY * z + x;  // cell (x, 0, z)
+            const long gc = x + (long)X * z;
+            const int gh = G.gh[gc];
+            const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
+            const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
+            const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
+            const Bounds bd{BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET}, bv{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET};
+            const bool ab = absmode || z == 0;
+            const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
+            float rhoB = 0.0f, vB = 0.0f;  // density and v of the level below
+            if (y0 > 0 && y0 < y1) {
+                rhoB = rho[col + (long)X * (y0 - 1)];
+                vB = v[col + (long)X * (y0 - 1)];
             }
-            r[i] = dv;
-            rhoB = cur;
-            vB = vc;
+            for (int y = y0; y < y1; y++) {
+                const long i = col + (long)X * y;
+                pp[i] = 0.0f;
+                const float cur = rho[i], vc = v[i];
+                float dv = 0.0f;
+                if (y > gh) {
+                    const float ur = u[i];
+                    const float ul = !inL ? bcv(bv.sides, ur, defU[y]) : (y <= ghL ? bcv(bv.ground, ur, defU[y]) : u[i - 1]);
+                    const float uu = vc;
+                    const float ud = y - 1 < 0 ? bcv(bv.sides, vc, 0.0f) : (y - 1 <= gh ? bcv(bv.ground, vc, 0.0f) : vB);
+                    const float uf = w[i];
+                    const float ub = !inB ? bcv(bv.sides, uf, defW[y]) : (y <= ghB ? bcv(bv.ground, uf, defW[y]) : w[i - XY]);
+                    float dr = !inR ? bcv(bd.sides, cur, 1.0f) : (y <= ghR ? bcv(bd.ground, rho[i + 1], 1.0f) : rho[i + 1]);
+                    float dl = !inL ? bcv(bd.sides, cur, 1.0f) : (y <= ghL ? bcv(bd.ground, rho[i - 1], 1.0f) : rho[i - 1]);
+                    float du = y + 1 > Y - 1 ? bcv(bd.sides, cur, 1.0f) : rho[i + X];
+                    float dd = y - 1 < 0 ? bcv(bd.sides, cur, 1.0f) : (y - 1 <= gh ? bcv(bd.ground, rhoB, 1.0f) : rhoB);
+                    float df = !inF ? bcv(bd.sides, cur, 1.0f) : (y <= ghF ? bcv(bd.ground, rho[i + XY], 1.0f) : rho[i + XY]);
+                    float db = !inB ? bcv(bd.sides, cur, 1.0f) : (y <= ghB ? bcv(bd.ground, rho[i - XY], 1.0f) : rho[i - XY]);
+                    dr = dr == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dr);
+                    dl = dl == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dl);
+                    du = du == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / du);
+                    dd = dd == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dd);
+                    df = df == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / df);
+                    db = db == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / db);
+                    const float mfd = ((dr * ur - dl * ul) + (du * uu - dd * ud) + (df * uf - db * ub));
+                    const float dc = (cur - rold[i]) / dt;
+                    dv = (mfd - dc);
+                    m = fmax(m, (double)(ab ? fabsf(dv) : dv));
+                }
+                r[i] = dv;
+                rhoB = cur;
+                vB = vc;
+            }
         }
+        m = blockMax(m, sh);
+        if (tid256() == 0) part[vb.id] = m;
     }
-    m = blockMax(m, sh);
-    if (tid256() == 0) part[blockIdx.x] = m;
-}
+};
 hipError_t launch_divergence(EnvState& e, float dt) {
     const int ref = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     const int absmode = ref ? e.cfg.maxabs_mode : 1;
     const unsigned nyc = (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
-    hipLaunchKernelGGL(k_divergence, dim3(divBlocks(e)), B3, 0, e.stream, geom(e), YC_DIV, nyc, e.pcg[0],
-                       e.f[SKT_PRESSURE_PROJ], e.f[SKT_DENSITY], e.f[SKT_DENSITY_OLD], e.f[SKT_VELX], e.f[SKT_VELY],
-                       e.f[SKT_VELZ], e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ], dt, absmode,
-                       e.partials + (size_t)PART_DIV * e.maxBlocks);
-    return hipGetLastError();
+    return klaunch<KDivergence>(e, dim3(divBlocks(e)), B3, 0, geom(e), YC_DIV, nyc, e.pcg[0], e.f[SKT_PRESSURE_PROJ],
+                                e.f[SKT_DENSITY], e.f[SKT_DENSITY_OLD], e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ],
+                                e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ], dt, absmode,
+                                e.partials + (size_t)PART_DIV * e.maxBlocks);
 }
 
 // Prologue PRO_START, then z = M^-1 r (applyPreconditionerGPU,
 // Game/kernelSky.cu:659-673) and the z.r partials (dotProductGPU, :675-717).
-KL void k_pcg_start(Geom G, int zc, const float* __restrict__ r, float* __restrict__ zz, const float* __restrict__ pre,
-                    const double* __restrict__ pin, unsigned n, double* pout, const PcgState* sin, PcgState* sout,
-                    int ref, float tol) {
-    ZM_SETUP;
-    __shared__ double sh[4];
-    __shared__ PcgState S;
-    PcgPart P;
-    pcgPartLoad<PRO_START>(pin, nullptr, n, P);
-    pcgPrologue<PRO_START>(pin, nullptr, n, P, sin, sout, ref, tol, 0, S, sh);
-    if (S.done) return;
-    double s = 0.0;
-    if (inb) {
-        for (int z = z0; z < z1; z++) {
-            const long i = gidx(G, x, y, z);
-            float zv = 0.0f;
-            if (!isGround(G, x, y, z)) {
-                zv = r[i] * pre[i];
-                s += (double)(zv * r[i]);
+struct KPcgStart {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, const float* __restrict__ r, float* __restrict__ zz, const float* __restrict__ pre,
+                        const double* __restrict__ pin, unsigned n, double* pout, const PcgState* sin, PcgState* sout,
+                        int ref, float tol) {
+        ZM_SETUP_V;
+        __shared__ double sh[4];
+        __shared__ PcgState S;
+        PcgPart P;
+        pcgPartLoad<PRO_START>(pin, nullptr, n, P);
+        pcgPrologue<PRO_START>(pin, nullptr, n, P, sin, sout, ref, tol, 0, S, sh, vb.id);
+        if (S.done) return;
+        double s = 0.0;
+        if (inb) {
+            for (int z = z0; z < z1; z++) {
+                const long i = gidx(G, x, y, z);
+                float zv = 0.0f;
+                if (!isGround(G, x, y, z)) {
+                    zv = r[i] * pre[i];
+                    s += (double)(zv * r[i]);
+                }
+                zz[i] = zv;
             }
-            zz[i] = zv;
         }
+        s = blockSum(s, sh);
+        if (tid256() == 0) pout[bid] = s;
     }
-    s = blockSum(s, sh);
-    if (tid256() == 0) pout[bid] = s;
-}
+};
 hipError_t launch_pcg_start(EnvState& e) {
     const int ref = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     const ZGrid zg = zgrid(e);
     const size_t M = e.maxBlocks;
-    hipLaunchKernelGGL(k_pcg_start, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.pcg[0], e.pcg[1], e.pcg[5],
-                       e.partials + PART_DIV * M, divBlocks(e), e.partials + PART_START * M, &e.sc->st[0],
-                       &e.sc->st[1], ref, e.cfg.pcg_tol);
-    return hipGetLastError();
+    return klaunch<KPcgStart>(e, zg.grid, B3, 0, geom(e), zg.zc, e.pcg[0], e.pcg[1], e.pcg[5], e.partials + PART_DIV * M,
+                              divBlocks(e), e.partials + PART_START * M, &e.sc->st[0], &e.sc->st[1], ref,
+                              e.cfg.pcg_tol);
 }
 
 // Prologue PRO_A0 / PRO_A, then the search-direction update (endIteration,
@@ -1081,179 +1118,181 @@ hipError_t launch_pcg_start(EnvState& e) {
 // Byte offsets into the arrays count from plane zb = max(z0 - 1, 0); the
 // launch keeps (chunk + 3 planes) below 4 GiB.
 template <bool FIRST, bool TRUNC>
-KL void k_pcg_A(Geom G, int zc, const float* __restrict__ zz, const float* __restrict__ s_old,
-                float* __restrict__ s_new, const float* __restrict__ irho, float* __restrict__ As,
-                const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, double* pout,
-                const PcgState* sin, PcgState* sout, int it, const int* __restrict__ faceExact) {
-    ZM_SETUP;
-    __shared__ double sh[4];
-    __shared__ PcgState S;
-    __shared__ float tS[2][TH][TW], tI[2][TH][TW];
-    const bool exact = *faceExact != 0;
-    constexpr int KIND = FIRST ? PRO_A0 : PRO_A;
-    PcgPart part;
-    pcgPartLoad<KIND>(pm, ps, n, part);
-    float beta = 0.0f;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int X = G.X, Y = G.Y, Z = G.Z;
-    const long XY = (long)X * Y;
-    const float INF = __builtin_inff();
-    auto snv = [&](float zv, float sv) { return FIRST ? zv : zv + beta * sv; };
-    auto lo = [&](float c) { return TRUNC ? trunc8(c) : c; };
-    // byte offsets of the lane's cell (clamped into the domain) in a plane
-    // and of its column in a ground-height plane
-    const int xc = min(x, X - 1), yc = min(y, Y - 1);
-    const unsigned ob = 4u * (unsigned)(xc + X * yc), gb = 4u * (unsigned)xc;
-    // One halo cell per lane at most: the row below the tile (wave 0), the
-    // row above it (wave 1), the columns left and right of it (lanes 0-3 and
-    // 4-7 of wave 2, one row each).
-    const int x0 = x - tx, y0 = y - ty;
-    int hx = x, hy = y0 - 1, hrow = 0, hcol = tx + 1;
-    if (ty == 1) hy = y0 + 4, hrow = 5;
-    if (ty == 2) hx = tx < 4 ? x0 - 1 : x0 + 64, hy = y0 + (tx & 3), hrow = 1 + (tx & 3), hcol = tx < 4 ? 0 : 65;
-    const bool hasH = ty < 2 || (ty == 2 && tx < 8);
-    const bool hIn = hx >= 0 && hx < X && hy >= 0 && hy < Y;
-    const int hxc = min(max(hx, 0), X - 1), hyc = min(max(hy, 0), Y - 1);
-    const unsigned hb = 4u * (unsigned)(hxc + X * hyc), hgb = 4u * (unsigned)hxc;
-    const unsigned hbL = hasH ? hb : ob, hgbL = hasH ? hgb : gb;
-    const int zb = max(z0 - 1, 0);
-    const unsigned PB = 4u * (unsigned)XY, RB = 4u * (unsigned)X;
-    const Rsrc rZ = arrayRsrc(zz + XY * zb), rS = arrayRsrc(s_old + XY * zb), rI = arrayRsrc(irho + XY * zb);
-    const Rsrc rG = arrayRsrc(G.gh + (long)X * zb);
-    const Rsrc wS = arrayRsrc(s_new + XY * zb), wA = arrayRsrc(As + XY * zb);
-    auto pOff = [&](int pl) { return PB * (unsigned)(pl - zb); };
-    auto gOff = [&](int pl) { return RB * (unsigned)(pl - zb); };
-    struct Own {
-        float z, s, i;
-        int gf;  // ground height of the next plane
-    };
-    struct Halo {
-        float z, s, i;
-        int g;  // ground height of the halo cell's column
-    };
-    auto loadOwn = [&](int pl, Own& o) __attribute__((always_inline)) {
-        const unsigned so = pOff(pl);
-        o.z = bldf(rZ, ob, so);
-        if (!FIRST) o.s = bldf(rS, ob, so);
-        o.i = bldf(rI, ob, so);
-        o.gf = bldi(rG, gb, gOff(min(pl + 1, Z - 1)));
-    };
-    // Every lane issues the same loads in every plane (a lane without a halo
-    // cell reads its own, a prefetch past the chunk reads a clamped plane):
-    // loads complete in issue order, and with no branches around them each
-    // wait covers exactly the loads its values come from.
-    auto loadHalo = [&](int pl, Halo& h) __attribute__((always_inline)) {
-        const unsigned so = pOff(pl);
-        h.z = bldf(rZ, hbL, so);
-        if (!FIRST) h.s = bldf(rS, hbL, so);
-        h.i = bldf(rI, hbL, so);
-        h.g = bldi(rG, hgbL, gOff(pl));
-    };
-    // the chunk's first planes and the plane below it are read before the
-    // prologue; s_new and the back face of the plane below need beta
-    int ghc = bldi(rG, gb, gOff(z0)), ghb = 0;
-    float bz = 0.0f, bs = 0.0f, bi = 0.0f;
-    if (z0 > 0) {
-        const unsigned so = pOff(z0 - 1);
-        ghb = bldi(rG, gb, gOff(z0 - 1));
-        bz = bldf(rZ, ob, so);
-        if (!FIRST) bs = bldf(rS, ob, so);
-        bi = bldf(rI, ob, so);
-    }
-    Own o0{0.0f, 0.0f, 0.0f, 0}, o1{0.0f, 0.0f, 0.0f, 0}, o2{0.0f, 0.0f, 0.0f, 0}, o3{0.0f, 0.0f, 0.0f, 0};
-    Halo h0{0.0f, 0.0f, 0.0f, 0}, h1{0.0f, 0.0f, 0.0f, 0};
-    // prefetches past the chunk read the next chunk's first planes: measured
-    // faster than clamping them to the chunk's last plane
-    const int zlast = Z - 1;
-    loadOwn(z0, o0);
-    loadOwn(min(z0 + 1, zlast), o1);
-    loadOwn(min(z0 + 2, zlast), o2);
-    loadHalo(z0, h0);
-    pcgPrologue<KIND>(pm, ps, n, part, sin, sout, 0, 0.0f, it, S, sh);
-    if (S.done) return;
-    beta = S.beta;
-    // plane below the chunk: s_new, and its front face (= this plane's back face)
-    float snb = 0.0f, fb = 0.0f;
-    bool skB = false;
-    if (inb && z0 > 0 && y > ghb) {
-        skB = true;
-        snb = snv(bz, bs);
-        if (y > ghc) fb = faceI(bi, o0.i);
-    }
-    bool skC = inb && y > ghc;
-    float snc = skC ? snv(o0.z, o0.s) : 0.0f;  // s_new of the current plane
-    double acc = 0.0;
-    // plane z from oc, its front plane from of; plane z + 3 is read into o3n
-    auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, Halo& hc, Halo& hn) __attribute__((always_inline)) {
-        // the halo (used next plane) goes first, so waiting for it does not
-        // wait for the plane read three ahead
-        loadHalo(min(z + 1, zlast), hn);
-        loadOwn(min(z + 3, zlast), o3n);
-        const bool inZ = z + 1 < Z;
-        const bool skF = inb && inZ && y > oc.gf;
-        tS[buf][ty + 1][tx + 1] = snc;
-        tI[buf][ty + 1][tx + 1] = skC ? oc.i : (inb ? INF : oc.i);
-        if (hasH) {
-            const bool sky = hIn && hy > hc.g;
-            tS[buf][hrow][hcol] = sky ? snv(hc.z, hc.s) : 0.0f;
-            tI[buf][hrow][hcol] = sky ? hc.i : (hIn ? INF : hc.i);
+struct KPcgA {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, const float* __restrict__ zz, const float* __restrict__ s_old,
+                    float* __restrict__ s_new, const float* __restrict__ irho, float* __restrict__ As,
+                    const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, double* pout,
+                    const PcgState* sin, PcgState* sout, int it, const int* __restrict__ faceExact) {
+        ZM_SETUP_V;
+        __shared__ double sh[4];
+        __shared__ PcgState S;
+        __shared__ float tS[2][TH][TW], tI[2][TH][TW];
+        const bool exact = *faceExact != 0;
+        constexpr int KIND = FIRST ? PRO_A0 : PRO_A;
+        PcgPart part;
+        pcgPartLoad<KIND>(pm, ps, n, part);
+        float beta = 0.0f;
+        const int tx = threadIdx.x, ty = threadIdx.y;
+        const int X = G.X, Y = G.Y, Z = G.Z;
+        const long XY = (long)X * Y;
+        const float INF = __builtin_inff();
+        auto snv = [&](float zv, float sv) { return FIRST ? zv : zv + beta * sv; };
+        auto lo = [&](float c) { return TRUNC ? trunc8(c) : c; };
+        // byte offsets of the lane's cell (clamped into the domain) in a plane
+        // and of its column in a ground-height plane
+        const int xc = min(x, X - 1), yc = min(y, Y - 1);
+        const unsigned ob = 4u * (unsigned)(xc + X * yc), gb = 4u * (unsigned)xc;
+        // One halo cell per lane at most: the row below the tile (wave 0), the
+        // row above it (wave 1), the columns left and right of it (lanes 0-3 and
+        // 4-7 of wave 2, one row each).
+        const int x0 = x - tx, y0 = y - ty;
+        int hx = x, hy = y0 - 1, hrow = 0, hcol = tx + 1;
+        if (ty == 1) hy = y0 + 4, hrow = 5;
+        if (ty == 2) hx = tx < 4 ? x0 - 1 : x0 + 64, hy = y0 + (tx & 3), hrow = 1 + (tx & 3), hcol = tx < 4 ? 0 : 65;
+        const bool hasH = ty < 2 || (ty == 2 && tx < 8);
+        const bool hIn = hx >= 0 && hx < X && hy >= 0 && hy < Y;
+        const int hxc = min(max(hx, 0), X - 1), hyc = min(max(hy, 0), Y - 1);
+        const unsigned hb = 4u * (unsigned)(hxc + X * hyc), hgb = 4u * (unsigned)hxc;
+        const unsigned hbL = hasH ? hb : ob, hgbL = hasH ? hgb : gb;
+        const int zb = max(z0 - 1, 0);
+        const unsigned PB = 4u * (unsigned)XY, RB = 4u * (unsigned)X;
+        const Rsrc rZ = arrayRsrc(zz + XY * zb), rS = arrayRsrc(s_old + XY * zb), rI = arrayRsrc(irho + XY * zb);
+        const Rsrc rG = arrayRsrc(G.gh + (long)X * zb);
+        const Rsrc wS = arrayRsrc(s_new + XY * zb), wA = arrayRsrc(As + XY * zb);
+        auto pOff = [&](int pl) { return PB * (unsigned)(pl - zb); };
+        auto gOff = [&](int pl) { return RB * (unsigned)(pl - zb); };
+        struct Own {
+            float z, s, i;
+            int gf;  // ground height of the next plane
+        };
+        struct Halo {
+            float z, s, i;
+            int g;  // ground height of the halo cell's column
+        };
+        auto loadOwn = [&](int pl, Own& o) __attribute__((always_inline)) {
+            const unsigned so = pOff(pl);
+            o.z = bldf(rZ, ob, so);
+            if (!FIRST) o.s = bldf(rS, ob, so);
+            o.i = bldf(rI, ob, so);
+            o.gf = bldi(rG, gb, gOff(min(pl + 1, Z - 1)));
+        };
+        // Every lane issues the same loads in every plane (a lane without a halo
+        // cell reads its own, a prefetch past the chunk reads a clamped plane):
+        // loads complete in issue order, and with no branches around them each
+        // wait covers exactly the loads its values come from.
+        auto loadHalo = [&](int pl, Halo& h) __attribute__((always_inline)) {
+            const unsigned so = pOff(pl);
+            h.z = bldf(rZ, hbL, so);
+            if (!FIRST) h.s = bldf(rS, hbL, so);
+            h.i = bldf(rI, hbL, so);
+            h.g = bldi(rG, hgbL, gOff(pl));
+        };
+        // the chunk's first planes and the plane below it are read before the
+        // prologue; s_new and the back face of the plane below need beta
+        int ghc = bldi(rG, gb, gOff(z0)), ghb = 0;
+        float bz = 0.0f, bs = 0.0f, bi = 0.0f;
+        if (z0 > 0) {
+            const unsigned so = pOff(z0 - 1);
+            ghb = bldi(rG, gb, gOff(z0 - 1));
+            bz = bldf(rZ, ob, so);
+            if (!FIRST) bs = bldf(rS, ob, so);
+            bi = bldf(rI, ob, so);
         }
-        __syncthreads();
-        const float snf = skF ? snv(of.z, of.s) : 0.0f;
-        float fcur = 0.0f;
-        if (skC) {
-            const float ic = oc.i;
-            const float iR = tI[buf][ty + 1][tx + 2], iL = tI[buf][ty + 1][tx];
-            const float iU = tI[buf][ty + 2][tx + 1], iD = tI[buf][ty][tx + 1];
-            const float sR = tS[buf][ty + 1][tx + 2], sL = tS[buf][ty + 1][tx];
-            const float sU = tS[buf][ty + 2][tx + 1], sD = tS[buf][ty][tx + 1];
-            const float iF = inZ ? (skF ? of.i : INF) : ic;
-            auto face = [&](float in) {
-                // computed before the select, so no branch skips it
-                const float v = faceFast(ic + in);
-                return isinf(in) ? 0.0f : v;
-            };
-            float r, l, u, d, f;
-            if (!exact) {
-                r = face(iR), l = face(iL), u = face(iU), d = face(iD), f = face(iF);
-            } else {
-                r = faceI(ic, iR), l = faceI(ic, iL), u = faceI(ic, iU), d = faceI(ic, iD), f = faceI(ic, iF);
+        Own o0{0.0f, 0.0f, 0.0f, 0}, o1{0.0f, 0.0f, 0.0f, 0}, o2{0.0f, 0.0f, 0.0f, 0}, o3{0.0f, 0.0f, 0.0f, 0};
+        Halo h0{0.0f, 0.0f, 0.0f, 0}, h1{0.0f, 0.0f, 0.0f, 0};
+        // prefetches past the chunk read the next chunk's first planes: measured
+        // faster than clamping them to the chunk's last plane
+        const int zlast = Z - 1;
+        loadOwn(z0, o0);
+        loadOwn(min(z0 + 1, zlast), o1);
+        loadOwn(min(z0 + 2, zlast), o2);
+        loadHalo(z0, h0);
+        pcgPrologue<KIND>(pm, ps, n, part, sin, sout, 0, 0.0f, it, S, sh, vb.id);
+        if (S.done) return;
+        beta = S.beta;
+        // plane below the chunk: s_new, and its front face (= this plane's back face)
+        float snb = 0.0f, fb = 0.0f;
+        bool skB = false;
+        if (inb && z0 > 0 && y > ghb) {
+            skB = true;
+            snb = snv(bz, bs);
+            if (y > ghc) fb = faceI(bi, o0.i);
+        }
+        bool skC = inb && y > ghc;
+        float snc = skC ? snv(o0.z, o0.s) : 0.0f;  // s_new of the current plane
+        double acc = 0.0;
+        // plane z from oc, its front plane from of; plane z + 3 is read into o3n
+        auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, Halo& hc, Halo& hn) __attribute__((always_inline)) {
+            // the halo (used next plane) goes first, so waiting for it does not
+            // wait for the plane read three ahead
+            loadHalo(min(z + 1, zlast), hn);
+            loadOwn(min(z + 3, zlast), o3n);
+            const bool inZ = z + 1 < Z;
+            const bool skF = inb && inZ && y > oc.gf;
+            tS[buf][ty + 1][tx + 1] = snc;
+            tI[buf][ty + 1][tx + 1] = skC ? oc.i : (inb ? INF : oc.i);
+            if (hasH) {
+                const bool sky = hIn && hy > hc.g;
+                tS[buf][hrow][hcol] = sky ? snv(hc.z, hc.s) : 0.0f;
+                tI[buf][hrow][hcol] = sky ? hc.i : (hIn ? INF : hc.i);
             }
-            float bb = skB ? fb : 0.0f;
-            if (z == 0) bb = faceI(ic, ic);
-            float w = 0.0f;
-            w -= r;
-            w -= l;
-            w -= u;
-            w -= d;
-            w -= f;
-            w -= bb;
-            const float aL = x == 0 ? 0.0f : lo(l), aD = lo(d), aB = skB ? lo(bb) : 0.0f;
-            const float o = w * snc + ((aL * sL + aD * sD + aB * snb + r * sR + u * sU + f * snf));
-            fcur = f;
-            bstf(wS, ob, pOff(z), snc);
-            bstf(wA, ob, pOff(z), o);
-            acc += (double)(o * snc);
-        } else if (inb) {
-            bstf(wS, ob, pOff(z), 0.0f);
-            bstf(wA, ob, pOff(z), 0.0f);
+            __syncthreads();
+            const float snf = skF ? snv(of.z, of.s) : 0.0f;
+            float fcur = 0.0f;
+            if (skC) {
+                const float ic = oc.i;
+                const float iR = tI[buf][ty + 1][tx + 2], iL = tI[buf][ty + 1][tx];
+                const float iU = tI[buf][ty + 2][tx + 1], iD = tI[buf][ty][tx + 1];
+                const float sR = tS[buf][ty + 1][tx + 2], sL = tS[buf][ty + 1][tx];
+                const float sU = tS[buf][ty + 2][tx + 1], sD = tS[buf][ty][tx + 1];
+                const float iF = inZ ? (skF ? of.i : INF) : ic;
+                auto face = [&](float in) {
+                    // computed before the select, so no branch skips it
+                    const float v = faceFast(ic + in);
+                    return isinf(in) ? 0.0f : v;
+                };
+                float r, l, u, d, f;
+                if (!exact) {
+                    r = face(iR), l = face(iL), u = face(iU), d = face(iD), f = face(iF);
+                } else {
+                    r = faceI(ic, iR), l = faceI(ic, iL), u = faceI(ic, iU), d = faceI(ic, iD), f = faceI(ic, iF);
+                }
+                float bb = skB ? fb : 0.0f;
+                if (z == 0) bb = faceI(ic, ic);
+                float w = 0.0f;
+                w -= r;
+                w -= l;
+                w -= u;
+                w -= d;
+                w -= f;
+                w -= bb;
+                const float aL = x == 0 ? 0.0f : lo(l), aD = lo(d), aB = skB ? lo(bb) : 0.0f;
+                const float o = w * snc + ((aL * sL + aD * sD + aB * snb + r * sR + u * sU + f * snf));
+                fcur = f;
+                bstf(wS, ob, pOff(z), snc);
+                bstf(wA, ob, pOff(z), o);
+                acc += (double)(o * snc);
+            } else if (inb) {
+                bstf(wS, ob, pOff(z), 0.0f);
+                bstf(wA, ob, pOff(z), 0.0f);
+            }
+            snb = snc;
+            snc = snf;
+            fb = fcur;
+            skB = skC;
+            skC = skF;
+            ghc = oc.gf;
+        };
+        for (int zg = z0; zg < z1; zg += 4) {
+            step(zg, 0, o0, o1, o3, h0, h1);
+            if (zg + 1 < z1) step(zg + 1, 1, o1, o2, o0, h1, h0);
+            if (zg + 2 < z1) step(zg + 2, 0, o2, o3, o1, h0, h1);
+            if (zg + 3 < z1) step(zg + 3, 1, o3, o0, o2, h1, h0);
         }
-        snb = snc;
-        snc = snf;
-        fb = fcur;
-        skB = skC;
-        skC = skF;
-        ghc = oc.gf;
-    };
-    for (int zg = z0; zg < z1; zg += 4) {
-        step(zg, 0, o0, o1, o3, h0, h1);
-        if (zg + 1 < z1) step(zg + 1, 1, o1, o2, o0, h1, h0);
-        if (zg + 2 < z1) step(zg + 2, 0, o2, o3, o1, h0, h1);
-        if (zg + 3 < z1) step(zg + 3, 1, o3, o0, o2, h1, h0);
+        acc = blockSum(acc, sh);
+        if (tid256() == 0) pout[bid] = acc;
     }
-    acc = blockSum(acc, sh);
-    if (tid256() == 0) pout[bid] = acc;
-}
+};
 hipError_t launch_pcg_A(EnvState& e, int it) {
     float* s_old = e.pcg[2 + ((it + 1) & 1)];
     float* s_new = e.pcg[2 + (it & 1)];
@@ -1266,11 +1305,13 @@ hipError_t launch_pcg_A(EnvState& e, int it) {
     const double* ps = it == 0 ? e.partials + PART_START * M : e.partials + PART_BSUM * M;
     const unsigned np = it == 0 ? e.gZ.nblocks : e.gB.nblocks;
     const bool trunc = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
-    auto k = it == 0 ? (trunc ? k_pcg_A<true, true> : k_pcg_A<true, false>)
-                     : (trunc ? k_pcg_A<false, true> : k_pcg_A<false, false>);
-    klaunch(e, k, zg.grid, B3, 0, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho, e.pcg[4], pm, ps,
-            np, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it, &e.sc->face_exact);
-    return hipGetLastError();
+    auto go = [&](auto body) {
+        return klaunch<decltype(body)>(e, zg.grid, B3, 0, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho, e.pcg[4],
+                                       pm, ps, np, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it,
+                                       &e.sc->face_exact);
+    };
+    if (it == 0) return trunc ? go(KPcgA<true, true>{}) : go(KPcgA<true, false>{});
+    return trunc ? go(KPcgA<false, true>{}) : go(KPcgA<false, false>{});
 }
 
 // Blocks of a kernel the device holds at once. A diffusion sweep's grid per
@@ -1285,7 +1326,12 @@ static long resident_blocks(K kernel, int device, int cus) {
 ResidentBlocks resident_blocks(int device) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 0;
-    return ResidentBlocks{resident_blocks(k_pcg_A<false, true>, device, cus), resident_blocks(k_diffuse, device, cus),
+    return ResidentBlocks{resident_blocks(&k_solo<KPcgA<false, true>, Geom, int, float*, float*, float*, float*, float*, const double*,
+                                          const double*, unsigned, double*, PcgState*, PcgState*, int, int*>,
+                                          device, cus),
+                          resident_blocks(&k_solo<KDiffuse, Geom, int, unsigned, DiffBatch, float, float*, float*,
+                                                  float*>,
+                                          device, cus),
                           cus};
 }
 
@@ -1293,7 +1339,7 @@ ResidentBlocks resident_blocks(int device) {
 // stop-test partials (Game/environment.cu:963-971) + applyPreconditionerGPU
 // + z.r partials: p' += alpha s, r -= alpha As, z = M^-1 r.
 // Elementwise, so a block walks whole rows (x fastest) of the Y*Z rows:
-// blockIdx.x takes rows [b rpb, (b + 1) rpb), V cells per lane step (V = 4
+// tile b takes rows [b rpb, (b + 1) rpb), V cells per lane step (V = 4
 // when X is a multiple of 4: 16-byte accesses). Lanes advance by 256 groups
 // and carry their (column, y, z) along. Ground cells keep p and r (the same
 // bits are stored back) and get z = 0. The partials of pcg_A and then the
@@ -1301,134 +1347,139 @@ ResidentBlocks resident_blocks(int device) {
 // after that each group's data is read two groups ahead (two register slots,
 // the loop handles one group per slot per trip).
 template <int V>
-KL void k_pcg_B(Geom G, unsigned rpb, float* __restrict__ pp, float* __restrict__ r, const float* __restrict__ s,
-                const float* __restrict__ As, const float* __restrict__ pre, float* __restrict__ zz,
-                const double* __restrict__ pin, unsigned n, double* pmax, double* psum, const PcgState* sin,
-                PcgState* sout, int ref, int absmode) {
-    typedef float __attribute__((ext_vector_type(V))) fv;
-    typedef int __attribute__((ext_vector_type(V))) iv;
-    __shared__ double sh[4];
-    __shared__ PcgState S;
-    PcgPart part;
-    pcgPartLoad<PRO_B>(nullptr, pin, n, part);
-    const int X = G.X, Y = G.Y, q = G.X / V;
-    const long rows = (long)Y * G.Z;
-    const long r0 = (long)blockIdx.x * rpb, r1 = std::min(rows, r0 + (long)rpb);
-    const int t = tid256();
-    const long gend = r1 * q;
-    struct Grp {
-        fv P, R, S, A, Q;
-    };
-    auto load = [&](long gk, Grp& d) __attribute__((always_inline)) {
-        if (gk < gend) {
-            d.P = ((const fv*)pp)[gk];
-            d.R = ((const fv*)r)[gk];
-            d.S = ((const fv*)s)[gk];
-            d.A = ((const fv*)As)[gk];
-            d.Q = ((const fv*)pre)[gk];
-        }
-    };
-    long g = r0 * q + t;
-    Grp d0, d1;
-    load(g, d0);
-    load(g + 256, d1);
-    pcgPrologue<PRO_B>(nullptr, pin, n, part, sin, sout, ref, 0.0f, 0, S, sh);
-    if (S.done) return;
-    const float alpha = S.alpha;
-    float mf = 0.0f;
-    double dz = 0.0;
-    if (r0 < r1) {
-        const long row = g / q;
-        int gx = (int)(g - row * q), y = (int)(row % Y), z = (int)(row / Y);
-        // a step of 256 groups: dr whole rows and dgx groups
-        const int dr = 256 / q, dgx = 256 % q;
-        auto group = [&](long gk, const Grp& d) __attribute__((always_inline)) {
-            const iv gh = *(const iv*)(G.gh + gx * V + (long)X * z);
-            fv Po = d.P, Ro = d.R, Zo;
-            const bool ab = absmode || z == 0;
-#pragma unroll
-            for (int c = 0; c < V; c++) {
-                float zv = 0.0f;
-                if (y > gh[c]) {
-                    Po[c] = d.P[c] + alpha * d.S[c];
-                    const float rv = d.R[c] - alpha * d.A[c];
-                    Ro[c] = rv;
-                    mf = fmaxf(mf, ab ? fabsf(rv) : rv);
-                    zv = rv * d.Q[c];
-                    dz += (double)(zv * rv);
-                }
-                Zo[c] = zv;
-            }
-            ((fv*)pp)[gk] = Po;
-            ((fv*)r)[gk] = Ro;
-            ((fv*)zz)[gk] = Zo;
-            gx += dgx;
-            y += dr;
-            if (gx >= q) gx -= q, ++y;
-            while (y >= Y) y -= Y, ++z;
+struct KPcgB {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, unsigned rpb, float* __restrict__ pp, float* __restrict__ r, const float* __restrict__ s,
+                    const float* __restrict__ As, const float* __restrict__ pre, float* __restrict__ zz,
+                    const double* __restrict__ pin, unsigned n, double* pmax, double* psum, const PcgState* sin,
+                    PcgState* sout, int ref, int absmode) {
+        typedef float __attribute__((ext_vector_type(V))) fv;
+        typedef int __attribute__((ext_vector_type(V))) iv;
+        __shared__ double sh[4];
+        __shared__ PcgState S;
+        PcgPart part;
+        pcgPartLoad<PRO_B>(nullptr, pin, n, part);
+        const int X = G.X, Y = G.Y, q = G.X / V;
+        const long rows = (long)Y * G.Z;
+        const long r0 = (long)vb.id * rpb, r1 = std::min(rows, r0 + (long)rpb);
+        const int t = tid256();
+        const long gend = r1 * q;
+        struct Grp {
+            fv P, R, S, A, Q;
         };
-        for (; g < gend; g += 512) {
-            group(g, d0);
-            load(g + 512, d0);
-            if (g + 256 < gend) {
-                group(g + 256, d1);
-                load(g + 768, d1);
+        auto load = [&](long gk, Grp& d) __attribute__((always_inline)) {
+            if (gk < gend) {
+                d.P = ((const fv*)pp)[gk];
+                d.R = ((const fv*)r)[gk];
+                d.S = ((const fv*)s)[gk];
+                d.A = ((const fv*)As)[gk];
+                d.Q = ((const fv*)pre)[gk];
+            }
+        };
+        long g = r0 * q + t;
+        Grp d0, d1;
+        load(g, d0);
+        load(g + 256, d1);
+        pcgPrologue<PRO_B>(nullptr, pin, n, part, sin, sout, ref, 0.0f, 0, S, sh, vb.id);
+        if (S.done) return;
+        const float alpha = S.alpha;
+        float mf = 0.0f;
+        double dz = 0.0;
+        if (r0 < r1) {
+            const long row = g / q;
+            int gx = (int)(g - row * q), y = (int)(row % Y), z = (int)(row / Y);
+            // a step of 256 groups: dr whole rows and dgx groups
+            const int dr = 256 / q, dgx = 256 % q;
+            auto group = [&](long gk, const Grp& d) __attribute__((always_inline)) {
+                const iv gh = *(const iv*)(G.gh + gx * V + (long)X * z);
+                fv Po = d.P, Ro = d.R, Zo;
+                const bool ab = absmode || z == 0;
+    #pragma unroll
+                for (int c = 0; c < V; c++) {
+                    float zv = 0.0f;
+                    if (y > gh[c]) {
+                        Po[c] = d.P[c] + alpha * d.S[c];
+                        const float rv = d.R[c] - alpha * d.A[c];
+                        Ro[c] = rv;
+                        mf = fmaxf(mf, ab ? fabsf(rv) : rv);
+                        zv = rv * d.Q[c];
+                        dz += (double)(zv * rv);
+                    }
+                    Zo[c] = zv;
+                }
+                ((fv*)pp)[gk] = Po;
+                ((fv*)r)[gk] = Ro;
+                ((fv*)zz)[gk] = Zo;
+                gx += dgx;
+                y += dr;
+                if (gx >= q) gx -= q, ++y;
+                while (y >= Y) y -= Y, ++z;
+            };
+            for (; g < gend; g += 512) {
+                group(g, d0);
+                load(g + 512, d0);
+                if (g + 256 < gend) {
+                    group(g + 256, d1);
+                    load(g + 768, d1);
+                }
             }
         }
+        const double m = blockMax((double)mf, sh);
+        dz = blockSum(dz, sh);
+        if (t == 0) {
+            pmax[vb.id] = m;
+            psum[vb.id] = dz;
+        }
     }
-    const double m = blockMax((double)mf, sh);
-    dz = blockSum(dz, sh);
-    if (t == 0) {
-        pmax[blockIdx.x] = m;
-        psum[blockIdx.x] = dz;
-    }
-}
+};
 hipError_t launch_pcg_B(EnvState& e, int it) {
     float* s_new = e.pcg[2 + (it & 1)];
     const int ref = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     const int absmode = ref ? e.cfg.maxabs_mode : 1;
     const size_t M = e.maxBlocks;
-    auto k = e.X % 4 == 0 ? k_pcg_B<4> : k_pcg_B<1>;
-    klaunch(e, k, dim3(e.gB.nblocks), B3, 0, geom(e), e.gB.rpb, e.f[SKT_PRESSURE_PROJ], e.pcg[0],
-            s_new, e.pcg[4], e.pcg[5], e.pcg[1], e.partials + PART_A * M, e.gA.nblocks,
-            e.partials + PART_BMAX * M, e.partials + PART_BSUM * M, &e.sc->st[0], &e.sc->st[1], ref,
-            absmode);
-    return hipGetLastError();
+    auto go = [&](auto body) {
+        return klaunch<decltype(body)>(e, dim3(e.gB.nblocks), B3, 0, geom(e), e.gB.rpb, e.f[SKT_PRESSURE_PROJ],
+                                       e.pcg[0], s_new, e.pcg[4], e.pcg[5], e.pcg[1], e.partials + PART_A * M,
+                                       e.gA.nblocks, e.partials + PART_BMAX * M, e.partials + PART_BSUM * M,
+                                       &e.sc->st[0], &e.sc->st[1], ref, absmode);
+    };
+    return e.X % 4 == 0 ? go(KPcgB<4>{}) : go(KPcgB<1>{});
 }
 
 // Stop test after the last iteration (one block): slot 1 -> slot 0.
-KL void k_pcg_final(const double* __restrict__ pm, unsigned n, const PcgState* sin, PcgState* sout, int it) {
-    __shared__ double sh[4];
-    __shared__ PcgState S;
-    PcgPart P;
-    pcgPartLoad<PRO_FINAL>(pm, nullptr, n, P);
-    pcgPrologue<PRO_FINAL>(pm, nullptr, n, P, sin, sout, 0, 0.0f, it, S, sh);
-}
+struct KPcgFinal {
+    static __device__ __forceinline__ void run(VB vb, const double* __restrict__ pm, unsigned n, const PcgState* sin, PcgState* sout, int it) {
+        __shared__ double sh[4];
+        __shared__ PcgState S;
+        PcgPart P;
+        pcgPartLoad<PRO_FINAL>(pm, nullptr, n, P);
+        pcgPrologue<PRO_FINAL>(pm, nullptr, n, P, sin, sout, 0, 0.0f, it, S, sh, vb.id);
+    }
+};
 hipError_t launch_pcg_final(EnvState& e, int it) {
-    hipLaunchKernelGGL(k_pcg_final, dim3(1), B3, 0, e.stream, e.partials + (size_t)PART_BMAX * e.maxBlocks,
-                       e.gB.nblocks, &e.sc->st[1], &e.sc->st[0], it);
-    return hipGetLastError();
+    return klaunch<KPcgFinal>(e, dim3(1), B3, 0, e.partials + (size_t)PART_BMAX * e.maxBlocks, e.gB.nblocks,
+                              &e.sc->st[1], &e.sc->st[0], it);
 }
 
 // updatePressure (Game/kernelSky.cu:1016-1030) fused with the initDensity that
 // follows it (Game/environment.cu:878-879).
-KL void k_update_pressure_density(Geom G, float* __restrict__ p, const float* __restrict__ pp, float* __restrict__ rho,
-                                  const float* __restrict__ th, const float* __restrict__ qv,
-                                  const float* __restrict__ gP) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
-    const long i = gidx(G, x, y, z);
-    const float pn = p[i] - pp[i] / 100.0f;
-    p[i] = pn;
-    const float T = th[i] * powf(pn / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
-    const float Tv = T * (0.608f * qv[i] + 1);
-    rho[i] = pn * 100 / (kc::Rsd * Tv);
-}
+struct KUpdatePressureDensity {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, float* __restrict__ p, const float* __restrict__ pp, float* __restrict__ rho,
+                                      const float* __restrict__ th, const float* __restrict__ qv,
+                                      const float* __restrict__ gP) {
+        CELL_SETUP_V;
+        (void)bid;
+        if (!inb || isGround(G, x, y, z)) return;
+        const long i = gidx(G, x, y, z);
+        const float pn = p[i] - pp[i] / 100.0f;
+        p[i] = pn;
+        const float T = th[i] * powf(pn / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+        const float Tv = T * (0.608f * qv[i] + 1);
+        rho[i] = pn * 100 / (kc::Rsd * Tv);
+    }
+};
 hipError_t launch_update_pressure_density(EnvState& e) {
-    hipLaunchKernelGGL(k_update_pressure_density, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_PRESSURE],
-                       e.f[SKT_PRESSURE_PROJ], e.f[SKT_DENSITY], e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P]);
-    return hipGetLastError();
+    return klaunch<KUpdatePressureDensity>(e, grid3(e), B3, 0, geom(e), e.f[SKT_PRESSURE], e.f[SKT_PRESSURE_PROJ],
+                                           e.f[SKT_DENSITY], e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P]);
 }
 
 // applyPresProjGPU, Game/kernelSky.cu:862-908 (pressure {D,N,N}, density
@@ -1439,60 +1490,61 @@ hipError_t launch_update_pressure_density(EnvState& e) {
 // next step, where they are the cell's own. Each neighbour value follows
 // nbOwn (beyond the domain: the sides condition on the cell's own value;
 // ground: the ground condition), with the same arithmetic per cell.
-KL void k_apply_projection(Geom G, int yc, unsigned nyc, const float* __restrict__ pp, const float* __restrict__ rho,
-                           float* __restrict__ u, float* __restrict__ v, float* __restrict__ w, float* __restrict__ p,
-                           const float* __restrict__ th, const float* __restrict__ qv, const float* __restrict__ gP,
-                           int newp) {
-    const unsigned lin = blockIdx.x;
-    const int chunk = (int)(lin % nyc);
-    const unsigned tile = lin / nyc;
-    const int gx = (G.X + 63) >> 6;
-    const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
-    const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
-    if (x >= G.X || z >= G.Z) return;
-    const int X = G.X, Y = G.Y, Z = G.Z;
-    const long XY = (long)X * Y;
-    const long col = XY * z + x;  // cell (x, 0, z)
-    const long gc = x + (long)X * z;
-    const int gh = G.gh[gc];
-    const bool inR = x < X - 1, inF = z < Z - 1;
-    const int ghR = inR ? G.gh[gc + 1] : 0, ghF = inF ? G.gh[gc + X] : 0;
-    const Bounds bp{BC_DIRICHLET, BC_NEUMANN, BC_NEUMANN}, bd{BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET};
-    const float gps = gP[gc];
-    const int y0 = max(chunk * yc, gh + 1), y1 = min(Y, chunk * yc + yc);
-    if (y0 >= y1) return;
-    float cp = pp[col + (long)X * y0], cd = rho[col + (long)X * y0];
-    for (int y = y0; y < y1; y++) {
-        const long i = col + (long)X * y;
-        const bool up = y + 1 <= Y - 1;  // the level above a sky cell is sky
-        const float ppU = up ? pp[i + X] : 0.0f, rhoU = up ? rho[i + X] : 0.0f;
-        const float pr = !inR ? bcv(bp.sides, cp, 0.0f) : (y <= ghR ? bcv(bp.ground, cp, 0.0f) : pp[i + 1]);
-        const float pu = !up ? bcv(bp.sides, cp, 0.0f) : ppU;
-        const float pf = !inF ? bcv(bp.sides, cp, 0.0f) : (y <= ghF ? bcv(bp.ground, cp, 0.0f) : pp[i + XY]);
-        float dr = !inR ? bcv(bd.sides, cd, 0.0f) : (y <= ghR ? bcv(bd.ground, cd, 0.0f) : rho[i + 1]);
-        float du = !up ? bcv(bd.sides, cd, 0.0f) : rhoU;
-        float df = !inF ? bcv(bd.sides, cd, 0.0f) : (y <= ghF ? bcv(bd.ground, cd, 0.0f) : rho[i + XY]);
-        dr = dr == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / dr);
-        du = du == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / du);
-        df = df == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / df);
-        u[i] += (pr - cp) / dr;
-        v[i] += (pu - cp) / du;
-        w[i] += (pf - cp) / df;
-        if (newp) {
-            const float T = th[i] * powf(p[i] / gps, kc::Rsd / kc::Cpd);
-            const float Tv = T * (0.608f * qv[i] + 1);
-            p[i] = cd * kc::Rsd * Tv / 100.0f;
+struct KApplyProjection {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, const float* __restrict__ pp, const float* __restrict__ rho,
+                               float* __restrict__ u, float* __restrict__ v, float* __restrict__ w, float* __restrict__ p,
+                               const float* __restrict__ th, const float* __restrict__ qv, const float* __restrict__ gP,
+                               int newp) {
+        const unsigned lin = vb.id;
+        const int chunk = (int)(lin % nyc);
+        const unsigned tile = lin / nyc;
+        const int gx = (G.X + 63) >> 6;
+        const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
+        const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
+        if (x >= G.X || z >= G.Z) return;
+        const int X = G.X, Y = G.Y, Z = G.Z;
+        const long XY = (long)X * Y;
+        const long col = XY * z + x;  // cell (x, 0, z)
+        const long gc = x + (long)X * z;
+        const int gh = G.gh[gc];
+        const bool inR = x < X - 1, inF = z < Z - 1;
+        const int ghR = inR ? G.gh[gc + 1] : 0, ghF = inF ? G.gh[gc + X] : 0;
+        const Bounds bp{BC_DIRICHLET, BC_NEUMANN, BC_NEUMANN}, bd{BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET};
+        const float gps = gP[gc];
+        const int y0 = max(chunk * yc, gh + 1), y1 = min(Y, chunk * yc + yc);
+        if (y0 >= y1) return;
+        float cp = pp[col + (long)X * y0], cd = rho[col + (long)X * y0];
+        for (int y = y0; y < y1; y++) {
+            const long i = col + (long)X * y;
+            const bool up = y + 1 <= Y - 1;  // the level above a sky cell is sky
+            const float ppU = up ? pp[i + X] : 0.0f, rhoU = up ? rho[i + X] : 0.0f;
+            const float pr = !inR ? bcv(bp.sides, cp, 0.0f) : (y <= ghR ? bcv(bp.ground, cp, 0.0f) : pp[i + 1]);
+            const float pu = !up ? bcv(bp.sides, cp, 0.0f) : ppU;
+            const float pf = !inF ? bcv(bp.sides, cp, 0.0f) : (y <= ghF ? bcv(bp.ground, cp, 0.0f) : pp[i + XY]);
+            float dr = !inR ? bcv(bd.sides, cd, 0.0f) : (y <= ghR ? bcv(bd.ground, cd, 0.0f) : rho[i + 1]);
+            float du = !up ? bcv(bd.sides, cd, 0.0f) : rhoU;
+            float df = !inF ? bcv(bd.sides, cd, 0.0f) : (y <= ghF ? bcv(bd.ground, cd, 0.0f) : rho[i + XY]);
+            dr = dr == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / dr);
+            du = du == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / du);
+            df = df == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / df);
+            u[i] += (pr - cp) / dr;
+            v[i] += (pu - cp) / du;
+            w[i] += (pf - cp) / df;
+            if (newp) {
+                const float T = th[i] * powf(p[i] / gps, kc::Rsd / kc::Cpd);
+                const float Tv = T * (0.608f * qv[i] + 1);
+                p[i] = cd * kc::Rsd * Tv / 100.0f;
+            }
+            cp = ppU;
+            cd = rhoU;
         }
-        cp = ppU;
-        cd = rhoU;
     }
-}
+};
 hipError_t launch_apply_projection(EnvState& e, int new_pressure) {
     const unsigned nyc = (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
-    hipLaunchKernelGGL(k_apply_projection, dim3(divBlocks(e)), B3, 0, e.stream, geom(e), YC_DIV, nyc,
-                       e.f[SKT_PRESSURE_PROJ], e.f[SKT_DENSITY], e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ],
-                       e.f[SKT_PRESSURE], e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P], new_pressure);
-    return hipGetLastError();
+    return klaunch<KApplyProjection>(e, dim3(divBlocks(e)), B3, 0, geom(e), YC_DIV, nyc, e.f[SKT_PRESSURE_PROJ],
+                                     e.f[SKT_DENSITY], e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ], e.f[SKT_PRESSURE],
+                                     e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P], new_pressure);
 }
 
 }  // namespace skt

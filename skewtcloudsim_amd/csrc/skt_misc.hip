@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <mutex>
 #include "skt_internal.h"
+#include "skt_launch.h"
 #include "skt_micro.h"
 #include "skt_reduce.h"
 
@@ -16,186 +17,202 @@ namespace skt {
 // advectGroundWaterGPU, Game/kernelSky.cu:120-253, reading a pre-step snapshot
 // of both stores (the reference's in-place tile update leaves its interior
 // halo unfilled; every column reading its neighbours' old values is intended).
-KL void k_ground_water(Geom G, const float* __restrict__ rs0, const float* __restrict__ gr0, float* __restrict__ rs,
-                       float* __restrict__ gr, float dt, float speed, float dx) {
-    const long g = blockIdx.x * 256L + threadIdx.x;
-    if (g >= (long)G.X * G.Z) return;
-    const int x = (int)(g % G.X), zc = (int)(g / G.X);
-    const long iL = x > 0 ? g - 1 : g, iR = x < G.X - 1 ? g + 1 : g;
-    const long iB = zc > 0 ? g - G.X : g, iF = zc < G.Z - 1 ? g + G.X : g;
-    const float n = 1 / 0.03f;
-    const float kk = 5e-5f;
-    const float diffR = 1 / (dx * dx) * 1.75e-3f;
-    const float diffW = 1 / (dx * dx) * 1.8e-5f;
-    const float grC = gr0[g], rsC = rs0[g];
-    float fGr = grC, fRs = rsC;
-    const float hC = float(G.gh[g]);
-    const long nb[4] = {iL, iR, iB, iF};
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const long j = nb[k];
-        const float hFrom = float(G.gh[j]), grFrom = gr0[j], rsFrom = rs0[j];
-        const float slope = hFrom - hC / 1.0f;
-        const bool fromPrev = slope > 0.0f;
-        const float sRain = fromPrev ? grFrom : grC;
-        const float sWater = fromPrev ? rsFrom : rsC;
-        float changeRain = 0.0f;
-        if (sRain != 0.0f) {
-            const float sp = n * powf(sRain, 2.0f / 3.0f);
-            const float vel = sp * powf(slope, 0.5f);
-            changeRain = vel * sRain;
+struct KGroundWater {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const float* __restrict__ rs0, const float* __restrict__ gr0, float* __restrict__ rs,
+                           float* __restrict__ gr, float dt, float speed, float dx) {
+        const long g = vb.id * 256L + threadIdx.x;
+        if (g >= (long)G.X * G.Z) return;
+        const int x = (int)(g % G.X), zc = (int)(g / G.X);
+        const long iL = x > 0 ? g - 1 : g, iR = x < G.X - 1 ? g + 1 : g;
+        const long iB = zc > 0 ? g - G.X : g, iF = zc < G.Z - 1 ? g + G.X : g;
+        const float n = 1 / 0.03f;
+        const float kk = 5e-5f;
+        const float diffR = 1 / (dx * dx) * 1.75e-3f;
+        const float diffW = 1 / (dx * dx) * 1.8e-5f;
+        const float grC = gr0[g], rsC = rs0[g];
+        float fGr = grC, fRs = rsC;
+        const float hC = float(G.gh[g]);
+        const long nb[4] = {iL, iR, iB, iF};
+    #pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const long j = nb[k];
+            const float hFrom = float(G.gh[j]), grFrom = gr0[j], rsFrom = rs0[j];
+            const float slope = hFrom - hC / 1.0f;
+            const bool fromPrev = slope > 0.0f;
+            const float sRain = fromPrev ? grFrom : grC;
+            const float sWater = fromPrev ? rsFrom : rsC;
+            float changeRain = 0.0f;
+            if (sRain != 0.0f) {
+                const float sp = n * powf(sRain, 2.0f / 3.0f);
+                const float vel = sp * powf(slope, 0.5f);
+                changeRain = vel * sRain;
+            }
+            const float changeWater = kc::g * kk * sWater * slope;
+            const float lapR = (grFrom - grC) * diffR;
+            const float lapSR = (rsFrom - rsC) * diffW;
+            const bool fpR = lapR > 0.0f, fpSR = lapSR > 0.0f;
+            const float fgr = dt * fminf(fabsf(lapR) * speed, fpR ? grFrom : grC) * (lapR > 0.0f ? 1.0f : -1.0f);
+            const float fgw = dt * fminf(fabsf(lapSR) * speed, fpSR ? rsFrom : rsC) * (lapSR > 0.0f ? 1.0f : -1.0f);
+            const float sfr = dt * fminf(fabsf(changeRain) * speed, fromPrev ? grFrom : grC) * (changeRain > 0.0f ? 1.0f : -1.0f);
+            const float sfw = dt * fminf(fabsf(changeWater) * speed, fromPrev ? rsFrom : rsC) * (changeWater > 0.0f ? 1.0f : -1.0f);
+            fGr += fgr + sfr;
+            fRs += fgw + sfw;
         }
-        const float changeWater = kc::g * kk * sWater * slope;
-        const float lapR = (grFrom - grC) * diffR;
-        const float lapSR = (rsFrom - rsC) * diffW;
-        const bool fpR = lapR > 0.0f, fpSR = lapSR > 0.0f;
-        const float fgr = dt * fminf(fabsf(lapR) * speed, fpR ? grFrom : grC) * (lapR > 0.0f ? 1.0f : -1.0f);
-        const float fgw = dt * fminf(fabsf(lapSR) * speed, fpSR ? rsFrom : rsC) * (lapSR > 0.0f ? 1.0f : -1.0f);
-        const float sfr = dt * fminf(fabsf(changeRain) * speed, fromPrev ? grFrom : grC) * (changeRain > 0.0f ? 1.0f : -1.0f);
-        const float sfw = dt * fminf(fabsf(changeWater) * speed, fromPrev ? rsFrom : rsC) * (changeWater > 0.0f ? 1.0f : -1.0f);
-        fGr += fgr + sfr;
-        fRs += fgw + sfw;
+        gr[g] = fGr;
+        rs[g] = fRs;
     }
-    gr[g] = fGr;
-    rs[g] = fRs;
-}
+};
+// Copy of two arrays of n floats (the pre-step snapshot of the ground stores).
+struct KCopy2 {
+    static __device__ __forceinline__ void run(VB vb, const float* __restrict__ a, const float* __restrict__ b,
+                                               float* __restrict__ da, float* __restrict__ db, long n) {
+        for (long i = vb.id * 256L + threadIdx.x; i < n; i += (long)vb.n * 256) {
+            da[i] = a[i];
+            db[i] = b[i];
+        }
+    }
+};
 hipError_t launch_ground_water_snapshot(EnvState& e, float dt, float speed) {
     float* rs0 = e.work[0];
     float* gr0 = e.work[0] + e.G;
-    hipError_t err = hipMemcpyAsync(rs0, e.f[SKT_QRS], e.G * sizeof(float), hipMemcpyDeviceToDevice, e.stream);
+    hipError_t err = klaunch<KCopy2>(e, grid1(e.G), dim3(256), 0, e.f[SKT_QRS], e.f[SKT_QGR], rs0, gr0, e.G);
     if (err != hipSuccess) return err;
-    err = hipMemcpyAsync(gr0, e.f[SKT_QGR], e.G * sizeof(float), hipMemcpyDeviceToDevice, e.stream);
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_ground_water, grid1(e.G), dim3(256), 0, e.stream, geom(e), rs0, gr0, e.f[SKT_QRS],
-                       e.f[SKT_QGR], dt, speed, e.dx);
-    return hipGetLastError();
+    return klaunch<KGroundWater>(e, grid1(e.G), dim3(256), 0, geom(e), rs0, gr0, e.f[SKT_QRS], e.f[SKT_QGR], dt,
+                                 speed, e.dx);
 }
 
 struct GroundArgs {
     float *Qv, *Qr, *Qs, *Qi, *Qc, *Qw, *th, *u, *p, *rho;
     float *Qrs, *Qgr, *Qgs, *Qgi, *gP, *gt, *gT, *cover;
+    const float* irrDev;  // irradiance on the device (null: the irr argument)
 };
 
 // One thread per ground column: the column-local part of the ground phase of
 // updateGPU (Game/environment.cu:427-455,469) in the reference's order.
-KL void k_ground_columns(Geom G, GroundArgs a, int stages, float dt, float speed, float irr, float dx, Gammas gm) {
-    const long g = blockIdx.x * 256L + threadIdx.x;
-    if (g >= (long)G.X * G.Z) return;
-    const int x = (int)(g % G.X), zc = (int)(g / G.X);
-    const float kap = kc::Rsd / kc::Cpd;
-    const int gh = G.gh[g];
-    if (stages & 1) {  // calculateCloudCoverGPU, Game/kernelSky.cu:1034-1044
-        float tot = 0.0f;
-        for (int y = gh + 1; y < G.Y; y++) {
-            const long i = gidx(G, x, y, zc);
-            tot += (a.Qc[i] + a.Qw[i]) * dx;
-        }
-        a.cover[g] = fminf(tot / 1.2f, 1.0f);
-    }
-    if (stages & 2) {  // calculateGroundTempGPU, Game/kernelSky.cu:1046-1059 (T^4 widened to double)
-        const float gtv = a.gT[g];
-        const double T4 = gtv * gtv * gtv * gtv;
-        a.gT[g] += (dt * speed) * ((1 - a.cover[g]) * (((1 - 0.25f) * irr - kc::ge * kc::oo * T4) / (1.0f * 1500.0f * kc::Cpds)));
-    }
-    if ((stages & (4 | 8 | 16)) && gh + 1 < G.Y) {
-        const long i = gidx(G, x, gh + 1, zc);
-        if (stages & 4) {  // calculatePrecipHittingGroundMicroPhysicsGPU, Game/microPhysics.cu:1544-1636
-            const float Qgr = a.Qgr[g] > 1e-18f ? a.Qgr[g] : 0.0f;
-            const float Qgs = a.Qgs[g] > 1e-18f ? a.Qgs[g] : 0.0f;
-            const float Qgi = a.Qgi[g] > 1e-18f ? a.Qgi[g] : 0.0f;
-            const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
-            const float Qr = a.Qr[i] > 1e-18f ? a.Qr[i] : 0.0f;
-            const float Qs = a.Qs[i] > 1e-18f ? a.Qs[i] : 0.0f;
-            const float Qi = a.Qi[i] > 1e-18f ? a.Qi[i] : 0.0f;
-            float freeze = 0.0f;
-            const float tGC = a.gT[g] - 273.15f;
-            const float ps = a.p[i];
-            const float wind = a.u[i];
-            float3 fv = make_float3(0.0f, 0.0f, 0.0f);
-            const float Dair = a.rho[i];
-            const float tAK = a.th[i] * powf(ps / a.gP[g], kap);
-            if (Qgr > 0.0f || Qgs > 0.0f || Qgi > 0.0f) fv = d_fallVel(Qgr, Qgs, Qgi, Dair, 3, gm.R48, gm.S425, gm.S425);
-            const float GR = dt * fminf(Qr * speed, Qr);
-            const float GS = dt * fminf(Qs * speed, Qs);
-            const float GI = dt * fminf(Qi * speed, Qi);
-            float PGRFR = mp::PGRFR(tAK, tGC, ps, Qr, Qv, Qi, Qgr, Qgs, Dair, wind, fv);
-            if (tAK < 273.15f) {
-                PGRFR = dt * fminf(Qr, speed * PGRFR);
-                freeze += PGRFR;
+struct KGroundColumns {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const GroundArgs& a, int stages, float dt,
+                                               float speed, float irrArg, float dx, const Gammas& gm) {
+        const float irr = a.irrDev ? *a.irrDev : irrArg;
+        const long g = vb.id * 256L + threadIdx.x;
+        if (g >= (long)G.X * G.Z) return;
+        const int x = (int)(g % G.X), zc = (int)(g / G.X);
+        const float kap = kc::Rsd / kc::Cpd;
+        const int gh = G.gh[g];
+        if (stages & 1) {  // calculateCloudCoverGPU, Game/kernelSky.cu:1034-1044
+            float tot = 0.0f;
+            for (int y = gh + 1; y < G.Y; y++) {
+                const long i = gidx(G, x, y, zc);
+                tot += (a.Qc[i] + a.Qw[i]) * dx;
             }
-            a.Qr[i] += -GR;
-            a.Qs[i] += -GS;
-            a.Qi[i] += -GI;
-            a.Qgr[g] += GR - PGRFR;
-            a.Qgs[g] += GS;
-            a.Qgi[g] += GI + PGRFR;
-            const float Mair = 0.02896f, Mwater = 0.01802f;
-            const float XV = (Qv / Mwater) / ((Qv / Mwater) + (1 - Qv) / Mair);
-            const float Mth = XV * Mwater + (1 - XV) * Mair;
-            const float YV = XV * (Mwater / Mth);
-            const float yth = YV * 1.33f + (1 - YV) * 1.4f;
-            const float cpth = kc::R / (Mth * (yth - 1));
-            float sum = 0.0f;
-            sum += kc::Lf / cpth * freeze;
-            a.gT[g] += sum;
+            a.cover[g] = fminf(tot / 1.2f, 1.0f);
         }
-        if (stages & 8) {  // calculateGroundMicroPhysicsGPU, Game/microPhysics.cu:1419-1541
-            const float Qrs = a.Qrs[g] > 1e-18f ? a.Qrs[g] : 0.0f;
-            const float Qgr = a.Qgr[g] > 1e-18f ? a.Qgr[g] : 0.0f;
-            const float Qgs = a.Qgs[g] > 1e-18f ? a.Qgs[g] : 0.0f;
-            const float Qgi = a.Qgi[g] > 1e-18f ? a.Qgi[g] : 0.0f;
-            const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
-            float condens = 0.0f, freeze = 0.0f, depos = 0.0f;
-            const float tGC = a.gT[g] - 273.15f;
-            const float ps = a.p[i];
-            const float wind = a.u[i];
-            const float cc = a.cover[g];
-            const float Dair = a.rho[i];
-            float tme = a.gt[g];
-            const float tAK = a.th[i] * powf(ps / a.gP[g], kap);
-            const float sI = d_slope(Dair, Qgi, 2);
-            float PGREVP = mp::PGREVP(tAK, tGC, ps, Qv, Qgr, irr, Dair, wind);
-            float PGSMLT = mp::PGSMLT(tAK, tGC, ps, Qgs, Qv, Qgr, irr, cc, Dair, wind, dt);
-            float PGGMLT = mp::PGGMLT(tGC, Qgi, Dair, sI);
-            float PGGFR = mp::PGGFR(tAK, tGC, ps, Qgr, Qv, Qgi, Dair, irr, cc, wind, dt);
-            float PGSFR = mp::PGSFR(tAK, tGC, ps, Qv, Qgr, Qgs, Dair, wind, dt);
-            float PGFLW = mp::PGFLW(Qgr);
-            float PGDEVP = mp::PGDEVP(&tme, Qgr, Qgs, Qgi, Qrs);
-            a.gt[g] = tme;
-            PGREVP = dt * fminf(Qgr, speed * PGREVP); condens -= PGREVP;
-            PGSMLT = dt * fminf(Qgs, speed * PGSMLT); freeze -= PGSMLT;
-            PGGMLT = dt * fminf(Qgi, speed * PGGMLT); freeze -= PGGMLT;
-            PGGFR = dt * fminf(Qgr, speed * PGGFR); freeze += PGGFR;
-            PGSFR = dt * fminf(Qgr, speed * PGSFR); freeze += PGSFR;
-            PGFLW = dt * fminf(Qgr, speed * PGFLW);
-            PGDEVP = dt * fminf(Qrs, speed * PGDEVP); condens -= PGDEVP;
-            a.Qrs[g] += PGFLW - PGDEVP;
-            a.Qv[i] += PGREVP + PGDEVP;
-            a.Qgr[g] += PGSMLT + PGGMLT - PGREVP - PGGFR - PGSFR - PGFLW;
-            a.Qgs[g] += PGSFR - PGSMLT;
-            a.Qgi[g] += PGGFR - PGGMLT;
-            const float cps = 1500.0f, rhos = 1500.0f, dl = 0.1f;
-            const float ms = rhos * dl;
-            float sum = 0.0f;
-            sum += d_Lwater(tGC) / (ms * cps) * condens;
-            sum += d_Lice(tGC) / (ms * cps) * depos;
-            sum += kc::Lf / (ms * cps) * freeze;
-            a.gT[g] += sum;
+        if (stages & 2) {  // calculateGroundTempGPU, Game/kernelSky.cu:1046-1059 (T^4 widened to double)
+            const float gtv = a.gT[g];
+            const double T4 = gtv * gtv * gtv * gtv;
+            a.gT[g] += (dt * speed) * ((1 - a.cover[g]) * (((1 - 0.25f) * irr - kc::ge * kc::oo * T4) / (1.0f * 1500.0f * kc::Cpds)));
         }
-        if ((stages & 16) && gh + 1 < G.Y - 1) {  // setTempsAtGroundGPU, Game/kernelSky.cu:255-268
-            const float T = d_potTemp(a.gT[g] - 273.15f, a.p[i], a.gP[g]) + 273.15f;
-            const float d2 = a.th[i] - T;
-            a.th[i] -= d2 * fminf(1.0f, dt * speed);
+        if ((stages & (4 | 8 | 16)) && gh + 1 < G.Y) {
+            const long i = gidx(G, x, gh + 1, zc);
+            if (stages & 4) {  // calculatePrecipHittingGroundMicroPhysicsGPU, Game/microPhysics.cu:1544-1636
+                const float Qgr = a.Qgr[g] > 1e-18f ? a.Qgr[g] : 0.0f;
+                const float Qgs = a.Qgs[g] > 1e-18f ? a.Qgs[g] : 0.0f;
+                const float Qgi = a.Qgi[g] > 1e-18f ? a.Qgi[g] : 0.0f;
+                const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
+                const float Qr = a.Qr[i] > 1e-18f ? a.Qr[i] : 0.0f;
+                const float Qs = a.Qs[i] > 1e-18f ? a.Qs[i] : 0.0f;
+                const float Qi = a.Qi[i] > 1e-18f ? a.Qi[i] : 0.0f;
+                float freeze = 0.0f;
+                const float tGC = a.gT[g] - 273.15f;
+                const float ps = a.p[i];
+                const float wind = a.u[i];
+                float3 fv = make_float3(0.0f, 0.0f, 0.0f);
+                const float Dair = a.rho[i];
+                const float tAK = a.th[i] * powf(ps / a.gP[g], kap);
+                if (Qgr > 0.0f || Qgs > 0.0f || Qgi > 0.0f) fv = d_fallVel(Qgr, Qgs, Qgi, Dair, 3, gm.R48, gm.S425, gm.S425);
+                const float GR = dt * fminf(Qr * speed, Qr);
+                const float GS = dt * fminf(Qs * speed, Qs);
+                const float GI = dt * fminf(Qi * speed, Qi);
+                float PGRFR = mp::PGRFR(tAK, tGC, ps, Qr, Qv, Qi, Qgr, Qgs, Dair, wind, fv);
+                if (tAK < 273.15f) {
+                    PGRFR = dt * fminf(Qr, speed * PGRFR);
+                    freeze += PGRFR;
+                }
+                a.Qr[i] += -GR;
+                a.Qs[i] += -GS;
+                a.Qi[i] += -GI;
+                a.Qgr[g] += GR - PGRFR;
+                a.Qgs[g] += GS;
+                a.Qgi[g] += GI + PGRFR;
+                const float Mair = 0.02896f, Mwater = 0.01802f;
+                const float XV = (Qv / Mwater) / ((Qv / Mwater) + (1 - Qv) / Mair);
+                const float Mth = XV * Mwater + (1 - XV) * Mair;
+                const float YV = XV * (Mwater / Mth);
+                const float yth = YV * 1.33f + (1 - YV) * 1.4f;
+                const float cpth = kc::R / (Mth * (yth - 1));
+                float sum = 0.0f;
+                sum += kc::Lf / cpth * freeze;
+                a.gT[g] += sum;
+            }
+            if (stages & 8) {  // calculateGroundMicroPhysicsGPU, Game/microPhysics.cu:1419-1541
+                const float Qrs = a.Qrs[g] > 1e-18f ? a.Qrs[g] : 0.0f;
+                const float Qgr = a.Qgr[g] > 1e-18f ? a.Qgr[g] : 0.0f;
+                const float Qgs = a.Qgs[g] > 1e-18f ? a.Qgs[g] : 0.0f;
+                const float Qgi = a.Qgi[g] > 1e-18f ? a.Qgi[g] : 0.0f;
+                const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
+                float condens = 0.0f, freeze = 0.0f, depos = 0.0f;
+                const float tGC = a.gT[g] - 273.15f;
+                const float ps = a.p[i];
+                const float wind = a.u[i];
+                const float cc = a.cover[g];
+                const float Dair = a.rho[i];
+                float tme = a.gt[g];
+                const float tAK = a.th[i] * powf(ps / a.gP[g], kap);
+                const float sI = d_slope(Dair, Qgi, 2);
+                float PGREVP = mp::PGREVP(tAK, tGC, ps, Qv, Qgr, irr, Dair, wind);
+                float PGSMLT = mp::PGSMLT(tAK, tGC, ps, Qgs, Qv, Qgr, irr, cc, Dair, wind, dt);
+                float PGGMLT = mp::PGGMLT(tGC, Qgi, Dair, sI);
+                float PGGFR = mp::PGGFR(tAK, tGC, ps, Qgr, Qv, Qgi, Dair, irr, cc, wind, dt);
+                float PGSFR = mp::PGSFR(tAK, tGC, ps, Qv, Qgr, Qgs, Dair, wind, dt);
+                float PGFLW = mp::PGFLW(Qgr);
+                float PGDEVP = mp::PGDEVP(&tme, Qgr, Qgs, Qgi, Qrs);
+                a.gt[g] = tme;
+                PGREVP = dt * fminf(Qgr, speed * PGREVP); condens -= PGREVP;
+                PGSMLT = dt * fminf(Qgs, speed * PGSMLT); freeze -= PGSMLT;
+                PGGMLT = dt * fminf(Qgi, speed * PGGMLT); freeze -= PGGMLT;
+                PGGFR = dt * fminf(Qgr, speed * PGGFR); freeze += PGGFR;
+                PGSFR = dt * fminf(Qgr, speed * PGSFR); freeze += PGSFR;
+                PGFLW = dt * fminf(Qgr, speed * PGFLW);
+                PGDEVP = dt * fminf(Qrs, speed * PGDEVP); condens -= PGDEVP;
+                a.Qrs[g] += PGFLW - PGDEVP;
+                a.Qv[i] += PGREVP + PGDEVP;
+                a.Qgr[g] += PGSMLT + PGGMLT - PGREVP - PGGFR - PGSFR - PGFLW;
+                a.Qgs[g] += PGSFR - PGSMLT;
+                a.Qgi[g] += PGGFR - PGGMLT;
+                const float cps = 1500.0f, rhos = 1500.0f, dl = 0.1f;
+                const float ms = rhos * dl;
+                float sum = 0.0f;
+                sum += d_Lwater(tGC) / (ms * cps) * condens;
+                sum += d_Lice(tGC) / (ms * cps) * depos;
+                sum += kc::Lf / (ms * cps) * freeze;
+                a.gT[g] += sum;
+            }
+            if ((stages & 16) && gh + 1 < G.Y - 1) {  // setTempsAtGroundGPU, Game/kernelSky.cu:255-268
+                const float T = d_potTemp(a.gT[g] - 273.15f, a.p[i], a.gP[g]) + 273.15f;
+                const float d2 = a.th[i] - T;
+                a.th[i] -= d2 * fminf(1.0f, dt * speed);
+            }
         }
     }
-}
+};
 hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed, float irr) {
     GroundArgs a{e.f[SKT_QV], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_QC], e.f[SKT_QW], e.f[SKT_POTTEMP],
                  e.f[SKT_VELX], e.f[SKT_PRESSURE], e.f[SKT_DENSITY], e.f[SKT_QRS], e.f[SKT_QGR], e.f[SKT_QGS],
-                 e.f[SKT_QGI], e.f[SKT_GROUND_P], e.f[SKT_GROUND_TIME], e.f[SKT_GROUND_T], e.f[SKT_CLOUD_COVER]};
-    hipLaunchKernelGGL(k_ground_columns, grid1(e.G), dim3(256), 0, e.stream, geom(e), a, stages, dt, speed, irr, e.dx,
-                       e.gm);
-    return hipGetLastError();
+                 e.f[SKT_QGI], e.f[SKT_GROUND_P], e.f[SKT_GROUND_TIME], e.f[SKT_GROUND_T], e.f[SKT_CLOUD_COVER],
+                 nullptr};
+    // a recorded ensemble step reads the member's irradiance of each step
+    // from its device slot
+    a.irrDev = e.rec ? e.rec->irr : nullptr;
+    return klaunch<KGroundColumns>(e, grid1(e.G), dim3(256), 0, geom(e), a, stages, dt, speed, irr, e.dx, e.gm);
 }
 
 // advectPrecipGPU, Game/kernelSky.cu:576-657. Each level reads its own and
@@ -207,71 +224,75 @@ hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed,
 // it, so the chunks run side by side with the results of one walk per column.
 constexpr int PRECIP_CHUNKS = 4;
 static_assert(64 * PRECIP_CHUNKS <= 256, "k_precip blocks (64 x PRECIP_CHUNKS) must fit the 256-thread launch bound");
-KL void k_precip(Geom G, PrecipBatch pbt, const float* __restrict__ th, const float* __restrict__ p,
-                 const float* __restrict__ qv, const float* __restrict__ gP, float dx, Gammas gm) {
-    const float dts = pbt.dts[blockIdx.y];
-    if (dts == 0.0f) return;  // the whole block
-    const long g = blockIdx.x * 64L + threadIdx.x;
-    const bool col = g < (long)G.X * G.Z;
-    const long gc = col ? g : 0;
-    float* q = pbt.q[blockIdx.y];
-    const int type = pbt.type[blockIdx.y];
-    const int x = (int)(gc % G.X), zc = (int)(gc / G.X);
-    const float kap = kc::Rsd / kc::Cpd;
-    const float gp = gP[gc];
-    int y0 = G.gh[gc] + 1;
-    if (y0 < 0) y0 = 0;
-    const int yc = (G.Y + PRECIP_CHUNKS - 1) / PRECIP_CHUNKS;
-    const int ylo = (int)threadIdx.y * yc, yhi = min(G.Y, ylo + yc);
-    const int ya = max(ylo, y0);
-    const bool work = col && ya < yhi;
-    const float qTop = work && yhi < G.Y ? q[gidx(G, x, yhi, zc)] : 0.0f;
-    __syncthreads();
-    if (!work) return;
-    // per level: Q, theta, p, qv; levels y+1 and y+2 are loaded ahead
-    struct Lv {
-        float q, t, pr, v;
-    };
-    auto fvAt = [&](const Lv& L) {
-        const float T = L.t * powf(L.pr / gp, kap);
-        const float Tv = T * (0.608f * L.v + 1);
-        const float den = L.pr * 100 / (kc::Rsd * Tv);
-        const float3 v3 = d_fallVel(L.q, L.q, L.q, den, type, gm.R48, gm.S425, gm.S425);
-        return type == 0 ? v3.x : (type == 1 ? v3.y : v3.z);
-    };
-    const long i0 = gidx(G, x, ya, zc);
-    auto load = [&](int y, Lv& L) {
-        if (y <= yhi && y < G.Y) {
-            const long j = i0 + (long)(y - ya) * G.X;
-            L.q = y == yhi ? qTop : q[j];
-            L.t = th[j];
-            L.pr = p[j];
-            L.v = qv[j];
+struct KPrecip {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const PrecipBatch& pbt, const float* __restrict__ th, const float* __restrict__ p,
+                     const float* __restrict__ qv, const float* __restrict__ gP, float dx, const Gammas& gm) {
+        // tiles: column blocks of 64 (fastest), then the batch's fields
+        const unsigned nbx = (unsigned)(((long)G.X * G.Z + 63) / 64);
+        const unsigned fi = vb.id / nbx, cb = vb.id % nbx;
+        const float dts = pbt.dts[fi];
+        if (dts == 0.0f) return;  // the whole block
+        const long g = cb * 64L + threadIdx.x;
+        const bool col = g < (long)G.X * G.Z;
+        const long gc = col ? g : 0;
+        float* q = pbt.q[fi];
+        const int type = pbt.type[fi];
+        const int x = (int)(gc % G.X), zc = (int)(gc / G.X);
+        const float kap = kc::Rsd / kc::Cpd;
+        const float gp = gP[gc];
+        int y0 = G.gh[gc] + 1;
+        if (y0 < 0) y0 = 0;
+        const int yc = (G.Y + PRECIP_CHUNKS - 1) / PRECIP_CHUNKS;
+        const int ylo = (int)threadIdx.y * yc, yhi = min(G.Y, ylo + yc);
+        const int ya = max(ylo, y0);
+        const bool work = col && ya < yhi;
+        const float qTop = work && yhi < G.Y ? q[gidx(G, x, yhi, zc)] : 0.0f;
+        __syncthreads();
+        if (!work) return;
+        // per level: Q, theta, p, qv; levels y+1 and y+2 are loaded ahead
+        struct Lv {
+            float q, t, pr, v;
+        };
+        auto fvAt = [&](const Lv& L) {
+            const float T = L.t * powf(L.pr / gp, kap);
+            const float Tv = T * (0.608f * L.v + 1);
+            const float den = L.pr * 100 / (kc::Rsd * Tv);
+            const float3 v3 = d_fallVel(L.q, L.q, L.q, den, type, gm.R48, gm.S425, gm.S425);
+            return type == 0 ? v3.x : (type == 1 ? v3.y : v3.z);
+        };
+        const long i0 = gidx(G, x, ya, zc);
+        auto load = [&](int y, Lv& L) {
+            if (y <= yhi && y < G.Y) {
+                const long j = i0 + (long)(y - ya) * G.X;
+                L.q = y == yhi ? qTop : q[j];
+                L.t = th[j];
+                L.pr = p[j];
+                L.v = qv[j];
+            }
+        };
+        Lv l0{0.0f, 0.0f, 0.0f, 0.0f}, l1 = l0, l2 = l0;
+        load(ya, l0);
+        load(ya + 1, l1);
+        float fv = fvAt(l0);
+        long i = i0;
+        for (int y = ya; y < yhi; y++, i += G.X) {
+            load(y + 2, l2);
+            const bool up = y < G.Y - 1;
+            const float Qc = l0.q, Qu = up ? l1.q : 0.0f;
+            const float fvu = up ? fvAt(l1) : 0.0f;
+            float nq = Qc;
+            nq -= dts * fminf((fv / dx) * Qc, Qc);
+            if (up) nq += dts * fminf((fvu / dx) * Qu, Qu);
+            q[i] = nq;
+            fv = fvu;
+            l0 = l1;
+            l1 = l2;
         }
-    };
-    Lv l0{0.0f, 0.0f, 0.0f, 0.0f}, l1 = l0, l2 = l0;
-    load(ya, l0);
-    load(ya + 1, l1);
-    float fv = fvAt(l0);
-    long i = i0;
-    for (int y = ya; y < yhi; y++, i += G.X) {
-        load(y + 2, l2);
-        const bool up = y < G.Y - 1;
-        const float Qc = l0.q, Qu = up ? l1.q : 0.0f;
-        const float fvu = up ? fvAt(l1) : 0.0f;
-        float nq = Qc;
-        nq -= dts * fminf((fv / dx) * Qc, Qc);
-        if (up) nq += dts * fminf((fvu / dx) * Qu, Qu);
-        q[i] = nq;
-        fv = fvu;
-        l0 = l1;
-        l1 = l2;
     }
-}
+};
 hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt) {
-    klaunch(e, k_precip, dim3((unsigned)((e.G + 63) / 64), pbt.n), dim3(64, PRECIP_CHUNKS), 0, geom(e), pbt,
-            e.f[SKT_POTTEMP], e.f[SKT_PRESSURE], e.f[SKT_QV], e.f[SKT_GROUND_P], e.dx, e.gm);
-    return hipGetLastError();
+    return klaunch<KPrecip>(e, dim3((unsigned)((e.G + 63) / 64), pbt.n), dim3(64, PRECIP_CHUNKS), 0, geom(e), pbt,
+                            e.f[SKT_POTTEMP], e.f[SKT_PRESSURE], e.f[SKT_QV], e.f[SKT_GROUND_P], e.dx, e.gm);
 }
 
 // -------------------------------------------------------------- buoyancy ---
@@ -287,148 +308,151 @@ hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt) {
 // cell inside the domain is sky. Blocks are 64 (x) x 4 (z) columns of one
 // chunk; a chunk that starts above the ground reads the level below it.
 constexpr int YC_BUOY = 16;
-KL void k_buoyancy(Geom G, int yc, unsigned nyc, float* __restrict__ v, const float* __restrict__ th,
-                   const float* __restrict__ qv, const float* __restrict__ Qr, const float* __restrict__ Qs,
-                   const float* __restrict__ Qi, const float* __restrict__ p, const float* __restrict__ gP, float dt) {
-    const unsigned lin = blockIdx.x;
-    const int chunk = (int)(lin % nyc);
-    const unsigned tile = lin / nyc;
-    const int gx = (G.X + 63) >> 6;
-    const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
-    const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
-    if (x >= G.X || z >= G.Z) return;
-    const int X = G.X, Y = G.Y;
-    const long col = (long)X * Y * z + x;  // cell (x, 0, z)
-    const long gc = x + (long)X * z;
-    const int gh = G.gh[gc];
-    const float gps = gP[gc];
-    const float kap = kc::Rsd / kc::Cpd;
-    // ground heights of the four horizontal neighbour columns (INT_MAX
-    // outside the domain: no level there is sky)
-    const int ghL = x > 0 ? G.gh[gc - 1] : INT_MAX, ghR = x < X - 1 ? G.gh[gc + 1] : INT_MAX;
-    const int ghB = z > 0 ? G.gh[gc - X] : INT_MAX, ghF = z < G.Z - 1 ? G.gh[gc + X] : INT_MAX;
-    const int y0 = max(chunk * yc, gh + 1), y1 = min(Y, chunk * yc + yc);
-    if (y0 >= y1) return;
-    struct Lev {
-        float th, p, q, T, ws;
-    };
-    auto level = [&](int l, Lev& L) __attribute__((always_inline)) {
-        const long j = col + (long)X * l;
-        L.th = th[j];
-        L.p = p[j];
-        L.q = qv[j];
-        L.T = L.th * powf(L.p / gps, kap);
-        L.ws = d_ws((L.T - 273.15f), L.p);
-    };
-    Lev lm{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, lc = lm, lp = lm;  // levels y - 1, y, y + 1
-    if (y0 - 1 >= 0 && y0 - 1 > gh) level(y0 - 1, lm);
-    level(y0, lc);
-    for (int y = y0; y < y1; y++) {
-        const bool up = y < Y - 1;
-        if (up) level(y + 1, lp);
-        const bool down = y > 0 && y - 1 > gh;
-        const long i = col + (long)X * y;
-        const bool l = y > ghL, r = y > ghR, bk = y > ghB, fw = y > ghF;
-        const int validEnv = int(l) + int(r) + int(bk) + int(fw);
-        if (validEnv > 0) {
-            const float cur = v[i];
-            const float psC = lc.p;
-            const float psU = up ? lp.p : psC;
-            const float psD = down ? lm.p : psC;
-            const float QP = lc.q;
-            const float Qenv = ((r ? qv[i + 1] : 0.0f) + (l ? qv[i - 1] : 0.0f) + (fw ? qv[i + (long)X * Y] : 0.0f) +
-                                (bk ? qv[i - (long)X * Y] : 0.0f)) / validEnv;
-            const float QenvUp = up ? lp.q : 0.0f;
-            const float QenvDown = down ? lm.q : 0.0f;
-            const float thD = down ? lm.th : 0.0f, thU = up ? lp.th : 0.0f;
-            const float T = lc.T;
-            float Tad = lc.th, TadU = lc.th, TadD = lc.th;
-            const float Tenv = ((r ? th[i + 1] : 0.0f) + (l ? th[i - 1] : 0.0f) + (fw ? th[i + (long)X * Y] : 0.0f) +
-                                (bk ? th[i - (long)X * Y] : 0.0f)) / validEnv;
-            const float TenvUp = thU, TenvDown = thD;
-            if (down) {
-                // lower neighbour: TDown = thD (psD / gps)^kappa is its carried temperature
-                const float TDown = lm.T;
-                if ((T < 0.0f && QP >= d_wi((TDown - 273.15f), psD)) || QP >= lm.ws) {
-                    TadD = T - d_MLR(T - 273.15f, psC) * (psC - psD);
-                    TadD = TadD * powf(gps / psD, kap);
+struct KBuoyancy {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, float* __restrict__ v, const float* __restrict__ th,
+                       const float* __restrict__ qv, const float* __restrict__ Qr, const float* __restrict__ Qs,
+                       const float* __restrict__ Qi, const float* __restrict__ p, const float* __restrict__ gP, float dt) {
+        const unsigned lin = vb.id;
+        const int chunk = (int)(lin % nyc);
+        const unsigned tile = lin / nyc;
+        const int gx = (G.X + 63) >> 6;
+        const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
+        const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
+        if (x >= G.X || z >= G.Z) return;
+        const int X = G.X, Y = G.Y;
+        const long col = (long)X * Y * z + x;  // cell (x, 0, z)
+        const long gc = x + (long)X * z;
+        const int gh = G.gh[gc];
+        const float gps = gP[gc];
+        const float kap = kc::Rsd / kc::Cpd;
+        // ground heights of the four horizontal neighbour columns (INT_MAX
+        // outside the domain: no level there is sky)
+        const int ghL = x > 0 ? G.gh[gc - 1] : INT_MAX, ghR = x < X - 1 ? G.gh[gc + 1] : INT_MAX;
+        const int ghB = z > 0 ? G.gh[gc - X] : INT_MAX, ghF = z < G.Z - 1 ? G.gh[gc + X] : INT_MAX;
+        const int y0 = max(chunk * yc, gh + 1), y1 = min(Y, chunk * yc + yc);
+        if (y0 >= y1) return;
+        struct Lev {
+            float th, p, q, T, ws;
+        };
+        auto level = [&](int l, Lev& L) __attribute__((always_inline)) {
+            const long j = col + (long)X * l;
+            L.th = th[j];
+            L.p = p[j];
+            L.q = qv[j];
+            L.T = L.th * powf(L.p / gps, kap);
+            L.ws = d_ws((L.T - 273.15f), L.p);
+        };
+        Lev lm{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, lc = lm, lp = lm;  // levels y - 1, y, y + 1
+        if (y0 - 1 >= 0 && y0 - 1 > gh) level(y0 - 1, lm);
+        level(y0, lc);
+        for (int y = y0; y < y1; y++) {
+            const bool up = y < Y - 1;
+            if (up) level(y + 1, lp);
+            const bool down = y > 0 && y - 1 > gh;
+            const long i = col + (long)X * y;
+            const bool l = y > ghL, r = y > ghR, bk = y > ghB, fw = y > ghF;
+            const int validEnv = int(l) + int(r) + int(bk) + int(fw);
+            if (validEnv > 0) {
+                const float cur = v[i];
+                const float psC = lc.p;
+                const float psU = up ? lp.p : psC;
+                const float psD = down ? lm.p : psC;
+                const float QP = lc.q;
+                const float Qenv = ((r ? qv[i + 1] : 0.0f) + (l ? qv[i - 1] : 0.0f) + (fw ? qv[i + (long)X * Y] : 0.0f) +
+                                    (bk ? qv[i - (long)X * Y] : 0.0f)) / validEnv;
+                const float QenvUp = up ? lp.q : 0.0f;
+                const float QenvDown = down ? lm.q : 0.0f;
+                const float thD = down ? lm.th : 0.0f, thU = up ? lp.th : 0.0f;
+                const float T = lc.T;
+                float Tad = lc.th, TadU = lc.th, TadD = lc.th;
+                const float Tenv = ((r ? th[i + 1] : 0.0f) + (l ? th[i - 1] : 0.0f) + (fw ? th[i + (long)X * Y] : 0.0f) +
+                                    (bk ? th[i - (long)X * Y] : 0.0f)) / validEnv;
+                const float TenvUp = thU, TenvDown = thD;
+                if (down) {
+                    // lower neighbour: TDown = thD (psD / gps)^kappa is its carried temperature
+                    const float TDown = lm.T;
+                    if ((T < 0.0f && QP >= d_wi((TDown - 273.15f), psD)) || QP >= lm.ws) {
+                        TadD = T - d_MLR(T - 273.15f, psC) * (psC - psD);
+                        TadD = TadD * powf(gps / psD, kap);
+                    }
                 }
-            }
-            if (up) {
-                const float TUp = lp.T;
-                if ((T < 0.0f && QP >= d_wi((TUp - 273.15f), psU)) || QP >= lp.ws) {
-                    TadU = d_MLR(T - 273.15f, psC) * (psU - psC) + T;
-                    TadU = TadU * powf(gps / psU, kap);
+                if (up) {
+                    const float TUp = lp.T;
+                    if ((T < 0.0f && QP >= d_wi((TUp - 273.15f), psU)) || QP >= lp.ws) {
+                        TadU = d_MLR(T - 273.15f, psC) * (psU - psC) + T;
+                        TadU = TadU * powf(gps / psU, kap);
+                    }
                 }
+                float B = 0.0f, BU = 0.0f, BD = 0.0f;
+                const float totalQ = Qr[i] + Qs[i] + Qi[i];
+                if (down) {
+                    const float VTP = TadD * (1.0f + 0.608f * QP), VTE = TenvDown * (1.0f + 0.608f * QenvDown);
+                    BD = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
+                }
+                {
+                    const float VTP = Tad * (1.0f + 0.608f * QP), VTE = Tenv * (1.0f + 0.608f * Qenv);
+                    B = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
+                }
+                if (up) {
+                    const float VTP = TadU * (1.0f + 0.608f * QP), VTE = TenvUp * (1.0f + 0.608f * QenvUp);
+                    BU = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
+                }
+                float bf = B;
+                if (cur > 0.0f) bf = BU;
+                else if (cur < 0.0f) bf = BD;
+                float change = bf * dt;
+                if ((up && cur > 0.0f && -change > cur) || (down && cur < 0.0f && -change < cur)) change = -cur;
+                v[i] = cur + change;
             }
-            float B = 0.0f, BU = 0.0f, BD = 0.0f;
-            const float totalQ = Qr[i] + Qs[i] + Qi[i];
-            if (down) {
-                const float VTP = TadD * (1.0f + 0.608f * QP), VTE = TenvDown * (1.0f + 0.608f * QenvDown);
-                BD = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
-            }
-            {
-                const float VTP = Tad * (1.0f + 0.608f * QP), VTE = Tenv * (1.0f + 0.608f * Qenv);
-                B = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
-            }
-            if (up) {
-                const float VTP = TadU * (1.0f + 0.608f * QP), VTE = TenvUp * (1.0f + 0.608f * QenvUp);
-                BU = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
-            }
-            float bf = B;
-            if (cur > 0.0f) bf = BU;
-            else if (cur < 0.0f) bf = BD;
-            float change = bf * dt;
-            if ((up && cur > 0.0f && -change > cur) || (down && cur < 0.0f && -change < cur)) change = -cur;
-            v[i] = cur + change;
+            lm = lc;
+            lc = lp;
         }
-        lm = lc;
-        lc = lp;
     }
-}
+};
 hipError_t launch_buoyancy(EnvState& e, float dt) {
     const unsigned nyc = (unsigned)((e.Y + YC_BUOY - 1) / YC_BUOY);
     const unsigned nb = (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * nyc;
-    hipLaunchKernelGGL(k_buoyancy, dim3(nb), B3, 0, e.stream, geom(e), YC_BUOY, nyc, e.f[SKT_VELY], e.f[SKT_POTTEMP],
-                       e.f[SKT_QV], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], dt);
-    return hipGetLastError();
+    return klaunch<KBuoyancy>(e, dim3(nb), B3, 0, geom(e), YC_BUOY, nyc, e.f[SKT_VELY], e.f[SKT_POTTEMP], e.f[SKT_QV],
+                              e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], dt);
 }
 
 // ------------------------------------------------------------------- EOS ---
 // initDensity, Game/kernelSky.cu:1432-1452 (+ the old-density copy)
-KL void k_init_density(Geom G, float* __restrict__ rho, float* __restrict__ rho_old, const float* __restrict__ th,
-                       const float* __restrict__ p, const float* __restrict__ qv, const float* __restrict__ gP) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
-    const long i = gidx(G, x, y, z);
-    const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
-    const float Tv = T * (0.608f * qv[i] + 1);
-    const float d = p[i] * 100 / (kc::Rsd * Tv);
-    rho[i] = d;
-    if (rho_old) rho_old[i] = d;
-}
+struct KInitDensity {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, float* __restrict__ rho, float* __restrict__ rho_old, const float* __restrict__ th,
+                           const float* __restrict__ p, const float* __restrict__ qv, const float* __restrict__ gP) {
+        CELL_SETUP_V;
+        (void)bid;
+        if (!inb || isGround(G, x, y, z)) return;
+        const long i = gidx(G, x, y, z);
+        const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+        const float Tv = T * (0.608f * qv[i] + 1);
+        const float d = p[i] * 100 / (kc::Rsd * Tv);
+        rho[i] = d;
+        if (rho_old) rho_old[i] = d;
+    }
+};
 hipError_t launch_init_density(EnvState& e, int copy_old) {
-    hipLaunchKernelGGL(k_init_density, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_DENSITY],
-                       copy_old ? e.f[SKT_DENSITY_OLD] : nullptr, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE], e.f[SKT_QV],
-                       e.f[SKT_GROUND_P]);
-    return hipGetLastError();
+    return klaunch<KInitDensity>(e, grid3(e), B3, 0, geom(e), e.f[SKT_DENSITY],
+                                 copy_old ? e.f[SKT_DENSITY_OLD] : nullptr, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE],
+                                 e.f[SKT_QV], e.f[SKT_GROUND_P]);
 }
 // calculateNewPressure, Game/kernelSky.cu:1454-1474
-KL void k_new_pressure(Geom G, float* __restrict__ p, const float* __restrict__ rho, const float* __restrict__ th,
-                       const float* __restrict__ qv, const float* __restrict__ gP) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || isGround(G, x, y, z)) return;
-    const long i = gidx(G, x, y, z);
-    const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
-    const float Tv = T * (0.608f * qv[i] + 1);
-    p[i] = rho[i] * kc::Rsd * Tv / 100.0f;
-}
+struct KNewPressure {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, float* __restrict__ p, const float* __restrict__ rho, const float* __restrict__ th,
+                           const float* __restrict__ qv, const float* __restrict__ gP) {
+        CELL_SETUP_V;
+        (void)bid;
+        if (!inb || isGround(G, x, y, z)) return;
+        const long i = gidx(G, x, y, z);
+        const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+        const float Tv = T * (0.608f * qv[i] + 1);
+        p[i] = rho[i] * kc::Rsd * Tv / 100.0f;
+    }
+};
 hipError_t launch_new_pressure(EnvState& e) {
-    hipLaunchKernelGGL(k_new_pressure, grid3(e), B3, 0, e.stream, geom(e), e.f[SKT_PRESSURE], e.f[SKT_DENSITY],
-                       e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P]);
-    return hipGetLastError();
+    return klaunch<KNewPressure>(e, grid3(e), B3, 0, geom(e), e.f[SKT_PRESSURE], e.f[SKT_DENSITY], e.f[SKT_POTTEMP],
+                                 e.f[SKT_QV], e.f[SKT_GROUND_P]);
 }
 
 // resetVelPressProj, Game/kernelSky.cu:1648-1682. The reference walks each
@@ -436,22 +460,23 @@ hipError_t launch_new_pressure(EnvState& e) {
 // cell is reset iff no cell of its column at or before its plane is ground:
 // y > ghz, the largest ground height over planes 0..z of its (x, z) column.
 // One thread per cell; the resets of different cells are independent.
-KL void k_reset_vel(Geom G, const int* __restrict__ ghz, float* __restrict__ u, float* __restrict__ v,
-                    float* __restrict__ w, const float* __restrict__ defU, const float* __restrict__ defW) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || y <= ghz[x + (long)G.X * z]) return;
-    const long i = gidx(G, x, y, z);
-    if (y == G.Y - 1 || isGround(G, x, y + 1, z)) v[i] = 0.0f;
-    if (x == G.X - 1) u[i] = defU[y];
-    if (x < G.X - 1 && isGround(G, x + 1, y, z)) u[i] = 0.0f;
-    if (z == G.Z - 1) w[i] = defW[y];
-    if (z < G.Z - 1 && isGround(G, x, y, z + 1)) w[i] = 0.0f;
-}
+struct KResetVel {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const int* __restrict__ ghz, float* __restrict__ u, float* __restrict__ v,
+                        float* __restrict__ w, const float* __restrict__ defU, const float* __restrict__ defW) {
+        CELL_SETUP_V;
+        (void)bid;
+        if (!inb || y <= ghz[x + (long)G.X * z]) return;
+        const long i = gidx(G, x, y, z);
+        if (y == G.Y - 1 || isGround(G, x, y + 1, z)) v[i] = 0.0f;
+        if (x == G.X - 1) u[i] = defU[y];
+        if (x < G.X - 1 && isGround(G, x + 1, y, z)) u[i] = 0.0f;
+        if (z == G.Z - 1) w[i] = defW[y];
+        if (z < G.Z - 1 && isGround(G, x, y, z + 1)) w[i] = 0.0f;
+    }
+};
 hipError_t launch_reset_vel(EnvState& e) {
-    hipLaunchKernelGGL(k_reset_vel, grid3(e), B3, 0, e.stream, geom(e), e.ghz, e.f[SKT_VELX], e.f[SKT_VELY],
-                       e.f[SKT_VELZ], e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ]);
-    return hipGetLastError();
+    return klaunch<KResetVel>(e, grid3(e), B3, 0, geom(e), e.ghz, e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ],
+                              e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ]);
 }
 
 // setToDefault (Game/kernelSky.cu:1714-1728) with a per-level profile: the
@@ -459,17 +484,18 @@ hipError_t launch_reset_vel(EnvState& e) {
 // cell, so a cell takes prof[y] iff y > ghz (the walk reaches it, as in
 // k_reset_vel). Without a profile, setToValue(0) (Game/kernelMaths.cu:12-23)
 // writes every cell, ground included.
-KL void k_reset_param(Geom G, const int* __restrict__ ghz, float* __restrict__ a, const float* __restrict__ prof) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb) return;
-    const long i = gidx(G, x, y, z);
-    if (!prof) a[i] = 0.0f;
-    else if (y > ghz[x + (long)G.X * z]) a[i] = prof[y];
-}
+struct KResetParam {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const int* __restrict__ ghz, float* __restrict__ a, const float* __restrict__ prof) {
+        CELL_SETUP_V;
+        (void)bid;
+        if (!inb) return;
+        const long i = gidx(G, x, y, z);
+        if (!prof) a[i] = 0.0f;
+        else if (y > ghz[x + (long)G.X * z]) a[i] = prof[y];
+    }
+};
 hipError_t launch_reset_param(EnvState& e, float* a, const float* prof) {
-    hipLaunchKernelGGL(k_reset_param, grid3(e), B3, 0, e.stream, geom(e), e.ghz, a, prof);
-    return hipGetLastError();
+    return klaunch<KResetParam>(e, grid3(e), B3, 0, geom(e), e.ghz, a, prof);
 }
 
 // ---------------------------------------------------------- microphysics ---
@@ -484,190 +510,192 @@ struct MicroArgs {
 // calculateEnvMicroPhysicsGPU (Game/microPhysics.cu:1035-1417) fused with
 // addHeatGPU (Game/kernelSky.cu:1249-1282): the condens/depos/freeze arrays
 // the reference passes between them are registers here.
-KL void k_microphysics(Geom G, MicroArgs a, float dt, float speed, Gammas gm) {
-    CELL_SETUP;
-    __shared__ double shr[4];
-    float rv[33];
-    bool inRegion = false;
-    if (inb && !isGround(G, x, y, z)) {
-        const long i = gidx(G, x, y, z);
-        const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
-        const float Qw = a.Qw[i] > 1e-18f ? a.Qw[i] : 0.0f;
-        const float Qc = a.Qc[i] > 1e-18f ? a.Qc[i] : 0.0f;
-        const float Qr = a.Qr[i] > 1e-18f ? a.Qr[i] : 0.0f;
-        const float Qs = a.Qs[i] > 1e-18f ? a.Qs[i] : 0.0f;
-        const float Qi = a.Qi[i] > 1e-18f ? a.Qi[i] : 0.0f;
-        const float QwMin = 0.001f, QcMin = 0.001f, QiMin = 0.0006f;
-        const float ps = a.p[i];
-        float3 fv = make_float3(0.0f, 0.0f, 0.0f);
-        const float Dair = a.rho[i];
-        const float tK = a.th[i] * powf(ps / a.gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
-        const float T = tK - 273.15f;
-        if (Qr > 0.0f || Qs > 0.0f || Qi > 0.0f) fv = d_fallVel(Qr, Qs, Qi, Dair, 3, gm.R48, gm.S425, gm.S425);
-        const float QWS = d_ws(T, ps);
-        const float QWI = d_wi(T, ps);
-        const float PT1 = T < 0.0f && Qw + Qc > 0 + 1e-6f ? 1.0f : 0.0f;
-        const float PT2 = T < 0.0f && Qr < 1e-4f && Qs < 1e-4f ? 1.0f : 0.0f;
-        const float PT3 = T < 0.0f && Qr < 1e-4f ? 1.0f : 0.0f;
-        const float sR = d_slope(Dair, Qr, 0), sS = d_slope(Dair, Qs, 1), sI = d_slope(Dair, Qi, 2);
-        const float dts = dt * speed;
-        float PVCON = mp::PVCON(T, ps, Qv, QWS, dt, speed);
-        float PVDEP = mp::PVDEP(T, ps, Qv, QWI, dt, speed);
-        PVDEP = fminf(PVDEP, 0.0f);
-        float PIMLT = mp::PIMLT(T, Qc);
-        float PIDW = mp::PIDW(dt, T, Qc, Qw, Dair, ps);
-        float PIHOM = mp::PIHOM(T, Qw);
-        float PIACR = mp::PIACR(gm, Qr, Qc, Dair, sR);
-        float PRACI = mp::PRACI(gm, Qr, Qc, Dair, sR);
-        float PRAUT = mp::PRAUT(Qw, QwMin, T);
-        float PRACW = mp::PRACW(gm, Qr, Qw, Dair, sR);
-        float PREVP = mp::PREVP(gm, T, Qr, Qv, QWS, Dair, ps, sR, PT1);
-        float PRACS = mp::PRACS(Qr, Qs, PT2, fv, Dair, sS, sR);
-        float PSACW = mp::PSACW(gm, Qs, Qw, Dair, sS);
-        float PSACR = mp::PSACR(Qr, Qs, fv, Dair, sR, sS);
-        float PSACI = mp::PSACI(gm, T, Qs, Qc, Dair, sS);
-        float PSAUT = mp::PSAUT(T, Qc, QcMin);
-        float PSFW = mp::PSFW(dt, T, ps, Qs, Qw, Qc, Qv, QWI, Dair);
-        float PSFI = mp::PSFI(T, ps, Qs, Qc, Qv, QWI);
-        float PSDEP = mp::PSDEP(gm, T, ps, Qv, QWI, Dair, sS);
-        float PSSUB = fmaxf(0.0f, -1 * PSDEP);
-        PSDEP = fmaxf(0.0f, PSDEP);
-        float PSMLT = mp::PSMLT(gm, T, ps, Qs, Qv, Dair, sS, PSACW, PSACR);
-        float PGAUT = mp::PGAUT(T, Qs, QiMin);
-        float PGFR = mp::PGFR(T, Qr, Dair, sR);
-        float PGACW = mp::PGACW(gm, Qi, Qw, sI, Dair);
-        float PGACI = mp::PGACI(gm, Qi, Qc, sI, Dair);
-        float PGACR = mp::PGACR(Qi, Qr, Dair, fv, sR, sI);
-        float PGACS = mp::PGACS(T, Qs, Qi, Dair, fv, sS, sI, false);
-        float PGSUB = mp::PGSUB(gm, T, ps, Qv, QWI, Qi, Dair, sI);
-        float PGMLT = mp::PGMLT(gm, T, ps, Qw, Qr, Qi, Qv, Dair, sI, PGACW, PGACR, dts);
-        float PGDRY = mp::PGDRY(Qw, Qc, Qr, Qs, PGACW, PGACI, PGACR, PGACS, dts);
-        float PGWET = mp::PGWET(gm, T, ps, Qc, Qs, Qi, Qv, fv, Dair, sS, sI, PGACI, PGACS, dts);
-        float PGACR1 = mp::PGACR1(T, Qw, Qc, Qs, Qi, Dair, fv, sS, sI, PGWET, PGACW, PGACI, PGACS, dts);
-        float W = 0.0f;
-        if (PGDRY > PGWET) W = 1.0f;
-        float cond = 0.0f, dep = 0.0f, frz = 0.0f;
-        if (PVCON >= 0) { PVCON = dt * fminf(Qv, speed * PVCON); cond += PVCON; }
-        else if (PVCON < 0) { PVCON = dt * fmaxf(-Qw, speed * PVCON); cond += PVCON; }
-        if (PVDEP >= 0) { PVDEP = dt * fminf(Qv, speed * PVDEP); dep += PVDEP; }
-        else if (PVDEP < 0) { PVDEP = dt * fmaxf(-Qc, speed * PVDEP); dep += PVDEP; }
-        float nQv = a.Qv[i], nQw = a.Qw[i], nQc = a.Qc[i], nQr = a.Qr[i], nQs = a.Qs[i], nQi = a.Qi[i];
-        if (T < 0) {
-            PIDW = dt * fminf(Qw, speed * PIDW); dep += PIDW;
-            PIHOM = dt * fminf(Qw, speed * PIHOM); frz += PIHOM;
-            PIACR = dt * fminf(Qr, speed * PIACR); frz += PIACR;
-            PRACI = dt * fminf(Qc, speed * PRACI);
-            PRAUT = dt * fminf(Qw, speed * PRAUT);
-            PRACW = dt * fminf(Qw, speed * PRACW);
-            PREVP = dt * fminf(Qr, speed * PREVP * (1 - PT1)); cond -= PREVP;
-            PRACS = dt * fminf(Qs, speed * PRACS * (1 - PT2));
-            PSACR = dt * fminf(Qr, speed * PSACR); frz += PSACR;
-            PSACW = dt * fminf(Qw, speed * PSACW); frz += PSACW;
-            PSACI = dt * fminf(Qc, speed * PSACI);
-            PSAUT = dt * fminf(Qc, speed * PSAUT);
-            PSFW = dt * fminf(Qw, speed * PSFW); frz += PSFW;
-            PSFI = dt * fminf(Qc, speed * PSFI);
-            PSDEP = dt * fminf(Qv, speed * PSDEP * (PT1)); dep += PSDEP;
-            PSSUB = dt * fminf(Qs, speed * PSSUB * (1 - PT1)); dep -= PSSUB;
-            PGAUT = dt * fminf(Qs, speed * PGAUT);
-            PGFR = dt * fminf(Qr, speed * PGFR); frz += PGFR;
-            PGACI = dt * fminf(Qc, speed * PGACI);
-            PGACR = dt * fminf(Qr, speed * PGACR * (1 - W)); frz += PGACR;
-            PGACS = dt * fminf(Qs, speed * PGACS);
-            PGSUB = dt * fminf(Qi, speed * PGSUB * (1 - PT1)); dep -= PGSUB;
-            if (W == 1.0f) {
-                if (PGACR1 >= 0) { PGACR1 = dt * fminf(Qr, speed * PGACR1); frz += PGACR1; }
-                else if (PGACR1 < 0) { PGACR1 = dt * fmaxf(-Qi, speed * PGACR1); frz += PGACR1; }
+struct KMicrophysics {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const MicroArgs& a, float dt, float speed, const Gammas& gm) {
+        CELL_SETUP_V;
+        __shared__ double shr[4];
+        float rv[33];
+        bool inRegion = false;
+        if (inb && !isGround(G, x, y, z)) {
+            const long i = gidx(G, x, y, z);
+            const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
+            const float Qw = a.Qw[i] > 1e-18f ? a.Qw[i] : 0.0f;
+            const float Qc = a.Qc[i] > 1e-18f ? a.Qc[i] : 0.0f;
+            const float Qr = a.Qr[i] > 1e-18f ? a.Qr[i] : 0.0f;
+            const float Qs = a.Qs[i] > 1e-18f ? a.Qs[i] : 0.0f;
+            const float Qi = a.Qi[i] > 1e-18f ? a.Qi[i] : 0.0f;
+            const float QwMin = 0.001f, QcMin = 0.001f, QiMin = 0.0006f;
+            const float ps = a.p[i];
+            float3 fv = make_float3(0.0f, 0.0f, 0.0f);
+            const float Dair = a.rho[i];
+            const float tK = a.th[i] * powf(ps / a.gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+            const float T = tK - 273.15f;
+            if (Qr > 0.0f || Qs > 0.0f || Qi > 0.0f) fv = d_fallVel(Qr, Qs, Qi, Dair, 3, gm.R48, gm.S425, gm.S425);
+            const float QWS = d_ws(T, ps);
+            const float QWI = d_wi(T, ps);
+            const float PT1 = T < 0.0f && Qw + Qc > 0 + 1e-6f ? 1.0f : 0.0f;
+            const float PT2 = T < 0.0f && Qr < 1e-4f && Qs < 1e-4f ? 1.0f : 0.0f;
+            const float PT3 = T < 0.0f && Qr < 1e-4f ? 1.0f : 0.0f;
+            const float sR = d_slope(Dair, Qr, 0), sS = d_slope(Dair, Qs, 1), sI = d_slope(Dair, Qi, 2);
+            const float dts = dt * speed;
+            float PVCON = mp::PVCON(T, ps, Qv, QWS, dt, speed);
+            float PVDEP = mp::PVDEP(T, ps, Qv, QWI, dt, speed);
+            PVDEP = fminf(PVDEP, 0.0f);
+            float PIMLT = mp::PIMLT(T, Qc);
+            float PIDW = mp::PIDW(dt, T, Qc, Qw, Dair, ps);
+            float PIHOM = mp::PIHOM(T, Qw);
+            float PIACR = mp::PIACR(gm, Qr, Qc, Dair, sR);
+            float PRACI = mp::PRACI(gm, Qr, Qc, Dair, sR);
+            float PRAUT = mp::PRAUT(Qw, QwMin, T);
+            float PRACW = mp::PRACW(gm, Qr, Qw, Dair, sR);
+            float PREVP = mp::PREVP(gm, T, Qr, Qv, QWS, Dair, ps, sR, PT1);
+            float PRACS = mp::PRACS(Qr, Qs, PT2, fv, Dair, sS, sR);
+            float PSACW = mp::PSACW(gm, Qs, Qw, Dair, sS);
+            float PSACR = mp::PSACR(Qr, Qs, fv, Dair, sR, sS);
+            float PSACI = mp::PSACI(gm, T, Qs, Qc, Dair, sS);
+            float PSAUT = mp::PSAUT(T, Qc, QcMin);
+            float PSFW = mp::PSFW(dt, T, ps, Qs, Qw, Qc, Qv, QWI, Dair);
+            float PSFI = mp::PSFI(T, ps, Qs, Qc, Qv, QWI);
+            float PSDEP = mp::PSDEP(gm, T, ps, Qv, QWI, Dair, sS);
+            float PSSUB = fmaxf(0.0f, -1 * PSDEP);
+            PSDEP = fmaxf(0.0f, PSDEP);
+            float PSMLT = mp::PSMLT(gm, T, ps, Qs, Qv, Dair, sS, PSACW, PSACR);
+            float PGAUT = mp::PGAUT(T, Qs, QiMin);
+            float PGFR = mp::PGFR(T, Qr, Dair, sR);
+            float PGACW = mp::PGACW(gm, Qi, Qw, sI, Dair);
+            float PGACI = mp::PGACI(gm, Qi, Qc, sI, Dair);
+            float PGACR = mp::PGACR(Qi, Qr, Dair, fv, sR, sI);
+            float PGACS = mp::PGACS(T, Qs, Qi, Dair, fv, sS, sI, false);
+            float PGSUB = mp::PGSUB(gm, T, ps, Qv, QWI, Qi, Dair, sI);
+            float PGMLT = mp::PGMLT(gm, T, ps, Qw, Qr, Qi, Qv, Dair, sI, PGACW, PGACR, dts);
+            float PGDRY = mp::PGDRY(Qw, Qc, Qr, Qs, PGACW, PGACI, PGACR, PGACS, dts);
+            float PGWET = mp::PGWET(gm, T, ps, Qc, Qs, Qi, Qv, fv, Dair, sS, sI, PGACI, PGACS, dts);
+            float PGACR1 = mp::PGACR1(T, Qw, Qc, Qs, Qi, Dair, fv, sS, sI, PGWET, PGACW, PGACI, PGACS, dts);
+            float W = 0.0f;
+            if (PGDRY > PGWET) W = 1.0f;
+            float cond = 0.0f, dep = 0.0f, frz = 0.0f;
+            if (PVCON >= 0) { PVCON = dt * fminf(Qv, speed * PVCON); cond += PVCON; }
+            else if (PVCON < 0) { PVCON = dt * fmaxf(-Qw, speed * PVCON); cond += PVCON; }
+            if (PVDEP >= 0) { PVDEP = dt * fminf(Qv, speed * PVDEP); dep += PVDEP; }
+            else if (PVDEP < 0) { PVDEP = dt * fmaxf(-Qc, speed * PVDEP); dep += PVDEP; }
+            float nQv = a.Qv[i], nQw = a.Qw[i], nQc = a.Qc[i], nQr = a.Qr[i], nQs = a.Qs[i], nQi = a.Qi[i];
+            if (T < 0) {
+                PIDW = dt * fminf(Qw, speed * PIDW); dep += PIDW;
+                PIHOM = dt * fminf(Qw, speed * PIHOM); frz += PIHOM;
+                PIACR = dt * fminf(Qr, speed * PIACR); frz += PIACR;
+                PRACI = dt * fminf(Qc, speed * PRACI);
+                PRAUT = dt * fminf(Qw, speed * PRAUT);
+                PRACW = dt * fminf(Qw, speed * PRACW);
+                PREVP = dt * fminf(Qr, speed * PREVP * (1 - PT1)); cond -= PREVP;
+                PRACS = dt * fminf(Qs, speed * PRACS * (1 - PT2));
+                PSACR = dt * fminf(Qr, speed * PSACR); frz += PSACR;
+                PSACW = dt * fminf(Qw, speed * PSACW); frz += PSACW;
+                PSACI = dt * fminf(Qc, speed * PSACI);
+                PSAUT = dt * fminf(Qc, speed * PSAUT);
+                PSFW = dt * fminf(Qw, speed * PSFW); frz += PSFW;
+                PSFI = dt * fminf(Qc, speed * PSFI);
+                PSDEP = dt * fminf(Qv, speed * PSDEP * (PT1)); dep += PSDEP;
+                PSSUB = dt * fminf(Qs, speed * PSSUB * (1 - PT1)); dep -= PSSUB;
+                PGAUT = dt * fminf(Qs, speed * PGAUT);
+                PGFR = dt * fminf(Qr, speed * PGFR); frz += PGFR;
+                PGACI = dt * fminf(Qc, speed * PGACI);
+                PGACR = dt * fminf(Qr, speed * PGACR * (1 - W)); frz += PGACR;
+                PGACS = dt * fminf(Qs, speed * PGACS);
+                PGSUB = dt * fminf(Qi, speed * PGSUB * (1 - PT1)); dep -= PGSUB;
+                if (W == 1.0f) {
+                    if (PGACR1 >= 0) { PGACR1 = dt * fminf(Qr, speed * PGACR1); frz += PGACR1; }
+                    else if (PGACR1 < 0) { PGACR1 = dt * fmaxf(-Qi, speed * PGACR1); frz += PGACR1; }
+                }
+                nQv += PSSUB * (1 - PT1) + PGSUB * (1 - PT1) + PREVP * (1 - PT1) - PSDEP * (PT1)-PVCON - PVDEP;
+                nQw += PVCON - PSACW - PSFW - PRAUT - PRACW - PIDW - PIHOM;
+                nQc += PVDEP + PIDW + PIHOM - PSAUT - PSACI - PRACI - PSFI - PGACI;
+                nQr += PRAUT + PRACW - PIACR - PSACR - PGACR * (1 - W) - PGACR1 * (W)-PGFR - PREVP * (1 - PT1);
+                nQs += PSAUT + PSACI + PSACW + PSFW + PSFI + PRACI * (PT3) + PIACR * (PT3) - PGACS - PGAUT -
+                       PRACS * (1 - PT2) + PSACR * (PT2) - PSSUB * (1 - PT1) + PSDEP * (PT1);
+                nQi += PGAUT + PGFR + PGDRY * (1 - W) + PGWET * (W) + PSACR * (1 - PT2) + PRACS * (1 - PT2) +
+                       PRACI * (1 - PT3) + PIACR * (1 - PT3) - PGSUB * (1 - PT1);
+            } else {
+                PIMLT = dt * fminf(Qc, speed * PIMLT); frz -= PIMLT;
+                PIACR = 0.0f;
+                PRAUT = dt * fminf(Qw, speed * PRAUT);
+                PRACW = dt * fminf(Qw, speed * PRACW);
+                PREVP = dt * fminf(Qr, speed * PREVP * (1 - PT1)); cond -= PREVP;
+                PSACW = dt * fminf(Qw, speed * PSACW);
+                PSMLT = dt * fminf(Qs, speed * PSMLT); frz -= PSMLT;
+                PGACW = dt * fminf(Qw, speed * PGACW); frz += PGACW;
+                PGACS = dt * fminf(Qs, speed * PGACS);
+                PGMLT = dt * fminf(Qi, speed * PGMLT); frz -= PGMLT;
+                nQv += PREVP * (1 - PT1) - PVCON - PVDEP;
+                nQw += PVCON + PIMLT - PRAUT - PRACW - PGACW - PSACW;
+                nQc += PVDEP - PIMLT;
+                nQr += PRAUT + PRACW + PSACW + PGACW + PGMLT + PSMLT - PREVP * (1 - PT1);
+                nQs += -PSMLT - PGACS;
+                nQi += -PGMLT + PGACS;
             }
-            nQv += PSSUB * (1 - PT1) + PGSUB * (1 - PT1) + PREVP * (1 - PT1) - PSDEP * (PT1)-PVCON - PVDEP;
-            nQw += PVCON - PSACW - PSFW - PRAUT - PRACW - PIDW - PIHOM;
-            nQc += PVDEP + PIDW + PIHOM - PSAUT - PSACI - PRACI - PSFI - PGACI;
-            nQr += PRAUT + PRACW - PIACR - PSACR - PGACR * (1 - W) - PGACR1 * (W)-PGFR - PREVP * (1 - PT1);
-            nQs += PSAUT + PSACI + PSACW + PSFW + PSFI + PRACI * (PT3) + PIACR * (PT3) - PGACS - PGAUT -
-                   PRACS * (1 - PT2) + PSACR * (PT2) - PSSUB * (1 - PT1) + PSDEP * (PT1);
-            nQi += PGAUT + PGFR + PGDRY * (1 - W) + PGWET * (W) + PSACR * (1 - PT2) + PRACS * (1 - PT2) +
-                   PRACI * (1 - PT3) + PIACR * (1 - PT3) - PGSUB * (1 - PT1);
-        } else {
-            PIMLT = dt * fminf(Qc, speed * PIMLT); frz -= PIMLT;
-            PIACR = 0.0f;
-            PRAUT = dt * fminf(Qw, speed * PRAUT);
-            PRACW = dt * fminf(Qw, speed * PRACW);
-            PREVP = dt * fminf(Qr, speed * PREVP * (1 - PT1)); cond -= PREVP;
-            PSACW = dt * fminf(Qw, speed * PSACW);
-            PSMLT = dt * fminf(Qs, speed * PSMLT); frz -= PSMLT;
-            PGACW = dt * fminf(Qw, speed * PGACW); frz += PGACW;
-            PGACS = dt * fminf(Qs, speed * PGACS);
-            PGMLT = dt * fminf(Qi, speed * PGMLT); frz -= PGMLT;
-            nQv += PREVP * (1 - PT1) - PVCON - PVDEP;
-            nQw += PVCON + PIMLT - PRAUT - PRACW - PGACW - PSACW;
-            nQc += PVDEP - PIMLT;
-            nQr += PRAUT + PRACW + PSACW + PGACW + PGMLT + PSMLT - PREVP * (1 - PT1);
-            nQs += -PSMLT - PGACS;
-            nQi += -PGMLT + PGACS;
+            a.Qv[i] = nQv; a.Qw[i] = nQw; a.Qc[i] = nQc; a.Qr[i] = nQr; a.Qs[i] = nQs; a.Qi[i] = nQi;
+            // addHeatGPU with the updated vapour
+            const float Mair = 0.02896f, Mwater = 0.01802f;
+            const float XV = (nQv / Mwater) / ((nQv / Mwater) + (1 - nQv) / Mair);
+            const float Mth = XV * Mwater + (1 - XV) * Mair;
+            const float YV = XV * (Mwater / Mth);
+            const float yth = YV * 1.33f + (1 - YV) * 1.4f;
+            const float cpth = yth * kc::R / (Mth * (yth - 1));
+            const float th = a.th[i];
+            float sum = 0.0f;
+            sum += d_Lwater(th - 273.15f) / cpth * cond;
+            sum += d_Lice(th - 273.15f) / cpth * dep;
+            sum += kc::Lf / cpth * frz;
+            a.th[i] = th + sum;
+            if (a.phase)
+                a.phase[i] = (uint8_t)(1 | (T < 0.0f ? 2 : 0) | (PT1 != 0.0f ? 4 : 0) | (PT2 != 0.0f ? 8 : 0) |
+                                       (PT3 != 0.0f ? 16 : 0) | (W != 0.0f ? 32 : 0));
+            if (a.region && x >= a.rmin[0] && x <= a.rmax[0] && y >= a.rmin[1] && y <= a.rmax[1] && z >= a.rmin[2] &&
+                z <= a.rmax[2]) {
+                inRegion = true;
+                float PVVAP = 0.0f, PVSUB = 0.0f, pc = PVCON, pd = PVDEP;
+                if (pc < 0.0f) { PVVAP = -pc; pc = 0.0f; }
+                if (pd < 0.0f) { PVSUB = -pd; pd = 0.0f; }
+                const float vals[33] = {pc, pd, PIMLT, PIDW, PIHOM, PIACR, PRACI, PRAUT, PRACW, PREVP, PRACS,
+                                        PSACW, PSACR, PSACI, PSAUT, PSFW, PSFI, PSDEP, PSSUB, PSMLT, PGAUT, PGFR,
+                                        PGACW, PGACI, PGACR, PGDRY * (1 - W), PGACS, PGSUB, PGMLT, PGWET * W, PGACR1, PVVAP, PVSUB};
+                for (int k = 0; k < 33; k++) rv[k] = vals[k];
+            }
+        } else if (inb) {
+            if (a.phase) a.phase[gidx(G, x, y, z)] = 0;
         }
-        a.Qv[i] = nQv; a.Qw[i] = nQw; a.Qc[i] = nQc; a.Qr[i] = nQr; a.Qs[i] = nQs; a.Qi[i] = nQi;
-        // addHeatGPU with the updated vapour
-        const float Mair = 0.02896f, Mwater = 0.01802f;
-        const float XV = (nQv / Mwater) / ((nQv / Mwater) + (1 - nQv) / Mair);
-        const float Mth = XV * Mwater + (1 - XV) * Mair;
-        const float YV = XV * (Mwater / Mth);
-        const float yth = YV * 1.33f + (1 - YV) * 1.4f;
-        const float cpth = yth * kc::R / (Mth * (yth - 1));
-        const float th = a.th[i];
-        float sum = 0.0f;
-        sum += d_Lwater(th - 273.15f) / cpth * cond;
-        sum += d_Lice(th - 273.15f) / cpth * dep;
-        sum += kc::Lf / cpth * frz;
-        a.th[i] = th + sum;
-        if (a.phase)
-            a.phase[i] = (uint8_t)(1 | (T < 0.0f ? 2 : 0) | (PT1 != 0.0f ? 4 : 0) | (PT2 != 0.0f ? 8 : 0) |
-                                   (PT3 != 0.0f ? 16 : 0) | (W != 0.0f ? 32 : 0));
-        if (a.region && x >= a.rmin[0] && x <= a.rmax[0] && y >= a.rmin[1] && y <= a.rmax[1] && z >= a.rmin[2] &&
-            z <= a.rmax[2]) {
-            inRegion = true;
-            float PVVAP = 0.0f, PVSUB = 0.0f, pc = PVCON, pd = PVDEP;
-            if (pc < 0.0f) { PVVAP = -pc; pc = 0.0f; }
-            if (pd < 0.0f) { PVSUB = -pd; pd = 0.0f; }
-            const float vals[33] = {pc, pd, PIMLT, PIDW, PIHOM, PIACR, PRACI, PRAUT, PRACW, PREVP, PRACS,
-                                    PSACW, PSACR, PSACI, PSAUT, PSFW, PSFI, PSDEP, PSSUB, PSMLT, PGAUT, PGFR,
-                                    PGACW, PGACI, PGACR, PGDRY * (1 - W), PGACS, PGSUB, PGMLT, PGWET * W, PGACR1, PVVAP, PVSUB};
-            for (int k = 0; k < 33; k++) rv[k] = vals[k];
-        }
-    } else if (inb) {
-        if (a.phase) a.phase[gidx(G, x, y, z)] = 0;
-    }
-    if (a.region) {
-        const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-        for (int k = 0; k < 33; k++) {
-            const double s = blockSum(inRegion ? (double)rv[k] : 0.0, shr);
-            if (tid256() == 0) a.region[(long)k * nb + bid] = s;
+        if (a.region) {
+            const unsigned nb = vb.n;
+            for (int k = 0; k < 33; k++) {
+                const double s = blockSum(inRegion ? (double)rv[k] : 0.0, shr);
+                if (tid256() == 0) a.region[(long)k * nb + bid] = s;
+            }
         }
     }
-}
+};
 hipError_t launch_microphysics(EnvState& e, float dt, float speed) {
     MicroArgs a{e.f[SKT_QV], e.f[SKT_QW], e.f[SKT_QC], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_POTTEMP],
                 e.f[SKT_DENSITY], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], e.phase,
                 e.region_active ? e.region : nullptr, {e.rmin[0], e.rmin[1], e.rmin[2]},
                 {e.rmax[0], e.rmax[1], e.rmax[2]}};
-    klaunch(e, k_microphysics, grid3(e), B3, 0, geom(e), a, dt, speed, e.gm);
-    return hipGetLastError();
+    return klaunch<KMicrophysics>(e, grid3(e), B3, 0, geom(e), a, dt, speed, e.gm);
 }
 // Fixed-order sum of the region partials into 33 floats at the tail of the
 // region buffer.
-KL void k_region_finalize(double* region, unsigned nb, float* out) {
-    __shared__ double sh[4];
-    for (int k = 0; k < 33; k++) {
-        double v = 0.0;
-        for (unsigned j = tid256(); j < nb; j += 256) v += region[(long)k * nb + j];
-        v = blockSum(v, sh);
-        if (tid256() == 0) out[k] = (float)v;
+struct KRegionFinalize {
+    static __device__ __forceinline__ void run(VB vb, double* region, unsigned nb, float* out) {
+        __shared__ double sh[4];
+        for (int k = 0; k < 33; k++) {
+            double v = 0.0;
+            for (unsigned j = tid256(); j < nb; j += 256) v += region[(long)k * nb + j];
+            v = blockSum(v, sh);
+            if (tid256() == 0) out[k] = (float)v;
+        }
     }
-}
+};
 hipError_t launch_region_finalize(EnvState& e) {
     const unsigned nb = (unsigned)blocks3d(e);
     float* out = (float*)(e.region + 33L * nb);
-    hipLaunchKernelGGL(k_region_finalize, dim3(1), dim3(64, 4, 1), 0, e.stream, e.region, nb, out);
-    return hipGetLastError();
+    return klaunch<KRegionFinalize>(e, dim3(1), dim3(64, 4, 1), 0, e.region, nb, out);
 }
 
 // ------------------------------------------------------ masks / editor ---
@@ -694,32 +722,34 @@ struct Ten {
     float* a[10];
 };
 // resetValueInGround over n fields (Game/kernelGround.cu:21-36)
-KL void k_reset_ground(Geom G, Ten t, int n) {
-    CELL_SETUP;
-    (void)bid;
-    if (!inb || !isGround(G, x, y, z)) return;
-    const long i = gidx(G, x, y, z);
-    for (int k = 0; k < n; k++) t.a[k][i] = 0.0f;
-}
+struct KResetGround {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const Ten& t, int n) {
+        CELL_SETUP_V;
+        (void)bid;
+        if (!inb || !isGround(G, x, y, z)) return;
+        const long i = gidx(G, x, y, z);
+        for (int k = 0; k < n; k++) t.a[k][i] = 0.0f;
+    }
+};
 hipError_t launch_editor_reset(EnvState& e, int nfields) {
     Ten t{{e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_QW], e.f[SKT_QC], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI],
            e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ]}};
-    hipLaunchKernelGGL(k_reset_ground, grid3(e), B3, 0, e.stream, geom(e), t, nfields);
-    return hipGetLastError();
+    return klaunch<KResetGround>(e, grid3(e), B3, 0, geom(e), t, nfields);
 }
 // computeIsenTempGroundGPU, Game/kernelGround.cu:38-47
-KL void k_isen_temp_ground(Geom G, float* gT, const float* isen, const float* gP, const float* p) {
-    const long g = blockIdx.x * 256L + threadIdx.x;
-    if (g >= (long)G.X * G.Z) return;
-    const int x = (int)(g % G.X), zc = (int)(g / G.X);
-    const int GH = G.gh[g] + 1 >= G.Y ? G.Y - 1 : G.gh[g] + 1;
-    const float T = d_potTemp(isen[GH - 1] - 273.15f, gP[g], p[gidx(G, x, GH, zc)]);
-    gT[g] = T + 273.15f;
-}
+struct KIsenTempGround {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, float* gT, const float* isen, const float* gP, const float* p) {
+        const long g = vb.id * 256L + threadIdx.x;
+        if (g >= (long)G.X * G.Z) return;
+        const int x = (int)(g % G.X), zc = (int)(g / G.X);
+        const int GH = G.gh[g] + 1 >= G.Y ? G.Y - 1 : G.gh[g] + 1;
+        const float T = d_potTemp(isen[GH - 1] - 273.15f, gP[g], p[gidx(G, x, GH, zc)]);
+        gT[g] = T + 273.15f;
+    }
+};
 hipError_t launch_isen_temp_ground(EnvState& e) {
-    hipLaunchKernelGGL(k_isen_temp_ground, grid1(e.G), dim3(256), 0, e.stream, geom(e), e.f[SKT_GROUND_T],
-                       e.f[SKT_ISEN_TEMP], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
-    return hipGetLastError();
+    return klaunch<KIsenTempGround>(e, grid1(e.G), dim3(256), 0, geom(e), e.f[SKT_GROUND_T], e.f[SKT_ISEN_TEMP],
+                                    e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
 }
 // Default profiles = the (x=0, z=0) column before the init noise
 // (cudaMemcpy2D of Game/environment.cu:299-303).
@@ -792,62 +822,64 @@ hipError_t launch_sounding_fill(EnvState& e, const float* lv, float groundT, flo
 // cells, the highest level whose QW + QC exceeds thr at a sky cell, and, in
 // the blocks whose tile holds level 0, the ground rain water QGR of its
 // columns (FP64). Each ground column lies in exactly one such block.
-KL void k_diag(Geom G, int zc, const float* __restrict__ v, const float* __restrict__ qw,
-               const float* __restrict__ qc, const float* __restrict__ qgr, float thr, double* pw, double* pl,
-               double* pr) {
-    ZM_SETUP;
-    __shared__ double sh[4];
-    float w = -__builtin_inff();
-    int lev = -1;
-    double rain = 0.0;
-    if (inb) {
-        for (int z = z0; z < z1; z++) {
-            const long i = gidx(G, x, y, z);
-            if (!isGround(G, x, y, z)) {
-                w = fmaxf(w, v[i]);
-                if (qw[i] + qc[i] > thr) lev = y;  // y is fixed per lane
+struct KDiag {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, const float* __restrict__ v, const float* __restrict__ qw,
+                   const float* __restrict__ qc, const float* __restrict__ qgr, float thr, double* pw, double* pl,
+                   double* pr) {
+        ZM_SETUP_V;
+        __shared__ double sh[4];
+        float w = -__builtin_inff();
+        int lev = -1;
+        double rain = 0.0;
+        if (inb) {
+            for (int z = z0; z < z1; z++) {
+                const long i = gidx(G, x, y, z);
+                if (!isGround(G, x, y, z)) {
+                    w = fmaxf(w, v[i]);
+                    if (qw[i] + qc[i] > thr) lev = y;  // y is fixed per lane
+                }
+                if (y == 0) rain += (double)qgr[x + (long)G.X * z];
             }
-            if (y == 0) rain += (double)qgr[x + (long)G.X * z];
+        }
+        const double m = blockMax((double)w, sh);
+        const double l = blockMax((double)lev, sh);
+        const double r = blockSum(rain, sh);
+        if (tid256() == 0) {
+            pw[bid] = m;
+            pl[bid] = l;
+            pr[bid] = r;
         }
     }
-    const double m = blockMax((double)w, sh);
-    const double l = blockMax((double)lev, sh);
-    const double r = blockSum(rain, sh);
-    if (tid256() == 0) {
-        pw[bid] = m;
-        pl[bid] = l;
-        pr[bid] = r;
-    }
-}
+};
 // One block: the final reductions in a fixed order.
-KL void k_diag_finalize(const double* __restrict__ pw, const double* __restrict__ pl, const double* __restrict__ pr,
-                        unsigned n, Scalars* sc) {
-    __shared__ double sh[4];
-    double m = -__builtin_inf(), l = -1.0, r = 0.0;
-    for (unsigned j = tid256(); j < n; j += 256) {
-        m = fmax(m, pw[j]);
-        l = fmax(l, pl[j]);
-        r += pr[j];
+struct KDiagFinalize {
+    static __device__ __forceinline__ void run(VB vb, const double* __restrict__ pw, const double* __restrict__ pl, const double* __restrict__ pr,
+                            unsigned n, Scalars* sc) {
+        __shared__ double sh[4];
+        double m = -__builtin_inf(), l = -1.0, r = 0.0;
+        for (unsigned j = tid256(); j < n; j += 256) {
+            m = fmax(m, pw[j]);
+            l = fmax(l, pl[j]);
+            r += pr[j];
+        }
+        m = blockMax(m, sh);
+        l = blockMax(l, sh);
+        r = blockSum(r, sh);
+        if (tid256() == 0) {
+            sc->dg_w = (float)m;
+            sc->dg_level = (int)l;
+            sc->dg_rain = r;
+        }
     }
-    m = blockMax(m, sh);
-    l = blockMax(l, sh);
-    r = blockSum(r, sh);
-    if (tid256() == 0) {
-        sc->dg_w = (float)m;
-        sc->dg_level = (int)l;
-        sc->dg_rain = r;
-    }
-}
+};
 hipError_t launch_diagnostics(EnvState& e, float thr) {
     const ZGrid zg = zgrid(e);
     const size_t M = e.maxBlocks;
     double *pw = e.partials + PART_DG_W * M, *pl = e.partials + PART_DG_L * M, *pr = e.partials + PART_DG_R * M;
-    hipLaunchKernelGGL(k_diag, zg.grid, B3, 0, e.stream, geom(e), zg.zc, e.f[SKT_VELY], e.f[SKT_QW], e.f[SKT_QC],
-                       e.f[SKT_QGR], thr, pw, pl, pr);
-    hipError_t err = hipGetLastError();
+    hipError_t err = klaunch<KDiag>(e, zg.grid, B3, 0, geom(e), zg.zc, e.f[SKT_VELY], e.f[SKT_QW], e.f[SKT_QC],
+                                    e.f[SKT_QGR], thr, pw, pl, pr);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_diag_finalize, dim3(1), B3, 0, e.stream, pw, pl, pr, zg.nblocks, e.sc);
-    return hipGetLastError();
+    return klaunch<KDiagFinalize>(e, dim3(1), B3, 0, pw, pl, pr, zg.nblocks, e.sc);
 }
 
 // ---------------------------------------------------------------- terrain ---

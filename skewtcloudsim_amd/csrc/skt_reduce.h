@@ -49,6 +49,17 @@ __device__ __forceinline__ double blockMax(double v, double* sh) {
     const bool inb = (x < G.X) && (y < G.Y);         \
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)
 
+// One thread per cell from a tile index: 64 (x) x 4 (y) tiles, x tiles
+// fastest, then y tiles, then z planes (the flattened three-dimensional grid
+// of the per-cell launches).
+#define CELL_SETUP_V                                                   \
+    const unsigned gx_ = (unsigned)(G.X + 63) >> 6, gy_ = (unsigned)(G.Y + 3) >> 2; \
+    const int x = (int)(vb.id % gx_) * 64 + (int)threadIdx.x;          \
+    const int y = (int)((vb.id / gx_) % gy_) * 4 + (int)threadIdx.y;   \
+    const int z = (int)(vb.id / (gx_ * gy_));                          \
+    const bool inb = (x < G.X) && (y < G.Y);                           \
+    const int bid = (int)vb.id
+
 // Workgroups are dispatched to the 8 XCDs round-robin by index. Remapping
 // gives each XCD a contiguous range of tiles, so the rows a tile reads from
 // its y neighbours were mostly fetched into the same XCD's L2.
@@ -61,6 +72,16 @@ __device__ __forceinline__ int xcdRemap(int b, int nb) {
 #define ZM_SETUP                                                   \
     const int gx_ = (G.X + 63) >> 6, gy_ = (G.Y + 3) >> 2;         \
     const int bid = xcdRemap(blockIdx.x, gridDim.x);               \
+    const int x = (bid % gx_) * 64 + threadIdx.x;                  \
+    const int y = ((bid / gx_) % gy_) * 4 + threadIdx.y;           \
+    const int z0 = (bid / (gx_ * gy_)) * zc;                       \
+    const int z1 = min(G.Z, z0 + zc);                              \
+    const bool inb = (x < G.X) && (y < G.Y)
+
+// The same from the launch's tile index.
+#define ZM_SETUP_V                                                 \
+    const int gx_ = (G.X + 63) >> 6, gy_ = (G.Y + 3) >> 2;         \
+    const int bid = xcdRemap((int)vb.id, (int)vb.n);               \
     const int x = (bid % gx_) * 64 + threadIdx.x;                  \
     const int y = ((bid / gx_) % gy_) * 4 + threadIdx.y;           \
     const int z0 = (bid / (gx_ * gy_)) * zc;                       \

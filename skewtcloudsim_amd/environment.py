@@ -399,3 +399,67 @@ class EnvironmentGPU:
 
     def synchronize(self):
         self._chk(self.lib.skt_env_synchronize(self.h), "synchronize")
+
+
+class EnsembleGPU:
+    """An ensemble of EnvironmentGPU members advanced by one launch per phase
+    (skt_batch_*). The members are borrowed: keep them alive while they are
+    in the ensemble. Each member's results equal its own updateGPU's."""
+
+    def __init__(self, device=0, use_graph=True, advection_substep_cap=4, launch_mode=0):
+        self.lib = _lib.load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.skt_batch_create(device, ctypes.byref(h))
+        if rc:
+            raise SktError(f"skt_batch_create failed ({rc})")
+        self.h = h
+        self.members = {}
+        self.set_options(use_graph=use_graph, advection_substep_cap=advection_substep_cap, launch_mode=launch_mode)
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = self.lib.skt_batch_last_error(self.h)
+            raise SktError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    def set_options(self, use_graph=True, advection_substep_cap=4, launch_mode=0):
+        o = _lib.BatchOptions(int(bool(use_graph)), int(advection_substep_cap), int(launch_mode))
+        self._chk(self.lib.skt_batch_set_options(self.h, ctypes.byref(o)), "set_options")
+
+    def add(self, env):
+        mid = ctypes.c_int32()
+        self._chk(self.lib.skt_batch_add(self.h, env.h, ctypes.byref(mid)), "add")
+        self.members[mid.value] = env
+        return mid.value
+
+    def remove(self, member_id):
+        self._chk(self.lib.skt_batch_remove(self.h, int(member_id)), "remove")
+        self.members.pop(member_id, None)
+
+    def __len__(self):
+        n = ctypes.c_int32()
+        self._chk(self.lib.skt_batch_size(self.h, ctypes.byref(n)), "size")
+        return n.value
+
+    def step(self, dt, speed=1.0):
+        self._chk(self.lib.skt_batch_step(self.h, dt, speed), "step")
+
+    def synchronize(self):
+        self._chk(self.lib.skt_batch_synchronize(self.h), "synchronize")
+
+    def autotune(self, dt, speed=1.0):
+        t = _lib.BatchTuning()
+        self._chk(self.lib.skt_batch_autotune(self.h, dt, speed, ctypes.byref(t)), "autotune")
+        return {"step_ms_by_width": list(t.step_ms), "tuned_step_ms": t.tuned_step_ms, "bodies": t.bodies,
+                "persistent_bodies": t.persistent_bodies}
+
+    def close(self):
+        if self.h:
+            self.lib.skt_batch_destroy(self.h)
+            self.h = None
+            self.members = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -840,3 +840,101 @@ def test_reset_parameter(param):
         assert nd == 0, (param, f, nd)
     env.close()
     orc.close()
+
+
+CFG0_BOUNDS = {"POTTEMP": 1e-5, "PRESSURE": 1e-5, "QV": 1e-5, "VELX": 5e-3, "VELZ": 5e-3, "VELY": 5e-2}
+
+
+def test_single_bubble_64cubed_200_steps_against_fp32_and_fp64_oracles():
+    """The plain method at 64x64x64 (dx 100 m, one 2 K bubble of 500 m at
+    1 km in the fixed 300 K / 14 g/kg sounding, the reference's init noise),
+    200 steps of dt 0.5 s with the pressure solve to a relative residual of
+    1e-6 (Jacobi-PCG on the symmetric operator). At every step the device,
+    the FP32 oracle and the FP64 oracle run from the same initial state:
+      * PCG iteration counts equal the FP32 oracle's, except where the
+        relative stop test sits within the two paths' round-off of 1e-6
+        (then they differ by one; the FP64 oracle then agrees with one or
+        the other): at most 6 such steps;
+      * every field stays within relative L2 CFG0_BOUNDS of the FP32 oracle
+        (measured after 200 steps: theta, p, qv 1.4e-6; u, w 7.3e-4;
+        v 8.5e-3; the condensates stay 0 in these 100 s);
+      * after 200 steps every field's distance to the FP64 oracle is at most
+        twice the FP32 oracle's own, plus 1e-6."""
+    import skewtcloudsim_amd as skt
+    n = 64
+    snd = skt.make_sounding(n, n, n, 100.0, seed=20261203, n_bubbles=0)
+    args = [snd[k] for k in KEYS]
+    cfg = dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=1000)
+    env = skt.EnvironmentGPU(n, n, n, 100.0, **cfg)
+    env.init(*args)
+    o32 = ob.Oracle(n, n, n, 100.0, **cfg)
+    o32.init(*args)
+    o64 = ob.Oracle(n, n, n, 100.0, fp64=True, **cfg)
+    o64.init(*args)
+    m = sky_mask(n, n, n)
+    off_by_one = 0
+    for s in range(200):
+        env.updateGPU(0.5, 1.0)
+        o32.step(0.5, 1.0)
+        o64.step(0.5, 1.0)
+        ig, ic = env.stats().pcg_iterations, o32.stats()["pcg_iterations"]
+        assert abs(ig - ic) <= 1, (s, ig, ic)
+        off_by_one += ig != ic
+        if s % 20 == 19:
+            curve = []
+            for f, b in CFG0_BOUNDS.items():
+                r = rel_l2(env.get(f)[m], o32.get(f)[m])
+                curve.append(f"{f} {r:.1e}")
+                assert r <= b, (s, f, r)
+            print(f"step {s + 1}: its {ig}; gpu vs fp32 oracle " + ", ".join(curve), flush=True)
+    assert off_by_one <= 6, off_by_one
+    for f in ("POTTEMP", "PRESSURE", "QV", "VELX", "VELY", "VELZ", "QW", "QC", "QR", "QS", "QI"):
+        g, c, d = env.get(f)[m], o32.get(f)[m], o64.get64(f)[m]
+        assert np.isfinite(g).all(), f
+        e_gpu, e_cpu = rel_l2(g, d), rel_l2(c, d)
+        print(f"{f}: gpu-fp64 {e_gpu:.1e} fp32-fp64 {e_cpu:.1e}", flush=True)
+        assert e_gpu <= 2.0 * e_cpu + 1e-6, (f, e_gpu, e_cpu)
+    env.close()
+    o32.close()
+    o64.close()
+
+
+RICH_BOUNDS = {"QW": 2e-3, "QC": 2e-3}
+
+
+def test_full_size_rich_step_against_oracle():
+    """One whole step of the 256x128x256 workload from a state with cloud
+    water, cloud ice, rain, snow, graupel and a sub-freezing upper half
+    (rich_state): every branch of the microphysics runs at full size. Equal
+    PCG iteration counts; relative L2 of every field within 1e-5 of the
+    oracle except cloud water and cloud ice (2e-3; measured 5.1e-4 and
+    3.1e-4): their differences sit in the few cells where a threshold of
+    the microphysics (autoconversion at QwMin, the phase selectors, wet or
+    dry growth) compares a value that the earlier phases left one float
+    rounding apart on the two paths, so the two sides take different
+    branches there."""
+    nx, ny, nz = 256, 128, 256
+    print("full-size rich step: set-up", flush=True)
+    env, orc = make_pair(nx, ny, nz, dx=100.0, seed=20261203, n_bubbles=3)
+    m = sky_mask(nx, ny, nz)
+    rich_state(env, orc, seed=91)
+    print("full-size rich step: device", flush=True)
+    env.updateGPU(0.5, 1.0)
+    print("full-size rich step: oracle", flush=True)
+    orc.step(0.5, 1.0)
+    assert env.stats().pcg_iterations == orc.stats()["pcg_iterations"]
+    for f in ("QW", "QC", "QR", "QS", "QI"):
+        assert np.linalg.norm(orc.get(f)[m]) > 0, f
+    rep = []
+    for f in SKY + GROUND:
+        a, b = env.get(f), orc.get(f)
+        if a.size == m.size:
+            a, b = a[m], b[m]
+        assert np.isfinite(a).all(), f
+        r = rel_l2(a, b)
+        far = int((np.abs(a.astype(np.float64) - b) > 1e-4 * max(np.abs(b).max(), 1e-30)).sum())
+        rep.append(f"{f} {r:.1e} ({far} cells beyond 1e-4 of scale)")
+        assert r <= RICH_BOUNDS.get(f, 1e-5), (f, r)
+    print("; ".join(rep), flush=True)
+    env.close()
+    orc.close()
