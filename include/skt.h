@@ -58,7 +58,25 @@ typedef struct skt_config {
     int32_t init_noise;        /* 1: add the reference's init perturbations */
     int32_t max_blocks;        /* upper bound on the blocks of one solver launch; 0 sizes launches from the
                                   device. A small bound gives each block more z planes (or rows). */
+    /* Mixed-precision form. The five hydrometeor fields (QW, QC, QR, QS, QI;
+     * Game/environment.cuh:13-25 holds them as float) may be stored in 16
+     * bits: value * 2^tracer_scale_log2[k], rounded to nearest even. Every
+     * kernel computes in FP32 and rounds once when it stores. */
+    int32_t tracer_format;         /* SKT_FMT_F32 (default), SKT_FMT_BF16 or SKT_FMT_F16 */
+    int32_t tracer_scale_log2[5];  /* per field, QW..QI; 16 by default (1e-6..2e-2 kg/kg lands in
+                                      [2^-4, 2^11], inside FP16's normal range; BF16 needs no scale) */
+    /* FP64 residual refinement of the pressure solve (standard solver only):
+     * after the first solve, each pass forms r = b - A x in FP64 from the
+     * FP64 sum x of the solves so far and solves A d = r in FP32. Every solve
+     * stops at max|r| <= max(pcg_refine_tol * its own max|r_start|,
+     * pcg_tol * max|b|); a pass whose residual already meets the second bound
+     * runs no iteration. */
+    int32_t pcg_refine;            /* refinement passes after the first solve (0: none) */
+    float pcg_refine_tol;          /* relative tolerance of each solve when refining (1e-3) */
 } skt_config;
+
+/* Storage formats of skt_config.tracer_format and skt_env_get_field_format. */
+enum { SKT_FMT_F32 = 0, SKT_FMT_BF16 = 1, SKT_FMT_F16 = 2 };
 
 /* Fills a configuration with the reference's settings for an nx*ny*nz grid. */
 void skt_config_default(skt_config* cfg, int32_t nx, int32_t ny, int32_t nz, float dx);
@@ -110,6 +128,11 @@ typedef struct skt_step_stats {
     float pcg_residual0;         /* stop-test value of the initial divergence */
     int32_t n_advect;            /* advectPPMWGPU calls in the last update */
     int32_t substeps[16];        /* substep count of each of those calls */
+    /* with FP64 refinement (skt_config.pcg_refine > 0); otherwise 1, pcg_iterations, 0, 0 */
+    int32_t pcg_passes;          /* solves that ran iterations in the last projection */
+    int32_t pcg_iterations_total;/* iterations over all its solves */
+    double pcg_residual64;       /* max |b - A x| in FP64 for the final x */
+    double pcg_rhs64;            /* max |b| */
 } skt_step_stats;
 
 /* environmentGPU::environmentGPU (Game/environment.cu:71-185). */
@@ -172,6 +195,11 @@ int skt_env_copy_field_from_host(skt_env* env, int32_t field, const float* host)
 int skt_env_set_params(skt_env* env, const skt_env_params* params);
 int skt_env_get_params(skt_env* env, skt_env_params* params);
 int skt_env_get_stats(skt_env* env, skt_step_stats* stats);
+/* Storage of a field: SKT_FMT_F32 for all fields but the hydrometeors of the
+ * mixed form, which hold value * scale in 16 bits (skt_env_get_field_ptr then
+ * points at uint16 words). The host copies convert. No counterpart in the
+ * reference, whose fields are all float. */
+int skt_env_get_field_format(skt_env* env, int32_t field, int32_t* format, float* scale);
 
 /* Boundary masks in the layout of computeNeighbourGPU's Neigh struct
  * (Game/kernelSky.cu:1284-1316): 12 bytes per cell. host_out[12*N]. */

@@ -32,7 +32,8 @@ class Config(ctypes.Structure):
         ("device", ctypes.c_int32), ("pcg_mode", ctypes.c_int32), ("pcg_tol", ctypes.c_float),
         ("pcg_max_iter", ctypes.c_int32), ("diffusion_sweeps", ctypes.c_int32), ("max_substeps", ctypes.c_int32),
         ("maxabs_mode", ctypes.c_int32), ("noise_seed", ctypes.c_uint32), ("init_noise", ctypes.c_int32),
-        ("max_blocks", ctypes.c_int32),
+        ("max_blocks", ctypes.c_int32), ("tracer_format", ctypes.c_int32),
+        ("tracer_scale_log2", ctypes.c_int32 * 5), ("pcg_refine", ctypes.c_int32), ("pcg_refine_tol", ctypes.c_float),
     ]
 
 
@@ -50,7 +51,8 @@ class Params(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [
         ("pcg_iterations", ctypes.c_int32), ("pcg_residual", ctypes.c_float), ("pcg_residual0", ctypes.c_float),
-        ("n_advect", ctypes.c_int32), ("substeps", ctypes.c_int32 * 16),
+        ("n_advect", ctypes.c_int32), ("substeps", ctypes.c_int32 * 16), ("pcg_passes", ctypes.c_int32),
+        ("pcg_iterations_total", ctypes.c_int32), ("pcg_residual64", ctypes.c_double), ("pcg_rhs64", ctypes.c_double),
     ]
 
 
@@ -135,6 +137,7 @@ SIGNATURES = {
     "skt_env_get_field_ptr": (_I, [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64)]),
     "skt_env_copy_field_to_host": (_I, [_P, _I, _P]),
     "skt_env_copy_field_from_host": (_I, [_P, _I, _P]),
+    "skt_env_get_field_format": (_I, [_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_F)]),
     "skt_env_set_params": (_I, [_P, ctypes.POINTER(Params)]),
     "skt_env_get_params": (_I, [_P, ctypes.POINTER(Params)]),
     "skt_env_get_stats": (_I, [_P, ctypes.POINTER(Stats)]),

@@ -97,6 +97,8 @@ long fieldLen(const EnvState& e, int f) {
     return e.Y;
 }
 bool validField(int f) { return f >= 0 && f < SKT_FIELD_COUNT; }
+bool is16(const EnvState& e, int f) { return e.fst[f].fmt != FMT_F32; }
+size_t fieldBytes(const EnvState& e, int f) { return (is16(e, f) ? 2 : 4) * (size_t)fieldLen(e, f); }
 
 }  // namespace
 
@@ -134,7 +136,11 @@ void diffusionBounds(EnvState& e, int type, const float** prof, Bounds* b) {
 
 // Diffusion of n independent fields (each reads only itself, the ground
 // temperature and pressure), swept together: field j ping-pongs with work
-// buffer j. Identical to diffusing them one after the other.
+// buffer j. Identical to diffusing them one after the other. A field stored
+// in 16 bits is read by the first sweep and written by the last; the sweeps
+// in between run in FP32 between work buffers j and j + SKT_BATCH, so the
+// field is rounded to its storage once per diffusion, not once per sweep.
+static_assert(SKT_NWORK >= 2 * SKT_BATCH, "two float buffers per diffused field");
 int diffuseFields(EnvState& e, const int* flds, const int* types, int n, float dt) {
     const float k = 0.005f * dt / (e.dx * e.dx);
     const int sweeps = e.cfg.diffusion_sweeps;
@@ -144,16 +150,33 @@ int diffuseFields(EnvState& e, const int* flds, const int* types, int n, float d
         diffusionBounds(e, types[j], &db.f[j].prof, &db.f[j].b);
         db.f[j].type = types[j];
     }
+    const Store f32 = storeF32();
     for (int s = 0; s < sweeps; s++) {
         for (int j = 0; j < n; j++) {
-            float* bufs[2] = {e.f[flds[j]], e.work[j]};
-            db.f[j].in = bufs[s & 1];
-            db.f[j].out = bufs[(s + 1) & 1];
+            DiffField& d = db.f[j];
+            float* fld = e.f[flds[j]];
+            const Store sf = e.fst[flds[j]];
+            if (sf.fmt == FMT_F32) {
+                float* bufs[2] = {fld, e.work[j]};
+                d.in = bufs[s & 1];
+                d.out = bufs[(s + 1) & 1];
+                d.sin = d.sout = f32;
+            } else {
+                float *A = e.work[j], *B = e.work[j + SKT_BATCH];
+                d.in = s == 0 ? fld : ((s & 1) ? A : B);
+                d.sin = s == 0 ? sf : f32;
+                const bool toField = sweeps > 1 && s == sweeps - 1;
+                d.out = toField ? fld : ((s & 1) ? B : A);
+                d.sout = toField ? sf : f32;
+            }
         }
         TL(e, TC_DIFFUSE, launch_diffuse_batch(e, db, k));
     }
-    if (sweeps & 1)
-        for (int j = 0; j < n; j++) TL(e, TC_DIFFUSE, launch_copy_sky(e, e.work[j], e.f[flds[j]]));
+    for (int j = 0; j < n; j++) {
+        const Store sf = e.fst[flds[j]];
+        if (sf.fmt == FMT_F32 ? (sweeps & 1) : sweeps == 1)
+            TL(e, TC_DIFFUSE, launch_copy_sky(e, e.work[j], e.f[flds[j]], f32, sf));
+    }
     return SKT_OK;
 }
 int diffuse(EnvState& e, int fld, int type, float dt) { return diffuseFields(e, &fld, &type, 1, dt); }
@@ -215,19 +238,21 @@ int advectFields(EnvState& e, const AdvField* fs, int n, float dt) {
     const float h = dts * 0.5f;
     float *U = e.f[SKT_VELX], *V = e.f[SKT_VELY], *W = e.f[SKT_VELZ];
     PpmBatch pb[5]{};
+    const Store f32 = storeF32();
     for (int j = 0; j < n; j++) {
         const int fld = fs[j].fld;
         float* A = e.f[fld];
+        const Store sa = e.fst[fld];  // the work buffers are float
         float *T1 = e.work[3 * j], *T2 = e.work[3 * j + 1], *T3 = e.work[3 * j + 2];
         // At ground positions each pass sees what the reference's buffers hold
         // there: the field's own ground values in passes 1, 3 and 5 (they read
         // advectArray, whose ground cells no pass writes) and 0 in passes 2 and
         // 4 (they read m_outputArray, written only at sky cells).
-        const PpmField f0{A, T1, fld == SKT_VELX ? A : U, fs[j].prof, fs[j].b, A};
-        const PpmField f1{T1, T2, fld == SKT_VELY ? A : V, fs[j].prof, fs[j].b, nullptr};
-        const PpmField f2{T2, T1, fld == SKT_VELZ ? T2 : W, fs[j].prof, fs[j].b, A};
-        const PpmField f3{T1, T3, fld == SKT_VELY ? T2 : V, fs[j].prof, fs[j].b, nullptr};
-        const PpmField f4{T3, A, fld == SKT_VELX ? T3 : U, fs[j].prof, fs[j].b, A};
+        const PpmField f0{A, T1, fld == SKT_VELX ? A : U, fs[j].prof, fs[j].b, A, sa, f32, sa};
+        const PpmField f1{T1, T2, fld == SKT_VELY ? A : V, fs[j].prof, fs[j].b, nullptr, f32, f32, f32};
+        const PpmField f2{T2, T1, fld == SKT_VELZ ? T2 : W, fs[j].prof, fs[j].b, A, f32, f32, sa};
+        const PpmField f3{T1, T3, fld == SKT_VELY ? T2 : V, fs[j].prof, fs[j].b, nullptr, f32, f32, f32};
+        const PpmField f4{T3, A, fld == SKT_VELX ? T3 : U, fs[j].prof, fs[j].b, A, f32, sa, sa};
         pb[0].f[j] = f0;
         pb[1].f[j] = f1;
         pb[2].f[j] = f2;
@@ -272,6 +297,7 @@ int precipFields(EnvState& e, const int* flds, const int* types, int n, float dt
         pbt.n = n;
         for (int j = 0; j < n; j++) {
             pbt.q[j] = e.f[flds[j]];
+            pbt.st[j] = e.fst[flds[j]];
             pbt.type[j] = types[j];
             pbt.dts[j] = s < sub[j] ? dt / sub[j] : 0.0f;
         }
@@ -289,9 +315,16 @@ int precip(EnvState& e, int fld, int type, float dt) { return precipFields(e, &f
 // host decides whether to enqueue the next. A solve that converges in chunk
 // c stops after chunk c + 1 has been queued (its kernels exit at once). The
 // final state is copied the same way and read by skt_env_get_stats.
-int solvePressure(EnvState& e, float dt) {
-    TL(e, TC_OTHER, launch_divergence(e, dt));
-    TL(e, TC_OTHER, launch_pcg_start(e));
+//
+// With FP64 refinement (pcg_refine passes) the divergence b is kept, the
+// first solve's p' starts the FP64 sum x64, and each pass solves for the
+// correction from r = b - A x64 formed in FP64; p' ends as float(x64), and the
+// final FP64 residual is reduced for the statistics. Every pass is queued
+// whole (a pass whose residual already meets the shared stop value runs no
+// iteration), so the device holds the control flow and a recorded step
+// replays it.
+int solveOnce(EnvState& e, int pass, int* slot_out) {
+    TL(e, TC_OTHER, launch_pcg_start(e, pass));
     const int maxIt = e.cfg.pcg_max_iter;
     const int chunk = 25;
     int it = 0, c = 0;
@@ -320,8 +353,31 @@ int solvePressure(EnvState& e, float dt) {
         TL(e, TC_OTHER, launch_pcg_final(e, it));
         slot = 0;
     }
+    *slot_out = slot;
+    return SKT_OK;
+}
+int solvePressure(EnvState& e, float dt) {
+    TL(e, TC_OTHER, launch_divergence(e, dt));
+    const int R = e.cfg.pcg_refine;
+    if (R > 0) TL(e, TC_OTHER, launch_refine_keep_rhs(e));
+    int slot = 1;
+    for (int pass = 0; pass <= R; pass++) {
+        if (pass > 0) {
+            TL(e, TC_OTHER, launch_refine_accum(e, pass == 1, 0));
+            TL(e, TC_OTHER, launch_refine_residual(e, 1));
+        }
+        const int rc = solveOnce(e, R > 0 ? pass : -1, &slot);
+        if (rc) return rc;
+    }
+    if (R > 0) {
+        TL(e, TC_OTHER, launch_refine_accum(e, 0, 1));
+        TL(e, TC_OTHER, launch_refine_residual(e, 0));
+        TL(e, TC_OTHER, launch_refine_report(e));
+    }
     if (e.rec) return SKT_OK;  // the ensemble reads the final state from the device
     HIPCHK(e, hipMemcpyAsync(&e.pcg_snap[2], &e.sc->st[slot], sizeof(PcgState), hipMemcpyDeviceToHost, e.stream));
+    if (R > 0)
+        HIPCHK(e, hipMemcpyAsync(e.ref_snap, &e.sc->ref, sizeof(RefineStats), hipMemcpyDeviceToHost, e.stream));
     HIPCHK(e, hipEventRecord(e.snap_ev[2], e.stream));
     e.pcg_stats_pending = true;
     return SKT_OK;
@@ -458,6 +514,8 @@ void freeAll(EnvState& e) {
     fr(e.ghz);
     fr(e.ghEdit);
     fr(e.irho);
+    fr(e.rhs);
+    fr(e.x64);
     fr(e.zeros);
     fr(e.ones);
     fr(e.partials);
@@ -473,6 +531,7 @@ void freeAll(EnvState& e) {
     if (e.mv_snap) (void)hipHostFree(e.mv_snap);
     if (e.mv_ev) (void)hipEventDestroy(e.mv_ev);
     if (e.pcg_snap) (void)hipHostFree(e.pcg_snap);
+    if (e.ref_snap) (void)hipHostFree(e.ref_snap);
     for (int k = 0; k < 3; k++)
         if (e.snap_ev[k]) (void)hipEventDestroy(e.snap_ev[k]);
     if (e.own_stream) (void)hipStreamDestroy(e.own_stream);
@@ -506,6 +565,10 @@ void skt_config_default(skt_config* c, int32_t nx, int32_t ny, int32_t nz, float
     c->noise_seed = 100u;
     c->init_noise = 1;
     c->max_blocks = 0;
+    c->tracer_format = SKT_FMT_F32;
+    for (int k = 0; k < 5; k++) c->tracer_scale_log2[k] = 16;
+    c->pcg_refine = 0;
+    c->pcg_refine_tol = 1e-3f;
 }
 
 int skt_env_create(const skt_config* cfg, skt_env** out) {
@@ -515,6 +578,15 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     const long N = (long)cfg->nx * cfg->ny * cfg->nz;
     if (N > (1L << 31) - 1 || cfg->diffusion_sweeps < 0 || cfg->pcg_max_iter < 0 || cfg->max_substeps < 0 ||
         cfg->max_blocks < 0)
+        return SKT_ERR_ARG;
+    if (cfg->tracer_format < SKT_FMT_F32 || cfg->tracer_format > SKT_FMT_F16) return SKT_ERR_ARG;
+    for (int k = 0; k < 5; k++)
+        if (cfg->tracer_scale_log2[k] < -60 || cfg->tracer_scale_log2[k] > 60) return SKT_ERR_ARG;
+    // refinement needs the symmetric operator, a solve to refine and a
+    // positive relative tolerance
+    if (cfg->pcg_refine < 0 || cfg->pcg_refine > 8 ||
+        (cfg->pcg_refine > 0 && (cfg->pcg_mode != SKT_PCG_STANDARD || cfg->pcg_max_iter < 1 ||
+                                 !(cfg->pcg_refine_tol > 0.0f && cfg->pcg_refine_tol < 1.0f))))
         return SKT_ERR_ARG;
     skt_env* env = new skt_env();
     EnvState& e = env->s;
@@ -526,6 +598,14 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     e.G = (long)cfg->nx * cfg->nz;
     e.dx = cfg->dx;
     e.device = cfg->device;
+    for (int k = 0; k < SKT_FIELD_COUNT; k++) e.fst[k] = storeF32();
+    if (cfg->tracer_format != SKT_FMT_F32) {
+        for (int k = 0; k < 5; k++) {
+            const float sc = std::ldexp(1.0f, cfg->tracer_scale_log2[k]);
+            e.fst[SKT_QW + k] = Store{cfg->tracer_format == SKT_FMT_BF16 ? FMT_BF16 : FMT_F16, sc, 1.0f / sc};
+        }
+        e.mix = true;
+    }
     auto bail = [&](int rc) {
         freeAll(e);
         delete env;
@@ -535,13 +615,16 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     if (hipStreamCreateWithFlags(&e.own_stream, hipStreamNonBlocking) != hipSuccess) return bail(SKT_ERR_HIP);
     e.stream = e.own_stream;
     for (int k = 0; k < SKT_FIELD_COUNT; k++)
-        if (dalloc(e, &e.f[k], sizeof(float) * fieldLen(e, k))) return bail(SKT_ERR_NOMEM);
+        if (dalloc(e, &e.f[k], fieldBytes(e, k))) return bail(SKT_ERR_NOMEM);
     for (int k = 0; k < SKT_NWORK; k++)
         if (dalloc(e, &e.work[k], sizeof(float) * std::max(e.N, 2 * e.G)) ||
             hipMemsetAsync(e.work[k], 0, sizeof(float) * std::max(e.N, 2 * e.G), e.stream) != hipSuccess)
             return bail(SKT_ERR_NOMEM);
     for (int k = 0; k < 6; k++)
         if (dalloc(e, &e.pcg[k], sizeof(float) * e.N)) return bail(SKT_ERR_NOMEM);
+    if (cfg->pcg_refine > 0 &&
+        (dalloc(e, &e.rhs, sizeof(float) * e.N) || dalloc(e, &e.x64, sizeof(double) * e.N)))
+        return bail(SKT_ERR_NOMEM);
     if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.ghm1, sizeof(int) * e.G) ||
         dalloc(e, &e.ghm2, sizeof(int) * e.G) || dalloc(e, &e.ghz, sizeof(int) * e.G) ||
         dalloc(e, &e.irho, sizeof(float) * e.N) ||
@@ -556,7 +639,8 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
         dalloc(e, &e.sc, sizeof(Scalars)))
         return bail(SKT_ERR_NOMEM);
     if (hipHostMalloc((void**)&e.sc_host, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) return bail(SKT_ERR_NOMEM);
-    if (hipHostMalloc((void**)&e.pcg_snap, 3 * sizeof(PcgState), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&e.pcg_snap, 3 * sizeof(PcgState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&e.ref_snap, sizeof(RefineStats), hipHostMallocDefault) != hipSuccess)
         return bail(SKT_ERR_NOMEM);
     for (int k = 0; k < 3; k++)
         if (hipEventCreateWithFlags(&e.snap_ev[k], hipEventDisableTiming) != hipSuccess) return bail(SKT_ERR_HIP);
@@ -565,7 +649,7 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     if (hipEventCreateWithFlags(&e.mv_ev, hipEventDisableTiming) != hipSuccess) return bail(SKT_ERR_HIP);
     // environmentGPU ctor defaults (Game/environment.cu:151-168)
     for (int k = 0; k < SKT_FIELD_COUNT; k++)
-        if (hipMemsetAsync(e.f[k], 0, sizeof(float) * fieldLen(e, k), e.stream) != hipSuccess) return bail(SKT_ERR_HIP);
+        if (hipMemsetAsync(e.f[k], 0, fieldBytes(e, k), e.stream) != hipSuccess) return bail(SKT_ERR_HIP);
     if (hipMemsetAsync(e.gh, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
         hipMemsetAsync(e.ghm1, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
         hipMemsetAsync(e.ghm2, 0, sizeof(int) * e.G, e.stream) != hipSuccess ||
@@ -855,18 +939,43 @@ int skt_env_get_field_ptr(skt_env* env, int32_t field, void** dev_ptr, int64_t* 
     if (count) *count = fieldLen(env->s, field);
     return SKT_OK;
 }
+// A 16-bit field crosses as its words and is converted on the host with the
+// device's conversions (fromH16 / toH16).
 int skt_env_copy_field_to_host(skt_env* env, int32_t field, float* host) {
     if (!env || !validField(field) || !host) return SKT_ERR_ARG;
     EnvState& e = env->s;
-    HIPCHK(e, hipMemcpyAsync(host, e.f[field], sizeof(float) * fieldLen(e, field), hipMemcpyDeviceToHost, e.stream));
+    const long n = fieldLen(e, field);
+    if (is16(e, field)) {
+        std::vector<unsigned short> h((size_t)n);
+        HIPCHK(e, hipMemcpyAsync(h.data(), e.f[field], fieldBytes(e, field), hipMemcpyDeviceToHost, e.stream));
+        HIPCHK(e, hipStreamSynchronize(e.stream));
+        for (long i = 0; i < n; i++) host[i] = fromH16(h[i], e.fst[field]);
+        return SKT_OK;
+    }
+    HIPCHK(e, hipMemcpyAsync(host, e.f[field], sizeof(float) * n, hipMemcpyDeviceToHost, e.stream));
     HIPCHK(e, hipStreamSynchronize(e.stream));
     return SKT_OK;
 }
 int skt_env_copy_field_from_host(skt_env* env, int32_t field, const float* host) {
     if (!env || !validField(field) || !host) return SKT_ERR_ARG;
     EnvState& e = env->s;
-    HIPCHK(e, hipMemcpyAsync(e.f[field], host, sizeof(float) * fieldLen(e, field), hipMemcpyHostToDevice, e.stream));
+    const long n = fieldLen(e, field);
+    if (is16(e, field)) {
+        std::vector<unsigned short> h((size_t)n);
+        for (long i = 0; i < n; i++) h[i] = toH16(host[i], e.fst[field]);
+        HIPCHK(e, hipMemcpyAsync(e.f[field], h.data(), fieldBytes(e, field), hipMemcpyHostToDevice, e.stream));
+        HIPCHK(e, hipStreamSynchronize(e.stream));
+        return SKT_OK;
+    }
+    HIPCHK(e, hipMemcpyAsync(e.f[field], host, sizeof(float) * n, hipMemcpyHostToDevice, e.stream));
     HIPCHK(e, hipStreamSynchronize(e.stream));
+    return SKT_OK;
+}
+int skt_env_get_field_format(skt_env* env, int32_t field, int32_t* format, float* scale) {
+    if (!env || !validField(field)) return SKT_ERR_ARG;
+    const Store& st = env->s.fst[field];
+    if (format) *format = st.fmt == FMT_F32 ? SKT_FMT_F32 : (st.fmt == FMT_BF16 ? SKT_FMT_BF16 : SKT_FMT_F16);
+    if (scale) *scale = st.fmt == FMT_F32 ? 1.0f : st.scale;
     return SKT_OK;
 }
 int skt_env_set_params(skt_env* env, const skt_env_params* p) {
@@ -879,6 +988,19 @@ int skt_env_get_params(skt_env* env, skt_env_params* p) {
     *p = env->s.params;
     return SKT_OK;
 }
+static void setRefineStats(EnvState& e, const PcgState& last) {
+    if (e.cfg.pcg_refine > 0) {
+        e.stats.pcg_passes = e.ref_snap->passes;
+        e.stats.pcg_iterations_total = e.ref_snap->iters_total;
+        e.stats.pcg_residual64 = e.ref_snap->res64;
+        e.stats.pcg_rhs64 = e.ref_snap->rhs64;
+    } else {
+        e.stats.pcg_passes = 1;
+        e.stats.pcg_iterations_total = last.iters;
+        e.stats.pcg_residual64 = 0.0;
+        e.stats.pcg_rhs64 = 0.0;
+    }
+}
 int skt_env_get_stats(skt_env* env, skt_step_stats* st) {
     if (!env || !st) return SKT_ERR_ARG;
     EnvState& e = env->s;
@@ -889,10 +1011,13 @@ int skt_env_get_stats(skt_env* env, skt_step_stats* st) {
         MaxvelSnap mv{};
         HIPCHK(e, hipMemcpyAsync(&ps, &e.sc->st[0], sizeof(PcgState), hipMemcpyDeviceToHost, e.stream));
         HIPCHK(e, hipMemcpyAsync(&mv, &e.sc->maxvel, sizeof(MaxvelSnap), hipMemcpyDeviceToHost, e.stream));
+        if (e.cfg.pcg_refine > 0)
+            HIPCHK(e, hipMemcpyAsync(e.ref_snap, &e.sc->ref, sizeof(RefineStats), hipMemcpyDeviceToHost, e.stream));
         HIPCHK(e, hipStreamSynchronize(e.stream));
         e.stats.pcg_iterations = ps.iters;
         e.stats.pcg_residual = ps.maxr;
         e.stats.pcg_residual0 = ps.maxr0;
+        setRefineStats(e, ps);
         for (int k = 0; k < std::min(e.stats.n_advect, 16); k++) e.stats.substeps[k] = mv.nsub;
         e.pcg_stats_pending = false;
         e.stats_on_device = false;
@@ -902,6 +1027,7 @@ int skt_env_get_stats(skt_env* env, skt_step_stats* st) {
         e.stats.pcg_iterations = e.pcg_snap[2].iters;
         e.stats.pcg_residual = e.pcg_snap[2].maxr;
         e.stats.pcg_residual0 = e.pcg_snap[2].maxr0;
+        setRefineStats(e, e.pcg_snap[2]);
         e.pcg_stats_pending = false;
     }
     *st = e.stats;
@@ -988,19 +1114,19 @@ int skt_env_reset_parameter(skt_env* env, int32_t param) {
     EnvState& e = env->s;
     HIPCHK(e, hipSetDevice(e.device));
     switch (param) {
-        case SKT_PARAM_POTTEMP: HIPCHK(e, launch_reset_param(e, e.f[SKT_POTTEMP], e.f[SKT_ISEN_TEMP])); break;
-        case SKT_PARAM_QV: HIPCHK(e, launch_reset_param(e, e.f[SKT_QV], e.f[SKT_ISEN_VAPOR])); break;
-        case SKT_PARAM_QW: HIPCHK(e, launch_reset_param(e, e.f[SKT_QW], nullptr)); break;
-        case SKT_PARAM_QC: HIPCHK(e, launch_reset_param(e, e.f[SKT_QC], nullptr)); break;
-        case SKT_PARAM_QR: HIPCHK(e, launch_reset_param(e, e.f[SKT_QR], nullptr)); break;
-        case SKT_PARAM_QS: HIPCHK(e, launch_reset_param(e, e.f[SKT_QS], nullptr)); break;
-        case SKT_PARAM_QI: HIPCHK(e, launch_reset_param(e, e.f[SKT_QI], nullptr)); break;
+        case SKT_PARAM_POTTEMP: HIPCHK(e, launch_reset_param(e, SKT_POTTEMP, e.f[SKT_ISEN_TEMP])); break;
+        case SKT_PARAM_QV: HIPCHK(e, launch_reset_param(e, SKT_QV, e.f[SKT_ISEN_VAPOR])); break;
+        case SKT_PARAM_QW: HIPCHK(e, launch_reset_param(e, SKT_QW, nullptr)); break;
+        case SKT_PARAM_QC: HIPCHK(e, launch_reset_param(e, SKT_QC, nullptr)); break;
+        case SKT_PARAM_QR: HIPCHK(e, launch_reset_param(e, SKT_QR, nullptr)); break;
+        case SKT_PARAM_QS: HIPCHK(e, launch_reset_param(e, SKT_QS, nullptr)); break;
+        case SKT_PARAM_QI: HIPCHK(e, launch_reset_param(e, SKT_QI, nullptr)); break;
         case SKT_PARAM_WIND:
-            HIPCHK(e, launch_reset_param(e, e.f[SKT_VELX], e.f[SKT_DEFAULT_VELX]));
-            HIPCHK(e, launch_reset_param(e, e.f[SKT_VELY], nullptr));
-            HIPCHK(e, launch_reset_param(e, e.f[SKT_VELZ], e.f[SKT_DEFAULT_VELZ]));
+            HIPCHK(e, launch_reset_param(e, SKT_VELX, e.f[SKT_DEFAULT_VELX]));
+            HIPCHK(e, launch_reset_param(e, SKT_VELY, nullptr));
+            HIPCHK(e, launch_reset_param(e, SKT_VELZ, e.f[SKT_DEFAULT_VELZ]));
             break;
-        case SKT_PARAM_PRESSURE: HIPCHK(e, launch_reset_param(e, e.f[SKT_PRESSURE], e.f[SKT_DEFAULT_PRESSURE])); break;
+        case SKT_PARAM_PRESSURE: HIPCHK(e, launch_reset_param(e, SKT_PRESSURE, e.f[SKT_DEFAULT_PRESSURE])); break;
         case SKT_PARAM_PGROUND: break;  // the reference has no reset for the ground
         default: return setErr(e, SKT_ERR_ARG, "resetParameterGPU: unknown parameter");
     }

@@ -371,8 +371,10 @@ int skt_batch_add(skt_batch* b, skt_env* env, int32_t* member_id) {
     for (const auto& m : b->members) {
         if (m.env == env) return berr(b, SKT_ERR_ARG, "environment already in the ensemble");
         const skt_config &a = m.env->s.cfg, &c = e.cfg;
-        if (a.pcg_mode != c.pcg_mode || a.pcg_max_iter != c.pcg_max_iter || a.diffusion_sweeps != c.diffusion_sweeps)
-            return berr(b, SKT_ERR_ARG, "members must share the solver mode, iteration limit and diffusion sweeps");
+        if (a.pcg_mode != c.pcg_mode || a.pcg_max_iter != c.pcg_max_iter || a.diffusion_sweeps != c.diffusion_sweeps ||
+            a.pcg_refine != c.pcg_refine)
+            return berr(b, SKT_ERR_ARG,
+                        "members must share the solver mode, iteration limit, refinement passes and diffusion sweeps");
     }
     // the member's work joins the ensemble's stream (after its own pending work)
     hipEvent_t ev = nullptr;

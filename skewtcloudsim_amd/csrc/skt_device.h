@@ -24,6 +24,53 @@ struct Bounds {
     int sides, up, ground;
 };
 
+// Storage of a field. FMT_F32 holds floats. The 16-bit forms hold
+// value * scale (scale a power of two, so the scaling itself is exact) as a
+// bfloat16 (FMT_BF16) or an IEEE half (FMT_F16), rounded to nearest even;
+// the mixed-precision form stores the five hydrometeor fields this way.
+enum { FMT_F32 = 0, FMT_BF16 = 1, FMT_F16 = 2 };
+struct Store {
+    int fmt;
+    float scale, inv;
+};
+__host__ __device__ inline Store storeF32() { return Store{FMT_F32, 1.0f, 1.0f}; }
+struct Store3 {
+    Store s[3];
+};
+__host__ __device__ inline float fromH16(unsigned short h, const Store& st) {
+    const float v = st.fmt == FMT_BF16 ? __builtin_bit_cast(float, (unsigned)h << 16)
+                                       : (float)__builtin_bit_cast(_Float16, h);
+    return v * st.inv;
+}
+__host__ __device__ inline unsigned short toH16(float v, const Store& st) {
+    const float s = v * st.scale;
+    if (st.fmt == FMT_BF16) {
+        const unsigned u = __builtin_bit_cast(unsigned, s);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // quiet NaN
+        return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    }
+    return __builtin_bit_cast(unsigned short, (_Float16)s);
+}
+// Element i of a field stored as st says (MIX: the storage may be 16-bit;
+// otherwise it is float and no format test is compiled).
+template <bool MIX>
+__device__ __forceinline__ float ldS(const void* p, long i, const Store& st) {
+    if constexpr (MIX) {
+        if (st.fmt != FMT_F32) return fromH16(static_cast<const unsigned short*>(p)[i], st);
+    }
+    return static_cast<const float*>(p)[i];
+}
+template <bool MIX>
+__device__ __forceinline__ void stS(void* p, long i, float v, const Store& st) {
+    if constexpr (MIX) {
+        if (st.fmt != FMT_F32) {
+            static_cast<unsigned short*>(p)[i] = toH16(v, st);
+            return;
+        }
+    }
+    static_cast<float*>(p)[i] = v;
+}
+
 // Grid geometry. Index = x + X*(y + Y*z); y is vertical; gh[x + X*z] is the
 // highest ground level of each column (cells with y <= gh are ground).
 // ghm1 / ghm2 hold the largest gh over the 3x3 / 5x5 neighbourhood of each

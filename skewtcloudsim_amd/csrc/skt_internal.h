@@ -24,6 +24,16 @@ struct PcgState {
     int pad[5];
 };
 
+// Totals of an FP64-refined pressure solve (written on the device).
+struct RefineStats {
+    int iters_total;  // iterations over the projection's solves
+    int passes;       // solves that ran iterations
+    double res64;     // max |b - A x| in FP64 for the final x
+    double rhs64;     // max |b|
+    float target;     // pcg_tol * max |b|: the stop value shared by the refinement passes
+    int pad;
+};
+
 // Device scalars shared by the reduction kernels.
 struct Scalars {
     PcgState st[2];
@@ -36,7 +46,8 @@ struct Scalars {
     double dg_rain;  //   ground rain water summed over the columns
     int face_exact;  // a sky 1/density of this solve lies outside the short face form's range
     int sub_overflow;  // an ensemble advection needed more substeps than its recorded passes
-    int pad[22];
+    int pad[2];
+    RefineStats ref;
 };
 
 // Host copy of the advection substep fields of Scalars (maxvel, nsub, dtsub).
@@ -123,6 +134,8 @@ struct EnvState {
     bool initialized = false;
 
     float* f[SKT_FIELD_COUNT] = {nullptr};  // field arrays (sizes N, G or Y)
+    Store fst[SKT_FIELD_COUNT];             // their storage (16-bit for the tracers of the mixed form)
+    bool mix = false;                       // some field is stored in 16 bits
     int* gh = nullptr;                      // ground height [G]
     int* ghm1 = nullptr;                    // max ground height over the 3x3 column neighbourhood [G]
     int* ghm2 = nullptr;                    // max ground height over the 5x5 column neighbourhood [G]
@@ -131,6 +144,8 @@ struct EnvState {
     float* work[SKT_NWORK] = {nullptr};     // N-sized work buffers
     float* pcg[6] = {nullptr};              // r, z, s_a, s_b, As, pre
     float* irho = nullptr;                  // 1/density of the last pressure solve (0 in ground)
+    float* rhs = nullptr;                   // FP64 refinement: the solve's right-hand side b [N]
+    double* x64 = nullptr;                  //   and the FP64 sum of its solves [N]
     float* zeros = nullptr;                 // [Y] zero profile
     float* ones = nullptr;                  // [Y] unit profile
     double* partials = nullptr;             // per-block reduction partials (PART_COUNT x maxBlocks)
@@ -166,6 +181,7 @@ struct EnvState {
     // pressure-solve state snapshots in pinned host memory: two alternating
     // chunk snapshots and the final state, each behind its event
     PcgState* pcg_snap = nullptr;
+    RefineStats* ref_snap = nullptr;  // refinement totals, copied with the final state
     hipEvent_t snap_ev[3] = {nullptr, nullptr, nullptr};
     bool pcg_stats_pending = false;
     // substep count launched ahead of an advection (startMaxvel)
@@ -219,7 +235,7 @@ ResidentBlocks resident_blocks(int device);
 // Fills gZ, gA and gD (at most one round of resident blocks each) and gB.
 void setup_grids(EnvState& e, ResidentBlocks rb);
 hipError_t launch_fill(EnvState& e, float* a, float v, long n);
-hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst);
+hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst, Store ss, Store ds);
 hipError_t launch_ground_water_snapshot(EnvState& e, float dt, float speed);
 // stages: 1 cloud cover, 2 ground temp, 4 precip hitting ground, 8 ground microphysics, 16 setTempsAtGround
 hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed, float irr);
@@ -231,6 +247,7 @@ struct DiffField {
     const float* prof;
     Bounds b;
     int type;
+    Store sin, sout;  // storage of in and out
 };
 struct DiffBatch {
     DiffField f[SKT_BATCH];
@@ -243,6 +260,7 @@ hipError_t launch_diagnostics(EnvState& e, float thr);
 struct EditArgs {
     int param;               // parameter id (POTTEMP 0 .. QI 6, WIND 7, PGROUND 8, PRESSURE 9)
     float* a[3];             // the parameter's arrays (wind: x, y, z components)
+    Store st[3];             // their storage
     float value, dir[3];     // applyValue, valueDir
     int erase;               // groundErase
     // brush
@@ -268,6 +286,7 @@ struct PpmField {
     const float* prof;
     Bounds b;
     const float* gsrc;  // values at ground positions (nullptr: 0), used under a NEUMANN ground condition
+    Store sin, sout, sg;  // storage of in, out and gsrc
 };
 struct PpmBatch {
     PpmField f[SKT_BATCH];
@@ -285,6 +304,7 @@ struct PrecipBatch {
     int type[3];
     float dts[3];
     int n;
+    Store st[3];
 };
 hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt);
 hipError_t launch_buoyancy(EnvState& e, float dt);
@@ -292,13 +312,18 @@ hipError_t launch_init_density(EnvState& e, int copy_old);
 hipError_t launch_new_pressure(EnvState& e);
 hipError_t launch_reset_vel(EnvState& e);
 // setToDefault (prof) or setToValue 0 (prof null) of one field
-hipError_t launch_reset_param(EnvState& e, float* a, const float* prof);
+hipError_t launch_reset_param(EnvState& e, int field, const float* prof);
 hipError_t launch_build_precon(EnvState& e);  // 1/density and 1/diagonal
 hipError_t launch_divergence(EnvState& e, float dt);  // r = div, partial max |r|
-hipError_t launch_pcg_start(EnvState& e);              // stop tolerance; z = pre r, partial z.r
+hipError_t launch_pcg_start(EnvState& e, int pass);    // stop tolerance; z = pre r, partial z.r (pass: refinement
+                                                       // solve index, -1 without refinement)
 hipError_t launch_pcg_A(EnvState& e, int it);          // stop test / beta; s update, As, partial s.As
 hipError_t launch_pcg_B(EnvState& e, int it);          // alpha; p', r, z updates, partial max |r| and z.r
 hipError_t launch_pcg_final(EnvState& e, int it);      // stop test after the last iteration
+hipError_t launch_refine_keep_rhs(EnvState& e);        // rhs = r (the divergence) before the first solve
+hipError_t launch_refine_accum(EnvState& e, int first, int last);  // x64 += p'
+hipError_t launch_refine_residual(EnvState& e, int write_r);      // r = b - A x64 in FP64 (+ max partials)
+hipError_t launch_refine_report(EnvState& e);          // final max |b - A x64|
 hipError_t launch_update_pressure_density(EnvState& e);
 hipError_t launch_apply_projection(EnvState& e, int new_pressure);
 hipError_t launch_microphysics(EnvState& e, float dt, float speed);

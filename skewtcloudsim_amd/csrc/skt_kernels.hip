@@ -92,17 +92,21 @@ hipError_t launch_fill(EnvState& e, float* a, float v, long n) {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<long>((n + 255) / 256, 65536)), dim3(256), 0, e.stream, a, v, n);
     return hipGetLastError();
 }
+template <bool MIX>
 struct KCopySky {
-    static __device__ __forceinline__ void run(VB vb, const Geom& G, const float* __restrict__ s, float* __restrict__ d) {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const float* __restrict__ s, float* __restrict__ d,
+                                               Store ss, Store ds) {
         CELL_SETUP_V;
         (void)bid;
         if (!inb || isGround(G, x, y, z)) return;
         const long i = gidx(G, x, y, z);
-        d[i] = s[i];
+        stS<MIX>(d, i, ldS<MIX>(s, i, ss), ds);
     }
 };
-hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst) {
-    return klaunch<KCopySky>(e, grid3(e), B3, 0, geom(e), src, dst);
+hipError_t launch_copy_sky(EnvState& e, const float* src, float* dst, Store ss, Store ds) {
+    if (ss.fmt != FMT_F32 || ds.fmt != FMT_F32)
+        return klaunch<KCopySky<true>>(e, grid3(e), B3, 0, geom(e), src, dst, ss, ds);
+    return klaunch<KCopySky<false>>(e, grid3(e), B3, 0, geom(e), src, dst, ss, ds);
 }
 // Plain streaming copy (float4 loads/stores): the measured bandwidth ceiling.
 KL void k_stream_copy(const float4* __restrict__ s, float4* __restrict__ d, long n4) {
@@ -132,6 +136,12 @@ __device__ __forceinline__ int bldi(Rsrc r, unsigned voff, unsigned soff) {
 }
 __device__ __forceinline__ void bstf(Rsrc r, unsigned voff, unsigned soff, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ unsigned short bldh(Rsrc r, unsigned voff, unsigned soff) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void bsth(Rsrc r, unsigned voff, unsigned soff, unsigned short v) {
+    __builtin_amdgcn_raw_buffer_store_b16(v, r, (int)voff, (int)soff, 0);
 }
 
 // LDS tile of one plane for the 64 x 4 block: rows y0-1 .. y0+4 and columns
@@ -164,6 +174,7 @@ constexpr int TH = 6, TW = 66;
 // needed over 100 VGPRs and ran slower at half the occupancy; direct
 // neighbour loads without the tile ran slower too.
 constexpr int DTH = 10;  // tile rows yb - 1 .. yb + 8
+template <bool MIX>
 struct KDiffuse {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, unsigned nbf, const DiffBatch& db, float k, const float* __restrict__ gT,
                       const float* __restrict__ gP, const float* __restrict__ p) {
@@ -196,7 +207,25 @@ struct KDiffuse {
         const unsigned hbL = hasH ? hb : obA;
         const int zb = max(z0 - 1, 0);
         const unsigned PB = 4u * (unsigned)XY, RB = 4u * (unsigned)X;
-        const Rsrc rV = arrayRsrc(fd.in + XY * zb), wV = arrayRsrc(fd.out + XY * zb);
+        // 16-bit storage (the mixed form's tracers): half the byte offsets
+        const bool in16 = MIX && fd.sin.fmt != FMT_F32, out16 = MIX && fd.sout.fmt != FMT_F32;
+        const Rsrc rV = arrayRsrc(static_cast<const char*>(static_cast<const void*>(fd.in)) + (in16 ? 2 : 4) * XY * zb);
+        const Rsrc wV = arrayRsrc(static_cast<char*>(static_cast<void*>(fd.out)) + (out16 ? 2 : 4) * XY * zb);
+        auto ldV = [&](unsigned off, unsigned soff) __attribute__((always_inline)) -> float {
+            if constexpr (MIX) {
+                if (in16) return fromH16(bldh(rV, off >> 1, soff >> 1), fd.sin);
+            }
+            return bldf(rV, off, soff);
+        };
+        auto stV = [&](unsigned off, unsigned soff, float v) __attribute__((always_inline)) {
+            if constexpr (MIX) {
+                if (out16) {
+                    bsth(wV, off >> 1, soff >> 1, toH16(v, fd.sout));
+                    return;
+                }
+            }
+            bstf(wV, off, soff, v);
+        };
         const Bounds b = fd.b;
         auto prof = [&](int yy) { return fd.prof ? fd.prof[yy] : 0.0f; };
         const float pvA = prof(yca), pvuA = prof(min(yca + 1, Y - 1));
@@ -210,13 +239,13 @@ struct KDiffuse {
             int gm, gf;  // ghm1 of the plane, ground height of the next plane
         };
         auto loadOwn = [&](int pl, Own& o) __attribute__((always_inline)) {
-            o.a = bldf(rV, obA, pOff(pl));
-            o.b = bldf(rV, obB, pOff(pl));
+            o.a = ldV(obA, pOff(pl));
+            o.b = ldV(obB, pOff(pl));
             o.gm = bldi(rM, gb, gOff(pl));
             o.gf = bldi(rG, gb, gOff(min(pl + 1, Z - 1)));
         };
         // plane below the chunk (clamped: plane 0 for the first chunk)
-        float vbA = bldf(rV, obA, pOff(zb)), vbB = bldf(rV, obB, pOff(zb));
+        float vbA = ldV(obA, pOff(zb)), vbB = ldV(obB, pOff(zb));
         int ghc = bldi(rG, gb, gOff(z0));
         Own o0{}, o1{}, o2{}, o3{};
         float h0 = 0.0f, h1 = 0.0f;
@@ -226,7 +255,7 @@ struct KDiffuse {
         loadOwn(z0, o0);
         loadOwn(min(z0 + 1, zlast), o1);
         loadOwn(min(z0 + 2, zlast), o2);
-        h0 = bldf(rV, hbL, pOff(z0));
+        h0 = ldV(hbL, pOff(z0));
         // planes (bit z - z0) whose cell of row a / row b is next to ground
         unsigned long long slowA = 0, slowB = 0;
         const bool oL = x == 0, oR = x == X - 1;
@@ -240,12 +269,12 @@ struct KDiffuse {
             const float l = oL ? bcv(b.sides, L, pv) : L, rr = oR ? bcv(b.sides, R, pv) : R;
             const float u = oU ? bcv(b.sides, U, pvu) : U;
             const float fr = oF ? bcv(b.sides, F, pv) : F, bb = oB ? bcv(b.sides, vb, pv) : vb;
-            bstf(wV, ob, pOff(z), (vc + k * (l + rr + u + D + fr + bb)) / den);
+            stV(ob, pOff(z), (vc + k * (l + rr + u + D + fr + bb)) / den);
         };
         auto step = [&](int z, int buf, Own& oc, Own& of, Own& o3n, float& hc, float& hn) __attribute__((always_inline)) {
             // loads complete in issue order: the halo (used next plane) goes first,
             // so waiting for it does not wait for the plane read three ahead
-            hn = bldf(rV, hbL, pOff(min(z + 1, zlast)));
+            hn = ldV(hbL, pOff(min(z + 1, zlast)));
             loadOwn(min(z + 3, zlast), o3n);
             tV[buf][ty + 1][tx + 1] = oc.a;
             tV[buf][ty + 5][tx + 1] = oc.b;
@@ -280,7 +309,7 @@ struct KDiffuse {
         // potential temperature of the ground temperature at this cell's pressure
         // (:95-100). Neighbour positions are clamped into the domain, as in the tile.
         const int xl = max(xc - 1, 0), xr = min(xc + 1, X - 1);
-        const float* __restrict__ in = fd.in;
+        auto in = [&](long i) { return ldS<MIX>(fd.in, i, fd.sin); };
         auto slowCells = [&](unsigned long long slow, int y) {
             const int yd = max(y - 1, 0), yu = min(y + 1, Y - 1);
             const float pv = prof(y), pvd = prof(yd), pvu = prof(yu);
@@ -296,10 +325,10 @@ struct KDiffuse {
                 // ground test at the clamped column is the in-domain test
                 const bool gL = y <= gz[xl], gR = y <= gz[xr], gD = y - 1 <= gc;
                 const bool gF = !oF && y <= gf, gB = !oB && y <= gbk;
-                const float vc = in[pz + xc + X * y];
-                const float L = in[pz + xl + X * y], R = in[pz + xr + X * y];
-                const float D = in[pz + xc + X * yd], U = in[pz + xc + X * yu];
-                const float F = in[XY * zf + xc + X * y], Bk = in[XY * zbk + xc + X * y];
+                const float vc = in(pz + xc + X * y);
+                const float L = in(pz + xl + X * y), R = in(pz + xr + X * y);
+                const float D = in(pz + xc + X * yd), U = in(pz + xc + X * yu);
+                const float F = in(XY * zf + xc + X * y), Bk = in(XY * zbk + xc + X * y);
                 float l = nbv(L, oL, gL, pv, b), r = nbv(R, oR, gR, pv, b);
                 float d = nbv(D, false, gD, pvd, b), u = nbv(U, oU, false, pvu, b);
                 float fr = nbv(F, oF, gF, pv, b), bb = nbv(Bk, oB, gB, pv, b);
@@ -312,7 +341,7 @@ struct KDiffuse {
                     if (gF) fr = tg(x + X * (z + 1));
                     if (gB) bb = tg(x + X * (z - 1));
                 }
-                fd.out[pz + xc + X * y] = (vc + k * (l + r + u + d + fr + bb)) / den;
+                stS<MIX>(fd.out, pz + xc + X * y, (vc + k * (l + r + u + d + fr + bb)) / den, fd.sout);
             }
         };
         __shared__ unsigned long long sMask[8][64];
@@ -329,8 +358,11 @@ struct KDiffuse {
 hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k) {
     const ZGrid zg = e.gD;
     if ((double)(zg.zc + 3) * (double)e.X * (double)e.Y * 4.0 >= 4294967296.0) return hipErrorInvalidValue;
-    return klaunch<KDiffuse>(e, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k, e.f[SKT_GROUND_T],
-                             e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
+    if (e.mix)
+        return klaunch<KDiffuse<true>>(e, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k,
+                                       e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
+    return klaunch<KDiffuse<false>>(e, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k,
+                                    e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
 }
 
 // ------------------------------------------------------------- advection ---
@@ -457,16 +489,14 @@ __device__ __forceinline__ bool ppmPassDt(const PpmBatch& pb, float& dt) {
     const PpmField& fd_ = pb.f[lin_ / nbf];                               \
     ZM_SETUP_B(lin_ % nbf);                                               \
     (void)bid;                                                            \
-    const float* __restrict__ in = fd_.in;                                \
-    float* __restrict__ out = fd_.out;                                    \
     const float* __restrict__ vel = fd_.vel;                              \
     const float* __restrict__ prof = fd_.prof;                            \
     const Bounds b = fd_.b;                                               \
     /* value that stands at a ground position: the field's own ground */  \
     /* value or 0 (read only under a NEUMANN ground condition)        */  \
-    const float* __restrict__ gsrc = fd_.gsrc;                            \
-    const bool gNeu_ = b.ground == BC_NEUMANN && gsrc != nullptr;         \
-    auto graw = [&](long gi) { return gNeu_ ? gsrc[gi] : 0.0f; }
+    const bool gNeu_ = b.ground == BC_NEUMANN && fd_.gsrc != nullptr;     \
+    auto graw = [&](long gi) { return gNeu_ ? ldS<MIX>(fd_.gsrc, gi, fd_.sg) : 0.0f; }; \
+    auto in = [&](long i) { return ldS<MIX>(fd_.in, i, fd_.sin); }
 
 // x pass as a march along the rows: each wave owns one row (y, z) of a
 // plane at a time and walks its 64-cell segments from x = 0 upward. The flux
@@ -480,6 +510,7 @@ __device__ __forceinline__ bool ppmPassDt(const PpmBatch& pb, float& dt) {
 // rows (one per wave) over a chunk of zc planes of one field; the loads of up
 // to XSEG_PPM segments are issued before any is used.
 constexpr int XSEG_PPM = 2;
+template <bool MIX>
 struct KPpmXr {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, unsigned nbf, const PpmBatch& pb,
                                                float dtArg, float dx, const float* __restrict__ cust) {
@@ -492,22 +523,20 @@ struct KPpmXr {
         const int y = (int)(rem % (unsigned)gy) * 4 + (int)threadIdx.y;
         const int z0 = (int)(rem / (unsigned)gy) * zc, z1 = min(G.Z, z0 + zc);
         if (y >= G.Y || z0 >= z1) return;  // whole waves (one row per wave)
-        const float* __restrict__ in = fd.in;
-        float* __restrict__ out = fd.out;
+        auto in = [&](long i) { return ldS<MIX>(fd.in, i, fd.sin); };
         const float* __restrict__ vel = fd.vel;
         const float* __restrict__ prof = fd.prof;
         const Bounds b = fd.b;
         // value that stands at a ground position: the field's own ground value
         // or 0 (read only under a NEUMANN ground condition)
-        const float* __restrict__ gsrc = fd.gsrc;
-        const bool gNeu = b.ground == BC_NEUMANN && gsrc != nullptr;
+        const bool gNeu = b.ground == BC_NEUMANN && fd.gsrc != nullptr;
         const int X = G.X, tx = (int)threadIdx.x;
         const long XY = (long)X * G.Y;
         const unsigned rowy = X * y;
         const float pv = prof ? prof[y] : 0.0f, cv = cust[y];
         const int nseg = (X + 63) >> 6;
         for (int z = z0; z < z1; z++) {
-            const float* pl = in + XY * z + rowy;
+            const long pl = XY * z + rowy;
             const float* vl = vel + XY * z + rowy;
             const int* gz = G.gh + X * z;
             const int* gmz = G.ghm2 + X * z;
@@ -520,7 +549,7 @@ struct KPpmXr {
                     // lanes past the row (and segments past the last) read its last cell
                     const int xr = min((k0 + j) * 64 + tx, X - 1);
     #pragma unroll
-                    for (int s = 0; s < 5; s++) Q[j][s] = pl[clampi(xr + s - 2, 0, X - 1)];
+                    for (int s = 0; s < 5; s++) Q[j][s] = in(pl + clampi(xr + s - 2, 0, X - 1));
                     VC[j] = vl[xr];
                     VM[j] = vl[max(xr - 1, 0)];
                     GM[j] = gmz[xr];
@@ -552,7 +581,7 @@ struct KPpmXr {
                                 const bool outp = (x + s - 2 < 0) || (x + s - 2 > X - 1);
                                 const bool gnd = y <= gh[s];
                                 const float raw =
-                                    gnd ? (gNeu ? gsrc[XY * z + rowy + clampi(x + s - 2, 0, X - 1)] : 0.0f) : Q[j][s];
+                                    gnd ? (gNeu ? ldS<MIX>(fd.gsrc, XY * z + rowy + clampi(x + s - 2, 0, X - 1), fd.sg) : 0.0f) : Q[j][s];
                                 q[s] = nbv(raw, outp, gnd, pv, b);
                                 o[s] = unitv(outp, gnd, b);
                             }
@@ -566,7 +595,7 @@ struct KPpmXr {
                     const float last = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Fr), 63));
                     if (sky) {
                         const float Fl = leftSky ? (tx > 0 ? up : carry) : ppmFaceL(q, vm, dt, dx);
-                        out[XY * z + rowy + x] = ppmFinish(q, o, VC[j], vm, Fr, Fl, dt, dx);
+                        stS<MIX>(fd.out, XY * z + rowy + x, ppmFinish(q, o, VC[j], vm, Fr, Fl, dt, dx), fd.sout);
                     }
                     carry = last;
                 }
@@ -586,6 +615,7 @@ struct KPpmXr {
 // it directly. Blocks are 64 (x) x 4 (z) columns of one chunk of one field.
 constexpr int YC_PPM = 16;
 constexpr int YG_PPM = 4;
+template <bool MIX>
 struct KPpmYm {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, unsigned nbf,
                                                const PpmBatch& pb, float dtArg, float dx) {
@@ -600,13 +630,11 @@ struct KPpmYm {
         const int x = (int)(tile % (unsigned)gx) * 64 + threadIdx.x;
         const int z = (int)(tile / (unsigned)gx) * 4 + threadIdx.y;
         if (x >= G.X || z >= G.Z) return;
-        const float* __restrict__ in = fd.in;
-        float* __restrict__ out = fd.out;
+        auto in = [&](long i) { return ldS<MIX>(fd.in, i, fd.sin); };
         const float* __restrict__ vel = fd.vel;
         const float* __restrict__ prof = fd.prof;
         const Bounds b = fd.b;
-        const float* __restrict__ gsrc = fd.gsrc;
-        const bool gNeu = b.ground == BC_NEUMANN && gsrc != nullptr;
+        const bool gNeu = b.ground == BC_NEUMANN && fd.gsrc != nullptr;
         const int X = G.X, Y = G.Y;
         const long col = (long)X * Y * z + x;  // cell (x, 0, z)
         const int gh = G.gh[x + (long)X * z];
@@ -617,7 +645,7 @@ struct KPpmYm {
             const int by = clampi(py, 0, Y - 1);
             const bool outp = (py < 0) || (py > Y - 1);
             const bool gnd = by <= gh;
-            if (gnd) raw = gNeu ? gsrc[col + (long)X * by] : 0.0f;
+            if (gnd) raw = gNeu ? ldS<MIX>(fd.gsrc, col + (long)X * by, fd.sg) : 0.0f;
             qv = nbv(raw, outp, gnd, prof ? prof[by] : 0.0f, b);
             ov = unitv(outp, gnd, b);
         };
@@ -625,13 +653,13 @@ struct KPpmYm {
     #pragma unroll
         for (int s = 0; s < 4; s++) {
             const int py = y0 - 2 + s;
-            pos(py, in[col + (long)X * clampi(py, 0, Y - 1)], q[s], o[s]);
+            pos(py, in(col + (long)X * clampi(py, 0, Y - 1)), q[s], o[s]);
         }
         // raw field values at levels g + 2 .. g + 5 and face velocities at g .. g + 3
         auto fetch = [&](int g, float* Q, float* V) {
     #pragma unroll
             for (int j = 0; j < YG_PPM; j++) {
-                Q[j] = in[col + (long)X * min(g + j + 2, Y - 1)];
+                Q[j] = in(col + (long)X * min(g + j + 2, Y - 1));
                 V[j] = vel[col + (long)X * min(g + j, Y - 1)];
             }
         };
@@ -651,7 +679,7 @@ struct KPpmYm {
                     const float Fr = ppmFaceR(q, vc, dt, dx);
                     const float Fl = (y > y0 && !below) ? fPrev : ppmFaceL(q, vm, dt, dx);
                     fPrev = Fr;
-                    out[col + (long)X * y] = ppmFinish(q, o, vc, vm, Fr, Fl, dt, dx);
+                    stS<MIX>(fd.out, col + (long)X * y, ppmFinish(q, o, vc, vm, Fr, Fl, dt, dx), fd.sout);
                 }
                 vPrev = vc;
     #pragma unroll
@@ -677,7 +705,7 @@ constexpr int ZB_PPM = 4;  // planes of the z pass whose loads go together
 // z pass: the stencil runs along this thread's march; a window of planes
 // zg-2 .. zg+ZB+1 (values and ground heights) stays in registers, so the
 // boundary rule needs no extra loads.
-template <int ZB>
+template <int ZB, bool MIX>
 struct KPpmZ {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, unsigned nbf, const PpmBatch& pb,
                                                float dtArg, float dx, const float* __restrict__ cust) {
@@ -694,14 +722,14 @@ struct KPpmZ {
     #pragma unroll
         for (int s = 0; s < 4; s++) {
             const int bz = clampi(z0 - 2 + s, 0, Z - 1);
-            qw[s] = (in + XY * bz)[rc];
+            qw[s] = in(XY * bz + rc);
             gw[s] = (G.gh + X * bz)[(unsigned)x];
         }
         vw[0] = (vel + XY * max(z0 - 1, 0))[rc];
         // window slot m holds plane zg - 2 + m; velocity slot m plane zg - 1 + m
         auto load = [&](int z, float& q, int& g, float& v) {
             const int bz = min(z + 2, Z - 1);
-            q = (in + XY * bz)[rc];
+            q = in(XY * bz + rc);
             g = (G.gh + X * bz)[(unsigned)x];
             v = (vel + XY * z)[rc];
         };
@@ -724,7 +752,7 @@ struct KPpmZ {
             const float Fr = ppmFaceR(q, vc, dt, dx);
             const float Fl = (z > z0 && y > gs[1]) ? fPrev : ppmFaceL(q, vm, dt, dx);
             fPrev = Fr;
-            (out + XY * z)[rc] = ppmFinish(q, o, vc, vm, Fr, Fl, dt, dx);
+            stS<MIX>(fd_.out, XY * z + rc, ppmFinish(q, o, vc, vm, Fr, Fl, dt, dx), fd_.sout);
         };
         int zg = z0;
         for (; zg + ZB <= z1; zg += ZB) {
@@ -761,14 +789,21 @@ hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) 
         const long nzc = std::max(1L, std::min((long)e.Z, 2048 / gy));
         const int zc = (int)((e.Z + nzc - 1) / nzc);
         const unsigned nbf = (unsigned)(gy * ((e.Z + zc - 1) / zc));
-        return klaunch<KPpmXr>(e, dim3(nbf * pb.n), B3, 0, geom(e), zc, nbf, pb, dt, e.dx, e.f[SKT_DEFAULT_VELX]);
+        return e.mix ? klaunch<KPpmXr<true>>(e, dim3(nbf * pb.n), B3, 0, geom(e), zc, nbf, pb, dt, e.dx,
+                                             e.f[SKT_DEFAULT_VELX])
+                     : klaunch<KPpmXr<false>>(e, dim3(nbf * pb.n), B3, 0, geom(e), zc, nbf, pb, dt, e.dx,
+                                              e.f[SKT_DEFAULT_VELX]);
     }
     if (dir == 1) {
         const unsigned nyc = (unsigned)((e.Y + YC_PPM - 1) / YC_PPM);
         const unsigned nbf = (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * nyc;
-        return klaunch<KPpmYm>(e, dim3(nbf * pb.n), B3, 0, geom(e), YC_PPM, nyc, nbf, pb, dt, e.dx);
+        return e.mix ? klaunch<KPpmYm<true>>(e, dim3(nbf * pb.n), B3, 0, geom(e), YC_PPM, nyc, nbf, pb, dt, e.dx)
+                     : klaunch<KPpmYm<false>>(e, dim3(nbf * pb.n), B3, 0, geom(e), YC_PPM, nyc, nbf, pb, dt, e.dx);
     }
-    return klaunch<KPpmZ<ZB_PPM>>(e, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx, e.f[SKT_DEFAULT_VELZ]);
+    return e.mix ? klaunch<KPpmZ<ZB_PPM, true>>(e, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
+                                                e.f[SKT_DEFAULT_VELZ])
+                 : klaunch<KPpmZ<ZB_PPM, false>>(e, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
+                                                 e.f[SKT_DEFAULT_VELZ]);
 }
 
 // ---------------------------------------------------- pressure projection ---
@@ -897,10 +932,16 @@ __device__ __forceinline__ void pcgPartLoad(const double* __restrict__ pm, const
         P.s[k] = wantS && j < n ? ps[j] : 0.0;
     }
 }
+// Under FP64 refinement (ro set; pass = the solve's index in the projection)
+// PRO_START also applies the projection's shared stop value ro->target =
+// tolOut * max|b|: pass 0 sets it, a later pass is done at once when its
+// residual already meets it, and every pass stops at the larger of its own
+// relative tolerance and the target.
 template <int KIND>
 __device__ void pcgPrologue(const double* __restrict__ pm, const double* __restrict__ ps, unsigned n,
                             const PcgPart& P, const PcgState* __restrict__ sin, PcgState* __restrict__ sout, int ref,
-                            float tol, int it, PcgState& S, double* sh, unsigned tile) {
+                            float tol, int it, PcgState& S, double* sh, unsigned tile, RefineStats* ro = nullptr,
+                            int pass = 0, float tolOut = 0.0f) {
     const int t = tid256();
     if (t == 0) {
         if (KIND == PRO_START) {
@@ -933,6 +974,17 @@ __device__ void pcgPrologue(const double* __restrict__ pm, const double* __restr
                     S.iters = 0;
                     S.done = (m0 == 0.0f) ? 1 : 0;
                     S.tol = ref ? tol : tol * m0;
+                    if (ro) {
+                        if (pass == 0) {
+                            const float tg = tolOut * m0;
+                            S.tol = fmaxf(S.tol, tg);
+                            if (tile == 0) ro->target = tg;
+                        } else {
+                            const float tg = ro->target;
+                            if (m0 <= tg) S.done = 1;
+                            S.tol = fmaxf(S.tol, tg);
+                        }
+                    }
                 } break;
                 case PRO_A0:
                     S.sigma0 = (double)(float)vsum;
@@ -1059,13 +1111,13 @@ hipError_t launch_divergence(EnvState& e, float dt) {
 struct KPcgStart {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, const float* __restrict__ r, float* __restrict__ zz, const float* __restrict__ pre,
                         const double* __restrict__ pin, unsigned n, double* pout, const PcgState* sin, PcgState* sout,
-                        int ref, float tol) {
+                        int ref, float tol, RefineStats* ro, int pass, float tolOut) {
         ZM_SETUP_V;
         __shared__ double sh[4];
         __shared__ PcgState S;
         PcgPart P;
         pcgPartLoad<PRO_START>(pin, nullptr, n, P);
-        pcgPrologue<PRO_START>(pin, nullptr, n, P, sin, sout, ref, tol, 0, S, sh, vb.id);
+        pcgPrologue<PRO_START>(pin, nullptr, n, P, sin, sout, ref, tol, 0, S, sh, vb.id, ro, pass, tolOut);
         if (S.done) return;
         double s = 0.0;
         if (inb) {
@@ -1083,13 +1135,15 @@ struct KPcgStart {
         if (tid256() == 0) pout[bid] = s;
     }
 };
-hipError_t launch_pcg_start(EnvState& e) {
+hipError_t launch_pcg_start(EnvState& e, int pass) {
     const int ref = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     const ZGrid zg = zgrid(e);
     const size_t M = e.maxBlocks;
+    const bool refine = pass >= 0;
     return klaunch<KPcgStart>(e, zg.grid, B3, 0, geom(e), zg.zc, e.pcg[0], e.pcg[1], e.pcg[5], e.partials + PART_DIV * M,
                               divBlocks(e), e.partials + PART_START * M, &e.sc->st[0], &e.sc->st[1], ref,
-                              e.cfg.pcg_tol);
+                              refine ? e.cfg.pcg_refine_tol : e.cfg.pcg_tol, refine ? &e.sc->ref : nullptr,
+                              refine ? pass : 0, e.cfg.pcg_tol);
 }
 
 // Prologue PRO_A0 / PRO_A, then the search-direction update (endIteration,
@@ -1329,7 +1383,7 @@ ResidentBlocks resident_blocks(int device) {
     return ResidentBlocks{resident_blocks(&k_solo<KPcgA<false, true>, Geom, int, float*, float*, float*, float*, float*, const double*,
                                           const double*, unsigned, double*, PcgState*, PcgState*, int, int*>,
                                           device, cus),
-                          resident_blocks(&k_solo<KDiffuse, Geom, int, unsigned, DiffBatch, float, float*, float*,
+                          resident_blocks(&k_solo<KDiffuse<false>, Geom, int, unsigned, DiffBatch, float, float*, float*,
                                                   float*>,
                                           device, cus),
                           cus};
@@ -1458,6 +1512,140 @@ struct KPcgFinal {
 hipError_t launch_pcg_final(EnvState& e, int it) {
     return klaunch<KPcgFinal>(e, dim3(1), B3, 0, e.partials + (size_t)PART_BMAX * e.maxBlocks, e.gB.nblocks,
                               &e.sc->st[1], &e.sc->st[0], it);
+}
+
+// ------------------------------------------------- FP64 residual refinement ---
+// The solves of one projection sum into x64 in FP64; each refinement pass
+// solves A d = b - A x64 in FP32 with the residual formed in FP64. A is the
+// FP32 solver's operator exactly (the face coefficients faceI of the two
+// cells' 1/density and the diagonal summed in float, in the SpMV's order, with
+// its boundary rules); only its products and sums are carried in FP64.
+//
+// b (the divergence) is kept before the first solve overwrites r.
+struct KCopyN {
+    static __device__ __forceinline__ void run(VB vb, const float* __restrict__ a, float* __restrict__ d, long n) {
+        for (long i = vb.id * 256L + tid256(); i < n; i += (long)vb.n * 256) d[i] = a[i];
+    }
+};
+// x64 = (first ? 0 : x64) + p' (the last solve's correction); p' = 0 for the
+// next solve or, after the last one, float(x64). Tile 0 totals the
+// iterations of the solve just finished (its final state is in st[0]).
+struct KRefineAccum {
+    static __device__ __forceinline__ void run(VB vb, long n, float* __restrict__ pp, double* __restrict__ x64,
+                                               int first, int last, Scalars* sc) {
+        for (long i = vb.id * 256L + tid256(); i < n; i += (long)vb.n * 256) {
+            const double v = (first ? 0.0 : x64[i]) + (double)pp[i];
+            x64[i] = v;
+            pp[i] = last ? (float)v : 0.0f;
+        }
+        if (vb.id == 0 && tid256() == 0) {
+            const PcgState& st = sc->st[0];
+            RefineStats& R = sc->ref;
+            if (first) {
+                R.iters_total = st.iters;
+                R.passes = 1;
+                R.rhs64 = (double)st.maxr0;
+            } else {
+                R.iters_total += st.iters;
+                R.passes += st.iters > 0 ? 1 : 0;
+            }
+        }
+    }
+};
+// r = b - A x64 in FP64 over the divergence kernel's column march, written
+// as float into r (when r is set) and its per-block max |r| into part, in the
+// divergence's partial layout (so the next solve's start reads it as it reads
+// the divergence's).
+struct KRefineResidual {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, const double* __restrict__ x64,
+                                               const float* __restrict__ irho, const float* __restrict__ b,
+                                               float* __restrict__ r, double* part) {
+        __shared__ double sh[4];
+        const unsigned lin = vb.id;
+        const int chunk = (int)(lin % nyc);
+        const unsigned tile = lin / nyc;
+        const int gx = (G.X + 63) >> 6;
+        const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
+        const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
+        double m = 0.0;
+        if (x < G.X && z < G.Z) {
+            const int X = G.X, Y = G.Y, Z = G.Z;
+            const long XY = (long)X * Y;
+            const long col = XY * z + x;
+            const long gc = x + (long)X * z;
+            const int gh = G.gh[gc];
+            const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
+            const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
+            const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
+            const float INF = __builtin_inff();
+            const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
+            for (int y = y0; y < y1; y++) {
+                const long i = col + (long)X * y;
+                double rv = 0.0;
+                if (y > gh) {
+                    const float ic = irho[i];
+                    // a neighbour beyond the domain takes the cell's own 1/density
+                    // (its value counts 0), a ground neighbour a zero face
+                    const bool sR = inR && y > ghR, sL = inL && y > ghL, sU = y + 1 < Y, sD = y > 0 && y - 1 > gh;
+                    const bool sF = inF && y > ghF, sB = inB && y > ghB;
+                    const float iR = !inR ? ic : (sR ? irho[i + 1] : INF);
+                    const float iL = !inL ? ic : (sL ? irho[i - 1] : INF);
+                    const float iU = sU ? irho[i + X] : ic;
+                    const float iD = y == 0 ? ic : (sD ? irho[i - X] : INF);
+                    const float iF = !inF ? ic : (sF ? irho[i + XY] : INF);
+                    const float fr = faceI(ic, iR), fl = faceI(ic, iL), fu = faceI(ic, iU), fd = faceI(ic, iD);
+                    const float ff = faceI(ic, iF);
+                    const float fb = z == 0 ? faceI(ic, ic) : (sB ? faceI(irho[i - XY], ic) : 0.0f);
+                    float w = 0.0f;
+                    w -= fr;
+                    w -= fl;
+                    w -= fu;
+                    w -= fd;
+                    w -= ff;
+                    w -= fb;
+                    double ax = (double)w * x64[i];
+                    if (sL) ax += (double)fl * x64[i - 1];
+                    if (sD) ax += (double)fd * x64[i - X];
+                    if (sB) ax += (double)fb * x64[i - XY];
+                    if (sR) ax += (double)fr * x64[i + 1];
+                    if (sU) ax += (double)fu * x64[i + X];
+                    if (sF) ax += (double)ff * x64[i + XY];
+                    rv = (double)b[i] - ax;
+                    m = fmax(m, fabs(rv));
+                }
+                if (r) r[i] = (float)rv;
+            }
+        }
+        m = blockMax(m, sh);
+        if (tid256() == 0) part[vb.id] = m;
+    }
+};
+// The final residual's maximum over the partials into the refinement totals.
+struct KRefineReport {
+    static __device__ __forceinline__ void run(VB vb, const double* __restrict__ part, unsigned n, Scalars* sc) {
+        __shared__ double sh[4];
+        (void)vb;
+        double m = 0.0;
+        for (unsigned j = tid256(); j < n; j += 256) m = fmax(m, part[j]);
+        m = blockMax(m, sh);
+        if (tid256() == 0) sc->ref.res64 = m;
+    }
+};
+hipError_t launch_refine_keep_rhs(EnvState& e) {
+    return klaunch<KCopyN>(e, grid1(e.N), dim3(256), 0, (const float*)e.pcg[0], e.rhs, e.N);
+}
+hipError_t launch_refine_accum(EnvState& e, int first, int last) {
+    return klaunch<KRefineAccum>(e, grid1(e.N), dim3(256), 0, e.N, e.f[SKT_PRESSURE_PROJ], e.x64, first, last, e.sc);
+}
+hipError_t launch_refine_residual(EnvState& e, int write_r) {
+    const unsigned nyc = (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
+    return klaunch<KRefineResidual>(e, dim3(divBlocks(e)), B3, 0, geom(e), YC_DIV, nyc, (const double*)e.x64,
+                                    (const float*)e.irho, (const float*)e.rhs, write_r ? e.pcg[0] : nullptr,
+                                    e.partials + (size_t)PART_DIV * e.maxBlocks);
+}
+hipError_t launch_refine_report(EnvState& e) {
+    return klaunch<KRefineReport>(e, dim3(1), dim3(256), 0, (const double*)(e.partials + (size_t)PART_DIV * e.maxBlocks),
+                                  (unsigned)divBlocks(e), e.sc);
 }
 
 // updatePressure (Game/kernelSky.cu:1016-1030) fused with the initDensity that

@@ -85,10 +85,12 @@ struct GroundArgs {
     float *Qv, *Qr, *Qs, *Qi, *Qc, *Qw, *th, *u, *p, *rho;
     float *Qrs, *Qgr, *Qgs, *Qgi, *gP, *gt, *gT, *cover;
     const float* irrDev;  // irradiance on the device (null: the irr argument)
+    Store sQr, sQs, sQi, sQc, sQw;  // storage of the hydrometeor fields
 };
 
 // One thread per ground column: the column-local part of the ground phase of
 // updateGPU (Game/environment.cu:427-455,469) in the reference's order.
+template <bool MIX>
 struct KGroundColumns {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, const GroundArgs& a, int stages, float dt,
                                                float speed, float irrArg, float dx, const Gammas& gm) {
@@ -102,7 +104,7 @@ struct KGroundColumns {
             float tot = 0.0f;
             for (int y = gh + 1; y < G.Y; y++) {
                 const long i = gidx(G, x, y, zc);
-                tot += (a.Qc[i] + a.Qw[i]) * dx;
+                tot += (ldS<MIX>(a.Qc, i, a.sQc) + ldS<MIX>(a.Qw, i, a.sQw)) * dx;
             }
             a.cover[g] = fminf(tot / 1.2f, 1.0f);
         }
@@ -118,9 +120,11 @@ struct KGroundColumns {
                 const float Qgs = a.Qgs[g] > 1e-18f ? a.Qgs[g] : 0.0f;
                 const float Qgi = a.Qgi[g] > 1e-18f ? a.Qgi[g] : 0.0f;
                 const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
-                const float Qr = a.Qr[i] > 1e-18f ? a.Qr[i] : 0.0f;
-                const float Qs = a.Qs[i] > 1e-18f ? a.Qs[i] : 0.0f;
-                const float Qi = a.Qi[i] > 1e-18f ? a.Qi[i] : 0.0f;
+                const float qr0 = ldS<MIX>(a.Qr, i, a.sQr), qs0 = ldS<MIX>(a.Qs, i, a.sQs);
+                const float qi0 = ldS<MIX>(a.Qi, i, a.sQi);
+                const float Qr = qr0 > 1e-18f ? qr0 : 0.0f;
+                const float Qs = qs0 > 1e-18f ? qs0 : 0.0f;
+                const float Qi = qi0 > 1e-18f ? qi0 : 0.0f;
                 float freeze = 0.0f;
                 const float tGC = a.gT[g] - 273.15f;
                 const float ps = a.p[i];
@@ -137,9 +141,9 @@ struct KGroundColumns {
                     PGRFR = dt * fminf(Qr, speed * PGRFR);
                     freeze += PGRFR;
                 }
-                a.Qr[i] += -GR;
-                a.Qs[i] += -GS;
-                a.Qi[i] += -GI;
+                stS<MIX>(a.Qr, i, qr0 + -GR, a.sQr);
+                stS<MIX>(a.Qs, i, qs0 + -GS, a.sQs);
+                stS<MIX>(a.Qi, i, qi0 + -GI, a.sQi);
                 a.Qgr[g] += GR - PGRFR;
                 a.Qgs[g] += GS;
                 a.Qgi[g] += GI + PGRFR;
@@ -208,11 +212,15 @@ hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed,
     GroundArgs a{e.f[SKT_QV], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_QC], e.f[SKT_QW], e.f[SKT_POTTEMP],
                  e.f[SKT_VELX], e.f[SKT_PRESSURE], e.f[SKT_DENSITY], e.f[SKT_QRS], e.f[SKT_QGR], e.f[SKT_QGS],
                  e.f[SKT_QGI], e.f[SKT_GROUND_P], e.f[SKT_GROUND_TIME], e.f[SKT_GROUND_T], e.f[SKT_CLOUD_COVER],
-                 nullptr};
+                 nullptr, e.fst[SKT_QR], e.fst[SKT_QS], e.fst[SKT_QI], e.fst[SKT_QC], e.fst[SKT_QW]};
     // a recorded ensemble step reads the member's irradiance of each step
     // from its device slot
     a.irrDev = e.rec ? e.rec->irr : nullptr;
-    return klaunch<KGroundColumns>(e, grid1(e.G), dim3(256), 0, geom(e), a, stages, dt, speed, irr, e.dx, e.gm);
+    if (e.mix)
+        return klaunch<KGroundColumns<true>>(e, grid1(e.G), dim3(256), 0, geom(e), a, stages, dt, speed, irr, e.dx,
+                                             e.gm);
+    return klaunch<KGroundColumns<false>>(e, grid1(e.G), dim3(256), 0, geom(e), a, stages, dt, speed, irr, e.dx,
+                                          e.gm);
 }
 
 // advectPrecipGPU, Game/kernelSky.cu:576-657. Each level reads its own and
@@ -224,6 +232,7 @@ hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed,
 // it, so the chunks run side by side with the results of one walk per column.
 constexpr int PRECIP_CHUNKS = 4;
 static_assert(64 * PRECIP_CHUNKS <= 256, "k_precip blocks (64 x PRECIP_CHUNKS) must fit the 256-thread launch bound");
+template <bool MIX>
 struct KPrecip {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, const PrecipBatch& pbt, const float* __restrict__ th, const float* __restrict__ p,
                      const float* __restrict__ qv, const float* __restrict__ gP, float dx, const Gammas& gm) {
@@ -236,6 +245,7 @@ struct KPrecip {
         const bool col = g < (long)G.X * G.Z;
         const long gc = col ? g : 0;
         float* q = pbt.q[fi];
+        const Store st = pbt.st[fi];
         const int type = pbt.type[fi];
         const int x = (int)(gc % G.X), zc = (int)(gc / G.X);
         const float kap = kc::Rsd / kc::Cpd;
@@ -246,7 +256,7 @@ struct KPrecip {
         const int ylo = (int)threadIdx.y * yc, yhi = min(G.Y, ylo + yc);
         const int ya = max(ylo, y0);
         const bool work = col && ya < yhi;
-        const float qTop = work && yhi < G.Y ? q[gidx(G, x, yhi, zc)] : 0.0f;
+        const float qTop = work && yhi < G.Y ? ldS<MIX>(q, gidx(G, x, yhi, zc), st) : 0.0f;
         __syncthreads();
         if (!work) return;
         // per level: Q, theta, p, qv; levels y+1 and y+2 are loaded ahead
@@ -264,7 +274,7 @@ struct KPrecip {
         auto load = [&](int y, Lv& L) {
             if (y <= yhi && y < G.Y) {
                 const long j = i0 + (long)(y - ya) * G.X;
-                L.q = y == yhi ? qTop : q[j];
+                L.q = y == yhi ? qTop : ldS<MIX>(q, j, st);
                 L.t = th[j];
                 L.pr = p[j];
                 L.v = qv[j];
@@ -283,7 +293,7 @@ struct KPrecip {
             float nq = Qc;
             nq -= dts * fminf((fv / dx) * Qc, Qc);
             if (up) nq += dts * fminf((fvu / dx) * Qu, Qu);
-            q[i] = nq;
+            stS<MIX>(q, i, nq, st);
             fv = fvu;
             l0 = l1;
             l1 = l2;
@@ -291,8 +301,12 @@ struct KPrecip {
     }
 };
 hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt) {
-    return klaunch<KPrecip>(e, dim3((unsigned)((e.G + 63) / 64), pbt.n), dim3(64, PRECIP_CHUNKS), 0, geom(e), pbt,
-                            e.f[SKT_POTTEMP], e.f[SKT_PRESSURE], e.f[SKT_QV], e.f[SKT_GROUND_P], e.dx, e.gm);
+    const dim3 grid((unsigned)((e.G + 63) / 64), pbt.n), block(64, PRECIP_CHUNKS);
+    if (e.mix)
+        return klaunch<KPrecip<true>>(e, grid, block, 0, geom(e), pbt, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE],
+                                      e.f[SKT_QV], e.f[SKT_GROUND_P], e.dx, e.gm);
+    return klaunch<KPrecip<false>>(e, grid, block, 0, geom(e), pbt, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE],
+                                   e.f[SKT_QV], e.f[SKT_GROUND_P], e.dx, e.gm);
 }
 
 // -------------------------------------------------------------- buoyancy ---
@@ -308,10 +322,12 @@ hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt) {
 // cell inside the domain is sky. Blocks are 64 (x) x 4 (z) columns of one
 // chunk; a chunk that starts above the ground reads the level below it.
 constexpr int YC_BUOY = 16;
+template <bool MIX>
 struct KBuoyancy {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, float* __restrict__ v, const float* __restrict__ th,
                        const float* __restrict__ qv, const float* __restrict__ Qr, const float* __restrict__ Qs,
-                       const float* __restrict__ Qi, const float* __restrict__ p, const float* __restrict__ gP, float dt) {
+                       const float* __restrict__ Qi, const float* __restrict__ p, const float* __restrict__ gP, float dt,
+                                               const Store3& sq) {
         const unsigned lin = vb.id;
         const int chunk = (int)(lin % nyc);
         const unsigned tile = lin / nyc;
@@ -384,7 +400,7 @@ struct KBuoyancy {
                     }
                 }
                 float B = 0.0f, BU = 0.0f, BD = 0.0f;
-                const float totalQ = Qr[i] + Qs[i] + Qi[i];
+                const float totalQ = ldS<MIX>(Qr, i, sq.s[0]) + ldS<MIX>(Qs, i, sq.s[1]) + ldS<MIX>(Qi, i, sq.s[2]);
                 if (down) {
                     const float VTP = TadD * (1.0f + 0.608f * QP), VTE = TenvDown * (1.0f + 0.608f * QenvDown);
                     BD = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
@@ -412,8 +428,14 @@ struct KBuoyancy {
 hipError_t launch_buoyancy(EnvState& e, float dt) {
     const unsigned nyc = (unsigned)((e.Y + YC_BUOY - 1) / YC_BUOY);
     const unsigned nb = (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * nyc;
-    return klaunch<KBuoyancy>(e, dim3(nb), B3, 0, geom(e), YC_BUOY, nyc, e.f[SKT_VELY], e.f[SKT_POTTEMP], e.f[SKT_QV],
-                              e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], dt);
+    const Store3 sq{{e.fst[SKT_QR], e.fst[SKT_QS], e.fst[SKT_QI]}};
+    if (e.mix)
+        return klaunch<KBuoyancy<true>>(e, dim3(nb), B3, 0, geom(e), YC_BUOY, nyc, e.f[SKT_VELY], e.f[SKT_POTTEMP],
+                                        e.f[SKT_QV], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_PRESSURE],
+                                        e.f[SKT_GROUND_P], dt, sq);
+    return klaunch<KBuoyancy<false>>(e, dim3(nb), B3, 0, geom(e), YC_BUOY, nyc, e.f[SKT_VELY], e.f[SKT_POTTEMP],
+                                     e.f[SKT_QV], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_PRESSURE],
+                                     e.f[SKT_GROUND_P], dt, sq);
 }
 
 // ------------------------------------------------------------------- EOS ---
@@ -485,17 +507,18 @@ hipError_t launch_reset_vel(EnvState& e) {
 // k_reset_vel). Without a profile, setToValue(0) (Game/kernelMaths.cu:12-23)
 // writes every cell, ground included.
 struct KResetParam {
-    static __device__ __forceinline__ void run(VB vb, const Geom& G, const int* __restrict__ ghz, float* __restrict__ a, const float* __restrict__ prof) {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const int* __restrict__ ghz, float* __restrict__ a,
+                                               const float* __restrict__ prof, Store st) {
         CELL_SETUP_V;
         (void)bid;
         if (!inb) return;
         const long i = gidx(G, x, y, z);
-        if (!prof) a[i] = 0.0f;
-        else if (y > ghz[x + (long)G.X * z]) a[i] = prof[y];
+        if (!prof) stS<true>(a, i, 0.0f, st);
+        else if (y > ghz[x + (long)G.X * z]) stS<true>(a, i, prof[y], st);
     }
 };
-hipError_t launch_reset_param(EnvState& e, float* a, const float* prof) {
-    return klaunch<KResetParam>(e, grid3(e), B3, 0, geom(e), e.ghz, a, prof);
+hipError_t launch_reset_param(EnvState& e, int field, const float* prof) {
+    return klaunch<KResetParam>(e, grid3(e), B3, 0, geom(e), e.ghz, e.f[field], prof, e.fst[field]);
 }
 
 // ---------------------------------------------------------- microphysics ---
@@ -505,11 +528,13 @@ struct MicroArgs {
     uint8_t* phase;
     double* region;  // [33 * nblocks] partials or null
     int rmin[3], rmax[3];
+    Store sQw, sQc, sQr, sQs, sQi;  // storage of the hydrometeor fields
 };
 
 // calculateEnvMicroPhysicsGPU (Game/microPhysics.cu:1035-1417) fused with
 // addHeatGPU (Game/kernelSky.cu:1249-1282): the condens/depos/freeze arrays
 // the reference passes between them are registers here.
+template <bool MIX>
 struct KMicrophysics {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, const MicroArgs& a, float dt, float speed, const Gammas& gm) {
         CELL_SETUP_V;
@@ -518,12 +543,15 @@ struct KMicrophysics {
         bool inRegion = false;
         if (inb && !isGround(G, x, y, z)) {
             const long i = gidx(G, x, y, z);
-            const float Qv = a.Qv[i] > 1e-18f ? a.Qv[i] : 0.0f;
-            const float Qw = a.Qw[i] > 1e-18f ? a.Qw[i] : 0.0f;
-            const float Qc = a.Qc[i] > 1e-18f ? a.Qc[i] : 0.0f;
-            const float Qr = a.Qr[i] > 1e-18f ? a.Qr[i] : 0.0f;
-            const float Qs = a.Qs[i] > 1e-18f ? a.Qs[i] : 0.0f;
-            const float Qi = a.Qi[i] > 1e-18f ? a.Qi[i] : 0.0f;
+            const float qv0 = a.Qv[i], qw0 = ldS<MIX>(a.Qw, i, a.sQw), qc0 = ldS<MIX>(a.Qc, i, a.sQc);
+            const float qr0 = ldS<MIX>(a.Qr, i, a.sQr), qs0 = ldS<MIX>(a.Qs, i, a.sQs);
+            const float qi0 = ldS<MIX>(a.Qi, i, a.sQi);
+            const float Qv = qv0 > 1e-18f ? qv0 : 0.0f;
+            const float Qw = qw0 > 1e-18f ? qw0 : 0.0f;
+            const float Qc = qc0 > 1e-18f ? qc0 : 0.0f;
+            const float Qr = qr0 > 1e-18f ? qr0 : 0.0f;
+            const float Qs = qs0 > 1e-18f ? qs0 : 0.0f;
+            const float Qi = qi0 > 1e-18f ? qi0 : 0.0f;
             const float QwMin = 0.001f, QcMin = 0.001f, QiMin = 0.0006f;
             const float ps = a.p[i];
             float3 fv = make_float3(0.0f, 0.0f, 0.0f);
@@ -578,7 +606,7 @@ struct KMicrophysics {
             else if (PVCON < 0) { PVCON = dt * fmaxf(-Qw, speed * PVCON); cond += PVCON; }
             if (PVDEP >= 0) { PVDEP = dt * fminf(Qv, speed * PVDEP); dep += PVDEP; }
             else if (PVDEP < 0) { PVDEP = dt * fmaxf(-Qc, speed * PVDEP); dep += PVDEP; }
-            float nQv = a.Qv[i], nQw = a.Qw[i], nQc = a.Qc[i], nQr = a.Qr[i], nQs = a.Qs[i], nQi = a.Qi[i];
+            float nQv = qv0, nQw = qw0, nQc = qc0, nQr = qr0, nQs = qs0, nQi = qi0;
             if (T < 0) {
                 PIDW = dt * fminf(Qw, speed * PIDW); dep += PIDW;
                 PIHOM = dt * fminf(Qw, speed * PIHOM); frz += PIHOM;
@@ -632,7 +660,12 @@ struct KMicrophysics {
                 nQs += -PSMLT - PGACS;
                 nQi += -PGMLT + PGACS;
             }
-            a.Qv[i] = nQv; a.Qw[i] = nQw; a.Qc[i] = nQc; a.Qr[i] = nQr; a.Qs[i] = nQs; a.Qi[i] = nQi;
+            a.Qv[i] = nQv;
+            stS<MIX>(a.Qw, i, nQw, a.sQw);
+            stS<MIX>(a.Qc, i, nQc, a.sQc);
+            stS<MIX>(a.Qr, i, nQr, a.sQr);
+            stS<MIX>(a.Qs, i, nQs, a.sQs);
+            stS<MIX>(a.Qi, i, nQi, a.sQi);
             // addHeatGPU with the updated vapour
             const float Mair = 0.02896f, Mwater = 0.01802f;
             const float XV = (nQv / Mwater) / ((nQv / Mwater) + (1 - nQv) / Mair);
@@ -676,8 +709,10 @@ hipError_t launch_microphysics(EnvState& e, float dt, float speed) {
     MicroArgs a{e.f[SKT_QV], e.f[SKT_QW], e.f[SKT_QC], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_POTTEMP],
                 e.f[SKT_DENSITY], e.f[SKT_PRESSURE], e.f[SKT_GROUND_P], e.phase,
                 e.region_active ? e.region : nullptr, {e.rmin[0], e.rmin[1], e.rmin[2]},
-                {e.rmax[0], e.rmax[1], e.rmax[2]}};
-    return klaunch<KMicrophysics>(e, grid3(e), B3, 0, geom(e), a, dt, speed, e.gm);
+                {e.rmax[0], e.rmax[1], e.rmax[2]}, e.fst[SKT_QW], e.fst[SKT_QC], e.fst[SKT_QR], e.fst[SKT_QS],
+                e.fst[SKT_QI]};
+    if (e.mix) return klaunch<KMicrophysics<true>>(e, grid3(e), B3, 0, geom(e), a, dt, speed, e.gm);
+    return klaunch<KMicrophysics<false>>(e, grid3(e), B3, 0, geom(e), a, dt, speed, e.gm);
 }
 // Fixed-order sum of the region partials into 33 floats at the tail of the
 // region buffer.
@@ -720,6 +755,7 @@ hipError_t launch_masks(EnvState& e, uint8_t* dev_out) {
 }
 struct Ten {
     float* a[10];
+    Store st[10];
 };
 // resetValueInGround over n fields (Game/kernelGround.cu:21-36)
 struct KResetGround {
@@ -728,12 +764,17 @@ struct KResetGround {
         (void)bid;
         if (!inb || !isGround(G, x, y, z)) return;
         const long i = gidx(G, x, y, z);
-        for (int k = 0; k < n; k++) t.a[k][i] = 0.0f;
+        for (int k = 0; k < n; k++) stS<true>(t.a[k], i, 0.0f, t.st[k]);
     }
 };
 hipError_t launch_editor_reset(EnvState& e, int nfields) {
-    Ten t{{e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_QW], e.f[SKT_QC], e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI],
-           e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ]}};
+    static const int ids[10] = {SKT_POTTEMP, SKT_QV,   SKT_QW,   SKT_QC,   SKT_QR,
+                                SKT_QS,      SKT_QI,   SKT_VELX, SKT_VELY, SKT_VELZ};
+    Ten t;
+    for (int k = 0; k < 10; k++) {
+        t.a[k] = e.f[ids[k]];
+        t.st[k] = e.fst[ids[k]];
+    }
     return klaunch<KResetGround>(e, grid3(e), B3, 0, geom(e), t, nfields);
 }
 // computeIsenTempGroundGPU, Game/kernelGround.cu:38-47
@@ -822,9 +863,10 @@ hipError_t launch_sounding_fill(EnvState& e, const float* lv, float groundT, flo
 // cells, the highest level whose QW + QC exceeds thr at a sky cell, and, in
 // the blocks whose tile holds level 0, the ground rain water QGR of its
 // columns (FP64). Each ground column lies in exactly one such block.
+template <bool MIX>
 struct KDiag {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, const float* __restrict__ v, const float* __restrict__ qw,
-                   const float* __restrict__ qc, const float* __restrict__ qgr, float thr, double* pw, double* pl,
+                   const float* __restrict__ qc, Store sw, Store sc, const float* __restrict__ qgr, float thr, double* pw, double* pl,
                    double* pr) {
         ZM_SETUP_V;
         __shared__ double sh[4];
@@ -836,7 +878,7 @@ struct KDiag {
                 const long i = gidx(G, x, y, z);
                 if (!isGround(G, x, y, z)) {
                     w = fmaxf(w, v[i]);
-                    if (qw[i] + qc[i] > thr) lev = y;  // y is fixed per lane
+                    if (ldS<MIX>(qw, i, sw) + ldS<MIX>(qc, i, sc) > thr) lev = y;  // y is fixed per lane
                 }
                 if (y == 0) rain += (double)qgr[x + (long)G.X * z];
             }
@@ -876,8 +918,13 @@ hipError_t launch_diagnostics(EnvState& e, float thr) {
     const ZGrid zg = zgrid(e);
     const size_t M = e.maxBlocks;
     double *pw = e.partials + PART_DG_W * M, *pl = e.partials + PART_DG_L * M, *pr = e.partials + PART_DG_R * M;
-    hipError_t err = klaunch<KDiag>(e, zg.grid, B3, 0, geom(e), zg.zc, e.f[SKT_VELY], e.f[SKT_QW], e.f[SKT_QC],
-                                    e.f[SKT_QGR], thr, pw, pl, pr);
+    hipError_t err =
+        e.mix ? klaunch<KDiag<true>>(e, zg.grid, B3, 0, geom(e), zg.zc, (const float*)e.f[SKT_VELY],
+                                     (const float*)e.f[SKT_QW], (const float*)e.f[SKT_QC], e.fst[SKT_QW],
+                                     e.fst[SKT_QC], (const float*)e.f[SKT_QGR], thr, pw, pl, pr)
+              : klaunch<KDiag<false>>(e, zg.grid, B3, 0, geom(e), zg.zc, (const float*)e.f[SKT_VELY],
+                                      (const float*)e.f[SKT_QW], (const float*)e.f[SKT_QC], e.fst[SKT_QW],
+                                      e.fst[SKT_QC], (const float*)e.f[SKT_QGR], thr, pw, pl, pr);
     if (err != hipSuccess) return err;
     return klaunch<KDiagFinalize>(e, dim3(1), B3, 0, pw, pl, pr, zg.nblocks, e.sc);
 }
@@ -1006,8 +1053,8 @@ __device__ __forceinline__ void applyEditCell(const Geom& G, const EditArgs& a, 
             a.a[2][i] = v3;
         }
     } else {
-        if (add) a.a[0][i] += v1;
-        else a.a[0][i] = v1;
+        // a hydrometeor may be stored in 16 bits
+        stS<true>(a.a[0], i, add ? ldS<true>(a.a[0], i, a.st[0]) + v1 : v1, a.st[0]);
     }
 }
 // applyBrushGPU (Game/kernelSky.cu:1476-1556): thread (x, y) of the
@@ -1051,7 +1098,7 @@ KL void k_apply_selection(Geom G, EditArgs a, int nx, int ny, int* __restrict__ 
 KL void k_reset_uncovered(Geom G, const int* __restrict__ ghNew, const float* __restrict__ isT,
                           const float* __restrict__ isV, const float* __restrict__ defP, float* Qv, float* Qw,
                           float* Qc, float* Qr, float* Qs, float* Qi, float* th, float* u, float* v, float* w,
-                          float* p) {
+                          float* p, Store sw, Store sc, Store sr, Store ss, Store si) {
     CELL_SETUP;
     (void)bid;
     if (!inb) return;
@@ -1059,11 +1106,11 @@ KL void k_reset_uncovered(Geom G, const int* __restrict__ ghNew, const float* __
     if (y > G.gh[g] || y <= ghNew[g]) return;
     const long i = gidx(G, x, y, z);
     Qv[i] = isV[y];
-    Qw[i] = 0.0f;
-    Qc[i] = 0.0f;
-    Qr[i] = 0.0f;
-    Qs[i] = 0.0f;
-    Qi[i] = 0.0f;
+    stS<true>(Qw, i, 0.0f, sw);
+    stS<true>(Qc, i, 0.0f, sc);
+    stS<true>(Qr, i, 0.0f, sr);
+    stS<true>(Qs, i, 0.0f, ss);
+    stS<true>(Qi, i, 0.0f, si);
     th[i] = isT[y];
     u[i] = 0.0f;
     v[i] = 0.0f;
@@ -1089,7 +1136,8 @@ hipError_t launch_reset_uncovered(EnvState& e) {
     hipLaunchKernelGGL(k_reset_uncovered, grid3(e), B3, 0, e.stream, geom(e), e.ghEdit, e.f[SKT_ISEN_TEMP],
                        e.f[SKT_ISEN_VAPOR], e.f[SKT_DEFAULT_PRESSURE], e.f[SKT_QV], e.f[SKT_QW], e.f[SKT_QC],
                        e.f[SKT_QR], e.f[SKT_QS], e.f[SKT_QI], e.f[SKT_POTTEMP], e.f[SKT_VELX], e.f[SKT_VELY],
-                       e.f[SKT_VELZ], e.f[SKT_PRESSURE]);
+                       e.f[SKT_VELZ], e.f[SKT_PRESSURE], e.fst[SKT_QW], e.fst[SKT_QC], e.fst[SKT_QR],
+                       e.fst[SKT_QS], e.fst[SKT_QI]);
     return hipGetLastError();
 }
 

@@ -26,7 +26,14 @@ def default_config(nx, ny, nz, dx=100.0, **overrides):
     cfg = Config()
     lib.skt_config_default(ctypes.byref(cfg), nx, ny, nz, dx)
     for k, v in overrides.items():
-        setattr(cfg, k, v)
+        if isinstance(v, (list, tuple)):
+            arr = getattr(cfg, k)
+            if len(v) != len(arr):
+                raise SktError(f"config {k}: expected {len(arr)} values")
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(cfg, k, v)
     return cfg
 
 
@@ -291,6 +298,16 @@ class EnvironmentGPU:
         if a.size != self.field_len(field):
             raise SktError(f"set {field}: expected {self.field_len(field)} values, got {a.size}")
         self._chk(self.lib.skt_env_copy_field_from_host(self.h, FIELD[field], _ptr(a)), f"set {field}")
+
+    FORMATS = {0: "f32", 1: "bf16", 2: "f16"}
+
+    def field_format(self, field):
+        """(storage, scale) of a field: ("f32", 1.0), or for a hydrometeor of
+        the mixed form ("bf16" | "f16", 2^k) -- the device holds value * scale."""
+        f, sc = ctypes.c_int32(), ctypes.c_float()
+        self._chk(self.lib.skt_env_get_field_format(self.h, FIELD[field], ctypes.byref(f), ctypes.byref(sc)),
+                  "field_format")
+        return self.FORMATS[f.value], sc.value
 
     def device_ptr(self, field):
         p, n = ctypes.c_void_p(), ctypes.c_int64()
