@@ -512,6 +512,11 @@ __device__ __forceinline__ bool ppmPassDt(const PpmBatch& pb, float& dt) {
 constexpr int XSEG_PPM = 2;
 template <bool MIX>
 struct KPpmXr {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        float dt = 0.0f;
+        return !ppmPassDt(nthArg<3>(a...), dt);  // pb
+    }
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, unsigned nbf, const PpmBatch& pb,
                                                float dtArg, float dx, const float* __restrict__ cust) {
         float dt = dtArg;
@@ -617,6 +622,11 @@ constexpr int YC_PPM = 16;
 constexpr int YG_PPM = 4;
 template <bool MIX>
 struct KPpmYm {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        float dt = 0.0f;
+        return !ppmPassDt(nthArg<4>(a...), dt);  // pb
+    }
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, unsigned nbf,
                                                const PpmBatch& pb, float dtArg, float dx) {
         float dt = dtArg;
@@ -707,6 +717,11 @@ constexpr int ZB_PPM = 4;  // planes of the z pass whose loads go together
 // boundary rule needs no extra loads.
 template <int ZB, bool MIX>
 struct KPpmZ {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        float dt = 0.0f;
+        return !ppmPassDt(nthArg<3>(a...), dt);  // pb
+    }
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, unsigned nbf, const PpmBatch& pb,
                                                float dtArg, float dx, const float* __restrict__ cust) {
         float dt = dtArg;
@@ -1173,6 +1188,11 @@ hipError_t launch_pcg_start(EnvState& e, int pass) {
 // launch keeps (chunk + 3 planes) below 4 GiB.
 template <bool FIRST, bool TRUNC>
 struct KPcgA {
+    // a finished solve (sin: argument 11)
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        return nthArg<11>(a...)->done != 0;
+    }
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, const float* __restrict__ zz, const float* __restrict__ s_old,
                     float* __restrict__ s_new, const float* __restrict__ irho, float* __restrict__ As,
                     const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, double* pout,
@@ -1402,6 +1422,10 @@ ResidentBlocks resident_blocks(int device) {
 // the loop handles one group per slot per trip).
 template <int V>
 struct KPcgB {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        return nthArg<12>(a...)->done != 0;  // sin
+    }
     static __device__ __forceinline__ void run(VB vb, const Geom& G, unsigned rpb, float* __restrict__ pp, float* __restrict__ r, const float* __restrict__ s,
                     const float* __restrict__ As, const float* __restrict__ pre, float* __restrict__ zz,
                     const double* __restrict__ pin, unsigned n, double* pmax, double* psum, const PcgState* sin,

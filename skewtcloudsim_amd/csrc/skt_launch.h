@@ -20,6 +20,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 #include "skt_internal.h"
 
@@ -47,6 +48,32 @@ inline void pfill(Pack<H, T...>& p, const H& h, const T&... t) {
     pfill(p.t, t...);
 }
 
+// Argument I of a body's argument list (for idle predicates).
+template <size_t I, class H, class... T>
+__host__ __device__ __forceinline__ const auto& nthArg(const H& h, const T&... t) {
+    if constexpr (I == 0) return h;
+    else return nthArg<I - 1>(t...);
+}
+
+// A body may declare  template <class... T> static bool idle(const T&...)
+// over its run arguments (without vb): true when every tile of the launch
+// except tile 0 returns without effect (a finished solve, an advection pass
+// past the member's substep count). The ensemble launches then skip a
+// member's other tiles without running them.
+template <class K, class P, class = void>
+struct HasIdle : std::false_type {};
+template <class K, class... A>
+struct HasIdle<K, Pack<A...>, std::void_t<decltype(K::idle(std::declval<const A&>()...))>> : std::true_type {};
+template <class K, class... A, size_t... I>
+__device__ __forceinline__ bool idlePacked(const Pack<A...>& p, std::index_sequence<I...>) {
+    return K::idle(pget<I>(p)...);
+}
+template <class K, class... A>
+__device__ __forceinline__ bool memberIdle(const Pack<A...>& p, unsigned local) {
+    if constexpr (HasIdle<K, Pack<A...>>::value) return local != 0 && idlePacked<K>(p, std::index_sequence_for<A...>{});
+    else return false;
+}
+
 template <class K, class... A>
 __global__ __launch_bounds__(256) void k_solo(A... a) {
     K::run(VB{blockIdx.x, gridDim.x}, a...);
@@ -63,9 +90,12 @@ __device__ __forceinline__ void runPacked(VB vb, const Pack<A...>& p, std::index
 // it contains (or contains none after a per-thread return), and the barrier
 // at the end of each item keeps a block's next tile from overwriting LDS a
 // wave of the previous one still reads.
+// An idle member's remaining items are skipped in one move: to the block's
+// first work index past the member's range (no tile of it ran, so no
+// barrier is needed).
 template <class K, class... A>
 __global__ __launch_bounds__(256) void k_batch(const Pack<A...>* __restrict__ args, WorkMap wm) {
-    for (unsigned w = blockIdx.x; w < wm.total; w += gridDim.x) {
+    for (unsigned w = blockIdx.x; w < wm.total;) {
         int lo = 0, hi = wm.M - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -77,8 +107,16 @@ __global__ __launch_bounds__(256) void k_batch(const Pack<A...>* __restrict__ ar
         lo = __builtin_amdgcn_readfirstlane(lo);
         const unsigned local = w - wm.start[lo];
         const unsigned nb = wm.nb[lo];
-        if (local < nb) runPacked<K>(VB{local, nb}, args[lo], std::index_sequence_for<A...>{});
+        if (local < nb) {
+            if (memberIdle<K>(args[lo], local)) {
+                const unsigned end = wm.start[lo] + nb, g = gridDim.x;
+                w += (end - w + g - 1) / g * g;
+                continue;
+            }
+            runPacked<K>(VB{local, nb}, args[lo], std::index_sequence_for<A...>{});
+        }
         __syncthreads();
+        w += gridDim.x;
     }
 }
 
@@ -95,7 +133,8 @@ __global__ __launch_bounds__(256) void k_batch1(const Pack<A...>* __restrict__ a
     lo = __builtin_amdgcn_readfirstlane(lo);
     const unsigned local = w - wm.start[lo];
     const unsigned nb = wm.nb[lo];
-    if (local < nb) runPacked<K>(VB{local, nb}, args[lo], std::index_sequence_for<A...>{});
+    if (local < nb && !memberIdle<K>(args[lo], local))
+        runPacked<K>(VB{local, nb}, args[lo], std::index_sequence_for<A...>{});
 }
 
 template <class K, class... A>

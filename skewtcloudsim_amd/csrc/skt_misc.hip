@@ -234,6 +234,13 @@ constexpr int PRECIP_CHUNKS = 4;
 static_assert(64 * PRECIP_CHUNKS <= 256, "k_precip blocks (64 x PRECIP_CHUNKS) must fit the 256-thread launch bound");
 template <bool MIX>
 struct KPrecip {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        const PrecipBatch& pbt = nthArg<1>(a...);
+        for (int f = 0; f < pbt.n; f++)
+            if (pbt.dts[f] != 0.0f) return false;
+        return true;
+    }
     static __device__ __forceinline__ void run(VB vb, const Geom& G, const PrecipBatch& pbt, const float* __restrict__ th, const float* __restrict__ p,
                      const float* __restrict__ qv, const float* __restrict__ gP, float dx, const Gammas& gm) {
         // tiles: column blocks of 64 (fastest), then the batch's fields
