@@ -137,6 +137,12 @@ typedef struct skt_step_stats {
 
 /* environmentGPU::environmentGPU (Game/environment.cu:71-185). */
 int skt_env_create(const skt_config* cfg, skt_env** out);
+/* HBM bytes an env of this configuration allocates (the sum of its device
+ * arrays), computed without allocating; skt_env_device_bytes gives the same
+ * figure for a live env. Used to admit jobs by footprint. No counterpart in
+ * the reference, which sizes its arrays at compile time (Game/config.h:3-9). */
+int skt_env_footprint(const skt_config* cfg, int64_t* bytes);
+int skt_env_device_bytes(skt_env* env, int64_t* bytes);
 /* environmentGPU::~environmentGPU (Game/environment.cu:187-247). */
 int skt_env_destroy(skt_env* env);
 const char* skt_env_last_error(const skt_env* env);
@@ -304,6 +310,7 @@ int skt_batch_synchronize(skt_batch* b);
  * set of member shapes (used by every later step of such an ensemble). */
 int skt_batch_autotune(skt_batch* b, float dt, float speed, skt_batch_tuning* out);
 
+
 /* Serving diagnostics reduced on the device (maximum updraft, cloud-top level
  * and height, total ground rain) with cloud meaning QW + QC > cloud_threshold
  * (kg/kg; 1e-5 is the usual choice). No reference counterpart: it replaces the
@@ -422,6 +429,70 @@ int skt_sounding_init_fields(const skt_sounding_table* table, int32_t nx, int32_
  * with no host copy of the 3-D fields. Same state as skt_env_init on the
  * skt_sounding_init_fields arrays. */
 int skt_env_init_sounding(skt_env* env, const skt_sounding_table* table);
+
+/* ---- Serving: jobs of 'simulate sounding S for T seconds' on one GPU. ----
+ * No counterpart in the reference (an interactive single-domain program).
+ * A server owns an ensemble (skt_batch). skt_server_submit queues a job;
+ * each skt_server_step admits queued jobs in arrival order while their HBM
+ * footprint (skt_env_footprint) fits the budget and a slot is free, builds
+ * each admitted job's initial state on the device (skt_sounding_generate +
+ * init), advances every active job by one ensemble step, reduces the
+ * diagnostics of the jobs whose simulated time reached a report boundary
+ * (skt_env_diagnostics; one device wait for all of them), and retires jobs
+ * that have run their T seconds, freeing their memory for the queue. Every
+ * job runs the step of its own configuration (the server's member_config
+ * with the job's grid): its reports are those of the same job run alone,
+ * bit for bit, whatever else shares the ensemble. Not thread-safe. */
+typedef struct skt_server skt_server;
+typedef struct skt_server_options {
+    int32_t device;
+    float dt;                     /* model step of every job, s (0.5) */
+    int64_t hbm_budget_bytes;     /* bytes all active jobs may hold; 0: 90 % of the free device memory */
+    int32_t max_active;           /* slots: jobs in flight at most (64) */
+    float cloud_threshold;        /* QW + QC above which a cell is cloud in the reports (1e-5) */
+    skt_config member_config;     /* solver and precision settings of every job (the grid is the job's) */
+    skt_batch_options batch;      /* options of the ensemble */
+} skt_server_options;
+typedef struct skt_job_request {
+    int32_t nx, ny, nz;           /* grid of the job (ny vertical) */
+    float dx;                     /* voxel size, m */
+    skt_sounding_spec sounding;   /* initial state: skt_sounding_generate of this spec */
+    float sim_seconds;            /* T: simulated time to run */
+    float report_every;           /* simulated seconds between reports (60); the last report is at T */
+} skt_job_request;
+enum { SKT_EVENT_ADMITTED = 0, SKT_EVENT_REPORT = 1, SKT_EVENT_DONE = 2 };
+typedef struct skt_job_event {
+    int64_t job;
+    int32_t kind;                 /* SKT_EVENT_* */
+    int32_t steps;                /* steps the job has run */
+    float sim_time;               /* simulated seconds the job has run */
+    int32_t pad;
+    double wall_s;                /* server clock (s since skt_server_create) at the event */
+    double submitted_s;           /* server clock when the job was submitted */
+    skt_diagnostics diag;         /* for SKT_EVENT_REPORT and SKT_EVENT_DONE */
+} skt_job_event;
+typedef struct skt_server_status {
+    int32_t active, queued;       /* jobs in flight and waiting */
+    int64_t bytes_in_use;         /* footprints of the active jobs */
+    int64_t budget_bytes;
+    int64_t steps;                /* ensemble steps run */
+    int64_t completed;            /* jobs retired */
+    double cell_steps;            /* cells x steps advanced over all jobs */
+    double busy_s;                /* wall seconds spent inside skt_server_step */
+} skt_server_status;
+void skt_server_options_default(skt_server_options* o);
+int skt_server_create(const skt_server_options* o, skt_server** out);
+int skt_server_destroy(skt_server* s);
+const char* skt_server_last_error(const skt_server* s);
+/* Queues a job; job_id (out) names it in its events. SKT_ERR_ARG when the
+ * job can never run (its grid is invalid or its footprint exceeds the budget). */
+int skt_server_submit(skt_server* s, const skt_job_request* req, int64_t* job_id);
+/* Admission, one ensemble step of the active jobs, reports and retirement
+ * (see above). With no active job it only admits. */
+int skt_server_step(skt_server* s);
+/* Moves up to 'capacity' pending events, oldest first, into out; count (out). */
+int skt_server_poll(skt_server* s, skt_job_event* out, int32_t capacity, int32_t* count);
+int skt_server_status_get(skt_server* s, skt_server_status* out);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,8 @@ gfx950 in libskt.so; this package is the Python mirror of its C ABI
 from ._lib import FIELD, FIELDS, NEUMANN, DIRICHLET, CUSTOM, PCG_REFERENCE, PCG_STANDARD, load_library
 from .environment import (EnsembleGPU, EnvironmentGPU, ReadTable, SktError, default_config, make_sounding,
                           terrain)
+from .serving import Server, job_footprint, run_poisson
 
 __all__ = ["FIELD", "FIELDS", "NEUMANN", "DIRICHLET", "CUSTOM", "PCG_REFERENCE", "PCG_STANDARD", "load_library",
-           "EnsembleGPU", "EnvironmentGPU", "ReadTable", "SktError", "default_config", "make_sounding", "terrain"]
+           "EnsembleGPU", "EnvironmentGPU", "ReadTable", "SktError", "default_config", "make_sounding", "terrain",
+           "Server", "job_footprint", "run_poisson"]

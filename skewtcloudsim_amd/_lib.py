@@ -102,6 +102,31 @@ class BatchTuning(ctypes.Structure):
                 ("persistent_bodies", ctypes.c_int32)]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("dt", ctypes.c_float), ("hbm_budget_bytes", ctypes.c_int64),
+                ("max_active", ctypes.c_int32), ("cloud_threshold", ctypes.c_float), ("member_config", Config),
+                ("batch", BatchOptions)]
+
+
+class JobRequest(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("dx", ctypes.c_float),
+                ("sounding", SoundingSpec), ("sim_seconds", ctypes.c_float), ("report_every", ctypes.c_float)]
+
+
+class JobEvent(ctypes.Structure):
+    _fields_ = [("job", ctypes.c_int64), ("kind", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("sim_time", ctypes.c_float), ("pad", ctypes.c_int32), ("wall_s", ctypes.c_double),
+                ("submitted_s", ctypes.c_double), ("diag", Diagnostics)]
+
+
+class ServerStatus(ctypes.Structure):
+    _fields_ = [("active", ctypes.c_int32), ("queued", ctypes.c_int32), ("bytes_in_use", ctypes.c_int64),
+                ("budget_bytes", ctypes.c_int64), ("steps", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("cell_steps", ctypes.c_double), ("busy_s", ctypes.c_double)]
+
+
+EVENT_KINDS = {0: "admitted", 1: "report", 2: "done"}
+
 _P = ctypes.c_void_p
 _F = ctypes.c_float
 _I = ctypes.c_int32
@@ -179,6 +204,16 @@ SIGNATURES = {
     "skt_batch_step": (_I, [_P, _F, _F]),
     "skt_batch_synchronize": (_I, [_P]),
     "skt_batch_autotune": (_I, [_P, _F, _F, ctypes.POINTER(BatchTuning)]),
+    "skt_env_footprint": (_I, [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_int64)]),
+    "skt_env_device_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "skt_server_options_default": (None, [ctypes.POINTER(ServerOptions)]),
+    "skt_server_create": (_I, [ctypes.POINTER(ServerOptions), ctypes.POINTER(_P)]),
+    "skt_server_destroy": (_I, [_P]),
+    "skt_server_last_error": (ctypes.c_char_p, [_P]),
+    "skt_server_submit": (_I, [_P, ctypes.POINTER(JobRequest), ctypes.POINTER(ctypes.c_int64)]),
+    "skt_server_step": (_I, [_P]),
+    "skt_server_poll": (_I, [_P, ctypes.POINTER(JobEvent), _I, ctypes.POINTER(_I)]),
+    "skt_server_status_get": (_I, [_P, ctypes.POINTER(ServerStatus)]),
 }
 
 _lib = None

@@ -537,8 +537,15 @@ void freeAll(EnvState& e) {
     if (e.own_stream) (void)hipStreamDestroy(e.own_stream);
 }
 
+// Device allocation of an env's array; in a dry run (footprint sizing) only
+// the bytes are counted.
 template <class T>
 int dalloc(EnvState& e, T** p, size_t bytes) {
+    e.dev_bytes += (long long)bytes;
+    if (e.dry) {
+        *p = nullptr;
+        return SKT_OK;
+    }
     hipError_t r = hipMalloc((void**)p, bytes);
     if (r != hipSuccess) return setErr(e, SKT_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(r));
     return SKT_OK;
@@ -571,30 +578,30 @@ void skt_config_default(skt_config* c, int32_t nx, int32_t ny, int32_t nz, float
     c->pcg_refine_tol = 1e-3f;
 }
 
-int skt_env_create(const skt_config* cfg, skt_env** out) {
-    if (!cfg || !out) return SKT_ERR_ARG;
-    *out = nullptr;
-    if (cfg->nx < 1 || cfg->ny < 2 || cfg->nz < 1 || !(cfg->dx > 0.0f)) return SKT_ERR_ARG;
+// Checks of a configuration (skt_env_create's SKT_ERR_ARG cases).
+static bool configOk(const skt_config* cfg) {
+    if (cfg->nx < 1 || cfg->ny < 2 || cfg->nz < 1 || !(cfg->dx > 0.0f)) return false;
     const long N = (long)cfg->nx * cfg->ny * cfg->nz;
     if (N > (1L << 31) - 1 || cfg->diffusion_sweeps < 0 || cfg->pcg_max_iter < 0 || cfg->max_substeps < 0 ||
         cfg->max_blocks < 0)
-        return SKT_ERR_ARG;
-    if (cfg->tracer_format < SKT_FMT_F32 || cfg->tracer_format > SKT_FMT_F16) return SKT_ERR_ARG;
+        return false;
+    if (cfg->tracer_format < SKT_FMT_F32 || cfg->tracer_format > SKT_FMT_F16) return false;
     for (int k = 0; k < 5; k++)
-        if (cfg->tracer_scale_log2[k] < -60 || cfg->tracer_scale_log2[k] > 60) return SKT_ERR_ARG;
+        if (cfg->tracer_scale_log2[k] < -60 || cfg->tracer_scale_log2[k] > 60) return false;
     // refinement needs the symmetric operator, a solve to refine and a
     // positive relative tolerance
     if (cfg->pcg_refine < 0 || cfg->pcg_refine > 8 ||
         (cfg->pcg_refine > 0 && (cfg->pcg_mode != SKT_PCG_STANDARD || cfg->pcg_max_iter < 1 ||
                                  !(cfg->pcg_refine_tol > 0.0f && cfg->pcg_refine_tol < 1.0f))))
-        return SKT_ERR_ARG;
-    skt_env* env = new skt_env();
-    EnvState& e = env->s;
+        return false;
+    return true;
+}
+static void initState(EnvState& e, const skt_config* cfg) {
     e.cfg = *cfg;
     e.X = cfg->nx;
     e.Y = cfg->ny;
     e.Z = cfg->nz;
-    e.N = N;
+    e.N = (long)cfg->nx * cfg->ny * cfg->nz;
     e.G = (long)cfg->nx * cfg->nz;
     e.dx = cfg->dx;
     e.device = cfg->device;
@@ -606,6 +613,57 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
         }
         e.mix = true;
     }
+}
+// Every device array of an env (its whole HBM footprint).
+static int allocDevice(EnvState& e) {
+    for (int k = 0; k < SKT_FIELD_COUNT; k++)
+        if (dalloc(e, &e.f[k], fieldBytes(e, k))) return SKT_ERR_NOMEM;
+    for (int k = 0; k < SKT_NWORK; k++)
+        if (dalloc(e, &e.work[k], sizeof(float) * std::max(e.N, 2 * e.G))) return SKT_ERR_NOMEM;
+    for (int k = 0; k < 6; k++)
+        if (dalloc(e, &e.pcg[k], sizeof(float) * e.N)) return SKT_ERR_NOMEM;
+    if (e.cfg.pcg_refine > 0 &&
+        (dalloc(e, &e.rhs, sizeof(float) * e.N) || dalloc(e, &e.x64, sizeof(double) * e.N)))
+        return SKT_ERR_NOMEM;
+    if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.ghm1, sizeof(int) * e.G) ||
+        dalloc(e, &e.ghm2, sizeof(int) * e.G) || dalloc(e, &e.ghz, sizeof(int) * e.G) ||
+        dalloc(e, &e.irho, sizeof(float) * e.N) ||
+        dalloc(e, &e.ghEdit, sizeof(int) * e.G) ||
+        dalloc(e, &e.zeros, sizeof(float) * e.Y) || dalloc(e, &e.ones, sizeof(float) * e.Y) ||
+        dalloc(e, &e.phase, (size_t)e.N))
+        return SKT_ERR_NOMEM;
+    setup_grids(e, resident_blocks(e.device));
+    e.maxBlocks = std::max(blocks3d(e), (int)e.gB.nblocks);
+    if (dalloc(e, &e.partials, sizeof(double) * PART_COUNT * (size_t)e.maxBlocks) ||
+        dalloc(e, &e.region, sizeof(double) * 33 * (size_t)e.maxBlocks + 64 * sizeof(float)) ||
+        dalloc(e, &e.sc, sizeof(Scalars)))
+        return SKT_ERR_NOMEM;
+    return SKT_OK;
+}
+
+int skt_env_footprint(const skt_config* cfg, int64_t* bytes) {
+    if (!cfg || !bytes || !configOk(cfg)) return SKT_ERR_ARG;
+    EnvState e;
+    initState(e, cfg);
+    e.dry = true;
+    if (hipSetDevice(e.device) != hipSuccess) return SKT_ERR_HIP;
+    allocDevice(e);
+    *bytes = e.dev_bytes;
+    return SKT_OK;
+}
+int skt_env_device_bytes(skt_env* env, int64_t* bytes) {
+    if (!env || !bytes) return SKT_ERR_ARG;
+    *bytes = env->s.dev_bytes;
+    return SKT_OK;
+}
+
+int skt_env_create(const skt_config* cfg, skt_env** out) {
+    if (!cfg || !out) return SKT_ERR_ARG;
+    *out = nullptr;
+    if (!configOk(cfg)) return SKT_ERR_ARG;
+    skt_env* env = new skt_env();
+    EnvState& e = env->s;
+    initState(e, cfg);
     auto bail = [&](int rc) {
         freeAll(e);
         delete env;
@@ -614,30 +672,10 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     if (hipSetDevice(e.device) != hipSuccess) return bail(SKT_ERR_HIP);
     if (hipStreamCreateWithFlags(&e.own_stream, hipStreamNonBlocking) != hipSuccess) return bail(SKT_ERR_HIP);
     e.stream = e.own_stream;
-    for (int k = 0; k < SKT_FIELD_COUNT; k++)
-        if (dalloc(e, &e.f[k], fieldBytes(e, k))) return bail(SKT_ERR_NOMEM);
+    if (allocDevice(e)) return bail(SKT_ERR_NOMEM);
     for (int k = 0; k < SKT_NWORK; k++)
-        if (dalloc(e, &e.work[k], sizeof(float) * std::max(e.N, 2 * e.G)) ||
-            hipMemsetAsync(e.work[k], 0, sizeof(float) * std::max(e.N, 2 * e.G), e.stream) != hipSuccess)
-            return bail(SKT_ERR_NOMEM);
-    for (int k = 0; k < 6; k++)
-        if (dalloc(e, &e.pcg[k], sizeof(float) * e.N)) return bail(SKT_ERR_NOMEM);
-    if (cfg->pcg_refine > 0 &&
-        (dalloc(e, &e.rhs, sizeof(float) * e.N) || dalloc(e, &e.x64, sizeof(double) * e.N)))
-        return bail(SKT_ERR_NOMEM);
-    if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.ghm1, sizeof(int) * e.G) ||
-        dalloc(e, &e.ghm2, sizeof(int) * e.G) || dalloc(e, &e.ghz, sizeof(int) * e.G) ||
-        dalloc(e, &e.irho, sizeof(float) * e.N) ||
-        dalloc(e, &e.ghEdit, sizeof(int) * e.G) ||
-        dalloc(e, &e.zeros, sizeof(float) * e.Y) || dalloc(e, &e.ones, sizeof(float) * e.Y) ||
-        dalloc(e, &e.phase, (size_t)e.N))
-        return bail(SKT_ERR_NOMEM);
-    setup_grids(e, resident_blocks(e.device));
-    e.maxBlocks = std::max(blocks3d(e), (int)e.gB.nblocks);
-    if (dalloc(e, &e.partials, sizeof(double) * PART_COUNT * (size_t)e.maxBlocks) ||
-        dalloc(e, &e.region, sizeof(double) * 33 * (size_t)e.maxBlocks + 64 * sizeof(float)) ||
-        dalloc(e, &e.sc, sizeof(Scalars)))
-        return bail(SKT_ERR_NOMEM);
+        if (hipMemsetAsync(e.work[k], 0, sizeof(float) * std::max(e.N, 2 * e.G), e.stream) != hipSuccess)
+            return bail(SKT_ERR_HIP);
     if (hipHostMalloc((void**)&e.sc_host, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) return bail(SKT_ERR_NOMEM);
     if (hipHostMalloc((void**)&e.pcg_snap, 3 * sizeof(PcgState), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&e.ref_snap, sizeof(RefineStats), hipHostMallocDefault) != hipSuccess)

@@ -195,6 +195,10 @@ struct EnvState {
     Recorder* rec = nullptr;
     // the last step ran in an ensemble: its solver state is read from the device
     bool stats_on_device = false;
+    // device bytes requested by the env's allocations; dry: count them
+    // without allocating (footprint sizing)
+    long long dev_bytes = 0;
+    bool dry = false;
 };
 
 }  // namespace skt

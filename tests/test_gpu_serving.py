@@ -1,0 +1,128 @@
+"""Serving (skt_server_*): jobs admitted by HBM footprint into one ensemble,
+reports every interval of simulated time, retirement.
+
+The central check: a job's reports are the same, bit for bit, when it runs
+alone on a server, when it shares the ensemble with jobs of other grids that
+come and go around it, and when its environment is stepped on its own by
+updateGPU with skt_env_diagnostics at the same steps.
+"""
+import numpy as np
+import pytest
+
+import skewtcloudsim_amd as skt
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=300, tracer_format=1, pcg_refine=2, pcg_refine_tol=1e-3)
+KEYS = ("potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures", "smallPressure")
+DIAG = ("max_updraft", "cloud_top_level", "cloud_top_height", "surface_rain")
+
+
+def _server(**kw):
+    return skt.Server(member_config=CFG, **kw)
+
+
+def _run_all(srv, until_done):
+    """Steps until every job in until_done has its done event; returns events."""
+    evs, done = [], set()
+    for _ in range(10000):
+        srv.step()
+        for e in srv.poll():
+            evs.append(e)
+            if e["kind"] == "done":
+                done.add(e["job"])
+        if until_done <= done:
+            return evs
+    raise AssertionError("jobs did not finish")
+
+
+def _reports(evs, job):
+    return [(e["steps"], e["kind"]) + tuple(e[k] for k in DIAG) for e in evs
+            if e["job"] == job and e["kind"] in ("report", "done")]
+
+
+@pytest.mark.parametrize("cfg", [dict(), CFG, dict(CFG, tracer_format=2)])
+def test_footprint_equals_the_allocation(cfg):
+    for shape in ((24, 20, 16), (64, 64, 64), (33, 18, 21)):
+        fp = skt.job_footprint(*shape, **cfg)
+        env = skt.EnvironmentGPU(*shape, **cfg)
+        import ctypes
+        b = ctypes.c_int64()
+        env._chk(env.lib.skt_env_device_bytes(env.h, ctypes.byref(b)), "device_bytes")
+        assert fp == b.value, (shape, fp, b.value)
+        env.close()
+
+
+def test_job_alone_in_the_mix_and_standalone_give_the_same_reports():
+    job = dict(nx=40, ny=32, nz=24, sim_seconds=10.0, seed=77, n_bubbles=3, report_every=2.5)
+    # alone on a server
+    srv = _server()
+    jid = srv.submit(**job)
+    alone = _reports(_run_all(srv, {jid}), jid)
+    srv.close()
+    assert [r[0] for r in alone] == [5, 10, 15, 20] and alone[-1][1] == "done"
+    # in the mix: jobs of other grids before, with and after it
+    srv = _server()
+    others = [srv.submit(24, 20, 16, 3.0, seed=5, report_every=1.0),
+              srv.submit(33, 18, 21, 20.0, seed=6, n_bubbles=1, report_every=4.0)]
+    srv.step()
+    srv.step()
+    jid2 = srv.submit(**job)
+    srv.step()
+    others.append(srv.submit(48, 40, 32, 6.0, seed=8, report_every=3.0))
+    evs = _run_all(srv, {jid2, *others})
+    mix = _reports(evs, jid2)
+    srv.close()
+    assert mix == alone
+    # stepped on its own
+    snd = skt.make_sounding(40, 32, 24, 100.0, seed=77, n_bubbles=3)
+    env = skt.EnvironmentGPU(40, 32, 24, 100.0, **CFG)
+    env.init(*[snd[k] for k in KEYS])
+    own = []
+    for s in range(1, 21):
+        env.updateGPU(0.5, 1.0)
+        if s % 5 == 0:
+            d = env.diagnostics(1e-5)
+            own.append((s, "done" if s == 20 else "report") + tuple(d[k] for k in DIAG))
+    env.close()
+    assert own == alone
+
+
+def test_admission_waits_for_memory_and_slots():
+    fp = skt.job_footprint(48, 40, 32, **CFG)
+    srv = _server(hbm_budget_bytes=2 * fp + fp // 2)
+    ids = [srv.submit(48, 40, 32, 2.0 + k, seed=10 + k, report_every=1.0) for k in range(4)]
+    srv.step()
+    st = srv.status()
+    assert (st["active"], st["queued"]) == (2, 2) and st["bytes_in_use"] == 2 * fp
+    evs = _run_all(srv, set(ids))
+    order = [(e["kind"], e["job"]) for e in evs if e["kind"] in ("admitted", "done")]
+    # the third job is admitted in the step after the first finishes
+    assert order.index(("admitted", ids[2])) > order.index(("done", ids[0]))
+    assert order.index(("admitted", ids[3])) > order.index(("done", ids[1]))
+    srv.close()
+    # slots
+    srv = _server(max_active=1)
+    a, b = srv.submit(24, 20, 16, 1.0, seed=1), srv.submit(24, 20, 16, 1.0, seed=2)
+    srv.step()
+    assert srv.status()["active"] == 1 and srv.status()["queued"] == 1
+    _run_all(srv, {a, b})
+    srv.close()
+
+
+def test_a_job_larger_than_the_budget_is_refused():
+    srv = _server(hbm_budget_bytes=skt.job_footprint(24, 20, 16, **CFG))
+    with pytest.raises(skt.SktError, match="budget"):
+        srv.submit(48, 40, 32, 1.0)
+    srv.close()
+
+
+def test_poisson_driver_completes_every_request():
+    grids = [(24, 20, 16), (40, 32, 24), (33, 18, 21)]
+    r = skt.run_poisson(40.0, 12, seed=3, grids=grids, t_range=(300.0, 900.0), time_scale=1.0 / 100,
+                        member_config=CFG)
+    print({k: v for k, v in r.items() if k != "events"}, flush=True)
+    assert r["completed"] == 12
+    assert 0 < r["latency_p50_s"] <= r["latency_p99_s"] <= r["latency_max_s"]
+    reports = [e for e in r["events"] if e["kind"] in ("report", "done")]
+    assert all(np.isfinite(e["max_updraft"]) for e in reports)
