@@ -73,8 +73,18 @@ typedef struct skt_config {
      * runs no iteration. */
     int32_t pcg_refine;            /* refinement passes after the first solve (0: none) */
     float pcg_refine_tol;          /* relative tolerance of each solve when refining (1e-3) */
+    /* Preconditioner of the standard solver: SKT_PRECON_JACOBI (the
+     * reference's 1/diagonal, Game/kernelSky.cu:659-673) or SKT_PRECON_MG, a
+     * geometric multigrid V-cycle on the same operator (2x2x2 coarsening,
+     * coarse faces the halved sums of the fine faces they cover, two damped
+     * Jacobi sweeps before and after each coarse correction, 64 sweeps on the
+     * coarsest level of at most 512 cells). The V-cycle is a fixed symmetric
+     * positive definite linear map, so the iteration is still conjugate
+     * gradients; it stops on the same test. No counterpart in the reference. */
+    int32_t pcg_precon;
 } skt_config;
 
+enum { SKT_PRECON_JACOBI = 0, SKT_PRECON_MG = 1 };
 /* Storage formats of skt_config.tracer_format and skt_env_get_field_format. */
 enum { SKT_FMT_F32 = 0, SKT_FMT_BF16 = 1, SKT_FMT_F16 = 2 };
 
@@ -331,10 +341,11 @@ int skt_mphys_region_get(skt_env* env, float out[33]);
  * is bracketed by HIP events on the env's stream and accumulated per class:
  * 0 diffusion sweep, 1 PPM pass, 2 PCG search-direction + SpMV, 3 PCG update
  * + stop test, 4 microphysics, 5 max-velocity reduction, 6 precipitation
- * fall, 7 everything else. */
+ * fall, 7 everything else, 8 the multigrid preconditioner (0x1FF = all). */
+#define SKT_TIMING_CLASSES 9
 typedef struct skt_timing {
-    int64_t launches[8];
-    double total_ms[8];
+    int64_t launches[SKT_TIMING_CLASSES];
+    double total_ms[SKT_TIMING_CLASSES];
 } skt_timing;
 int skt_env_set_timing(skt_env* env, int32_t class_mask); /* also clears the accumulators */
 /* Time only every stride-th launch of each enabled class (1, the default:

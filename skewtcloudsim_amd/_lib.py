@@ -34,6 +34,7 @@ class Config(ctypes.Structure):
         ("maxabs_mode", ctypes.c_int32), ("noise_seed", ctypes.c_uint32), ("init_noise", ctypes.c_int32),
         ("max_blocks", ctypes.c_int32), ("tracer_format", ctypes.c_int32),
         ("tracer_scale_log2", ctypes.c_int32 * 5), ("pcg_refine", ctypes.c_int32), ("pcg_refine_tol", ctypes.c_float),
+        ("pcg_precon", ctypes.c_int32),
     ]
 
 
@@ -64,11 +65,11 @@ class Diagnostics(ctypes.Structure):
 
 
 class Timing(ctypes.Structure):
-    _fields_ = [("launches", ctypes.c_int64 * 8), ("total_ms", ctypes.c_double * 8)]
+    _fields_ = [("launches", ctypes.c_int64 * 9), ("total_ms", ctypes.c_double * 9)]
 
 
 TIMING_CLASSES = ["diffusion_sweep", "ppm_pass", "pcg_spmv", "pcg_update", "microphysics", "maxvel", "precip_fall",
-                  "other"]
+                  "other", "multigrid"]
 
 
 # skt_sounding_table columns, in declaration order (radioSondeData, Game/readTable.h:8-15)

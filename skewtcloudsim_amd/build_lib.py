@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["skt_kernels.hip", "skt_misc.hip", "skt_api.cpp", "skt_batch.cpp", "skt_serve.cpp", "skt_sounding.cpp", "skt_ingest.cpp"]
+SOURCES = ["skt_kernels.hip", "skt_misc.hip", "skt_mg.hip", "skt_api.cpp", "skt_batch.cpp", "skt_serve.cpp", "skt_sounding.cpp", "skt_ingest.cpp"]
 LIBS = ["-lz"]  # zip members of the radiosonde archives are deflated
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

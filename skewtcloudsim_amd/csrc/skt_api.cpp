@@ -324,13 +324,22 @@ int precip(EnvState& e, int fld, int type, float dt) { return precipFields(e, &f
 // iteration), so the device holds the control flow and a recorded step
 // replays it.
 int solveOnce(EnvState& e, int pass, int* slot_out) {
+    const bool mg = e.mg_depth > 0;
     TL(e, TC_OTHER, launch_pcg_start(e, pass));
+    if (mg) {
+        TL(e, TC_MG, launch_mg_vcycle(e));
+        TL(e, TC_MG, launch_mg_dot(e, 1));
+    }
     const int maxIt = e.cfg.pcg_max_iter;
     const int chunk = 25;
     int it = 0, c = 0;
     for (; it < maxIt; it++) {
         TL(e, TC_PCG_A, launch_pcg_A(e, it));
         TL(e, TC_PCG_B, launch_pcg_B(e, it));
+        if (mg) {
+            TL(e, TC_MG, launch_mg_vcycle(e));
+            TL(e, TC_MG, launch_mg_dot(e, 0));
+        }
         // a recorded ensemble step holds every iteration (those after the
         // member's solve has finished return at once)
         if (!e.rec && (it + 1) % chunk == 0 && it + 1 < maxIt) {
@@ -389,6 +398,7 @@ int solvePressure(EnvState& e, float dt) {
 int pressureProject(EnvState& e, float dt, int new_pressure) {
     TL(e, TC_OTHER, launch_reset_vel(e));
     TL(e, TC_OTHER, launch_build_precon(e));
+    if (e.mg_depth > 0) TL(e, TC_MG, launch_mg_setup(e));
     int rc = solvePressure(e, dt);
     if (rc) return rc;
     TL(e, TC_OTHER, launch_update_pressure_density(e));
@@ -516,6 +526,14 @@ void freeAll(EnvState& e) {
     fr(e.irho);
     fr(e.rhs);
     fr(e.x64);
+    for (int l = 0; l < e.mg_depth; l++) {
+        fr(e.mg[l].C);
+        fr(e.mg[l].xa);
+        if (l > 0) {
+            fr(e.mg[l].r);
+            fr(e.mg[l].xb);
+        }
+    }
     fr(e.zeros);
     fr(e.ones);
     fr(e.partials);
@@ -576,6 +594,7 @@ void skt_config_default(skt_config* c, int32_t nx, int32_t ny, int32_t nz, float
     for (int k = 0; k < 5; k++) c->tracer_scale_log2[k] = 16;
     c->pcg_refine = 0;
     c->pcg_refine_tol = 1e-3f;
+    c->pcg_precon = SKT_PRECON_JACOBI;
 }
 
 // Checks of a configuration (skt_env_create's SKT_ERR_ARG cases).
@@ -594,6 +613,30 @@ static bool configOk(const skt_config* cfg) {
         (cfg->pcg_refine > 0 && (cfg->pcg_mode != SKT_PCG_STANDARD || cfg->pcg_max_iter < 1 ||
                                  !(cfg->pcg_refine_tol > 0.0f && cfg->pcg_refine_tol < 1.0f))))
         return false;
+    // the multigrid cycle preconditions the symmetric operator only
+    if (cfg->pcg_precon != SKT_PRECON_JACOBI &&
+        (cfg->pcg_precon != SKT_PRECON_MG || cfg->pcg_mode != SKT_PCG_STANDARD))
+        return false;
+    return true;
+}
+// Multigrid levels of a grid: halve each dimension (rounding up) until a
+// level holds at most kMgCoarsest cells; false if kMgLevels do not reach it.
+static bool mgLevels(EnvState& e) {
+    MgLevel* L = e.mg;
+    L[0].X = e.X;
+    L[0].Y = e.Y;
+    L[0].Z = e.Z;
+    L[0].N = e.N;
+    int d = 1;
+    while (L[d - 1].N > kMgCoarsest) {
+        if (d == kMgLevels) return false;
+        L[d].X = (L[d - 1].X + 1) / 2;
+        L[d].Y = (L[d - 1].Y + 1) / 2;
+        L[d].Z = (L[d - 1].Z + 1) / 2;
+        L[d].N = (long)L[d].X * L[d].Y * L[d].Z;
+        d++;
+    }
+    e.mg_depth = d;
     return true;
 }
 static void initState(EnvState& e, const skt_config* cfg) {
@@ -625,6 +668,18 @@ static int allocDevice(EnvState& e) {
     if (e.cfg.pcg_refine > 0 &&
         (dalloc(e, &e.rhs, sizeof(float) * e.N) || dalloc(e, &e.x64, sizeof(double) * e.N)))
         return SKT_ERR_NOMEM;
+    if (e.cfg.pcg_precon == SKT_PRECON_MG) {
+        if (!mgLevels(e)) return setErr(e, SKT_ERR_ARG, "grid too large for the multigrid levels");
+        for (int l = 0; l < e.mg_depth; l++) {
+            MgLevel& L = e.mg[l];
+            if (dalloc(e, &L.C, sizeof(float4) * L.N) || dalloc(e, &L.xa, sizeof(float) * L.N)) return SKT_ERR_NOMEM;
+            if (l > 0 && (dalloc(e, &L.r, sizeof(float) * L.N) || dalloc(e, &L.xb, sizeof(float) * L.N)))
+                return SKT_ERR_NOMEM;
+        }
+        // level 0 works on the solver's residual and preconditioned residual
+        e.mg[0].r = e.pcg[0];
+        e.mg[0].xb = e.pcg[1];
+    }
     if (dalloc(e, &e.gh, sizeof(int) * e.G) || dalloc(e, &e.ghm1, sizeof(int) * e.G) ||
         dalloc(e, &e.ghm2, sizeof(int) * e.G) || dalloc(e, &e.ghz, sizeof(int) * e.G) ||
         dalloc(e, &e.irho, sizeof(float) * e.N) ||
@@ -647,7 +702,7 @@ int skt_env_footprint(const skt_config* cfg, int64_t* bytes) {
     initState(e, cfg);
     e.dry = true;
     if (hipSetDevice(e.device) != hipSuccess) return SKT_ERR_HIP;
-    allocDevice(e);
+    if (allocDevice(e)) return SKT_ERR_ARG;
     *bytes = e.dev_bytes;
     return SKT_OK;
 }
@@ -672,7 +727,7 @@ int skt_env_create(const skt_config* cfg, skt_env** out) {
     if (hipSetDevice(e.device) != hipSuccess) return bail(SKT_ERR_HIP);
     if (hipStreamCreateWithFlags(&e.own_stream, hipStreamNonBlocking) != hipSuccess) return bail(SKT_ERR_HIP);
     e.stream = e.own_stream;
-    if (allocDevice(e)) return bail(SKT_ERR_NOMEM);
+    if (const int rc = allocDevice(e)) return bail(rc);
     for (int k = 0; k < SKT_NWORK; k++)
         if (hipMemsetAsync(e.work[k], 0, sizeof(float) * std::max(e.N, 2 * e.G), e.stream) != hipSuccess)
             return bail(SKT_ERR_HIP);
@@ -910,6 +965,7 @@ int skt_env_calculate_pressure_project(skt_env* env, float dt) {
     EnvState& e = env->s;
     HIPCHK(e, launch_reset_vel(e));
     HIPCHK(e, launch_build_precon(e));
+    if (e.mg_depth > 0) HIPCHK(e, launch_mg_setup(e));
     return solvePressure(e, dt);
 }
 int skt_env_editor_data(skt_env* env) {
@@ -1207,8 +1263,8 @@ int skt_env_set_timing(skt_env* env, int32_t class_mask) {
     if (!env) return SKT_ERR_ARG;
     EnvState& e = env->s;
     RC(resolveTiming(e));
-    e.timing = class_mask & 0xFF;
-    for (int k = 0; k < 8; k++) {
+    e.timing = class_mask & ((1 << SKT_TIMING_CLASSES) - 1);
+    for (int k = 0; k < SKT_TIMING_CLASSES; k++) {
         e.t_ms[k] = 0.0;
         e.t_count[k] = 0;
         e.t_seen[k] = 0;
@@ -1224,7 +1280,7 @@ int skt_env_get_timing(skt_env* env, skt_timing* out) {
     if (!env || !out) return SKT_ERR_ARG;
     EnvState& e = env->s;
     RC(resolveTiming(e));
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < SKT_TIMING_CLASSES; k++) {
         out->launches[k] = e.t_count[k];
         out->total_ms[k] = e.t_ms[k];
     }

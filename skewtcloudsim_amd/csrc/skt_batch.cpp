@@ -209,6 +209,7 @@ int buildPlan(skt_batch* b, float dt, float speed) {
             }
             st[G] = pos;
             P.total = pos;
+            if (pos == 0) continue;  // no member has tiles in this launch
             auto it = wmIndex.find(st);
             if (it == wmIndex.end()) {
                 it = wmIndex.emplace(st, wm.size()).first;
@@ -372,9 +373,9 @@ int skt_batch_add(skt_batch* b, skt_env* env, int32_t* member_id) {
         if (m.env == env) return berr(b, SKT_ERR_ARG, "environment already in the ensemble");
         const skt_config &a = m.env->s.cfg, &c = e.cfg;
         if (a.pcg_mode != c.pcg_mode || a.pcg_max_iter != c.pcg_max_iter || a.diffusion_sweeps != c.diffusion_sweeps ||
-            a.pcg_refine != c.pcg_refine)
+            a.pcg_refine != c.pcg_refine || a.pcg_precon != c.pcg_precon)
             return berr(b, SKT_ERR_ARG,
-                        "members must share the solver mode, iteration limit, refinement passes and diffusion sweeps");
+                        "members must share the solver mode, iteration limit, refinement passes, preconditioner and diffusion sweeps");
     }
     // the member's work joins the ensemble's stream (after its own pending work)
     hipEvent_t ev = nullptr;

@@ -285,4 +285,50 @@ __device__ __forceinline__ bool faceValueOk(float i) { return i >= 0x1p-127f && 
 // the reference.
 __device__ __forceinline__ float faceI(float ic, float in) { return isinf(in) ? 0.0f : faceOfSum(ic + in); }
 
+// One row of the symmetric pressure operator at sky cell i = (x, y, z), as
+// the SpMV forms it: the six face coefficients (faceI of the two cells'
+// 1/density; a neighbour beyond the domain takes the cell's own 1/density
+// and counts 0, a ground neighbour gives a zero face; the back face at z = 0
+// is the domain boundary) and the diagonal w = -(sum of the six) summed in
+// float in the SpMV's order. s* say which neighbours are sky cells in the
+// domain (the off-diagonal entries). gh* are the ground heights of the
+// cell's column and of its four neighbour columns (unused outside).
+struct OpRow {
+    float fr, fl, fu, fd, ff, fb, w;
+    bool sR, sL, sU, sD, sF, sB;
+};
+__device__ __forceinline__ OpRow opRow(const float* __restrict__ irho, long i, int x, int y, int z, int X, int Y,
+                                       int Z, long XY, int gh, int ghL, int ghR, int ghB, int ghF) {
+    const float INF = __builtin_inff();
+    const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
+    OpRow o;
+    o.sR = inR && y > ghR;
+    o.sL = inL && y > ghL;
+    o.sU = y + 1 < Y;
+    o.sD = y > 0 && y - 1 > gh;
+    o.sF = inF && y > ghF;
+    o.sB = inB && y > ghB;
+    const float ic = irho[i];
+    const float iR = !inR ? ic : (o.sR ? irho[i + 1] : INF);
+    const float iL = !inL ? ic : (o.sL ? irho[i - 1] : INF);
+    const float iU = o.sU ? irho[i + X] : ic;
+    const float iD = y == 0 ? ic : (o.sD ? irho[i - X] : INF);
+    const float iF = !inF ? ic : (o.sF ? irho[i + XY] : INF);
+    o.fr = faceI(ic, iR);
+    o.fl = faceI(ic, iL);
+    o.fu = faceI(ic, iU);
+    o.fd = faceI(ic, iD);
+    o.ff = faceI(ic, iF);
+    o.fb = z == 0 ? faceI(ic, ic) : (o.sB ? faceI(irho[i - XY], ic) : 0.0f);
+    float w = 0.0f;
+    w -= o.fr;
+    w -= o.fl;
+    w -= o.fu;
+    w -= o.fd;
+    w -= o.ff;
+    w -= o.fb;
+    o.w = w;
+    return o;
+}
+
 }  // namespace skt

@@ -24,6 +24,20 @@ struct PcgState {
     int pad[5];
 };
 
+// One level of the multigrid preconditioner: dims, the operator as
+// {+x face, +y face, +z face, diagonal} per cell (faces as positive
+// couplings, diagonal 0 marks a ground cell), the right-hand side r and the
+// two iterate buffers (the level's result ends in xb). Level 0's r is the
+// PCG residual and its xb the preconditioned residual z.
+constexpr int kMgLevels = 8;
+constexpr long kMgCoarsest = 512;  // cells of a coarsest level at most
+struct MgLevel {
+    int X = 0, Y = 0, Z = 0;
+    long N = 0;
+    float4* C = nullptr;
+    float *r = nullptr, *xa = nullptr, *xb = nullptr;
+};
+
 // Totals of an FP64-refined pressure solve (written on the device).
 struct RefineStats {
     int iters_total;  // iterations over the projection's solves
@@ -146,6 +160,11 @@ struct EnvState {
     float* irho = nullptr;                  // 1/density of the last pressure solve (0 in ground)
     float* rhs = nullptr;                   // FP64 refinement: the solve's right-hand side b [N]
     double* x64 = nullptr;                  //   and the FP64 sum of its solves [N]
+    // multigrid preconditioner: levels 0..mg_depth-1 in use (0: Jacobi); the
+    // launch sequence always covers kMgLevels levels (a level past a grid's
+    // depth has no tiles), so grids of any size share it in an ensemble
+    int mg_depth = 0;
+    MgLevel mg[kMgLevels];
     float* zeros = nullptr;                 // [Y] zero profile
     float* ones = nullptr;                  // [Y] unit profile
     double* partials = nullptr;             // per-block reduction partials (PART_COUNT x maxBlocks)
@@ -174,9 +193,9 @@ struct EnvState {
     int timing = 0;
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> evpool;
-    double t_ms[8] = {0};
-    long long t_count[8] = {0};
-    long long t_seen[8] = {0};  // launches of each class since skt_env_set_timing
+    double t_ms[SKT_TIMING_CLASSES] = {0};
+    long long t_count[SKT_TIMING_CLASSES] = {0};
+    long long t_seen[SKT_TIMING_CLASSES] = {0};  // launches of each class since skt_env_set_timing
     int t_stride = 1;           // time every t_stride-th launch of a class
     // pressure-solve state snapshots in pinned host memory: two alternating
     // chunk snapshots and the final state, each behind its event
@@ -210,10 +229,10 @@ struct skt_env {
 
 namespace skt {
 
-enum { TC_DIFFUSE = 0, TC_PPM, TC_PCG_A, TC_PCG_B, TC_MICRO, TC_MAXVEL, TC_PRECIP, TC_OTHER };
+enum { TC_DIFFUSE = 0, TC_PPM, TC_PCG_A, TC_PCG_B, TC_MICRO, TC_MAXVEL, TC_PRECIP, TC_OTHER, TC_MG };
 // Classes whose launch functions issue one kernel through klaunch, so their
 // timing events ride on the kernel's own dispatch.
-constexpr bool kTimedOnDispatch[8] = {true, true, true, true, true, false, true, false};
+constexpr bool kTimedOnDispatch[SKT_TIMING_CLASSES] = {true, true, true, true, true, false, true, false, false};
 
 
 // --------------------------------------------------- step (skt_api.cpp) ---
@@ -328,6 +347,9 @@ hipError_t launch_refine_keep_rhs(EnvState& e);        // rhs = r (the divergenc
 hipError_t launch_refine_accum(EnvState& e, int first, int last);  // x64 += p'
 hipError_t launch_refine_residual(EnvState& e, int write_r);      // r = b - A x64 in FP64 (+ max partials)
 hipError_t launch_refine_report(EnvState& e);          // final max |b - A x64|
+hipError_t launch_mg_setup(EnvState& e);               // the levels' operators from 1/density
+hipError_t launch_mg_vcycle(EnvState& e);              // z = V-cycle(r)
+hipError_t launch_mg_dot(EnvState& e, int start);      // z.r partials for the next search direction
 hipError_t launch_update_pressure_density(EnvState& e);
 hipError_t launch_apply_projection(EnvState& e, int new_pressure);
 hipError_t launch_microphysics(EnvState& e, float dt, float speed);

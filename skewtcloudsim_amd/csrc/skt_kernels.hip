@@ -1601,39 +1601,19 @@ struct KRefineResidual {
             const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
             const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
             const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
-            const float INF = __builtin_inff();
             const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
             for (int y = y0; y < y1; y++) {
                 const long i = col + (long)X * y;
                 double rv = 0.0;
                 if (y > gh) {
-                    const float ic = irho[i];
-                    // a neighbour beyond the domain takes the cell's own 1/density
-                    // (its value counts 0), a ground neighbour a zero face
-                    const bool sR = inR && y > ghR, sL = inL && y > ghL, sU = y + 1 < Y, sD = y > 0 && y - 1 > gh;
-                    const bool sF = inF && y > ghF, sB = inB && y > ghB;
-                    const float iR = !inR ? ic : (sR ? irho[i + 1] : INF);
-                    const float iL = !inL ? ic : (sL ? irho[i - 1] : INF);
-                    const float iU = sU ? irho[i + X] : ic;
-                    const float iD = y == 0 ? ic : (sD ? irho[i - X] : INF);
-                    const float iF = !inF ? ic : (sF ? irho[i + XY] : INF);
-                    const float fr = faceI(ic, iR), fl = faceI(ic, iL), fu = faceI(ic, iU), fd = faceI(ic, iD);
-                    const float ff = faceI(ic, iF);
-                    const float fb = z == 0 ? faceI(ic, ic) : (sB ? faceI(irho[i - XY], ic) : 0.0f);
-                    float w = 0.0f;
-                    w -= fr;
-                    w -= fl;
-                    w -= fu;
-                    w -= fd;
-                    w -= ff;
-                    w -= fb;
-                    double ax = (double)w * x64[i];
-                    if (sL) ax += (double)fl * x64[i - 1];
-                    if (sD) ax += (double)fd * x64[i - X];
-                    if (sB) ax += (double)fb * x64[i - XY];
-                    if (sR) ax += (double)fr * x64[i + 1];
-                    if (sU) ax += (double)fu * x64[i + X];
-                    if (sF) ax += (double)ff * x64[i + XY];
+                    const OpRow o = opRow(irho, i, x, y, z, X, Y, Z, XY, gh, ghL, ghR, ghB, ghF);
+                    double ax = (double)o.w * x64[i];
+                    if (o.sL) ax += (double)o.fl * x64[i - 1];
+                    if (o.sD) ax += (double)o.fd * x64[i - X];
+                    if (o.sB) ax += (double)o.fb * x64[i - XY];
+                    if (o.sR) ax += (double)o.fr * x64[i + 1];
+                    if (o.sU) ax += (double)o.fu * x64[i + X];
+                    if (o.sF) ax += (double)o.ff * x64[i + XY];
                     rv = (double)b[i] - ax;
                     m = fmax(m, fabs(rv));
                 }

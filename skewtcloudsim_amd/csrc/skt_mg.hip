@@ -1,0 +1,312 @@
+// Geometric multigrid preconditioner of the standard pressure solve
+// (skt_config.pcg_precon = SKT_PRECON_MG). No counterpart in the reference,
+// whose PCG is preconditioned by 1/diagonal (Game/kernelSky.cu:659-673).
+//
+// The operator on level 0 is the solver's own: per sky cell the positive
+// couplings to its +x, +y and +z neighbours (minus the face coefficients of
+// opRow) and the diagonal w; a coupling to a -x/-y/-z neighbour is that
+// neighbour's + coupling (the operator is symmetric). Level l + 1 coarsens
+// 2x2x2 (a last odd cell alone); a coarse coupling is half the sum of the
+// fine couplings across the coarse face, the coarse diagonal half the sum of
+// all fine couplings that leave the coarse cell (domain-boundary faces
+// included). A coarse cell with no sky cell has diagonal 0 and is excluded.
+// The V-cycle: two damped Jacobi sweeps (the first from zero), the residual
+// summed over the 2x2x2 children onto the coarse level, recursion, the
+// coarse correction added to every child, two more sweeps; on the coarsest
+// level (at most 512 cells) 64 sweeps from zero in LDS by one block. The
+// restriction is the transpose of the prolongation and the pre- and
+// post-smoothers are equal, so the cycle is a symmetric positive definite
+// linear map and conjugate gradients keep their stop test.
+//
+// The launch sequence covers kMgLevels levels whatever the grid (a level
+// past the grid's depth has no tiles), so members of any size share it in
+// an ensemble. Every body returns at once when the solve has finished.
+#include <hip/hip_runtime.h>
+#include "skt_device.h"
+#include "skt_launch.h"
+#include "skt_reduce.h"
+
+namespace skt {
+namespace {
+
+constexpr float kOmega = 0.8f;     // Jacobi damping
+constexpr int kCoarseSweeps = 64;  // sweeps on the coarsest level (even: the result lands in xb)
+
+__device__ __forceinline__ long gstride(const VB& vb) { return (long)vb.n * 256; }
+__device__ __forceinline__ long gfirst(const VB& vb) { return (long)vb.id * 256 + tid256(); }
+
+struct MgDims {
+    int X, Y, Z;
+    long N;
+};
+
+// (A x)_i on a level: d x_i - sum of couplings times neighbours.
+__device__ __forceinline__ float applyRow(const MgDims& D, const float4* __restrict__ C, const float* __restrict__ x,
+                                          long i, const float4& c) {
+    const int X = D.X, Y = D.Y;
+    const long XY = (long)X * Y;
+    const int xx = (int)(i % X), yy = (int)((i / X) % Y), zz = (int)(i / XY);
+    float s = 0.0f;
+    if (xx + 1 < X) s += c.x * x[i + 1];
+    if (xx > 0) s += C[i - 1].x * x[i - 1];
+    if (yy + 1 < Y) s += c.y * x[i + X];
+    if (yy > 0) s += C[i - X].y * x[i - X];
+    if (zz + 1 < D.Z) s += c.z * x[i + XY];
+    if (zz > 0) s += C[i - XY].z * x[i - XY];
+    return c.w * x[i] - s;
+}
+
+// Level-0 operator from 1/density (after the preconditioner pass wrote irho).
+struct KMgCoef0 {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, const float* __restrict__ irho,
+                                               float4* __restrict__ C) {
+        const int X = G.X, Y = G.Y, Z = G.Z;
+        const long XY = (long)X * Y, N = XY * Z;
+        for (long i = gfirst(vb); i < N; i += gstride(vb)) {
+            const int x = (int)(i % X), y = (int)((i / X) % Y), z = (int)(i / XY);
+            const long gc = x + (long)X * z;
+            const int gh = G.gh[gc];
+            if (y <= gh) {
+                C[i] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+                continue;
+            }
+            const int ghL = x > 0 ? G.gh[gc - 1] : 0, ghR = x < X - 1 ? G.gh[gc + 1] : 0;
+            const int ghB = z > 0 ? G.gh[gc - X] : 0, ghF = z < Z - 1 ? G.gh[gc + X] : 0;
+            const OpRow o = opRow(irho, i, x, y, z, X, Y, Z, XY, gh, ghL, ghR, ghB, ghF);
+            C[i] = float4{o.sR ? -o.fr : 0.0f, o.sU ? -o.fu : 0.0f, o.sF ? -o.ff : 0.0f, o.w};
+        }
+    }
+};
+
+// Level l + 1's operator from level l's.
+struct KMgCoarsen {
+    static __device__ __forceinline__ void run(VB vb, MgDims F, const float4* __restrict__ Cf, MgDims D,
+                                               float4* __restrict__ Cc) {
+        const long FXY = (long)F.X * F.Y, XY = (long)D.X * D.Y;
+        for (long k = gfirst(vb); k < D.N; k += gstride(vb)) {
+            const int I = (int)(k % D.X), J = (int)((k / D.X) % D.Y), K = (int)(k / XY);
+            const int x0 = 2 * I, y0 = 2 * J, z0 = 2 * K;
+            const int nx = min(2, F.X - x0), ny = min(2, F.Y - y0), nz = min(2, F.Z - z0);
+            float sumd = 0.0f, inner = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+            for (int c = 0; c < nz; c++)
+                for (int b = 0; b < ny; b++)
+                    for (int a = 0; a < nx; a++) {
+                        const float4 f = Cf[(x0 + a) + (long)F.X * (y0 + b) + FXY * (z0 + c)];
+                        sumd += f.w;
+                        if (a == 0 && nx == 2) inner += f.x;
+                        if (b == 0 && ny == 2) inner += f.y;
+                        if (c == 0 && nz == 2) inner += f.z;
+                        if (a == nx - 1) bx += f.x;
+                        if (b == ny - 1) by += f.y;
+                        if (c == nz - 1) bz += f.z;
+                    }
+            // a sibling face is in both children's diagonals
+            const float d = 0.5f * (sumd - 2.0f * inner);
+            Cc[k] = d > 0.0f ? float4{0.5f * bx, 0.5f * by, 0.5f * bz, d} : float4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    }
+};
+
+// One damped Jacobi sweep xout = xin + omega (r - A xin) / d (first: from
+// xin = 0, xout = omega r / d).
+struct KMgSmooth {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        return nthArg<5>(a...)->done != 0;  // st
+    }
+    static __device__ __forceinline__ void run(VB vb, MgDims D, const float4* __restrict__ C,
+                                               const float* __restrict__ r, const float* __restrict__ xin,
+                                               float* __restrict__ xout, const PcgState* st, int first) {
+        if (st->done) return;
+        for (long i = gfirst(vb); i < D.N; i += gstride(vb)) {
+            const float4 c = C[i];
+            float v = 0.0f;
+            if (c.w > 0.0f) {
+                if (first) v = kOmega * r[i] / c.w;
+                else v = xin[i] + kOmega * (r[i] - applyRow(D, C, xin, i, c)) / c.w;
+            }
+            xout[i] = v;
+        }
+    }
+};
+
+// The residual r - A x of level l summed over each coarse cell's children
+// into level l + 1's right-hand side.
+struct KMgRestrict {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        return nthArg<6>(a...)->done != 0;  // st
+    }
+    static __device__ __forceinline__ void run(VB vb, MgDims F, const float4* __restrict__ Cf,
+                                               const float* __restrict__ rf, const float* __restrict__ xf, MgDims D,
+                                               float* __restrict__ rc, const PcgState* st) {
+        if (st->done) return;
+        const long FXY = (long)F.X * F.Y, XY = (long)D.X * D.Y;
+        for (long k = gfirst(vb); k < D.N; k += gstride(vb)) {
+            const int I = (int)(k % D.X), J = (int)((k / D.X) % D.Y), K = (int)(k / XY);
+            const int x0 = 2 * I, y0 = 2 * J, z0 = 2 * K;
+            const int nx = min(2, F.X - x0), ny = min(2, F.Y - y0), nz = min(2, F.Z - z0);
+            float s = 0.0f;
+            for (int c = 0; c < nz; c++)
+                for (int b = 0; b < ny; b++)
+                    for (int a = 0; a < nx; a++) {
+                        const long i = (x0 + a) + (long)F.X * (y0 + b) + FXY * (z0 + c);
+                        const float4 cf = Cf[i];
+                        if (cf.w > 0.0f) s += rf[i] - applyRow(F, Cf, xf, i, cf);
+                    }
+            rc[k] = s;
+        }
+    }
+};
+
+// x_l += the coarse correction of each cell's parent (sky cells).
+struct KMgProlong {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        return nthArg<5>(a...)->done != 0;  // st
+    }
+    static __device__ __forceinline__ void run(VB vb, MgDims F, const float4* __restrict__ Cf, float* __restrict__ xf,
+                                               MgDims D, const float* __restrict__ xc, const PcgState* st) {
+        if (st->done) return;
+        const long FXY = (long)F.X * F.Y;
+        for (long i = gfirst(vb); i < F.N; i += gstride(vb)) {
+            if (Cf[i].w > 0.0f) {
+                const int x = (int)(i % F.X), y = (int)((i / F.X) % F.Y), z = (int)(i / FXY);
+                xf[i] += xc[(x >> 1) + (long)D.X * (y >> 1) + (long)D.X * D.Y * (z >> 1)];
+            }
+        }
+    }
+};
+
+// The coarsest level: kCoarseSweeps damped Jacobi sweeps from zero, in LDS,
+// by one block; the result is written to x.
+struct KMgCoarsest {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        return nthArg<4>(a...)->done != 0;  // st
+    }
+    static __device__ __forceinline__ void run(VB vb, MgDims D, const float4* __restrict__ C,
+                                               const float* __restrict__ r, float* __restrict__ x,
+                                               const PcgState* st) {
+        (void)vb;
+        if (st->done) return;
+        __shared__ float4 sC[kMgCoarsest];
+        __shared__ float sr[kMgCoarsest], sx[2][kMgCoarsest];
+        const int t = tid256();
+        const int n = (int)D.N;  // at most kMgCoarsest (checked on the host)
+        for (int i = t; i < n; i += 256) {
+            sC[i] = C[i];
+            sr[i] = r[i];
+        }
+        __syncthreads();
+        const int X = D.X, Y = D.Y, Z = D.Z, XY = X * Y;
+        for (int i = t; i < n; i += 256) sx[0][i] = sC[i].w > 0.0f ? kOmega * sr[i] / sC[i].w : 0.0f;
+        __syncthreads();
+        for (int k = 1; k < kCoarseSweeps; k++) {
+            const float* xin = sx[(k - 1) & 1];
+            float* xo = sx[k & 1];
+            for (int i = t; i < n; i += 256) {
+                const float4 c = sC[i];
+                float v = 0.0f;
+                if (c.w > 0.0f) {
+                    const int xx = i % X, yy = (i / X) % Y, zz = i / XY;
+                    float s = 0.0f;
+                    if (xx + 1 < X) s += c.x * xin[i + 1];
+                    if (xx > 0) s += sC[i - 1].x * xin[i - 1];
+                    if (yy + 1 < Y) s += c.y * xin[i + X];
+                    if (yy > 0) s += sC[i - X].y * xin[i - X];
+                    if (zz + 1 < Z) s += c.z * xin[i + XY];
+                    if (zz > 0) s += sC[i - XY].z * xin[i - XY];
+                    v = xin[i] + kOmega * (sr[i] - (c.w * xin[i] - s)) / c.w;
+                }
+                xo[i] = v;
+            }
+            __syncthreads();
+        }
+        for (int i = t; i < n; i += 256) x[i] = sx[(kCoarseSweeps - 1) & 1][i];
+    }
+};
+
+// z.r partials of the preconditioned residual in the layout the next
+// search-direction kernel reduces: n blocks, block b summing cells
+// [b chunk, (b + 1) chunk) of float products in double.
+struct KMgDot {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        return nthArg<4>(a...)->done != 0;  // st
+    }
+    static __device__ __forceinline__ void run(VB vb, long N, const float* __restrict__ z,
+                                               const float* __restrict__ r, double* part, const PcgState* st) {
+        __shared__ double sh[4];
+        if (st->done) return;
+        const long chunk = (N + vb.n - 1) / vb.n;
+        const long a = (long)vb.id * chunk, b = min(N, a + chunk);
+        double s = 0.0;
+        for (long i = a + tid256(); i < b; i += 256) s += (double)(z[i] * r[i]);
+        s = blockSum(s, sh);
+        if (tid256() == 0) part[vb.id] = s;
+    }
+};
+
+MgDims dims(const MgLevel& L) { return MgDims{L.X, L.Y, L.Z, L.N}; }
+// Tiles of an elementwise pass over n cells (none past the grid's depth).
+dim3 tiles(long n, bool on) {
+    if (!on || n <= 0) return dim3(0);
+    return dim3((unsigned)std::min<long>((n + 255) / 256, 8192));
+}
+
+}  // namespace
+
+hipError_t launch_mg_setup(EnvState& e) {
+    const MgLevel* L = e.mg;
+    hipError_t r = klaunch<KMgCoef0>(e, tiles(e.N, true), dim3(256), 0, geom(e), (const float*)e.irho, L[0].C);
+    for (int l = 0; l + 1 < kMgLevels && r == hipSuccess; l++) {
+        const bool on = l + 1 < e.mg_depth;
+        r = klaunch<KMgCoarsen>(e, tiles(L[l + 1].N, on), dim3(256), 0, dims(L[l]), (const float4*)L[l].C,
+                                dims(L[l + 1]), L[l + 1].C);
+    }
+    return r;
+}
+
+hipError_t launch_mg_vcycle(EnvState& e) {
+    const MgLevel* L = e.mg;
+    const PcgState* st = &e.sc->st[1];
+    const int D = e.mg_depth;
+    hipError_t r = hipSuccess;
+    auto go = [&](hipError_t x) {
+        if (r == hipSuccess) r = x;
+    };
+    for (int l = 0; l + 1 < kMgLevels; l++) {
+        const bool on = l + 1 < D;
+        const dim3 g = tiles(L[l].N, on);
+        go(klaunch<KMgSmooth>(e, g, dim3(256), 0, dims(L[l]), (const float4*)L[l].C, (const float*)L[l].r,
+                              (const float*)L[l].xb, L[l].xa, st, 1));
+        go(klaunch<KMgSmooth>(e, g, dim3(256), 0, dims(L[l]), (const float4*)L[l].C, (const float*)L[l].r,
+                              (const float*)L[l].xa, L[l].xb, st, 0));
+        go(klaunch<KMgRestrict>(e, tiles(L[l + 1].N, on), dim3(256), 0, dims(L[l]), (const float4*)L[l].C,
+                                (const float*)L[l].r, (const float*)L[l].xb, dims(L[l + 1]), L[l + 1].r, st));
+    }
+    const MgLevel& B = L[D - 1];
+    go(klaunch<KMgCoarsest>(e, dim3(1), dim3(256), 0, dims(B), (const float4*)B.C, (const float*)B.r, B.xb, st));
+    for (int l = kMgLevels - 2; l >= 0; l--) {
+        const bool on = l + 1 < D;
+        const dim3 g = tiles(L[l].N, on);
+        go(klaunch<KMgProlong>(e, g, dim3(256), 0, dims(L[l]), (const float4*)L[l].C, L[l].xb, dims(L[l + 1]),
+                               (const float*)L[l + 1].xb, st));
+        go(klaunch<KMgSmooth>(e, g, dim3(256), 0, dims(L[l]), (const float4*)L[l].C, (const float*)L[l].r,
+                              (const float*)L[l].xb, L[l].xa, st, 0));
+        go(klaunch<KMgSmooth>(e, g, dim3(256), 0, dims(L[l]), (const float4*)L[l].C, (const float*)L[l].r,
+                              (const float*)L[l].xa, L[l].xb, st, 0));
+    }
+    return r;
+}
+
+hipError_t launch_mg_dot(EnvState& e, int start) {
+    const size_t M = e.maxBlocks;
+    const unsigned n = start ? e.gZ.nblocks : e.gB.nblocks;
+    double* part = e.partials + (start ? PART_START : PART_BSUM) * M;
+    return klaunch<KMgDot>(e, dim3(n), dim3(256), 0, e.N, (const float*)e.pcg[1], (const float*)e.pcg[0], part,
+                           (const PcgState*)&e.sc->st[1]);
+}
+
+}  // namespace skt

@@ -397,7 +397,7 @@ class EnvironmentGPU:
         launch of a class; enable=False switches it off."""
         mask = 0
         if enable:
-            mask = 0xFF if classes is None else sum(1 << _lib.TIMING_CLASSES.index(c) for c in classes)
+            mask = 0x1FF if classes is None else sum(1 << _lib.TIMING_CLASSES.index(c) for c in classes)
         self._chk(self.lib.skt_env_set_timing_stride(self.h, int(stride)), "set_timing_stride")
         self._chk(self.lib.skt_env_set_timing(self.h, mask), "set_timing")
 
