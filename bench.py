@@ -13,12 +13,14 @@ Beside it, each in its own key (skip any with --skip):
   standard  the same step with the textbook Jacobi-PCG at a relative residual
             of 1e-6, K steps as the headline;
   mixed     the mixed-precision form at 512 x 512 x 256 (hydrometeors in BF16
-            and in FP16 with power-of-two scales, two FP64 refinement passes
-            around Jacobi-PCG to 1e-6) next to the FP32 step at that size;
+            and in FP16 with power-of-two scales, multigrid-preconditioned CG
+            to 1e-6 with two FP64 refinement passes) next to the FP32 step at
+            that size with the Jacobi and with the multigrid preconditioner;
   ensemble  64 members drawn from {64^3, 128^3, 192x192x96, 128x128x160},
-            1-4 bubbles each, in mixed precision, one launch per phase
-            (HIP graph), against the same members run one after the other as
-            plain FP32 steps; also 64 members of 64^3 (the launch-bound case);
+            1-4 bubbles each, in the mixed form, one launch per phase (HIP
+            graph), against the same members run one after the other as plain
+            FP32 steps (Jacobi-PCG to 1e-6) and as mixed-form steps; also 64
+            members of 64^3 (the launch-bound case);
   serving   Poisson arrivals at 2, 5, 10 and 20 requests/s of 'simulate
             sounding S for T s' jobs (grids of the ensemble plus
             256x256x128, T uniform 300-1800 s) with T and the 60 s report
@@ -52,10 +54,13 @@ HBM_PEAK_GBS = 8000.0
 TIMING_STRIDE = 10
 KEYS = ("potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures", "smallPressure")
 STANDARD = dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=1000)
-MIXED = {"bf16": dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=400, tracer_format=1, pcg_refine=2,
-                      pcg_refine_tol=1e-3),
-         "f16": dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=400, tracer_format=2, pcg_refine=2,
-                     pcg_refine_tol=1e-3)}
+# the mixed-precision form: 16-bit hydrometeors, multigrid-preconditioned CG
+# to a relative residual of 1e-6 with two FP64 refinement passes
+# (at most 60 iterations per solve: an ensemble step holds that many per pass)
+MIXED = {"bf16": dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=60, tracer_format=1, pcg_refine=2,
+                      pcg_refine_tol=1e-3, pcg_precon=1),
+         "f16": dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=60, tracer_format=2, pcg_refine=2,
+                     pcg_refine_tol=1e-3, pcg_precon=1)}
 ENSEMBLE_GRIDS = [(64, 64, 64), (128, 128, 128), (192, 96, 192), (128, 160, 128)]  # nx, ny vertical, nz
 SERVE_RATES = (2.0, 5.0, 10.0, 20.0)
 SERVE_REQUESTS = 16
@@ -232,7 +237,8 @@ def run_mixed(steps, warmup):
     snd = skt.make_sounding(nx, ny, nz, DX, seed=SEED, n_bubbles=3)
     args = [snd[k] for k in KEYS]
     out = {"grid": [nx, ny, nz], "cells": nx * ny * nz}
-    for name, cfg, tb in [("fp32", dict(STANDARD, pcg_max_iter=400), 4)] + [(k, v, 2) for k, v in MIXED.items()]:
+    legs = [("fp32", dict(STANDARD, pcg_max_iter=1000), 4), ("fp32_multigrid", dict(STANDARD, pcg_precon=1), 4)]
+    for name, cfg, tb in legs + [(k, v, 2) for k, v in MIXED.items()]:
         env = skt.EnvironmentGPU(nx, ny, nz, DX, **cfg)
         env.init(*args)
         r = _timed_env(env, steps, warmup, tb)
@@ -283,6 +289,15 @@ def run_ensemble(steps, members=64):
         for e in plain:
             e.close()
         mixed = _members(shapes, MIXED["bf16"], np.random.default_rng(SEED + 1), SEED)
+        for e in mixed:
+            e.updateGPU(DT, 1.0)
+        _sync()
+        t0 = time.perf_counter()
+        for e in mixed:
+            for _ in range(steps):
+                e.updateGPU(DT, 1.0)
+        _sync()
+        seq_mixed = (time.perf_counter() - t0) / steps
         ens = skt.EnsembleGPU()
         for e in mixed:
             ens.add(e)
@@ -298,11 +313,12 @@ def run_ensemble(steps, members=64):
         for e in mixed:
             e.close()
         out[case] = {"members": members, "cells": cells, "steps": steps,
-                     "sequential_plain_fp32_ms_per_round": seq * 1e3, "ensemble_mixed_ms_per_step": bat * 1e3,
+                     "sequential_plain_fp32_ms_per_round": seq * 1e3,
+                     "sequential_mixed_ms_per_round": seq_mixed * 1e3, "ensemble_mixed_ms_per_step": bat * 1e3,
                      "member_steps_per_s": members / bat, "member_steps_per_s_sequential": members / seq,
                      "members_per_s_600s_runs": members / (1200 * bat),
                      "members_per_s_600s_runs_sequential": members / (1200 * seq),
-                     "speedup": seq / bat, "autotune": tun}
+                     "speedup": seq / bat, "speedup_batching_alone": seq_mixed / bat, "autotune": tun}
         print(f"ensemble leg {case}: sequential {seq*1e3:.1f} ms, ensemble {bat*1e3:.1f} ms", file=sys.stderr,
               flush=True)
     return out
@@ -322,7 +338,7 @@ def run_serving():
     return {"time_scale": SERVE_TIME_SCALE, "requests_per_rate": SERVE_REQUESTS,
             "job": "grid from the ensemble sizes plus 256x256x128, random sounding with 1-4 bubbles, "
                    "T uniform 300-1800 s and reports every 60 s, both times time_scale; BF16 hydrometeors, "
-                   "Jacobi-PCG to 1e-6 with two FP64 refinement passes",
+                   "multigrid-preconditioned CG to 1e-6 with two FP64 refinement passes",
             "rates": res}
 
 

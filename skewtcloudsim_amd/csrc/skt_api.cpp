@@ -309,7 +309,8 @@ int precip(EnvState& e, int fld, int type, float dt) { return precipFields(e, &f
 
 // calculatePressureProject, Game/environment.cu:898-987. The scalar state
 // lives on the device (two slots) and every kernel of a finished solve
-// returns at once. After each chunk of 25 iterations the state is copied to
+// returns at once. After each chunk of 25 iterations (4 with the multigrid
+// preconditioner) the state is copied to
 // pinned host memory behind an event, and the host waits for the copy of the
 // chunk before: the device always has the current chunk queued while the
 // host decides whether to enqueue the next. A solve that converges in chunk
@@ -331,7 +332,9 @@ int solveOnce(EnvState& e, int pass, int* slot_out) {
         TL(e, TC_MG, launch_mg_dot(e, 1));
     }
     const int maxIt = e.cfg.pcg_max_iter;
-    const int chunk = 25;
+    // a multigrid iteration costs a few Jacobi ones and a solve takes tens,
+    // not hundreds: poll in chunks of 4
+    const int chunk = mg ? 4 : 25;
     int it = 0, c = 0;
     for (; it < maxIt; it++) {
         TL(e, TC_PCG_A, launch_pcg_A(e, it));

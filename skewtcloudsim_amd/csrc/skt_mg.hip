@@ -22,6 +22,7 @@
 // past the grid's depth has no tiles), so members of any size share it in
 // an ensemble. Every body returns at once when the solve has finished.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "skt_device.h"
 #include "skt_launch.h"
 #include "skt_reduce.h"
@@ -35,24 +36,44 @@ constexpr int kCoarseSweeps = 64;  // sweeps on the coarsest level (even: the re
 __device__ __forceinline__ long gstride(const VB& vb) { return (long)vb.n * 256; }
 __device__ __forceinline__ long gfirst(const VB& vb) { return (long)vb.id * 256 + tid256(); }
 
+// A level's dims and the z-chunk of its tiles. Tiles are 64 (x) x 4 (y)
+// columns of zc planes; a thread marches its column through the chunk, so a
+// cell's coordinates are never divided out of its index (levels hold fewer
+// than 2^31 cells, so 32-bit indices serve).
 struct MgDims {
     int X, Y, Z;
     long N;
+    int zc;
 };
+struct MgCol {
+    int x, y, z0, z1;
+    bool in;
+};
+__device__ __forceinline__ MgCol mgCol(const VB& vb, const MgDims& D) {
+    const unsigned gx = (unsigned)(D.X + 63) >> 6, gy = (unsigned)(D.Y + 3) >> 2;
+    const unsigned bx = vb.id % gx, by = (vb.id / gx) % gy, bz = vb.id / (gx * gy);
+    MgCol c;
+    c.x = (int)bx * 64 + (int)threadIdx.x;
+    c.y = (int)by * 4 + (int)threadIdx.y;
+    c.z0 = (int)bz * D.zc;
+    c.z1 = min(D.Z, c.z0 + D.zc);
+    c.in = c.x < D.X && c.y < D.Y && c.z0 < c.z1;
+    return c;
+}
 
-// (A x)_i on a level: d x_i - sum of couplings times neighbours.
-__device__ __forceinline__ float applyRow(const MgDims& D, const float4* __restrict__ C, const float* __restrict__ x,
-                                          long i, const float4& c) {
-    const int X = D.X, Y = D.Y;
-    const long XY = (long)X * Y;
-    const int xx = (int)(i % X), yy = (int)((i / X) % Y), zz = (int)(i / XY);
+// (A x)_i at cell (x, y, z) of a level: d x_i - sum of couplings times
+// neighbours (a -x/-y/-z coupling is that neighbour's + coupling).
+__device__ __forceinline__ float applyAt(const MgDims& D, const float4* __restrict__ C, const float* __restrict__ x,
+                                         int i, int xx, int yy, int zz, const float4& c) {
+    const int X = D.X, XY = D.X * D.Y;
+    const float* Cf = reinterpret_cast<const float*>(C);
     float s = 0.0f;
     if (xx + 1 < X) s += c.x * x[i + 1];
-    if (xx > 0) s += C[i - 1].x * x[i - 1];
-    if (yy + 1 < Y) s += c.y * x[i + X];
-    if (yy > 0) s += C[i - X].y * x[i - X];
+    if (xx > 0) s += Cf[4L * (i - 1)] * x[i - 1];
+    if (yy + 1 < D.Y) s += c.y * x[i + X];
+    if (yy > 0) s += Cf[4L * (i - X) + 1] * x[i - X];
     if (zz + 1 < D.Z) s += c.z * x[i + XY];
-    if (zz > 0) s += C[i - XY].z * x[i - XY];
+    if (zz > 0) s += Cf[4L * (i - XY) + 2] * x[i - XY];
     return c.w * x[i] - s;
 }
 
@@ -108,7 +129,8 @@ struct KMgCoarsen {
 };
 
 // One damped Jacobi sweep xout = xin + omega (r - A xin) / d (first: from
-// xin = 0, xout = omega r / d).
+// xin = 0, xout = omega r / d). The column's x at z - 1, z and z + 1 and the
+// coupling to z - 1 ride in registers.
 struct KMgSmooth {
     template <class... T>
     static __device__ __forceinline__ bool idle(const T&... a) {
@@ -118,20 +140,42 @@ struct KMgSmooth {
                                                const float* __restrict__ r, const float* __restrict__ xin,
                                                float* __restrict__ xout, const PcgState* st, int first) {
         if (st->done) return;
-        for (long i = gfirst(vb); i < D.N; i += gstride(vb)) {
+        const MgCol t = mgCol(vb, D);
+        if (!t.in) return;
+        const int X = D.X, XY = D.X * D.Y;
+        const float* Cf = reinterpret_cast<const float*>(C);
+        int i = t.x + X * t.y + XY * t.z0;
+        if (first) {
+            for (int z = t.z0; z < t.z1; z++, i += XY) {
+                const float w = Cf[4L * i + 3];
+                xout[i] = w > 0.0f ? kOmega * r[i] / w : 0.0f;
+            }
+            return;
+        }
+        float xm = t.z0 > 0 ? xin[i - XY] : 0.0f, czm = t.z0 > 0 ? Cf[4L * (i - XY) + 2] : 0.0f;
+        float xc = xin[i];
+        for (int z = t.z0; z < t.z1; z++, i += XY) {
             const float4 c = C[i];
+            const float xp = z + 1 < D.Z ? xin[i + XY] : 0.0f;
             float v = 0.0f;
             if (c.w > 0.0f) {
-                if (first) v = kOmega * r[i] / c.w;
-                else v = xin[i] + kOmega * (r[i] - applyRow(D, C, xin, i, c)) / c.w;
+                float s = czm * xm + c.z * xp;
+                if (t.x + 1 < X) s += c.x * xin[i + 1];
+                if (t.x > 0) s += Cf[4L * (i - 1)] * xin[i - 1];
+                if (t.y + 1 < D.Y) s += c.y * xin[i + X];
+                if (t.y > 0) s += Cf[4L * (i - X) + 1] * xin[i - X];
+                v = xc + kOmega * (r[i] - (c.w * xc - s)) / c.w;
             }
             xout[i] = v;
+            xm = xc;
+            xc = xp;
+            czm = c.z;
         }
     }
 };
 
 // The residual r - A x of level l summed over each coarse cell's children
-// into level l + 1's right-hand side.
+// into level l + 1's right-hand side (tiles over the coarse level).
 struct KMgRestrict {
     template <class... T>
     static __device__ __forceinline__ bool idle(const T&... a) {
@@ -141,40 +185,42 @@ struct KMgRestrict {
                                                const float* __restrict__ rf, const float* __restrict__ xf, MgDims D,
                                                float* __restrict__ rc, const PcgState* st) {
         if (st->done) return;
-        const long FXY = (long)F.X * F.Y, XY = (long)D.X * D.Y;
-        for (long k = gfirst(vb); k < D.N; k += gstride(vb)) {
-            const int I = (int)(k % D.X), J = (int)((k / D.X) % D.Y), K = (int)(k / XY);
-            const int x0 = 2 * I, y0 = 2 * J, z0 = 2 * K;
-            const int nx = min(2, F.X - x0), ny = min(2, F.Y - y0), nz = min(2, F.Z - z0);
+        const MgCol t = mgCol(vb, D);
+        if (!t.in) return;
+        const int FXY = F.X * F.Y, XY = D.X * D.Y;
+        const int x0 = 2 * t.x, y0 = 2 * t.y;
+        const int nx = min(2, F.X - x0), ny = min(2, F.Y - y0);
+        for (int K = t.z0; K < t.z1; K++) {
+            const int z0 = 2 * K, nz = min(2, F.Z - z0);
             float s = 0.0f;
             for (int c = 0; c < nz; c++)
                 for (int b = 0; b < ny; b++)
                     for (int a = 0; a < nx; a++) {
-                        const long i = (x0 + a) + (long)F.X * (y0 + b) + FXY * (z0 + c);
+                        const int i = (x0 + a) + F.X * (y0 + b) + FXY * (z0 + c);
                         const float4 cf = Cf[i];
-                        if (cf.w > 0.0f) s += rf[i] - applyRow(F, Cf, xf, i, cf);
+                        if (cf.w > 0.0f) s += rf[i] - applyAt(F, Cf, xf, i, x0 + a, y0 + b, z0 + c, cf);
                     }
-            rc[k] = s;
+            rc[t.x + D.X * t.y + XY * K] = s;
         }
     }
 };
 
-// x_l += the coarse correction of each cell's parent (sky cells).
+// x_l += the coarse correction of each cell's parent. Ground cells take it
+// too: their couplings are 0 and the next sweep sets them to 0.
 struct KMgProlong {
     template <class... T>
     static __device__ __forceinline__ bool idle(const T&... a) {
-        return nthArg<5>(a...)->done != 0;  // st
+        return nthArg<4>(a...)->done != 0;  // st
     }
-    static __device__ __forceinline__ void run(VB vb, MgDims F, const float4* __restrict__ Cf, float* __restrict__ xf,
-                                               MgDims D, const float* __restrict__ xc, const PcgState* st) {
+    static __device__ __forceinline__ void run(VB vb, MgDims F, float* __restrict__ xf, MgDims D,
+                                               const float* __restrict__ xc, const PcgState* st) {
         if (st->done) return;
-        const long FXY = (long)F.X * F.Y;
-        for (long i = gfirst(vb); i < F.N; i += gstride(vb)) {
-            if (Cf[i].w > 0.0f) {
-                const int x = (int)(i % F.X), y = (int)((i / F.X) % F.Y), z = (int)(i / FXY);
-                xf[i] += xc[(x >> 1) + (long)D.X * (y >> 1) + (long)D.X * D.Y * (z >> 1)];
-            }
-        }
+        const MgCol t = mgCol(vb, F);
+        if (!t.in) return;
+        const int FXY = F.X * F.Y, XY = D.X * D.Y;
+        int i = t.x + F.X * t.y + FXY * t.z0;
+        const int pc = (t.x >> 1) + D.X * (t.y >> 1);
+        for (int z = t.z0; z < t.z1; z++, i += FXY) xf[i] += xc[pc + XY * (z >> 1)];
     }
 };
 
@@ -248,12 +294,27 @@ struct KMgDot {
     }
 };
 
-MgDims dims(const MgLevel& L) { return MgDims{L.X, L.Y, L.Z, L.N}; }
+// A level's dims with its z-chunk (about 2048 tiles when the level has the
+// planes for it), and its tile count.
+MgDims dims(const MgLevel& L) {
+    MgDims d{L.X, L.Y, L.Z, L.N, 1};
+    if (L.N <= 0) return d;
+    const long cols = (long)((L.X + 63) / 64) * ((L.Y + 3) / 4);
+    const long nzc = std::max(1L, std::min((long)L.Z, (2048 + cols - 1) / cols));
+    d.zc = (int)((L.Z + nzc - 1) / nzc);
+    return d;
+}
+dim3 colTiles(const MgDims& d, bool on) {
+    if (!on || d.N <= 0) return dim3(0);
+    const long cols = (long)((d.X + 63) / 64) * ((d.Y + 3) / 4);
+    return dim3((unsigned)(cols * ((d.Z + d.zc - 1) / d.zc)));
+}
 // Tiles of an elementwise pass over n cells (none past the grid's depth).
 dim3 tiles(long n, bool on) {
     if (!on || n <= 0) return dim3(0);
     return dim3((unsigned)std::min<long>((n + 255) / 256, 8192));
 }
+constexpr dim3 kB3(64, 4, 1);
 
 }  // namespace
 
@@ -278,25 +339,26 @@ hipError_t launch_mg_vcycle(EnvState& e) {
     };
     for (int l = 0; l + 1 < kMgLevels; l++) {
         const bool on = l + 1 < D;
-        const dim3 g = tiles(L[l].N, on);
-        go(klaunch<KMgSmooth>(e, g, dim3(256), 0, dims(L[l]), (const float4*)L[l].C, (const float*)L[l].r,
-                              (const float*)L[l].xb, L[l].xa, st, 1));
-        go(klaunch<KMgSmooth>(e, g, dim3(256), 0, dims(L[l]), (const float4*)L[l].C, (const float*)L[l].r,
-                              (const float*)L[l].xa, L[l].xb, st, 0));
-        go(klaunch<KMgRestrict>(e, tiles(L[l + 1].N, on), dim3(256), 0, dims(L[l]), (const float4*)L[l].C,
-                                (const float*)L[l].r, (const float*)L[l].xb, dims(L[l + 1]), L[l + 1].r, st));
+        const MgDims f = dims(L[l]), c = dims(L[l + 1]);
+        const dim3 g = colTiles(f, on);
+        go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xb,
+                              L[l].xa, st, 1));
+        go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xa,
+                              L[l].xb, st, 0));
+        go(klaunch<KMgRestrict>(e, colTiles(c, on), kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
+                                (const float*)L[l].xb, c, L[l + 1].r, st));
     }
     const MgLevel& B = L[D - 1];
     go(klaunch<KMgCoarsest>(e, dim3(1), dim3(256), 0, dims(B), (const float4*)B.C, (const float*)B.r, B.xb, st));
     for (int l = kMgLevels - 2; l >= 0; l--) {
         const bool on = l + 1 < D;
-        const dim3 g = tiles(L[l].N, on);
-        go(klaunch<KMgProlong>(e, g, dim3(256), 0, dims(L[l]), (const float4*)L[l].C, L[l].xb, dims(L[l + 1]),
-                               (const float*)L[l + 1].xb, st));
-        go(klaunch<KMgSmooth>(e, g, dim3(256), 0, dims(L[l]), (const float4*)L[l].C, (const float*)L[l].r,
-                              (const float*)L[l].xb, L[l].xa, st, 0));
-        go(klaunch<KMgSmooth>(e, g, dim3(256), 0, dims(L[l]), (const float4*)L[l].C, (const float*)L[l].r,
-                              (const float*)L[l].xa, L[l].xb, st, 0));
+        const MgDims f = dims(L[l]), c = dims(L[l + 1]);
+        const dim3 g = colTiles(f, on);
+        go(klaunch<KMgProlong>(e, g, kB3, 0, f, L[l].xb, c, (const float*)L[l + 1].xb, st));
+        go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xb,
+                              L[l].xa, st, 0));
+        go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xa,
+                              L[l].xb, st, 0));
     }
     return r;
 }

@@ -133,11 +133,12 @@ void skt_server_options_default(skt_server_options* o) {
     o->max_active = 64;
     o->cloud_threshold = 1e-5f;
     skt_config_default(&o->member_config, 64, 64, 64, 100.0f);
-    // the mixed-precision form: BF16 hydrometeors, Jacobi-PCG to 1e-6 with
-    // two FP64 refinement passes
+    // the mixed-precision form: BF16 hydrometeors, multigrid-preconditioned
+    // CG to 1e-6 with two FP64 refinement passes
     o->member_config.pcg_mode = SKT_PCG_STANDARD;
+    o->member_config.pcg_precon = SKT_PRECON_MG;
     o->member_config.pcg_tol = 1e-6f;
-    o->member_config.pcg_max_iter = 400;
+    o->member_config.pcg_max_iter = 60;
     o->member_config.tracer_format = SKT_FMT_BF16;
     o->member_config.pcg_refine = 2;
     o->member_config.pcg_refine_tol = 1e-3f;

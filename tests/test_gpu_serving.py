@@ -13,7 +13,8 @@ import skewtcloudsim_amd as skt
 
 pytestmark = pytest.mark.gpu
 
-CFG = dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=300, tracer_format=1, pcg_refine=2, pcg_refine_tol=1e-3)
+CFG = dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=60, tracer_format=1, pcg_refine=2, pcg_refine_tol=1e-3,
+           pcg_precon=1)
 KEYS = ("potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures", "smallPressure")
 DIAG = ("max_updraft", "cloud_top_level", "cloud_top_height", "surface_rain")
 
