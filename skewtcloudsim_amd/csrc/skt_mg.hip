@@ -175,32 +175,66 @@ struct KMgSmooth {
 };
 
 // The residual r - A x of level l summed over each coarse cell's children
-// into level l + 1's right-hand side (tiles over the coarse level).
+// into level l + 1's right-hand side, in one pass over the fine level: a
+// block is 64 x 4 fine columns (32 x 2 coarse) marching the fine planes of
+// a coarse z-chunk; each thread forms its cells' residuals (the column's x
+// at z - 1, z, z + 1 in registers), sums each pair of planes, adds its x
+// neighbour's sum by a lane exchange and its y neighbour's through LDS.
 struct KMgRestrict {
     template <class... T>
     static __device__ __forceinline__ bool idle(const T&... a) {
         return nthArg<6>(a...)->done != 0;  // st
     }
-    static __device__ __forceinline__ void run(VB vb, MgDims F, const float4* __restrict__ Cf,
-                                               const float* __restrict__ rf, const float* __restrict__ xf, MgDims D,
+    static __device__ __forceinline__ void run(VB vb, MgDims F, const float4* __restrict__ C,
+                                               const float* __restrict__ r, const float* __restrict__ x, MgDims D,
                                                float* __restrict__ rc, const PcgState* st) {
         if (st->done) return;
-        const MgCol t = mgCol(vb, D);
-        if (!t.in) return;
-        const int FXY = F.X * F.Y, XY = D.X * D.Y;
-        const int x0 = 2 * t.x, y0 = 2 * t.y;
-        const int nx = min(2, F.X - x0), ny = min(2, F.Y - y0);
-        for (int K = t.z0; K < t.z1; K++) {
-            const int z0 = 2 * K, nz = min(2, F.Z - z0);
-            float s = 0.0f;
-            for (int c = 0; c < nz; c++)
-                for (int b = 0; b < ny; b++)
-                    for (int a = 0; a < nx; a++) {
-                        const int i = (x0 + a) + F.X * (y0 + b) + FXY * (z0 + c);
-                        const float4 cf = Cf[i];
-                        if (cf.w > 0.0f) s += rf[i] - applyAt(F, Cf, xf, i, x0 + a, y0 + b, z0 + c, cf);
-                    }
-            rc[t.x + D.X * t.y + XY * K] = s;
+        __shared__ float sh[4][64];
+        const unsigned gx = (unsigned)(F.X + 63) >> 6, gy = (unsigned)(F.Y + 3) >> 2;
+        const unsigned bx = vb.id % gx, by = (vb.id / gx) % gy, bz = vb.id / (gx * gy);
+        const int tx = threadIdx.x, ty = threadIdx.y;
+        const int fx = (int)bx * 64 + tx, fy = (int)by * 4 + ty;
+        const int zs = 2 * (int)bz * D.zc, ze = min(F.Z, 2 * ((int)bz + 1) * D.zc);
+        const bool in = fx < F.X && fy < F.Y;
+        const int X = F.X, XY = F.X * F.Y;
+        const float* Cf = reinterpret_cast<const float*>(C);
+        int i = in ? fx + X * fy + XY * zs : 0;
+        float xm = 0.0f, czm = 0.0f, xc = 0.0f;
+        if (in) {
+            xc = x[i];
+            if (zs > 0) {
+                xm = x[i - XY];
+                czm = Cf[4L * (i - XY) + 2];
+            }
+        }
+        const int cx = fx >> 1, cy = fy >> 1, DXY = D.X * D.Y;
+        float acc = 0.0f;
+        for (int z = zs; z < ze; z++) {
+            if (in) {
+                const float4 c = C[i];
+                const float xp = z + 1 < F.Z ? x[i + XY] : 0.0f;
+                if (c.w > 0.0f) {
+                    float s = czm * xm + c.z * xp;
+                    if (fx + 1 < X) s += c.x * x[i + 1];
+                    if (fx > 0) s += Cf[4L * (i - 1)] * x[i - 1];
+                    if (fy + 1 < F.Y) s += c.y * x[i + X];
+                    if (fy > 0) s += Cf[4L * (i - X) + 1] * x[i - X];
+                    acc += r[i] - (c.w * xc - s);
+                }
+                xm = xc;
+                xc = xp;
+                czm = c.z;
+                i += XY;
+            }
+            if ((z & 1) || z == F.Z - 1) {  // a coarse plane is complete (uniform over the block)
+                const float v = acc + __shfl_xor(acc, 1);
+                sh[ty][tx] = v;
+                __syncthreads();
+                if (!(ty & 1) && !(tx & 1) && cx < D.X && cy < D.Y)
+                    rc[cx + D.X * cy + DXY * (z >> 1)] = v + sh[ty + 1][tx];
+                __syncthreads();
+                acc = 0.0f;
+            }
         }
     }
 };
@@ -309,6 +343,12 @@ dim3 colTiles(const MgDims& d, bool on) {
     const long cols = (long)((d.X + 63) / 64) * ((d.Y + 3) / 4);
     return dim3((unsigned)(cols * ((d.Z + d.zc - 1) / d.zc)));
 }
+// Tiles of a restriction: the fine level's columns times the coarse z-chunks.
+dim3 restrictTiles(const MgDims& f, const MgDims& c, bool on) {
+    if (!on || c.N <= 0) return dim3(0);
+    const long cols = (long)((f.X + 63) / 64) * ((f.Y + 3) / 4);
+    return dim3((unsigned)(cols * ((c.Z + c.zc - 1) / c.zc)));
+}
 // Tiles of an elementwise pass over n cells (none past the grid's depth).
 dim3 tiles(long n, bool on) {
     if (!on || n <= 0) return dim3(0);
@@ -345,7 +385,7 @@ hipError_t launch_mg_vcycle(EnvState& e) {
                               L[l].xa, st, 1));
         go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xa,
                               L[l].xb, st, 0));
-        go(klaunch<KMgRestrict>(e, colTiles(c, on), kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
+        go(klaunch<KMgRestrict>(e, restrictTiles(f, c, on), kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
                                 (const float*)L[l].xb, c, L[l + 1].r, st));
     }
     const MgLevel& B = L[D - 1];
