@@ -56,10 +56,11 @@ KEYS = ("potTemps", "velField", "Qv", "groundTemp", "groundPres", "pressures", "
 STANDARD = dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=1000)
 # the mixed-precision form: 16-bit hydrometeors, multigrid-preconditioned CG
 # to a relative residual of 1e-6 with two FP64 refinement passes
-# (at most 60 iterations per solve: an ensemble step holds that many per pass)
-MIXED = {"bf16": dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=60, tracer_format=1, pcg_refine=2,
+# (at most 20 iterations per solve: an ensemble step holds that many per
+# pass; a multigrid solve takes about 10 per pass here)
+MIXED = {"bf16": dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=20, tracer_format=1, pcg_refine=2,
                       pcg_refine_tol=1e-3, pcg_precon=1),
-         "f16": dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=60, tracer_format=2, pcg_refine=2,
+         "f16": dict(pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=20, tracer_format=2, pcg_refine=2,
                      pcg_refine_tol=1e-3, pcg_precon=1)}
 ENSEMBLE_GRIDS = [(64, 64, 64), (128, 128, 128), (192, 96, 192), (128, 160, 128)]  # nx, ny vertical, nz
 SERVE_RATES = (2.0, 5.0, 10.0, 20.0)

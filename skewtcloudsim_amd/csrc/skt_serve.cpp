@@ -138,7 +138,7 @@ void skt_server_options_default(skt_server_options* o) {
     o->member_config.pcg_mode = SKT_PCG_STANDARD;
     o->member_config.pcg_precon = SKT_PRECON_MG;
     o->member_config.pcg_tol = 1e-6f;
-    o->member_config.pcg_max_iter = 60;
+    o->member_config.pcg_max_iter = 20;
     o->member_config.tracer_format = SKT_FMT_BF16;
     o->member_config.pcg_refine = 2;
     o->member_config.pcg_refine_tol = 1e-3f;
