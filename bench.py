@@ -131,9 +131,10 @@ def cpu_baseline(snd_args, threads, steps):
     o.step(DT, 1.0)
     t0 = time.perf_counter()
     its = []
-    for _ in range(steps):
+    for k in range(steps):
         o.step(DT, 1.0)
         its.append(o.stats()["pcg_iterations"])
+        print(f"cpu baseline step {k + 1}/{steps}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     env = skt.EnvironmentGPU(NX, NY, NZ, DX)
     env.init(*snd_args)
@@ -160,6 +161,42 @@ def cpu_baseline(snd_args, threads, steps):
     o1.close()
     one = n1 ** 3 / (t3 - t2)
     return (NX * NY * NZ) * steps / (t1 - t0), (t1 - t0) / steps, its, one, t3 - t2, check
+
+
+def cpu_single_bubble(threads, steps=200):
+    """The plain method's CPU-runnable case: 64^3, the fixed single-bubble
+    sounding, Jacobi-PCG to a relative residual of 1e-6, 'steps' steps of
+    dt 0.5 s, on the oracle with 'threads' threads and on the GPU."""
+    import skewtcloudsim_amd as skt
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle
+    n = 64
+    snd = skt.make_sounding(n, n, n, DX, seed=SEED, n_bubbles=0)
+    args = [snd[k] for k in KEYS]
+    o = Oracle(n, n, n, DX, threads=threads, **{k: v for k, v in STANDARD.items()})
+    o.init(*args)
+    its = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        o.step(DT, 1.0)
+        its.append(o.stats()["pcg_iterations"])
+        if k % 20 == 19:
+            print(f"cpu 64^3 step {k + 1}/{steps}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    cpu_s = time.perf_counter() - t0
+    o.close()
+    env = skt.EnvironmentGPU(n, n, n, DX, **STANDARD)
+    env.init(*args)
+    env.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        env.updateGPU(DT, 1.0)
+    env.device_synchronize()
+    gpu_s = time.perf_counter() - t0
+    env.close()
+    cells = n ** 3 * steps
+    return {"grid": [n, n, n], "steps": steps, "threads": threads, "cpu_s": cpu_s, "cpu_cell_steps_per_s": cells / cpu_s,
+            "gpu_s": gpu_s, "gpu_cell_steps_per_s": cells / gpu_s, "gpu_speedup": cpu_s / gpu_s,
+            "pcg_iterations_oracle": {"min": min(its), "max": max(its), "mean": sum(its) / len(its)}}
 
 
 def traffic_from_profile(kernel_class):
@@ -405,6 +442,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="OpenMP threads of the CPU baseline "
                     "(default: the CPUs this process may use, at most 16)")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-single-bubble", action="store_true",
+                    help="also run the 64^3 single-bubble case for 200 steps on the CPU (several minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the step is single-domain, one GPU per "
@@ -480,6 +519,8 @@ def main():
                "host": info, "one_thread": {"value": one, "unit": "cell-steps/s",
                                             "sample": f"one 64^3 step after a warm-up step ({one_s:.1f} s)"},
                "gpu_speedup": value / cps}
+        if args.cpu_single_bubble:
+            cpu["single_bubble_64cubed_200_steps"] = cpu_single_bubble(threads)
 
     out = {
         "metric": "cell-steps/s at 256x256x128 (FP32, reference solver: 200 PCG iterations per step)",
