@@ -140,7 +140,8 @@ def test_refined_solution_matches_a_tight_solution():
 def test_mixed_steps_against_the_fp64_oracle():
     """20 steps from a state with every hydrometeor, a cold upper half and
     winds: the FP32 step, the BF16 form and the FP16 form (both with FP64
-    refinement) against the FP64 oracle. Per field, relative L2 over the sky
+    refinement), and the BF16 form with the multigrid preconditioner,
+    against the FP64 oracle. Per field, relative L2 over the sky
     cells. Each mixed form stays within twice the FP32 step's distance plus
     a storage term: half a 16-bit unit in the last place (2^-9 BF16, 2^-11
     FP16) times the number of rounded stores a value takes (about ten per
@@ -158,8 +159,11 @@ def test_mixed_steps_against_the_fp64_oracle():
     for f in ob.FIELDS:
         o64.set(f, o32.get(f))
     mixed = {}
-    for name, fmt in FMT.items():
-        e, _ = _env(shape, tracer_format=fmt, pcg_refine=2, pcg_refine_tol=1e-3, **cfg)
+    variants = [(name, dict(tracer_format=fmt)) for name, fmt in FMT.items()]
+    # the benchmarked form: BF16 with the multigrid preconditioner
+    variants.append(("bf16+mg", dict(tracer_format=1, pcg_precon=1)))
+    for name, extra in variants:
+        e, _ = _env(shape, pcg_refine=2, pcg_refine_tol=1e-3, **extra, **cfg)
         for f in ob.FIELDS:
             e.set(f, o32.get(f))
         mixed[name] = e
@@ -170,7 +174,7 @@ def test_mixed_steps_against_the_fp64_oracle():
         for e in mixed.values():
             e.updateGPU(0.5, 1.0)
     m = sky_mask(*shape)
-    ulp = {"bf16": 2.0 ** -9, "f16": 2.0 ** -11}
+    ulp = {"bf16": 2.0 ** -9, "f16": 2.0 ** -11, "bf16+mg": 2.0 ** -9}
     rows = []
     for f in ("POTTEMP", "QV", "PRESSURE", "VELX", "VELY", "VELZ") + TRACERS:
         d = o64.get64(f)[m]
@@ -183,9 +187,9 @@ def test_mixed_steps_against_the_fp64_oracle():
             assert em <= 2.0 * e32 + storage, (name, f, em, e32)
             row.append(em)
         rows.append(row)
-    print("field      fp32-fp64  bf16-fp64  f16-fp64", flush=True)
+    print("field      fp32-fp64  " + "  ".join(f"{n + '-fp64':>12}" for n in mixed), flush=True)
     for r in rows:
-        print(f"{r[0]:<10} {r[1]:9.2e}  {r[2]:9.2e}  {r[3]:9.2e}", flush=True)
+        print(f"{r[0]:<10} {r[1]:9.2e}  " + "  ".join(f"{v:12.2e}" for v in r[2:]), flush=True)
     for e in [env32, *mixed.values()]:
         e.close()
     o32.close()
