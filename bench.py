@@ -347,6 +347,8 @@ def run_ensemble(steps, members=64):
             ens.step(DT, 1.0)
         ens.synchronize()
         bat = (time.perf_counter() - t0) / steps
+        prof = ens.profile(DT, 1.0)  # one more step, timed launch by launch
+        top = sorted(prof.items(), key=lambda kv: -kv[1][0])[:12]
         ens.close()
         for e in mixed:
             e.close()
@@ -356,7 +358,8 @@ def run_ensemble(steps, members=64):
                      "member_steps_per_s": members / bat, "member_steps_per_s_sequential": members / seq,
                      "members_per_s_600s_runs": members / (1200 * bat),
                      "members_per_s_600s_runs_sequential": members / (1200 * seq),
-                     "speedup": seq / bat, "speedup_batching_alone": seq_mixed / bat, "autotune": tun}
+                     "speedup": seq / bat, "speedup_batching_alone": seq_mixed / bat, "autotune": tun,
+                     "bodies_ms_one_step": {k: round(v[0], 3) for k, v in top}}
         print(f"ensemble leg {case}: sequential {seq*1e3:.1f} ms, ensemble {bat*1e3:.1f} ms", file=sys.stderr,
               flush=True)
     return out

@@ -299,7 +299,7 @@ typedef struct skt_batch_tuning {
     double step_ms[3];       /* one ensemble step with every body at launch width 0, 1, 2 */
     double tuned_step_ms;    /* the sum of each body's fastest width */
     int32_t bodies;          /* distinct kernel bodies of the step */
-    int32_t persistent_bodies; /* bodies whose fastest width is persistent */
+    int32_t persistent_bodies; /* launches of the step whose fastest width is persistent */
 } skt_batch_tuning;
 void skt_batch_options_default(skt_batch_options* o);
 int skt_batch_create(int32_t device, skt_batch** out);
@@ -316,9 +316,13 @@ int skt_batch_step(skt_batch* b, float dt, float speed);
  * more substeps than the cap since the last call. */
 int skt_batch_synchronize(skt_batch* b);
 /* Runs three ensemble steps (ordinary steps of every member), one per launch
- * width, timing every launch, and keeps each body's fastest width for this
+ * width, timing every launch, and keeps each launch's fastest width for this
  * set of member shapes (used by every later step of such an ensemble). */
 int skt_batch_autotune(skt_batch* b, float dt, float speed, skt_batch_tuning* out);
+/* Measurement: one ordinary ensemble step issued launch by launch with events
+ * around each, summed per kernel body into 'out' as text lines
+ * "body<TAB>milliseconds<TAB>launches" (at most cap - 1 characters). */
+int skt_batch_profile(skt_batch* b, float dt, float speed, char* out, int64_t cap);
 
 
 /* Serving diagnostics reduced on the device (maximum updraft, cloud-top level

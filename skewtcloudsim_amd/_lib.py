@@ -205,6 +205,7 @@ SIGNATURES = {
     "skt_batch_step": (_I, [_P, _F, _F]),
     "skt_batch_synchronize": (_I, [_P]),
     "skt_batch_autotune": (_I, [_P, _F, _F, ctypes.POINTER(BatchTuning)]),
+    "skt_batch_profile": (_I, [_P, _F, _F, ctypes.c_char_p, ctypes.c_int64]),
     "skt_env_footprint": (_I, [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_int64)]),
     "skt_env_device_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "skt_server_options_default": (None, [ctypes.POINTER(ServerOptions)]),

@@ -15,12 +15,15 @@
 // Launch width: each body runs either one block per work item or a
 // persistent grid of one or two rounds of resident blocks striding over the
 // work list. skt_batch_autotune times the three on the ensemble and keeps the
-// fastest per body, keyed by the body and the members' grid shapes.
+// fastest per launch of the plan, keyed by the body, its index in the plan
+// and the members' grid shapes.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <set>
+#include <tuple>
 #include <string>
 #include <vector>
 #include "skt_internal.h"
@@ -72,7 +75,8 @@ struct skt_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // launch width per (body, member shapes)
-    std::map<std::pair<const void*, std::string>, int> tuned;
+    // launch width per plan launch: (body, index in the plan, member shapes)
+    std::map<std::tuple<const void*, size_t, std::string>, int> tuned;
     long long steps = 0;
 };
 
@@ -216,7 +220,7 @@ int buildPlan(skt_batch* b, float dt, float speed) {
                 wm.insert(wm.end(), st.begin(), st.end());
             }
             P.wm_off = it->second;
-            auto t = b->tuned.find({(const void*)P.proto.fn, key});
+            auto t = b->tuned.find({(const void*)P.proto.fn, b->plan.size(), key});
             P.mode = t != b->tuned.end() ? t->second : b->opt.launch_mode;
             P.grid = gridFor(b, P, P.mode);
             b->plan.push_back(std::move(P));
@@ -467,7 +471,8 @@ int skt_batch_autotune(skt_batch* b, float dt, float speed, skt_batch_tuning* ou
     const size_t L = b->plan.size();
     std::vector<hipEvent_t> ev(L + 1);
     for (auto& x : ev) BCHK(b, hipEventCreate(&x));
-    std::map<const void*, double> ms[kModes];
+    std::vector<double> ms[kModes];
+    for (auto& m : ms) m.assign(L, 0.0);
     int rc = SKT_OK;
     for (int mode = 0; mode < kModes && !rc; mode++) {
         rc = beforeStep(b);
@@ -485,32 +490,82 @@ int skt_batch_autotune(skt_batch* b, float dt, float speed, skt_batch_tuning* ou
         for (size_t i = 0; i < L; i++) {
             float t = 0.0f;
             (void)hipEventElapsedTime(&t, ev[i], ev[i + 1]);
-            ms[mode][(const void*)b->plan[i].proto.fn] += t;
+            ms[mode][i] = t;
         }
     }
     for (auto& x : ev) (void)hipEventDestroy(x);
     if (rc) return rc;
     const std::string key = shapesKey(b);
+    // each launch of the plan keeps its own fastest width: a body's launches
+    // differ in size (multigrid levels) and in how many members are idle
     double tot[kModes] = {0, 0, 0}, best = 0.0;
-    int bodies = 0, persistent = 0;
-    for (const auto& kv : ms[0]) {
+    int persistent = 0;
+    std::set<const void*> bodies;
+    for (size_t i = 0; i < L; i++) {
         int bm = 0;
         for (int mode = 0; mode < kModes; mode++) {
-            tot[mode] += ms[mode][kv.first];
-            if (ms[mode][kv.first] < ms[bm][kv.first]) bm = mode;
+            tot[mode] += ms[mode][i];
+            if (ms[mode][i] < ms[bm][i]) bm = mode;
         }
-        best += ms[bm][kv.first];
-        b->tuned[{kv.first, key}] = bm;
-        bodies++;
+        best += ms[bm][i];
+        const void* fn = (const void*)b->plan[i].proto.fn;
+        b->tuned[{fn, i, key}] = bm;
+        bodies.insert(fn);
         persistent += bm != 0;
     }
     if (out) {
         for (int mode = 0; mode < kModes; mode++) out->step_ms[mode] = tot[mode];
         out->tuned_step_ms = best;
-        out->bodies = bodies;
+        out->bodies = (int32_t)bodies.size();
         out->persistent_bodies = persistent;
     }
     b->dirty = true;
+    return SKT_OK;
+}
+
+
+int skt_batch_profile(skt_batch* b, float dt, float speed, char* out, int64_t cap) {
+    if (!b || (cap > 0 && !out)) return SKT_ERR_ARG;
+    if (b->members.empty()) return SKT_OK;
+    BCHK(b, hipSetDevice(b->device));
+    if (b->dirty || dt != b->plan_dt || speed != b->plan_speed) {
+        const int rc = buildPlan(b, dt, speed);
+        if (rc) return rc;
+    }
+    const size_t L = b->plan.size();
+    std::vector<hipEvent_t> ev(L + 1);
+    for (auto& x : ev) BCHK(b, hipEventCreate(&x));
+    int rc = beforeStep(b);
+    for (size_t i = 0; i < L && !rc; i++) {
+        if (hipEventRecord(ev[i], b->stream) != hipSuccess) rc = berr(b, SKT_ERR_HIP, "event record");
+        else if (issue(b, b->plan[i]) != hipSuccess) rc = berr(b, SKT_ERR_HIP, "profiled launch failed");
+    }
+    if (!rc && (hipEventRecord(ev[L], b->stream) != hipSuccess || hipEventSynchronize(ev[L]) != hipSuccess))
+        rc = berr(b, SKT_ERR_HIP, "event synchronize");
+    std::map<std::string, std::pair<double, int>> per;
+    if (!rc) {
+        afterBatchStep(b, dt, speed);
+        for (size_t i = 0; i < L; i++) {
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+            std::string n = b->plan[i].proto.name ? b->plan[i].proto.name : "?";
+            const size_t k = n.find("K = ");
+            if (k != std::string::npos) n = n.substr(k + 4, n.size() - k - 5);
+            auto& p = per[n];
+            p.first += t;
+            p.second++;
+        }
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    if (rc) return rc;
+    std::string text;
+    for (const auto& kv : per)
+        text += kv.first + "\t" + std::to_string(kv.second.first) + "\t" + std::to_string(kv.second.second) + "\n";
+    if (cap > 0) {
+        const size_t n = std::min<size_t>(text.size(), (size_t)cap - 1);
+        std::memcpy(out, text.data(), n);
+        out[n] = 0;
+    }
     return SKT_OK;
 }
 

@@ -117,6 +117,7 @@ struct RecLaunch {
     hipError_t (*fn)(const void* dargs, const WorkMap& wm, unsigned grid, dim3 block, uint32_t shm,
                      hipStream_t s) = nullptr;
     int (*occ)(int threads, uint32_t shm) = nullptr;
+    const char* name = nullptr;  // the body's name (measurement reports)
     unsigned nb = 0;
     dim3 block;
     uint32_t shm = 0;

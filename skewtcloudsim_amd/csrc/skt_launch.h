@@ -154,6 +154,13 @@ int batchOccupancy(int threads, uint32_t shm) {
     return nb;
 }
 
+// The name of body K, from the compiler's signature of this function
+// ("... [K = skt::KPcgA<false, true>]").
+template <class K>
+const char* bodyName() {
+    return __PRETTY_FUNCTION__;
+}
+
 // Launch of body K with 'grid' tiles (its x, y and z counts multiplied: the
 // bodies take a one-dimensional tile index) on the env's stream. When a
 // timed launch has armed kevA/kevB, the events are attached to this dispatch
@@ -169,6 +176,7 @@ hipError_t klaunch(EnvState& e, dim3 grid, dim3 block, uint32_t shm, A... args) 
         RecLaunch L;
         L.fn = &batchLaunch<K, A...>;
         L.occ = &batchOccupancy<K, A...>;
+        L.name = bodyName<K>();
         L.nb = (unsigned)nb;
         L.block = block;
         L.shm = shm;

@@ -32,6 +32,7 @@ namespace {
 
 constexpr float kOmega = 0.8f;     // Jacobi damping
 constexpr int kCoarseSweeps = 64;  // sweeps on the coarsest level (even: the result lands in xb)
+constexpr int kSweeps = 2;         // Jacobi sweeps before and after each coarse correction (1 or 2)
 
 __device__ __forceinline__ long gstride(const VB& vb) { return (long)vb.n * 256; }
 __device__ __forceinline__ long gfirst(const VB& vb) { return (long)vb.id * 256 + tid256(); }
@@ -377,16 +378,20 @@ hipError_t launch_mg_vcycle(EnvState& e) {
     auto go = [&](hipError_t x) {
         if (r == hipSuccess) r = x;
     };
+    // pre-smoothing ends in xa (one sweep) or xb (two); the post-smoothing
+    // starts there and ends in xb, each level's result
+    auto fpre = [&](int l) { return kSweeps == 2 ? L[l].xb : L[l].xa; };
     for (int l = 0; l + 1 < kMgLevels; l++) {
         const bool on = l + 1 < D;
         const MgDims f = dims(L[l]), c = dims(L[l + 1]);
         const dim3 g = colTiles(f, on);
         go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xb,
                               L[l].xa, st, 1));
-        go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xa,
-                              L[l].xb, st, 0));
+        if (kSweeps == 2)
+            go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
+                                  (const float*)L[l].xa, L[l].xb, st, 0));
         go(klaunch<KMgRestrict>(e, restrictTiles(f, c, on), kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
-                                (const float*)L[l].xb, c, L[l + 1].r, st));
+                                (const float*)fpre(l), c, L[l + 1].r, st));
     }
     const MgLevel& B = L[D - 1];
     go(klaunch<KMgCoarsest>(e, dim3(1), dim3(256), 0, dims(B), (const float4*)B.C, (const float*)B.r, B.xb, st));
@@ -394,9 +399,10 @@ hipError_t launch_mg_vcycle(EnvState& e) {
         const bool on = l + 1 < D;
         const MgDims f = dims(L[l]), c = dims(L[l + 1]);
         const dim3 g = colTiles(f, on);
-        go(klaunch<KMgProlong>(e, g, kB3, 0, f, L[l].xb, c, (const float*)L[l + 1].xb, st));
-        go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xb,
-                              L[l].xa, st, 0));
+        go(klaunch<KMgProlong>(e, g, kB3, 0, f, fpre(l), c, (const float*)L[l + 1].xb, st));
+        if (kSweeps == 2)
+            go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
+                                  (const float*)L[l].xb, L[l].xa, st, 0));
         go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xa,
                               L[l].xb, st, 0));
     }

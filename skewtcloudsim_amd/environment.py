@@ -469,6 +469,16 @@ class EnsembleGPU:
         return {"step_ms_by_width": list(t.step_ms), "tuned_step_ms": t.tuned_step_ms, "bodies": t.bodies,
                 "persistent_bodies": t.persistent_bodies}
 
+    def profile(self, dt, speed=1.0):
+        """One ordinary ensemble step timed launch by launch: {body: (ms, launches)}."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._chk(self.lib.skt_batch_profile(self.h, dt, speed, buf, len(buf)), "profile")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, ms, n = line.split("\t")
+            out[name] = (float(ms), int(n))
+        return out
+
     def close(self):
         if self.h:
             self.lib.skt_batch_destroy(self.h)
