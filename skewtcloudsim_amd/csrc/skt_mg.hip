@@ -175,6 +175,110 @@ struct KMgSmooth {
     }
 };
 
+// The two pre-smoothing sweeps in one pass: the first sweep from zero gives
+// a_j = omega r_j / d_j at every cell, which the second sweep's stencil
+// reads at the cell and its six neighbours, so it is formed where it is
+// needed: xout = a + omega (r - A a) / d.
+struct KMgPreSmooth {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        return nthArg<4>(a...)->done != 0;  // st
+    }
+    static __device__ __forceinline__ void run(VB vb, MgDims D, const float4* __restrict__ C,
+                                               const float* __restrict__ r, float* __restrict__ xout,
+                                               const PcgState* st) {
+        if (st->done) return;
+        const MgCol t = mgCol(vb, D);
+        if (!t.in) return;
+        const int X = D.X, XY = D.X * D.Y;
+        auto a0 = [&](const float4& c, float rv) { return c.w > 0.0f ? kOmega * rv / c.w : 0.0f; };
+        int i = t.x + X * t.y + XY * t.z0;
+        float4 cm{0.0f, 0.0f, 0.0f, 0.0f}, cc = C[i];
+        float am = 0.0f, ac = a0(cc, r[i]);
+        if (t.z0 > 0) {
+            cm = C[i - XY];
+            am = a0(cm, r[i - XY]);
+        }
+        for (int z = t.z0; z < t.z1; z++, i += XY) {
+            float4 cp{0.0f, 0.0f, 0.0f, 0.0f};
+            float ap = 0.0f;
+            if (z + 1 < D.Z) {
+                cp = C[i + XY];
+                ap = a0(cp, r[i + XY]);
+            }
+            float v = 0.0f;
+            if (cc.w > 0.0f) {
+                float s = cm.z * am + cc.z * ap;
+                if (t.x + 1 < X) s += cc.x * a0(C[i + 1], r[i + 1]);
+                if (t.x > 0) {
+                    const float4 l = C[i - 1];
+                    s += l.x * a0(l, r[i - 1]);
+                }
+                if (t.y + 1 < D.Y) s += cc.y * a0(C[i + X], r[i + X]);
+                if (t.y > 0) {
+                    const float4 d = C[i - X];
+                    s += d.y * a0(d, r[i - X]);
+                }
+                v = ac + kOmega * (r[i] - (cc.w * ac - s)) / cc.w;
+            }
+            xout[i] = v;
+            cm = cc;
+            am = ac;
+            cc = cp;
+            ac = ap;
+        }
+    }
+};
+
+// The coarse correction added to x and the first post-smoothing sweep in one
+// pass: the stencil reads x + (the parent's correction) at the cell and its
+// six neighbours; xout = xe + omega (r - A xe) / d with xe corrected.
+struct KMgProlongSmooth {
+    template <class... T>
+    static __device__ __forceinline__ bool idle(const T&... a) {
+        return nthArg<7>(a...)->done != 0;  // st
+    }
+    static __device__ __forceinline__ void run(VB vb, MgDims F, const float4* __restrict__ C,
+                                               const float* __restrict__ r, const float* __restrict__ xin,
+                                               float* __restrict__ xout, MgDims D, const float* __restrict__ xc,
+                                               const PcgState* st) {
+        if (st->done) return;
+        const MgCol t = mgCol(vb, F);
+        if (!t.in) return;
+        const int X = F.X, XY = F.X * F.Y, DXY = D.X * D.Y;
+        const float* Cf = reinterpret_cast<const float*>(C);
+        // the parent rows of the column and of its -x/+x/-y/+y neighbours
+        const int px = t.x >> 1, py = t.y >> 1;
+        const int pxl = t.x > 0 ? (t.x - 1) >> 1 : 0, pxr = t.x + 1 < X ? (t.x + 1) >> 1 : 0;
+        const int pyd = t.y > 0 ? (t.y - 1) >> 1 : 0, pyu = t.y + 1 < F.Y ? (t.y + 1) >> 1 : 0;
+        auto corr = [&](int cx, int cy, int z) { return xc[cx + D.X * cy + DXY * (z >> 1)]; };
+        int i = t.x + X * t.y + XY * t.z0;
+        float xm = 0.0f, czm = 0.0f;
+        if (t.z0 > 0) {
+            xm = xin[i - XY] + corr(px, py, t.z0 - 1);
+            czm = Cf[4L * (i - XY) + 2];
+        }
+        float xe = xin[i] + corr(px, py, t.z0);
+        for (int z = t.z0; z < t.z1; z++, i += XY) {
+            const float4 c = C[i];
+            const float xp = z + 1 < F.Z ? xin[i + XY] + corr(px, py, z + 1) : 0.0f;
+            float v = 0.0f;
+            if (c.w > 0.0f) {
+                float s = czm * xm + c.z * xp;
+                if (t.x + 1 < X) s += c.x * (xin[i + 1] + corr(pxr, py, z));
+                if (t.x > 0) s += Cf[4L * (i - 1)] * (xin[i - 1] + corr(pxl, py, z));
+                if (t.y + 1 < F.Y) s += c.y * (xin[i + X] + corr(px, pyu, z));
+                if (t.y > 0) s += Cf[4L * (i - X) + 1] * (xin[i - X] + corr(px, pyd, z));
+                v = xe + kOmega * (r[i] - (c.w * xe - s)) / c.w;
+            }
+            xout[i] = v;
+            xm = xe;
+            xe = xp;
+            czm = c.z;
+        }
+    }
+};
+
 // The residual r - A x of level l summed over each coarse cell's children
 // into level l + 1's right-hand side, in one pass over the fine level: a
 // block is 64 x 4 fine columns (32 x 2 coarse) marching the fine planes of
@@ -237,25 +341,6 @@ struct KMgRestrict {
                 acc = 0.0f;
             }
         }
-    }
-};
-
-// x_l += the coarse correction of each cell's parent. Ground cells take it
-// too: their couplings are 0 and the next sweep sets them to 0.
-struct KMgProlong {
-    template <class... T>
-    static __device__ __forceinline__ bool idle(const T&... a) {
-        return nthArg<4>(a...)->done != 0;  // st
-    }
-    static __device__ __forceinline__ void run(VB vb, MgDims F, float* __restrict__ xf, MgDims D,
-                                               const float* __restrict__ xc, const PcgState* st) {
-        if (st->done) return;
-        const MgCol t = mgCol(vb, F);
-        if (!t.in) return;
-        const int FXY = F.X * F.Y, XY = D.X * D.Y;
-        int i = t.x + F.X * t.y + FXY * t.z0;
-        const int pc = (t.x >> 1) + D.X * (t.y >> 1);
-        for (int z = t.z0; z < t.z1; z++, i += FXY) xf[i] += xc[pc + XY * (z >> 1)];
     }
 };
 
@@ -378,20 +463,16 @@ hipError_t launch_mg_vcycle(EnvState& e) {
     auto go = [&](hipError_t x) {
         if (r == hipSuccess) r = x;
     };
-    // pre-smoothing ends in xa (one sweep) or xb (two); the post-smoothing
-    // starts there and ends in xb, each level's result
-    auto fpre = [&](int l) { return kSweeps == 2 ? L[l].xb : L[l].xa; };
+    // two Jacobi sweeps before (one fused pass into xb) and after (the
+    // coarse correction and the first sweep fused into xa, the second back
+    // into xb, each level's result)
     for (int l = 0; l + 1 < kMgLevels; l++) {
         const bool on = l + 1 < D;
         const MgDims f = dims(L[l]), c = dims(L[l + 1]);
-        const dim3 g = colTiles(f, on);
-        go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xb,
-                              L[l].xa, st, 1));
-        if (kSweeps == 2)
-            go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
-                                  (const float*)L[l].xa, L[l].xb, st, 0));
+        go(klaunch<KMgPreSmooth>(e, colTiles(f, on), kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, L[l].xb,
+                                 st));
         go(klaunch<KMgRestrict>(e, restrictTiles(f, c, on), kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
-                                (const float*)fpre(l), c, L[l + 1].r, st));
+                                (const float*)L[l].xb, c, L[l + 1].r, st));
     }
     const MgLevel& B = L[D - 1];
     go(klaunch<KMgCoarsest>(e, dim3(1), dim3(256), 0, dims(B), (const float4*)B.C, (const float*)B.r, B.xb, st));
@@ -399,10 +480,8 @@ hipError_t launch_mg_vcycle(EnvState& e) {
         const bool on = l + 1 < D;
         const MgDims f = dims(L[l]), c = dims(L[l + 1]);
         const dim3 g = colTiles(f, on);
-        go(klaunch<KMgProlong>(e, g, kB3, 0, f, fpre(l), c, (const float*)L[l + 1].xb, st));
-        if (kSweeps == 2)
-            go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
-                                  (const float*)L[l].xb, L[l].xa, st, 0));
+        go(klaunch<KMgProlongSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
+                                     (const float*)L[l].xb, L[l].xa, c, (const float*)L[l + 1].xb, st));
         go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xa,
                               L[l].xb, st, 0));
     }
