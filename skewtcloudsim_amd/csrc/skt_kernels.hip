@@ -44,6 +44,13 @@ static ZGrid zgridChunks(const EnvState& e, long nzc, int rows = 4) {
     return ZGrid{dim3(nb), zc, nb};
 }
 ZGrid zgrid(const EnvState& e) { return e.gZ; }
+// Chunks of the stencil passes in an ensemble step: the members together
+// fill the device, so each member's tiles take 16 planes where its own
+// launch splits z finer to fill the device alone (a pure stencil gives the
+// same values for any chunking).
+constexpr int kEnsembleZc = 16;
+static long ensembleChunks(const EnvState& e) { return (e.Z + kEnsembleZc - 1) / kEnsembleZc; }
+
 void setup_grids(EnvState& e, ResidentBlocks rb) {
     const long cols = zcols(e), cap = e.cfg.max_blocks;
     // about 2048 blocks (at least one chunk per tile column), or at most cap
@@ -356,7 +363,7 @@ struct KDiffuse {
     }
 };
 hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k) {
-    const ZGrid zg = e.gD;
+    const ZGrid zg = e.rec ? zgridChunks(e, ensembleChunks(e), 8) : e.gD;
     if ((double)(zg.zc + 3) * (double)e.X * (double)e.Y * 4.0 >= 4294967296.0) return hipErrorInvalidValue;
     if (e.mix)
         return klaunch<KDiffuse<true>>(e, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k,
@@ -796,12 +803,12 @@ struct KPpmZ {
 };
 
 hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) {
-    const ZGrid zg = zgrid(e);
+    const ZGrid zg = e.rec ? zgridChunks(e, ensembleChunks(e)) : zgrid(e);
     const dim3 grid(zg.nblocks * pb.n);
     if (dir == 0) {
         // rows of four per block, chunks of zc planes: about 2048 blocks per field
         const long gy = (e.Y + 3) / 4;
-        const long nzc = std::max(1L, std::min((long)e.Z, 2048 / gy));
+        const long nzc = e.rec ? ensembleChunks(e) : std::max(1L, std::min((long)e.Z, 2048 / gy));
         const int zc = (int)((e.Z + nzc - 1) / nzc);
         const unsigned nbf = (unsigned)(gy * ((e.Z + zc - 1) / zc));
         return e.mix ? klaunch<KPpmXr<true>>(e, dim3(nbf * pb.n), B3, 0, geom(e), zc, nbf, pb, dt, e.dx,

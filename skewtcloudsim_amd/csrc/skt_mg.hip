@@ -416,11 +416,13 @@ struct KMgDot {
 
 // A level's dims with its z-chunk (about 2048 tiles when the level has the
 // planes for it), and its tile count.
-MgDims dims(const MgLevel& L) {
+// In an ensemble step (ens) the chunks are 16 planes: the members together
+// fill the device (no level kernel's values depend on its chunking).
+MgDims dims(const MgLevel& L, bool ens = false) {
     MgDims d{L.X, L.Y, L.Z, L.N, 1};
     if (L.N <= 0) return d;
     const long cols = (long)((L.X + 63) / 64) * ((L.Y + 3) / 4);
-    const long nzc = std::max(1L, std::min((long)L.Z, (2048 + cols - 1) / cols));
+    const long nzc = ens ? (L.Z + 15) / 16 : std::max(1L, std::min((long)L.Z, (2048 + cols - 1) / cols));
     d.zc = (int)((L.Z + nzc - 1) / nzc);
     return d;
 }
@@ -457,6 +459,7 @@ hipError_t launch_mg_setup(EnvState& e) {
 
 hipError_t launch_mg_vcycle(EnvState& e) {
     const MgLevel* L = e.mg;
+    const bool ens = e.rec != nullptr;
     const PcgState* st = &e.sc->st[1];
     const int D = e.mg_depth;
     hipError_t r = hipSuccess;
@@ -468,7 +471,7 @@ hipError_t launch_mg_vcycle(EnvState& e) {
     // into xb, each level's result)
     for (int l = 0; l + 1 < kMgLevels; l++) {
         const bool on = l + 1 < D;
-        const MgDims f = dims(L[l]), c = dims(L[l + 1]);
+        const MgDims f = dims(L[l], ens), c = dims(L[l + 1], ens);
         go(klaunch<KMgPreSmooth>(e, colTiles(f, on), kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, L[l].xb,
                                  st));
         go(klaunch<KMgRestrict>(e, restrictTiles(f, c, on), kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
@@ -478,7 +481,7 @@ hipError_t launch_mg_vcycle(EnvState& e) {
     go(klaunch<KMgCoarsest>(e, dim3(1), dim3(256), 0, dims(B), (const float4*)B.C, (const float*)B.r, B.xb, st));
     for (int l = kMgLevels - 2; l >= 0; l--) {
         const bool on = l + 1 < D;
-        const MgDims f = dims(L[l]), c = dims(L[l + 1]);
+        const MgDims f = dims(L[l], ens), c = dims(L[l + 1], ens);
         const dim3 g = colTiles(f, on);
         go(klaunch<KMgProlongSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
                                      (const float*)L[l].xb, L[l].xa, c, (const float*)L[l + 1].xb, st));
