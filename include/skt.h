@@ -494,6 +494,9 @@ typedef struct skt_server_status {
     int64_t completed;            /* jobs retired */
     double cell_steps;            /* cells x steps advanced over all jobs */
     double busy_s;                /* wall seconds spent inside skt_server_step */
+    double admit_s;               /*   of which building admitted jobs' initial states */
+    double replan_s;              /*   of which issuing the steps that followed a change of membership (the
+                                       ensemble re-records its plan and re-captures its graph) */
 } skt_server_status;
 void skt_server_options_default(skt_server_options* o);
 int skt_server_create(const skt_server_options* o, skt_server** out);

@@ -123,7 +123,8 @@ class JobEvent(ctypes.Structure):
 class ServerStatus(ctypes.Structure):
     _fields_ = [("active", ctypes.c_int32), ("queued", ctypes.c_int32), ("bytes_in_use", ctypes.c_int64),
                 ("budget_bytes", ctypes.c_int64), ("steps", ctypes.c_int64), ("completed", ctypes.c_int64),
-                ("cell_steps", ctypes.c_double), ("busy_s", ctypes.c_double)]
+                ("cell_steps", ctypes.c_double), ("busy_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("replan_s", ctypes.c_double)]
 
 
 EVENT_KINDS = {0: "admitted", 1: "report", 2: "done"}

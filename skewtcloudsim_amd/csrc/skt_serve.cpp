@@ -56,7 +56,8 @@ struct skt_server {
     std::vector<Job> active;
     std::deque<skt_job_event> events;
     int64_t next_id = 1, bytes = 0, budget = 0, steps = 0, completed = 0;
-    double cell_steps = 0.0, busy = 0.0;
+    double cell_steps = 0.0, busy = 0.0, admit_s = 0.0, replan_s = 0.0;
+    bool changed = true;  // membership changed since the last ensemble step
     Clock::time_point t0;
     DgPin* pin = nullptr;  // pinned, one per slot
     std::string err;
@@ -221,8 +222,11 @@ int skt_server_step(skt_server* s) {
             continue;
         }
         Job j = *it;
+        const Clock::time_point ta = Clock::now();
         int rc = admit(s, j);
+        s->admit_s += std::chrono::duration<double>(Clock::now() - ta).count();
         if (rc) return rc;
+        s->changed = true;
         s->bytes += j.bytes;
         pushEvent(s, j, SKT_EVENT_ADMITTED, nullptr);
         s->active.push_back(j);
@@ -232,8 +236,11 @@ int skt_server_step(skt_server* s) {
         s->busy += std::chrono::duration<double>(Clock::now() - tb).count();
         return SKT_OK;
     }
+    const Clock::time_point tp = Clock::now();
     int rc = skt_batch_step(s->batch, s->opt.dt, 1.0f);
     if (rc) return serr(s, rc, std::string("ensemble step: ") + skt_batch_last_error(s->batch));
+    if (s->changed) s->replan_s += std::chrono::duration<double>(Clock::now() - tp).count();
+    s->changed = false;
     s->steps++;
     // reports: diagnostics of every job at a boundary, then one wait
     std::vector<int> rep;
@@ -271,6 +278,7 @@ int skt_server_step(skt_server* s) {
         }
         rc = skt_batch_remove(s->batch, j.member);
         if (rc) return serr(s, rc, "retiring a job");
+        s->changed = true;
         skt_env_destroy(j.env);
         s->bytes -= j.bytes;
         s->completed++;
@@ -301,6 +309,8 @@ int skt_server_status_get(skt_server* s, skt_server_status* o) {
     o->completed = s->completed;
     o->cell_steps = s->cell_steps;
     o->busy_s = s->busy;
+    o->admit_s = s->admit_s;
+    o->replan_s = s->replan_s;
     return SKT_OK;
 }
 

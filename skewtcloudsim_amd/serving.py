@@ -149,4 +149,5 @@ def run_poisson(rate, n_requests, seed=1, grids=REQUEST_GRIDS, t_range=(300.0, 1
             "throughput_jobs_per_s": len(done) / (last - first), "cell_steps_per_s": st["cell_steps"] / wall,
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_max_s": float(lat.max()), "ensemble_steps": st["steps"], "busy_s": st["busy_s"],
+            "admit_s": st["admit_s"], "replan_s": st["replan_s"],
             "wall_s": wall, "offered_cell_steps": cells, "events": events}
