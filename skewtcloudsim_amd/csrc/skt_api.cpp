@@ -675,7 +675,7 @@ static int allocDevice(EnvState& e) {
         if (!mgLevels(e)) return setErr(e, SKT_ERR_ARG, "grid too large for the multigrid levels");
         for (int l = 0; l < e.mg_depth; l++) {
             MgLevel& L = e.mg[l];
-            if (dalloc(e, &L.C, sizeof(float4) * L.N) || dalloc(e, &L.xa, sizeof(float) * L.N)) return SKT_ERR_NOMEM;
+            if (dalloc(e, &L.C, sizeof(MgC) * L.N) || dalloc(e, &L.xa, sizeof(float) * L.N)) return SKT_ERR_NOMEM;
             if (l > 0 && (dalloc(e, &L.r, sizeof(float) * L.N) || dalloc(e, &L.xb, sizeof(float) * L.N)))
                 return SKT_ERR_NOMEM;
         }

@@ -26,15 +26,22 @@ struct PcgState {
 
 // One level of the multigrid preconditioner: dims, the operator as
 // {+x face, +y face, +z face, diagonal} per cell (faces as positive
-// couplings, diagonal 0 marks a ground cell), the right-hand side r and the
-// two iterate buffers (the level's result ends in xb). Level 0's r is the
-// PCG residual and its xb the preconditioned residual z.
+// couplings, diagonal 0 marks a ground cell; level l holds its operator
+// divided by 2^l, so every level's values stay in the 16-bit range), the
+// right-hand side r (likewise divided by 2^l) and the two iterate buffers
+// (the level's result ends in xb). Level 0's r is the PCG residual and its
+// xb the preconditioned residual z.
 constexpr int kMgLevels = 8;
 constexpr long kMgCoarsest = 512;  // cells of a coarsest level at most
+// A cell's operator row on a multigrid level in 16-bit floats: the +x, +y,
+// +z couplings and the diagonal.
+struct alignas(8) MgC {
+    _Float16 x, y, z, w;
+};
 struct MgLevel {
     int X = 0, Y = 0, Z = 0;
     long N = 0;
-    float4* C = nullptr;
+    MgC* C = nullptr;
     float *r = nullptr, *xa = nullptr, *xb = nullptr;
 };
 

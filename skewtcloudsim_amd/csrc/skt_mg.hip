@@ -10,6 +10,10 @@
 // fine couplings across the coarse face, the coarse diagonal half the sum of
 // all fine couplings that leave the coarse cell (domain-boundary faces
 // included). A coarse cell with no sky cell has diagonal 0 and is excluded.
+// The rows are stored as 16-bit floats (8 B per cell instead of 16), level l
+// divided by 2^l (its right-hand side too; power-of-two scaling is exact),
+// with the couplings rounded toward zero and the diagonal away from it, so
+// each stored level stays diagonally dominant.
 // The V-cycle: two damped Jacobi sweeps (the first from zero), the residual
 // summed over the 2x2x2 children onto the coarse level, recursion, the
 // coarse correction added to every child, two more sweeps; on the coarsest
@@ -36,6 +40,38 @@ constexpr int kSweeps = 2;         // Jacobi sweeps before and after each coarse
 
 __device__ __forceinline__ long gstride(const VB& vb) { return (long)vb.n * 256; }
 __device__ __forceinline__ long gfirst(const VB& vb) { return (long)vb.id * 256 + tid256(); }
+
+// The 16-bit operator rows: loads, and stores that round the couplings
+// toward zero and the diagonal away from it, so that every stored level
+// stays diagonally dominant (its smoother and cycle stay symmetric positive
+// definite).
+__device__ __forceinline__ float4 ldC(const MgC* __restrict__ C, long i) {
+    const MgC c = C[i];
+    return float4{(float)c.x, (float)c.y, (float)c.z, (float)c.w};
+}
+__device__ __forceinline__ float ldCx(const MgC* __restrict__ C, long i) { return (float)C[i].x; }
+__device__ __forceinline__ float ldCy(const MgC* __restrict__ C, long i) { return (float)C[i].y; }
+__device__ __forceinline__ float ldCz(const MgC* __restrict__ C, long i) { return (float)C[i].z; }
+__device__ __forceinline__ float ldCw(const MgC* __restrict__ C, long i) { return (float)C[i].w; }
+__device__ __forceinline__ _Float16 halfStep(_Float16 h, int d) {
+    return __builtin_bit_cast(_Float16, (unsigned short)(__builtin_bit_cast(unsigned short, h) + d));
+}
+__device__ __forceinline__ _Float16 halfDown(float v) {  // v >= 0
+    _Float16 h = (_Float16)v;
+    return (float)h > v ? halfStep(h, -1) : h;
+}
+__device__ __forceinline__ _Float16 halfUp(float v) {  // v >= 0
+    _Float16 h = (_Float16)v;
+    return (float)h < v ? halfStep(h, 1) : h;
+}
+__device__ __forceinline__ void stC(MgC* __restrict__ C, long i, float x, float y, float z, float w) {
+    MgC c;
+    c.x = halfDown(x);
+    c.y = halfDown(y);
+    c.z = halfDown(z);
+    c.w = halfUp(w);
+    C[i] = c;
+}
 
 // A level's dims and the z-chunk of its tiles. Tiles are 64 (x) x 4 (y)
 // columns of zc planes; a thread marches its column through the chunk, so a
@@ -64,24 +100,23 @@ __device__ __forceinline__ MgCol mgCol(const VB& vb, const MgDims& D) {
 
 // (A x)_i at cell (x, y, z) of a level: d x_i - sum of couplings times
 // neighbours (a -x/-y/-z coupling is that neighbour's + coupling).
-__device__ __forceinline__ float applyAt(const MgDims& D, const float4* __restrict__ C, const float* __restrict__ x,
+__device__ __forceinline__ float applyAt(const MgDims& D, const MgC* __restrict__ C, const float* __restrict__ x,
                                          int i, int xx, int yy, int zz, const float4& c) {
     const int X = D.X, XY = D.X * D.Y;
-    const float* Cf = reinterpret_cast<const float*>(C);
     float s = 0.0f;
     if (xx + 1 < X) s += c.x * x[i + 1];
-    if (xx > 0) s += Cf[4L * (i - 1)] * x[i - 1];
+    if (xx > 0) s += ldCx(C, i - 1) * x[i - 1];
     if (yy + 1 < D.Y) s += c.y * x[i + X];
-    if (yy > 0) s += Cf[4L * (i - X) + 1] * x[i - X];
+    if (yy > 0) s += ldCy(C, i - X) * x[i - X];
     if (zz + 1 < D.Z) s += c.z * x[i + XY];
-    if (zz > 0) s += Cf[4L * (i - XY) + 2] * x[i - XY];
+    if (zz > 0) s += ldCz(C, i - XY) * x[i - XY];
     return c.w * x[i] - s;
 }
 
 // Level-0 operator from 1/density (after the preconditioner pass wrote irho).
 struct KMgCoef0 {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, const float* __restrict__ irho,
-                                               float4* __restrict__ C) {
+                                               MgC* __restrict__ C) {
         const int X = G.X, Y = G.Y, Z = G.Z;
         const long XY = (long)X * Y, N = XY * Z;
         for (long i = gfirst(vb); i < N; i += gstride(vb)) {
@@ -89,21 +124,21 @@ struct KMgCoef0 {
             const long gc = x + (long)X * z;
             const int gh = G.gh[gc];
             if (y <= gh) {
-                C[i] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+                stC(C, i, 0.0f, 0.0f, 0.0f, 0.0f);
                 continue;
             }
             const int ghL = x > 0 ? G.gh[gc - 1] : 0, ghR = x < X - 1 ? G.gh[gc + 1] : 0;
             const int ghB = z > 0 ? G.gh[gc - X] : 0, ghF = z < Z - 1 ? G.gh[gc + X] : 0;
             const OpRow o = opRow(irho, i, x, y, z, X, Y, Z, XY, gh, ghL, ghR, ghB, ghF);
-            C[i] = float4{o.sR ? -o.fr : 0.0f, o.sU ? -o.fu : 0.0f, o.sF ? -o.ff : 0.0f, o.w};
+            stC(C, i, o.sR ? -o.fr : 0.0f, o.sU ? -o.fu : 0.0f, o.sF ? -o.ff : 0.0f, o.w);
         }
     }
 };
 
 // Level l + 1's operator from level l's.
 struct KMgCoarsen {
-    static __device__ __forceinline__ void run(VB vb, MgDims F, const float4* __restrict__ Cf, MgDims D,
-                                               float4* __restrict__ Cc) {
+    static __device__ __forceinline__ void run(VB vb, MgDims F, const MgC* __restrict__ Cf, MgDims D,
+                                               MgC* __restrict__ Cc) {
         const long FXY = (long)F.X * F.Y, XY = (long)D.X * D.Y;
         for (long k = gfirst(vb); k < D.N; k += gstride(vb)) {
             const int I = (int)(k % D.X), J = (int)((k / D.X) % D.Y), K = (int)(k / XY);
@@ -113,7 +148,7 @@ struct KMgCoarsen {
             for (int c = 0; c < nz; c++)
                 for (int b = 0; b < ny; b++)
                     for (int a = 0; a < nx; a++) {
-                        const float4 f = Cf[(x0 + a) + (long)F.X * (y0 + b) + FXY * (z0 + c)];
+                        const float4 f = ldC(Cf, (x0 + a) + (long)F.X * (y0 + b) + FXY * (z0 + c));
                         sumd += f.w;
                         if (a == 0 && nx == 2) inner += f.x;
                         if (b == 0 && ny == 2) inner += f.y;
@@ -122,9 +157,12 @@ struct KMgCoarsen {
                         if (b == ny - 1) by += f.y;
                         if (c == nz - 1) bz += f.z;
                     }
-            // a sibling face is in both children's diagonals
-            const float d = 0.5f * (sumd - 2.0f * inner);
-            Cc[k] = d > 0.0f ? float4{0.5f * bx, 0.5f * by, 0.5f * bz, d} : float4{0.0f, 0.0f, 0.0f, 0.0f};
+            // a sibling face is in both children's diagonals; the coarse
+            // operator (half sums) is stored divided by 2 once more, the
+            // level's power-of-two scale
+            const float d = 0.25f * (sumd - 2.0f * inner);
+            if (d > 0.0f) stC(Cc, k, 0.25f * bx, 0.25f * by, 0.25f * bz, d);
+            else stC(Cc, k, 0.0f, 0.0f, 0.0f, 0.0f);
         }
     }
 };
@@ -137,34 +175,33 @@ struct KMgSmooth {
     static __device__ __forceinline__ bool idle(const T&... a) {
         return nthArg<5>(a...)->done != 0;  // st
     }
-    static __device__ __forceinline__ void run(VB vb, MgDims D, const float4* __restrict__ C,
+    static __device__ __forceinline__ void run(VB vb, MgDims D, const MgC* __restrict__ C,
                                                const float* __restrict__ r, const float* __restrict__ xin,
                                                float* __restrict__ xout, const PcgState* st, int first) {
         if (st->done) return;
         const MgCol t = mgCol(vb, D);
         if (!t.in) return;
         const int X = D.X, XY = D.X * D.Y;
-        const float* Cf = reinterpret_cast<const float*>(C);
         int i = t.x + X * t.y + XY * t.z0;
         if (first) {
             for (int z = t.z0; z < t.z1; z++, i += XY) {
-                const float w = Cf[4L * i + 3];
+                const float w = ldCw(C, i);
                 xout[i] = w > 0.0f ? kOmega * r[i] / w : 0.0f;
             }
             return;
         }
-        float xm = t.z0 > 0 ? xin[i - XY] : 0.0f, czm = t.z0 > 0 ? Cf[4L * (i - XY) + 2] : 0.0f;
+        float xm = t.z0 > 0 ? xin[i - XY] : 0.0f, czm = t.z0 > 0 ? ldCz(C, i - XY) : 0.0f;
         float xc = xin[i];
         for (int z = t.z0; z < t.z1; z++, i += XY) {
-            const float4 c = C[i];
+            const float4 c = ldC(C, i);
             const float xp = z + 1 < D.Z ? xin[i + XY] : 0.0f;
             float v = 0.0f;
             if (c.w > 0.0f) {
                 float s = czm * xm + c.z * xp;
                 if (t.x + 1 < X) s += c.x * xin[i + 1];
-                if (t.x > 0) s += Cf[4L * (i - 1)] * xin[i - 1];
+                if (t.x > 0) s += ldCx(C, i - 1) * xin[i - 1];
                 if (t.y + 1 < D.Y) s += c.y * xin[i + X];
-                if (t.y > 0) s += Cf[4L * (i - X) + 1] * xin[i - X];
+                if (t.y > 0) s += ldCy(C, i - X) * xin[i - X];
                 v = xc + kOmega * (r[i] - (c.w * xc - s)) / c.w;
             }
             xout[i] = v;
@@ -184,7 +221,7 @@ struct KMgPreSmooth {
     static __device__ __forceinline__ bool idle(const T&... a) {
         return nthArg<4>(a...)->done != 0;  // st
     }
-    static __device__ __forceinline__ void run(VB vb, MgDims D, const float4* __restrict__ C,
+    static __device__ __forceinline__ void run(VB vb, MgDims D, const MgC* __restrict__ C,
                                                const float* __restrict__ r, float* __restrict__ xout,
                                                const PcgState* st) {
         if (st->done) return;
@@ -193,30 +230,30 @@ struct KMgPreSmooth {
         const int X = D.X, XY = D.X * D.Y;
         auto a0 = [&](const float4& c, float rv) { return c.w > 0.0f ? kOmega * rv / c.w : 0.0f; };
         int i = t.x + X * t.y + XY * t.z0;
-        float4 cm{0.0f, 0.0f, 0.0f, 0.0f}, cc = C[i];
+        float4 cm{0.0f, 0.0f, 0.0f, 0.0f}, cc = ldC(C, i);
         float am = 0.0f, ac = a0(cc, r[i]);
         if (t.z0 > 0) {
-            cm = C[i - XY];
+            cm = ldC(C, i - XY);
             am = a0(cm, r[i - XY]);
         }
         for (int z = t.z0; z < t.z1; z++, i += XY) {
             float4 cp{0.0f, 0.0f, 0.0f, 0.0f};
             float ap = 0.0f;
             if (z + 1 < D.Z) {
-                cp = C[i + XY];
+                cp = ldC(C, i + XY);
                 ap = a0(cp, r[i + XY]);
             }
             float v = 0.0f;
             if (cc.w > 0.0f) {
                 float s = cm.z * am + cc.z * ap;
-                if (t.x + 1 < X) s += cc.x * a0(C[i + 1], r[i + 1]);
+                if (t.x + 1 < X) s += cc.x * a0(ldC(C, i + 1), r[i + 1]);
                 if (t.x > 0) {
-                    const float4 l = C[i - 1];
+                    const float4 l = ldC(C, i - 1);
                     s += l.x * a0(l, r[i - 1]);
                 }
-                if (t.y + 1 < D.Y) s += cc.y * a0(C[i + X], r[i + X]);
+                if (t.y + 1 < D.Y) s += cc.y * a0(ldC(C, i + X), r[i + X]);
                 if (t.y > 0) {
-                    const float4 d = C[i - X];
+                    const float4 d = ldC(C, i - X);
                     s += d.y * a0(d, r[i - X]);
                 }
                 v = ac + kOmega * (r[i] - (cc.w * ac - s)) / cc.w;
@@ -238,7 +275,7 @@ struct KMgProlongSmooth {
     static __device__ __forceinline__ bool idle(const T&... a) {
         return nthArg<7>(a...)->done != 0;  // st
     }
-    static __device__ __forceinline__ void run(VB vb, MgDims F, const float4* __restrict__ C,
+    static __device__ __forceinline__ void run(VB vb, MgDims F, const MgC* __restrict__ C,
                                                const float* __restrict__ r, const float* __restrict__ xin,
                                                float* __restrict__ xout, MgDims D, const float* __restrict__ xc,
                                                const PcgState* st) {
@@ -246,7 +283,6 @@ struct KMgProlongSmooth {
         const MgCol t = mgCol(vb, F);
         if (!t.in) return;
         const int X = F.X, XY = F.X * F.Y, DXY = D.X * D.Y;
-        const float* Cf = reinterpret_cast<const float*>(C);
         // the parent rows of the column and of its -x/+x/-y/+y neighbours
         const int px = t.x >> 1, py = t.y >> 1;
         const int pxl = t.x > 0 ? (t.x - 1) >> 1 : 0, pxr = t.x + 1 < X ? (t.x + 1) >> 1 : 0;
@@ -256,19 +292,19 @@ struct KMgProlongSmooth {
         float xm = 0.0f, czm = 0.0f;
         if (t.z0 > 0) {
             xm = xin[i - XY] + corr(px, py, t.z0 - 1);
-            czm = Cf[4L * (i - XY) + 2];
+            czm = ldCz(C, i - XY);
         }
         float xe = xin[i] + corr(px, py, t.z0);
         for (int z = t.z0; z < t.z1; z++, i += XY) {
-            const float4 c = C[i];
+            const float4 c = ldC(C, i);
             const float xp = z + 1 < F.Z ? xin[i + XY] + corr(px, py, z + 1) : 0.0f;
             float v = 0.0f;
             if (c.w > 0.0f) {
                 float s = czm * xm + c.z * xp;
                 if (t.x + 1 < X) s += c.x * (xin[i + 1] + corr(pxr, py, z));
-                if (t.x > 0) s += Cf[4L * (i - 1)] * (xin[i - 1] + corr(pxl, py, z));
+                if (t.x > 0) s += ldCx(C, i - 1) * (xin[i - 1] + corr(pxl, py, z));
                 if (t.y + 1 < F.Y) s += c.y * (xin[i + X] + corr(px, pyu, z));
-                if (t.y > 0) s += Cf[4L * (i - X) + 1] * (xin[i - X] + corr(px, pyd, z));
+                if (t.y > 0) s += ldCy(C, i - X) * (xin[i - X] + corr(px, pyd, z));
                 v = xe + kOmega * (r[i] - (c.w * xe - s)) / c.w;
             }
             xout[i] = v;
@@ -290,7 +326,7 @@ struct KMgRestrict {
     static __device__ __forceinline__ bool idle(const T&... a) {
         return nthArg<6>(a...)->done != 0;  // st
     }
-    static __device__ __forceinline__ void run(VB vb, MgDims F, const float4* __restrict__ C,
+    static __device__ __forceinline__ void run(VB vb, MgDims F, const MgC* __restrict__ C,
                                                const float* __restrict__ r, const float* __restrict__ x, MgDims D,
                                                float* __restrict__ rc, const PcgState* st) {
         if (st->done) return;
@@ -302,28 +338,27 @@ struct KMgRestrict {
         const int zs = 2 * (int)bz * D.zc, ze = min(F.Z, 2 * ((int)bz + 1) * D.zc);
         const bool in = fx < F.X && fy < F.Y;
         const int X = F.X, XY = F.X * F.Y;
-        const float* Cf = reinterpret_cast<const float*>(C);
         int i = in ? fx + X * fy + XY * zs : 0;
         float xm = 0.0f, czm = 0.0f, xc = 0.0f;
         if (in) {
             xc = x[i];
             if (zs > 0) {
                 xm = x[i - XY];
-                czm = Cf[4L * (i - XY) + 2];
+                czm = ldCz(C, i - XY);
             }
         }
         const int cx = fx >> 1, cy = fy >> 1, DXY = D.X * D.Y;
         float acc = 0.0f;
         for (int z = zs; z < ze; z++) {
             if (in) {
-                const float4 c = C[i];
+                const float4 c = ldC(C, i);
                 const float xp = z + 1 < F.Z ? x[i + XY] : 0.0f;
                 if (c.w > 0.0f) {
                     float s = czm * xm + c.z * xp;
                     if (fx + 1 < X) s += c.x * x[i + 1];
-                    if (fx > 0) s += Cf[4L * (i - 1)] * x[i - 1];
+                    if (fx > 0) s += ldCx(C, i - 1) * x[i - 1];
                     if (fy + 1 < F.Y) s += c.y * x[i + X];
-                    if (fy > 0) s += Cf[4L * (i - X) + 1] * x[i - X];
+                    if (fy > 0) s += ldCy(C, i - X) * x[i - X];
                     acc += r[i] - (c.w * xc - s);
                 }
                 xm = xc;
@@ -335,8 +370,9 @@ struct KMgRestrict {
                 const float v = acc + __shfl_xor(acc, 1);
                 sh[ty][tx] = v;
                 __syncthreads();
+                // the coarse level's right-hand side is divided by 2 once more
                 if (!(ty & 1) && !(tx & 1) && cx < D.X && cy < D.Y)
-                    rc[cx + D.X * cy + DXY * (z >> 1)] = v + sh[ty + 1][tx];
+                    rc[cx + D.X * cy + DXY * (z >> 1)] = 0.5f * (v + sh[ty + 1][tx]);
                 __syncthreads();
                 acc = 0.0f;
             }
@@ -351,7 +387,7 @@ struct KMgCoarsest {
     static __device__ __forceinline__ bool idle(const T&... a) {
         return nthArg<4>(a...)->done != 0;  // st
     }
-    static __device__ __forceinline__ void run(VB vb, MgDims D, const float4* __restrict__ C,
+    static __device__ __forceinline__ void run(VB vb, MgDims D, const MgC* __restrict__ C,
                                                const float* __restrict__ r, float* __restrict__ x,
                                                const PcgState* st) {
         (void)vb;
@@ -361,7 +397,7 @@ struct KMgCoarsest {
         const int t = tid256();
         const int n = (int)D.N;  // at most kMgCoarsest (checked on the host)
         for (int i = t; i < n; i += 256) {
-            sC[i] = C[i];
+            sC[i] = ldC(C, i);
             sr[i] = r[i];
         }
         __syncthreads();
@@ -451,7 +487,7 @@ hipError_t launch_mg_setup(EnvState& e) {
     hipError_t r = klaunch<KMgCoef0>(e, tiles(e.N, true), dim3(256), 0, geom(e), (const float*)e.irho, L[0].C);
     for (int l = 0; l + 1 < kMgLevels && r == hipSuccess; l++) {
         const bool on = l + 1 < e.mg_depth;
-        r = klaunch<KMgCoarsen>(e, tiles(L[l + 1].N, on), dim3(256), 0, dims(L[l]), (const float4*)L[l].C,
+        r = klaunch<KMgCoarsen>(e, tiles(L[l + 1].N, on), dim3(256), 0, dims(L[l]), (const MgC*)L[l].C,
                                 dims(L[l + 1]), L[l + 1].C);
     }
     return r;
@@ -472,20 +508,20 @@ hipError_t launch_mg_vcycle(EnvState& e) {
     for (int l = 0; l + 1 < kMgLevels; l++) {
         const bool on = l + 1 < D;
         const MgDims f = dims(L[l], ens), c = dims(L[l + 1], ens);
-        go(klaunch<KMgPreSmooth>(e, colTiles(f, on), kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, L[l].xb,
+        go(klaunch<KMgPreSmooth>(e, colTiles(f, on), kB3, 0, f, (const MgC*)L[l].C, (const float*)L[l].r, L[l].xb,
                                  st));
-        go(klaunch<KMgRestrict>(e, restrictTiles(f, c, on), kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
+        go(klaunch<KMgRestrict>(e, restrictTiles(f, c, on), kB3, 0, f, (const MgC*)L[l].C, (const float*)L[l].r,
                                 (const float*)L[l].xb, c, L[l + 1].r, st));
     }
     const MgLevel& B = L[D - 1];
-    go(klaunch<KMgCoarsest>(e, dim3(1), dim3(256), 0, dims(B), (const float4*)B.C, (const float*)B.r, B.xb, st));
+    go(klaunch<KMgCoarsest>(e, dim3(1), dim3(256), 0, dims(B), (const MgC*)B.C, (const float*)B.r, B.xb, st));
     for (int l = kMgLevels - 2; l >= 0; l--) {
         const bool on = l + 1 < D;
         const MgDims f = dims(L[l], ens), c = dims(L[l + 1], ens);
         const dim3 g = colTiles(f, on);
-        go(klaunch<KMgProlongSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r,
+        go(klaunch<KMgProlongSmooth>(e, g, kB3, 0, f, (const MgC*)L[l].C, (const float*)L[l].r,
                                      (const float*)L[l].xb, L[l].xa, c, (const float*)L[l + 1].xb, st));
-        go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const float4*)L[l].C, (const float*)L[l].r, (const float*)L[l].xa,
+        go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const MgC*)L[l].C, (const float*)L[l].r, (const float*)L[l].xa,
                               L[l].xb, st, 0));
     }
     return r;
