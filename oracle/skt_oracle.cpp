@@ -24,12 +24,28 @@
 //   EOS                   : initDensity / calculateNewPressure round trip
 //   microphysics (warm)   : total-water conservation, zero-hydrometeor cells
 //   sounding (hydrostatic): closed forms for isothermal / constant lapse layers
-//   parity unpinned       : the cold-branch (T<0) microphysics chain and the
-//                           ground-surface rates are pinned only by the
-//                           restatement itself and the invariants above; no
-//                           reference test, fixture or buildable source fixes
-//                           their numbers (Game/microPhys.cpp does not build
-//                           with this toolchain).
+//   pressure operator, divergence, projection correction, velocity reset,
+//   resetParameter        : numpy brute force over terrain with the
+//                           reference's neighbour-fill rule per boundary set
+//   buoyancy, sedimentation, cloud cover, ground temperature, ground water,
+//   setTempsAtGround      : closed forms (uniform states, single warm or wet
+//                           cells, telescoping column budgets, fall-speed
+//                           power laws, relaxation targets)
+//   surface and sky rates : closed forms, power laws in the Marshall-Palmer
+//                           slope, zero without source, source bounds
+//   cold-branch limiting,
+//   latent heat           : limited rates bounded by dt x source, single-
+//                           process cells move equal and opposite amounts and
+//                           change theta by the latent heat over cp
+//   (tests/test_oracle_pins.py, tests/test_oracle_pins_physics.py; the pins
+//   fail on the slips tests/test_oracle_mutations.py plants)
+//   parity unpinned       : only the exact combination of several cold-branch
+//                           processes active in one cell (the PT1/PT2/PT3/W
+//                           selectors with many rates at once), where no
+//                           invariant holds; the single-process cases are
+//                           pinned. No reference test, fixture or buildable
+//                           source fixes those numbers (Game/microPhys.cpp
+//                           does not build with this toolchain).
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
