@@ -130,6 +130,51 @@ __device__ __forceinline__ bool skyAt(const Geom& G, int x, int y, int z) {
     return x >= 0 && x < G.X && y >= 0 && y < G.Y && z >= 0 && z < G.Z && !isGround(G, x, y, z);
 }
 
+// ------------------------------------------------------------ powers ---
+// x^y for the physics' powers from the hardware base-2 logarithm and
+// exponential (v_log_f32, v_exp_f32), in place of the library powf, which
+// spends about 160 instructions on an extended-precision logarithm and on
+// the special cases of every sign and size. The per-cell phases that raise
+// p / p_ground to Rd/cp or take Marshall-Palmer slopes to fractional powers
+// are bound by those instructions. tests/test_pow.py measures both forms
+// against the exact power over the ranges the phases use.
+//
+// powNear1: x within a few octaves of 1 and |y log2 x| of order 1 (the
+// pressure ratio to Rd/cp, the temperature ratios of the vapour
+// diffusivity and the air viscosity): log2 x is small, so its rounding and
+// the product's stay well under an ulp of the result. Three instructions.
+__device__ __forceinline__ float powNear1(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+// powPos: any normal x > 0 and any y. With x = m 2^e, m in [sqrt(1/2),
+// sqrt(2)): y log2 x = y e + y log2 m. Both products are split exactly into
+// their rounded values and rounding errors (fma); the integer n nearest y e
+// is taken out before the fraction (y e - n, exact) is summed with y log2 m
+// and the two errors, so the exponential's argument keeps a fraction's
+// precision however large y log2 x is, and 2^n is applied exactly (ldexp).
+// What remains is the logarithm's own error, |y| times an ulp of |log2 m|
+// <= 1/2, and the exponential's. Other x (0, subnormal, negative, infinite,
+// NaN) take the library powf.
+__device__ __forceinline__ float powPos(float x, float y) {
+    if (!(x >= 0x1p-126f && x <= 3.40282347e38f)) return powf(x, y);
+    float m = __builtin_amdgcn_frexp_mantf(x);
+    int e = __builtin_amdgcn_frexp_expf(x);
+    if (m < 0.70710678f) {
+        m *= 2.0f;
+        e -= 1;
+    }
+    const float fe = (float)e;
+    const float a = y * fe;
+    const float ae = __builtin_fmaf(y, fe, -a);
+    const float l = __builtin_amdgcn_logf(m);
+    const float b = y * l;
+    const float be = __builtin_fmaf(y, l, -b);
+    const float n = __builtin_rintf(a);
+    const float f = ((a - n) + b) + (ae + be);
+    const float nf = __builtin_rintf(f);
+    return __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(f - nf), (int)(n + nf));
+}
+
 // ------------------------------------------------------------ thermo ---
 // Game/meteoformulas.cuh:16-316
 __device__ __forceinline__ float d_es(float T) { return 0.61078f * expf((17.27f * T) / (T + 237.3f)); }
@@ -160,10 +205,10 @@ __device__ __forceinline__ float d_Lwater(float T) {
 }
 __device__ __forceinline__ float d_Lice(float T) { return (2834.1f - 0.29f * T - 0.004f * T * T) * 1e3f; }
 __device__ __forceinline__ float d_DQVair(float T, float P) {
-    return 2.26e-5f * powf((T + 273.15f) / 273.15f, 1.94f) * (1013.25f / P);
+    return 2.26e-5f * powNear1((T + 273.15f) / 273.15f, 1.94f) * (1013.25f / P);
 }
 __device__ __forceinline__ float d_ViscAir(float T) {
-    return powf((T + 273.15f) / 273.15f, 3.0f / 2.0f) * ((273.15f + 111.0f) / (T + 111.0f)) * 1.716e-5f;
+    return powNear1((T + 273.15f) / 273.15f, 3.0f / 2.0f) * ((273.15f + 111.0f) / (T + 111.0f)) * 1.716e-5f;
 }
 __device__ __forceinline__ float d_slope(float D, float Qj, int type) {
     float num;
@@ -172,7 +217,7 @@ __device__ __forceinline__ float d_slope(float D, float Qj, int type) {
     else if (type == 2) num = kc::PI * 0.91f * 4e-4f;
     else return 0.0f;
     const float den = fmaxf(D * Qj * 0.001f, 1e-14f);
-    return powf((num / den), 0.25f);
+    return powPos((num / den), 0.25f);
 }
 // calculateFallingVelocityGPU: type 3 ('all') evaluates the rain branch only.
 __device__ __forceinline__ float3 d_fallVel(float Qr, float Qs, float Qi, float dens, int type, float GR, float GS,
@@ -181,15 +226,15 @@ __device__ __forceinline__ float3 d_fallVel(float Qr, float Qs, float Qi, float 
     float UR = 0.0f, US = 0.0f, UI = 0.0f;
     if (type == 0 || all) {
         float sR = d_slope(dens, Qr, 0);
-        UR = 2115.0f * (GR / (6 * powf(sR, 0.8f))) * sqrtf(1.225f / dens);
+        UR = 2115.0f * (GR / (6 * powPos(sR, 0.8f))) * sqrtf(1.225f / dens);
         UR *= 0.01f;
     } else if (type == 1 || all) {
         float sS = d_slope(dens, Qs, 1);
-        US = 152.93f * (GS / (6 * powf(sS, 0.25f))) * sqrtf(1.225f / dens);
+        US = 152.93f * (GS / (6 * powPos(sS, 0.25f))) * sqrtf(1.225f / dens);
         US *= 0.01f;
     } else if (type == 2 || all) {
         float sI = d_slope(dens, Qi, 2);
-        UI = (GI / (6 * powf(sI, 0.5f))) * powf(4 * kc::g * 100 * 0.91f / (3 * 0.6f * dens * 0.001f), 0.5f);
+        UI = (GI / (6 * sqrtf(sI))) * sqrtf(4 * kc::g * 100 * 0.91f / (3 * 0.6f * dens * 0.001f));
         UI *= 0.01f;
     }
     return make_float3(UR, US, UI);
@@ -200,10 +245,10 @@ __device__ __forceinline__ float d_cvd(float T, float P, float dens) {
     const float A = (kc::Ls / (kc::Ka * (T + 273.15f))) * (kc::Ls / (kc::Rsw * (T + 273.15f)) - 1);
     const float B = (kc::Rsw * (T + 273.15f) * P) / (2.21f * EI);
     const float Ni = 10000 * expf((12.96f * (ES - EI)) / (EI - 0.639f));
-    return 65.2f * ((powf(Ni, 0.5f) * (ES - EI)) / (powf(dens, 0.5f) * (A + B) * EI));
+    return 65.2f * ((sqrtf(Ni) * (ES - EI)) / (sqrtf(dens) * (A + B) * EI));
 }
 __device__ __forceinline__ float d_potTemp(float T, float Pk, float Pt) {
-    return (T + 273.15f) * powf((Pt / Pk), kc::Rsd / kc::Cpd) - 273.15f;
+    return (T + 273.15f) * powNear1((Pt / Pk), kc::Rsd / kc::Cpd) - 273.15f;
 }
 __device__ __forceinline__ float d_MLR(float T, float P) {
     float TK = T + 273.15f;

@@ -1671,7 +1671,7 @@ struct KUpdatePressureDensity {
         const long i = gidx(G, x, y, z);
         const float pn = p[i] - pp[i] / 100.0f;
         p[i] = pn;
-        const float T = th[i] * powf(pn / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+        const float T = th[i] * powNear1(pn / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
         const float Tv = T * (0.608f * qv[i] + 1);
         rho[i] = pn * 100 / (kc::Rsd * Tv);
     }
@@ -1730,7 +1730,7 @@ struct KApplyProjection {
             v[i] += (pu - cp) / du;
             w[i] += (pf - cp) / df;
             if (newp) {
-                const float T = th[i] * powf(p[i] / gps, kc::Rsd / kc::Cpd);
+                const float T = th[i] * powNear1(p[i] / gps, kc::Rsd / kc::Cpd);
                 const float Tv = T * (0.608f * qv[i] + 1);
                 p[i] = cd * kc::Rsd * Tv / 100.0f;
             }

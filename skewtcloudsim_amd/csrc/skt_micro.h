@@ -26,7 +26,7 @@ __device__ __forceinline__ float PIDW(float dt, float T, float Qc, float Qw, flo
     if (T < 0.0f && Qw > 0.0f) {
         const float a = 0.5f;
         const float quu = fmaxf(1e-12f * d_Ni(T) / (Dair * 0.001f), Qc);
-        return powf((1 - a) * d_cvd(T, ps, Dair * 0.001f) * dt + powf(quu, 1 - a), 1 / (1 - a)) - Qc;
+        return powPos((1 - a) * d_cvd(T, ps, Dair * 0.001f) * dt + powPos(quu, 1 - a), 1 / (1 - a)) - Qc;
     }
     return 0.0f;
 }
@@ -35,16 +35,16 @@ __device__ __forceinline__ float PIHOM(float T, float Qw) { return (T < -40 && Q
 // :124-145
 __device__ __forceinline__ float PIACR(const Gammas& gm, float Qr, float Qc, float Dair, float sR) {
     if (Qr > 0 && Qc > 0.0f) {
-        const float density = powf(1.225f / Dair, 0.5f);
-        return (PI * PI * 0.3f * 8e-2f * 2115.0f * Qc * 0.99f * gm.gRC) / (24 * 4.19e-10f * powf(sR, 6 + 0.8f)) * density;
+        const float density = sqrtf(1.225f / Dair);
+        return (PI * PI * 0.3f * 8e-2f * 2115.0f * Qc * 0.99f * gm.gRC) / (24 * 4.19e-10f * powPos(sR, 6 + 0.8f)) * density;
     }
     return 0.0f;
 }
 // :147-165
 __device__ __forceinline__ float PRACI(const Gammas& gm, float Qr, float Qc, float Dair, float sR) {
     if (Qr > 0 && Qc > 0.0f) {
-        const float density = powf(1.225f / Dair, 0.5f);
-        return (PI * 0.3f * 8e-2f * 2115.0f * Qc * gm.gR) / (4 * powf(sR, 3 + 0.8f)) * density;
+        const float density = sqrtf(1.225f / Dair);
+        return (PI * 0.3f * 8e-2f * 2115.0f * Qc * gm.gR) / (4 * powPos(sR, 3 + 0.8f)) * density;
     }
     return 0.0f;
 }
@@ -57,16 +57,16 @@ __device__ __forceinline__ float PRAUT(float Qw, float QwMin, float T) {
         const float disE = 0.5f * 0.5f;
         const float dispersion = ((1 + 3 * disE) * (1 + 4 * disE) * (1 + 5 * disE)) / ((1 + disE) * (1 + 2 * disE));
         const float L = Qw * 1.225f * 1e-3f;
-        return 1e3f * 1.1e10f * dispersion * (L * L * L) * powf(Nc, -1) *
-               static_cast<float>(1 - expf(-powf(1.03e16f * powf(Nc, -2.0f / 3.0f) * (L * L), 1.0f)));
+        return 1e3f * 1.1e10f * dispersion * (L * L * L) * (1.0f / Nc) *
+               static_cast<float>(1 - expf(-(1.03e16f * powPos(Nc, -2.0f / 3.0f) * (L * L))));
     }
     return 0.0f;
 }
 // :189-208
 __device__ __forceinline__ float PRACW(const Gammas& gm, float Qr, float Qw, float Dair, float sR) {
     if (Qr > 0 && Qw > 0.0f) {
-        const float density = powf(1.225f / Dair, 0.5f);
-        return (PI * 1.0f * 8e-2f * 2115.0f * Qw * gm.gR) / (4 * powf(sR, 3 + 0.8f)) * density;
+        const float density = sqrtf(1.225f / Dair);
+        return (PI * 1.0f * 8e-2f * 2115.0f * Qw * gm.gR) / (4 * powPos(sR, 3 + 0.8f)) * density;
     }
     return 0.0f;
 }
@@ -80,17 +80,17 @@ __device__ __forceinline__ float PREVP(const Gammas& gm, float T, float Qr, floa
         const float dQv = d_DQVair(T, ps);
         const float kv = d_ViscAir(T);
         float Sc = kv / dQv;
-        const float density = powf(1.225f / (Dair * 0.001f), 0.25f);
+        const float density = powPos(1.225f / (Dair * 0.001f), 0.25f);
         return fmaxf(0.0f, -1 * 2 * PI * (S - 1) * 8e-2f *
-                               (0.78f * powf(sR, -2) + 0.31f * powf(Sc, 1.0f / 3.0f) * gm.gER * powf(a, 0.5f) *
-                                                           powf(kv, -0.5f) * density * powf(sR, -((b + 5) / 2))) *
+                               (0.78f * powPos(sR, -2) + 0.31f * powPos(Sc, 1.0f / 3.0f) * gm.gER * sqrtf(a) *
+                                                           powPos(kv, -0.5f) * density * powPos(sR, -((b + 5) / 2))) *
                                1 / (Dair * 0.001f) *
-                               powf(E0v * E0v / (Ka * Rsw * tK * tK) + (1 / (Dair * 0.001f * QWS * dQv)), -1.0f));
+                               (1.0f / (E0v * E0v / (Ka * Rsw * tK * tK) + (1 / (Dair * 0.001f * QWS * dQv)))));
     }
     return 0.0f;
 }
 __device__ __forceinline__ float coll(float sA, float sB) {
-    return (5 / (powf(sA, 6) * sB) + (2 / (powf(sA, 5) * powf(sB, 2))) + (0.5f / (powf(sA, 4) * powf(sB, 3))));
+    return (5 / (powPos(sA, 6) * sB) + (2 / (powPos(sA, 5) * powPos(sB, 2))) + (0.5f / (powPos(sA, 4) * powPos(sB, 3))));
 }
 // :238-261
 __device__ __forceinline__ float PRACS(float Qr, float Qs, float PT2, float3 fv, float Dair, float sS, float sR) {
@@ -103,8 +103,8 @@ __device__ __forceinline__ float PRACS(float Qr, float Qs, float PT2, float3 fv,
 // :263-281
 __device__ __forceinline__ float PSACW(const Gammas& gm, float Qs, float Qw, float Dair, float sS) {
     if (Qs > 0 && Qw > 0) {
-        const float density = powf(1.225f / Dair, 0.5f);
-        return (PI * 1.0f * 3e-2f * 152.93f * Qw * gm.gS) / (4 * powf(sS, 3 + 0.25f)) * density;
+        const float density = sqrtf(1.225f / Dair);
+        return (PI * 1.0f * 3e-2f * 152.93f * Qw * gm.gS) / (4 * powPos(sS, 3 + 0.25f)) * density;
     }
     return 0.0f;
 }
@@ -120,8 +120,8 @@ __device__ __forceinline__ float PSACR(float Qr, float Qs, float3 fv, float Dair
 __device__ __forceinline__ float PSACI(const Gammas& gm, float T, float Qs, float Qc, float Dair, float sS) {
     if (Qs > 0 && Qc > 0) {
         const float ESI = expf(0.025f * (T));
-        const float density = powf(1.225f / Dair, 0.5f);
-        return (PI * ESI * 3e-2f * 152.93f * Qc * gm.gS) / (4 * powf(sS, 3 + 0.25f)) * density;
+        const float density = sqrtf(1.225f / Dair);
+        return (PI * ESI * 3e-2f * 152.93f * Qc * gm.gS) / (4 * powPos(sS, 3 + 0.25f)) * density;
     }
     return 0.0f;
 }
@@ -147,10 +147,10 @@ __device__ __forceinline__ float PSFW(float dt, float T, float ps, float Qs, flo
         const float RI50 = 5e-3f, mI50 = 4.8e-7f, mI40 = 2.46e-7f, UI50 = 100.0f;
         float a1g, a2g;
         bergeron(T, ps, Qv, QWI, a1g, a2g);
-        const float dt1 = 1 / (a1g * (1 - a2g)) * (powf(mI50, 1 - a2g) - powf(mI40, 1 - a2g));
+        const float dt1 = 1 / (a1g * (1 - a2g)) * (powPos(mI50, 1 - a2g) - powPos(mI40, 1 - a2g));
         const float Qc50 = Qc * (dt / dt1) / 100;
         const float NI50 = Qc50 / mI50;
-        return NI50 * (a1g * powf(mI50, a2g) + PI * 1.0f * (Dair * 0.001f) * Qw * RI50 * RI50 * UI50);
+        return NI50 * (a1g * powPos(mI50, a2g) + PI * 1.0f * (Dair * 0.001f) * Qw * RI50 * RI50 * UI50);
     }
     return 0.0f;
 }
@@ -160,7 +160,7 @@ __device__ __forceinline__ float PSFI(float T, float ps, float Qs, float Qc, flo
         const float mI50 = 4.8e-7f, mI40 = 2.46e-7f;
         float a1g, a2g;
         bergeron(T, ps, Qv, QWI, a1g, a2g);
-        if (a1g > 0) return a2g * a1g * Qc / (powf(mI50, 0.5f) - powf(mI40, 0.5f));
+        if (a1g > 0) return a2g * a1g * Qc / (sqrtf(mI50) - sqrtf(mI40));
     }
     return 0.0f;
 }
@@ -175,10 +175,10 @@ __device__ __forceinline__ float PSDEP(const Gammas& gm, float T, float ps, floa
         float Si = Qv / QWI;
         const float A = Ls * Ls / (Ka * Rsw * tK * tK);
         const float B = 1 / (Dair * 0.001f * d_wi(T, ps) * dQv);
-        const float density = powf(1.225f / Dair, 0.25f);
+        const float density = powPos(1.225f / Dair, 0.25f);
         return PI * PI * (Si - 1) / (Dair * 0.001f * (A + B)) * 3e-2f *
-               (0.78f * powf(sS, -2) + 0.31f * powf(Sc, 1.0f / 3.0f) * gm.gSS * powf(c, 0.5f) * density * powf(kv, -0.5f) *
-                                           powf(sS, -(d + 5) / 2));
+               (0.78f * powPos(sS, -2) + 0.31f * powPos(Sc, 1.0f / 3.0f) * gm.gSS * sqrtf(c) * density * powPos(kv, -0.5f) *
+                                           powPos(sS, -(d + 5) / 2));
     }
     return 0.0f;
 }
@@ -191,10 +191,10 @@ __device__ __forceinline__ float PSMLT(const Gammas& gm, float T, float ps, floa
         const float kv = d_ViscAir(T);
         float Sc = kv / dQv;
         const float Drs = d_ws(0.0f, ps) - Qv;
-        const float density = powf(1.225f / Dair, 0.25f);
+        const float density = powPos(1.225f / Dair, 0.25f);
         return fmaxf(0.0f, -1 * (-(2 * PI / (density * 0.001f * Lf)) * (Ka * T - E0v * dQv * density * 0.001f * Drs) * 3e-2f *
-                                     (0.78f * powf(sS, -2) + 0.31f * powf(Sc, 1.0f / 3.0f) * gm.gSS * powf(c, 0.5f) *
-                                                                 density * powf(kv, -0.5f) * powf(sS, -(d + 5) / 2)) -
+                                     (0.78f * powPos(sS, -2) + 0.31f * powPos(Sc, 1.0f / 3.0f) * gm.gSS * sqrtf(c) *
+                                                                 density * powPos(kv, -0.5f) * powPos(sS, -(d + 5) / 2)) -
                                  (Cvl * T / Lf) * (psacw + psacr)));
     }
     return 0.0f;
@@ -208,20 +208,20 @@ __device__ __forceinline__ float PGAUT(float T, float Qs, float QiMin) {
 // :472-487
 __device__ __forceinline__ float PGFR(float T, float Qr, float Dair, float sR) {
     if (Qr > 0.0f && T < 0.0f)
-        return 20 * PI * PI * 100.0f * 8e-2f * (0.99f / (Dair * 0.001f)) * (expf(0.66f * (-T)) - 1) * powf(sR, -7);
+        return 20 * PI * PI * 100.0f * 8e-2f * (0.99f / (Dair * 0.001f)) * (expf(0.66f * (-T)) - 1) * powPos(sR, -7);
     return 0.0f;
 }
 __device__ __forceinline__ float hailTerm(float Dair) {
-    return powf(4 * g * 100 * 0.91f / (3 * 0.6f * Dair * 0.001f), 0.5f);
+    return sqrtf(4 * g * 100 * 0.91f / (3 * 0.6f * Dair * 0.001f));
 }
 // :489-504
 __device__ __forceinline__ float PGACW(const Gammas& gm, float Qi, float Qw, float sI, float Dair) {
-    if (Qi > 0.0f && Qw > 0.0f) return PI * 1.0f * 4e-4f * Qw * gm.gI / (4 * powf(sI, 3.5f)) * hailTerm(Dair);
+    if (Qi > 0.0f && Qw > 0.0f) return PI * 1.0f * 4e-4f * Qw * gm.gI / (4 * powPos(sI, 3.5f)) * hailTerm(Dair);
     return 0.0f;
 }
 // :506-521
 __device__ __forceinline__ float PGACI(const Gammas& gm, float Qi, float Qc, float sI, float Dair) {
-    if (Qi > 0.0f && Qc > 0.0f) return PI * 1.0f * 4e-4f * Qc * gm.gI / (4 * powf(sI, 3.5f)) * hailTerm(Dair);
+    if (Qi > 0.0f && Qc > 0.0f) return PI * 1.0f * 4e-4f * Qc * gm.gI / (4 * powPos(sI, 3.5f)) * hailTerm(Dair);
     return 0.0f;
 }
 // :523-539
@@ -245,7 +245,7 @@ __device__ __forceinline__ float PGDRY(float Qw, float Qc, float Qr, float Qs, f
     return a + b + c + d;
 }
 __device__ __forceinline__ float brackI(const Gammas& gm, float sI, float Sc, float kv, float density) {
-    return (0.78f * powf(sI, -2) + 0.31f * powf(Sc, 1.0f / 3.0f) * gm.gRI * density * powf(kv, -0.5f) * powf(sI, -2.75f));
+    return (0.78f * powPos(sI, -2) + 0.31f * powPos(Sc, 1.0f / 3.0f) * gm.gRI * density * powPos(kv, -0.5f) * powPos(sI, -2.75f));
 }
 // :572-600
 __device__ __forceinline__ float PGSUB(const Gammas& gm, float T, float ps, float Qv, float QWI, float Qi, float Dair,
@@ -258,7 +258,7 @@ __device__ __forceinline__ float PGSUB(const Gammas& gm, float T, float ps, floa
         float Sc = kv / dQv;
         const float A = Ls * Ls / (Ka * Rsw * tK * tK);
         const float B = 1 / (Dair * 0.001f * d_wi(T, ps) * dQv);
-        const float density = powf(4 * g * 100 * 0.91f / (3 * 0.6f * Dair * 0.001f), 0.25f);
+        const float density = powPos(4 * g * 100 * 0.91f / (3 * 0.6f * Dair * 0.001f), 0.25f);
         return fmaxf(0.0f, -1 * PI * PI * (Si - 1) / (Dair * 0.001f * (A + B)) * 4e-4f * brackI(gm, sI, Sc, kv, density));
     }
     return 0.0f;
@@ -326,7 +326,7 @@ __device__ __forceinline__ float PGGMLT(float tGC, float Qi, float Dair, float s
         const float af = 0.45f;
         const float QGround = af * k4 * tGC / iceHeight;
         const float meltRate = fmaxf(0.0f, QGround / Lf);
-        return meltRate * 4e-4f * powf(sI, -2) * af / (Dair * 0.001f);
+        return meltRate * 4e-4f * powPos(sI, -2) * af / (Dair * 0.001f);
     }
     return 0.0f;
 }
@@ -380,7 +380,7 @@ __device__ __forceinline__ float PGREVP(float tAK, float tGC, float ps, float Qv
         const float y = Cpd * ps * 100.0f / (0.622f * E0v);
         const float esw = 0.96f, a = 0.08f;
         const float aP = d_es(tAK - 273.15f) * 10.0f * (Qv / d_ws(tAK - 273.15f, ps));
-        const float ea = 1.24f * powf((aP / tAK), 1.0f / 7.0f);
+        const float ea = 1.24f * powPos((aP / tAK), 1.0f / 7.0f);
         const float Rn = irr * (1 - a) + oo * esw * (ea * (tAK * tAK * tAK * tAK) - (tGK * tGK * tGK * tGK));
         const float RnG = Rn - 0.2f * Rn;
         const float m = d_es(tGC) * 1000.0f * E0v / (Rsw * tGK * tGK);
@@ -467,7 +467,7 @@ __device__ __forceinline__ float PGRFR(float tAK, float tGC, float ps, float Qr,
         const float densW = 0.99f;
         const float Rr = Qr * Dair * fv.x;
         const float P = Rr * 3600.0f / (densW * 1000.0f) * 1000;
-        const float FFW = sqrtf(powf(P * densW / 3.6f, 2.0f) + powf(0.067f * powf(P, 0.846f) * wind, 2.0f));
+        const float FFW = sqrtf(powPos(P * densW / 3.6f, 2.0f) + powPos(0.067f * powPos(P, 0.846f) * wind, 2.0f));
         Qf = Lf * 0.001f * FFW;
         Qw = Cvl * 0.001f * (0 - Tac) * FFW;
         const float k = 0.41f, z0m = 0.1f, z0h = z0m * 0.1f;

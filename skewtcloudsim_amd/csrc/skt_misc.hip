@@ -43,8 +43,8 @@ struct KGroundWater {
             const float sWater = fromPrev ? rsFrom : rsC;
             float changeRain = 0.0f;
             if (sRain != 0.0f) {
-                const float sp = n * powf(sRain, 2.0f / 3.0f);
-                const float vel = sp * powf(slope, 0.5f);
+                const float sp = n * powPos(sRain, 2.0f / 3.0f);
+                const float vel = sp * sqrtf(slope);
                 changeRain = vel * sRain;
             }
             const float changeWater = kc::g * kk * sWater * slope;
@@ -131,7 +131,7 @@ struct KGroundColumns {
                 const float wind = a.u[i];
                 float3 fv = make_float3(0.0f, 0.0f, 0.0f);
                 const float Dair = a.rho[i];
-                const float tAK = a.th[i] * powf(ps / a.gP[g], kap);
+                const float tAK = a.th[i] * powNear1(ps / a.gP[g], kap);
                 if (Qgr > 0.0f || Qgs > 0.0f || Qgi > 0.0f) fv = d_fallVel(Qgr, Qgs, Qgi, Dair, 3, gm.R48, gm.S425, gm.S425);
                 const float GR = dt * fminf(Qr * speed, Qr);
                 const float GS = dt * fminf(Qs * speed, Qs);
@@ -170,7 +170,7 @@ struct KGroundColumns {
                 const float cc = a.cover[g];
                 const float Dair = a.rho[i];
                 float tme = a.gt[g];
-                const float tAK = a.th[i] * powf(ps / a.gP[g], kap);
+                const float tAK = a.th[i] * powNear1(ps / a.gP[g], kap);
                 const float sI = d_slope(Dair, Qgi, 2);
                 float PGREVP = mp::PGREVP(tAK, tGC, ps, Qv, Qgr, irr, Dair, wind);
                 float PGSMLT = mp::PGSMLT(tAK, tGC, ps, Qgs, Qv, Qgr, irr, cc, Dair, wind, dt);
@@ -230,6 +230,10 @@ hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed,
 // thread reads the pre-step value of the level above its chunk (which the
 // chunk above overwrites) before the block's barrier, and writes only after
 // it, so the chunks run side by side with the results of one walk per column.
+// One walk carries every field of the batch (rain, snow, ice): a level's
+// temperature theta (p / p_ground)^kappa, virtual temperature and density
+// are computed once and serve each field's fall speed, and the level's
+// theta, p and vapour are read once.
 constexpr int PRECIP_CHUNKS = 4;
 static_assert(64 * PRECIP_CHUNKS <= 256, "k_precip blocks (64 x PRECIP_CHUNKS) must fit the 256-thread launch bound");
 template <bool MIX>
@@ -243,17 +247,17 @@ struct KPrecip {
     }
     static __device__ __forceinline__ void run(VB vb, const Geom& G, const PrecipBatch& pbt, const float* __restrict__ th, const float* __restrict__ p,
                      const float* __restrict__ qv, const float* __restrict__ gP, float dx, const Gammas& gm) {
-        // tiles: column blocks of 64 (fastest), then the batch's fields
-        const unsigned nbx = (unsigned)(((long)G.X * G.Z + 63) / 64);
-        const unsigned fi = vb.id / nbx, cb = vb.id % nbx;
-        const float dts = pbt.dts[fi];
-        if (dts == 0.0f) return;  // the whole block
-        const long g = cb * 64L + threadIdx.x;
+        // tiles: column blocks of 64
+        const long g = vb.id * 64L + threadIdx.x;
         const bool col = g < (long)G.X * G.Z;
         const long gc = col ? g : 0;
-        float* q = pbt.q[fi];
-        const Store st = pbt.st[fi];
-        const int type = pbt.type[fi];
+        bool on[3];
+        float dts[3];
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            on[f] = f < pbt.n && pbt.dts[f] != 0.0f;
+            dts[f] = on[f] ? pbt.dts[f] : 0.0f;
+        }
         const int x = (int)(gc % G.X), zc = (int)(gc / G.X);
         const float kap = kc::Rsd / kc::Cpd;
         const float gp = gP[gc];
@@ -263,52 +267,73 @@ struct KPrecip {
         const int ylo = (int)threadIdx.y * yc, yhi = min(G.Y, ylo + yc);
         const int ya = max(ylo, y0);
         const bool work = col && ya < yhi;
-        const float qTop = work && yhi < G.Y ? ldS<MIX>(q, gidx(G, x, yhi, zc), st) : 0.0f;
+        float qTop[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+            if (on[f] && work && yhi < G.Y) qTop[f] = ldS<MIX>(pbt.q[f], gidx(G, x, yhi, zc), pbt.st[f]);
         __syncthreads();
         if (!work) return;
-        // per level: Q, theta, p, qv; levels y+1 and y+2 are loaded ahead
+        // per level: each field's Q, theta, p, qv; levels y+1 and y+2 are
+        // loaded ahead
         struct Lv {
-            float q, t, pr, v;
+            float q[3], t, pr, v;
         };
-        auto fvAt = [&](const Lv& L) {
-            const float T = L.t * powf(L.pr / gp, kap);
+        auto densOf = [&](const Lv& L) {
+            const float T = L.t * powNear1(L.pr / gp, kap);
             const float Tv = T * (0.608f * L.v + 1);
-            const float den = L.pr * 100 / (kc::Rsd * Tv);
-            const float3 v3 = d_fallVel(L.q, L.q, L.q, den, type, gm.R48, gm.S425, gm.S425);
+            return L.pr * 100 / (kc::Rsd * Tv);
+        };
+        auto fvOf = [&](float q, float den, int f) {
+            const int type = pbt.type[f];
+            const float3 v3 = d_fallVel(q, q, q, den, type, gm.R48, gm.S425, gm.S425);
             return type == 0 ? v3.x : (type == 1 ? v3.y : v3.z);
         };
         const long i0 = gidx(G, x, ya, zc);
         auto load = [&](int y, Lv& L) {
             if (y <= yhi && y < G.Y) {
                 const long j = i0 + (long)(y - ya) * G.X;
-                L.q = y == yhi ? qTop : ldS<MIX>(q, j, st);
+#pragma unroll
+                for (int f = 0; f < 3; f++)
+                    if (on[f]) L.q[f] = y == yhi ? qTop[f] : ldS<MIX>(pbt.q[f], j, pbt.st[f]);
                 L.t = th[j];
                 L.pr = p[j];
                 L.v = qv[j];
             }
         };
-        Lv l0{0.0f, 0.0f, 0.0f, 0.0f}, l1 = l0, l2 = l0;
+        Lv l0{{0.0f, 0.0f, 0.0f}, 0.0f, 0.0f, 0.0f}, l1 = l0, l2 = l0;
         load(ya, l0);
         load(ya + 1, l1);
-        float fv = fvAt(l0);
+        float fv[3] = {0.0f, 0.0f, 0.0f};
+        {
+            const float den = densOf(l0);
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+                if (on[f]) fv[f] = fvOf(l0.q[f], den, f);
+        }
         long i = i0;
         for (int y = ya; y < yhi; y++, i += G.X) {
             load(y + 2, l2);
             const bool up = y < G.Y - 1;
-            const float Qc = l0.q, Qu = up ? l1.q : 0.0f;
-            const float fvu = up ? fvAt(l1) : 0.0f;
-            float nq = Qc;
-            nq -= dts * fminf((fv / dx) * Qc, Qc);
-            if (up) nq += dts * fminf((fvu / dx) * Qu, Qu);
-            stS<MIX>(q, i, nq, st);
-            fv = fvu;
+            const float denU = up ? densOf(l1) : 0.0f;
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                if (!on[f]) continue;
+                const float Qc = l0.q[f], Qu = up ? l1.q[f] : 0.0f;
+                const float fvu = up ? fvOf(l1.q[f], denU, f) : 0.0f;
+                float nq = Qc;
+                nq -= dts[f] * fminf((fv[f] / dx) * Qc, Qc);
+                if (up) nq += dts[f] * fminf((fvu / dx) * Qu, Qu);
+                stS<MIX>(pbt.q[f], i, nq, pbt.st[f]);
+                fv[f] = fvu;
+            }
             l0 = l1;
             l1 = l2;
         }
     }
 };
 hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt) {
-    const dim3 grid((unsigned)((e.G + 63) / 64), pbt.n), block(64, PRECIP_CHUNKS);
+    if (pbt.n < 1 || pbt.n > 3) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((e.G + 63) / 64)), block(64, PRECIP_CHUNKS);
     if (e.mix)
         return klaunch<KPrecip<true>>(e, grid, block, 0, geom(e), pbt, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE],
                                       e.f[SKT_QV], e.f[SKT_GROUND_P], e.dx, e.gm);
@@ -362,7 +387,7 @@ struct KBuoyancy {
             L.th = th[j];
             L.p = p[j];
             L.q = qv[j];
-            L.T = L.th * powf(L.p / gps, kap);
+            L.T = L.th * powNear1(L.p / gps, kap);
             L.ws = d_ws((L.T - 273.15f), L.p);
         };
         Lev lm{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, lc = lm, lp = lm;  // levels y - 1, y, y + 1
@@ -396,14 +421,14 @@ struct KBuoyancy {
                     const float TDown = lm.T;
                     if ((T < 0.0f && QP >= d_wi((TDown - 273.15f), psD)) || QP >= lm.ws) {
                         TadD = T - d_MLR(T - 273.15f, psC) * (psC - psD);
-                        TadD = TadD * powf(gps / psD, kap);
+                        TadD = TadD * powNear1(gps / psD, kap);
                     }
                 }
                 if (up) {
                     const float TUp = lp.T;
                     if ((T < 0.0f && QP >= d_wi((TUp - 273.15f), psU)) || QP >= lp.ws) {
                         TadU = d_MLR(T - 273.15f, psC) * (psU - psC) + T;
-                        TadU = TadU * powf(gps / psU, kap);
+                        TadU = TadU * powNear1(gps / psU, kap);
                     }
                 }
                 float B = 0.0f, BU = 0.0f, BD = 0.0f;
@@ -454,7 +479,7 @@ struct KInitDensity {
         (void)bid;
         if (!inb || isGround(G, x, y, z)) return;
         const long i = gidx(G, x, y, z);
-        const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+        const float T = th[i] * powNear1(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
         const float Tv = T * (0.608f * qv[i] + 1);
         const float d = p[i] * 100 / (kc::Rsd * Tv);
         rho[i] = d;
@@ -474,7 +499,7 @@ struct KNewPressure {
         (void)bid;
         if (!inb || isGround(G, x, y, z)) return;
         const long i = gidx(G, x, y, z);
-        const float T = th[i] * powf(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+        const float T = th[i] * powNear1(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
         const float Tv = T * (0.608f * qv[i] + 1);
         p[i] = rho[i] * kc::Rsd * Tv / 100.0f;
     }
@@ -563,7 +588,7 @@ struct KMicrophysics {
             const float ps = a.p[i];
             float3 fv = make_float3(0.0f, 0.0f, 0.0f);
             const float Dair = a.rho[i];
-            const float tK = a.th[i] * powf(ps / a.gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
+            const float tK = a.th[i] * powNear1(ps / a.gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
             const float T = tK - 273.15f;
             if (Qr > 0.0f || Qs > 0.0f || Qi > 0.0f) fv = d_fallVel(Qr, Qs, Qi, Dair, 3, gm.R48, gm.S425, gm.S425);
             const float QWS = d_ws(T, ps);

@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from skewtcloudsim_amd.build_lib import HIPCC_FLAGS, hipcc  # noqa: E402
 
-LIBS = {"libfacecheck.so": ["face_check.hip"]}
+LIBS = {"libfacecheck.so": ["face_check.hip"], "libpowcheck.so": ["pow_check.hip"]}
 
 
 def build():
