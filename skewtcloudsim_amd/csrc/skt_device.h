@@ -293,14 +293,35 @@ __device__ __forceinline__ float ppmFlux(float vel, float L, float Cc, float Rr,
     return vel * qq;
 }
 
-// ------------------------------------------------------ pressure operator ---
-// Face coefficient of initAMatrix, Game/kernelSky.cu:1372-1377.
-__device__ __forceinline__ float faceCoef(float cur, float nb) {
-    return nb == 0.0f ? 0.0f : -fmaxf(1.0f, 1.225f / (2.0f / (1.0f / cur + 1.0f / nb)));
+// ---------------------------------------------------------- reciprocals ---
+// The correctly rounded reciprocal in three instructions: the hardware
+// reciprocal estimate with one FMA refinement. For |s| in [2^-126, 2^126]
+// it equals the IEEE 1.0f / s bit for bit (checked for every such float by
+// tests/test_face_coefficient.py); other s take the IEEE division (eleven
+// instructions).
+__device__ __forceinline__ bool recipOk(float s) {
+    const float m = fabsf(s);
+    return m >= 0x1p-126f && m <= 0x1p126f;
+}
+__device__ __forceinline__ float recipRNfast(float s) {
+    const float e = __builtin_amdgcn_rcpf(s);
+    return __builtin_fmaf(__builtin_fmaf(-s, e, 1.0f), e, e);
+}
+__device__ __forceinline__ float recipRN(float s) {
+    if (!recipOk(s)) return 1.0f / s;
+    return recipRNfast(s);
+}
+// 2 / s rounded to nearest: 2 RN(1/s) (scaling by 2 is exact there).
+__device__ __forceinline__ float twoOverRN(float s) {
+    if (!recipOk(s)) return 2.0f / s;
+    return 2.0f * recipRNfast(s);
 }
 
-// The face coefficient as a function of s = 1/cur + 1/nb (the sum the
-// reference forms): -max(1, 1.225 / (2 / s)) with both divisions rounded to
+// ------------------------------------------------------ pressure operator ---
+// Face coefficient of initAMatrix (Game/kernelSky.cu:1372-1377),
+// nb == 0 ? 0 : -max(1, 1.225 / (2 / (1/cur + 1/nb))), as a function of
+// s = 1/cur + 1/nb (the sum the reference forms): -max(1, 1.225 / (2 / s))
+// with both divisions rounded to
 // nearest. Scaling by 2 is exact, so 2 / s rounds to 2 RN(1/s) and the second
 // quotient equals RN(0.6125 / RN(1/s)) (0.6125f is 1.225f / 2 exactly). The
 // short form takes RN(1/s) from the hardware reciprocal estimate with one FMA
@@ -321,6 +342,22 @@ __device__ __forceinline__ float faceOfSum(float s) {
     if (faceFastOk(s)) return faceFast(s);
     return -fmaxf(1.0f, 1.225f / (2.0f / s));
 }
+// Densities whose reciprocals, face sums, harmonic means and operator
+// diagonals all stay inside the ranges where recipRNfast and faceFast equal
+// the IEEE expressions: |d| in [2^-100, 2^100] gives reciprocals there, sums
+// of two in [2^-100, 2^101], faces and diagonals below 2^104. A pass checks
+// this once per cell and takes the fast forms for the whole cell.
+__device__ __forceinline__ bool densOk(float d) {
+    const float m = fabsf(d);
+    return m >= 0x1p-100f && m <= 0x1p100f;
+}
+// Fast face from two reciprocals of densities that pass densOk (an infinite
+// one, a zero neighbour density or a ground face, gives 0).
+__device__ __forceinline__ float faceIfast(float ic, float in) {
+    const float v = faceFast(ic + in);
+    return isinf(in) ? 0.0f : v;
+}
+
 // A sky cell's 1/density for which every face sum with another such value
 // lies in the short form's range (or is infinite, a zero face).
 __device__ __forceinline__ bool faceValueOk(float i) { return i >= 0x1p-127f && i <= 0x1p125f; }

@@ -23,6 +23,7 @@
 // updated state in the other of two slots.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <type_traits>
 #include "skt_internal.h"
 #include "skt_launch.h"
 #include "skt_reduce.h"
@@ -834,7 +835,7 @@ __device__ __forceinline__ float trunc8(float c) { return fminf(fmaxf(truncf(c),
 
 // Column marches (preconditioner, divergence, projection): chunks of YC_DIV
 // levels, blocks of 64 (x) x 4 (z) columns of one chunk.
-constexpr int YC_DIV = 16;
+constexpr int YC_DIV = 8;
 static unsigned divBlocks(const EnvState& e) {
     return (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
 }
@@ -848,14 +849,22 @@ struct KSetInt {
         if (threadIdx.x == 0) *a = v;
     }
 };
-// A march up the columns in the chunks of the divergence march: the ground
-// heights of the column and its four neighbour columns are read once and the
-// densities of the levels below and above are carried. Each neighbour read
-// follows extv: beyond the domain the sides condition on the cell's own
-// density (the clamped position), at ground the ground condition.
+// A march up the columns in the chunks of the divergence march. Each sky
+// cell's diagonal is the SpMV's own (opRow's faces and summation order:
+// faceI of the two cells' 1/density, the cell's own value beyond the
+// domain, a zero face at ground), which is the reference's
+// -sum of -max(1, 1.225 / (2 / (1/rho + 1/rho_nb))) bit for bit; ground
+// cells take irho 0 and preconditioner 1. Reciprocals are the correctly
+// rounded recipRN; the level above's is carried into the next step as the
+// cell's own, and the own one into the step after as the level below's.
+// The loads of a group of PRE_G levels (the level above and the four
+// horizontal neighbours, at clamped in-range positions, so that they are
+// unconditional) are issued together before the group is computed.
+constexpr int PRE_G = 4;
 struct KBuildPrecon {
-    static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, const float* __restrict__ rho, float* __restrict__ irho,
-                           float* __restrict__ pre, int* __restrict__ faceExact) {
+    static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, const float* __restrict__ rho,
+                                               float* __restrict__ irho, float* __restrict__ pre,
+                                               int* __restrict__ faceExact) {
         const unsigned lin = vb.id;
         const int chunk = (int)(lin % nyc);
         const unsigned tile = lin / nyc;
@@ -871,46 +880,71 @@ struct KBuildPrecon {
         const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
         const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
         const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
-        const Bounds b{BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET};
+        const long oR = inR ? 1 : 0, oL = inL ? -1 : 0, oF = inF ? XY : 0, oB = inB ? -XY : 0;
         const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
         if (y0 >= y1) return;
-        float rd = y0 > 0 ? rho[col + (long)X * (y0 - 1)] : 0.0f;  // density of the level below
-        float c = rho[col + (long)X * y0];
+        const float INF = __builtin_inff();
+        float icDn = y0 > 0 ? recipRN(rho[col + (long)X * (y0 - 1)]) : 0.0f;  // level below
+        float ic = recipRN(rho[col + (long)X * y0]);
         bool bad = false;
-        for (int y = y0; y < y1; y++) {
-            const long i = col + (long)X * y;
-            const float ru = y < Y - 1 ? rho[i + X] : 0.0f;
-            if (y <= gh) {
-                irho[i] = 0.0f;
-                pre[i] = 1.0f;
-            } else {
-                const float nR = !inR ? bcv(b.sides, c, 0.0f) : (y <= ghR ? bcv(b.ground, rho[i + 1], 0.0f) : rho[i + 1]);
-                const float nL = !inL ? bcv(b.sides, c, 0.0f) : (y <= ghL ? bcv(b.ground, rho[i - 1], 0.0f) : rho[i - 1]);
-                const float nU = y + 1 > Y - 1 ? bcv(b.sides, c, 0.0f) : ru;  // the level above a sky cell is sky
-                const float nD = y - 1 < 0 ? bcv(b.sides, c, 0.0f) : (y - 1 <= gh ? bcv(b.ground, rd, 0.0f) : rd);
-                const float nF = !inF ? bcv(b.sides, c, 0.0f) : (y <= ghF ? bcv(b.ground, rho[i + XY], 0.0f) : rho[i + XY]);
-                const float nB = !inB ? bcv(b.sides, c, 0.0f) : (y <= ghB ? bcv(b.ground, rho[i - XY], 0.0f) : rho[i - XY]);
-                const float r = faceCoef(c, nR);
-                const float l = faceCoef(c, nL);
-                const float u = faceCoef(c, nU);
-                const float d = faceCoef(c, nD);
-                const float f = faceCoef(c, nF);
-                const float bb = faceCoef(c, nB);
-                float w = 0.0f;
-                w -= r;
-                w -= l;
-                w -= u;
-                w -= d;
-                w -= f;
-                w -= bb;
-                const float ic = 1.0f / c;
-                irho[i] = ic;
-                pre[i] = w == 0.0f ? 1.0f : 1.0f / w;
+        for (int yb = y0; yb < y1; yb += PRE_G) {
+            float rU[PRE_G], rR[PRE_G], rL[PRE_G], rF[PRE_G], rB[PRE_G];
+#pragma unroll
+            for (int k = 0; k < PRE_G; k++) {
+                const int y = min(yb + k, y1 - 1);
+                const long i = col + (long)X * y;
+                rU[k] = rho[y + 1 < Y ? i + X : i];
+                rR[k] = rho[i + oR];
+                rL[k] = rho[i + oL];
+                rF[k] = rho[i + oF];
+                rB[k] = rho[i + oB];
+            }
+#pragma unroll
+            for (int k = 0; k < PRE_G; k++) {
+                const int y = yb + k;
+                if (y >= y1) break;  // uniform: the chunk is the block's
+                const long i = col + (long)X * y;
+                const bool sU = y + 1 < Y;
+                if (y <= gh) {
+                    irho[i] = 0.0f;
+                    pre[i] = 1.0f;
+                    icDn = ic;
+                    ic = sU ? recipRN(rU[k]) : 0.0f;
+                    continue;
+                }
+                const bool sR = inR && y > ghR, sL = inL && y > ghL, sF = inF && y > ghF, sB = inB && y > ghB;
+                const bool sD = y > 0 && y - 1 > gh;
+                // one range test for the cell: its own 1/density and every
+                // sky neighbour density it takes a reciprocal of
+                const bool fast = densOk(ic) && (!sU || densOk(rU[k])) && (!sR || densOk(rR[k])) &&
+                                  (!sL || densOk(rL[k])) && (!sF || densOk(rF[k])) && (!sB || densOk(rB[k]));
+                auto cell = [&](auto fastTag) __attribute__((always_inline)) {
+                    constexpr bool F = decltype(fastTag)::value;
+                    auto rcp = [&](float d) { return F ? recipRNfast(d) : recipRN(d); };
+                    auto face = [&](float a, float b) { return F ? faceIfast(a, b) : faceI(a, b); };
+                    const float icU = sU ? rcp(rU[k]) : 0.0f;
+                    const float iR = !inR ? ic : (sR ? rcp(rR[k]) : INF);
+                    const float iL = !inL ? ic : (sL ? rcp(rL[k]) : INF);
+                    const float iU = sU ? icU : ic;
+                    const float iD = y == 0 ? ic : (sD ? icDn : INF);
+                    const float iF = !inF ? ic : (sF ? rcp(rF[k]) : INF);
+                    float w = 0.0f;
+                    w -= face(ic, iR);
+                    w -= face(ic, iL);
+                    w -= face(ic, iU);
+                    w -= face(ic, iD);
+                    w -= face(ic, iF);
+                    w -= z == 0 ? face(ic, ic) : (sB ? face(rcp(rB[k]), ic) : 0.0f);
+                    irho[i] = ic;
+                    pre[i] = w == 0.0f ? 1.0f : rcp(w);
+                    return icU;
+                };
+                const float icU = fast ? cell(std::true_type{}) : cell(std::false_type{});
                 // the SpMV's short face form is exact when every sky 1/density is in range
                 bad |= !faceValueOk(ic);
+                icDn = ic;
+                ic = icU;
             }
-            rd = c;
-            c = ru;
         }
         if (bad) atomicOr(faceExact, 1);
     }
@@ -919,8 +953,8 @@ hipError_t launch_build_precon(EnvState& e) {
     hipError_t r = klaunch<KSetInt>(e, dim3(1), dim3(64), 0, &e.sc->face_exact, 0);
     if (r != hipSuccess) return r;
     const unsigned nyc = (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
-    return klaunch<KBuildPrecon>(e, dim3(divBlocks(e)), B3, 0, geom(e), YC_DIV, nyc, e.f[SKT_DENSITY], e.irho,
-                                 e.pcg[5], &e.sc->face_exact);
+    return klaunch<KBuildPrecon>(e, dim3(divBlocks(e)), B3, 0, geom(e), YC_DIV, nyc, (const float*)e.f[SKT_DENSITY],
+                                 e.irho, e.pcg[5], &e.sc->face_exact);
 }
 
 // Scalar steps of the solve, run by every block of the consuming kernel
@@ -1044,12 +1078,18 @@ __device__ void pcgPrologue(const double* __restrict__ pm, const double* __restr
 // (Game/environment.cu:933-936). p' is cleared.
 // A march up the columns: each thread walks a chunk of YC_DIV levels of its
 // (x, z) column. The ground heights of the column and of its four neighbour
-// columns are read once, and the density and v of the level below come from
-// the step before. Each neighbour value follows extv / nbOwn: a position
-// beyond the domain clamps onto this sky cell and takes the sides condition,
-// a ground position the ground condition (the level above a sky cell is
-// sky); the arithmetic per cell is unchanged. Blocks are 64 (x) x 4 (z)
-// columns of one chunk, each writing one partial maximum.
+// columns are read once; the density, its reciprocal and v of the level
+// below come from the step before, and the level above's density and
+// reciprocal become the next step's own. Each neighbour value follows extv /
+// nbOwn: a position beyond the domain clamps onto this sky cell and takes
+// the sides condition, a ground position the ground condition (the level
+// above a sky cell is sky); the arithmetic per cell is the reference's, with
+// the correctly rounded recipRN / twoOverRN for 1/density and the harmonic
+// face densities. The loads of a group of DIV_G levels are issued together
+// (at clamped in-range positions, so that they are unconditional) before the
+// group is computed. Blocks are 64 (x) x 4 (z) columns of one chunk, each
+// writing one partial maximum.
+constexpr int DIV_G = 4;
 struct KDivergence {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, float* __restrict__ r, float* __restrict__ pp,
                          const float* __restrict__ rho, const float* __restrict__ rold, const float* __restrict__ u,
@@ -1063,7 +1103,8 @@ struct KDivergence {
         const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
         const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
         double m = 0.0;
-        if (x < G.X && z < G.Z) {
+        const int y0 = chunk * yc, y1 = min(G.Y, y0 + yc);
+        if (x < G.X && z < G.Z && y0 < y1) {
             const int X = G.X, Y = G.Y, Z = G.Z;
             const long XY = (long)X * Y;
             const long col = XY * z + x;  // cell (x, 0, z)
@@ -1072,46 +1113,96 @@ struct KDivergence {
             const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
             const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
             const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
+            const long oR = inR ? 1 : 0, oL = inL ? -1 : 0, oF = inF ? XY : 0, oB = inB ? -XY : 0;
             const Bounds bd{BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET}, bv{BC_CUSTOM, BC_CUSTOM, BC_DIRICHLET};
             const bool ab = absmode || z == 0;
-            const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
             float rhoB = 0.0f, vB = 0.0f;  // density and v of the level below
-            if (y0 > 0 && y0 < y1) {
+            if (y0 > 0) {
                 rhoB = rho[col + (long)X * (y0 - 1)];
                 vB = v[col + (long)X * (y0 - 1)];
             }
-            for (int y = y0; y < y1; y++) {
-                const long i = col + (long)X * y;
-                pp[i] = 0.0f;
-                const float cur = rho[i], vc = v[i];
-                float dv = 0.0f;
-                if (y > gh) {
-                    const float ur = u[i];
-                    const float ul = !inL ? bcv(bv.sides, ur, defU[y]) : (y <= ghL ? bcv(bv.ground, ur, defU[y]) : u[i - 1]);
-                    const float uu = vc;
-                    const float ud = y - 1 < 0 ? bcv(bv.sides, vc, 0.0f) : (y - 1 <= gh ? bcv(bv.ground, vc, 0.0f) : vB);
-                    const float uf = w[i];
-                    const float ub = !inB ? bcv(bv.sides, uf, defW[y]) : (y <= ghB ? bcv(bv.ground, uf, defW[y]) : w[i - XY]);
-                    float dr = !inR ? bcv(bd.sides, cur, 1.0f) : (y <= ghR ? bcv(bd.ground, rho[i + 1], 1.0f) : rho[i + 1]);
-                    float dl = !inL ? bcv(bd.sides, cur, 1.0f) : (y <= ghL ? bcv(bd.ground, rho[i - 1], 1.0f) : rho[i - 1]);
-                    float du = y + 1 > Y - 1 ? bcv(bd.sides, cur, 1.0f) : rho[i + X];
-                    float dd = y - 1 < 0 ? bcv(bd.sides, cur, 1.0f) : (y - 1 <= gh ? bcv(bd.ground, rhoB, 1.0f) : rhoB);
-                    float df = !inF ? bcv(bd.sides, cur, 1.0f) : (y <= ghF ? bcv(bd.ground, rho[i + XY], 1.0f) : rho[i + XY]);
-                    float db = !inB ? bcv(bd.sides, cur, 1.0f) : (y <= ghB ? bcv(bd.ground, rho[i - XY], 1.0f) : rho[i - XY]);
-                    dr = dr == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dr);
-                    dl = dl == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dl);
-                    du = du == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / du);
-                    dd = dd == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / dd);
-                    df = df == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / df);
-                    db = db == 0.0f ? 0.0f : 2.0f / (1.0f / cur + 1.0f / db);
-                    const float mfd = ((dr * ur - dl * ul) + (du * uu - dd * ud) + (df * uf - db * ub));
-                    const float dc = (cur - rold[i]) / dt;
-                    dv = (mfd - dc);
-                    m = fmax(m, (double)(ab ? fabsf(dv) : dv));
+            float cur = rho[col + (long)X * y0];
+            float icur = recipRN(cur), icPrev = recipRN(rhoB);
+            for (int yb = y0; yb < y1; yb += DIV_G) {
+                float qU[DIV_G], qR[DIV_G], qL[DIV_G], qF[DIV_G], qB[DIV_G];
+                float qv[DIV_G], qu[DIV_G], quL[DIV_G], qw[DIV_G], qwB[DIV_G], qo[DIV_G];
+#pragma unroll
+                for (int k = 0; k < DIV_G; k++) {
+                    const int y = min(yb + k, y1 - 1);
+                    const long i = col + (long)X * y;
+                    qU[k] = rho[y + 1 < Y ? i + X : i];
+                    qR[k] = rho[i + oR];
+                    qL[k] = rho[i + oL];
+                    qF[k] = rho[i + oF];
+                    qB[k] = rho[i + oB];
+                    qv[k] = v[i];
+                    qu[k] = u[i];
+                    quL[k] = u[i + oL];
+                    qw[k] = w[i];
+                    qwB[k] = w[i + oB];
+                    qo[k] = rold[i];
                 }
-                r[i] = dv;
-                rhoB = cur;
-                vB = vc;
+#pragma unroll
+                for (int k = 0; k < DIV_G; k++) {
+                    const int y = yb + k;
+                    if (y >= y1) break;  // uniform: the chunk is the block's
+                    const long i = col + (long)X * y;
+                    pp[i] = 0.0f;
+                    const float vc = qv[k];
+                    const bool topS = y + 1 > Y - 1;
+                    const float rU = topS ? 0.0f : qU[k];
+                    // one range test for the cell: its own 1/density and every
+                    // sky neighbour density it takes a reciprocal of; the cell
+                    // is then computed with the fast forms throughout
+                    const bool skR = inR && y > ghR, skL = inL && y > ghL, skF = inF && y > ghF, skB = inB && y > ghB;
+                    const bool skD = y - 1 >= 0 && y - 1 > gh;
+                    const bool fast = densOk(icur) && (topS || densOk(rU)) && (!skD || densOk(icPrev)) &&
+                                      (!skR || densOk(qR[k])) && (!skL || densOk(qL[k])) && (!skF || densOk(qF[k])) &&
+                                      (!skB || densOk(qB[k]));
+                    float dv = 0.0f;
+                    auto cell = [&](auto fastTag) __attribute__((always_inline)) {
+                        constexpr bool F = decltype(fastTag)::value;
+                        auto rcp = [&](float d) { return F ? recipRNfast(d) : recipRN(d); };
+                        auto two = [&](float d) { return F ? 2.0f * recipRNfast(d) : twoOverRN(d); };
+                        const float icU = topS ? 0.0f : rcp(rU);
+                        if (y > gh) {
+                            const float ur = qu[k];
+                            const float ul = !inL ? bcv(bv.sides, ur, defU[y]) : (y <= ghL ? bcv(bv.ground, ur, defU[y]) : quL[k]);
+                            const float uu = vc;
+                            const float ud = y - 1 < 0 ? bcv(bv.sides, vc, 0.0f) : (y - 1 <= gh ? bcv(bv.ground, vc, 0.0f) : vB);
+                            const float uf = qw[k];
+                            const float ub = !inB ? bcv(bv.sides, uf, defW[y]) : (y <= ghB ? bcv(bv.ground, uf, defW[y]) : qwB[k]);
+                            float dr = !inR ? bcv(bd.sides, cur, 1.0f) : (y <= ghR ? bcv(bd.ground, qR[k], 1.0f) : qR[k]);
+                            float dl = !inL ? bcv(bd.sides, cur, 1.0f) : (y <= ghL ? bcv(bd.ground, qL[k], 1.0f) : qL[k]);
+                            float du = topS ? bcv(bd.sides, cur, 1.0f) : rU;
+                            float dd = y - 1 < 0 ? bcv(bd.sides, cur, 1.0f) : (y - 1 <= gh ? bcv(bd.ground, rhoB, 1.0f) : rhoB);
+                            float df = !inF ? bcv(bd.sides, cur, 1.0f) : (y <= ghF ? bcv(bd.ground, qF[k], 1.0f) : qF[k]);
+                            float db = !inB ? bcv(bd.sides, cur, 1.0f) : (y <= ghB ? bcv(bd.ground, qB[k], 1.0f) : qB[k]);
+                            // the level below's 1/density is that cell's carried own
+                            // one when it is sky (at y = 0 the cell's own; at ground
+                            // dd is 0 and takes no reciprocal)
+                            const float idd = skD ? icPrev : icur;
+                            dr = dr == 0.0f ? 0.0f : two(icur + (inR ? rcp(dr) : icur));
+                            dl = dl == 0.0f ? 0.0f : two(icur + (inL ? rcp(dl) : icur));
+                            du = du == 0.0f ? 0.0f : two(icur + (topS ? icur : icU));
+                            dd = dd == 0.0f ? 0.0f : two(icur + idd);
+                            df = df == 0.0f ? 0.0f : two(icur + (inF ? rcp(df) : icur));
+                            db = db == 0.0f ? 0.0f : two(icur + (inB ? rcp(db) : icur));
+                            const float mfd = ((dr * ur - dl * ul) + (du * uu - dd * ud) + (df * uf - db * ub));
+                            const float dc = (cur - qo[k]) / dt;
+                            dv = (mfd - dc);
+                            m = fmax(m, (double)(ab ? fabsf(dv) : dv));
+                        }
+                        return icU;
+                    };
+                    const float icU = fast ? cell(std::true_type{}) : cell(std::false_type{});
+                    r[i] = dv;
+                    rhoB = cur;
+                    vB = vc;
+                    icPrev = icur;
+                    cur = rU;
+                    icur = icU;
+                }
             }
         }
         m = blockMax(m, sh);
@@ -1660,24 +1751,45 @@ hipError_t launch_refine_report(EnvState& e) {
 }
 
 // updatePressure (Game/kernelSky.cu:1016-1030) fused with the initDensity that
-// follows it (Game/environment.cu:878-879).
+// follows it (Game/environment.cu:878-879). Each thread takes ZPT cells
+// along z; their loads are unconditional (a ground cell's values are read
+// and dropped), so they issue together.
 struct KUpdatePressureDensity {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, float* __restrict__ p, const float* __restrict__ pp, float* __restrict__ rho,
                                       const float* __restrict__ th, const float* __restrict__ qv,
                                       const float* __restrict__ gP) {
-        CELL_SETUP_V;
+        constexpr int zc = ZPT;
+        ZM_SETUP_V;
         (void)bid;
-        if (!inb || isGround(G, x, y, z)) return;
-        const long i = gidx(G, x, y, z);
-        const float pn = p[i] - pp[i] / 100.0f;
-        p[i] = pn;
-        const float T = th[i] * powNear1(pn / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
-        const float Tv = T * (0.608f * qv[i] + 1);
-        rho[i] = pn * 100 / (kc::Rsd * Tv);
+        if (!inb || z0 >= z1) return;
+        float lp[ZPT], lpp[ZPT], t[ZPT], q[ZPT], g[ZPT];
+        int gh[ZPT];
+#pragma unroll
+        for (int k = 0; k < ZPT; k++) {
+            const int z = min(z0 + k, z1 - 1);
+            const long i = gidx(G, x, y, z);
+            lp[k] = p[i];
+            lpp[k] = pp[i];
+            t[k] = th[i];
+            q[k] = qv[i];
+            g[k] = gP[x + (long)G.X * z];
+            gh[k] = G.gh[x + G.X * z];
+        }
+#pragma unroll
+        for (int k = 0; k < ZPT; k++) {
+            const int z = z0 + k;
+            if (z >= z1 || y <= gh[k]) continue;
+            const long i = gidx(G, x, y, z);
+            const float pn = lp[k] - lpp[k] / 100.0f;
+            p[i] = pn;
+            const float T = t[k] * powNear1(pn / g[k], kc::Rsd / kc::Cpd);
+            const float Tv = T * (0.608f * q[k] + 1);
+            rho[i] = pn * 100 / (kc::Rsd * Tv);
+        }
     }
 };
 hipError_t launch_update_pressure_density(EnvState& e) {
-    return klaunch<KUpdatePressureDensity>(e, grid3(e), B3, 0, geom(e), e.f[SKT_PRESSURE], e.f[SKT_PRESSURE_PROJ],
+    return klaunch<KUpdatePressureDensity>(e, dim3(gridZpt(e)), B3, 0, geom(e), e.f[SKT_PRESSURE], e.f[SKT_PRESSURE_PROJ],
                                            e.f[SKT_DENSITY], e.f[SKT_POTTEMP], e.f[SKT_QV], e.f[SKT_GROUND_P]);
 }
 
@@ -1712,30 +1824,78 @@ struct KApplyProjection {
         const float gps = gP[gc];
         const int y0 = max(chunk * yc, gh + 1), y1 = min(Y, chunk * yc + yc);
         if (y0 >= y1) return;
+        // p' and density of the cell, carried with 1/density from the level
+        // below (correctly rounded reciprocals, the reference's bits). The
+        // loads of a group of DIV_G levels are issued together (at clamped
+        // in-range positions, so that they are unconditional) before the
+        // group is computed.
+        const long oR = inR ? 1 : 0, oF = inF ? XY : 0;
         float cp = pp[col + (long)X * y0], cd = rho[col + (long)X * y0];
-        for (int y = y0; y < y1; y++) {
-            const long i = col + (long)X * y;
-            const bool up = y + 1 <= Y - 1;  // the level above a sky cell is sky
-            const float ppU = up ? pp[i + X] : 0.0f, rhoU = up ? rho[i + X] : 0.0f;
-            const float pr = !inR ? bcv(bp.sides, cp, 0.0f) : (y <= ghR ? bcv(bp.ground, cp, 0.0f) : pp[i + 1]);
-            const float pu = !up ? bcv(bp.sides, cp, 0.0f) : ppU;
-            const float pf = !inF ? bcv(bp.sides, cp, 0.0f) : (y <= ghF ? bcv(bp.ground, cp, 0.0f) : pp[i + XY]);
-            float dr = !inR ? bcv(bd.sides, cd, 0.0f) : (y <= ghR ? bcv(bd.ground, cd, 0.0f) : rho[i + 1]);
-            float du = !up ? bcv(bd.sides, cd, 0.0f) : rhoU;
-            float df = !inF ? bcv(bd.sides, cd, 0.0f) : (y <= ghF ? bcv(bd.ground, cd, 0.0f) : rho[i + XY]);
-            dr = dr == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / dr);
-            du = du == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / du);
-            df = df == 0.0f ? cd : 2.0f / (1.0f / cd + 1.0f / df);
-            u[i] += (pr - cp) / dr;
-            v[i] += (pu - cp) / du;
-            w[i] += (pf - cp) / df;
-            if (newp) {
-                const float T = th[i] * powNear1(p[i] / gps, kc::Rsd / kc::Cpd);
-                const float Tv = T * (0.608f * qv[i] + 1);
-                p[i] = cd * kc::Rsd * Tv / 100.0f;
+        float icd = recipRN(cd);
+        for (int yb = y0; yb < y1; yb += DIV_G) {
+            float qpU[DIV_G], qdU[DIV_G], qpR[DIV_G], qpF[DIV_G], qdR[DIV_G], qdF[DIV_G];
+            float lu[DIV_G], lv[DIV_G], lw[DIV_G], qp[DIV_G], qt[DIV_G], qq[DIV_G];
+#pragma unroll
+            for (int k = 0; k < DIV_G; k++) {
+                const int y = min(yb + k, y1 - 1);
+                const long i = col + (long)X * y;
+                const long iu = y + 1 <= Y - 1 ? i + X : i;
+                qpU[k] = pp[iu];
+                qdU[k] = rho[iu];
+                qpR[k] = pp[i + oR];
+                qpF[k] = pp[i + oF];
+                qdR[k] = rho[i + oR];
+                qdF[k] = rho[i + oF];
+                lu[k] = u[i];
+                lv[k] = v[i];
+                lw[k] = w[i];
+                if (newp) {
+                    qp[k] = p[i];
+                    qt[k] = th[i];
+                    qq[k] = qv[i];
+                }
             }
-            cp = ppU;
-            cd = rhoU;
+#pragma unroll
+            for (int k = 0; k < DIV_G; k++) {
+                const int y = yb + k;
+                if (y >= y1) break;
+                const long i = col + (long)X * y;
+                const bool up = y + 1 <= Y - 1;  // the level above a sky cell is sky
+                const float ppU = up ? qpU[k] : 0.0f, rhoU = up ? qdU[k] : 0.0f;
+                const float pr = !inR ? bcv(bp.sides, cp, 0.0f) : (y <= ghR ? bcv(bp.ground, cp, 0.0f) : qpR[k]);
+                const float pu = !up ? bcv(bp.sides, cp, 0.0f) : ppU;
+                const float pf = !inF ? bcv(bp.sides, cp, 0.0f) : (y <= ghF ? bcv(bp.ground, cp, 0.0f) : qpF[k]);
+                float dr = !inR ? bcv(bd.sides, cd, 0.0f) : (y <= ghR ? bcv(bd.ground, cd, 0.0f) : qdR[k]);
+                float du = !up ? bcv(bd.sides, cd, 0.0f) : rhoU;
+                float df = !inF ? bcv(bd.sides, cd, 0.0f) : (y <= ghF ? bcv(bd.ground, cd, 0.0f) : qdF[k]);
+                // one range test for the cell (its own 1/density and every sky
+                // neighbour density it takes a reciprocal of), then the fast
+                // forms throughout
+                const bool fast = densOk(icd) && (!up || densOk(rhoU)) && (!(inR && y > ghR) || densOk(qdR[k])) &&
+                                  (!(inF && y > ghF) || densOk(qdF[k]));
+                auto harm = [&](auto fastTag) __attribute__((always_inline)) {
+                    constexpr bool F = decltype(fastTag)::value;
+                    auto rcp = [&](float d) { return F ? recipRNfast(d) : recipRN(d); };
+                    auto two = [&](float d) { return F ? 2.0f * recipRNfast(d) : twoOverRN(d); };
+                    const float iU = up ? rcp(rhoU) : 0.0f;
+                    dr = dr == 0.0f ? cd : two(icd + (inR ? rcp(dr) : icd));
+                    du = du == 0.0f ? cd : two(icd + (up ? iU : icd));
+                    df = df == 0.0f ? cd : two(icd + (inF ? rcp(df) : icd));
+                    return iU;
+                };
+                const float icU = fast ? harm(std::true_type{}) : harm(std::false_type{});
+                u[i] = lu[k] + (pr - cp) / dr;
+                v[i] = lv[k] + (pu - cp) / du;
+                w[i] = lw[k] + (pf - cp) / df;
+                if (newp) {
+                    const float T = qt[k] * powNear1(qp[k] / gps, kc::Rsd / kc::Cpd);
+                    const float Tv = T * (0.608f * qq[k] + 1);
+                    p[i] = cd * kc::Rsd * Tv / 100.0f;
+                }
+                cp = ppU;
+                cd = rhoU;
+                icd = icU;
+            }
         }
     }
 };

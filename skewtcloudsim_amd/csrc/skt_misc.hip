@@ -88,26 +88,54 @@ struct GroundArgs {
     Store sQr, sQs, sQi, sQc, sQw;  // storage of the hydrometeor fields
 };
 
-// One thread per ground column: the column-local part of the ground phase of
-// updateGPU (Game/environment.cu:427-455,469) in the reference's order.
+// The column-local part of the ground phase of updateGPU
+// (Game/environment.cu:427-455,469) in the reference's order. A block holds
+// GC_COLS ground columns. The cloud cover's column sum is the one step that
+// walks a whole column: the block's 256 threads load the levels of its
+// columns side by side (level y of column c by the thread of chunk
+// y mod GC_CHUNKS) and put each level's term (Qc + Qw) dx into LDS, GC_LV
+// levels at a time; the chunk-0 thread of each column adds the terms in
+// level order, as the reference's loop does. Everything after it is one
+// thread per column (the chunk-0 threads).
+constexpr int GC_COLS = 32, GC_CHUNKS = 8, GC_LV = 128;
+static_assert(GC_COLS * GC_CHUNKS == 256, "ground-column blocks hold 256 threads");
 template <bool MIX>
 struct KGroundColumns {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, const GroundArgs& a, int stages, float dt,
                                                float speed, float irrArg, float dx, const Gammas& gm) {
         const float irr = a.irrDev ? *a.irrDev : irrArg;
-        const long g = vb.id * 256L + threadIdx.x;
-        if (g >= (long)G.X * G.Z) return;
-        const int x = (int)(g % G.X), zc = (int)(g / G.X);
+        // thread (tx, ty) of the 64 x 4 block: column tx mod 32, chunk
+        // tx / 32 + 2 ty (the chunk-0 threads are lanes 0-31 of the first wave)
+        const int cl = (int)threadIdx.x & (GC_COLS - 1);
+        const int ch = ((int)threadIdx.x >> 5) + 2 * (int)threadIdx.y;
+        const long g = vb.id * (long)GC_COLS + cl;
+        const bool colOk = g < (long)G.X * G.Z;
+        const long gs = colOk ? g : 0;
+        const int x = (int)(gs % G.X), zc = (int)(gs / G.X);
         const float kap = kc::Rsd / kc::Cpd;
-        const int gh = G.gh[g];
+        const int gh = G.gh[gs];
         if (stages & 1) {  // calculateCloudCoverGPU, Game/kernelSky.cu:1034-1044
+            __shared__ float terms[GC_LV][GC_COLS];
             float tot = 0.0f;
-            for (int y = gh + 1; y < G.Y; y++) {
-                const long i = gidx(G, x, y, zc);
-                tot += (ldS<MIX>(a.Qc, i, a.sQc) + ldS<MIX>(a.Qw, i, a.sQw)) * dx;
+            for (int base = 0; base < G.Y; base += GC_LV) {  // uniform
+                const int top = min(G.Y, base + GC_LV);
+                if (colOk) {
+#pragma unroll 4
+                    for (int y = base + ch; y < top; y += GC_CHUNKS) {
+                        if (y > gh) {
+                            const long i = gidx(G, x, y, zc);
+                            terms[y - base][cl] = (ldS<MIX>(a.Qc, i, a.sQc) + ldS<MIX>(a.Qw, i, a.sQw)) * dx;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (colOk && ch == 0)
+                    for (int y = max(gh + 1, base); y < top; y++) tot += terms[y - base][cl];
+                __syncthreads();
             }
-            a.cover[g] = fminf(tot / 1.2f, 1.0f);
+            if (colOk && ch == 0) a.cover[g] = fminf(tot / 1.2f, 1.0f);
         }
+        if (!colOk || ch != 0) return;
         if (stages & 2) {  // calculateGroundTempGPU, Game/kernelSky.cu:1046-1059 (T^4 widened to double)
             const float gtv = a.gT[g];
             const double T4 = gtv * gtv * gtv * gtv;
@@ -216,11 +244,10 @@ hipError_t launch_ground_columns(EnvState& e, int stages, float dt, float speed,
     // a recorded ensemble step reads the member's irradiance of each step
     // from its device slot
     a.irrDev = e.rec ? e.rec->irr : nullptr;
+    const dim3 grid((unsigned)((e.G + GC_COLS - 1) / GC_COLS)), block(64, 4);
     if (e.mix)
-        return klaunch<KGroundColumns<true>>(e, grid1(e.G), dim3(256), 0, geom(e), a, stages, dt, speed, irr, e.dx,
-                                             e.gm);
-    return klaunch<KGroundColumns<false>>(e, grid1(e.G), dim3(256), 0, geom(e), a, stages, dt, speed, irr, e.dx,
-                                          e.gm);
+        return klaunch<KGroundColumns<true>>(e, grid, block, 0, geom(e), a, stages, dt, speed, irr, e.dx, e.gm);
+    return klaunch<KGroundColumns<false>>(e, grid, block, 0, geom(e), a, stages, dt, speed, irr, e.dx, e.gm);
 }
 
 // advectPrecipGPU, Game/kernelSky.cu:576-657. Each level reads its own and
@@ -353,7 +380,7 @@ hipError_t launch_precip_batch(EnvState& e, const PrecipBatch& pbt) {
 // its raw value when it is a sky cell and 0 otherwise; the level above a sky
 // cell inside the domain is sky. Blocks are 64 (x) x 4 (z) columns of one
 // chunk; a chunk that starts above the ground reads the level below it.
-constexpr int YC_BUOY = 16;
+constexpr int YC_BUOY = 8;
 template <bool MIX>
 struct KBuoyancy {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, float* __restrict__ v, const float* __restrict__ th,
@@ -377,6 +404,8 @@ struct KBuoyancy {
         // outside the domain: no level there is sky)
         const int ghL = x > 0 ? G.gh[gc - 1] : INT_MAX, ghR = x < X - 1 ? G.gh[gc + 1] : INT_MAX;
         const int ghB = z > 0 ? G.gh[gc - X] : INT_MAX, ghF = z < G.Z - 1 ? G.gh[gc + X] : INT_MAX;
+        const long oR = x < X - 1 ? 1 : 0, oL = x > 0 ? -1 : 0;
+        const long oF = z < G.Z - 1 ? (long)X * Y : 0, oB = z > 0 ? -(long)X * Y : 0;
         const int y0 = max(chunk * yc, gh + 1), y1 = min(Y, chunk * yc + yc);
         if (y0 >= y1) return;
         struct Lev {
@@ -406,15 +435,18 @@ struct KBuoyancy {
                 const float psU = up ? lp.p : psC;
                 const float psD = down ? lm.p : psC;
                 const float QP = lc.q;
-                const float Qenv = ((r ? qv[i + 1] : 0.0f) + (l ? qv[i - 1] : 0.0f) + (fw ? qv[i + (long)X * Y] : 0.0f) +
-                                    (bk ? qv[i - (long)X * Y] : 0.0f)) / validEnv;
+                // the four horizontal neighbours' vapour and theta, read at
+                // clamped in-domain positions (unconditional loads) and taken
+                // where the neighbour is sky
+                const float qR = qv[i + oR], qL = qv[i + oL], qF = qv[i + oF], qB = qv[i + oB];
+                const float tR = th[i + oR], tL = th[i + oL], tF = th[i + oF], tB = th[i + oB];
+                const float Qenv = ((r ? qR : 0.0f) + (l ? qL : 0.0f) + (fw ? qF : 0.0f) + (bk ? qB : 0.0f)) / validEnv;
                 const float QenvUp = up ? lp.q : 0.0f;
                 const float QenvDown = down ? lm.q : 0.0f;
                 const float thD = down ? lm.th : 0.0f, thU = up ? lp.th : 0.0f;
                 const float T = lc.T;
                 float Tad = lc.th, TadU = lc.th, TadD = lc.th;
-                const float Tenv = ((r ? th[i + 1] : 0.0f) + (l ? th[i - 1] : 0.0f) + (fw ? th[i + (long)X * Y] : 0.0f) +
-                                    (bk ? th[i - (long)X * Y] : 0.0f)) / validEnv;
+                const float Tenv = ((r ? tR : 0.0f) + (l ? tL : 0.0f) + (fw ? tF : 0.0f) + (bk ? tB : 0.0f)) / validEnv;
                 const float TenvUp = thU, TenvDown = thD;
                 if (down) {
                     // lower neighbour: TDown = thD (psD / gps)^kappa is its carried temperature
@@ -471,23 +503,43 @@ hipError_t launch_buoyancy(EnvState& e, float dt) {
 }
 
 // ------------------------------------------------------------------- EOS ---
-// initDensity, Game/kernelSky.cu:1432-1452 (+ the old-density copy)
+// initDensity, Game/kernelSky.cu:1432-1452 (+ the old-density copy). Each
+// thread takes ZPT cells along z; their loads are unconditional (a ground
+// cell's values are read and dropped), so they issue together.
 struct KInitDensity {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, float* __restrict__ rho, float* __restrict__ rho_old, const float* __restrict__ th,
                            const float* __restrict__ p, const float* __restrict__ qv, const float* __restrict__ gP) {
-        CELL_SETUP_V;
+        constexpr int zc = ZPT;
+        ZM_SETUP_V;
         (void)bid;
-        if (!inb || isGround(G, x, y, z)) return;
-        const long i = gidx(G, x, y, z);
-        const float T = th[i] * powNear1(p[i] / gP[x + (long)G.X * z], kc::Rsd / kc::Cpd);
-        const float Tv = T * (0.608f * qv[i] + 1);
-        const float d = p[i] * 100 / (kc::Rsd * Tv);
-        rho[i] = d;
-        if (rho_old) rho_old[i] = d;
+        if (!inb || z0 >= z1) return;
+        float t[ZPT], pr[ZPT], q[ZPT], g[ZPT];
+        int gh[ZPT];
+#pragma unroll
+        for (int k = 0; k < ZPT; k++) {
+            const int z = min(z0 + k, z1 - 1);
+            const long i = gidx(G, x, y, z);
+            t[k] = th[i];
+            pr[k] = p[i];
+            q[k] = qv[i];
+            g[k] = gP[x + (long)G.X * z];
+            gh[k] = G.gh[x + G.X * z];
+        }
+#pragma unroll
+        for (int k = 0; k < ZPT; k++) {
+            const int z = z0 + k;
+            if (z >= z1 || y <= gh[k]) continue;
+            const long i = gidx(G, x, y, z);
+            const float T = t[k] * powNear1(pr[k] / g[k], kc::Rsd / kc::Cpd);
+            const float Tv = T * (0.608f * q[k] + 1);
+            const float d = pr[k] * 100 / (kc::Rsd * Tv);
+            rho[i] = d;
+            if (rho_old) rho_old[i] = d;
+        }
     }
 };
 hipError_t launch_init_density(EnvState& e, int copy_old) {
-    return klaunch<KInitDensity>(e, grid3(e), B3, 0, geom(e), e.f[SKT_DENSITY],
+    return klaunch<KInitDensity>(e, dim3(gridZpt(e)), B3, 0, geom(e), e.f[SKT_DENSITY],
                                  copy_old ? e.f[SKT_DENSITY_OLD] : nullptr, e.f[SKT_POTTEMP], e.f[SKT_PRESSURE],
                                  e.f[SKT_QV], e.f[SKT_GROUND_P]);
 }
@@ -514,22 +566,27 @@ hipError_t launch_new_pressure(EnvState& e) {
 // cell is reset iff no cell of its column at or before its plane is ground:
 // y > ghz, the largest ground height over planes 0..z of its (x, z) column.
 // One thread per cell; the resets of different cells are independent.
+// (each thread takes ZPT cells along z)
 struct KResetVel {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, const int* __restrict__ ghz, float* __restrict__ u, float* __restrict__ v,
                         float* __restrict__ w, const float* __restrict__ defU, const float* __restrict__ defW) {
-        CELL_SETUP_V;
+        constexpr int zc = ZPT;
+        ZM_SETUP_V;
         (void)bid;
-        if (!inb || y <= ghz[x + (long)G.X * z]) return;
-        const long i = gidx(G, x, y, z);
-        if (y == G.Y - 1 || isGround(G, x, y + 1, z)) v[i] = 0.0f;
-        if (x == G.X - 1) u[i] = defU[y];
-        if (x < G.X - 1 && isGround(G, x + 1, y, z)) u[i] = 0.0f;
-        if (z == G.Z - 1) w[i] = defW[y];
-        if (z < G.Z - 1 && isGround(G, x, y, z + 1)) w[i] = 0.0f;
+        if (!inb) return;
+        for (int z = z0; z < z1; z++) {
+            if (y <= ghz[x + (long)G.X * z]) continue;
+            const long i = gidx(G, x, y, z);
+            if (y == G.Y - 1 || isGround(G, x, y + 1, z)) v[i] = 0.0f;
+            if (x == G.X - 1) u[i] = defU[y];
+            if (x < G.X - 1 && isGround(G, x + 1, y, z)) u[i] = 0.0f;
+            if (z == G.Z - 1) w[i] = defW[y];
+            if (z < G.Z - 1 && isGround(G, x, y, z + 1)) w[i] = 0.0f;
+        }
     }
 };
 hipError_t launch_reset_vel(EnvState& e) {
-    return klaunch<KResetVel>(e, grid3(e), B3, 0, geom(e), e.ghz, e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ],
+    return klaunch<KResetVel>(e, dim3(gridZpt(e)), B3, 0, geom(e), e.ghz, e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ],
                               e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ]);
 }
 

@@ -99,6 +99,12 @@ __device__ __forceinline__ int xcdRemap(int b, int nb) {
     const bool inb = (x < G.X) && (y < G.Y)
 
 static inline dim3 grid3(const EnvState& e) { return dim3((e.X + 63) / 64, (e.Y + 3) / 4, e.Z); }
+// Per-cell passes that give each thread ZPT cells along z (ZM_SETUP_V with
+// zc = ZPT): the loads of the thread's cells are issued together.
+constexpr int ZPT = 4;
+static inline unsigned gridZpt(const EnvState& e) {
+    return (unsigned)((e.X + 63) / 64) * (unsigned)((e.Y + 3) / 4) * (unsigned)((e.Z + ZPT - 1) / ZPT);
+}
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 static const dim3 B3(64, 4, 1);
 

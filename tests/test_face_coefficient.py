@@ -33,3 +33,32 @@ def test_face_coefficient_all_inputs():
     bad, first = ctypes.c_ulonglong(0), ctypes.c_uint32(0)
     assert L.face_check(0, 1 << 32, ctypes.byref(bad), ctypes.byref(first)) == 0
     assert bad.value == 0, f"{bad.value} inputs differ, first bit pattern {first.value:#010x}"
+
+
+
+def _rlib():
+    L = _lib()
+    L.recip_check.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_ulonglong),
+                              ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_ulonglong)]
+    L.recip_check.restype = ctypes.c_int
+    return L
+
+
+def test_recip_check_library_exports():
+    assert hasattr(_lib(), "recip_check")
+
+
+@pytest.mark.gpu
+def test_fast_reciprocal_all_inputs():
+    """skt::recipRNfast (hardware reciprocal, one FMA refinement), which the
+    preconditioner, divergence and projection passes take for 1/density and
+    the harmonic face densities, equals the IEEE 1.0f / s bit for bit for
+    every float s with |s| in [2^-126, 2^126]."""
+    L = _rlib()
+    bad, first, skipped = ctypes.c_ulonglong(0), ctypes.c_uint32(0), ctypes.c_ulonglong(0)
+    assert L.recip_check(0, 1 << 32, ctypes.byref(bad), ctypes.byref(first), ctypes.byref(skipped)) == 0
+    assert bad.value == 0, f"{bad.value} inputs differ, first bit pattern {first.value:#010x}"
+    # outside the range, per sign: zero and the subnormals (exponent field 0),
+    # (2^126, 2^127) (field 253 with a nonzero mantissa), field 254, and the
+    # infinities and NaNs (field 255)
+    assert skipped.value == 2 * (4 * (1 << 23) - 1), skipped.value
