@@ -369,7 +369,10 @@ int solveOnce(EnvState& e, int pass, int* slot_out) {
     return SKT_OK;
 }
 int solvePressure(EnvState& e, float dt) {
+    // the divergence pass also writes 1/density and the Jacobi preconditioner
+    // (initAMatrix + initPrecon), which the multigrid levels are built from
     TL(e, TC_OTHER, launch_divergence(e, dt));
+    if (e.mg_depth > 0) TL(e, TC_MG, launch_mg_setup(e));
     const int R = e.cfg.pcg_refine;
     if (R > 0) TL(e, TC_OTHER, launch_refine_keep_rhs(e));
     int slot = 1;
@@ -400,8 +403,6 @@ int solvePressure(EnvState& e, float dt) {
 // the same pass.
 int pressureProject(EnvState& e, float dt, int new_pressure) {
     TL(e, TC_OTHER, launch_reset_vel(e));
-    TL(e, TC_OTHER, launch_build_precon(e));
-    if (e.mg_depth > 0) TL(e, TC_MG, launch_mg_setup(e));
     int rc = solvePressure(e, dt);
     if (rc) return rc;
     TL(e, TC_OTHER, launch_update_pressure_density(e));
@@ -967,8 +968,6 @@ int skt_env_calculate_pressure_project(skt_env* env, float dt) {
     if (!env) return SKT_ERR_ARG;
     EnvState& e = env->s;
     HIPCHK(e, launch_reset_vel(e));
-    HIPCHK(e, launch_build_precon(e));
-    if (e.mg_depth > 0) HIPCHK(e, launch_mg_setup(e));
     return solvePressure(e, dt);
 }
 int skt_env_editor_data(skt_env* env) {

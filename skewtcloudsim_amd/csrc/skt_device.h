@@ -331,12 +331,15 @@ __device__ __forceinline__ float twoOverRN(float s) {
 // (checked exhaustively over all 2^32 inputs by
 // tests/test_face_coefficient.py); other s take the IEEE path.
 __device__ __forceinline__ bool faceFastOk(float s) { return s >= 0x1p-126f && s <= 0x1p126f; }
-__device__ __forceinline__ float faceFast(float s) {
-    const float e = __builtin_amdgcn_rcpf(s);
-    const float t = __builtin_fmaf(__builtin_fmaf(-s, e, 1.0f), e, e);
+// The second half of the short form, from t = RN(1/s) (recipRNfast of s).
+__device__ __forceinline__ float faceOfRecip(float t) {
     const float r = __builtin_amdgcn_rcpf(t);
     const float q = 0.6125f * r;
     return -fmaxf(1.0f, __builtin_fmaf(__builtin_fmaf(-t, q, 0.6125f), r, q));
+}
+__device__ __forceinline__ float faceFast(float s) {
+    const float e = __builtin_amdgcn_rcpf(s);
+    return faceOfRecip(__builtin_fmaf(__builtin_fmaf(-s, e, 1.0f), e, e));
 }
 __device__ __forceinline__ float faceOfSum(float s) {
     if (faceFastOk(s)) return faceFast(s);

@@ -344,8 +344,7 @@ hipError_t launch_new_pressure(EnvState& e);
 hipError_t launch_reset_vel(EnvState& e);
 // setToDefault (prof) or setToValue 0 (prof null) of one field
 hipError_t launch_reset_param(EnvState& e, int field, const float* prof);
-hipError_t launch_build_precon(EnvState& e);  // 1/density and 1/diagonal
-hipError_t launch_divergence(EnvState& e, float dt);  // r = div, partial max |r|
+hipError_t launch_divergence(EnvState& e, float dt);  // r = div, partial max |r|; irho and the preconditioner
 hipError_t launch_pcg_start(EnvState& e, int pass);    // stop tolerance; z = pre r, partial z.r (pass: refinement
                                                        // solve index, -1 without refinement)
 hipError_t launch_pcg_A(EnvState& e, int it);          // stop test / beta; s update, As, partial s.As

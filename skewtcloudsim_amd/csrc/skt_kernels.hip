@@ -843,120 +843,13 @@ static unsigned divBlocks(const EnvState& e) {
 // initAMatrix + initPrecon, Game/kernelSky.cu:1320-1430, density bounds
 // {N,D,D}. The operator is not stored: the SpMV rebuilds its coefficients
 // from 1/density (irho, 4 B per cell instead of 16 for a float4 operator).
-// This pass writes irho and the Jacobi preconditioner 1/diagonal.
+// The divergence pass below writes irho and the Jacobi preconditioner
+// 1/diagonal: it forms the same face sums for its harmonic densities.
 struct KSetInt {
     static __device__ __forceinline__ void run(VB vb, int* a, int v) {
         if (threadIdx.x == 0) *a = v;
     }
 };
-// A march up the columns in the chunks of the divergence march. Each sky
-// cell's diagonal is the SpMV's own (opRow's faces and summation order:
-// faceI of the two cells' 1/density, the cell's own value beyond the
-// domain, a zero face at ground), which is the reference's
-// -sum of -max(1, 1.225 / (2 / (1/rho + 1/rho_nb))) bit for bit; ground
-// cells take irho 0 and preconditioner 1. Reciprocals are the correctly
-// rounded recipRN; the level above's is carried into the next step as the
-// cell's own, and the own one into the step after as the level below's.
-// The loads of a group of PRE_G levels (the level above and the four
-// horizontal neighbours, at clamped in-range positions, so that they are
-// unconditional) are issued together before the group is computed.
-constexpr int PRE_G = 4;
-struct KBuildPrecon {
-    static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, const float* __restrict__ rho,
-                                               float* __restrict__ irho, float* __restrict__ pre,
-                                               int* __restrict__ faceExact) {
-        const unsigned lin = vb.id;
-        const int chunk = (int)(lin % nyc);
-        const unsigned tile = lin / nyc;
-        const int gx = (G.X + 63) >> 6;
-        const int x = (int)(tile % (unsigned)gx) * 64 + (int)threadIdx.x;
-        const int z = (int)(tile / (unsigned)gx) * 4 + (int)threadIdx.y;
-        if (x >= G.X || z >= G.Z) return;
-        const int X = G.X, Y = G.Y, Z = G.Z;
-        const long XY = (long)X * Y;
-        const long col = XY * z + x;  // cell (x, 0, z)
-        const long gc = x + (long)X * z;
-        const int gh = G.gh[gc];
-        const bool inL = x > 0, inR = x < X - 1, inB = z > 0, inF = z < Z - 1;
-        const int ghL = inL ? G.gh[gc - 1] : 0, ghR = inR ? G.gh[gc + 1] : 0;
-        const int ghB = inB ? G.gh[gc - X] : 0, ghF = inF ? G.gh[gc + X] : 0;
-        const long oR = inR ? 1 : 0, oL = inL ? -1 : 0, oF = inF ? XY : 0, oB = inB ? -XY : 0;
-        const int y0 = chunk * yc, y1 = min(Y, y0 + yc);
-        if (y0 >= y1) return;
-        const float INF = __builtin_inff();
-        float icDn = y0 > 0 ? recipRN(rho[col + (long)X * (y0 - 1)]) : 0.0f;  // level below
-        float ic = recipRN(rho[col + (long)X * y0]);
-        bool bad = false;
-        for (int yb = y0; yb < y1; yb += PRE_G) {
-            float rU[PRE_G], rR[PRE_G], rL[PRE_G], rF[PRE_G], rB[PRE_G];
-#pragma unroll
-            for (int k = 0; k < PRE_G; k++) {
-                const int y = min(yb + k, y1 - 1);
-                const long i = col + (long)X * y;
-                rU[k] = rho[y + 1 < Y ? i + X : i];
-                rR[k] = rho[i + oR];
-                rL[k] = rho[i + oL];
-                rF[k] = rho[i + oF];
-                rB[k] = rho[i + oB];
-            }
-#pragma unroll
-            for (int k = 0; k < PRE_G; k++) {
-                const int y = yb + k;
-                if (y >= y1) break;  // uniform: the chunk is the block's
-                const long i = col + (long)X * y;
-                const bool sU = y + 1 < Y;
-                if (y <= gh) {
-                    irho[i] = 0.0f;
-                    pre[i] = 1.0f;
-                    icDn = ic;
-                    ic = sU ? recipRN(rU[k]) : 0.0f;
-                    continue;
-                }
-                const bool sR = inR && y > ghR, sL = inL && y > ghL, sF = inF && y > ghF, sB = inB && y > ghB;
-                const bool sD = y > 0 && y - 1 > gh;
-                // one range test for the cell: its own 1/density and every
-                // sky neighbour density it takes a reciprocal of
-                const bool fast = densOk(ic) && (!sU || densOk(rU[k])) && (!sR || densOk(rR[k])) &&
-                                  (!sL || densOk(rL[k])) && (!sF || densOk(rF[k])) && (!sB || densOk(rB[k]));
-                auto cell = [&](auto fastTag) __attribute__((always_inline)) {
-                    constexpr bool F = decltype(fastTag)::value;
-                    auto rcp = [&](float d) { return F ? recipRNfast(d) : recipRN(d); };
-                    auto face = [&](float a, float b) { return F ? faceIfast(a, b) : faceI(a, b); };
-                    const float icU = sU ? rcp(rU[k]) : 0.0f;
-                    const float iR = !inR ? ic : (sR ? rcp(rR[k]) : INF);
-                    const float iL = !inL ? ic : (sL ? rcp(rL[k]) : INF);
-                    const float iU = sU ? icU : ic;
-                    const float iD = y == 0 ? ic : (sD ? icDn : INF);
-                    const float iF = !inF ? ic : (sF ? rcp(rF[k]) : INF);
-                    float w = 0.0f;
-                    w -= face(ic, iR);
-                    w -= face(ic, iL);
-                    w -= face(ic, iU);
-                    w -= face(ic, iD);
-                    w -= face(ic, iF);
-                    w -= z == 0 ? face(ic, ic) : (sB ? face(rcp(rB[k]), ic) : 0.0f);
-                    irho[i] = ic;
-                    pre[i] = w == 0.0f ? 1.0f : rcp(w);
-                    return icU;
-                };
-                const float icU = fast ? cell(std::true_type{}) : cell(std::false_type{});
-                // the SpMV's short face form is exact when every sky 1/density is in range
-                bad |= !faceValueOk(ic);
-                icDn = ic;
-                ic = icU;
-            }
-        }
-        if (bad) atomicOr(faceExact, 1);
-    }
-};
-hipError_t launch_build_precon(EnvState& e) {
-    hipError_t r = klaunch<KSetInt>(e, dim3(1), dim3(64), 0, &e.sc->face_exact, 0);
-    if (r != hipSuccess) return r;
-    const unsigned nyc = (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
-    return klaunch<KBuildPrecon>(e, dim3(divBlocks(e)), B3, 0, geom(e), YC_DIV, nyc, (const float*)e.f[SKT_DENSITY],
-                                 e.irho, e.pcg[5], &e.sc->face_exact);
-}
-
 // Scalar steps of the solve, run by every block of the consuming kernel
 // (fixed-order reductions, identical in every block); block 0 stores the
 // result. Steps follow calculatePressureProject (Game/environment.cu:930-985):
@@ -1075,7 +968,14 @@ __device__ void pcgPrologue(const double* __restrict__ pm, const double* __restr
 
 // calculateDivergenceGPU, Game/kernelSky.cu:780-860 (density {N,N,D}, velocity
 // {C,C,D}) with the per-block maxima of the initial stop-test value
-// (Game/environment.cu:933-936). p' is cleared.
+// (Game/environment.cu:933-936). p' is cleared. The pass also sets up the
+// pressure solve (initAMatrix + initPrecon, Game/kernelSky.cu:1320-1430,
+// density bounds {N,D,D}): the divergence's harmonic face densities 2 / s and
+// the operator's face coefficients -max(1, 1.225 / (2 / s)) share the sum
+// s = 1/rho + 1/rho_nb with the same neighbour rules (beyond the domain the
+// cell's own density, at ground a zero face; the top face takes the sides
+// condition in both), so it writes irho (1/density on sky cells, 0 in
+// ground) and the Jacobi preconditioner 1/diagonal, and flags faceExact.
 // A march up the columns: each thread walks a chunk of YC_DIV levels of its
 // (x, z) column. The ground heights of the column and of its four neighbour
 // columns are read once; the density, its reciprocal and v of the level
@@ -1094,7 +994,8 @@ struct KDivergence {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int yc, unsigned nyc, float* __restrict__ r, float* __restrict__ pp,
                          const float* __restrict__ rho, const float* __restrict__ rold, const float* __restrict__ u,
                          const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ defU,
-                         const float* __restrict__ defW, float dt, int absmode, double* part) {
+                         const float* __restrict__ defW, float dt, int absmode, double* part, float* __restrict__ irho,
+                         float* __restrict__ pre, int* __restrict__ faceExact) {
         __shared__ double sh[4];
         const unsigned lin = vb.id;
         const int chunk = (int)(lin % nyc);
@@ -1123,6 +1024,7 @@ struct KDivergence {
             }
             float cur = rho[col + (long)X * y0];
             float icur = recipRN(cur), icPrev = recipRN(rhoB);
+            bool bad = false;
             for (int yb = y0; yb < y1; yb += DIV_G) {
                 float qU[DIV_G], qR[DIV_G], qL[DIV_G], qF[DIV_G], qB[DIV_G];
                 float qv[DIV_G], qu[DIV_G], quL[DIV_G], qw[DIV_G], qwB[DIV_G], qo[DIV_G];
@@ -1163,7 +1065,6 @@ struct KDivergence {
                     auto cell = [&](auto fastTag) __attribute__((always_inline)) {
                         constexpr bool F = decltype(fastTag)::value;
                         auto rcp = [&](float d) { return F ? recipRNfast(d) : recipRN(d); };
-                        auto two = [&](float d) { return F ? 2.0f * recipRNfast(d) : twoOverRN(d); };
                         const float icU = topS ? 0.0f : rcp(rU);
                         if (y > gh) {
                             const float ur = qu[k];
@@ -1182,16 +1083,51 @@ struct KDivergence {
                             // one when it is sky (at y = 0 the cell's own; at ground
                             // dd is 0 and takes no reciprocal)
                             const float idd = skD ? icPrev : icur;
-                            dr = dr == 0.0f ? 0.0f : two(icur + (inR ? rcp(dr) : icur));
-                            dl = dl == 0.0f ? 0.0f : two(icur + (inL ? rcp(dl) : icur));
-                            du = du == 0.0f ? 0.0f : two(icur + (topS ? icur : icU));
-                            dd = dd == 0.0f ? 0.0f : two(icur + idd);
-                            df = df == 0.0f ? 0.0f : two(icur + (inF ? rcp(df) : icur));
-                            db = db == 0.0f ? 0.0f : two(icur + (inB ? rcp(db) : icur));
+                            // each face's harmonic density 2 / s, s = 1/rho + 1/rho_nb, and
+                            // the pressure operator's coefficient of the same face,
+                            // -max(1, 1.225 / (2 / s)) (initAMatrix's faceCoef; a zero
+                            // density, ground, gives 0 to both)
+                            float cR, cL, cU, cD, cF, cB;
+                            auto hf = [&](float& d, float in, float& c) {
+                                if (d == 0.0f) {
+                                    c = 0.0f;
+                                    return;
+                                }
+                                const float sum = icur + in;
+                                if constexpr (F) {
+                                    const float t = recipRNfast(sum);
+                                    d = 2.0f * t;
+                                    c = faceOfRecip(t);
+                                } else {
+                                    d = twoOverRN(sum);
+                                    c = -fmaxf(1.0f, 1.225f / d);
+                                }
+                            };
+                            hf(dr, inR ? rcp(dr) : icur, cR);
+                            hf(dl, inL ? rcp(dl) : icur, cL);
+                            hf(du, topS ? icur : icU, cU);
+                            hf(dd, idd, cD);
+                            hf(df, inF ? rcp(df) : icur, cF);
+                            hf(db, inB ? rcp(db) : icur, cB);
+                            // the Jacobi preconditioner: 1 / diagonal, the diagonal
+                            // minus the six coefficients in initAMatrix's order
+                            float wd = 0.0f;
+                            wd -= cR;
+                            wd -= cL;
+                            wd -= cU;
+                            wd -= cD;
+                            wd -= cF;
+                            wd -= cB;
+                            irho[i] = icur;
+                            pre[i] = wd == 0.0f ? 1.0f : rcp(wd);
+                            bad |= !faceValueOk(icur);
                             const float mfd = ((dr * ur - dl * ul) + (du * uu - dd * ud) + (df * uf - db * ub));
                             const float dc = (cur - qo[k]) / dt;
                             dv = (mfd - dc);
                             m = fmax(m, (double)(ab ? fabsf(dv) : dv));
+                        } else {
+                            irho[i] = 0.0f;
+                            pre[i] = 1.0f;
                         }
                         return icU;
                     };
@@ -1204,6 +1140,8 @@ struct KDivergence {
                     icur = icU;
                 }
             }
+            // the SpMV's short face form is exact when every sky 1/density is in range
+            if (bad) atomicOr(faceExact, 1);
         }
         m = blockMax(m, sh);
         if (tid256() == 0) part[vb.id] = m;
@@ -1213,10 +1151,12 @@ hipError_t launch_divergence(EnvState& e, float dt) {
     const int ref = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     const int absmode = ref ? e.cfg.maxabs_mode : 1;
     const unsigned nyc = (unsigned)((e.Y + YC_DIV - 1) / YC_DIV);
+    hipError_t rc = klaunch<KSetInt>(e, dim3(1), dim3(64), 0, &e.sc->face_exact, 0);
+    if (rc != hipSuccess) return rc;
     return klaunch<KDivergence>(e, dim3(divBlocks(e)), B3, 0, geom(e), YC_DIV, nyc, e.pcg[0], e.f[SKT_PRESSURE_PROJ],
                                 e.f[SKT_DENSITY], e.f[SKT_DENSITY_OLD], e.f[SKT_VELX], e.f[SKT_VELY], e.f[SKT_VELZ],
                                 e.f[SKT_DEFAULT_VELX], e.f[SKT_DEFAULT_VELZ], dt, absmode,
-                                e.partials + (size_t)PART_DIV * e.maxBlocks);
+                                e.partials + (size_t)PART_DIV * e.maxBlocks, e.irho, e.pcg[5], &e.sc->face_exact);
 }
 
 // Prologue PRO_START, then z = M^-1 r (applyPreconditionerGPU,
@@ -1275,7 +1215,7 @@ hipError_t launch_pcg_start(EnvState& e, int pass) {
 // (faceI gives 0), a position just outside the domain holds s = 0 and the
 // boundary cell's own 1/density (every thread reads its position clamped into
 // the domain). Faces take the short division (faceFast); it is exact when
-// each sky value lies in faceValueOk's range, which the preconditioner pass
+// each sky value lies in faceValueOk's range, which the divergence pass
 // checks for the whole solve (faceExact set: every face takes faceI).
 // s_new of the front plane, computed for this plane's front face, is the
 // next plane's own s_new.

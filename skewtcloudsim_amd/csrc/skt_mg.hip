@@ -36,7 +36,6 @@ namespace {
 
 constexpr float kOmega = 0.8f;     // Jacobi damping
 constexpr int kCoarseSweeps = 64;  // sweeps on the coarsest level (even: the result lands in xb)
-constexpr int kSweeps = 2;         // Jacobi sweeps before and after each coarse correction (1 or 2)
 
 // omega / d with the hardware reciprocal (about 1 ulp): the smoother's
 // diagonal only has to be a fixed positive diagonal for the cycle to stay
@@ -103,22 +102,7 @@ __device__ __forceinline__ MgCol mgCol(const VB& vb, const MgDims& D) {
     return c;
 }
 
-// (A x)_i at cell (x, y, z) of a level: d x_i - sum of couplings times
-// neighbours (a -x/-y/-z coupling is that neighbour's + coupling).
-__device__ __forceinline__ float applyAt(const MgDims& D, const MgC* __restrict__ C, const float* __restrict__ x,
-                                         int i, int xx, int yy, int zz, const float4& c) {
-    const int X = D.X, XY = D.X * D.Y;
-    float s = 0.0f;
-    if (xx + 1 < X) s += c.x * x[i + 1];
-    if (xx > 0) s += ldCx(C, i - 1) * x[i - 1];
-    if (yy + 1 < D.Y) s += c.y * x[i + X];
-    if (yy > 0) s += ldCy(C, i - X) * x[i - X];
-    if (zz + 1 < D.Z) s += c.z * x[i + XY];
-    if (zz > 0) s += ldCz(C, i - XY) * x[i - XY];
-    return c.w * x[i] - s;
-}
-
-// Level-0 operator from 1/density (after the preconditioner pass wrote irho).
+// Level-0 operator from 1/density (after the divergence pass wrote irho).
 struct KMgCoef0 {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, const float* __restrict__ irho,
                                                MgC* __restrict__ C) {
