@@ -440,26 +440,33 @@ struct KBuoyancy {
                 // where the neighbour is sky
                 const float qR = qv[i + oR], qL = qv[i + oL], qF = qv[i + oF], qB = qv[i + oB];
                 const float tR = th[i + oR], tL = th[i + oL], tF = th[i + oF], tB = th[i + oB];
-                const float Qenv = ((r ? qR : 0.0f) + (l ? qL : 0.0f) + (fw ? qF : 0.0f) + (bk ? qB : 0.0f)) / validEnv;
+                // the mean over the sky neighbours: dividing by 4, 2 or 1 is the
+                // exact scaling (the same rounding as the division)
+                auto envMean = [&](float sum) {
+                    if (validEnv == 3) return sum / 3.0f;
+                    return validEnv == 4 ? sum * 0.25f : (validEnv == 2 ? sum * 0.5f : sum);
+                };
+                const float Qenv = envMean((r ? qR : 0.0f) + (l ? qL : 0.0f) + (fw ? qF : 0.0f) + (bk ? qB : 0.0f));
                 const float QenvUp = up ? lp.q : 0.0f;
                 const float QenvDown = down ? lm.q : 0.0f;
                 const float thD = down ? lm.th : 0.0f, thU = up ? lp.th : 0.0f;
                 const float T = lc.T;
                 float Tad = lc.th, TadU = lc.th, TadD = lc.th;
-                const float Tenv = ((r ? tR : 0.0f) + (l ? tL : 0.0f) + (fw ? tF : 0.0f) + (bk ? tB : 0.0f)) / validEnv;
+                const float Tenv = envMean((r ? tR : 0.0f) + (l ? tL : 0.0f) + (fw ? tF : 0.0f) + (bk ? tB : 0.0f));
                 const float TenvUp = thU, TenvDown = thD;
-                if (down) {
-                    // lower neighbour: TDown = thD (psD / gps)^kappa is its carried temperature
-                    const float TDown = lm.T;
-                    if ((T < 0.0f && QP >= d_wi((TDown - 273.15f), psD)) || QP >= lm.ws) {
-                        TadD = T - d_MLR(T - 273.15f, psC) * (psC - psD);
+                // lower and upper neighbours: TDown = thD (psD / gps)^kappa and TUp
+                // are their carried temperatures; both adiabats take the moist
+                // lapse rate of this cell, computed once
+                const bool adD = down && ((T < 0.0f && QP >= d_wi((lm.T - 273.15f), psD)) || QP >= lm.ws);
+                const bool adU = up && ((T < 0.0f && QP >= d_wi((lp.T - 273.15f), psU)) || QP >= lp.ws);
+                if (adD || adU) {
+                    const float mlr = d_MLR(T - 273.15f, psC);
+                    if (adD) {
+                        TadD = T - mlr * (psC - psD);
                         TadD = TadD * powNear1(gps / psD, kap);
                     }
-                }
-                if (up) {
-                    const float TUp = lp.T;
-                    if ((T < 0.0f && QP >= d_wi((TUp - 273.15f), psU)) || QP >= lp.ws) {
-                        TadU = d_MLR(T - 273.15f, psC) * (psU - psC) + T;
+                    if (adU) {
+                        TadU = mlr * (psU - psC) + T;
                         TadU = TadU * powNear1(gps / psU, kap);
                     }
                 }
