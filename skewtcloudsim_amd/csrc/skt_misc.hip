@@ -660,7 +660,15 @@ struct KMicrophysics {
             const float PT1 = T < 0.0f && Qw + Qc > 0 + 1e-6f ? 1.0f : 0.0f;
             const float PT2 = T < 0.0f && Qr < 1e-4f && Qs < 1e-4f ? 1.0f : 0.0f;
             const float PT3 = T < 0.0f && Qr < 1e-4f ? 1.0f : 0.0f;
-            const float sR = d_slope(Dair, Qr, 0), sS = d_slope(Dair, Qs, 1), sI = d_slope(Dair, Qi, 2);
+            // Marshall-Palmer slopes. Every rate that takes the rain or ice
+            // slope is zero unless that species is present (Qr > 0 / Qi > 0),
+            // and the snow slope is taken without snow only by the deposition
+            // of the cold branch (PSDEP), so a slope is evaluated only where a
+            // rate can use it (the rates' values are unchanged)
+            float sR = 0.0f, sS = 0.0f, sI = 0.0f;
+            if (Qr > 0.0f) sR = d_slope(Dair, Qr, 0);
+            if (T < 0.0f || Qs > 0.0f) sS = d_slope(Dair, Qs, 1);
+            if (Qi > 0.0f) sI = d_slope(Dair, Qi, 2);
             const float dts = dt * speed;
             float PVCON = mp::PVCON(T, ps, Qv, QWS, dt, speed);
             float PVDEP = mp::PVDEP(T, ps, Qv, QWI, dt, speed);
