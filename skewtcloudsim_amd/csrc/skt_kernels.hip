@@ -1169,19 +1169,49 @@ struct KPcgStart {
         __shared__ double sh[4];
         __shared__ PcgState S;
         PcgPart P;
+        // the planes are read in groups of four at positions clamped into the
+        // domain (unconditional loads); the first group goes out before the
+        // prologue, each later one while the group before is used
+        constexpr int PG = 4;
+        const int xc = min(x, G.X - 1), yc = min(y, G.Y - 1);
+        float rv[PG], pv[PG];
+        int gv[PG];
+        auto loadGroup = [&](int zb) __attribute__((always_inline)) {
+#pragma unroll
+            for (int k = 0; k < PG; k++) {
+                const int z = min(zb + k, z1 - 1);
+                const long i = gidx(G, xc, yc, z);
+                rv[k] = r[i];
+                pv[k] = pre[i];
+                gv[k] = G.gh[xc + (long)G.X * z];
+            }
+        };
+        if (z0 < z1) loadGroup(z0);
         pcgPartLoad<PRO_START>(pin, nullptr, n, P);
         pcgPrologue<PRO_START>(pin, nullptr, n, P, sin, sout, ref, tol, 0, S, sh, vb.id, ro, pass, tolOut);
         if (S.done) return;
         double s = 0.0;
-        if (inb) {
-            for (int z = z0; z < z1; z++) {
-                const long i = gidx(G, x, y, z);
+        for (int zb = z0; zb < z1; zb += PG) {
+            float cr[PG], cp[PG];
+            int cg[PG];
+#pragma unroll
+            for (int k = 0; k < PG; k++) {
+                cr[k] = rv[k];
+                cp[k] = pv[k];
+                cg[k] = gv[k];
+            }
+            if (zb + PG < z1) loadGroup(zb + PG);
+#pragma unroll
+            for (int k = 0; k < PG; k++) {
+                const int z = zb + k;
+                if (z >= z1) break;
+                if (!inb) continue;
                 float zv = 0.0f;
-                if (!isGround(G, x, y, z)) {
-                    zv = r[i] * pre[i];
-                    s += (double)(zv * r[i]);
+                if (y > cg[k]) {
+                    zv = cr[k] * cp[k];
+                    s += (double)(zv * cr[k]);
                 }
-                zz[i] = zv;
+                zz[gidx(G, x, y, z)] = zv;
             }
         }
         s = blockSum(s, sh);

@@ -120,9 +120,13 @@ struct KGroundColumns {
             for (int base = 0; base < G.Y; base += GC_LV) {  // uniform
                 const int top = min(G.Y, base + GC_LV);
                 if (colOk) {
-#pragma unroll 4
-                    for (int y = base + ch; y < top; y += GC_CHUNKS) {
-                        if (y > gh) {
+                    // every level of the column is in memory: the loads are
+                    // unconditional, so a thread's are issued together, and
+                    // the sum below reads only the sky levels' terms
+#pragma unroll
+                    for (int k = 0; k < GC_LV / GC_CHUNKS; k++) {
+                        const int y = base + ch + k * GC_CHUNKS;
+                        if (y < top) {
                             const long i = gidx(G, x, y, zc);
                             terms[y - base][cl] = (ldS<MIX>(a.Qc, i, a.sQc) + ldS<MIX>(a.Qw, i, a.sQw)) * dx;
                         }
@@ -429,17 +433,19 @@ struct KBuoyancy {
             const long i = col + (long)X * y;
             const bool l = y > ghL, r = y > ghR, bk = y > ghB, fw = y > ghF;
             const int validEnv = int(l) + int(r) + int(bk) + int(fw);
-            if (validEnv > 0) {
-                const float cur = v[i];
+            // the cell's loads come first and unconditionally (the four
+            // horizontal neighbours' vapour and theta at clamped in-domain
+            // positions, taken below where the neighbour is sky); a cell with
+            // no sky neighbour (validEnv 0) computes a value it does not store
+            const float cur = v[i];
+            const float qR = qv[i + oR], qL = qv[i + oL], qF = qv[i + oF], qB = qv[i + oB];
+            const float tR = th[i + oR], tL = th[i + oL], tF = th[i + oF], tB = th[i + oB];
+            const float totalQ = ldS<MIX>(Qr, i, sq.s[0]) + ldS<MIX>(Qs, i, sq.s[1]) + ldS<MIX>(Qi, i, sq.s[2]);
+            {
                 const float psC = lc.p;
                 const float psU = up ? lp.p : psC;
                 const float psD = down ? lm.p : psC;
                 const float QP = lc.q;
-                // the four horizontal neighbours' vapour and theta, read at
-                // clamped in-domain positions (unconditional loads) and taken
-                // where the neighbour is sky
-                const float qR = qv[i + oR], qL = qv[i + oL], qF = qv[i + oF], qB = qv[i + oB];
-                const float tR = th[i + oR], tL = th[i + oL], tF = th[i + oF], tB = th[i + oB];
                 // the mean over the sky neighbours: dividing by 4, 2 or 1 is the
                 // exact scaling (the same rounding as the division)
                 auto envMean = [&](float sum) {
@@ -471,7 +477,6 @@ struct KBuoyancy {
                     }
                 }
                 float B = 0.0f, BU = 0.0f, BD = 0.0f;
-                const float totalQ = ldS<MIX>(Qr, i, sq.s[0]) + ldS<MIX>(Qs, i, sq.s[1]) + ldS<MIX>(Qi, i, sq.s[2]);
                 if (down) {
                     const float VTP = TadD * (1.0f + 0.608f * QP), VTE = TenvDown * (1.0f + 0.608f * QenvDown);
                     BD = kc::g * ((VTP - VTE) / VTE) - totalQ * kc::g;
@@ -489,7 +494,7 @@ struct KBuoyancy {
                 else if (cur < 0.0f) bf = BD;
                 float change = bf * dt;
                 if ((up && cur > 0.0f && -change > cur) || (down && cur < 0.0f && -change < cur)) change = -cur;
-                v[i] = cur + change;
+                if (validEnv > 0) v[i] = cur + change;
             }
             lm = lc;
             lc = lp;
