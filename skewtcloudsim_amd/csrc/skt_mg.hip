@@ -181,18 +181,24 @@ struct KMgSmooth {
         }
         float xm = t.z0 > 0 ? xin[i - XY] : 0.0f, czm = t.z0 > 0 ? ldCz(C, i - XY) : 0.0f;
         float xc = xin[i];
+        // neighbour offsets clamped into the level: every load of a plane is
+        // unconditional and issued before its arithmetic; a neighbour
+        // beyond the level (or a ground cell, w = 0) is dropped by a select
+        const bool hxR = t.x + 1 < X, hxL = t.x > 0, hyU = t.y + 1 < D.Y, hyD = t.y > 0;
+        const int oR = hxR ? 1 : 0, oL = hxL ? -1 : 0, oU = hyU ? X : 0, oD = hyD ? -X : 0;
         for (int z = t.z0; z < t.z1; z++, i += XY) {
             const float4 c = ldC(C, i);
-            const float xp = z + 1 < D.Z ? xin[i + XY] : 0.0f;
-            float v = 0.0f;
-            if (c.w > 0.0f) {
-                float s = czm * xm + c.z * xp;
-                if (t.x + 1 < X) s += c.x * xin[i + 1];
-                if (t.x > 0) s += ldCx(C, i - 1) * xin[i - 1];
-                if (t.y + 1 < D.Y) s += c.y * xin[i + X];
-                if (t.y > 0) s += ldCy(C, i - X) * xin[i - X];
-                v = xc + (r[i] - (c.w * xc - s)) * omegaOver(c.w);
-            }
+            const bool hzP = z + 1 < D.Z;
+            const float xpl = xin[hzP ? i + XY : i];
+            const float xr = xin[i + oR], xl = xin[i + oL], xu = xin[i + oU], xd = xin[i + oD];
+            const float cl = ldCx(C, i + oL), cd = ldCy(C, i + oD), rv = r[i];
+            const float xp = hzP ? xpl : 0.0f;
+            float s = czm * xm + c.z * xp;
+            s = hxR ? s + c.x * xr : s;
+            s = hxL ? s + cl * xl : s;
+            s = hyU ? s + c.y * xu : s;
+            s = hyD ? s + cd * xd : s;
+            const float v = c.w > 0.0f ? xc + (rv - (c.w * xc - s)) * omegaOver(c.w) : 0.0f;
             xout[i] = v;
             xm = xc;
             xc = xp;
@@ -225,28 +231,25 @@ struct KMgPreSmooth {
             cm = ldC(C, i - XY);
             am = a0(cm, r[i - XY]);
         }
+        // neighbour offsets clamped into the level: every load of a plane is
+        // unconditional and issued before its arithmetic; a neighbour
+        // beyond the level (or a ground cell, w = 0) is dropped by a select
+        const bool hxR = t.x + 1 < X, hxL = t.x > 0, hyU = t.y + 1 < D.Y, hyD = t.y > 0;
+        const int oR = hxR ? 1 : 0, oL = hxL ? -1 : 0, oU = hyU ? X : 0, oD = hyD ? -X : 0;
         for (int z = t.z0; z < t.z1; z++, i += XY) {
-            float4 cp{0.0f, 0.0f, 0.0f, 0.0f};
-            float ap = 0.0f;
-            if (z + 1 < D.Z) {
-                cp = ldC(C, i + XY);
-                ap = a0(cp, r[i + XY]);
-            }
-            float v = 0.0f;
-            if (cc.w > 0.0f) {
-                float s = cm.z * am + cc.z * ap;
-                if (t.x + 1 < X) s += cc.x * a0(ldC(C, i + 1), r[i + 1]);
-                if (t.x > 0) {
-                    const float4 l = ldC(C, i - 1);
-                    s += l.x * a0(l, r[i - 1]);
-                }
-                if (t.y + 1 < D.Y) s += cc.y * a0(ldC(C, i + X), r[i + X]);
-                if (t.y > 0) {
-                    const float4 d = ldC(C, i - X);
-                    s += d.y * a0(d, r[i - X]);
-                }
-                v = ac + (r[i] - (cc.w * ac - s)) * omegaOver(cc.w);
-            }
+            const bool hzP = z + 1 < D.Z;
+            const int ip = hzP ? i + XY : i;
+            const float4 cpl = ldC(C, ip), cR = ldC(C, i + oR), cL = ldC(C, i + oL), cU = ldC(C, i + oU),
+                         cD = ldC(C, i + oD);
+            const float rpl = r[ip], rR = r[i + oR], rL = r[i + oL], rU = r[i + oU], rD = r[i + oD], rv = r[i];
+            const float4 cp = hzP ? cpl : float4{0.0f, 0.0f, 0.0f, 0.0f};
+            const float ap = hzP ? a0(cp, rpl) : 0.0f;
+            float s = cm.z * am + cc.z * ap;
+            s = hxR ? s + cc.x * a0(cR, rR) : s;
+            s = hxL ? s + cL.x * a0(cL, rL) : s;
+            s = hyU ? s + cc.y * a0(cU, rU) : s;
+            s = hyD ? s + cD.y * a0(cD, rD) : s;
+            const float v = cc.w > 0.0f ? ac + (rv - (cc.w * ac - s)) * omegaOver(cc.w) : 0.0f;
             xout[i] = v;
             cm = cc;
             am = ac;
@@ -284,18 +287,25 @@ struct KMgProlongSmooth {
             czm = ldCz(C, i - XY);
         }
         float xe = xin[i] + corr(px, py, t.z0);
+        // neighbour offsets clamped into the level: every load of a plane is
+        // unconditional and issued before its arithmetic; a neighbour
+        // beyond the level (or a ground cell, w = 0) is dropped by a select
+        const bool hxR = t.x + 1 < X, hxL = t.x > 0, hyU = t.y + 1 < F.Y, hyD = t.y > 0;
+        const int oR = hxR ? 1 : 0, oL = hxL ? -1 : 0, oU = hyU ? X : 0, oD = hyD ? -X : 0;
         for (int z = t.z0; z < t.z1; z++, i += XY) {
             const float4 c = ldC(C, i);
-            const float xp = z + 1 < F.Z ? xin[i + XY] + corr(px, py, z + 1) : 0.0f;
-            float v = 0.0f;
-            if (c.w > 0.0f) {
-                float s = czm * xm + c.z * xp;
-                if (t.x + 1 < X) s += c.x * (xin[i + 1] + corr(pxr, py, z));
-                if (t.x > 0) s += ldCx(C, i - 1) * (xin[i - 1] + corr(pxl, py, z));
-                if (t.y + 1 < F.Y) s += c.y * (xin[i + X] + corr(px, pyu, z));
-                if (t.y > 0) s += ldCy(C, i - X) * (xin[i - X] + corr(px, pyd, z));
-                v = xe + (r[i] - (c.w * xe - s)) * omegaOver(c.w);
-            }
+            const bool hzP = z + 1 < F.Z;
+            const float xpl = xin[hzP ? i + XY : i] + corr(px, py, hzP ? z + 1 : z);
+            const float xR = xin[i + oR] + corr(pxr, py, z), xL = xin[i + oL] + corr(pxl, py, z);
+            const float xU = xin[i + oU] + corr(px, pyu, z), xD = xin[i + oD] + corr(px, pyd, z);
+            const float cl = ldCx(C, i + oL), cd = ldCy(C, i + oD), rv = r[i];
+            const float xp = hzP ? xpl : 0.0f;
+            float s = czm * xm + c.z * xp;
+            s = hxR ? s + c.x * xR : s;
+            s = hxL ? s + cl * xL : s;
+            s = hyU ? s + c.y * xU : s;
+            s = hyD ? s + cd * xD : s;
+            const float v = c.w > 0.0f ? xe + (rv - (c.w * xe - s)) * omegaOver(c.w) : 0.0f;
             xout[i] = v;
             xm = xe;
             xe = xp;
@@ -338,18 +348,25 @@ struct KMgRestrict {
         }
         const int cx = fx >> 1, cy = fy >> 1, DXY = D.X * D.Y;
         float acc = 0.0f;
+        // neighbour offsets clamped into the level: every load of a plane is
+        // unconditional and issued before its arithmetic; a neighbour
+        // beyond the level (or a ground cell, w = 0) is dropped by a select
+        const bool hxR = fx + 1 < X, hxL = fx > 0, hyU = fy + 1 < F.Y, hyD = fy > 0;
+        const int oR = hxR ? 1 : 0, oL = hxL ? -1 : 0, oU = hyU ? X : 0, oD = hyD ? -X : 0;
         for (int z = zs; z < ze; z++) {
             if (in) {
                 const float4 c = ldC(C, i);
-                const float xp = z + 1 < F.Z ? x[i + XY] : 0.0f;
-                if (c.w > 0.0f) {
-                    float s = czm * xm + c.z * xp;
-                    if (fx + 1 < X) s += c.x * x[i + 1];
-                    if (fx > 0) s += ldCx(C, i - 1) * x[i - 1];
-                    if (fy + 1 < F.Y) s += c.y * x[i + X];
-                    if (fy > 0) s += ldCy(C, i - X) * x[i - X];
-                    acc += r[i] - (c.w * xc - s);
-                }
+                const bool hzP = z + 1 < F.Z;
+                const float xpl = x[hzP ? i + XY : i];
+                const float xr = x[i + oR], xl = x[i + oL], xu = x[i + oU], xd = x[i + oD];
+                const float cl = ldCx(C, i + oL), cd = ldCy(C, i + oD), rv = r[i];
+                const float xp = hzP ? xpl : 0.0f;
+                float s = czm * xm + c.z * xp;
+                s = hxR ? s + c.x * xr : s;
+                s = hxL ? s + cl * xl : s;
+                s = hyU ? s + c.y * xu : s;
+                s = hyD ? s + cd * xd : s;
+                acc = c.w > 0.0f ? acc + (rv - (c.w * xc - s)) : acc;
                 xm = xc;
                 xc = xp;
                 czm = c.z;
