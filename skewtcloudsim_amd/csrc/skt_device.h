@@ -393,18 +393,23 @@ __device__ __forceinline__ OpRow opRow(const float* __restrict__ irho, long i, i
     o.sD = y > 0 && y - 1 > gh;
     o.sF = inF && y > ghF;
     o.sB = inB && y > ghB;
+    // the six neighbours are read at offsets clamped into the domain
+    // (unconditional loads, issued together) and replaced by the rules below
     const float ic = irho[i];
-    const float iR = !inR ? ic : (o.sR ? irho[i + 1] : INF);
-    const float iL = !inL ? ic : (o.sL ? irho[i - 1] : INF);
-    const float iU = o.sU ? irho[i + X] : ic;
-    const float iD = y == 0 ? ic : (o.sD ? irho[i - X] : INF);
-    const float iF = !inF ? ic : (o.sF ? irho[i + XY] : INF);
+    const float vR = irho[i + (inR ? 1 : 0)], vL = irho[i - (inL ? 1 : 0)];
+    const float vU = irho[i + (o.sU ? X : 0)], vD = irho[i - (y > 0 ? X : 0)];
+    const float vF = irho[i + (inF ? XY : 0)], vB = irho[i - (inB ? XY : 0)];
+    const float iR = !inR ? ic : (o.sR ? vR : INF);
+    const float iL = !inL ? ic : (o.sL ? vL : INF);
+    const float iU = o.sU ? vU : ic;
+    const float iD = y == 0 ? ic : (o.sD ? vD : INF);
+    const float iF = !inF ? ic : (o.sF ? vF : INF);
     o.fr = faceI(ic, iR);
     o.fl = faceI(ic, iL);
     o.fu = faceI(ic, iU);
     o.fd = faceI(ic, iD);
     o.ff = faceI(ic, iF);
-    o.fb = z == 0 ? faceI(ic, ic) : (o.sB ? faceI(irho[i - XY], ic) : 0.0f);
+    o.fb = z == 0 ? faceI(ic, ic) : (o.sB ? faceI(vB, ic) : 0.0f);
     float w = 0.0f;
     w -= o.fr;
     w -= o.fl;

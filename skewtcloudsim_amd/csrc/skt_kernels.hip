@@ -1675,13 +1675,18 @@ struct KRefineResidual {
                 double rv = 0.0;
                 if (y > gh) {
                     const OpRow o = opRow(irho, i, x, y, z, X, Y, Z, XY, gh, ghL, ghR, ghB, ghF);
+                    // neighbours at clamped offsets (unconditional loads),
+                    // added in the same order where they are sky
+                    const double xL = x64[i - (inL ? 1 : 0)], xD = x64[i - (y > 0 ? X : 0)];
+                    const double xB = x64[i - (inB ? XY : 0)], xR = x64[i + (inR ? 1 : 0)];
+                    const double xU = x64[i + (y + 1 < Y ? X : 0)], xF = x64[i + (inF ? XY : 0)];
                     double ax = (double)o.w * x64[i];
-                    if (o.sL) ax += (double)o.fl * x64[i - 1];
-                    if (o.sD) ax += (double)o.fd * x64[i - X];
-                    if (o.sB) ax += (double)o.fb * x64[i - XY];
-                    if (o.sR) ax += (double)o.fr * x64[i + 1];
-                    if (o.sU) ax += (double)o.fu * x64[i + X];
-                    if (o.sF) ax += (double)o.ff * x64[i + XY];
+                    ax = o.sL ? ax + (double)o.fl * xL : ax;
+                    ax = o.sD ? ax + (double)o.fd * xD : ax;
+                    ax = o.sB ? ax + (double)o.fb * xB : ax;
+                    ax = o.sR ? ax + (double)o.fr * xR : ax;
+                    ax = o.sU ? ax + (double)o.fu * xU : ax;
+                    ax = o.sF ? ax + (double)o.ff * xF : ax;
                     rv = (double)b[i] - ax;
                     m = fmax(m, fabs(rv));
                 }
