@@ -324,23 +324,20 @@ __device__ __forceinline__ float twoOverRN(float s) {
 // with both divisions rounded to
 // nearest. Scaling by 2 is exact, so 2 / s rounds to 2 RN(1/s) and the second
 // quotient equals RN(0.6125 / RN(1/s)) (0.6125f is 1.225f / 2 exactly). The
-// short form takes RN(1/s) from the hardware reciprocal estimate with one FMA
-// refinement, and the quotient from the estimate of 1/RN(1/s) with one
-// remainder correction (Markstein's iteration): seven instructions. For s in
-// [2^-126, 2^126] every float gives the same bits as the IEEE expression
-// (checked exhaustively over all 2^32 inputs by
-// tests/test_face_coefficient.py); other s take the IEEE path.
+// short form takes t = RN(1/s) from the hardware reciprocal estimate with one
+// FMA refinement, and the quotient from q = RN(0.6125 s) with one remainder
+// correction, q + (0.6125 - t q) s, in which s itself stands for 1/t
+// (Markstein's iteration; s and 1/t differ by one rounding): six instructions
+// and the max, one of them a reciprocal. For s in [2^-126, 2^126] every float
+// gives the same bits as the IEEE expression (checked exhaustively over all
+// 2^32 inputs by tests/test_face_coefficient.py); other s take the IEEE path.
 __device__ __forceinline__ bool faceFastOk(float s) { return s >= 0x1p-126f && s <= 0x1p126f; }
-// The second half of the short form, from t = RN(1/s) (recipRNfast of s).
-__device__ __forceinline__ float faceOfRecip(float t) {
-    const float r = __builtin_amdgcn_rcpf(t);
-    const float q = 0.6125f * r;
-    return -fmaxf(1.0f, __builtin_fmaf(__builtin_fmaf(-t, q, 0.6125f), r, q));
+// The second half of the short form, from s and t = RN(1/s) (recipRNfast of s).
+__device__ __forceinline__ float faceOfRecip(float t, float s) {
+    const float q = 0.6125f * s;
+    return -fmaxf(1.0f, __builtin_fmaf(__builtin_fmaf(-t, q, 0.6125f), s, q));
 }
-__device__ __forceinline__ float faceFast(float s) {
-    const float e = __builtin_amdgcn_rcpf(s);
-    return faceOfRecip(__builtin_fmaf(__builtin_fmaf(-s, e, 1.0f), e, e));
-}
+__device__ __forceinline__ float faceFast(float s) { return faceOfRecip(recipRNfast(s), s); }
 __device__ __forceinline__ float faceOfSum(float s) {
     if (faceFastOk(s)) return faceFast(s);
     return -fmaxf(1.0f, 1.225f / (2.0f / s));

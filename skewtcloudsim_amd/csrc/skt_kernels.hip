@@ -1097,7 +1097,7 @@ struct KDivergence {
                                 if constexpr (F) {
                                     const float t = recipRNfast(sum);
                                     d = 2.0f * t;
-                                    c = faceOfRecip(t);
+                                    c = faceOfRecip(t, sum);
                                 } else {
                                     d = twoOverRN(sum);
                                     c = -fmaxf(1.0f, 1.225f / d);
