@@ -9,7 +9,9 @@
 // replay), the report diagnostics of the jobs at a report boundary (one
 // kernel each on the ensemble's stream, values copied to pinned memory, one
 // wait for all), retirement. Between steps the host does no device waits
-// unless a report or a retirement is due, so the device stays fed.
+// unless a report or a retirement is due, so the device stays fed. A queued
+// job's initial state is generated on the host by a worker thread while it
+// waits (the first few jobs of the queue), so admitting it only uploads it.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <chrono>
@@ -17,6 +19,8 @@
 #include <cstddef>
 #include <cstring>
 #include <deque>
+#include <future>
+#include <memory>
 #include <string>
 #include <vector>
 #include "skt_internal.h"
@@ -27,7 +31,41 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// A job's initial state on the host (skt_sounding_generate), generated by a
+// worker thread. `done` is declared last, so destroying a Prep first waits
+// for its thread and only then frees the arrays the thread writes.
+struct Prep {
+    std::vector<float> th, vel, qv, gT, gP, p, pS;
+    std::future<int> done;
+};
+// Queued jobs whose initial states are generated ahead of admission (host
+// memory: 7 floats per cell each).
+constexpr int kPrefetch = 4;
+
+std::shared_ptr<Prep> startPrep(const skt_job_request& r) {
+    auto pp = std::make_shared<Prep>();
+    Prep* q = pp.get();
+    q->done = std::async(std::launch::async, [q, r]() -> int {
+        try {
+            const size_t N = (size_t)r.nx * r.ny * r.nz, G = (size_t)r.nx * r.nz;
+            q->th.resize(N);
+            q->vel.resize(3 * N);
+            q->qv.resize(N);
+            q->gT.resize(G);
+            q->gP.resize(G);
+            q->p.resize(N);
+            q->pS.resize((size_t)r.ny);
+        } catch (...) {
+            return SKT_ERR_NOMEM;
+        }
+        return skt_sounding_generate(&r.sounding, r.nx, r.ny, r.nz, r.dx, q->th.data(), q->vel.data(), q->qv.data(),
+                                     q->gT.data(), q->gP.data(), q->p.data(), q->pS.data());
+    });
+    return pp;
+}
+
 struct Job {
+    std::shared_ptr<Prep> prep;  // the initial state being generated, if started
     int64_t id = 0;
     skt_job_request req{};
     skt_config cfg{};
@@ -98,17 +136,30 @@ void pushEvent(skt_server* s, const Job& j, int kind, const DgPin* d) {
     s->events.push_back(ev);
 }
 
-// Builds the job's environment (the sounding generated on the host, then
-// init) and adds it to the ensemble.
+// Builds the job's environment (its initial state, generated on the host by
+// a worker thread or here, then init) and adds it to the ensemble. The host
+// arrays are freed once uploaded.
 int admit(skt_server* s, Job& j) {
-    int rc = skt_env_create(&j.cfg, &j.env);
-    if (rc) return serr(s, rc, "creating a job's environment");
-    const skt_job_request& r = j.req;
-    const size_t N = (size_t)r.nx * r.ny * r.nz, G = (size_t)r.nx * r.nz;
-    std::vector<float> th(N), vel(3 * N), qv(N), gT(G), gP(G), p(N), pS((size_t)r.ny);
-    rc = skt_sounding_generate(&r.sounding, r.nx, r.ny, r.nz, r.dx, th.data(), vel.data(), qv.data(), gT.data(),
-                               gP.data(), p.data(), pS.data());
-    if (!rc) rc = skt_env_init(j.env, th.data(), vel.data(), qv.data(), gT.data(), gP.data(), p.data(), pS.data());
+    int rc;
+    try {
+        if (!j.prep) j.prep = startPrep(j.req);
+        rc = j.prep->done.get();
+    } catch (...) {  // no thread or no memory for one
+        rc = SKT_ERR_NOMEM;
+    }
+    if (rc) {
+        j.prep.reset();
+        return serr(s, rc, "generating a job's initial state");
+    }
+    rc = skt_env_create(&j.cfg, &j.env);
+    if (rc) {
+        j.prep.reset();
+        return serr(s, rc, "creating a job's environment");
+    }
+    Prep& q = *j.prep;
+    rc = skt_env_init(j.env, q.th.data(), q.vel.data(), q.qv.data(), q.gT.data(), q.gP.data(), q.p.data(),
+                      q.pS.data());
+    j.prep.reset();
     if (!rc) rc = skt_batch_add(s->batch, j.env, &j.member);
     if (rc) {
         const char* m = skt_env_last_error(j.env);
@@ -119,6 +170,21 @@ int admit(skt_server* s, Job& j) {
         return serr(s, rc, what);
     }
     return SKT_OK;
+}
+
+// Starts the initial states of the first kPrefetch queued jobs. A job that
+// first fit admits from further back is generated at admission.
+void prefetch(skt_server* s) {
+    int k = 0;
+    for (Job& j : s->queue) {
+        if (k++ >= kPrefetch) break;
+        if (j.prep) continue;
+        try {
+            j.prep = startPrep(j.req);
+        } catch (...) {  // no thread now: admission generates it
+            return;
+        }
+    }
 }
 
 }  // namespace
@@ -207,6 +273,7 @@ int skt_server_submit(skt_server* s, const skt_job_request* r, int64_t* job_id) 
     j.submitted = now(s);
     if (job_id) *job_id = j.id;
     s->queue.push_back(j);
+    prefetch(s);
     return SKT_OK;
 }
 
@@ -232,6 +299,7 @@ int skt_server_step(skt_server* s) {
         s->active.push_back(j);
         it = s->queue.erase(it);
     }
+    prefetch(s);
     if (s->active.empty()) {
         s->busy += std::chrono::duration<double>(Clock::now() - tb).count();
         return SKT_OK;

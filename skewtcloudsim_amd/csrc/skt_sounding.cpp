@@ -4,9 +4,11 @@
 // surface pressure (Game/readTable.cpp:402, theta = T (p_s/p)^(Rd/cp)),
 // horizontally uniform wind on MAC faces (:442-460), vapour mixing ratio,
 // pressure per cell and per level, ground temperature and pressure.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 #include "../../include/skt.h"
 
@@ -120,27 +122,42 @@ int skt_sounding_generate(const skt_sounding_spec* s, int32_t nx, int32_t ny, in
         ul[y] = s->shear_u * zs;
         pS[y] = (float)plev[y];
     }
-    for (int z = 0; z < nz; z++)
-        for (int y = 0; y < ny; y++)
-            for (int x = 0; x < nx; x++) {
-                const long i = x + (long)nx * (y + (long)ny * z);
-                double t = thl[y];
-                for (int b = 0; b < s->n_bubbles && b < 8; b++) {
-                    const double ddx = x * (double)dx - s->bubble_x[b], ddy = y * (double)dx - s->bubble_y[b],
-                                 ddz = z * (double)dx - s->bubble_z[b];
-                    const double r = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-                    if (r < s->bubble_radius[b]) {
-                        const double c = std::cos(0.5 * M_PI * r / s->bubble_radius[b]);
-                        t += s->bubble_amp[b] * c * c;
+    // every cell is a function of its own position: planes are filled by up
+    // to 16 threads, each a contiguous range of z
+    auto planes = [&](int za, int zb) {
+        for (int z = za; z < zb; z++)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) {
+                    const long i = x + (long)nx * (y + (long)ny * z);
+                    double t = thl[y];
+                    for (int b = 0; b < s->n_bubbles && b < 8; b++) {
+                        const double ddx = x * (double)dx - s->bubble_x[b], ddy = y * (double)dx - s->bubble_y[b],
+                                     ddz = z * (double)dx - s->bubble_z[b];
+                        const double r = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                        if (r < s->bubble_radius[b]) {
+                            const double c = std::cos(0.5 * M_PI * r / s->bubble_radius[b]);
+                            t += s->bubble_amp[b] * c * c;
+                        }
                     }
+                    th[i] = (float)t;
+                    qv[i] = (float)ql[y];
+                    p[i] = (float)plev[y];
+                    vel[3 * i + 0] = (float)ul[y];
+                    vel[3 * i + 1] = 0.0f;
+                    vel[3 * i + 2] = 0.0f;
                 }
-                th[i] = (float)t;
-                qv[i] = (float)ql[y];
-                p[i] = (float)plev[y];
-                vel[3 * i + 0] = (float)ul[y];
-                vel[3 * i + 1] = 0.0f;
-                vel[3 * i + 2] = 0.0f;
-            }
+    };
+    const long cells = (long)nx * ny * nz;
+    const int nt = (int)std::min<long>({16L, (long)std::max(1u, std::thread::hardware_concurrency()), (long)nz,
+                                        std::max(1L, cells / (1L << 18))});
+    if (nt <= 1) {
+        planes(0, nz);
+    } else {
+        std::vector<std::thread> pool;
+        pool.reserve((size_t)nt);
+        for (int k = 0; k < nt; k++) pool.emplace_back(planes, (int)((long)nz * k / nt), (int)((long)nz * (k + 1) / nt));
+        for (std::thread& t : pool) t.join();
+    }
     for (long g = 0; g < (long)nx * nz; g++) {
         gT[g] = (float)pr.T(0.0);
         gP[g] = (float)ps;
