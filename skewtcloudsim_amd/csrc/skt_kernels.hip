@@ -454,11 +454,17 @@ hipError_t launch_maxvel(EnvState& e, float dt) {
 // face velocity reads through the advected field's bounds (:415,486,559).
 // q[0..4] / o[0..4]: the field and the unit field at offsets -2..2 along the
 // pass direction; vc, vm: the face velocities at +1/2 and -1/2.
+// The upwind side is chosen by selecting the stencil operands of a single
+// ppmFlux call (the same function of the same values as the reference's two
+// calls), so a wave whose face velocities differ in sign evaluates one flux,
+// not both.
 __device__ __forceinline__ float ppmFaceR(const float* q, float vc, float dt, float dx) {
-    return vc >= 0.0f ? ppmFlux(vc, q[1], q[2], q[3], dt, dx) : ppmFlux(vc, q[4], q[3], q[2], dt, dx);
+    const bool up = vc >= 0.0f;
+    return ppmFlux(vc, up ? q[1] : q[4], up ? q[2] : q[3], up ? q[3] : q[2], dt, dx);
 }
 __device__ __forceinline__ float ppmFaceL(const float* q, float vm, float dt, float dx) {
-    return vm < 0.0f ? ppmFlux(vm, q[3], q[2], q[1], dt, dx) : ppmFlux(vm, q[0], q[1], q[2], dt, dx);
+    const bool dn = vm < 0.0f;
+    return ppmFlux(vm, dn ? q[3] : q[0], dn ? q[2] : q[1], dn ? q[1] : q[2], dt, dx);
 }
 // The cell update from its two face fluxes of the field (Fr, Fl).
 __device__ __forceinline__ float ppmFinish(const float* q, const float* o, float vc, float vm, float Fr, float Fl,
