@@ -449,10 +449,11 @@ int skt_env_init_sounding(skt_env* env, const skt_sounding_table* table);
  * No counterpart in the reference (an interactive single-domain program).
  * A server owns an ensemble (skt_batch). skt_server_submit queues a job;
  * each skt_server_step admits queued jobs in arrival order while their HBM
- * footprint (skt_env_footprint) fits the budget and a slot is free, builds
- * each admitted job's initial state on the device (skt_sounding_generate +
- * init), advances every active job by one ensemble step, reduces the
- * diagnostics of the jobs whose simulated time reached a report boundary
+ * footprint (skt_env_footprint) fits the budget and a slot is free, uploads
+ * each admitted job's initial state (skt_sounding_generate on the host,
+ * started by a worker thread for the first four queued jobs from their
+ * submission on; then init), advances every active job by one ensemble step,
+ * reduces the diagnostics of the jobs whose simulated time reached a report boundary
  * (skt_env_diagnostics; one device wait for all of them), and retires jobs
  * that have run their T seconds, freeing their memory for the queue. Every
  * job runs the step of its own configuration (the server's member_config
