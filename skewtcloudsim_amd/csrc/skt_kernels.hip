@@ -363,10 +363,28 @@ struct KDiffuse {
         }
     }
 };
+// A batch with a field stored in 16 bits needs the kernels' 16-bit form; a
+// batch of FP32 fields (in the mixed form: every field but the hydrometeors,
+// and the hydrometeors' sweeps between their first and last) runs the FP32
+// form, whose loads and stores are the same for FP32 storage without the
+// format tests and conversions.
+static bool has16(const DiffBatch& db) {
+    for (int j = 0; j < db.n; j++)
+        if (db.f[j].sin.fmt != FMT_F32 || db.f[j].sout.fmt != FMT_F32) return true;
+    return false;
+}
+static bool has16(const PpmBatch& pb) {
+    for (int j = 0; j < pb.n; j++)
+        if (pb.f[j].sin.fmt != FMT_F32 || pb.f[j].sout.fmt != FMT_F32 ||
+            (pb.f[j].gsrc && pb.f[j].sg.fmt != FMT_F32))
+            return true;
+    return false;
+}
+
 hipError_t launch_diffuse_batch(EnvState& e, const DiffBatch& db, float k) {
     const ZGrid zg = e.rec ? zgridChunks(e, ensembleChunks(e), 8) : e.gD;
     if ((double)(zg.zc + 3) * (double)e.X * (double)e.Y * 4.0 >= 4294967296.0) return hipErrorInvalidValue;
-    if (e.mix)
+    if (e.mix && has16(db))
         return klaunch<KDiffuse<true>>(e, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k,
                                        e.f[SKT_GROUND_T], e.f[SKT_GROUND_P], e.f[SKT_PRESSURE]);
     return klaunch<KDiffuse<false>>(e, dim3(zg.nblocks * db.n), B3, 0, geom(e), zg.zc, zg.nblocks, db, k,
@@ -812,13 +830,14 @@ struct KPpmZ {
 hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) {
     const ZGrid zg = e.rec ? zgridChunks(e, ensembleChunks(e)) : zgrid(e);
     const dim3 grid(zg.nblocks * pb.n);
+    const bool m16 = e.mix && has16(pb);
     if (dir == 0) {
         // rows of four per block, chunks of zc planes: about 2048 blocks per field
         const long gy = (e.Y + 3) / 4;
         const long nzc = e.rec ? ensembleChunks(e) : std::max(1L, std::min((long)e.Z, 2048 / gy));
         const int zc = (int)((e.Z + nzc - 1) / nzc);
         const unsigned nbf = (unsigned)(gy * ((e.Z + zc - 1) / zc));
-        return e.mix ? klaunch<KPpmXr<true>>(e, dim3(nbf * pb.n), B3, 0, geom(e), zc, nbf, pb, dt, e.dx,
+        return m16 ? klaunch<KPpmXr<true>>(e, dim3(nbf * pb.n), B3, 0, geom(e), zc, nbf, pb, dt, e.dx,
                                              e.f[SKT_DEFAULT_VELX])
                      : klaunch<KPpmXr<false>>(e, dim3(nbf * pb.n), B3, 0, geom(e), zc, nbf, pb, dt, e.dx,
                                               e.f[SKT_DEFAULT_VELX]);
@@ -826,10 +845,10 @@ hipError_t launch_ppm_batch(EnvState& e, const PpmBatch& pb, int dir, float dt) 
     if (dir == 1) {
         const unsigned nyc = (unsigned)((e.Y + YC_PPM - 1) / YC_PPM);
         const unsigned nbf = (unsigned)((e.X + 63) / 64) * (unsigned)((e.Z + 3) / 4) * nyc;
-        return e.mix ? klaunch<KPpmYm<true>>(e, dim3(nbf * pb.n), B3, 0, geom(e), YC_PPM, nyc, nbf, pb, dt, e.dx)
+        return m16 ? klaunch<KPpmYm<true>>(e, dim3(nbf * pb.n), B3, 0, geom(e), YC_PPM, nyc, nbf, pb, dt, e.dx)
                      : klaunch<KPpmYm<false>>(e, dim3(nbf * pb.n), B3, 0, geom(e), YC_PPM, nyc, nbf, pb, dt, e.dx);
     }
-    return e.mix ? klaunch<KPpmZ<ZB_PPM, true>>(e, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
+    return m16 ? klaunch<KPpmZ<ZB_PPM, true>>(e, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
                                                 e.f[SKT_DEFAULT_VELZ])
                  : klaunch<KPpmZ<ZB_PPM, false>>(e, grid, B3, 0, geom(e), zg.zc, zg.nblocks, pb, dt, e.dx,
                                                  e.f[SKT_DEFAULT_VELZ]);
