@@ -102,6 +102,37 @@ __device__ __forceinline__ MgCol mgCol(const VB& vb, const MgDims& D) {
     return c;
 }
 
+// The LDS tile of a plane for the level kernels whose stencil reads a value
+// at the four in-plane neighbours: 64 x 4 cells of the block and a one-cell
+// halo. Each lane publishes its own cell's value; the halo is read by the row
+// below the tile (wave 0), the row above it (wave 1) and the columns left and
+// right of it (lanes 0-3 and 4-7 of wave 2). Lanes past the level work on the
+// cell clamped into it (their neighbours drop them by a select), take part
+// in every barrier and store nothing.
+struct MgTile {
+    int ib, hb;            // the lane's cell and halo cell, index within a plane (clamped)
+    int hx, hy;            // the halo cell's coordinates (clamped)
+    int hrow, hcol;
+    bool hasH, in;
+};
+__device__ __forceinline__ MgTile mgTile(const MgCol& t, int X, int Y) {
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+    const int x0 = t.x - tx, y0 = t.y - ty;
+    MgTile g;
+    int hx = t.x, hy = y0 - 1;
+    g.hrow = 0;
+    g.hcol = tx + 1;
+    if (ty == 1) hy = y0 + 4, g.hrow = 5;
+    if (ty == 2) hx = tx < 4 ? x0 - 1 : x0 + 64, hy = y0 + (tx & 3), g.hrow = 1 + (tx & 3), g.hcol = tx < 4 ? 0 : 65;
+    g.hasH = ty < 2 || (ty == 2 && tx < 8);
+    g.in = t.x < X && t.y < Y;
+    g.ib = min(t.x, X - 1) + X * min(t.y, Y - 1);
+    g.hx = g.hasH ? min(max(hx, 0), X - 1) : min(t.x, X - 1);
+    g.hy = g.hasH ? min(max(hy, 0), Y - 1) : min(t.y, Y - 1);
+    g.hb = g.hx + X * g.hy;
+    return g;
+}
+
 // Level-0 operator from 1/density (after the divergence pass wrote irho).
 struct KMgCoef0 {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, const float* __restrict__ irho,
@@ -169,29 +200,40 @@ struct KMgSmooth {
                                                float* __restrict__ xout, const PcgState* st, int first) {
         if (st->done) return;
         const MgCol t = mgCol(vb, D);
-        if (!t.in) return;
         const int X = D.X, XY = D.X * D.Y;
-        int i = t.x + X * t.y + XY * t.z0;
         if (first) {
+            if (!t.in) return;
+            int i = t.x + X * t.y + XY * t.z0;
             for (int z = t.z0; z < t.z1; z++, i += XY) {
                 const float w = ldCw(C, i);
                 xout[i] = w > 0.0f ? r[i] * omegaOver(w) : 0.0f;
             }
             return;
         }
+        if (t.z0 >= t.z1) return;  // the whole block
+        // x of the in-plane neighbours from the plane's LDS tile
+        __shared__ float sX[2][6][66];
+        const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+        const MgTile g = mgTile(t, X, D.Y);
+        int i = g.ib + XY * t.z0;
         float xm = t.z0 > 0 ? xin[i - XY] : 0.0f, czm = t.z0 > 0 ? ldCz(C, i - XY) : 0.0f;
-        float xc = xin[i];
-        // neighbour offsets clamped into the level: every load of a plane is
-        // unconditional and issued before its arithmetic; a neighbour
-        // beyond the level (or a ground cell, w = 0) is dropped by a select
+        float xc = xin[i], hv = xin[g.hb + XY * t.z0];
+        // a neighbour beyond the level (or a ground cell, w = 0) is dropped
+        // by a select; couplings are read at offsets clamped into the level
         const bool hxR = t.x + 1 < X, hxL = t.x > 0, hyU = t.y + 1 < D.Y, hyD = t.y > 0;
-        const int oR = hxR ? 1 : 0, oL = hxL ? -1 : 0, oU = hyU ? X : 0, oD = hyD ? -X : 0;
+        const int oL = hxL ? -1 : 0, oD = hyD ? -X : 0;
         for (int z = t.z0; z < t.z1; z++, i += XY) {
+            const int buf = (z - t.z0) & 1;
             const float4 c = ldC(C, i);
             const bool hzP = z + 1 < D.Z;
             const float xpl = xin[hzP ? i + XY : i];
-            const float xr = xin[i + oR], xl = xin[i + oL], xu = xin[i + oU], xd = xin[i + oD];
             const float cl = ldCx(C, i + oL), cd = ldCy(C, i + oD), rv = r[i];
+            const float hvn = xin[g.hb + XY * min(z + 1, D.Z - 1)];
+            sX[buf][ty + 1][tx + 1] = xc;
+            if (g.hasH) sX[buf][g.hrow][g.hcol] = hv;
+            __syncthreads();
+            const float xr = sX[buf][ty + 1][tx + 2], xl = sX[buf][ty + 1][tx];
+            const float xu = sX[buf][ty + 2][tx + 1], xd = sX[buf][ty][tx + 1];
             const float xp = hzP ? xpl : 0.0f;
             float s = czm * xm + c.z * xp;
             s = hxR ? s + c.x * xr : s;
@@ -199,18 +241,18 @@ struct KMgSmooth {
             s = hyU ? s + c.y * xu : s;
             s = hyD ? s + cd * xd : s;
             const float v = c.w > 0.0f ? xc + (rv - (c.w * xc - s)) * omegaOver(c.w) : 0.0f;
-            xout[i] = v;
+            if (g.in) xout[i] = v;
             xm = xc;
             xc = xp;
             czm = c.z;
+            hv = hvn;
         }
     }
 };
 
 // The two pre-smoothing sweeps in one pass: the first sweep from zero gives
 // a_j = omega r_j / d_j at every cell, which the second sweep's stencil
-// reads at the cell and its six neighbours, so it is formed where it is
-// needed: xout = a + omega (r - A a) / d.
+// reads at the cell and its six neighbours: xout = a + omega (r - A a) / d.
 struct KMgPreSmooth {
     template <class... T>
     static __device__ __forceinline__ bool idle(const T&... a) {
@@ -221,40 +263,57 @@ struct KMgPreSmooth {
                                                const PcgState* st) {
         if (st->done) return;
         const MgCol t = mgCol(vb, D);
-        if (!t.in) return;
-        const int X = D.X, XY = D.X * D.Y;
-        auto a0 = [&](const float4& c, float rv) { return c.w > 0.0f ? rv * omegaOver(c.w) : 0.0f; };
-        int i = t.x + X * t.y + XY * t.z0;
+        if (t.z0 >= t.z1) return;  // the whole block
+        // Each cell forms its own a (the first sweep's value) once per plane
+        // and publishes it in the plane's LDS tile (two buffers, one barrier
+        // per plane); its four in-plane neighbours read it there. Along z, a
+        // stays in registers.
+        __shared__ float sA[2][6][66];
+        const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+        const int X = D.X, Y = D.Y, XY = D.X * D.Y;
+        auto a0 = [&](float w, float rv) { return w > 0.0f ? rv * omegaOver(w) : 0.0f; };
+        const MgTile g = mgTile(t, X, Y);
+        const int hb = g.hb;
+        int i = g.ib + XY * t.z0;
         float4 cm{0.0f, 0.0f, 0.0f, 0.0f}, cc = ldC(C, i);
-        float am = 0.0f, ac = a0(cc, r[i]);
+        float am = 0.0f, ac = a0(cc.w, r[i]);
         if (t.z0 > 0) {
             cm = ldC(C, i - XY);
-            am = a0(cm, r[i - XY]);
+            am = a0(cm.w, r[i - XY]);
         }
-        // neighbour offsets clamped into the level: every load of a plane is
-        // unconditional and issued before its arithmetic; a neighbour
-        // beyond the level (or a ground cell, w = 0) is dropped by a select
-        const bool hxR = t.x + 1 < X, hxL = t.x > 0, hyU = t.y + 1 < D.Y, hyD = t.y > 0;
-        const int oR = hxR ? 1 : 0, oL = hxL ? -1 : 0, oU = hyU ? X : 0, oD = hyD ? -X : 0;
+        float hw = ldCw(C, hb + XY * t.z0), hr = r[hb + XY * t.z0];
+        // a neighbour beyond the level (or a ground cell, w = 0) is dropped
+        // by a select; its coupling is read at an offset clamped into the level
+        const bool hxR = t.x + 1 < X, hxL = t.x > 0, hyU = t.y + 1 < Y, hyD = t.y > 0;
+        const int oL = hxL ? -1 : 0, oD = hyD ? -X : 0;
         for (int z = t.z0; z < t.z1; z++, i += XY) {
+            const int buf = (z - t.z0) & 1;
             const bool hzP = z + 1 < D.Z;
             const int ip = hzP ? i + XY : i;
-            const float4 cpl = ldC(C, ip), cR = ldC(C, i + oR), cL = ldC(C, i + oL), cU = ldC(C, i + oU),
-                         cD = ldC(C, i + oD);
-            const float rpl = r[ip], rR = r[i + oR], rL = r[i + oL], rU = r[i + oU], rD = r[i + oD], rv = r[i];
+            const float4 cpl = ldC(C, ip);
+            const float rpl = r[ip], rv = r[i], cl = ldCx(C, i + oL), cd = ldCy(C, i + oD);
+            const int hn = hb + XY * min(z + 1, D.Z - 1);
+            const float hwn = ldCw(C, hn), hrn = r[hn];
+            sA[buf][ty + 1][tx + 1] = ac;
+            if (g.hasH) sA[buf][g.hrow][g.hcol] = a0(hw, hr);
+            __syncthreads();
+            const float aR = sA[buf][ty + 1][tx + 2], aL = sA[buf][ty + 1][tx];
+            const float aU = sA[buf][ty + 2][tx + 1], aD = sA[buf][ty][tx + 1];
             const float4 cp = hzP ? cpl : float4{0.0f, 0.0f, 0.0f, 0.0f};
-            const float ap = hzP ? a0(cp, rpl) : 0.0f;
+            const float ap = hzP ? a0(cp.w, rpl) : 0.0f;
             float s = cm.z * am + cc.z * ap;
-            s = hxR ? s + cc.x * a0(cR, rR) : s;
-            s = hxL ? s + cL.x * a0(cL, rL) : s;
-            s = hyU ? s + cc.y * a0(cU, rU) : s;
-            s = hyD ? s + cD.y * a0(cD, rD) : s;
+            s = hxR ? s + cc.x * aR : s;
+            s = hxL ? s + cl * aL : s;
+            s = hyU ? s + cc.y * aU : s;
+            s = hyD ? s + cd * aD : s;
             const float v = cc.w > 0.0f ? ac + (rv - (cc.w * ac - s)) * omegaOver(cc.w) : 0.0f;
-            xout[i] = v;
+            if (g.in) xout[i] = v;
             cm = cc;
             am = ac;
             cc = cp;
             ac = ap;
+            hw = hwn;
+            hr = hrn;
         }
     }
 };
@@ -273,32 +332,40 @@ struct KMgProlongSmooth {
                                                const PcgState* st) {
         if (st->done) return;
         const MgCol t = mgCol(vb, F);
-        if (!t.in) return;
+        if (t.z0 >= t.z1) return;  // the whole block
+        // the corrected x of the in-plane neighbours from the plane's LDS tile
+        __shared__ float sX[2][6][66];
+        const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
         const int X = F.X, XY = F.X * F.Y, DXY = D.X * D.Y;
-        // the parent rows of the column and of its -x/+x/-y/+y neighbours
-        const int px = t.x >> 1, py = t.y >> 1;
-        const int pxl = t.x > 0 ? (t.x - 1) >> 1 : 0, pxr = t.x + 1 < X ? (t.x + 1) >> 1 : 0;
-        const int pyd = t.y > 0 ? (t.y - 1) >> 1 : 0, pyu = t.y + 1 < F.Y ? (t.y + 1) >> 1 : 0;
+        const MgTile g = mgTile(t, X, F.Y);
+        // the parent rows of the cell and of the halo cell
+        const int px = (g.ib % X) >> 1, py = (g.ib / X) >> 1, hpx = g.hx >> 1, hpy = g.hy >> 1;
         auto corr = [&](int cx, int cy, int z) { return xc[cx + D.X * cy + DXY * (z >> 1)]; };
-        int i = t.x + X * t.y + XY * t.z0;
+        int i = g.ib + XY * t.z0;
         float xm = 0.0f, czm = 0.0f;
         if (t.z0 > 0) {
             xm = xin[i - XY] + corr(px, py, t.z0 - 1);
             czm = ldCz(C, i - XY);
         }
         float xe = xin[i] + corr(px, py, t.z0);
-        // neighbour offsets clamped into the level: every load of a plane is
-        // unconditional and issued before its arithmetic; a neighbour
-        // beyond the level (or a ground cell, w = 0) is dropped by a select
+        float hv = xin[g.hb + XY * t.z0] + corr(hpx, hpy, t.z0);
+        // a neighbour beyond the level (or a ground cell, w = 0) is dropped
+        // by a select; couplings are read at offsets clamped into the level
         const bool hxR = t.x + 1 < X, hxL = t.x > 0, hyU = t.y + 1 < F.Y, hyD = t.y > 0;
-        const int oR = hxR ? 1 : 0, oL = hxL ? -1 : 0, oU = hyU ? X : 0, oD = hyD ? -X : 0;
+        const int oL = hxL ? -1 : 0, oD = hyD ? -X : 0;
         for (int z = t.z0; z < t.z1; z++, i += XY) {
+            const int buf = (z - t.z0) & 1;
             const float4 c = ldC(C, i);
             const bool hzP = z + 1 < F.Z;
             const float xpl = xin[hzP ? i + XY : i] + corr(px, py, hzP ? z + 1 : z);
-            const float xR = xin[i + oR] + corr(pxr, py, z), xL = xin[i + oL] + corr(pxl, py, z);
-            const float xU = xin[i + oU] + corr(px, pyu, z), xD = xin[i + oD] + corr(px, pyd, z);
             const float cl = ldCx(C, i + oL), cd = ldCy(C, i + oD), rv = r[i];
+            const int zn = min(z + 1, F.Z - 1);
+            const float hvn = xin[g.hb + XY * zn] + corr(hpx, hpy, zn);
+            sX[buf][ty + 1][tx + 1] = xe;
+            if (g.hasH) sX[buf][g.hrow][g.hcol] = hv;
+            __syncthreads();
+            const float xR = sX[buf][ty + 1][tx + 2], xL = sX[buf][ty + 1][tx];
+            const float xU = sX[buf][ty + 2][tx + 1], xD = sX[buf][ty][tx + 1];
             const float xp = hzP ? xpl : 0.0f;
             float s = czm * xm + c.z * xp;
             s = hxR ? s + c.x * xR : s;
@@ -306,10 +373,11 @@ struct KMgProlongSmooth {
             s = hyU ? s + c.y * xU : s;
             s = hyD ? s + cd * xD : s;
             const float v = c.w > 0.0f ? xe + (rv - (c.w * xe - s)) * omegaOver(c.w) : 0.0f;
-            xout[i] = v;
+            if (g.in) xout[i] = v;
             xm = xe;
             xe = xp;
             czm = c.z;
+            hv = hvn;
         }
     }
 };
@@ -318,8 +386,9 @@ struct KMgProlongSmooth {
 // into level l + 1's right-hand side, in one pass over the fine level: a
 // block is 64 x 4 fine columns (32 x 2 coarse) marching the fine planes of
 // a coarse z-chunk; each thread forms its cells' residuals (the column's x
-// at z - 1, z, z + 1 in registers), sums each pair of planes, adds its x
-// neighbour's sum by a lane exchange and its y neighbour's through LDS.
+// at z - 1, z, z + 1 in registers, the in-plane neighbours' x from the
+// plane's LDS tile), sums each pair of planes, adds its x neighbour's sum by
+// a lane exchange and its y neighbour's through LDS.
 struct KMgRestrict {
     template <class... T>
     static __device__ __forceinline__ bool idle(const T&... a) {
@@ -335,43 +404,47 @@ struct KMgRestrict {
         const int tx = threadIdx.x, ty = threadIdx.y;
         const int fx = (int)bx * 64 + tx, fy = (int)by * 4 + ty;
         const int zs = 2 * (int)bz * D.zc, ze = min(F.Z, 2 * ((int)bz + 1) * D.zc);
-        const bool in = fx < F.X && fy < F.Y;
+        if (zs >= ze) return;  // the whole block
+        // x of the in-plane neighbours from the plane's LDS tile
+        __shared__ float sX[2][6][66];
         const int X = F.X, XY = F.X * F.Y;
-        int i = in ? fx + X * fy + XY * zs : 0;
-        float xm = 0.0f, czm = 0.0f, xc = 0.0f;
-        if (in) {
-            xc = x[i];
-            if (zs > 0) {
-                xm = x[i - XY];
-                czm = ldCz(C, i - XY);
-            }
+        const MgTile g = mgTile(MgCol{fx, fy, zs, ze, true}, X, F.Y);
+        int i = g.ib + XY * zs;
+        float xm = 0.0f, czm = 0.0f, xc = x[i], hv = x[g.hb + XY * zs];
+        if (zs > 0) {
+            xm = x[i - XY];
+            czm = ldCz(C, i - XY);
         }
         const int cx = fx >> 1, cy = fy >> 1, DXY = D.X * D.Y;
         float acc = 0.0f;
-        // neighbour offsets clamped into the level: every load of a plane is
-        // unconditional and issued before its arithmetic; a neighbour
-        // beyond the level (or a ground cell, w = 0) is dropped by a select
+        // a neighbour beyond the level (or a ground cell, w = 0) is dropped
+        // by a select; couplings are read at offsets clamped into the level;
+        // lanes past the level add nothing
         const bool hxR = fx + 1 < X, hxL = fx > 0, hyU = fy + 1 < F.Y, hyD = fy > 0;
-        const int oR = hxR ? 1 : 0, oL = hxL ? -1 : 0, oU = hyU ? X : 0, oD = hyD ? -X : 0;
-        for (int z = zs; z < ze; z++) {
-            if (in) {
-                const float4 c = ldC(C, i);
-                const bool hzP = z + 1 < F.Z;
-                const float xpl = x[hzP ? i + XY : i];
-                const float xr = x[i + oR], xl = x[i + oL], xu = x[i + oU], xd = x[i + oD];
-                const float cl = ldCx(C, i + oL), cd = ldCy(C, i + oD), rv = r[i];
-                const float xp = hzP ? xpl : 0.0f;
-                float s = czm * xm + c.z * xp;
-                s = hxR ? s + c.x * xr : s;
-                s = hxL ? s + cl * xl : s;
-                s = hyU ? s + c.y * xu : s;
-                s = hyD ? s + cd * xd : s;
-                acc = c.w > 0.0f ? acc + (rv - (c.w * xc - s)) : acc;
-                xm = xc;
-                xc = xp;
-                czm = c.z;
-                i += XY;
-            }
+        const int oL = hxL ? -1 : 0, oD = hyD ? -X : 0;
+        for (int z = zs; z < ze; z++, i += XY) {
+            const int buf = (z - zs) & 1;
+            const float4 c = ldC(C, i);
+            const bool hzP = z + 1 < F.Z;
+            const float xpl = x[hzP ? i + XY : i];
+            const float cl = ldCx(C, i + oL), cd = ldCy(C, i + oD), rv = r[i];
+            const float hvn = x[g.hb + XY * min(z + 1, F.Z - 1)];
+            sX[buf][ty + 1][tx + 1] = xc;
+            if (g.hasH) sX[buf][g.hrow][g.hcol] = hv;
+            __syncthreads();
+            const float xr = sX[buf][ty + 1][tx + 2], xl = sX[buf][ty + 1][tx];
+            const float xu = sX[buf][ty + 2][tx + 1], xd = sX[buf][ty][tx + 1];
+            const float xp = hzP ? xpl : 0.0f;
+            float s = czm * xm + c.z * xp;
+            s = hxR ? s + c.x * xr : s;
+            s = hxL ? s + cl * xl : s;
+            s = hyU ? s + c.y * xu : s;
+            s = hyD ? s + cd * xd : s;
+            acc = g.in && c.w > 0.0f ? acc + (rv - (c.w * xc - s)) : acc;
+            xm = xc;
+            xc = xp;
+            czm = c.z;
+            hv = hvn;
             if ((z & 1) || z == F.Z - 1) {  // a coarse plane is complete (uniform over the block)
                 const float v = acc + __shfl_xor(acc, 1);
                 sh[ty][tx] = v;
