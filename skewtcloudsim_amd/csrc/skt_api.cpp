@@ -327,10 +327,7 @@ int precip(EnvState& e, int fld, int type, float dt) { return precipFields(e, &f
 int solveOnce(EnvState& e, int pass, int* slot_out) {
     const bool mg = e.mg_depth > 0;
     TL(e, TC_OTHER, launch_pcg_start(e, pass));
-    if (mg) {
-        TL(e, TC_MG, launch_mg_vcycle(e));
-        TL(e, TC_MG, launch_mg_dot(e, 1));
-    }
+    if (mg) TL(e, TC_MG, launch_mg_vcycle(e, 1));
     const int maxIt = e.cfg.pcg_max_iter;
     // a multigrid iteration costs a few Jacobi ones and a solve takes tens,
     // not hundreds: poll in chunks of 4
@@ -339,10 +336,7 @@ int solveOnce(EnvState& e, int pass, int* slot_out) {
     for (; it < maxIt; it++) {
         TL(e, TC_PCG_A, launch_pcg_A(e, it));
         TL(e, TC_PCG_B, launch_pcg_B(e, it));
-        if (mg) {
-            TL(e, TC_MG, launch_mg_vcycle(e));
-            TL(e, TC_MG, launch_mg_dot(e, 0));
-        }
+        if (mg) TL(e, TC_MG, launch_mg_vcycle(e, 0));
         // a recorded ensemble step holds every iteration (those after the
         // member's solve has finished return at once)
         if (!e.rec && (it + 1) % chunk == 0 && it + 1 < maxIt) {

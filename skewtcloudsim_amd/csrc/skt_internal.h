@@ -173,6 +173,7 @@ struct EnvState {
     // depth has no tiles), so grids of any size share it in an ensemble
     int mg_depth = 0;
     MgLevel mg[kMgLevels];
+    unsigned mg_dot_n = 0;                  // z.r partials the last V-cycle wrote
     float* zeros = nullptr;                 // [Y] zero profile
     float* ones = nullptr;                  // [Y] unit profile
     double* partials = nullptr;             // per-block reduction partials (PART_COUNT x maxBlocks)
@@ -355,8 +356,9 @@ hipError_t launch_refine_accum(EnvState& e, int first, int last);  // x64 += p'
 hipError_t launch_refine_residual(EnvState& e, int write_r);      // r = b - A x64 in FP64 (+ max partials)
 hipError_t launch_refine_report(EnvState& e);          // final max |b - A x64|
 hipError_t launch_mg_setup(EnvState& e);               // the levels' operators from 1/density
-hipError_t launch_mg_vcycle(EnvState& e);              // z = V-cycle(r)
-hipError_t launch_mg_dot(EnvState& e, int start);      // z.r partials for the next search direction
+// z = V-cycle(r) and the z.r partials for the next search direction (into
+// PART_START when start, else PART_BSUM; e.mg_dot_n of them)
+hipError_t launch_mg_vcycle(EnvState& e, int start);
 hipError_t launch_update_pressure_density(EnvState& e);
 hipError_t launch_apply_projection(EnvState& e, int new_pressure);
 hipError_t launch_microphysics(EnvState& e, float dt, float speed);

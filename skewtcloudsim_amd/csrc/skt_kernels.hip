@@ -894,16 +894,16 @@ struct PcgPart {
     double m[8], s[8];
 };
 template <int KIND>
-__device__ __forceinline__ void pcgPartLoad(const double* __restrict__ pm, const double* __restrict__ ps, unsigned n,
-                                            PcgPart& P) {
+__device__ __forceinline__ void pcgPartLoad(const double* __restrict__ pm, const double* __restrict__ ps, unsigned nm,
+                                            unsigned ns, PcgPart& P) {
     const unsigned t = tid256();
     constexpr bool wantM = KIND == PRO_START || KIND == PRO_A || KIND == PRO_FINAL;
     constexpr bool wantS = KIND == PRO_A0 || KIND == PRO_A || KIND == PRO_B;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const unsigned j = t + 256u * k;
-        P.m[k] = wantM && j < n ? pm[j] : 0.0;
-        P.s[k] = wantS && j < n ? ps[j] : 0.0;
+        P.m[k] = wantM && j < nm ? pm[j] : 0.0;
+        P.s[k] = wantS && j < ns ? ps[j] : 0.0;
     }
 }
 // Under FP64 refinement (ro set; pass = the solve's index in the projection)
@@ -912,7 +912,7 @@ __device__ __forceinline__ void pcgPartLoad(const double* __restrict__ pm, const
 // residual already meets it, and every pass stops at the larger of its own
 // relative tolerance and the target.
 template <int KIND>
-__device__ void pcgPrologue(const double* __restrict__ pm, const double* __restrict__ ps, unsigned n,
+__device__ void pcgPrologue(const double* __restrict__ pm, const double* __restrict__ ps, unsigned nm, unsigned ns,
                             const PcgPart& P, const PcgState* __restrict__ sin, PcgState* __restrict__ sout, int ref,
                             float tol, int it, PcgState& S, double* sh, unsigned tile, RefineStats* ro = nullptr,
                             int pass = 0, float tolOut = 0.0f) {
@@ -930,13 +930,13 @@ __device__ void pcgPrologue(const double* __restrict__ pm, const double* __restr
         if (KIND == PRO_START || KIND == PRO_A || KIND == PRO_FINAL) {
 #pragma unroll
             for (int k = 0; k < 8; k++) vmax = fmax(vmax, P.m[k]);
-            for (unsigned j = t + 2048; j < n; j += 256) vmax = fmax(vmax, pm[j]);
+            for (unsigned j = t + 2048; j < nm; j += 256) vmax = fmax(vmax, pm[j]);
             vmax = blockMax(vmax, sh);
         }
         if (KIND == PRO_A0 || KIND == PRO_A || KIND == PRO_B) {
 #pragma unroll
             for (int k = 0; k < 8; k++) vsum += P.s[k];
-            for (unsigned j = t + 2048; j < n; j += 256) vsum += ps[j];
+            for (unsigned j = t + 2048; j < ns; j += 256) vsum += ps[j];
             vsum = blockSum(vsum, sh);
         }
         if (t == 0) {
@@ -1212,8 +1212,8 @@ struct KPcgStart {
             }
         };
         if (z0 < z1) loadGroup(z0);
-        pcgPartLoad<PRO_START>(pin, nullptr, n, P);
-        pcgPrologue<PRO_START>(pin, nullptr, n, P, sin, sout, ref, tol, 0, S, sh, vb.id, ro, pass, tolOut);
+        pcgPartLoad<PRO_START>(pin, nullptr, n, n, P);
+        pcgPrologue<PRO_START>(pin, nullptr, n, n, P, sin, sout, ref, tol, 0, S, sh, vb.id, ro, pass, tolOut);
         if (S.done) return;
         double s = 0.0;
         for (int zb = z0; zb < z1; zb += PG) {
@@ -1281,14 +1281,14 @@ hipError_t launch_pcg_start(EnvState& e, int pass) {
 // launch keeps (chunk + 3 planes) below 4 GiB.
 template <bool FIRST, bool TRUNC>
 struct KPcgA {
-    // a finished solve (sin: argument 11)
+    // a finished solve (sin: argument 12)
     template <class... T>
     static __device__ __forceinline__ bool idle(const T&... a) {
-        return nthArg<11>(a...)->done != 0;
+        return nthArg<12>(a...)->done != 0;
     }
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, const float* __restrict__ zz, const float* __restrict__ s_old,
                     float* __restrict__ s_new, const float* __restrict__ irho, float* __restrict__ As,
-                    const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, double* pout,
+                    const double* __restrict__ pm, const double* __restrict__ ps, unsigned n, unsigned ns, double* pout,
                     const PcgState* sin, PcgState* sout, int it, const int* __restrict__ faceExact) {
         ZM_SETUP_V;
         __shared__ double sh[4];
@@ -1297,7 +1297,7 @@ struct KPcgA {
         const bool exact = *faceExact != 0;
         constexpr int KIND = FIRST ? PRO_A0 : PRO_A;
         PcgPart part;
-        pcgPartLoad<KIND>(pm, ps, n, part);
+        pcgPartLoad<KIND>(pm, ps, n, ns, part);
         float beta = 0.0f;
         const int tx = threadIdx.x, ty = threadIdx.y;
         const int X = G.X, Y = G.Y, Z = G.Z;
@@ -1374,7 +1374,7 @@ struct KPcgA {
         loadOwn(min(z0 + 1, zlast), o1);
         loadOwn(min(z0 + 2, zlast), o2);
         loadHalo(z0, h0);
-        pcgPrologue<KIND>(pm, ps, n, part, sin, sout, 0, 0.0f, it, S, sh, vb.id);
+        pcgPrologue<KIND>(pm, ps, n, ns, part, sin, sout, 0, 0.0f, it, S, sh, vb.id);
         if (S.done) return;
         beta = S.beta;
         // plane below the chunk: s_new, and its front face (= this plane's back face)
@@ -1467,14 +1467,17 @@ hipError_t launch_pcg_A(EnvState& e, int it) {
     // byte offsets from the chunk's base plane stay below 4 GiB
     if ((double)(zg.zc + 3) * (double)e.X * (double)e.Y * 4.0 >= 4294967296.0) return hipErrorInvalidValue;
     const size_t M = e.maxBlocks;
-    // the first iteration reduces pcg_start's partials, later ones pcg_B's
+    // the first iteration reduces pcg_start's partials, later ones pcg_B's;
+    // with the multigrid preconditioner the z.r partials are the V-cycle's
+    // (e.mg_dot_n of them)
     const double* pm = e.partials + PART_BMAX * M;
     const double* ps = it == 0 ? e.partials + PART_START * M : e.partials + PART_BSUM * M;
     const unsigned np = it == 0 ? e.gZ.nblocks : e.gB.nblocks;
+    const unsigned ns = e.mg_depth > 0 ? e.mg_dot_n : np;
     const bool trunc = e.cfg.pcg_mode == SKT_PCG_REFERENCE;
     auto go = [&](auto body) {
         return klaunch<decltype(body)>(e, zg.grid, B3, 0, geom(e), zg.zc, e.pcg[1], s_old, s_new, e.irho, e.pcg[4],
-                                       pm, ps, np, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it,
+                                       pm, ps, np, ns, e.partials + PART_A * M, &e.sc->st[1], &e.sc->st[0], it,
                                        &e.sc->face_exact);
     };
     if (it == 0) return trunc ? go(KPcgA<true, true>{}) : go(KPcgA<true, false>{});
@@ -1494,7 +1497,7 @@ ResidentBlocks resident_blocks(int device) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 0;
     return ResidentBlocks{resident_blocks(&k_solo<KPcgA<false, true>, Geom, int, float*, float*, float*, float*, float*, const double*,
-                                          const double*, unsigned, double*, PcgState*, PcgState*, int, int*>,
+                                          const double*, unsigned, unsigned, double*, PcgState*, PcgState*, int, int*>,
                                           device, cus),
                           resident_blocks(&k_solo<KDiffuse<false>, Geom, int, unsigned, DiffBatch, float, float*, float*,
                                                   float*>,
@@ -1512,7 +1515,9 @@ ResidentBlocks resident_blocks(int device) {
 // bits are stored back) and get z = 0. The partials of pcg_A and then the
 // lane's first two groups are read before the prologue reduces the partials;
 // after that each group's data is read two groups ahead (two register slots,
-// the loop handles one group per slot per trip).
+// the loop handles one group per slot per trip). Under the multigrid
+// preconditioner (jacobi 0) the V-cycle that follows forms z and its z.r
+// partials, so the pass reads no preconditioner and writes neither.
 template <int V>
 struct KPcgB {
     template <class... T>
@@ -1522,13 +1527,13 @@ struct KPcgB {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, unsigned rpb, float* __restrict__ pp, float* __restrict__ r, const float* __restrict__ s,
                     const float* __restrict__ As, const float* __restrict__ pre, float* __restrict__ zz,
                     const double* __restrict__ pin, unsigned n, double* pmax, double* psum, const PcgState* sin,
-                    PcgState* sout, int ref, int absmode) {
+                    PcgState* sout, int ref, int absmode, int jacobi) {
         typedef float __attribute__((ext_vector_type(V))) fv;
         typedef int __attribute__((ext_vector_type(V))) iv;
         __shared__ double sh[4];
         __shared__ PcgState S;
         PcgPart part;
-        pcgPartLoad<PRO_B>(nullptr, pin, n, part);
+        pcgPartLoad<PRO_B>(nullptr, pin, n, n, part);
         const int X = G.X, Y = G.Y, q = G.X / V;
         const long rows = (long)Y * G.Z;
         const long r0 = (long)vb.id * rpb, r1 = std::min(rows, r0 + (long)rpb);
@@ -1543,14 +1548,14 @@ struct KPcgB {
                 d.R = ((const fv*)r)[gk];
                 d.S = ((const fv*)s)[gk];
                 d.A = ((const fv*)As)[gk];
-                d.Q = ((const fv*)pre)[gk];
+                if (jacobi) d.Q = ((const fv*)pre)[gk];
             }
         };
         long g = r0 * q + t;
         Grp d0, d1;
         load(g, d0);
         load(g + 256, d1);
-        pcgPrologue<PRO_B>(nullptr, pin, n, part, sin, sout, ref, 0.0f, 0, S, sh, vb.id);
+        pcgPrologue<PRO_B>(nullptr, pin, n, n, part, sin, sout, ref, 0.0f, 0, S, sh, vb.id);
         if (S.done) return;
         const float alpha = S.alpha;
         float mf = 0.0f;
@@ -1572,14 +1577,16 @@ struct KPcgB {
                         const float rv = d.R[c] - alpha * d.A[c];
                         Ro[c] = rv;
                         mf = fmaxf(mf, ab ? fabsf(rv) : rv);
-                        zv = rv * d.Q[c];
-                        dz += (double)(zv * rv);
+                        if (jacobi) {
+                            zv = rv * d.Q[c];
+                            dz += (double)(zv * rv);
+                        }
                     }
                     Zo[c] = zv;
                 }
                 ((fv*)pp)[gk] = Po;
                 ((fv*)r)[gk] = Ro;
-                ((fv*)zz)[gk] = Zo;
+                if (jacobi) ((fv*)zz)[gk] = Zo;
                 gx += dgx;
                 y += dr;
                 if (gx >= q) gx -= q, ++y;
@@ -1595,10 +1602,10 @@ struct KPcgB {
             }
         }
         const double m = blockMax((double)mf, sh);
-        dz = blockSum(dz, sh);
+        if (jacobi) dz = blockSum(dz, sh);
         if (t == 0) {
             pmax[vb.id] = m;
-            psum[vb.id] = dz;
+            if (jacobi) psum[vb.id] = dz;
         }
     }
 };
@@ -1611,7 +1618,7 @@ hipError_t launch_pcg_B(EnvState& e, int it) {
         return klaunch<decltype(body)>(e, dim3(e.gB.nblocks), B3, 0, geom(e), e.gB.rpb, e.f[SKT_PRESSURE_PROJ],
                                        e.pcg[0], s_new, e.pcg[4], e.pcg[5], e.pcg[1], e.partials + PART_A * M,
                                        e.gA.nblocks, e.partials + PART_BMAX * M, e.partials + PART_BSUM * M,
-                                       &e.sc->st[0], &e.sc->st[1], ref, absmode);
+                                       &e.sc->st[0], &e.sc->st[1], ref, absmode, e.mg_depth > 0 ? 0 : 1);
     };
     return e.X % 4 == 0 ? go(KPcgB<4>{}) : go(KPcgB<1>{});
 }
@@ -1622,8 +1629,8 @@ struct KPcgFinal {
         __shared__ double sh[4];
         __shared__ PcgState S;
         PcgPart P;
-        pcgPartLoad<PRO_FINAL>(pm, nullptr, n, P);
-        pcgPrologue<PRO_FINAL>(pm, nullptr, n, P, sin, sout, 0, 0.0f, it, S, sh, vb.id);
+        pcgPartLoad<PRO_FINAL>(pm, nullptr, n, n, P);
+        pcgPrologue<PRO_FINAL>(pm, nullptr, n, n, P, sin, sout, 0, 0.0f, it, S, sh, vb.id);
     }
 };
 hipError_t launch_pcg_final(EnvState& e, int it) {

@@ -197,7 +197,8 @@ struct KMgSmooth {
     }
     static __device__ __forceinline__ void run(VB vb, MgDims D, const MgC* __restrict__ C,
                                                const float* __restrict__ r, const float* __restrict__ xin,
-                                               float* __restrict__ xout, const PcgState* st, int first) {
+                                               float* __restrict__ xout, const PcgState* st, int first,
+                                               double* dpart) {
         if (st->done) return;
         const MgCol t = mgCol(vb, D);
         const int X = D.X, XY = D.X * D.Y;
@@ -210,11 +211,16 @@ struct KMgSmooth {
             }
             return;
         }
-        if (t.z0 >= t.z1) return;  // the whole block
+        if (t.z0 >= t.z1) {  // the whole block
+            if (dpart && tid256() == 0) dpart[vb.id] = 0.0;
+            return;
+        }
         // x of the in-plane neighbours from the plane's LDS tile
         __shared__ float sX[2][6][66];
+        __shared__ double sh[4];
         const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
         const MgTile g = mgTile(t, X, D.Y);
+        double dz = 0.0;  // this lane's share of z.r (level 0's last sweep)
         int i = g.ib + XY * t.z0;
         float xm = t.z0 > 0 ? xin[i - XY] : 0.0f, czm = t.z0 > 0 ? ldCz(C, i - XY) : 0.0f;
         float xc = xin[i], hv = xin[g.hb + XY * t.z0];
@@ -241,11 +247,18 @@ struct KMgSmooth {
             s = hyU ? s + c.y * xu : s;
             s = hyD ? s + cd * xd : s;
             const float v = c.w > 0.0f ? xc + (rv - (c.w * xc - s)) * omegaOver(c.w) : 0.0f;
-            if (g.in) xout[i] = v;
+            if (g.in) {
+                xout[i] = v;
+                dz += (double)(v * rv);
+            }
             xm = xc;
             xc = xp;
             czm = c.z;
             hv = hvn;
+        }
+        if (dpart) {
+            dz = blockSum(dz, sh);
+            if (tid256() == 0) dpart[vb.id] = dz;
         }
     }
 };
@@ -508,8 +521,8 @@ struct KMgCoarsest {
     }
 };
 
-// z.r partials of the preconditioned residual in the layout the next
-// search-direction kernel reduces: n blocks, block b summing cells
+// z.r partials of the preconditioned residual for a grid of one level (no
+// level-0 sweep to form them): n blocks, block b summing cells
 // [b chunk, (b + 1) chunk) of float products in double.
 struct KMgDot {
     template <class... T>
@@ -572,7 +585,7 @@ hipError_t launch_mg_setup(EnvState& e) {
     return r;
 }
 
-hipError_t launch_mg_vcycle(EnvState& e) {
+hipError_t launch_mg_vcycle(EnvState& e, int start) {
     const MgLevel* L = e.mg;
     const bool ens = e.rec != nullptr;
     const PcgState* st = &e.sc->st[1];
@@ -581,6 +594,13 @@ hipError_t launch_mg_vcycle(EnvState& e) {
     auto go = [&](hipError_t x) {
         if (r == hipSuccess) r = x;
     };
+    // the z.r partials of the next search direction: one per tile of level
+    // 0's last sweep, which forms z; a grid of one level (no level-0 sweep)
+    // reduces them in a pass of its own
+    const size_t M = e.maxBlocks;
+    double* part = e.partials + (start ? PART_START : PART_BSUM) * M;
+    const unsigned t0 = colTiles(dims(L[0], ens), D > 1).x;
+    const bool fused = D > 1 && t0 <= M;
     // two Jacobi sweeps before (one fused pass into xb) and after (the
     // coarse correction and the first sweep fused into xa, the second back
     // into xb, each level's result)
@@ -601,17 +621,14 @@ hipError_t launch_mg_vcycle(EnvState& e) {
         go(klaunch<KMgProlongSmooth>(e, g, kB3, 0, f, (const MgC*)L[l].C, (const float*)L[l].r,
                                      (const float*)L[l].xb, L[l].xa, c, (const float*)L[l + 1].xb, st));
         go(klaunch<KMgSmooth>(e, g, kB3, 0, f, (const MgC*)L[l].C, (const float*)L[l].r, (const float*)L[l].xa,
-                              L[l].xb, st, 0));
+                              L[l].xb, st, 0, l == 0 && fused ? part : (double*)nullptr));
     }
+    // the pass of its own has no tiles when level 0's sweep formed the
+    // partials (every grid records the same launch sequence)
+    e.mg_dot_n = fused ? t0 : (start ? e.gZ.nblocks : e.gB.nblocks);
+    go(klaunch<KMgDot>(e, dim3(fused ? 0u : e.mg_dot_n), dim3(256), 0, e.N, (const float*)e.pcg[1],
+                       (const float*)e.pcg[0], part, (const PcgState*)&e.sc->st[1]));
     return r;
-}
-
-hipError_t launch_mg_dot(EnvState& e, int start) {
-    const size_t M = e.maxBlocks;
-    const unsigned n = start ? e.gZ.nblocks : e.gB.nblocks;
-    double* part = e.partials + (start ? PART_START : PART_BSUM) * M;
-    return klaunch<KMgDot>(e, dim3(n), dim3(256), 0, e.N, (const float*)e.pcg[1], (const float*)e.pcg[0], part,
-                           (const PcgState*)&e.sc->st[1]);
 }
 
 }  // namespace skt

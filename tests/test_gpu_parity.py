@@ -651,15 +651,15 @@ def test_kernel_timing_classes():
             continue
         assert n > 0 and ms > 0.0, (c, n, ms)
     assert t["pcg_spmv"][0] == t["pcg_update"][0]
-    # with the multigrid preconditioner: set-up plus a V-cycle and a dot
-    # product per iteration and one at the start of the solve
+    # with the multigrid preconditioner: set-up plus a V-cycle (which also
+    # forms the z.r partials) per iteration and one at the start of the solve
     mg = skt.EnvironmentGPU(nx, ny, nz, 100.0, pcg_mode=1, pcg_tol=1e-6, pcg_max_iter=50, pcg_precon=1)
     mg.init(*[snd[k] for k in KEYS])
     mg.set_timing(True, classes=["multigrid"])
     mg.updateGPU(0.5, 1.0)
     tm = mg.timing()
     its = mg.stats().pcg_iterations
-    assert its > 0 and tm["multigrid"][0] >= 1 + 2 * (its + 1), (tm["multigrid"], its)
+    assert its > 0 and tm["multigrid"][0] >= 1 + (its + 1), (tm["multigrid"], its)
     mg.close()
     env.set_timing(True, classes=["pcg_spmv"])
     env.updateGPU(0.5, 1.0)
