@@ -1185,11 +1185,12 @@ hipError_t launch_divergence(EnvState& e, float dt) {
 }
 
 // Prologue PRO_START, then z = M^-1 r (applyPreconditionerGPU,
-// Game/kernelSky.cu:659-673) and the z.r partials (dotProductGPU, :675-717).
+// Game/kernelSky.cu:659-673) and the z.r partials (dotProductGPU, :675-717);
+// under the multigrid preconditioner (jacobi 0) only the prologue.
 struct KPcgStart {
     static __device__ __forceinline__ void run(VB vb, const Geom& G, int zc, const float* __restrict__ r, float* __restrict__ zz, const float* __restrict__ pre,
                         const double* __restrict__ pin, unsigned n, double* pout, const PcgState* sin, PcgState* sout,
-                        int ref, float tol, RefineStats* ro, int pass, float tolOut) {
+                        int ref, float tol, RefineStats* ro, int pass, float tolOut, int jacobi) {
         ZM_SETUP_V;
         __shared__ double sh[4];
         __shared__ PcgState S;
@@ -1211,10 +1212,12 @@ struct KPcgStart {
                 gv[k] = G.gh[xc + (long)G.X * z];
             }
         };
-        if (z0 < z1) loadGroup(z0);
+        if (jacobi && z0 < z1) loadGroup(z0);
         pcgPartLoad<PRO_START>(pin, nullptr, n, n, P);
         pcgPrologue<PRO_START>(pin, nullptr, n, n, P, sin, sout, ref, tol, 0, S, sh, vb.id, ro, pass, tolOut);
-        if (S.done) return;
+        // under the multigrid preconditioner the V-cycle that follows forms z
+        // and its partials: only the prologue runs
+        if (S.done || !jacobi) return;
         double s = 0.0;
         for (int zb = z0; zb < z1; zb += PG) {
             float cr[PG], cp[PG];
@@ -1251,7 +1254,7 @@ hipError_t launch_pcg_start(EnvState& e, int pass) {
     return klaunch<KPcgStart>(e, zg.grid, B3, 0, geom(e), zg.zc, e.pcg[0], e.pcg[1], e.pcg[5], e.partials + PART_DIV * M,
                               divBlocks(e), e.partials + PART_START * M, &e.sc->st[0], &e.sc->st[1], ref,
                               refine ? e.cfg.pcg_refine_tol : e.cfg.pcg_tol, refine ? &e.sc->ref : nullptr,
-                              refine ? pass : 0, e.cfg.pcg_tol);
+                              refine ? pass : 0, e.cfg.pcg_tol, e.mg_depth > 0 ? 0 : 1);
 }
 
 // Prologue PRO_A0 / PRO_A, then the search-direction update (endIteration,
